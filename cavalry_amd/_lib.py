@@ -1,0 +1,94 @@
+"""ctypes binding of libcavalry_amd.so (the C ABI declared in include/cavalry.h).
+
+The library is built in-tree by ``python -m cavalry_amd.build``. There is no
+fallback: if the shared object is missing or fails to load, every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcavalry_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cavalry.h")
+
+CAV_OK = 0
+STATUS = {
+    0: "CAV_OK", 1: "CAV_ERR_INVALID", 2: "CAV_ERR_HIP", 3: "CAV_ERR_OOM",
+    4: "CAV_ERR_OVERFLOW", 5: "CAV_ERR_PARSE", 6: "CAV_ERR_BUSY", 7: "CAV_ERR_NOT_FOUND",
+}
+
+
+class CavError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class BvhStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_float) for k in (
+        "ms_bounds", "ms_morton", "ms_sort", "ms_karras", "ms_refit", "ms_treelet",
+        "ms_flatten", "ms_total", "root_cost_lbvh", "root_cost_final")] + [
+        ("n", ctypes.c_int), ("morton_bits", ctypes.c_int), ("opt_rounds", ctypes.c_int)]
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+I64 = ctypes.c_int64
+
+_SIGS = {
+    "cav_ctx_create": ([I, ctypes.POINTER(P)], I),
+    "cav_ctx_destroy": ([P], I),
+    "cav_last_error": ([P], ctypes.c_char_p),
+    "cav_ctx_stream": ([P], P),
+    "cav_ctx_synchronize": ([P], I),
+    "cav_version": ([], ctypes.c_char_p),
+    "cav_bvh_build": ([P, P, I, P, P, I, I, ctypes.POINTER(P)], I),
+    "cav_bvh_build_device": ([P, P, I, P, P, I, I, ctypes.POINTER(P)], I),
+    "cav_bvh_rebuild_device": ([P, P, P, P, I, I], I),
+    "cav_bvh_destroy": ([P], I),
+    "cav_bvh_stats_get": ([P, ctypes.POINTER(BvhStats)], I),
+    "cav_bvh_export": ([P, P, P, P, P, P, P, P, P, P, P], I),
+    "cav_trace_closest": ([P, P, P, I64, P, P, P, P], I),
+    "cav_trace_closest_device": ([P, P, P, I64, P, P, P, P], I),
+    "cav_trace_occluded": ([P, P, P, P, P, P, I64, P], I),
+    "cav_trace_occluded_device": ([P, P, P, P, P, P, I64, P], I),
+    "cav_trace_closest_async": ([P, P, P, I64, P, P], I),
+    "cav_trace_occluded_async": ([P, P, P, P, I64, P], I),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library, raising if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m cavalry_amd.build` "
+                "(there is no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(l, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = res
+        _lib = l
+    return _lib
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/cavalry.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(cav_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(ctx, rc: int) -> None:
+    if rc != CAV_OK:
+        msg = lib().cav_last_error(ctx) if ctx else b""
+        raise CavError(rc, (msg or b"").decode(errors="replace"))

@@ -1,0 +1,147 @@
+"""Host-side BVH and context objects over the C ABI.
+
+Mirrors the reference's interfaces for this path:
+``BVH.build(triangles, trianglesCount, sceneBounds, optimizationRounds)``
+(BVH/BVH.h:19), ``BVH::intersect`` (BVH/BVH.h:23-107) and
+``BVH::testVisibility`` (BVH/BVH.h:109-189), with the triangles given as a
+float32[N, 9] soup and rays as float32[R, 6].
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    if hasattr(a, "data_ptr"):  # torch tensor on the device
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device, one HIP stream (cav_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().cav_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise _lib.CavError(rc, f"cav_ctx_create(device={device}) failed")
+        self.handle = h
+
+    @property
+    def stream(self) -> int:
+        return lib().cav_ctx_stream(self.handle) or 0
+
+    def synchronize(self) -> None:
+        check(self.handle, lib().cav_ctx_synchronize(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().cav_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BVH:
+    """A built BVH living in device memory (cav_bvh)."""
+
+    def __init__(self, ctx: Context, handle, n: int):
+        self.ctx = ctx
+        self.handle = handle
+        self.n = n
+
+    @staticmethod
+    def build(ctx: Context, triangles, scene_bounds=None, optimization_rounds: int = 0,
+              morton_bits: int = 30, tri_mesh=None, device: bool = False) -> "BVH":
+        """LBVH build + ``optimization_rounds`` treelet rounds. ``triangles`` is a
+        float32[N,9] array (host) or, with device=True, a device pointer/tensor."""
+        if device:
+            n = int(triangles.shape[0]) if hasattr(triangles, "shape") else None
+            if n is None:
+                raise ValueError("device build needs a tensor with a shape")
+        else:
+            triangles = np.ascontiguousarray(triangles, np.float32)
+            n = len(triangles)
+            if tri_mesh is not None:
+                tri_mesh = np.ascontiguousarray(tri_mesh, np.int32)
+        sb = None if scene_bounds is None else np.ascontiguousarray(scene_bounds, np.float32)
+        h = ctypes.c_void_p()
+        fn = lib().cav_bvh_build_device if device else lib().cav_bvh_build
+        check(ctx.handle, fn(ctx.handle, _ptr(triangles), n, _ptr(sb), _ptr(tri_mesh),
+                             morton_bits, optimization_rounds, ctypes.byref(h)))
+        return BVH(ctx, h, n)
+
+    def rebuild(self, triangles_dev, scene_bounds=None, optimization_rounds: int = 0,
+                morton_bits: int = 30) -> None:
+        sb = None if scene_bounds is None else np.ascontiguousarray(scene_bounds, np.float32)
+        check(self.ctx.handle, lib().cav_bvh_rebuild_device(
+            self.ctx.handle, self.handle, _ptr(triangles_dev), _ptr(sb), morton_bits,
+            optimization_rounds))
+
+    def stats(self) -> dict:
+        s = _lib.BvhStats()
+        check(self.ctx.handle, lib().cav_bvh_stats_get(self.handle, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def export(self) -> dict:
+        """Node arrays in the unified 2N-1 id space plus sorted codes and order."""
+        m = 2 * self.n - 1
+        out = {
+            "box": np.empty((m, 6), np.float32), "sa": np.empty(m, np.float32),
+            "cost": np.empty(m, np.float32), "left": np.empty(m, np.int32),
+            "right": np.empty(m, np.int32), "parent": np.empty(m, np.int32),
+            "prim": np.empty(m, np.int32), "codes": np.empty(self.n, np.uint64),
+            "order": np.empty(self.n, np.int32),
+        }
+        check(self.ctx.handle, lib().cav_bvh_export(
+            self.ctx.handle, self.handle, *[_ptr(out[k]) for k in (
+                "box", "sa", "cost", "left", "right", "parent", "prim", "codes", "order")]))
+        return out
+
+    def intersect(self, rays) -> dict:
+        """Closest hit per ray: prim (-1 = miss), t, u, v."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        r = len(rays)
+        out = {k: np.empty(r, np.float32) for k in ("t", "u", "v")}
+        out["prim"] = np.empty(r, np.int32)
+        check(self.ctx.handle, lib().cav_trace_closest(
+            self.ctx.handle, self.handle, _ptr(rays), r, _ptr(out["prim"]), _ptr(out["t"]),
+            _ptr(out["u"]), _ptr(out["v"])))
+        return out
+
+    def test_visibility(self, rays, limit, src_mesh=None, dst_mesh=None) -> np.ndarray:
+        """1 where nothing blocks the ray before its limit (+inf = no limit)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        limit = np.ascontiguousarray(limit, np.float32)
+        r = len(rays)
+        vis = np.empty(r, np.uint8)
+        src = None if src_mesh is None else np.ascontiguousarray(src_mesh, np.int32)
+        dst = None if dst_mesh is None else np.ascontiguousarray(dst_mesh, np.int32)
+        check(self.ctx.handle, lib().cav_trace_occluded(
+            self.ctx.handle, self.handle, _ptr(rays), _ptr(limit), _ptr(src), _ptr(dst), r,
+            _ptr(vis)))
+        return vis
+
+    def close(self) -> None:
+        if self.handle:
+            lib().cav_bvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

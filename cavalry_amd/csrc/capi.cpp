@@ -1,0 +1,429 @@
+// C ABI implementation: contexts, BVH objects and traversal entry points.
+// Declarations and their reference counterparts are in include/cavalry.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cavalry.h"
+#include "capi_internal.h"
+#include "lbvh.h"
+
+using namespace cav;
+
+const char* cav_version(void) { return "cavalry_amd 0.1 (gfx950)"; }
+
+namespace cav {
+
+int fail(cav_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(cav_ctx* ctx, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, CAV_ERR_HIP, m);
+}
+
+int dev_alloc(cav_ctx* ctx, void** p, size_t bytes, const char* what) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, CAV_ERR_OOM, std::string("hipMalloc ") + what + " (" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    }
+    return CAV_OK;
+}
+
+}  // namespace cav
+
+#define HIPCHK(ctx, call)                                              \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return cav::hip_fail((ctx), e_, #call);  \
+    } while (0)
+
+struct cav_bvh {
+    cav_ctx* ctx = nullptr;
+    int n = 0;
+    int morton_bits = 30;
+    int opt_rounds = 0;
+    LbvhBuffers b;
+    TNode* tn = nullptr;
+    TTri* tt = nullptr;
+    float* root = nullptr;      // 8 floats
+    int32_t* mesh = nullptr;    // n, original order (may be null)
+    cav_bvh_stats stats{};
+    StageEvents ev{};
+    bool events = false;
+};
+
+static void bvh_free(cav_bvh* h) {
+    void* ptrs[] = {h->b.keys[0], h->b.keys[1], h->b.vals[0], h->b.vals[1], h->b.hist, h->b.counts,
+                    h->b.offsets, h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,
+                    h->b.partial, h->b.leaf_ids, h->b.counters, h->tn, h->tt, h->root, h->mesh};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->events)
+        for (auto& e : h->ev.e) (void)hipEventDestroy(e);
+}
+
+static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
+    const int64_t m = 2 * (int64_t)n - 1;
+    const int64_t tiles = sort_tiles(n);
+    struct { void** p; size_t bytes; const char* what; } req[] = {
+        {&h->b.keys[0], sizeof(uint64_t) * n, "keys0"},
+        {&h->b.keys[1], sizeof(uint64_t) * n, "keys1"},
+        {(void**)&h->b.vals[0], sizeof(uint32_t) * n, "vals0"},
+        {(void**)&h->b.vals[1], sizeof(uint32_t) * n, "vals1"},
+        {(void**)&h->b.hist, sizeof(uint32_t) * 8 * 256, "hist"},
+        {(void**)&h->b.counts, sizeof(uint32_t) * 256 * tiles, "counts"},
+        {(void**)&h->b.offsets, sizeof(uint32_t) * 256 * tiles, "offsets"},
+        {(void**)&h->b.scene_ord, 64, "scene"},
+        {(void**)&h->b.nbox, sizeof(NodeBox) * m, "nbox"},
+        {(void**)&h->b.topo, sizeof(int4) * m, "topo"},
+        {(void**)&h->b.visited, sizeof(uint32_t) * m, "visited"},
+        {(void**)&h->b.ref, sizeof(int32_t) * m, "ref"},
+        {(void**)&h->b.partial, sizeof(uint32_t) * (lbvh_partials(m) + 1), "partial"},
+        {(void**)&h->b.leaf_ids, sizeof(uint32_t) * n, "leaf_ids"},
+        {(void**)&h->b.counters, 256, "counters"},
+        {(void**)&h->tn, sizeof(TNode) * (n - 1), "tnodes"},
+        {(void**)&h->tt, sizeof(TTri) * n, "ttris"},
+        {(void**)&h->root, 64, "root"},
+    };
+    for (auto& r : req) {
+        int rc = dev_alloc(ctx, r.p, r.bytes, r.what);
+        if (rc != CAV_OK) return rc;
+    }
+    for (auto& e : h->ev.e) HIPCHK(ctx, hipEventCreate(&e));
+    h->events = true;
+    return CAV_OK;
+}
+
+static void host_to_ord(const float* f6, uint32_t* o6) {
+    for (int i = 0; i < 6; ++i) {
+        uint32_t u;
+        std::memcpy(&u, &f6[i], 4);
+        o6[i] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    }
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) { (void)hipGetLastError(); return -1.f; }
+    return ms;
+}
+
+// Runs the whole build on the context stream and waits for it.
+static int bvh_run(cav_ctx* ctx, cav_bvh* h, const float* tris_dev, const float* scene_bounds,
+                   int morton_bits, int opt_rounds) {
+    hipStream_t s = ctx->stream;
+    const int n = h->n;
+    h->morton_bits = morton_bits;
+    h->opt_rounds = opt_rounds;
+    HIPCHK(ctx, hipEventRecord(h->ev.e[0], s));
+    if (scene_bounds) {
+        uint32_t o6[6];
+        host_to_ord(scene_bounds, o6);
+        HIPCHK(ctx, hipMemcpyAsync(h->b.scene_ord, o6, sizeof o6, hipMemcpyHostToDevice, s));
+    } else {
+        HIPCHK(ctx, lbvh_scene_bounds(tris_dev, n, h->b.scene_ord, s));
+    }
+    HIPCHK(ctx, lbvh_build(h->b, tris_dev, n, morton_bits, s, &h->ev));
+    float root_lbvh[8];
+    HIPCHK(ctx, hipMemcpyAsync(root_lbvh, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
+    for (int r = 0; r < opt_rounds; ++r) HIPCHK(ctx, treelet_round_impl(h->b, n, r == 0, s));
+    HIPCHK(ctx, hipEventRecord(h->ev.e[6], s));
+    if (opt_rounds > 0) HIPCHK(ctx, lbvh_refs(h->b, n, s));
+    HIPCHK(ctx, lbvh_flatten(h->b, tris_dev, h->mesh, n, opt_rounds == 0, h->tn, h->tt, h->root, s));
+    HIPCHK(ctx, hipEventRecord(h->ev.e[7], s));
+    float root_final[8];
+    HIPCHK(ctx, hipMemcpyAsync(root_final, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    cav_bvh_stats& st = h->stats;
+    st.n = n;
+    st.morton_bits = morton_bits;
+    st.opt_rounds = opt_rounds;
+    st.ms_bounds = elapsed(h->ev.e[0], h->ev.e[1]);
+    st.ms_morton = elapsed(h->ev.e[1], h->ev.e[2]);
+    st.ms_sort = elapsed(h->ev.e[2], h->ev.e[3]);
+    st.ms_karras = elapsed(h->ev.e[3], h->ev.e[4]);
+    st.ms_refit = elapsed(h->ev.e[4], h->ev.e[5]);
+    st.ms_treelet = elapsed(h->ev.e[5], h->ev.e[6]);
+    st.ms_flatten = elapsed(h->ev.e[6], h->ev.e[7]);
+    st.ms_total = elapsed(h->ev.e[0], h->ev.e[7]);
+    st.root_cost_lbvh = root_lbvh[7];
+    st.root_cost_final = root_final[7];
+    return CAV_OK;
+}
+
+extern "C" {
+
+int cav_ctx_create(int device, cav_ctx** out) {
+    if (!out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return CAV_ERR_HIP;
+    if (device < 0 || device >= count) return CAV_ERR_INVALID;
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return CAV_ERR_HIP;
+    cav_ctx* c = new cav_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return CAV_ERR_HIP; }
+    *out = c;
+    return CAV_OK;
+}
+
+int cav_ctx_destroy(cav_ctx* ctx) {
+    if (!ctx) return CAV_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx_release(ctx);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CAV_OK;
+}
+
+const char* cav_last_error(const cav_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* cav_ctx_stream(cav_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int cav_ctx_synchronize(cav_ctx* ctx) {
+    if (!ctx) return CAV_ERR_INVALID;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CAV_OK;
+}
+
+int cav_bvh_build_device(cav_ctx* ctx, const float* tris_dev, int n, const float* scene_bounds,
+                         const int32_t* tri_mesh_dev, int morton_bits, int opt_rounds,
+                         cav_bvh** out) {
+    if (!ctx || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    if (!tris_dev) return fail(ctx, CAV_ERR_INVALID, "triangles is null");
+    if (n < 2) return fail(ctx, CAV_ERR_INVALID, "BVH needs at least 2 triangles");
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ctx, CAV_ERR_INVALID, "morton_bits must be 30 or 63");
+    if (opt_rounds < 0) return fail(ctx, CAV_ERR_INVALID, "opt_rounds must be >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    cav_bvh* h = new cav_bvh();
+    h->ctx = ctx;
+    h->n = n;
+    int rc = bvh_alloc(ctx, h, n);
+    if (rc == CAV_OK && tri_mesh_dev) {
+        rc = dev_alloc(ctx, (void**)&h->mesh, sizeof(int32_t) * n, "mesh");
+        if (rc == CAV_OK) {
+            hipError_t e = hipMemcpyAsync(h->mesh, tri_mesh_dev, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "copy mesh ids");
+        }
+    }
+    if (rc == CAV_OK) rc = bvh_run(ctx, h, tris_dev, scene_bounds, morton_bits, opt_rounds);
+    if (rc != CAV_OK) { bvh_free(h); delete h; return rc; }
+    *out = h;
+    return CAV_OK;
+}
+
+int cav_bvh_build(cav_ctx* ctx, const float* tris, int n, const float* scene_bounds,
+                  const int32_t* tri_mesh, int morton_bits, int opt_rounds, cav_bvh** out) {
+    if (!ctx || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    if (!tris) return fail(ctx, CAV_ERR_INVALID, "triangles is null");
+    if (n < 2) return fail(ctx, CAV_ERR_INVALID, "BVH needs at least 2 triangles");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    float* dt = nullptr;
+    int32_t* dm = nullptr;
+    int rc = dev_alloc(ctx, (void**)&dt, sizeof(float) * 9 * (size_t)n, "triangles");
+    if (rc != CAV_OK) return rc;
+    if (tri_mesh) rc = dev_alloc(ctx, (void**)&dm, sizeof(int32_t) * (size_t)n, "mesh");
+    hipError_t e = hipSuccess;
+    if (rc == CAV_OK) e = hipMemcpy(dt, tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e == hipSuccess && dm) e = hipMemcpy(dm, tri_mesh, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e != hipSuccess) rc = hip_fail(ctx, e, "upload triangles");
+    if (rc == CAV_OK) rc = cav_bvh_build_device(ctx, dt, n, scene_bounds, dm, morton_bits, opt_rounds, out);
+    (void)hipFree(dt);
+    if (dm) (void)hipFree(dm);
+    return rc;
+}
+
+int cav_bvh_rebuild_device(cav_ctx* ctx, cav_bvh* h, const float* tris_dev,
+                           const float* scene_bounds, int morton_bits, int opt_rounds) {
+    if (!ctx || !h || !tris_dev) return CAV_ERR_INVALID;
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ctx, CAV_ERR_INVALID, "morton_bits must be 30 or 63");
+    if (opt_rounds < 0) return fail(ctx, CAV_ERR_INVALID, "opt_rounds must be >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return bvh_run(ctx, h, tris_dev, scene_bounds, morton_bits, opt_rounds);
+}
+
+int cav_bvh_destroy(cav_bvh* h) {
+    if (!h) return CAV_ERR_INVALID;
+    if (h->ctx) { (void)hipSetDevice(h->ctx->device); (void)hipStreamSynchronize(h->ctx->stream); }
+    bvh_free(h);
+    delete h;
+    return CAV_OK;
+}
+
+int cav_bvh_stats_get(const cav_bvh* h, cav_bvh_stats* out) {
+    if (!h || !out) return CAV_ERR_INVALID;
+    *out = h->stats;
+    return CAV_OK;
+}
+
+int cav_bvh_export(cav_ctx* ctx, const cav_bvh* h, float* box6, float* sa, float* cost,
+                   int32_t* left, int32_t* right, int32_t* parent, int32_t* prim,
+                   uint64_t* sorted_codes, int32_t* order) {
+    if (!ctx || !h) return CAV_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const int n = h->n;
+    const int64_t m = 2 * (int64_t)n - 1;
+    std::vector<NodeBox> nb(m);
+    std::vector<int4> tp(m);
+    HIPCHK(ctx, hipMemcpy(nb.data(), h->b.nbox, sizeof(NodeBox) * m, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(tp.data(), h->b.topo, sizeof(int4) * m, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; ++i) {
+        if (box6) {
+            box6[i * 6 + 0] = nb[i].a.x; box6[i * 6 + 1] = nb[i].a.y; box6[i * 6 + 2] = nb[i].a.z;
+            box6[i * 6 + 3] = nb[i].b.x; box6[i * 6 + 4] = nb[i].b.y; box6[i * 6 + 5] = nb[i].b.z;
+        }
+        if (sa) sa[i] = nb[i].a.w;
+        if (cost) cost[i] = nb[i].b.w;
+        if (left) left[i] = tp[i].x;
+        if (right) right[i] = tp[i].y;
+        if (parent) parent[i] = tp[i].z;
+        if (prim) prim[i] = tp[i].w;
+    }
+    if (sorted_codes) {
+        if (h->morton_bits == 63) {
+            HIPCHK(ctx, hipMemcpy(sorted_codes, h->b.keys[h->b.sorted], sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint32_t> k(n);
+            HIPCHK(ctx, hipMemcpy(k.data(), h->b.keys[h->b.sorted], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; ++i) sorted_codes[i] = k[i];
+        }
+    }
+    if (order) HIPCHK(ctx, hipMemcpy(order, h->b.order, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return CAV_OK;
+}
+
+int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
+                            int32_t* prim, float* t) {
+    if (!ctx || !h || (nrays > 0 && (!rays || !prim || !t))) return CAV_ERR_INVALID;
+    HIPCHK(ctx, trace_closest(h->tn, h->tt, h->n, h->root, rays, nrays, prim, t, nullptr, nullptr,
+                              h->b.counters, ctx->stream));
+    return CAV_OK;
+}
+
+int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
+                             const float* limit, int64_t nrays, uint8_t* visible) {
+    if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
+    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, nullptr, nullptr, nrays, visible,
+                               h->b.counters + 1, ctx->stream));
+    return CAV_OK;
+}
+
+int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
+                             int32_t* prim, float* t, float* u, float* v) {
+    if (!ctx || !h) return CAV_ERR_INVALID;
+    if (nrays < 0 || (nrays > 0 && (!rays || !prim || !t)))
+        return fail(ctx, CAV_ERR_INVALID, "null ray or output buffer");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 4, ctx->stream));
+    HIPCHK(ctx, trace_closest(h->tn, h->tt, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->stream));
+    uint32_t ovf = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ovf) return fail(ctx, CAV_ERR_OVERFLOW, "traversal stack overflow in " + std::to_string(ovf) + " pushes");
+    return CAV_OK;
+}
+
+int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
+                              const float* limit, const int32_t* src, const int32_t* dst,
+                              int64_t nrays, uint8_t* visible) {
+    if (!ctx || !h) return CAV_ERR_INVALID;
+    if (nrays < 0 || (nrays > 0 && (!rays || !limit || !visible)))
+        return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
+    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, src, dst, nrays, visible, h->b.counters + 1, ctx->stream));
+    uint32_t ovf = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ovf) return fail(ctx, CAV_ERR_OVERFLOW, "traversal stack overflow in " + std::to_string(ovf) + " pushes");
+    return CAV_OK;
+}
+
+int cav_trace_closest(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
+                      int32_t* prim, float* t, float* u, float* v) {
+    if (!ctx || !h) return CAV_ERR_INVALID;
+    if (nrays < 0 || (nrays > 0 && (!rays || !prim || !t)))
+        return fail(ctx, CAV_ERR_INVALID, "null ray or output buffer");
+    if (nrays == 0) return CAV_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t r = (size_t)nrays;
+    void *dr = nullptr, *dp = nullptr, *dt = nullptr, *du = nullptr, *dv = nullptr;
+    int rc = dev_alloc(ctx, &dr, 24 * r, "rays");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, &dp, 4 * r, "hit prim");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, &dt, 4 * r, "hit t");
+    if (rc == CAV_OK && u) rc = dev_alloc(ctx, &du, 4 * r, "hit u");
+    if (rc == CAV_OK && v) rc = dev_alloc(ctx, &dv, 4 * r, "hit v");
+    hipError_t e = hipSuccess;
+    if (rc == CAV_OK) e = hipMemcpy(dr, rays, 24 * r, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e != hipSuccess) rc = hip_fail(ctx, e, "upload rays");
+    int trc = CAV_OK;
+    if (rc == CAV_OK) {
+        trc = cav_trace_closest_device(ctx, h, (const float*)dr, nrays, (int32_t*)dp, (float*)dt, (float*)du, (float*)dv);
+        if (trc != CAV_OK && trc != CAV_ERR_OVERFLOW) rc = trc;
+    }
+    if (rc == CAV_OK) {
+        e = hipMemcpy(prim, dp, 4 * r, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(t, dt, 4 * r, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && u) e = hipMemcpy(u, du, 4 * r, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && v) e = hipMemcpy(v, dv, 4 * r, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "download hits");
+    }
+    for (void* p : {dr, dp, dt, du, dv}) if (p) (void)hipFree(p);
+    return rc != CAV_OK ? rc : trc;
+}
+
+int cav_trace_occluded(cav_ctx* ctx, const cav_bvh* h, const float* rays, const float* limit,
+                       const int32_t* src, const int32_t* dst, int64_t nrays, uint8_t* visible) {
+    if (!ctx || !h) return CAV_ERR_INVALID;
+    if (nrays < 0 || (nrays > 0 && (!rays || !limit || !visible)))
+        return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
+    if (nrays == 0) return CAV_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t r = (size_t)nrays;
+    void *dr = nullptr, *dl = nullptr, *ds = nullptr, *dd = nullptr, *dvis = nullptr;
+    int rc = dev_alloc(ctx, &dr, 24 * r, "rays");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, &dl, 4 * r, "limits");
+    if (rc == CAV_OK && src) rc = dev_alloc(ctx, &ds, 4 * r, "src mesh");
+    if (rc == CAV_OK && dst) rc = dev_alloc(ctx, &dd, 4 * r, "dst mesh");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, &dvis, r, "visible");
+    hipError_t e = hipSuccess;
+    if (rc == CAV_OK) e = hipMemcpy(dr, rays, 24 * r, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e == hipSuccess) e = hipMemcpy(dl, limit, 4 * r, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e == hipSuccess && src) e = hipMemcpy(ds, src, 4 * r, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e == hipSuccess && dst) e = hipMemcpy(dd, dst, 4 * r, hipMemcpyHostToDevice);
+    if (rc == CAV_OK && e != hipSuccess) rc = hip_fail(ctx, e, "upload rays");
+    int trc = CAV_OK;
+    if (rc == CAV_OK) {
+        trc = cav_trace_occluded_device(ctx, h, (const float*)dr, (const float*)dl, (const int32_t*)ds,
+                                        (const int32_t*)dd, nrays, (uint8_t*)dvis);
+        if (trc != CAV_OK && trc != CAV_ERR_OVERFLOW) rc = trc;
+    }
+    if (rc == CAV_OK) {
+        e = hipMemcpy(visible, dvis, r, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "download visibility");
+    }
+    for (void* p : {dr, dl, ds, dd, dvis}) if (p) (void)hipFree(p);
+    return rc != CAV_OK ? rc : trc;
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// LBVH build kernels: scene bounds, Morton codes, Karras radix tree, refit,
+// and the flatten into the traversal layout.
+//
+// Reference: BVH/BVH.cu:46-370 (fillLeafBoundingBoxes, fillLeafMortonCodes,
+// buildRadixTree, computeBounds, mergeNodesArray, copyToBVH) and the scene
+// bounds loop of Core/Scene.cu:131-136.
+//
+// Data layout in HBM (N triangles, M = 2N-1 nodes, unified id space: internal
+// node i at i, the leaf of sorted position k at N-1+k before restructuring):
+//   tris   : float[N][9]  v0, v1, v2 (caller-owned, original order)
+//   keys   : u32[N] or u64[N] Morton codes (sorted in place by the radix sort)
+//   order  : u32[N] sorted position -> original triangle index
+//   nbox   : NodeBox[M]   (lo, sa | hi, cost), 32 B
+//   ntopo  : int4[M]      (left, right, parent, prim), 16 B
+//   visited: u32[M]       refit / treelet climb gates
+#include "dev_common.h"
+#include "lbvh.h"
+
+namespace cav {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ v3 load_v(const float* __restrict__ t, int64_t tri, int j) {
+    const float* p = t + tri * 9 + j * 3;
+    return mk3(p[0], p[1], p[2]);
+}
+
+// Core/Triangle.h:242-268
+__device__ __forceinline__ void tri_box(v3 a, v3 b, v3 c, v3& lo, v3& hi) {
+    lo = mk3(fmn(a.x, fmn(b.x, c.x)), fmn(a.y, fmn(b.y, c.y)), fmn(a.z, fmn(b.z, c.z)));
+    hi = mk3(fmx(a.x, fmx(b.x, c.x)), fmx(a.y, fmx(b.y, c.y)), fmx(a.z, fmx(b.z, c.z)));
+}
+
+__global__ void k_bounds_init(uint32_t* ob) {
+    if (threadIdx.x < 3) ob[threadIdx.x] = 0xffffffffu;
+    else if (threadIdx.x < 6) ob[threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+
+// Union of all triangle boxes as order-preserving integers (NaN skipped), so
+// the result does not depend on reduction order.
+__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris, int64_t n,
+                                                   uint32_t* __restrict__ ob) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const float* p = tris + i * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            float f = p[k];
+            if (f == f) {
+                uint32_t o = f2ord(f);
+                mn[k % 3] = min(mn[k % 3], o);
+                mx[k % 3] = max(mx[k % 3], o);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { mn[a] = wave_min(mn[a]); mx[a] = wave_max(mx[a]); }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { atomicMin(&ob[a], mn[a]); atomicMax(&ob[3 + a], mx[a]); }
+    }
+}
+
+__device__ __forceinline__ uint64_t sat_u64(float v) {
+    if (!(v > 0.f)) return 0;
+    if (v >= 18446744073709551616.0f) return ~0ull;
+    return (uint64_t)v;
+}
+__device__ __forceinline__ uint32_t sat_u32(float v) {
+    if (!(v > 0.f)) return 0;
+    if (v >= 4294967296.0f) return ~0u;
+    return (uint32_t)v;
+}
+// BVH/BVH.cu:59-76
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    if (x >= (1ull << 21)) x = 1ull << 20;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    if (x >= (1u << 10)) x = 1u << 9;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+template <typename K>
+__device__ __forceinline__ K morton_code(v3 c);
+template <>
+__device__ __forceinline__ uint64_t morton_code<uint64_t>(v3 c) {
+    const float f = 2097152.0f;
+    return (spread21(sat_u64(c.x * f)) << 2) | (spread21(sat_u64(c.y * f)) << 1) |
+           spread21(sat_u64(c.z * f));
+}
+template <>
+__device__ __forceinline__ uint32_t morton_code<uint32_t>(v3 c) {
+    const float f = 1024.0f;
+    return (spread10(sat_u32(c.x * f)) << 2) | (spread10(sat_u32(c.y * f)) << 1) |
+           spread10(sat_u32(c.z * f));
+}
+
+// BVH/BVH.cu:79-92: code of the triangle-box centroid normalised by the scene
+// box; values[i] = i so the stable sort yields the original index.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tris, int64_t n,
+                                                   const uint32_t* __restrict__ ob,
+                                                   K* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const v3 slo = mk3(ord2f(ob[0]), ord2f(ob[1]), ord2f(ob[2]));
+    const v3 shi = mk3(ord2f(ob[3]), ord2f(ob[4]), ord2f(ob[5]));
+    const v3 sc = scale3(add3(slo, shi), 0.5f);
+    const v3 se = sub3(shi, slo);
+    v3 lo, hi;
+    tri_box(load_v(tris, i, 0), load_v(tris, i, 1), load_v(tris, i, 2), lo, hi);
+    const v3 c = div3(sub3(scale3(add3(lo, hi), 0.5f), sc), se);
+    keys[i] = morton_code<K>(c);
+    vals[i] = (uint32_t)i;
+}
+
+// BVH/BVH.cu:101-120 with W-bit key words holding keybits-bit codes.
+template <typename K>
+__device__ __forceinline__ int prefix_len(const K* __restrict__ codes, int keybits, uint32_t i,
+                                          uint32_t j) {
+    const K a = codes[i], b = codes[j];
+    if (a != b) {
+        const int w = (int)sizeof(K) * 8;
+        const int clz = sizeof(K) == 8 ? __clzll((long long)(a ^ b)) : __clz((int)(a ^ b));
+        return keybits - w + clz;
+    }
+    const uint32_t y = i ^ j;
+    if (y == 0) return keybits + 32;
+    return keybits + __clz((int)y);
+}
+
+// BVH/BVH.cu:122-215: internal node i of the Karras radix tree.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_karras(const K* __restrict__ codes, int n,
+                                                   int keybits, int4* __restrict__ topo,
+                                                   uint32_t* __restrict__ visited) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = 1;
+    if (i > 0) {
+        int a = prefix_len(codes, keybits, i, i + 1) - prefix_len(codes, keybits, i, i - 1);
+        d = a >= 0 ? 1 : -1;
+    }
+    const int dmin = i > 0 ? prefix_len(codes, keybits, i, i - d) : 0;
+    uint32_t maxlen = 2;
+    while (true) {
+        const uint32_t j = (uint32_t)i + maxlen * (uint32_t)d;
+        if (!(j < (uint32_t)n)) break;
+        if (!(prefix_len(codes, keybits, i, j) > dmin)) break;
+        maxlen *= 2;
+    }
+    uint32_t len = 0;
+    for (int p = 1;; ++p) {
+        const uint32_t t = maxlen >> p;
+        if (t == 0) break;
+        const int j = (int)((uint32_t)i + (len + t) * (uint32_t)d);
+        if (j >= 0 && j < n && prefix_len(codes, keybits, i, j) > dmin) len += t;
+    }
+    int j = (int)((uint32_t)i + len * (uint32_t)d);
+    if (i == 0) j = n - 1;
+    const int plen = prefix_len(codes, keybits, i, j);
+    uint32_t split = 0;
+    for (int p = 1;; ++p) {
+        const uint32_t t = (uint32_t)ceilf((float)len / (float)(1 << p));
+        const int k = (int)((uint32_t)i + (split + t) * (uint32_t)d);
+        if (k >= 0 && k < n && prefix_len(codes, keybits, i, k) > plen) split += t;
+        if (t <= 1) break;
+    }
+    const int spos = (int)((uint32_t)i + split * (uint32_t)d) + min(0, d);
+    const int lc = spos, rc = spos + 1;
+    const bool lleaf = min(i, j) == lc;
+    const bool rleaf = (max(i, j) == rc) || (rc == n - 1);
+    const int lid = lleaf ? (n - 1 + lc) : lc;
+    const int rid = rleaf ? (n - 1 + rc) : rc;
+    int* me = reinterpret_cast<int*>(&topo[i]);
+    me[0] = lid;
+    me[1] = rid;
+    me[3] = -1;
+    reinterpret_cast<int*>(&topo[lid])[2] = i;
+    reinterpret_cast<int*>(&topo[rid])[2] = i;
+    if (i == 0) me[2] = 0;
+    visited[i] = 0;
+}
+
+// BVH/BVH.cu:46-56 + 217-259: leaf k fills its node from triangle order[k],
+// then climbs; the second visitor of an internal node unions its children.
+// Release/acquire fences at agent scope order the child writes before the
+// gate increment and the sibling reads after it.
+__global__ __launch_bounds__(kBlock) void k_refit(const float* __restrict__ tris, int n,
+                                                  const uint32_t* __restrict__ order,
+                                                  NodeBox* nbox, int4* topo,
+                                                  uint32_t* visited) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int t = (int)order[k];
+    v3 lo, hi;
+    tri_box(load_v(tris, t, 0), load_v(tris, t, 1), load_v(tris, t, 2), lo, hi);
+    const float sa = surface_area(lo, hi);
+    const int node = n - 1 + k;
+    int cur = topo[node].z;
+    nbox[node] = NodeBox{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa))};
+    topo[node] = make_int4(-1, -1, cur, t);
+    // a valid tree is at most n-1 levels deep; the cap only guards corruption
+    for (int step = 0; step < n; ++step) {
+        __threadfence();
+        if (atomicAdd(&visited[cur], 1u) != 1u) return;  // first visitor stops
+        __threadfence();
+        const int4 tp = topo[cur];
+        const NodeBox bl = nbox[tp.x], br = nbox[tp.y];
+        const v3 ulo = mk3(fmn(bl.a.x, br.a.x), fmn(bl.a.y, br.a.y), fmn(bl.a.z, br.a.z));
+        const v3 uhi = mk3(fmx(bl.b.x, br.b.x), fmx(bl.b.y, br.b.y), fmx(bl.b.z, br.b.z));
+        const float usa = surface_area(ulo, uhi);
+        const float c = internal_cost3(usa, bl.b.w, br.b.w);
+        nbox[cur] = NodeBox{make_float4(ulo.x, ulo.y, ulo.z, usa), make_float4(uhi.x, uhi.y, uhi.z, c)};
+        if (cur == 0) return;
+        cur = tp.z;
+    }
+}
+
+// ---- device-wide exclusive scan of leaf flags (restructured trees) ----
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl(v);
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) { if (w < wave) off += ws[w]; total += ws[w]; }
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_leafcount(const int4* __restrict__ topo, int64_t m,
+                                                      uint32_t* __restrict__ partial) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t c = 0;
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < m && topo[base + j].w >= 0) ++c;
+    uint32_t tot;
+    block_excl(c, ws, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(kBlock) void k_partials(uint32_t* partial, int np) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const int per = (np + kBlock - 1) / kBlock;
+    const int t0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int t = t0; t < t0 + per && t < np; ++t) s += partial[t];
+    uint32_t tot;
+    uint32_t run = block_excl(s, ws, tot);
+    for (int t = t0; t < t0 + per && t < np; ++t) { uint32_t c = partial[t]; partial[t] = run; run += c; }
+}
+// ref[id] = ~(leaf rank) for leaves, id - (leaf rank) for internal nodes.
+__global__ __launch_bounds__(kBlock) void k_refs(const int4* __restrict__ topo, int64_t m,
+                                                 const uint32_t* __restrict__ partial,
+                                                 int32_t* __restrict__ ref) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    bool leaf[kScanItems];
+    uint32_t c = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        leaf[j] = base + j < m && topo[base + j].w >= 0;
+        c += leaf[j];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl(c, ws, tot) + partial[blockIdx.x];
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j >= m) break;
+        ref[base + j] = leaf[j] ? ~(int32_t)run : (int32_t)(base + j - run);
+        run += leaf[j];
+    }
+}
+
+__device__ __forceinline__ int32_t ref_of(const int32_t* ref, int n, int id) {
+    if (ref) return ref[id];
+    return id < n - 1 ? id : ~(id - (n - 1));
+}
+
+// copyToBVH (BVH/BVH.cu:304-315) into the traversal layout: one 64-byte node
+// per internal node holding both child boxes, one 48-byte triangle per leaf.
+__global__ __launch_bounds__(kBlock) void k_flatten(const NodeBox* __restrict__ nbox,
+                                                    const int4* __restrict__ topo, int n,
+                                                    const int32_t* __restrict__ ref,
+                                                    const float* __restrict__ tris,
+                                                    const int32_t* __restrict__ mesh,
+                                                    TNode* __restrict__ tn,
+                                                    TTri* __restrict__ tt,
+                                                    float* __restrict__ root) {
+    const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t m = 2 * (int64_t)n - 1;
+    if (id >= m) return;
+    const int4 tp = topo[id];
+    const int32_t me = ref_of(ref, n, (int)id);
+    if (id == 0) {
+        const NodeBox b = nbox[0];
+        root[0] = b.a.x; root[1] = b.a.y; root[2] = b.a.z;
+        root[3] = b.b.x; root[4] = b.b.y; root[5] = b.b.z;
+    }
+    if (tp.w < 0) {
+        const NodeBox bl = nbox[tp.x], br = nbox[tp.y];
+        TNode o;
+        o.l0 = make_float4(bl.a.x, bl.a.y, bl.a.z, bl.b.x);
+        o.l1 = make_float4(bl.b.y, bl.b.z, br.a.x, br.a.y);
+        o.r1 = make_float4(br.a.z, br.b.x, br.b.y, br.b.z);
+        o.refs = make_int4(ref_of(ref, n, tp.x), ref_of(ref, n, tp.y), 0, 0);
+        tn[me] = o;
+    } else {
+        const int p = tp.w;
+        const v3 v0 = load_v(tris, p, 0), v1 = load_v(tris, p, 1), v2 = load_v(tris, p, 2);
+        const v3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
+        TTri o;
+        o.a = make_float4(v0.x, v0.y, v0.z, __int_as_float(p));
+        o.b = make_float4(e1.x, e1.y, e1.z, __int_as_float(mesh ? mesh[p] : 0));
+        o.c = make_float4(e2.x, e2.y, e2.z, 0.f);
+        tt[~me] = o;
+    }
+}
+
+template <typename K>
+hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hipStream_t s,
+                      StageEvents* ev) {
+    SortBuffers<K> sb;
+    sb.keys[0] = reinterpret_cast<K*>(b.keys[0]);
+    sb.keys[1] = reinterpret_cast<K*>(b.keys[1]);
+    sb.vals[0] = b.vals[0];
+    sb.vals[1] = b.vals[1];
+    sb.hist = b.hist;
+    sb.counts = b.counts;
+    sb.offsets = b.offsets;
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (ev) hipEventRecord(ev->e[1], s);
+    hipLaunchKernelGGL(k_morton<K>, dim3(nb), dim3(kBlock), 0, s, tris, (int64_t)n,
+                       b.scene_ord, sb.keys[0], sb.vals[0]);
+    if (ev) hipEventRecord(ev->e[2], s);
+    hipError_t e = radix_sort_pairs(sb, n, keybits, s);
+    if (e != hipSuccess) return e;
+    if (ev) hipEventRecord(ev->e[3], s);
+    b.sorted = sb.result;
+    const K* codes = sb.keys[sb.result];
+    b.order = sb.vals[sb.result];
+    const int nk = (n - 1 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_karras<K>, dim3(nk), dim3(kBlock), 0, s, codes, n, keybits, b.topo,
+                       b.visited);
+    if (ev) hipEventRecord(ev->e[4], s);
+    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kBlock), 0, s, tris, n, b.order, b.nbox, b.topo,
+                       b.visited);
+    if (ev) hipEventRecord(ev->e[5], s);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t lbvh_scene_bounds(const float* tris, int n, uint32_t* scene_ord, hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, scene_ord);
+    int blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, s, tris, (int64_t)n, scene_ord);
+    return hipGetLastError();
+}
+
+hipError_t lbvh_build(LbvhBuffers& b, const float* tris, int n, int morton_bits, hipStream_t s,
+                      StageEvents* ev) {
+    if (morton_bits == 63) return build_impl<uint64_t>(b, tris, n, 63, s, ev);
+    return build_impl<uint32_t>(b, tris, n, 30, s, ev);
+}
+
+hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s) {
+    const int64_t m = 2 * (int64_t)n - 1;
+    const int np = (int)((m + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_leafcount, dim3(np), dim3(kBlock), 0, s, b.topo, m, b.partial);
+    hipLaunchKernelGGL(k_partials, dim3(1), dim3(kBlock), 0, s, b.partial, np);
+    hipLaunchKernelGGL(k_refs, dim3(np), dim3(kBlock), 0, s, b.topo, m, b.partial, b.ref);
+    return hipGetLastError();
+}
+
+hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, int n,
+                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s) {
+    const int64_t m = 2 * (int64_t)n - 1;
+    const int nb = (int)((m + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_flatten, dim3(nb), dim3(kBlock), 0, s, b.nbox, b.topo, n,
+                       identity_refs ? nullptr : b.ref, tris, mesh, tn, tt, root);
+    return hipGetLastError();
+}
+
+int64_t lbvh_partials(int64_t m) { return (m + kScanTile - 1) / kScanTile; }
+
+}  // namespace cav

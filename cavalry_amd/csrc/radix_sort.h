@@ -1,0 +1,23 @@
+// Stable LSD radix sort of (key, u32 value) pairs; see radix_sort.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cav {
+
+template <typename K>
+struct SortBuffers {
+    K* keys[2] = {nullptr, nullptr};        // keys[0] holds the input
+    uint32_t* vals[2] = {nullptr, nullptr};  // vals[0] holds the input
+    uint32_t* hist = nullptr;                // passes * 256
+    uint32_t* counts = nullptr;              // 256 * tiles
+    uint32_t* offsets = nullptr;             // 256 * tiles
+    int result = 0;                          // index of the buffer holding the output
+};
+
+int64_t sort_tiles(int64_t n);
+int sort_radix();
+hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s);
+hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s);
+
+}  // namespace cav

@@ -1,0 +1,120 @@
+/* C ABI of the MI355X-native Cavalry hot path (libcavalry_amd.so).
+ *
+ * Plain pointers and sizes only. Every entry point returns a cav_status; the
+ * message of the last failure on a context is available from cav_last_error.
+ * Nothing calls exit(). One HIP stream per context; entry points are
+ * synchronous with respect to the host unless their name ends in _async.
+ *
+ * Geometry conventions:
+ *   triangles : float[n][9]  v0.xyz v1.xyz v2.xyz (a triangle soup, the
+ *               reference's Triangle{mesh, index} resolved to its vertices)
+ *   rays      : float[r][6]  origin.xyz direction.xyz
+ *   bounds    : float[6]     min.xyz max.xyz
+ * "_device" variants take pointers to device (HBM) memory the caller owns;
+ * the others take host memory and copy it in and out.
+ */
+#ifndef CAVALRY_AMD_H
+#define CAVALRY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CAV_OK = 0,
+    CAV_ERR_INVALID = 1,   /* bad argument (null pointer, n < 2, bad morton_bits ...) */
+    CAV_ERR_HIP = 2,       /* a HIP runtime call failed */
+    CAV_ERR_OOM = 3,       /* device allocation failed */
+    CAV_ERR_OVERFLOW = 4,  /* a traversal exceeded the 64-entry stack (results partial) */
+    CAV_ERR_PARSE = 5,     /* malformed scene description */
+    CAV_ERR_BUSY = 6,      /* serving: no capacity for the request right now */
+    CAV_ERR_NOT_FOUND = 7  /* serving: unknown request id */
+} cav_status;
+
+typedef struct cav_ctx cav_ctx;
+typedef struct cav_bvh cav_bvh;
+
+/* ---- context ---------------------------------------------------------- */
+int cav_ctx_create(int device, cav_ctx** out);
+int cav_ctx_destroy(cav_ctx* ctx);
+const char* cav_last_error(const cav_ctx* ctx);
+/* the context's hipStream_t, for callers that record events on it */
+void* cav_ctx_stream(cav_ctx* ctx);
+int cav_ctx_synchronize(cav_ctx* ctx);
+/* library version string */
+const char* cav_version(void);
+
+/* ---- LBVH build -------------------------------------------------------- */
+/* Per-stage device time of the last build (HIP events on the context stream)
+ * and the SAH cost of the root before and after treelet restructuring. */
+typedef struct {
+    float ms_bounds, ms_morton, ms_sort, ms_karras, ms_refit, ms_treelet, ms_flatten, ms_total;
+    float root_cost_lbvh, root_cost_final;
+    int n, morton_bits, opt_rounds;
+} cav_bvh_stats;
+
+/* Replaces static BVH BVH::build(Triangle*, int, const AABB&, int)
+ * (BVH/BVH.h:19, BVH/BVH.cu:317-370) together with optimizeBVH
+ * (BVH/Optimization.h:4, BVH/Optimization.cu:509-525): Morton codes
+ * (morton_bits 30 or 63; BVH/BVH.cu:14-27 selects 63), stable radix sort,
+ * Karras radix tree, bottom-up refit, then opt_rounds treelet rounds.
+ * scene_bounds may be NULL: the union of the triangle boxes is used, as
+ * Scene::prepareForRender computes it (Core/Scene.cu:131-136).
+ * tri_mesh (may be NULL = all 0) gives each triangle's mesh id for the
+ * source/target exclusion of shadow rays (BVH/BVH.h:139). n >= 2. */
+int cav_bvh_build(cav_ctx* ctx, const float* triangles, int n, const float* scene_bounds,
+                  const int32_t* tri_mesh, int morton_bits, int opt_rounds, cav_bvh** out);
+int cav_bvh_build_device(cav_ctx* ctx, const float* triangles_dev, int n,
+                         const float* scene_bounds, const int32_t* tri_mesh_dev,
+                         int morton_bits, int opt_rounds, cav_bvh** out);
+/* Rebuilds an existing BVH in place from device triangles of the same count,
+ * reusing its device buffers (no allocation). */
+int cav_bvh_rebuild_device(cav_ctx* ctx, cav_bvh* bvh, const float* triangles_dev,
+                           const float* scene_bounds, int morton_bits, int opt_rounds);
+int cav_bvh_destroy(cav_bvh* bvh);
+int cav_bvh_stats_get(const cav_bvh* bvh, cav_bvh_stats* out);
+/* Node arrays in the reference's unified id space (BVHRestructureNode,
+ * BVH/BVHNode.h:48-66; internal i at i, leaves from N-1 before
+ * restructuring): 2n-1 entries each. prim = original triangle index of a leaf
+ * or -1; left/right = -1 for leaves; the root's parent is 0. Any pointer may
+ * be NULL. sorted_codes (u64[n]) / order (int32[n]) are the sorted Morton
+ * codes and the triangle index at each sorted position. */
+int cav_bvh_export(cav_ctx* ctx, const cav_bvh* bvh, float* box6, float* surface_area,
+                   float* cost, int32_t* left, int32_t* right, int32_t* parent, int32_t* prim,
+                   uint64_t* sorted_codes, int32_t* order);
+
+/* ---- traversal --------------------------------------------------------- */
+/* Replaces BVH::intersect (BVH/BVH.h:23-107) as used by
+ * SceneHandle::intersect (Core/Scene.h:42-51), geometry only: the closest
+ * hit's original triangle index (-1 on a miss), distance and barycentric u, v.
+ * t, u, v of a miss are 0. u, v may be NULL. */
+int cav_trace_closest(cav_ctx* ctx, const cav_bvh* bvh, const float* rays, int64_t nrays,
+                      int32_t* hit_prim, float* hit_t, float* hit_u, float* hit_v);
+int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
+                             int64_t nrays, int32_t* hit_prim_dev, float* hit_t_dev,
+                             float* hit_u_dev, float* hit_v_dev);
+/* Replaces BVH::testVisibility (BVH/BVH.h:109-189) as used by
+ * SceneHandle::testVisibility (Core/Scene.h:53-56): visible[i] = 1 when no
+ * triangle blocks ray i before limit[i] (+inf = no limit, the reference's
+ * useDistanceLimit = false). Triangles of mesh src_mesh[i] or dst_mesh[i]
+ * are ignored (arrays may be NULL = -1). */
+int cav_trace_occluded(cav_ctx* ctx, const cav_bvh* bvh, const float* rays, const float* limit,
+                       const int32_t* src_mesh, const int32_t* dst_mesh, int64_t nrays,
+                       uint8_t* visible);
+int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
+                              const float* limit_dev, const int32_t* src_mesh_dev,
+                              const int32_t* dst_mesh_dev, int64_t nrays, uint8_t* visible_dev);
+/* Launch without synchronising (device pointers); errors surface at the next
+ * synchronising call. Used by timing harnesses. */
+int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
+                            int64_t nrays, int32_t* hit_prim_dev, float* hit_t_dev);
+int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
+                             const float* limit_dev, int64_t nrays, uint8_t* visible_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

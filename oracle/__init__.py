@@ -1,0 +1,129 @@
+"""CPU oracle loader (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this package. It restates the reference algorithms on the CPU so the HIP
+kernels can be checked against it; the product never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libcavalry_oracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        _lib = ctypes.CDLL(_SO)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def scene_bounds(tris: np.ndarray) -> np.ndarray:
+    out = np.empty(6, np.float32)
+    lib().ref_scene_bounds(_p(tris), ctypes.c_int(len(tris)), _p(out))
+    return out
+
+
+def morton(tris: np.ndarray, bits: int, scene6=None) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, np.float32)
+    if scene6 is None:
+        scene6 = scene_bounds(tris)
+    out = np.empty(len(tris), np.uint64)
+    lib().ref_morton(_p(tris), ctypes.c_int(len(tris)), _p(np.asarray(scene6, np.float32)),
+                     ctypes.c_int(bits), _p(out))
+    return out
+
+
+class Nodes:
+    """BVH nodes in the reference's unified id space (2N-1 nodes)."""
+
+    def __init__(self, n: int):
+        m = 2 * n - 1
+        self.n = n
+        self.box = np.zeros((m, 6), np.float32)
+        self.sa = np.zeros(m, np.float32)
+        self.cost = np.zeros(m, np.float32)
+        self.left = np.zeros(m, np.int32)
+        self.right = np.zeros(m, np.int32)
+        self.parent = np.zeros(m, np.int32)
+        self.prim = np.zeros(m, np.int32)
+
+    def copy(self) -> "Nodes":
+        c = Nodes(self.n)
+        for k in ("box", "sa", "cost", "left", "right", "parent", "prim"):
+            setattr(c, k, getattr(self, k).copy())
+        return c
+
+
+def lbvh_build(tris: np.ndarray, bits: int = 30, scene6=None):
+    """Returns (sorted_codes, order, Nodes) before any treelet round."""
+    tris = np.ascontiguousarray(tris, np.float32)
+    n = len(tris)
+    nd = Nodes(n)
+    codes = np.empty(n, np.uint64)
+    order = np.empty(n, np.int32)
+    rc = lib().ref_lbvh_build(
+        _p(tris), ctypes.c_int(n), ctypes.c_int(bits),
+        None if scene6 is None else _p(np.asarray(scene6, np.float32)),
+        _p(codes), _p(order), _p(nd.box), _p(nd.sa), _p(nd.cost), _p(nd.left), _p(nd.right),
+        _p(nd.parent), _p(nd.prim))
+    if rc != 0:
+        raise ValueError(f"ref_lbvh_build failed ({rc})")
+    return codes, order, nd
+
+
+def treelet_optimize(nd: Nodes, rounds: int) -> int:
+    done = np.zeros(1, np.int64)
+    rc = lib().ref_treelet_optimize(ctypes.c_int(nd.n), ctypes.c_int(rounds), _p(nd.box),
+                                    _p(nd.sa), _p(nd.cost), _p(nd.left), _p(nd.right),
+                                    _p(nd.parent), _p(nd.prim), _p(done))
+    if rc != 0:
+        raise ValueError(f"ref_treelet_optimize failed ({rc})")
+    return int(done[0])
+
+
+def trace_closest(nd: Nodes, tris: np.ndarray, rays: np.ndarray, stats: bool = False):
+    rays = np.ascontiguousarray(rays, np.float32)
+    r = len(rays)
+    prim = np.empty(r, np.int32)
+    t = np.empty(r, np.float32)
+    u = np.empty(r, np.float32)
+    v = np.empty(r, np.float32)
+    vis = np.empty(r, np.int32) if stats else None
+    dep = np.empty(r, np.int32) if stats else None
+    overflow = lib().ref_trace_closest(
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(tris), _p(rays),
+        ctypes.c_int64(r), _p(prim), _p(t), _p(u), _p(v), _p(vis), _p(dep))
+    out = {"prim": prim, "t": t, "u": u, "v": v, "overflow": overflow}
+    if stats:
+        out["visits"] = vis
+        out["depth"] = dep
+    return out
+
+
+def trace_occluded(nd: Nodes, tris: np.ndarray, rays: np.ndarray, limit: np.ndarray,
+                   mesh=None, src=None, dst=None) -> np.ndarray:
+    rays = np.ascontiguousarray(rays, np.float32)
+    r = len(rays)
+    vis = np.empty(r, np.uint8)
+    lib().ref_trace_occluded(
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(tris), _p(mesh), _p(rays),
+        _p(np.ascontiguousarray(limit, np.float32)), _p(src), _p(dst), ctypes.c_int64(r),
+        _p(vis))
+    return vis

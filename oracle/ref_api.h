@@ -1,0 +1,23 @@
+// CPU oracle: C entry points loaded by the tests and bench.py (ctypes).
+//
+// TEST INFRASTRUCTURE ONLY. The product never links or calls these.
+#pragma once
+#include <cstdint>
+
+extern "C" {
+void ref_scene_bounds(const float* tris, int n, float* out6);
+void ref_morton(const float* tris, int n, const float* scene6, int bits, uint64_t* codes);
+int ref_lbvh_build(const float* tris, int n, int bits, const float* scene6_or_null,
+                   uint64_t* sorted_codes, int32_t* order, float* box, float* sa, float* cost,
+                   int32_t* left, int32_t* right, int32_t* parent, int32_t* prim);
+int ref_treelet_optimize(int n, int rounds, float* box, float* sa, float* cost, int32_t* left,
+                         int32_t* right, int32_t* parent, int32_t* prim, int64_t* treelets_done);
+int ref_trace_closest(const float* box, const int32_t* left, const int32_t* right,
+                      const int32_t* prim, const float* tris, const float* rays, int64_t nrays,
+                      int32_t* hit_prim, float* hit_t, float* hit_u, float* hit_v,
+                      int32_t* visits, int32_t* max_stack);
+int ref_trace_occluded(const float* box, const int32_t* left, const int32_t* right,
+                       const int32_t* prim, const float* tris, const int32_t* mesh,
+                       const float* rays, const float* limit, const int32_t* src_mesh,
+                       const int32_t* dst_mesh, int64_t nrays, uint8_t* visible);
+}
