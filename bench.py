@@ -1,0 +1,246 @@
+"""Benchmark of the LBVH + traversal hot path on one MI355X.
+
+Workload (one step): the LBVH build + traversal microbenchmark over N = 2^24
+random triangles: scene bounds, 30-bit Morton codes, stable radix sort,
+Karras radix tree, refit, 3 rounds of 7-leaf treelet restructuring, flatten,
+then 2^24 closest-hit rays and 2^24 shadow rays. Inputs are generated once,
+uploaded, and resident in HBM before the timed region.
+
+Prints one JSON line. `value` is rays traced per second over the whole step
+(build included); the same line carries the LBVH build rate in Mtris/s, the
+per-stage and per-kernel device times (HIP events on the library's stream),
+the roofline of the dominant LBVH kernel and the CPU-oracle baseline.
+
+  python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 16777216] ...
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes per triangle of the LBVH stages (30-bit keys, K = 4 B):
+# Morton reads the 36 B soup and writes key + index + a 24 B box; each of the
+# four 8-bit sort passes reads and writes key + value; the histogram reads the
+# keys once; Karras reads a key and writes 8 B of children and 8 B of parents;
+# refit reads the parent chain, the gate, both child boxes and writes a box.
+ALG_BYTES_30 = {
+    "morton": 36 + 4 + 4 + 24,
+    "sort_hist": 4,
+    "sort_downsweep": 2 * (4 + 4),  # per pass
+    "karras": 4 + 8 + 8,
+    "refit": 4 + 24 + 24 + 8 + 8 + 24,
+}
+ALG_BYTES_PER_TRI_30 = 248  # S1 68 + S2 68 + S3 20 + S4 92
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, nrays: int, rounds: int, seed: int) -> dict:
+    """The CPU oracle (a C++ restatement of the reference algorithms) on a
+    bounded sample of the same pipeline, timed on this host's cores."""
+    import oracle
+    from cavalry_amd import scenes
+    tris = scenes.gmm_triangles(n, seed=seed)
+    rays = scenes.random_rays(nrays, seed=seed + 1)
+    srays = scenes.random_rays(nrays, seed=seed + 2, stream=2)
+    lim = scenes.shadow_limits(nrays, seed=seed + 3)
+    oracle.lib()
+    t0 = time.perf_counter()
+    _, _, nd = oracle.lbvh_build(tris, 30)
+    t1 = time.perf_counter()
+    oracle.treelet_optimize(nd, rounds)
+    t2 = time.perf_counter()
+    oracle.trace_closest(nd, tris, rays)
+    t3 = time.perf_counter()
+    oracle.trace_occluded(nd, tris, srays, lim)
+    t4 = time.perf_counter()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    total = t4 - t0
+    return {
+        "value": round(2 * nrays / total / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"same pipeline at N=2^{n.bit_length() - 1} triangles with 2^"
+                   f"{nrays.bit_length() - 1} closest + 2^{nrays.bit_length() - 1} shadow rays, "
+                   f"{rounds} treelet rounds; LBVH/traversal OpenMP on {threads} threads, "
+                   f"treelet climb serial; {total:.1f} s"),
+        "lbvh_build_mtris_s": round(n / (t1 - t0) / 1e6, 3),
+        "stage_s": {"lbvh": round(t1 - t0, 3), "treelet": round(t2 - t1, 3),
+                    "closest": round(t3 - t2, 3), "shadow": round(t4 - t3, 3)},
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--rounds", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-rays", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "this build has no multi-GPU path", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    import numpy as np
+    import torch
+
+    from cavalry_amd import BVH, Context, scenes
+    from cavalry_amd._lib import check, lib
+
+    n, r = args.n, args.rays
+    log(f"[bench] generating {n} triangles and 2x{r} rays (seed {args.seed})")
+    tris_h = scenes.gmm_triangles(n, seed=args.seed)
+    rays_h = scenes.random_rays(r, seed=args.seed + 1)
+    srays_h = scenes.random_rays(r, seed=args.seed + 2, stream=2)
+    lim_h = scenes.shadow_limits(r, seed=args.seed + 3)
+    dev = torch.device("cuda:0")
+    tris = torch.from_numpy(tris_h).to(dev)
+    rays = torch.from_numpy(rays_h).to(dev)
+    srays = torch.from_numpy(srays_h).to(dev)
+    lim = torch.from_numpy(lim_h).to(dev)
+    hit_prim = torch.empty(r, dtype=torch.int32, device=dev)
+    hit_t = torch.empty(r, dtype=torch.float32, device=dev)
+    vis = torch.empty(r, dtype=torch.uint8, device=dev)
+    del tris_h, rays_h, srays_h, lim_h
+    torch.cuda.synchronize()
+
+    ctx = Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+    log("[bench] first build (allocates)")
+    bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
+
+    def step(record: bool):
+        bvh.rebuild(tris, optimization_rounds=args.rounds, morton_bits=30)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        check(ctx.handle, lib().cav_trace_closest_async(
+            ctx.handle, bvh.handle, rays.data_ptr(), r, hit_prim.data_ptr(), hit_t.data_ptr()))
+        e[1].record(stream)
+        check(ctx.handle, lib().cav_trace_occluded_async(
+            ctx.handle, bvh.handle, srays.data_ptr(), lim.data_ptr(), r, vis.data_ptr()))
+        e[2].record(stream)
+        ctx.synchronize()
+        if not record:
+            return None
+        return bvh.stats(), bvh.kernel_times(), e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+
+    for i in range(args.warmup):
+        step(False)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        recs.append(step(True))
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"[bench] {args.steps} timed steps in {wall:.3f} s")
+
+    ms_step = wall * 1000.0 / args.steps
+    stages = {}
+    for k in ("ms_bounds", "ms_morton", "ms_sort", "ms_karras", "ms_refit", "ms_treelet", "ms_flatten"):
+        stages[k[3:]] = round(float(np.mean([rec[0][k] for rec in recs])), 4)
+    stages["trace_closest"] = round(float(np.mean([rec[2] for rec in recs])), 4)
+    stages["trace_shadow"] = round(float(np.mean([rec[3] for rec in recs])), 4)
+    lbvh_ms = sum(stages[k] for k in ("bounds", "morton", "sort", "karras", "refit"))
+
+    # per-kernel averages over the timed steps (same launch sequence every step)
+    kernels = {}
+    for rec in recs:
+        for name, ms in rec[1]:
+            kernels.setdefault(name, []).append(ms)
+    per_launch = {k: float(np.mean(v)) for k, v in kernels.items()}
+    launches = {k: len(v) // args.steps for k, v in kernels.items()}
+    totals = {k: per_launch[k] * launches[k] for k in per_launch}
+
+    # roofline of the dominant HBM-bound LBVH kernel
+    lbvh_kernels = [k for k in ALG_BYTES_30 if k in totals]
+    dom = max(lbvh_kernels, key=lambda k: totals[k])
+    alg = ALG_BYTES_30[dom] * n
+    achieved = alg / (per_launch[dom] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_ms": round(per_launch[dom], 4),
+        "alg_bytes_per_launch": alg,
+        "lbvh_build_frac": round(ALG_BYTES_PER_TRI_30 * n / (lbvh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f).get(dom)
+        if tr:
+            roofline["traffic"] = tr
+
+    value = 2 * r / (ms_step * 1e-3) / 1e6
+    out = {
+        "metric": "Mrays/s path tracing; p50/p99 request latency; LBVH build Mtris/s",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"LBVH build (30-bit Morton, radix sort, Karras, refit) + {args.rounds} "
+                         f"treelet rounds + flatten over N=2^{n.bit_length() - 1} Gaussian-mixture "
+                         f"triangles, then 2^{r.bit_length() - 1} closest-hit + "
+                         f"2^{r.bit_length() - 1} shadow rays"),
+            "n_tris": n, "closest_rays": r, "shadow_rays": r, "treelet_rounds": args.rounds,
+            "morton_bits": 30, "parallelism": "single GPU",
+        },
+        "lbvh_build_mtris_s": round(n / (lbvh_ms * 1e-3) / 1e6, 2),
+        "lbvh_build_ms": round(lbvh_ms, 4),
+        "treelet_ms_per_round": round(stages["treelet"] / max(args.rounds, 1), 4),
+        "trace_closest_mrays_s": round(r / (stages["trace_closest"] * 1e-3) / 1e6, 2),
+        "trace_shadow_mrays_s": round(r / (stages["trace_shadow"] * 1e-3) / 1e6, 2),
+        "stage_ms": stages,
+        "kernel_ms": {k: {"per_launch": round(per_launch[k], 4), "launches": launches[k]}
+                      for k in per_launch},
+        "sah_root": {"lbvh": float(recs[-1][0]["root_cost_lbvh"]),
+                     "final": float(recs[-1][0]["root_cost_final"])},
+        "hit_fraction": round(float((hit_prim >= 0).float().mean().item()), 4),
+        "visible_fraction": round(float(vis.float().mean().item()), 4),
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    bvh.close()
+    ctx.close()
+    del tris, rays, srays, lim, hit_prim, hit_t, vis
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -52,6 +52,7 @@ _SIGS = {
     "cav_bvh_rebuild_device": ([P, P, P, P, I, I], I),
     "cav_bvh_destroy": ([P], I),
     "cav_bvh_stats_get": ([P, ctypes.POINTER(BvhStats)], I),
+    "cav_bvh_kernel_times": ([P, P, P, I], I),
     "cav_bvh_export": ([P, P, P, P, P, P, P, P, P, P, P], I),
     "cav_trace_closest": ([P, P, P, I64, P, P, P, P], I),
     "cav_trace_closest_device": ([P, P, P, I64, P, P, P, P], I),

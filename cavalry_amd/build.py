@@ -20,6 +20,7 @@ LIB = os.path.join(OUT_DIR, "libcavalry_amd.so")
 SOURCES = [
     "radix_sort.hip",
     "lbvh.hip",
+    "levels.hip",
     "treelet.hip",
     "traverse.hip",
     "capi.cpp",

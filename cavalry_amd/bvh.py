@@ -96,6 +96,14 @@ class BVH:
         check(self.ctx.handle, lib().cav_bvh_stats_get(self.handle, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def kernel_times(self) -> list:
+        """[(kernel name, ms)] of the last build, in launch order."""
+        k = lib().cav_bvh_kernel_times(self.handle, None, None, 0)
+        names = (ctypes.c_char_p * max(k, 1))()
+        ms = np.zeros(max(k, 1), np.float32)
+        lib().cav_bvh_kernel_times(self.handle, names, _ptr(ms), k)
+        return [(names[i].decode(), float(ms[i])) for i in range(k)]
+
     def export(self) -> dict:
         """Node arrays in the unified 2N-1 id space plus sorted codes and order."""
         m = 2 * self.n - 1

@@ -59,18 +59,25 @@ struct cav_bvh {
     float* root = nullptr;      // 8 floats
     int32_t* mesh = nullptr;    // n, original order (may be null)
     cav_bvh_stats stats{};
-    StageEvents ev{};
+    Marker mk;
     bool events = false;
+    int nk = 0;                          // kernel timings of the last build
+    const char* kname[Marker::kMax];
+    float kms[Marker::kMax];
 };
 
 static void bvh_free(cav_bvh* h) {
     void* ptrs[] = {h->b.keys[0], h->b.keys[1], h->b.vals[0], h->b.vals[1], h->b.hist, h->b.counts,
                     h->b.offsets, h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,
-                    h->b.partial, h->b.leaf_ids, h->b.counters, h->tn, h->tt, h->root, h->mesh};
+                    h->b.partial, h->b.leaf_ids, h->b.counters, h->tn, h->tt, h->root, h->mesh,
+                    h->b.lvl_list, h->b.lvl_cnt, h->b.lvl_cursor, h->b.lvl_off,
+                    h->b.bounds_part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->b.h_lvl) (void)hipHostFree(h->b.h_lvl);
+    delete[] h->b.h_off;
     if (h->events)
-        for (auto& e : h->ev.e) (void)hipEventDestroy(e);
+        for (auto& e : h->mk.ev) (void)hipEventDestroy(e);
 }
 
 static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
@@ -85,6 +92,7 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {(void**)&h->b.counts, sizeof(uint32_t) * 256 * tiles, "counts"},
         {(void**)&h->b.offsets, sizeof(uint32_t) * 256 * tiles, "offsets"},
         {(void**)&h->b.scene_ord, 64, "scene"},
+        {(void**)&h->b.bounds_part, sizeof(uint32_t) * 6 * 1024, "bounds partials"},
         {(void**)&h->b.nbox, sizeof(NodeBox) * m, "nbox"},
         {(void**)&h->b.topo, sizeof(int4) * m, "topo"},
         {(void**)&h->b.visited, sizeof(uint32_t) * m, "visited"},
@@ -95,13 +103,20 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {(void**)&h->tn, sizeof(TNode) * (n - 1), "tnodes"},
         {(void**)&h->tt, sizeof(TTri) * n, "ttris"},
         {(void**)&h->root, 64, "root"},
+        {(void**)&h->b.lvl_list, sizeof(uint32_t) * m, "level list"},
+        {(void**)&h->b.lvl_cnt, sizeof(uint32_t) * (kMaxLevels + 2), "level counts"},
+        {(void**)&h->b.lvl_cursor, sizeof(uint32_t) * (kMaxLevels + 2), "level cursors"},
+        {(void**)&h->b.lvl_off, sizeof(uint32_t) * (kMaxLevels + 2), "level offsets"},
     };
     for (auto& r : req) {
         int rc = dev_alloc(ctx, r.p, r.bytes, r.what);
         if (rc != CAV_OK) return rc;
     }
-    for (auto& e : h->ev.e) HIPCHK(ctx, hipEventCreate(&e));
+    HIPCHK(ctx, hipHostMalloc((void**)&h->b.h_lvl, sizeof(uint32_t) * (kMaxLevels + 2), hipHostMallocDefault));
+    h->b.h_off = new uint32_t[kMaxLevels + 2]();
+    for (auto& e : h->mk.ev) HIPCHK(ctx, hipEventCreate(&e));
     h->events = true;
+    h->mk.on = true;
     return CAV_OK;
 }
 
@@ -124,41 +139,51 @@ static int bvh_run(cav_ctx* ctx, cav_bvh* h, const float* tris_dev, const float*
                    int morton_bits, int opt_rounds) {
     hipStream_t s = ctx->stream;
     const int n = h->n;
+    Marker* mk = &h->mk;
     h->morton_bits = morton_bits;
     h->opt_rounds = opt_rounds;
-    HIPCHK(ctx, hipEventRecord(h->ev.e[0], s));
+    mk->start(s);
     if (scene_bounds) {
         uint32_t o6[6];
         host_to_ord(scene_bounds, o6);
         HIPCHK(ctx, hipMemcpyAsync(h->b.scene_ord, o6, sizeof o6, hipMemcpyHostToDevice, s));
+        mark(mk, "bounds_upload");
     } else {
-        HIPCHK(ctx, lbvh_scene_bounds(tris_dev, n, h->b.scene_ord, s));
+        HIPCHK(ctx, lbvh_scene_bounds(tris_dev, n, h->b.scene_ord, h->b.bounds_part, s, mk));
     }
-    HIPCHK(ctx, lbvh_build(h->b, tris_dev, n, morton_bits, s, &h->ev));
+    HIPCHK(ctx, lbvh_build(h->b, tris_dev, n, morton_bits, s, mk));
     float root_lbvh[8];
     HIPCHK(ctx, hipMemcpyAsync(root_lbvh, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
-    for (int r = 0; r < opt_rounds; ++r) HIPCHK(ctx, treelet_round_impl(h->b, n, r == 0, s));
-    HIPCHK(ctx, hipEventRecord(h->ev.e[6], s));
-    if (opt_rounds > 0) HIPCHK(ctx, lbvh_refs(h->b, n, s));
-    HIPCHK(ctx, lbvh_flatten(h->b, tris_dev, h->mesh, n, opt_rounds == 0, h->tn, h->tt, h->root, s));
-    HIPCHK(ctx, hipEventRecord(h->ev.e[7], s));
+    mark(mk, "readback");
+    for (int r = 0; r < opt_rounds; ++r) HIPCHK(ctx, treelet_round_impl(h->b, n, r == 0, s, mk));
+    if (opt_rounds > 0) HIPCHK(ctx, lbvh_refs(h->b, n, s, mk, "flatten_refs"));
+    HIPCHK(ctx, lbvh_flatten(h->b, tris_dev, h->mesh, n, opt_rounds == 0, h->tn, h->tt, h->root, s, mk));
     float root_final[8];
     HIPCHK(ctx, hipMemcpyAsync(root_final, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
-    cav_bvh_stats& st = h->stats;
+    cav_bvh_stats st{};
     st.n = n;
     st.morton_bits = morton_bits;
     st.opt_rounds = opt_rounds;
-    st.ms_bounds = elapsed(h->ev.e[0], h->ev.e[1]);
-    st.ms_morton = elapsed(h->ev.e[1], h->ev.e[2]);
-    st.ms_sort = elapsed(h->ev.e[2], h->ev.e[3]);
-    st.ms_karras = elapsed(h->ev.e[3], h->ev.e[4]);
-    st.ms_refit = elapsed(h->ev.e[4], h->ev.e[5]);
-    st.ms_treelet = elapsed(h->ev.e[5], h->ev.e[6]);
-    st.ms_flatten = elapsed(h->ev.e[6], h->ev.e[7]);
-    st.ms_total = elapsed(h->ev.e[0], h->ev.e[7]);
+    h->nk = 0;
+    for (int i = 1; i < mk->n; ++i) {
+        const char* nm = mk->name[i];
+        const float ms = elapsed(mk->ev[i - 1], mk->ev[i]);
+        h->kname[h->nk] = nm;
+        h->kms[h->nk++] = ms;
+        const std::string k(nm);
+        if (k.rfind("bounds", 0) == 0) st.ms_bounds += ms;
+        else if (k == "morton") st.ms_morton += ms;
+        else if (k.rfind("sort", 0) == 0) st.ms_sort += ms;
+        else if (k == "karras") st.ms_karras += ms;
+        else if (k.rfind("refit", 0) == 0 || k == "levels") st.ms_refit += ms;
+        else if (k.rfind("treelet", 0) == 0) st.ms_treelet += ms;
+        else if (k.rfind("flatten", 0) == 0) st.ms_flatten += ms;
+    }
+    st.ms_total = mk->n > 1 ? elapsed(mk->ev[0], mk->ev[mk->n - 1]) : 0.f;
     st.root_cost_lbvh = root_lbvh[7];
     st.root_cost_final = root_final[7];
+    h->stats = st;
     return CAV_OK;
 }
 
@@ -211,6 +236,8 @@ int cav_bvh_build_device(cav_ctx* ctx, const float* tris_dev, int n, const float
     if (morton_bits != 30 && morton_bits != 63)
         return fail(ctx, CAV_ERR_INVALID, "morton_bits must be 30 or 63");
     if (opt_rounds < 0) return fail(ctx, CAV_ERR_INVALID, "opt_rounds must be >= 0");
+    if ((reinterpret_cast<uintptr_t>(tris_dev) & 15u) != 0)
+        return fail(ctx, CAV_ERR_INVALID, "device triangles must be 16-byte aligned");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     cav_bvh* h = new cav_bvh();
     h->ctx = ctx;
@@ -254,6 +281,8 @@ int cav_bvh_build(cav_ctx* ctx, const float* tris, int n, const float* scene_bou
 int cav_bvh_rebuild_device(cav_ctx* ctx, cav_bvh* h, const float* tris_dev,
                            const float* scene_bounds, int morton_bits, int opt_rounds) {
     if (!ctx || !h || !tris_dev) return CAV_ERR_INVALID;
+    if ((reinterpret_cast<uintptr_t>(tris_dev) & 15u) != 0)
+        return fail(ctx, CAV_ERR_INVALID, "device triangles must be 16-byte aligned");
     if (morton_bits != 30 && morton_bits != 63)
         return fail(ctx, CAV_ERR_INVALID, "morton_bits must be 30 or 63");
     if (opt_rounds < 0) return fail(ctx, CAV_ERR_INVALID, "opt_rounds must be >= 0");
@@ -273,6 +302,16 @@ int cav_bvh_stats_get(const cav_bvh* h, cav_bvh_stats* out) {
     if (!h || !out) return CAV_ERR_INVALID;
     *out = h->stats;
     return CAV_OK;
+}
+
+int cav_bvh_kernel_times(const cav_bvh* h, const char** names, float* ms, int max) {
+    if (!h || max < 0) return -CAV_ERR_INVALID;
+    const int k = h->nk < max ? h->nk : max;
+    for (int i = 0; i < k; ++i) {
+        if (names) names[i] = h->kname[i];
+        if (ms) ms[i] = h->kms[i];
+    }
+    return h->nk;
 }
 
 int cav_bvh_export(cav_ctx* ctx, const cav_bvh* h, float* box6, float* sa, float* cost,
@@ -324,7 +363,7 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
                              const float* limit, int64_t nrays, uint8_t* visible) {
     if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
     HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, nullptr, nullptr, nrays, visible,
-                               h->b.counters + 1, ctx->stream));
+                               h->b.counters, ctx->stream));
     return CAV_OK;
 }
 
@@ -351,7 +390,7 @@ int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
         return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, src, dst, nrays, visible, h->b.counters + 1, ctx->stream));
+    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -8,9 +8,31 @@
 
 namespace cav {
 
-struct StageEvents {
-    hipEvent_t e[12];
+// Optional per-kernel timing: an event is recorded on the stream after each
+// launch, named after the kernel just launched. Durations are the gaps
+// between consecutive events.
+struct Marker {
+    static constexpr int kMax = 160;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[kMax];
+    const char* name[kMax];
+    int n = 0;
+    bool on = false;
+    void start(hipStream_t st) {
+        s = st;
+        n = 0;
+        if (on) { hipEventRecord(ev[0], s); name[0] = "start"; n = 1; }
+    }
+    void mark(const char* nm) {
+        if (on && n < kMax) { hipEventRecord(ev[n], s); name[n++] = nm; }
+    }
 };
+inline void mark(Marker* m, const char* nm) { if (m) m->mark(nm); }
+
+constexpr int kMaxLevels = 4096;  // node heights supported by the level scheduler
+// top levels holding at most this many nodes run in one workgroup
+constexpr int kTailNodes = 16384;        // refit
+constexpr int kTreeletTailNodes = 256;   // treelet rounds
 
 // Device buffers of one BVH build (library-owned; sized for n triangles).
 struct LbvhBuffers {
@@ -20,36 +42,54 @@ struct LbvhBuffers {
     uint32_t* counts = nullptr;               // 256 * tiles
     uint32_t* offsets = nullptr;              // 256 * tiles
     uint32_t* scene_ord = nullptr;            // 6 order-preserving ints
+    uint32_t* bounds_part = nullptr;          // per-workgroup partial bounds (6 each)
     NodeBox* nbox = nullptr;                  // 2n-1
     int4* topo = nullptr;                     // 2n-1
-    uint32_t* visited = nullptr;              // 2n-1
+    uint32_t* visited = nullptr;              // 2n-1 (arrivals << 16 | height)
     int32_t* ref = nullptr;                   // 2n-1 (restructured trees)
     uint32_t* partial = nullptr;              // scan partials
     uint32_t* leaf_ids = nullptr;             // n (treelet round start list)
     uint32_t* counters = nullptr;             // small device counters
+    uint32_t* lvl_list = nullptr;             // 2n-1 internal node ids grouped by height
+    uint32_t* lvl_cnt = nullptr;              // kMaxLevels + 2 (count per height, error flag)
+    uint32_t* lvl_cursor = nullptr;           // kMaxLevels + 2
+    uint32_t* lvl_off = nullptr;              // kMaxLevels + 2 (start of each height in lvl_list)
+    uint32_t* h_lvl = nullptr;                // pinned host copy of lvl_cnt
+    uint32_t* h_off = nullptr;                // host offsets (kMaxLevels + 2)
+    int height = 0;                           // root height of the bucketed tree
     int sorted = 0;                           // which keys/vals buffer is sorted
     uint32_t* order = nullptr;                // = vals[sorted]
 };
 
-hipError_t lbvh_scene_bounds(const float* tris, int n, uint32_t* scene_ord, hipStream_t s);
+// tris must be 16-byte aligned
+hipError_t lbvh_scene_bounds(const float* tris, int n, uint32_t* scene_ord, uint32_t* part,
+                             hipStream_t s, Marker* mk);
 hipError_t lbvh_build(LbvhBuffers& b, const float* tris, int n, int morton_bits, hipStream_t s,
-                      StageEvents* ev);
-hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s);
+                      Marker* mk);
+hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s, Marker* mk, const char* label);
 hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, int n,
-                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s);
+                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s,
+                        Marker* mk);
 int64_t lbvh_partials(int64_t m);
 
+// levels.hip: height climb + bucketing + level-ordered refit
+hipError_t levels_from_leaves(LbvhBuffers& b, const float* tris, int n, hipStream_t s, Marker* mk);
+hipError_t levels_from_list(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
+hipError_t levels_bucket(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
+hipError_t levels_refit(LbvhBuffers& b, hipStream_t s, Marker* mk);
+
 // treelet.hip: one restructuring round; `first` = leaves still at ids N-1..2N-2
-hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s);
+hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, Marker* mk);
 
 // traverse.hip
-// Traversal over a flattened BVH of ntris triangles. overflow counts stack
-// overflows and step-cap exits (either means the results are partial).
+// Traversal over a flattened BVH of ntris triangles. counters[0] (closest) and
+// counters[1] (occluded) count stack overflows and step-cap exits (either
+// means the results are partial); counters[2..5] hold the ray dispensers.
 hipError_t trace_closest(const TNode* tn, const TTri* tt, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* overflow, hipStream_t s);
+                         float* v, uint32_t* counters, hipStream_t s);
 hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
-                          int64_t nrays, uint8_t* visible, uint32_t* overflow, hipStream_t s);
+                          int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s);
 
 }  // namespace cav

@@ -32,10 +32,7 @@ __device__ __forceinline__ void tri_box(v3 a, v3 b, v3 c, v3& lo, v3& hi) {
     hi = mk3(fmx(a.x, fmx(b.x, c.x)), fmx(a.y, fmx(b.y, c.y)), fmx(a.z, fmx(b.z, c.z)));
 }
 
-__global__ void k_bounds_init(uint32_t* ob) {
-    if (threadIdx.x < 3) ob[threadIdx.x] = 0xffffffffu;
-    else if (threadIdx.x < 6) ob[threadIdx.x] = 0u;
-}
+constexpr int kBoundsBlocks = 1024;
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 #pragma unroll
@@ -48,30 +45,69 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-// Union of all triangle boxes as order-preserving integers (NaN skipped), so
-// the result does not depend on reduction order.
-__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris, int64_t n,
-                                                   uint32_t* __restrict__ ob) {
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock) {
-        const float* p = tris + i * 9;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            float f = p[k];
-            if (f == f) {
-                uint32_t o = f2ord(f);
-                mn[k % 3] = min(mn[k % 3], o);
-                mx[k % 3] = max(mx[k % 3], o);
-            }
-        }
+__device__ __forceinline__ void acc(uint32_t* mn, uint32_t* mx, int axis, float f) {
+    if (f == f) {
+        const uint32_t o = f2ord(f);
+        mn[axis] = min(mn[axis], o);
+        mx[axis] = max(mx[axis], o);
     }
+}
+
+// Block-reduces 6 per-thread values (3 minima, 3 maxima) and writes them.
+__device__ __forceinline__ void block_store6(uint32_t* mn, uint32_t* mx, uint32_t* out) {
+    __shared__ uint32_t red[kBlock / 64][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) { mn[a] = wave_min(mn[a]); mx[a] = wave_max(mx[a]); }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) { atomicMin(&ob[a], mn[a]); atomicMax(&ob[3 + a], mx[a]); }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) { red[wave][a] = mn[a]; red[wave][3 + a] = mx[a]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        uint32_t v = red[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w)
+            v = threadIdx.x < 3 ? min(v, red[w][threadIdx.x]) : max(v, red[w][threadIdx.x]);
+        out[threadIdx.x] = v;
     }
+}
+
+// Union of all triangle boxes as order-preserving integers (NaN skipped), so
+// the result does not depend on reduction order. Stage 1: each workgroup
+// reduces a grid-strided share of the soup, four triangles (nine 16-byte
+// loads) per thread and step, into one partial.
+__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris, int64_t n,
+                                                   uint32_t* __restrict__ part) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    const int64_t quads = n / 4;
+    const float4* q4 = reinterpret_cast<const float4*>(tris);
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < quads;
+         q += (int64_t)gridDim.x * kBlock) {
+        float4 v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = q4[q * 9 + k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            acc(mn, mx, (4 * k + 0) % 3, v[k].x);
+            acc(mn, mx, (4 * k + 1) % 3, v[k].y);
+            acc(mn, mx, (4 * k + 2) % 3, v[k].z);
+            acc(mn, mx, (4 * k + 3) % 3, v[k].w);
+        }
+    }
+    if (blockIdx.x == 0)
+        for (int64_t t = quads * 4 + threadIdx.x; t < n; t += kBlock)
+            for (int k = 0; k < 9; ++k) acc(mn, mx, k % 3, tris[t * 9 + k]);
+    block_store6(mn, mx, part + (int64_t)blockIdx.x * 6);
+}
+
+// Stage 2: one workgroup reduces the partials.
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const uint32_t* __restrict__ part,
+                                                         int nparts, uint32_t* __restrict__ ob) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (int i = threadIdx.x; i < nparts; i += kBlock)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = min(mn[a], part[i * 6 + a]);
+            mx[a] = max(mx[a], part[i * 6 + 3 + a]);
+        }
+    block_store6(mn, mx, ob);
 }
 
 __device__ __forceinline__ uint64_t sat_u64(float v) {
@@ -206,41 +242,6 @@ __global__ __launch_bounds__(kBlock) void k_karras(const K* __restrict__ codes, 
     visited[i] = 0;
 }
 
-// BVH/BVH.cu:46-56 + 217-259: leaf k fills its node from triangle order[k],
-// then climbs; the second visitor of an internal node unions its children.
-// Release/acquire fences at agent scope order the child writes before the
-// gate increment and the sibling reads after it.
-__global__ __launch_bounds__(kBlock) void k_refit(const float* __restrict__ tris, int n,
-                                                  const uint32_t* __restrict__ order,
-                                                  NodeBox* nbox, int4* topo,
-                                                  uint32_t* visited) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    const int t = (int)order[k];
-    v3 lo, hi;
-    tri_box(load_v(tris, t, 0), load_v(tris, t, 1), load_v(tris, t, 2), lo, hi);
-    const float sa = surface_area(lo, hi);
-    const int node = n - 1 + k;
-    int cur = topo[node].z;
-    nbox[node] = NodeBox{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa))};
-    topo[node] = make_int4(-1, -1, cur, t);
-    // a valid tree is at most n-1 levels deep; the cap only guards corruption
-    for (int step = 0; step < n; ++step) {
-        __threadfence();
-        if (atomicAdd(&visited[cur], 1u) != 1u) return;  // first visitor stops
-        __threadfence();
-        const int4 tp = topo[cur];
-        const NodeBox bl = nbox[tp.x], br = nbox[tp.y];
-        const v3 ulo = mk3(fmn(bl.a.x, br.a.x), fmn(bl.a.y, br.a.y), fmn(bl.a.z, br.a.z));
-        const v3 uhi = mk3(fmx(bl.b.x, br.b.x), fmx(bl.b.y, br.b.y), fmx(bl.b.z, br.b.z));
-        const float usa = surface_area(ulo, uhi);
-        const float c = internal_cost3(usa, bl.b.w, br.b.w);
-        nbox[cur] = NodeBox{make_float4(ulo.x, ulo.y, ulo.z, usa), make_float4(uhi.x, uhi.y, uhi.z, c)};
-        if (cur == 0) return;
-        cur = tp.z;
-    }
-}
-
 // ---- device-wide exclusive scan of leaf flags (restructured trees) ----
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const NodeBox* __restrict__ 
 
 template <typename K>
 hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hipStream_t s,
-                      StageEvents* ev) {
+                      Marker* mk) {
     SortBuffers<K> sb;
     sb.keys[0] = reinterpret_cast<K*>(b.keys[0]);
     sb.keys[1] = reinterpret_cast<K*>(b.keys[1]);
@@ -365,58 +366,62 @@ hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hip
     sb.counts = b.counts;
     sb.offsets = b.offsets;
     const int nb = (n + kBlock - 1) / kBlock;
-    if (ev) hipEventRecord(ev->e[1], s);
     hipLaunchKernelGGL(k_morton<K>, dim3(nb), dim3(kBlock), 0, s, tris, (int64_t)n,
                        b.scene_ord, sb.keys[0], sb.vals[0]);
-    if (ev) hipEventRecord(ev->e[2], s);
-    hipError_t e = radix_sort_pairs(sb, n, keybits, s);
+    mark(mk, "morton");
+    hipError_t e = radix_sort_pairs(sb, n, keybits, s, mk);
     if (e != hipSuccess) return e;
-    if (ev) hipEventRecord(ev->e[3], s);
     b.sorted = sb.result;
     const K* codes = sb.keys[sb.result];
     b.order = sb.vals[sb.result];
     const int nk = (n - 1 + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_karras<K>, dim3(nk), dim3(kBlock), 0, s, codes, n, keybits, b.topo,
                        b.visited);
-    if (ev) hipEventRecord(ev->e[4], s);
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kBlock), 0, s, tris, n, b.order, b.nbox, b.topo,
-                       b.visited);
-    if (ev) hipEventRecord(ev->e[5], s);
-    return hipGetLastError();
+    mark(mk, "karras");
+    // refit (BVH/BVH.cu:217-259): leaf boxes + heights, then level by level
+    if ((e = levels_from_leaves(b, tris, n, s, mk)) != hipSuccess) return e;
+    if ((e = levels_bucket(b, n, s, mk)) != hipSuccess) return e;
+    return levels_refit(b, s, mk);
 }
 
 }  // namespace
 
-hipError_t lbvh_scene_bounds(const float* tris, int n, uint32_t* scene_ord, hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, scene_ord);
-    int blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
-    if (blocks > 4096) blocks = 4096;
+hipError_t lbvh_scene_bounds(const float* tris, int n, uint32_t* scene_ord, uint32_t* part,
+                             hipStream_t s, Marker* mk) {
+    if ((reinterpret_cast<uintptr_t>(tris) & 15u) != 0) return hipErrorInvalidValue;
+    int blocks = (int)(((int64_t)n / 4 + kBlock * 4 - 1) / (kBlock * 4));
+    if (blocks > kBoundsBlocks) blocks = kBoundsBlocks;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, s, tris, (int64_t)n, scene_ord);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, s, tris, (int64_t)n, part);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, s, part, blocks, scene_ord);
+    mark(mk, "bounds");
     return hipGetLastError();
 }
 
 hipError_t lbvh_build(LbvhBuffers& b, const float* tris, int n, int morton_bits, hipStream_t s,
-                      StageEvents* ev) {
-    if (morton_bits == 63) return build_impl<uint64_t>(b, tris, n, 63, s, ev);
-    return build_impl<uint32_t>(b, tris, n, 30, s, ev);
+                      Marker* mk) {
+    if (morton_bits == 63) return build_impl<uint64_t>(b, tris, n, 63, s, mk);
+    return build_impl<uint32_t>(b, tris, n, 30, s, mk);
 }
 
-hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s) {
+hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s, Marker* mk, const char* label) {
     const int64_t m = 2 * (int64_t)n - 1;
     const int np = (int)((m + kScanTile - 1) / kScanTile);
     hipLaunchKernelGGL(k_leafcount, dim3(np), dim3(kBlock), 0, s, b.topo, m, b.partial);
     hipLaunchKernelGGL(k_partials, dim3(1), dim3(kBlock), 0, s, b.partial, np);
     hipLaunchKernelGGL(k_refs, dim3(np), dim3(kBlock), 0, s, b.topo, m, b.partial, b.ref);
+    mark(mk, label);
     return hipGetLastError();
 }
 
 hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, int n,
-                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s) {
+                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s,
+                        Marker* mk) {
     const int64_t m = 2 * (int64_t)n - 1;
     const int nb = (int)((m + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_flatten, dim3(nb), dim3(kBlock), 0, s, b.nbox, b.topo, n,
                        identity_refs ? nullptr : b.ref, tris, mesh, tn, tt, root);
+    mark(mk, "flatten");
     return hipGetLastError();
 }
 

@@ -15,9 +15,12 @@ struct SortBuffers {
     int result = 0;                          // index of the buffer holding the output
 };
 
+struct Marker;
 int64_t sort_tiles(int64_t n);
 int sort_radix();
-hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s);
-hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s);
+hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
+                            Marker* mk);
+hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
+                            Marker* mk);
 
 }  // namespace cav

@@ -12,6 +12,7 @@
 //               in element order), stages the tile in LDS in digit order and
 //               writes each digit run contiguously to its global offset.
 #include "dev_common.h"
+#include "lbvh.h"
 #include "radix_sort.h"
 
 namespace cav {
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
 }
 
 template <typename K>
-hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s) {
+hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s, Marker* mk) {
     const int passes = (key_bits + 7) / 8;
     const int tiles = (int)((n + kTile - 1) / kTile);
     hipError_t e;
@@ -224,17 +225,22 @@ hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s) 
     int hblocks = (int)((n + kBlock * 16 - 1) / (kBlock * 16));
     if (hblocks > 2048) hblocks = 2048;
     if (hblocks < 1) hblocks = 1;
+    mark(mk, "sort_clear");
     hipLaunchKernelGGL(k_hist<K>, dim3(hblocks), dim3(kBlock), 0, s, b.keys[0], n, passes, b.hist);
+    mark(mk, "sort_hist");
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_upsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur], n, shift,
                            tiles, b.counts);
+        mark(mk, "sort_upsweep");
         hipLaunchKernelGGL(k_scan, dim3(kRadix), dim3(kBlock), 0, s, b.counts, tiles,
                            b.hist + p * kRadix, b.offsets);
+        mark(mk, "sort_scan");
         hipLaunchKernelGGL(k_downsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur],
                            b.vals[cur], b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, tiles,
                            b.offsets);
+        mark(mk, "sort_downsweep");
         cur ^= 1;
     }
     b.result = cur;
@@ -246,11 +252,13 @@ hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s) 
 int64_t sort_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
 int sort_radix() { return kRadix; }
 
-hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s) {
-    return sort_impl<uint32_t>(b, n, key_bits, s);
+hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
+                            Marker* mk) {
+    return sort_impl<uint32_t>(b, n, key_bits, s, mk);
 }
-hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s) {
-    return sort_impl<uint64_t>(b, n, key_bits, s);
+hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
+                            Marker* mk) {
+    return sort_impl<uint64_t>(b, n, key_bits, s, mk);
 }
 
 }  // namespace cav

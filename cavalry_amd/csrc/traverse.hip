@@ -19,6 +19,27 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kStack = 64;  // BVH/BVH.h:41,122
 
+// Traversal stack of kStack entries per ray: the first D in LDS (one column
+// per thread, so a wave's accesses to one depth hit 64 distinct banks), the
+// rest in private memory, which only unusually deep traversals touch.
+template <int D>
+struct RayStack {
+    int (*lds)[kBlock];
+    int spill[kStack - D];
+    int sp = 0;
+    __device__ __forceinline__ void push(int v) {
+        if (sp < D) lds[sp][threadIdx.x] = v;
+        else spill[sp - D] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        return sp < D ? lds[sp][threadIdx.x] : spill[sp - D];
+    }
+};
+constexpr int kLdsDepth = 32;
+constexpr int kChunk = 256;  // rays taken from the global counter at a time
+
 // Core/BoundingBox.h:55-74; returns the entry distance or -1.
 __device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz,
                                       v3 o, v3 inv) {
@@ -56,137 +77,194 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
     d = mk3(b.y, c.x, c.y);
 }
 
-__global__ __launch_bounds__(kBlock) void k_closest(const TNode* __restrict__ tn,
-                                                    const TTri* __restrict__ tt,
-                                                    const float* __restrict__ root,
-                                                    const float* __restrict__ rays, int64_t nrays,
-                                                    int32_t* __restrict__ oprim,
-                                                    float* __restrict__ ot, float* __restrict__ ou,
-                                                    float* __restrict__ ov,
-                                                    uint32_t* __restrict__ overflow,
-                                                    int max_steps) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nrays) return;
-    v3 o, d;
-    load_ray(rays, i, o, d);
-    const v3 inv = div3(mk3(1.f, 1.f, 1.f), d);
-    bool hit = false;
-    float best = FLT_MAX, bu = 0.f, bv = 0.f;
-    int bp = -1;
+// Persistent traversal: a fixed grid whose lanes each keep one ray in flight
+// and pull a new one as soon as theirs finishes, so a wave is not held by its
+// slowest ray (nodes visited per ray vary by 1000x on scenes with many
+// coincident Morton codes). Rays are handed out per wave from chunks taken
+// with one atomic. Each ray's node order is the reference's, so results do not
+// depend on which lane or wave traced it.
+template <bool kClosest>
+__global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
+                                                  const TTri* __restrict__ tt,
+                                                  const float* __restrict__ root,
+                                                  const float* __restrict__ rays, int64_t nrays,
+                                                  const float* __restrict__ limit,
+                                                  const int32_t* __restrict__ src,
+                                                  const int32_t* __restrict__ dst,
+                                                  int32_t* __restrict__ oprim,
+                                                  float* __restrict__ ot, float* __restrict__ ou,
+                                                  float* __restrict__ ov,
+                                                  uint8_t* __restrict__ ovis,
+                                                  uint32_t* __restrict__ overflow,
+                                                  unsigned long long* __restrict__ next_ray,
+                                                  int max_steps) {
+    __shared__ int lstack[kLdsDepth][kBlock];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+
+    int64_t ri = -1;       // ray held by this lane; -1 = needs one, -2 = pool drained
+    v3 o = mk3(0, 0, 0), d = o, inv = o;
+    float best = FLT_MAX, bu = 0.f, bv = 0.f, lim = 0.f;
+    int bp = -1, sm = -1, dm = -1, cur = 0, steps = 0;
+    bool hit = false, use_lim = false;
     uint32_t ovf = 0;
-    if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) >= 0.f) {
-        int stack[kStack];
-        int sp = 0;
-        int cur = 0;
-        // every node is visited at most once in a valid tree
-        for (int step = 0; step < max_steps; ++step) {
-            if (cur >= 0) {
-                const TNode nd = tn[cur];
-                const float dl = slab(nd.l0.x, nd.l0.y, nd.l0.z, nd.l0.w, nd.l1.x, nd.l1.y, o, inv);
-                const float dr = slab(nd.l1.z, nd.l1.w, nd.r1.x, nd.r1.y, nd.r1.z, nd.r1.w, o, inv);
-                const bool hl = dl >= 0 && dl < best, hr = dr >= 0 && dr < best;
-                if (hl && hr) {
-                    const bool lfar = dl > dr;
-                    const int far = lfar ? nd.refs.x : nd.refs.y;
-                    const int near = lfar ? nd.refs.y : nd.refs.x;
-                    if (sp < kStack) stack[sp++] = far; else ++ovf;
-                    if (sp < kStack) { cur = near; continue; }
-                    ++ovf;
-                } else if (hl || hr) {
-                    if (sp < kStack) { cur = hl ? nd.refs.x : nd.refs.y; continue; }
-                    ++ovf;
-                }
+    RayStack<kLdsDepth> st;
+    st.lds = lstack;
+    int64_t chunk_base = 0;
+    int chunk_avail = 0;
+
+    while (true) {
+        // ---- hand new rays to idle lanes (wave-uniform control flow)
+        const uint64_t need = __ballot(ri == -1);
+        if (need) {
+            const int cnt = __popcll(need);
+            const int rank = __popcll(need & lt);
+            int64_t mine;
+            if (chunk_avail < cnt) {
+                unsigned long long nb = 0;
+                const int first = __ffsll((long long)need) - 1;
+                if (lane == first) nb = atomicAdd(next_ray, (unsigned long long)kChunk);
+                nb = __shfl(nb, first, 64);
+                mine = rank < chunk_avail ? chunk_base + rank : (int64_t)nb + (rank - chunk_avail);
+                chunk_base = (int64_t)nb + (cnt - chunk_avail);
+                chunk_avail = kChunk - (cnt - chunk_avail);
             } else {
-                const TTri tr = tt[~cur];
-                float t, u, v;
-                if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
-                    hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+                mine = chunk_base + rank;
+                chunk_base += cnt;
+                chunk_avail -= cnt;
+            }
+            if (ri == -1) {
+                if (mine >= nrays) {
+                    ri = -2;
+                } else {
+                    ri = mine;
+                    load_ray(rays, ri, o, d);
+                    inv = div3(mk3(1.f, 1.f, 1.f), d);
+                    best = FLT_MAX; hit = false; bp = -1; bu = 0.f; bv = 0.f;
+                    st.sp = 0; cur = 0; steps = 0;
+                    if (kClosest) {
+                        // BVH/BVH.h:34-38: a ray missing the root box has no hit
+                        if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
+                            oprim[ri] = -1; ot[ri] = 0.f;
+                            if (ou) ou[ri] = 0.f;
+                            if (ov) ov[ri] = 0.f;
+                            ri = -1;
+                        }
+                    } else {
+                        lim = limit[ri];
+                        use_lim = !(lim == INFINITY);
+                        sm = src ? src[ri] : -1;
+                        dm = dst ? dst[ri] : -1;
+                    }
                 }
             }
-            if (sp == 0) break;
-            cur = stack[--sp];
         }
-    }
-    oprim[i] = hit ? bp : -1;
-    ot[i] = hit ? best : 0.f;
-    if (ou) ou[i] = hit ? bu : 0.f;
-    if (ov) ov[i] = hit ? bv : 0.f;
-    if (ovf) atomicAdd(overflow, ovf);
-}
+        if (!__ballot(ri >= 0)) {
+            if (!__ballot(ri == -1)) break;  // every lane drained
+            continue;                        // lanes whose ray missed the root refill
+        }
+        if (ri < 0) continue;
 
-__global__ __launch_bounds__(kBlock) void k_occluded(const TNode* __restrict__ tn,
-                                                     const TTri* __restrict__ tt,
-                                                     const float* __restrict__ rays,
-                                                     const float* __restrict__ limit,
-                                                     const int32_t* __restrict__ src,
-                                                     const int32_t* __restrict__ dst,
-                                                     int64_t nrays, uint8_t* __restrict__ visible,
-                                                     uint32_t* __restrict__ overflow,
-                                                     int max_steps) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nrays) return;
-    v3 o, d;
-    load_ray(rays, i, o, d);
-    const v3 inv = div3(mk3(1.f, 1.f, 1.f), d);
-    const float lim = limit[i];
-    const bool use_lim = !(lim == INFINITY);
-    const int sm = src ? src[i] : -1, dm = dst ? dst[i] : -1;
-    uint32_t ovf = 0;
-    bool vis = true;
-    int stack[kStack];
-    int sp = 0;
-    int cur = 0;
-    for (int step = 0; step < max_steps; ++step) {
-        if (cur >= 0) {
+        // ---- one traversal step for this lane's ray
+        bool done = false;
+        if (++steps > max_steps) {
+            ++ovf;
+            done = true;
+        } else if (cur >= 0) {
             const TNode nd = tn[cur];
             const float dl = slab(nd.l0.x, nd.l0.y, nd.l0.z, nd.l0.w, nd.l1.x, nd.l1.y, o, inv);
             const float dr = slab(nd.l1.z, nd.l1.w, nd.r1.x, nd.r1.y, nd.r1.z, nd.r1.w, o, inv);
-            const bool hl = dl >= 0, hr = dr >= 0;
+            const bool hl = kClosest ? (dl >= 0 && dl < best) : dl >= 0;
+            const bool hr = kClosest ? (dr >= 0 && dr < best) : dr >= 0;
+            bool next = false;
             if (hl && hr) {
                 const bool lfar = dl > dr;
                 const int far = lfar ? nd.refs.x : nd.refs.y;
                 const int near = lfar ? nd.refs.y : nd.refs.x;
-                if (sp < kStack) stack[sp++] = far; else ++ovf;
-                if (sp < kStack) { cur = near; continue; }
-                ++ovf;
+                if (st.sp < kStack) st.push(far); else ++ovf;
+                if (st.sp < kStack) { cur = near; next = true; } else ++ovf;
             } else if (hl || hr) {
-                if (sp < kStack) { cur = hl ? nd.refs.x : nd.refs.y; continue; }
-                ++ovf;
+                if (st.sp < kStack) { cur = hl ? nd.refs.x : nd.refs.y; next = true; } else ++ovf;
+            }
+            if (!next) {
+                if (st.sp == 0) done = true;
+                else cur = st.pop();
             }
         } else {
             const TTri tr = tt[~cur];
-            const int mesh = __float_as_int(tr.b.w);
-            if (mesh != sm && mesh != dm) {
-                float t, u, v;
-                if (tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim)) { vis = false; break; }
+            float t, u, v;
+            if (kClosest) {
+                if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
+                    hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+                }
+            } else {
+                const int mesh = __float_as_int(tr.b.w);
+                if (mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim))
+                    hit = true;  // occluded
             }
+            if (!kClosest && hit) done = true;
+            else if (st.sp == 0) done = true;
+            else cur = st.pop();
         }
-        if (sp == 0) break;
-        cur = stack[--sp];
+        if (done) {
+            if (kClosest) {
+                oprim[ri] = hit ? bp : -1;
+                ot[ri] = hit ? best : 0.f;
+                if (ou) ou[ri] = hit ? bu : 0.f;
+                if (ov) ov[ri] = hit ? bv : 0.f;
+            } else {
+                ovis[ri] = hit ? 0 : 1;
+            }
+            ri = -1;
+        }
     }
-    visible[i] = vis ? 1 : 0;
     if (ovf) atomicAdd(overflow, ovf);
+}
+
+int persistent_blocks(const void* kernel) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) per = 4;
+    return cus * per;
 }
 
 }  // namespace
 
 hipError_t trace_closest(const TNode* tn, const TTri* tt, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* overflow, hipStream_t s) {
+                         float* v, uint32_t* counters, hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
-    const int64_t nb = (nrays + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_closest, dim3((unsigned)nb), dim3(kBlock), 0, s, tn, tt, root, rays,
-                       nrays, prim, t, u, v, overflow, 2 * ntris - 1);
+    hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
+    if (e != hipSuccess) return e;
+    const void* k = reinterpret_cast<const void*>(&k_trace<true>);
+    int64_t blocks = persistent_blocks(k);
+    const int64_t need = (nrays + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, root, rays,
+                       nrays, nullptr, nullptr, nullptr, prim, t, u, v, nullptr, counters,
+                       reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
 
 hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
-                          int64_t nrays, uint8_t* visible, uint32_t* overflow, hipStream_t s) {
+                          int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
-    const int64_t nb = (nrays + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_occluded, dim3((unsigned)nb), dim3(kBlock), 0, s, tn, tt, rays, limit,
-                       src, dst, nrays, visible, overflow, 2 * ntris - 1);
+    hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
+    if (e != hipSuccess) return e;
+    const void* k = reinterpret_cast<const void*>(&k_trace<false>);
+    int64_t blocks = persistent_blocks(k);
+    const int64_t need = (nrays + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, nullptr,
+                       rays, nrays, limit, src, dst, nullptr, nullptr, nullptr, nullptr, visible,
+                       counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
+                       2 * ntris - 1);
     return hipGetLastError();
 }
 

@@ -281,38 +281,32 @@ __device__ void optimize_treelet(int root, NodeBox* nbox, int4* topo, GroupLds& 
     }
 }
 
-// BVH/Optimization.cu:467-507: group g starts at leaf leaf_ids[g] and climbs;
-// the second group to reach a node optimises the treelet rooted there.
-__global__ __launch_bounds__(kBlock) void k_treelet(NodeBox* nbox, int4* topo, uint32_t* visited,
-                                                    const uint32_t* __restrict__ leaf_ids,
-                                                    int nleaves) {
+// BVH/Optimization.cu:467-507, scheduled by height: every internal node of
+// height h is the root of one treelet; its 32-lane group runs once all nodes of
+// lower height are done (kernel boundaries or, for the sparse top levels, the
+// barriers of a single workgroup).
+__global__ __launch_bounds__(kBlock) void k_treelet_level(NodeBox* nbox, int4* topo,
+                                                          const uint32_t* __restrict__ list,
+                                                          int count) {
     __shared__ GroupLds lds[kGroups];
     const int group = (int)(((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 5);
-    if (group >= nleaves) return;
-    const int lane = threadIdx.x & 31;
-    GroupLds& L = lds[threadIdx.x >> 5];
-    int cur = (int)leaf_ids[group];
-    for (int step = 0; cur != 0 && step < nleaves; ++step) {
-        const int p = topo[cur].z;
-        __threadfence();
-        uint32_t old = 0;
-        if (lane == 0) old = atomicAdd(&visited[p], 1u);
-        old = (uint32_t)shi((int)old, 0);
-        if (old != 1u) return;
-        __threadfence();
-        cur = p;
-        optimize_treelet(cur, nbox, topo, L);
+    if (group >= count) return;
+    optimize_treelet((int)list[group], nbox, topo, lds[threadIdx.x >> 5]);
+}
+
+__global__ __launch_bounds__(1024) void k_treelet_tail(NodeBox* nbox, int4* topo,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ off, int h0,
+                                                       int h1) {
+    __shared__ GroupLds lds[32];
+    const int g = threadIdx.x >> 5;
+    for (int h = h0; h <= h1; ++h) {
+        const uint32_t b = off[h], e = off[h + 1];
+        for (uint32_t i = b + g; i < e; i += 32) optimize_treelet((int)list[i], nbox, topo, lds[g]);
+        __syncthreads();
     }
 }
 
-__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-__global__ void k_iota_leaves(uint32_t* ids, int n) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) ids[k] = (uint32_t)(n - 1 + k);
-}
 __global__ void k_leaf_ids(const int32_t* __restrict__ ref, int64_t m, uint32_t* __restrict__ ids) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) { const int32_t r = ref[i]; if (r < 0) ids[~r] = (uint32_t)i; }
@@ -320,22 +314,33 @@ __global__ void k_leaf_ids(const int32_t* __restrict__ ref, int64_t m, uint32_t*
 
 }  // namespace
 
-// One restructuring round. `first` rounds start from the Karras layout, whose
-// leaves are ids N-1..2N-2; later rounds list the leaves by a scan.
-hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s) {
+// One restructuring round. The first round reuses the height levels of the
+// refit (same tree); later rounds list the leaves by a scan and climb again.
+hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, Marker* mk) {
     const int64_t m = 2 * (int64_t)n - 1;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, b.visited, m, 0u);
-    if (first) {
-        hipLaunchKernelGGL(k_iota_leaves, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b.leaf_ids, n);
-    } else {
-        hipError_t e = lbvh_refs(b, n, s);
-        if (e != hipSuccess) return e;
+    hipError_t e;
+    if (!first) {
+        if ((e = lbvh_refs(b, n, s, nullptr, nullptr)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_leaf_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, b.ref, m, b.leaf_ids);
+        if ((e = levels_from_list(b, n, s, nullptr)) != hipSuccess) return e;
+        if ((e = levels_bucket(b, n, s, nullptr)) != hipSuccess) return e;
+        mark(mk, "treelet_setup");
     }
-    const int64_t groups = n;
-    const int64_t blocks = (groups * 32 + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_treelet, dim3((unsigned)blocks), dim3(kBlock), 0, s, b.nbox, b.topo,
-                       b.visited, b.leaf_ids, n);
+    const int top = b.height;
+    for (int h = 3; h <= top; ++h) {
+        const uint32_t rest = b.h_off[top + 1] - b.h_off[h];
+        if (rest <= (uint32_t)kTreeletTailNodes) {
+            hipLaunchKernelGGL(k_treelet_tail, dim3(1), dim3(1024), 0, s, b.nbox, b.topo, b.lvl_list,
+                               b.lvl_off, h, top);
+            break;
+        }
+        const int cnt = (int)b.h_lvl[h];
+        if (cnt == 0) continue;
+        const int64_t blocks = ((int64_t)cnt * 32 + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_treelet_level, dim3((unsigned)blocks), dim3(kBlock), 0, s, b.nbox,
+                           b.topo, b.lvl_list + b.h_off[h], cnt);
+    }
+    mark(mk, "treelet");
     return hipGetLastError();
 }
 

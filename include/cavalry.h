@@ -76,6 +76,10 @@ int cav_bvh_rebuild_device(cav_ctx* ctx, cav_bvh* bvh, const float* triangles_de
                            const float* scene_bounds, int morton_bits, int opt_rounds);
 int cav_bvh_destroy(cav_bvh* bvh);
 int cav_bvh_stats_get(const cav_bvh* bvh, cav_bvh_stats* out);
+/* Device time of every kernel (or copy) of the last build, in launch order,
+ * from HIP events recorded on the context stream between launches. Writes up
+ * to max entries (names are static strings) and returns the total count. */
+int cav_bvh_kernel_times(const cav_bvh* bvh, const char** names, float* ms, int max);
 /* Node arrays in the reference's unified id space (BVHRestructureNode,
  * BVH/BVHNode.h:48-66; internal i at i, leaves from N-1 before
  * restructuring): 2n-1 entries each. prim = original triangle index of a leaf

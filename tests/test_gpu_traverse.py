@@ -11,10 +11,12 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("rounds", [0, 3])
-@pytest.mark.parametrize("n,nrays", [(2, 1000), (64, 5000), (10000, 50000), (200000, 100000)])
-def test_closest_and_occluded(ctx, n, nrays, rounds):
-    print(f"[trace] n={n} rays={nrays} rounds={rounds}", flush=True)
-    tris = scenes.gmm_triangles(n, seed=n + 1)
+@pytest.mark.parametrize("n,nrays,edge", [(2, 1000, 0.3), (64, 5000, 0.1), (10000, 50000, 1e-3),
+                                          (10000, 50000, 0.02), (200000, 100000, 1e-3),
+                                          (200000, 100000, 0.01)])
+def test_closest_and_occluded(ctx, n, nrays, edge, rounds):
+    print(f"[trace] n={n} rays={nrays} edge={edge} rounds={rounds}", flush=True)
+    tris = scenes.gmm_triangles(n, seed=n + 1, edge=edge)
     _, _, nd = oracle.lbvh_build(tris, 30)
     if rounds:
         oracle.treelet_optimize(nd, rounds)
