@@ -34,6 +34,21 @@ class BvhStats(ctypes.Structure):
         ("n", ctypes.c_int), ("morton_bits", ctypes.c_int), ("opt_rounds", ctypes.c_int)]
 
 
+class Camera(ctypes.Structure):
+    _fields_ = [("cam_to_world", ctypes.c_float * 16), ("depth", ctypes.c_float),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("integrator", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("max_depth", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("closest_rays", ctypes.c_int64),
+                ("shadow_rays", ctypes.c_int64), ("mrays_per_s", ctypes.c_double)]
+
+
 _lib = None
 
 P = ctypes.c_void_p
@@ -60,6 +75,12 @@ _SIGS = {
     "cav_trace_occluded_device": ([P, P, P, P, P, P, I64, P], I),
     "cav_trace_closest_async": ([P, P, P, I64, P, P], I),
     "cav_trace_occluded_async": ([P, P, P, P, I64, P], I),
+    "cav_camera_look_at": ([P, P, P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
+    "cav_scene_create": ([P, P, I, P, I, P, I, I, ctypes.POINTER(P)], I),
+    "cav_scene_destroy": ([P], I),
+    "cav_scene_bvh": ([P], P),
+    "cav_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
+                    ctypes.POINTER(RenderStats)], I),
 }
 
 

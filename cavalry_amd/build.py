@@ -23,7 +23,10 @@ SOURCES = [
     "levels.hip",
     "treelet.hip",
     "traverse.hip",
+    "render.hip",
+    "render_capi.cpp",
     "capi.cpp",
+    "camera.cpp",
     "ctx_release.cpp",
 ]
 

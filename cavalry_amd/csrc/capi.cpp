@@ -48,24 +48,6 @@ int dev_alloc(cav_ctx* ctx, void** p, size_t bytes, const char* what) {
         if (e_ != hipSuccess) return cav::hip_fail((ctx), e_, #call);  \
     } while (0)
 
-struct cav_bvh {
-    cav_ctx* ctx = nullptr;
-    int n = 0;
-    int morton_bits = 30;
-    int opt_rounds = 0;
-    LbvhBuffers b;
-    TNode* tn = nullptr;
-    TTri* tt = nullptr;
-    float* root = nullptr;      // 8 floats
-    int32_t* mesh = nullptr;    // n, original order (may be null)
-    cav_bvh_stats stats{};
-    Marker mk;
-    bool events = false;
-    int nk = 0;                          // kernel timings of the last build
-    const char* kname[Marker::kMax];
-    float kms[Marker::kMax];
-};
-
 static void bvh_free(cav_bvh* h) {
     void* ptrs[] = {h->b.keys[0], h->b.keys[1], h->b.vals[0], h->b.vals[1], h->b.hist, h->b.counts,
                     h->b.offsets, h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,

@@ -4,11 +4,32 @@
 
 #include <string>
 
+#include "../../include/cavalry.h"
+#include "lbvh.h"
+
 struct cav_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
     void* render = nullptr;  // render-path state owned by the context (render.cpp)
+};
+
+struct cav_bvh {
+    cav_ctx* ctx = nullptr;
+    int n = 0;
+    int morton_bits = 30;
+    int opt_rounds = 0;
+    cav::LbvhBuffers b;
+    cav::TNode* tn = nullptr;
+    cav::TTri* tt = nullptr;
+    float* root = nullptr;      // 8 floats
+    int32_t* mesh = nullptr;    // n, original order (may be null)
+    cav_bvh_stats stats{};
+    cav::Marker mk;
+    bool events = false;
+    int nk = 0;                          // kernel timings of the last build
+    const char* kname[cav::Marker::kMax];
+    float kms[cav::Marker::kMax];
 };
 
 namespace cav {

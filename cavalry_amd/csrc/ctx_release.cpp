@@ -2,5 +2,6 @@
 #include "capi_internal.h"
 
 namespace cav {
-void ctx_release(cav_ctx* ctx) { ctx->render = nullptr; }
+void render_release(cav_ctx* ctx);
+void ctx_release(cav_ctx* ctx) { render_release(ctx); }
 }  // namespace cav

@@ -92,4 +92,13 @@ hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const floa
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s);
 
+hipError_t trace_closest_queue(const TNode* tn, const TTri* tt, int ntris, const float* root,
+                               const float* rays, const uint32_t* nrays_dev, int64_t capacity,
+                               int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
+                               hipStream_t s);
+hipError_t trace_occluded_queue(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+                                const uint32_t* nrays_dev, int64_t capacity, const float* limit,
+                                const int32_t* src, const int32_t* dst, uint8_t* visible,
+                                uint32_t* counters, hipStream_t s);
+
 }  // namespace cav

@@ -1,0 +1,310 @@
+// C ABI for scenes and rendering (cav_scene_*, cav_render); see
+// include/cavalry.h for the reference counterparts.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cavalry.h"
+#include "capi_internal.h"
+#include "render.h"
+
+using namespace cav;
+
+#define RCHK(ctx, call)                                                \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return cav::hip_fail((ctx), e_, #call);  \
+    } while (0)
+
+struct cav_scene {
+    cav_ctx* ctx = nullptr;
+    cav_bvh* bvh = nullptr;
+    int ntris = 0, nmeshes = 0, nlights = 0;
+    float* soup = nullptr;
+    int32_t* tri_mesh = nullptr;
+    cav_mesh* meshes = nullptr;
+    float* area = nullptr;
+    int32_t* lights = nullptr;
+    float* uv = nullptr;
+    std::vector<cav_mesh> host_meshes;
+    std::vector<float> host_area;
+};
+
+namespace {
+
+// Per-context render state: Halton tables and the sample/queue buffers,
+// grown on demand and kept between renders.
+struct RenderState {
+    uint32_t* primes = nullptr;
+    uint32_t* start = nullptr;
+    uint16_t* perm = nullptr;
+    int64_t cap = 0;      // samples the buffers hold
+    int64_t pix_cap = 0;  // pixels the film buffers hold
+    std::vector<void*> bufs;
+    RenderArgs a{};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+void free_bufs(RenderState* rs) {
+    for (void* p : rs->bufs) (void)hipFree(p);
+    rs->bufs.clear();
+    rs->cap = 0;
+    rs->pix_cap = 0;
+}
+
+// Halton digit permutations (Samplers/HaltonSampler.cu:7-57): the first 1024
+// primes, each shuffled by Fisher-Yates with the Microsoft C runtime rand()
+// stream (state * 214013 + 2531011, bits 16..30) seeded with 533799.
+void halton_tables(std::vector<uint32_t>& primes, std::vector<uint32_t>& start,
+                   std::vector<uint16_t>& perm) {
+    primes.clear();
+    for (uint32_t c = 2; primes.size() < 1024; ++c) {
+        bool p = true;
+        for (uint32_t q : primes) {
+            if (q * q > c) break;
+            if (c % q == 0) { p = false; break; }
+        }
+        if (p) primes.push_back(c);
+    }
+    start.resize(1024);
+    uint32_t tot = 0;
+    for (int k = 0; k < 1024; ++k) { start[k] = tot; tot += primes[k]; }
+    perm.resize(tot);
+    uint32_t state = 533799u;
+    for (int k = 0; k < 1024; ++k) {
+        const uint32_t p = primes[k];
+        uint16_t* v = perm.data() + start[k];
+        for (uint32_t j = 0; j < p; ++j) v[j] = (uint16_t)j;
+        for (int i = (int)p - 1; i > 0; --i) {
+            state = state * 214013u + 2531011u;
+            const int r = (int)((state >> 16) & 0x7fffu);
+            std::swap(v[i], v[r % (i + 1)]);
+        }
+    }
+}
+
+int ensure_state(cav_ctx* ctx, RenderState*& rs) {
+    if (!ctx->render) {
+        rs = new RenderState();
+        ctx->render = rs;
+        std::vector<uint32_t> primes, start;
+        std::vector<uint16_t> perm;
+        halton_tables(primes, start, perm);
+        int rc = dev_alloc(ctx, (void**)&rs->primes, 4 * 1024, "halton primes");
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&rs->start, 4 * 1024, "halton starts");
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&rs->perm, 2 * perm.size(), "halton perms");
+        if (rc != CAV_OK) return rc;
+        RCHK(ctx, hipMemcpy(rs->primes, primes.data(), 4 * 1024, hipMemcpyHostToDevice));
+        RCHK(ctx, hipMemcpy(rs->start, start.data(), 4 * 1024, hipMemcpyHostToDevice));
+        RCHK(ctx, hipMemcpy(rs->perm, perm.data(), 2 * perm.size(), hipMemcpyHostToDevice));
+        RCHK(ctx, hipEventCreate(&rs->ev[0]));
+        RCHK(ctx, hipEventCreate(&rs->ev[1]));
+    }
+    rs = static_cast<RenderState*>(ctx->render);
+    return CAV_OK;
+}
+
+template <typename T>
+int grab(cav_ctx* ctx, RenderState* rs, T*& p, size_t bytes, const char* what) {
+    void* q = nullptr;
+    const int rc = dev_alloc(ctx, &q, bytes, what);
+    if (rc != CAV_OK) return rc;
+    rs->bufs.push_back(q);
+    p = static_cast<T*>(q);
+    return CAV_OK;
+}
+
+int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t npix) {
+    if (nsamp <= rs->cap && npix <= rs->pix_cap) return CAV_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    free_bufs(rs);
+    const size_t S = (size_t)nsamp, P = (size_t)npix;
+    RenderArgs& a = rs->a;
+    int rc = CAV_OK;
+    for (int k = 0; k < 2 && rc == CAV_OK; ++k) {
+        rc = grab(ctx, rs, a.q[k].ray, 24 * S, "ray queue");
+        if (rc == CAV_OK) rc = grab(ctx, rs, a.q[k].sid, 4 * S, "ray queue sid");
+        if (rc == CAV_OK) rc = grab(ctx, rs, a.q[k].mult, 12 * S, "ray queue throughput");
+        if (rc == CAV_OK) rc = grab(ctx, rs, a.q[k].pdf, 4 * S, "ray queue pdf");
+        if (rc == CAV_OK) rc = grab(ctx, rs, a.q[k].delta, S, "ray queue delta");
+    }
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.ray, 24 * S, "shadow rays");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.limit, 4 * S, "shadow limits");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.src, 4 * S, "shadow src");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.dst, 4 * S, "shadow dst");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.sid, 4 * S, "shadow sid");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sh.contrib, 12 * S, "shadow contributions");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.vis, S, "visibility");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_prim, 4 * S, "hit prim");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_t, 4 * S, "hit t");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.res, 12 * S, "sample radiance");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sxy, 8 * S, "sample positions");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.counts, 64, "queue counts");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.rays, 64, "ray tallies");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 64, "trace counters");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
+    if (rc != CAV_OK) { free_bufs(rs); return rc; }
+    rs->cap = nsamp;
+    rs->pix_cap = npix;
+    return CAV_OK;
+}
+
+}  // namespace
+
+namespace cav {
+void render_release(cav_ctx* ctx) {
+    RenderState* rs = static_cast<RenderState*>(ctx->render);
+    if (!rs) return;
+    free_bufs(rs);
+    for (void* p : {(void*)rs->primes, (void*)rs->start, (void*)rs->perm})
+        if (p) (void)hipFree(p);
+    for (auto& e : rs->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete rs;
+    ctx->render = nullptr;
+}
+}  // namespace cav
+
+extern "C" {
+
+int cav_scene_create(cav_ctx* ctx, const float* tris, int n, const cav_mesh* meshes, int nmeshes,
+                     const float* tri_uv, int bvh_opt_rounds, int morton_bits, cav_scene** out) {
+    if (!ctx || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    if (!tris || !meshes || n < 2 || nmeshes < 1) return fail(ctx, CAV_ERR_INVALID, "scene needs >= 2 triangles and >= 1 mesh");
+    // meshes must tile the soup in order
+    std::vector<int32_t> tri_mesh(n, -1);
+    int nlights = 0;
+    int next = 0;
+    for (int m = 0; m < nmeshes; ++m) {
+        const cav_mesh& M = meshes[m];
+        if (M.first_tri != next || M.num_tris < 1 || M.first_tri + M.num_tris > n)
+            return fail(ctx, CAV_ERR_INVALID, "meshes must be contiguous triangle ranges covering the soup in order");
+        if (M.material != CAV_MAT_MATTE && M.material != CAV_MAT_MIRROR && M.material != CAV_MAT_GLASS &&
+            M.material != CAV_MAT_PLASTIC)
+            return fail(ctx, CAV_ERR_INVALID, "unsupported material " + std::to_string(M.material));
+        for (int t = M.first_tri; t < M.first_tri + M.num_tris; ++t) tri_mesh[t] = m;
+        next += M.num_tris;
+        nlights += M.is_light ? 1 : 0;
+    }
+    if (next != n) return fail(ctx, CAV_ERR_INVALID, "meshes do not cover every triangle");
+    if (nlights == 0) return fail(ctx, CAV_ERR_INVALID, "scene has no area light");
+    RCHK(ctx, hipSetDevice(ctx->device));
+    cav_scene* sc = new cav_scene();
+    sc->ctx = ctx;
+    sc->ntris = n;
+    sc->nmeshes = nmeshes;
+    sc->nlights = nlights;
+    sc->host_meshes.assign(meshes, meshes + nmeshes);
+    // TriangleMesh::computeArea (Core/TriangleMesh.cu:189-199)
+    sc->host_area.resize(nmeshes);
+    std::vector<int32_t> lights;
+    for (int m = 0; m < nmeshes; ++m) {
+        float a = 0;
+        for (int t = meshes[m].first_tri; t < meshes[m].first_tri + meshes[m].num_tris; ++t) {
+            const float* p = tris + (int64_t)t * 9;
+            const float e1x = p[3] - p[0], e1y = p[4] - p[1], e1z = p[5] - p[2];
+            const float e2x = p[6] - p[0], e2y = p[7] - p[1], e2z = p[8] - p[2];
+            const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+            a += std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5f;
+        }
+        sc->host_area[m] = a;
+        if (meshes[m].is_light) lights.push_back(m);
+    }
+    int rc = dev_alloc(ctx, (void**)&sc->soup, 36 * (size_t)n, "scene soup");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->tri_mesh, 4 * (size_t)n, "triangle meshes");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->meshes, sizeof(cav_mesh) * nmeshes, "meshes");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->area, 4 * (size_t)nmeshes, "mesh areas");
+    if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->lights, 4 * (size_t)nlights, "lights");
+    if (rc == CAV_OK && tri_uv) rc = dev_alloc(ctx, (void**)&sc->uv, 24 * (size_t)n, "uv");
+    hipError_t e = hipSuccess;
+    if (rc == CAV_OK) {
+        e = hipMemcpy(sc->soup, tris, 36 * (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(sc->tri_mesh, tri_mesh.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(sc->meshes, meshes, sizeof(cav_mesh) * nmeshes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(sc->area, sc->host_area.data(), 4 * (size_t)nmeshes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(sc->lights, lights.data(), 4 * (size_t)nlights, hipMemcpyHostToDevice);
+        if (e == hipSuccess && tri_uv) e = hipMemcpy(sc->uv, tri_uv, 24 * (size_t)n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "upload scene");
+    }
+    if (rc == CAV_OK)
+        rc = cav_bvh_build_device(ctx, sc->soup, n, nullptr, sc->tri_mesh, morton_bits, bvh_opt_rounds, &sc->bvh);
+    if (rc != CAV_OK) { cav_scene_destroy(sc); return rc; }
+    *out = sc;
+    return CAV_OK;
+}
+
+int cav_scene_destroy(cav_scene* sc) {
+    if (!sc) return CAV_ERR_INVALID;
+    if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
+    if (sc->bvh) cav_bvh_destroy(sc->bvh);
+    for (void* p : {(void*)sc->soup, (void*)sc->tri_mesh, (void*)sc->meshes, (void*)sc->area,
+                    (void*)sc->lights, (void*)sc->uv})
+        if (p) (void)hipFree(p);
+    delete sc;
+    return CAV_OK;
+}
+
+const cav_bvh* cav_scene_bvh(const cav_scene* sc) { return sc ? sc->bvh : nullptr; }
+
+int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_render_params* prm,
+               float* film_rgb, float* film_weight, cav_render_stats* stats) {
+    if (!ctx || !sc || !cam || !prm || !film_rgb || !film_weight) return CAV_ERR_INVALID;
+    if (prm->integrator != CAV_INTEGRATOR_PATH)
+        return fail(ctx, CAV_ERR_INVALID, "integrator not supported by this build");
+    if (prm->spp < 1 || prm->max_depth < 1 || cam->width < 1 || cam->height < 1)
+        return fail(ctx, CAV_ERR_INVALID, "spp, max_depth and resolution must be positive");
+    const int64_t npix = (int64_t)cam->width * cam->height;
+    const int64_t nsamp = npix * prm->spp;
+    if (nsamp >= (int64_t)1 << 32) return fail(ctx, CAV_ERR_INVALID, "more than 2^32 samples in one render");
+    // dimensions used: 2 + 10 per bounce + 1 per bounce after the fourth
+    const int64_t dims = 2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0);
+    if (dims > 1024) return fail(ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
+    RCHK(ctx, hipSetDevice(ctx->device));
+    RenderState* rs = nullptr;
+    int rc = ensure_state(ctx, rs);
+    if (rc != CAV_OK) return rc;
+    if ((rc = ensure_buffers(ctx, rs, nsamp, npix)) != CAV_OK) return rc;
+    RenderArgs a = rs->a;
+    a.scene = SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
+                       sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
+    a.halton = HaltonDev{rs->primes, rs->start, rs->perm};
+    std::memcpy(a.cam.m, cam->cam_to_world, sizeof a.cam.m);
+    a.cam.depth = cam->depth;
+    a.cam.width = cam->width;
+    a.cam.height = cam->height;
+    a.width = cam->width;
+    a.height = cam->height;
+    a.spp = prm->spp;
+    a.max_depth = prm->max_depth;
+    a.nsamp = nsamp;
+    hipStream_t s = ctx->stream;
+    RCHK(ctx, hipMemsetAsync(a.rays, 0, 16, s));
+    RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));
+    RCHK(ctx, hipEventRecord(rs->ev[0], s));
+    RCHK(ctx, render_path(a, s));
+    RCHK(ctx, hipEventRecord(rs->ev[1], s));
+    unsigned long long tally[2] = {0, 0};
+    uint32_t tc[2] = {0, 0};
+    RCHK(ctx, hipMemcpyAsync(film_rgb, a.film_rgb, 12 * (size_t)npix, hipMemcpyDeviceToHost, s));
+    RCHK(ctx, hipMemcpyAsync(film_weight, a.film_w, 4 * (size_t)npix, hipMemcpyDeviceToHost, s));
+    RCHK(ctx, hipMemcpyAsync(tally, a.rays, 16, hipMemcpyDeviceToHost, s));
+    RCHK(ctx, hipMemcpyAsync(tc, a.trace_counters, 8, hipMemcpyDeviceToHost, s));
+    RCHK(ctx, hipStreamSynchronize(s));
+    float ms = 0.f;
+    RCHK(ctx, hipEventElapsedTime(&ms, rs->ev[0], rs->ev[1]));
+    if (stats) {
+        stats->ms_total = ms;
+        stats->closest_rays = (int64_t)tally[0];
+        stats->shadow_rays = (int64_t)tally[1];
+        stats->mrays_per_s = ms > 0 ? (double)(tally[0] + tally[1]) / (ms * 1e3) : 0.0;
+    }
+    if (tc[0] || tc[1]) return fail(ctx, CAV_ERR_OVERFLOW, "traversal stack overflow while rendering");
+    return CAV_OK;
+}
+
+}  // extern "C"

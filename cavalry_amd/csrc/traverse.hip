@@ -87,7 +87,8 @@ template <bool kClosest>
 __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
                                                   const TTri* __restrict__ tt,
                                                   const float* __restrict__ root,
-                                                  const float* __restrict__ rays, int64_t nrays,
+                                                  const float* __restrict__ rays, int64_t nrays_arg,
+                                                  const uint32_t* __restrict__ nrays_dev,
                                                   const float* __restrict__ limit,
                                                   const int32_t* __restrict__ src,
                                                   const int32_t* __restrict__ dst,
@@ -101,6 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
     __shared__ int lstack[kLdsDepth][kBlock];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t nrays = nrays_dev ? (int64_t)*nrays_dev : nrays_arg;
 
     int64_t ri = -1;       // ray held by this lane; -1 = needs one, -2 = pool drained
     v3 o = mk3(0, 0, 0), d = o, inv = o;
@@ -246,7 +248,7 @@ hipError_t trace_closest(const TNode* tn, const TTri* tt, int ntris, const float
     const int64_t need = (nrays + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, root, rays,
-                       nrays, nullptr, nullptr, nullptr, prim, t, u, v, nullptr, counters,
+                       nrays, nullptr, nullptr, nullptr, nullptr, prim, t, u, v, nullptr, counters,
                        reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
@@ -262,8 +264,45 @@ hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const floa
     const int64_t need = (nrays + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, nullptr,
-                       rays, nrays, limit, src, dst, nullptr, nullptr, nullptr, nullptr, visible,
+                       rays, nrays, nullptr, limit, src, dst, nullptr, nullptr, nullptr, nullptr, visible,
                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
+                       2 * ntris - 1);
+    return hipGetLastError();
+}
+
+// Queue variants: the ray count is read on the device (no host round trip),
+// the grid is the full persistent size. counters as above.
+hipError_t trace_closest_queue(const TNode* tn, const TTri* tt, int ntris, const float* root,
+                               const float* rays, const uint32_t* nrays_dev, int64_t capacity,
+                               int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
+                               hipStream_t s) {
+    hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
+    if (e != hipSuccess) return e;
+    const void* k = reinterpret_cast<const void*>(&k_trace<true>);
+    int64_t blocks = persistent_blocks(k);
+    const int64_t need = (capacity + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, root, rays,
+                       capacity, nrays_dev, nullptr, nullptr, nullptr, prim, t, u, v, nullptr,
+                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
+    return hipGetLastError();
+}
+
+hipError_t trace_occluded_queue(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+                                const uint32_t* nrays_dev, int64_t capacity, const float* limit,
+                                const int32_t* src, const int32_t* dst, uint8_t* visible,
+                                uint32_t* counters, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
+    if (e != hipSuccess) return e;
+    const void* k = reinterpret_cast<const void*>(&k_trace<false>);
+    int64_t blocks = persistent_blocks(k);
+    const int64_t need = (capacity + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, nullptr,
+                       rays, capacity, nrays_dev, limit, src, dst, nullptr, nullptr, nullptr, nullptr,
+                       visible, counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
                        2 * ntris - 1);
     return hipGetLastError();
 }

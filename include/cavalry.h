@@ -118,6 +118,76 @@ int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_
 int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
                              const float* limit_dev, int64_t nrays, uint8_t* visible_dev);
 
+/* ---- scenes and rendering ---------------------------------------------- */
+enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_GLASS = 4, CAV_MAT_PLASTIC = 5 };
+enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1 };
+
+/* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
+ * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),
+ * 1 mirror (kr[0]), 4 glass (kr, kt, ior; perfectly specular), 5 plastic
+ * (kd, ks, roughness = GGX alpha). texture = -1 for none. is_light marks a
+ * DiffuseAreaLight of radiance `emission` (Lights/DiffuseAreaLight.h). */
+typedef struct {
+    int32_t first_tri, num_tris;
+    int32_t material;
+    int32_t texture;
+    float kd[3], ks[3], kr[3], kt[3];
+    float roughness, ior;
+    float emission[3];
+    int32_t is_light, reverse_orientation;
+} cav_mesh;
+
+/* Perspective camera: column-major camera-to-world matrix (glm layout), film
+ * depth for unit-size pixels, resolution (Cameras/PerspectiveCamera.h). */
+typedef struct {
+    float cam_to_world[16];
+    float depth;
+    int32_t width, height;
+} cav_camera;
+
+/* LookAt + fov (degrees, of the shorter image axis) -> camera, as the
+ * reference's parser and PerspectiveCamera constructor compute it
+ * (SceneLoading/Parsing.cu:196-204,257; Cameras/PerspectiveCamera.cu:6-20). */
+int cav_camera_look_at(const float* eye3, const float* center3, const float* up3,
+                       float fov_degrees, int width, int height, cav_camera* out);
+
+typedef struct cav_scene cav_scene;
+
+/* Replaces Scene::prepareForRender (Core/Scene.cu:117-151): uploads the
+ * triangle soup (float[n][9], meshes as contiguous ranges covering it) and
+ * the mesh table, computes mesh areas (TriangleMesh::computeArea) and builds
+ * the BVH with bvh_opt_rounds treelet rounds (the reference's default is 3,
+ * or 10 when spp >= 64; resolve it before calling). tri_uv (float[n][6]) may
+ * be NULL. At least one mesh must be an area light. */
+int cav_scene_create(cav_ctx* ctx, const float* triangles, int n, const cav_mesh* meshes,
+                     int nmeshes, const float* tri_uv, int bvh_opt_rounds, int morton_bits,
+                     cav_scene** out);
+int cav_scene_destroy(cav_scene* scene);
+/* the scene's BVH (owned by the scene) */
+const cav_bvh* cav_scene_bvh(const cav_scene* scene);
+
+typedef struct {
+    int32_t integrator;  /* CAV_INTEGRATOR_PATH or CAV_INTEGRATOR_RLPATH */
+    int32_t spp;         /* samples per pixel */
+    int32_t max_depth;   /* bounces (the scene's "integer maxdepth") */
+    int32_t flags;       /* reserved, 0 */
+} cav_render_params;
+
+typedef struct {
+    double ms_total;            /* device time of the render (HIP events) */
+    int64_t closest_rays;       /* camera + continuation rays traced */
+    int64_t shadow_rays;        /* shadow rays traced */
+    double mrays_per_s;         /* (closest + shadow) / ms_total */
+} cav_render_stats;
+
+/* Replaces Renderer::render -> Integrator::render (Core/Renderer.cu:3-6,
+ * Integrators/PathTracingIntegrator.cu:318-425): renders all spp samples of
+ * every pixel and writes the box-filtered film as per-pixel sums (rgb,
+ * float[h][w][3]) and weights (float[h][w]); the image is rgb / weight. */
+int cav_render(cav_ctx* ctx, cav_scene* scene, const cav_camera* camera,
+               const cav_render_params* params, float* film_rgb, float* film_weight,
+               cav_render_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

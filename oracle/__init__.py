@@ -127,3 +127,49 @@ def trace_occluded(nd: Nodes, tris: np.ndarray, rays: np.ndarray, limit: np.ndar
         _p(np.ascontiguousarray(limit, np.float32)), _p(src), _p(dst), ctypes.c_int64(r),
         _p(vis))
     return vis
+
+
+def halton(index: int, dim: int) -> float:
+    f = lib().ref_halton
+    f.restype = ctypes.c_float
+    return float(f(ctypes.c_uint64(index), ctypes.c_int(dim)))
+
+
+def halton_table():
+    primes = np.empty(1024, np.uint32)
+    start = np.empty(1024, np.uint32)
+    lib().ref_halton_table(_p(primes), _p(start), None, ctypes.c_int(0))
+    total = int(start[-1] + primes[-1])
+    perm = np.empty(total, np.uint16)
+    lib().ref_halton_table(_p(primes), _p(start), _p(perm), ctypes.c_int(total))
+    return primes, start, perm
+
+
+def mesh_areas(tris, meshes) -> np.ndarray:
+    out = np.empty(len(meshes), np.float32)
+    lib().ref_mesh_areas(_p(np.ascontiguousarray(tris, np.float32)), _p(meshes),
+                         ctypes.c_int(len(meshes)), _p(out))
+    return out
+
+
+def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30):
+    """`path` integrator on the CPU. Returns (film_rgb [H,W,3], film_w [H,W],
+    rays_traced). camera is a cavalry_amd Camera (cav_camera)."""
+    tris = np.ascontiguousarray(scene.tris, np.float32)
+    _, _, nd = lbvh_build(tris, morton_bits)
+    if bvh_rounds:
+        treelet_optimize(nd, bvh_rounds)
+    w, h = scene.width, scene.height
+    rgb = np.zeros((h, w, 3), np.float32)
+    wt = np.zeros((h, w), np.float32)
+    rays = np.zeros(1, np.int64)
+    cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
+    tri_mesh = scene.tri_mesh
+    rc = lib().ref_render_path(
+        _p(tris), ctypes.c_int(len(tris)), _p(tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(cam16), ctypes.c_float(camera.depth),
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), ctypes.c_int(scene.max_depth),
+        _p(rgb), _p(wt), _p(rays))
+    if rc != 0:
+        raise ValueError(f"ref_render_path failed ({rc})")
+    return rgb, wt, int(rays[0])

@@ -4,6 +4,18 @@
 #pragma once
 #include <cstdint>
 
+// One mesh of a scene; triangles [first_tri, first_tri + num_tris) of the
+// soup belong to it. Same layout as cav_mesh in include/cavalry.h.
+typedef struct {
+    int32_t first_tri, num_tris;
+    int32_t material;  // 0 matte, 1 mirror, 4 glass, 5 plastic (Core/Material.h:10-12)
+    int32_t texture;   // -1: none
+    float kd[3], ks[3], kr[3], kt[3];
+    float roughness, ior;
+    float emission[3];
+    int32_t is_light, reverse_orientation;
+} cav_mesh_ref;
+
 extern "C" {
 void ref_scene_bounds(const float* tris, int n, float* out6);
 void ref_morton(const float* tris, int n, const float* scene6, int bits, uint64_t* codes);

@@ -22,6 +22,7 @@ static inline f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b
 static inline f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 static inline f3 operator*(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+static inline f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
 static inline f3 mul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 static inline f3 divv(f3 a, f3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
 // dot and cross in the operand order of the reference's float3 helpers

@@ -1,0 +1,671 @@
+// CPU oracle: the `path` integrator, one sample at a time.
+//
+// TEST INFRASTRUCTURE ONLY: a restatement of Integrators/PathTracingIntegrator.cu
+// and the device code it calls (Halton sampler, perspective camera, matte /
+// plastic / mirror / glass BSDFs, diffuse area lights, box-filter film), used
+// to check the HIP wavefront renderer and as the timed CPU baseline.
+//
+// In the reference every sample of a queue consumes the same number of
+// Halton dimensions per bounce (8 for light selection and sampling, 2 for the
+// BSDF, 1 more for Russian roulette after depth 3), so syncDimension never
+// changes a dimension and each sample can be traced on its own.
+//
+// Float semantics: IEEE binary32 operations in the reference's order, no
+// contraction; expressions the reference evaluates in double (through M_PI
+// and double literals) are evaluated in double here too. normalize(v) is
+// v * (1/sqrt(dot(v,v))) with a correctly rounded sqrt and division.
+#include "ref_api.h"
+#include "ref_math.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace ref;
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- Halton
+// Samplers/HaltonSampler.cu:7-57: first 1024 primes, per-prime digit
+// permutations from a Fisher-Yates shuffle driven by std::rand() after
+// std::srand(533799). The table is frozen to the Microsoft C runtime's rand()
+// (state = state * 214013 + 2531011, output bits 16..30), the runtime the
+// reference was built with.
+struct Halton {
+    std::vector<uint32_t> primes;
+    std::vector<uint32_t> start;
+    std::vector<uint16_t> perm;
+    Halton() {
+        primes.push_back(2);
+        for (uint32_t i = 3; primes.size() < 1024; ++i) {
+            bool prime = true;
+            for (uint32_t p : primes) {
+                if (p * p > i) break;
+                if (i % p == 0) { prime = false; break; }
+            }
+            if (prime) primes.push_back(i);
+        }
+        uint32_t total = 0;
+        for (uint32_t p : primes) { start.push_back(total); total += p; }
+        perm.resize(total);
+        uint32_t state = 533799u;
+        auto rnd = [&]() { state = state * 214013u + 2531011u; return (int)((state >> 16) & 0x7fffu); };
+        for (size_t k = 0; k < primes.size(); ++k) {
+            const int p = (int)primes[k];
+            std::vector<uint16_t> v(p);
+            for (int j = 0; j < p; ++j) v[j] = (uint16_t)j;
+            for (int i = p - 1; i > 0; --i) std::swap(v[i], v[rnd() % (i + 1)]);
+            for (int j = 0; j < p; ++j) perm[start[k] + j] = v[j];
+        }
+    }
+    // HaltonSampler::rand1 / runHalton (HaltonSampler.h:13-23, 58-65)
+    float rand1(uint64_t index, int& dim) const {
+        const uint32_t base = primes[dim];
+        const uint16_t* pm = perm.data() + start[dim];
+        ++dim;
+        float r = 0, f = 1;
+        uint64_t i = index;
+        while (i > 0) {
+            f = f / (float)base;
+            r = r + f * (float)pm[i % base];
+            i = i / base;
+        }
+        return r;
+    }
+    int rand_int(int n, uint64_t index, int& dim) const {  // HaltonSampler.h:50-56
+        float f = rand1(index, dim);
+        int r = (int)(f * (float)n);
+        if (r >= n) r = n - 1;
+        return r;
+    }
+};
+
+const Halton& halton() {
+    static Halton h;
+    return h;
+}
+
+// ---------------------------------------------------------------- vectors
+f3 normalize(f3 v) {
+    float inv = 1.0f / std::sqrt(dot(v, v));
+    return v * inv;
+}
+float length(f3 v) { return std::sqrt(dot(v, v)); }
+f3 neg(f3 v) { return mk(-v.x, -v.y, -v.z); }
+bool all_zero(f3 c) { return c.x == 0 && c.y == 0 && c.z == 0; }
+f3 mulv(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+float clampf(float x, float lo, float hi) { return fmaxr(lo, fminr(x, hi)); }
+
+// Utils/MathsCommons.h:193-203
+void coordinate_system(f3 n, f3& t0, f3& t1) {
+    if (std::fabs(n.x) < 0.5) t0 = cross(mk(1, 0, 0), n);
+    else t0 = cross(mk(0, 1, 0), n);
+    t0 = normalize(t0);
+    t1 = normalize(cross(n, t0));
+}
+// Core/IntersectionResult.h:28-52 (glm mat3 products, rows / columns)
+f3 to_local(f3 v, f3 t0, f3 t1, f3 n) { return mk(dot(t0, v), dot(t1, v), dot(n, v)); }
+f3 to_world(f3 v, f3 t0, f3 t1, f3 n) {
+    return mk(t0.x * v.x + t1.x * v.y + n.x * v.z, t0.y * v.x + t1.y * v.y + n.y * v.z,
+              t0.z * v.x + t1.z * v.y + n.z * v.z);
+}
+
+// Utils/MathsCommons.h:109-162 (the sinZenith == 0 test compares a function
+// pointer and is always false, so the clamped quotient is always taken)
+float cos2z(f3 w) { return w.z * w.z; }
+float sin2z(f3 w) { return fmaxr(0.f, 1.f - cos2z(w)); }
+float sinz(f3 w) { return std::sqrt(sin2z(w)); }
+float tanz(f3 w) { return sinz(w) / w.z; }
+float tan2z(f3 w) { return sin2z(w) / cos2z(w); }
+float cos_az(f3 w) { return clampf(w.x / sinz(w), -1.f, 1.f); }
+float sin_az(f3 w) { return clampf(w.y / sinz(w), -1.f, 1.f); }
+bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0; }
+float pow5(float f) { return (f * f) * (f * f) * f; }
+f3 reflect(f3 i, f3 n) { return (2.0f * n) * dot(n, i) - i; }
+bool refract(f3 i, f3 n, float eta, f3& out) {
+    float c = dot(n, i);
+    float s2 = fmaxr(0.f, 1.f - c * c);
+    float s2t = eta * eta * s2;
+    if (s2t >= 1) return false;
+    float ct = std::sqrt(1 - s2t);
+    out = eta * (i * -1.f) + (eta * c - ct) * n;
+    return true;
+}
+bool veq(f3 a, f3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// Utils/RandomUtils.h:20-41 (theta is double through M_PI)
+f3 cosine_hemisphere(float ux, float uy) {
+    float ox = 2.f * ux - 1.f, oy = 2.f * uy - 1.f;
+    float dx = 0, dy = 0;
+    if (!(ox == 0 && oy == 0)) {
+        double theta;
+        float r;
+        if (std::fabs(ox) > std::fabs(oy)) { r = ox; theta = (kPi / 4.f) * (double)(oy / ox); }
+        else { r = oy; theta = (kPi / 2.f) - (kPi / 4.f) * (double)(ox / oy); }
+        dx = r * (float)std::cos(theta);
+        dy = r * (float)std::sin(theta);
+    }
+    float z = std::sqrt(fmaxr(0.f, 1 - dx * dx - dy * dy));
+    return mk(dx, dy, z);
+}
+float cos_pdf(float z) { return (float)std::fabs((double)z / kPi); }
+
+// ---------------------------------------------------------------- BSDFs
+struct GGX {
+    float ax, ay;
+    float D(f3 h) const {  // MicrofacetDistribution.h:8-22
+        float t2 = tan2z(h);
+        if (std::isinf(t2)) return 0;
+        const float c4 = cos2z(h) * cos2z(h);
+        float e = (cos_az(h) * cos_az(h) / (ax * ax) + sin_az(h) * sin_az(h) / (ay * ay)) * t2;
+        return (float)(1 / (kPi * ax * ay * c4 * (1 + e) * (1 + e)));
+    }
+    float lambda(f3 w) const {
+        if (dot(w, mk(0, 0, 1)) == 0) return 0;
+        float at = std::fabs(tanz(w));
+        float a = std::sqrt(cos_az(w) * cos_az(w) * ax * ax + sin_az(w) * sin_az(w) * ay * ay);
+        float a2 = (a * at) * (a * at);
+        return (-1 + std::sqrt(1.f + a2)) / 2;
+    }
+    float G1(f3 w) const { return 1 / (1 + lambda(w)); }
+    float G(f3 i, f3 o) const { return 1.f / (1.f + lambda(i) + lambda(o)); }
+    float pdf(f3 h, f3 o) const { return D(h) * G1(o) * std::fabs(dot(o, h)) / std::fabs(o.z); }
+    static void sample11(float cosT, float u1, float u2, float* sx, float* sy) {
+        if (cosT > .9999) {  // phi is a float here, so cos/sin are the float versions
+            float r = std::sqrt(u1 / (1 - u1));
+            float phi = (float)(6.28318530718 * u2);
+            *sx = r * std::cos(phi);
+            *sy = r * std::sin(phi);
+            return;
+        }
+        float sinT = std::sqrt(fmaxr(0.f, 1.f - cosT * cosT));
+        float tanT = sinT / cosT;
+        float a = 1 / tanT;
+        float g1 = 2 / (1 + std::sqrt(1.f + 1.f / (a * a)));
+        float A = 2 * u1 / g1 - 1;
+        float tmp = 1.f / (A * A - 1.f);
+        if (tmp > 1e10) tmp = 1e10;
+        float B = tanT;
+        float Dd = std::sqrt(fmaxr(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
+        float s1 = B * tmp - Dd, s2 = B * tmp + Dd;
+        *sx = (A < 0 || s2 > 1.f / tanT) ? s1 : s2;
+        float S;
+        if (u2 > 0.5f) { S = 1.f; u2 = 2.f * (u2 - .5f); }
+        else { S = -1.f; u2 = 2.f * (.5f - u2); }
+        float z = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f)) /
+                  (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
+        *sy = S * z * std::sqrt(1.f + *sx * *sx);
+    }
+    f3 sample(float u1, float u2, f3 o) const {  // MicrofacetDistribution.h:288-317
+        bool flip = o.z < 0;
+        if (flip) o = o * -1.f;
+        f3 os = normalize(mk(ax * o.x, ay * o.y, o.z));
+        float sx, sy;
+        sample11(os.z, u1, u2, &sx, &sy);
+        float t = cos_az(os) * sx - sin_az(os) * sy;
+        sy = sin_az(os) * sx + cos_az(os) * sy;
+        sx = t;
+        sx = ax * sx;
+        sy = ay * sy;
+        f3 r = normalize(mk(-sx, -sy, 1.f));
+        if (flip) r = r * -1.f;
+        return r;
+    }
+};
+
+f3 schlick(float c, f3 f0) {  // BSDFs/Fresnel.h:12-17
+    float t = pow5(1 - std::fabs(c));
+    return f0 + (mk(1, 1, 1) - f0) * t;
+}
+
+enum { kMatte = 0, kMirror = 1, kGlass = 4, kPlastic = 5 };
+
+struct Bsdf {
+    int type;
+    f3 kd;            // matte / plastic diffuse
+    f3 ks;            // plastic specular colour
+    GGX ggx;          // plastic
+    f3 f0;            // Fresnel (plastic: from IOR 1.5; glass: from its IOR)
+    float kr;         // mirror reflectivity
+    f3 refl, trans;   // glass
+    float ior;        // glass below IOR (above = 1)
+
+    bool is_delta() const { return type == kMirror || type == kGlass; }
+
+    // LambertianBSDF (BSDFs/Lambertian.h)
+    f3 lam_eval(f3 i, f3 o) const {
+        if (!same_hemi(i, o)) return mk(0, 0, 0);
+        return divs(kd, (float)kPi);
+    }
+    // MicrofacetBSDF reflection only (BSDFs/Microfacet.h:36-53, 97-116)
+    f3 mf_eval(f3 i, f3 o) const {
+        if (!same_hemi(i, o)) return mk(0, 0, 0);
+        float co = std::fabs(o.z), ci = std::fabs(i.z);
+        f3 h = i + o;
+        if (ci == 0 || co == 0) return mk(0, 0, 0);
+        if (h.x == 0 && h.y == 0 && h.z == 0) return mk(0, 0, 0);
+        h = normalize(h);
+        f3 F = schlick(dot(o, h), f0);
+        return divs(mulv(ks * ggx.D(h) * ggx.G(o, i), F), 4 * ci * co);
+    }
+    float mf_pdf(f3 i, f3 o, f3 h) const { return ggx.pdf(h, o) / (4.f * dot(o, h)); }
+    float mf_pdf(f3 i, f3 o) const { return mf_pdf(i, o, normalize(i + o)); }
+
+    f3 eval(f3 i, f3 o) const {
+        switch (type) {
+            case kMatte: return lam_eval(i, o);
+            case kPlastic: return lam_eval(i, o) + mf_eval(i, o);
+            case kMirror:  // BSDFs/MirrorBSDF.h:18-24
+                if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk(0, 0, 0);
+                return mk(1, 1, 1) * kr / std::fabs(i.z);
+            default: {  // SpecularBSDF with transmission (BSDFs/Specular.h:86-107)
+                if (same_hemi(i, o)) {
+                    if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk(0, 0, 0);
+                    return spec_refl(i, o);
+                }
+                f3 refr;
+                if (!spec_refract(o, refr)) return mk(0, 0, 0);
+                if (!veq(refr, i)) return mk(0, 0, 0);
+                return spec_trans(i, o);
+            }
+        }
+    }
+    f3 spec_refl(f3 i, f3 o) const {
+        if (!same_hemi(i, o)) return mk(0, 0, 0);
+        f3 F = schlick(std::fabs(i.z), f0);
+        return divs(mulv(refl, F), std::fabs(i.z));
+    }
+    f3 spec_trans(f3 i, f3 o) const {
+        if (same_hemi(i, o)) return mk(0, 0, 0);
+        f3 F = schlick(std::fabs(i.z), f0);
+        return divs(mulv(trans, mk(1, 1, 1) - F), std::fabs(i.z));
+    }
+    bool spec_refract(f3 o, f3& out) const {
+        float eta = o.z > 0 ? (1.f / ior) : (ior / 1.f);
+        f3 n = mk(0, 0, 1);
+        if (o.z < 0) n.z = -1;
+        return refract(o, n, eta, out);
+    }
+    float spec_refl_prob(f3 o) const {
+        float p = schlick(std::fabs(o.z), f0).x;
+        if (all_zero(refl)) p = 0;
+        else if (all_zero(trans)) p = 1;
+        return p;
+    }
+
+    float pdf(f3 i, f3 o) const {
+        switch (type) {
+            case kMatte: return cos_pdf(i.z);
+            case kPlastic: return 0.5f * cos_pdf(i.z) + (1.f - 0.5f) * mf_pdf(i, o);
+            case kMirror:
+                if (i.x == -o.x && i.y == -o.y && i.z == o.z) return 1;
+                return 0;
+            default: {
+                if (same_hemi(i, o)) {
+                    if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return 0;
+                    return spec_refl_prob(o);
+                }
+                f3 refr;
+                if (!spec_refract(o, refr)) return 0;
+                if (!veq(refr, i)) return 0;
+                return 1.f - spec_refl_prob(o);
+            }
+        }
+    }
+
+    f3 sample(float ux, float uy, f3& wi, f3 o, float& pdfv) const {
+        switch (type) {
+            case kMatte: {
+                wi = cosine_hemisphere(ux, uy);
+                pdfv = cos_pdf(wi.z);
+                if (o.z < 0) wi.z *= -1;
+                return lam_eval(wi, o);
+            }
+            case kPlastic: {  // BSDFs/PlasticBSDF.h:34-49
+                float p = ux;
+                if (p < 0.5f) {
+                    p = p * (1.f / 0.5f);
+                    wi = cosine_hemisphere(p, uy);
+                    if (o.z < 0) wi.z *= -1;
+                } else {
+                    p = (p - 0.5f) * (1.f / (1.f - 0.5f));
+                    f3 h = ggx.sample(p, uy, o);
+                    wi = reflect(o, h);
+                }
+                pdfv = pdf(wi, o);
+                return eval(wi, o);
+            }
+            case kMirror: {
+                wi = o * -1;
+                wi.z = o.z;
+                pdfv = 1;
+                return eval(wi, o);
+            }
+            default: {  // SpecularBSDF::sample (BSDFs/Specular.h:122-144)
+                float pr = spec_refl_prob(o);
+                if (ux < pr) {
+                    wi = mk(-o.x, -o.y, o.z);
+                    pdfv = 1;
+                    f3 r = spec_refl(wi, o);
+                    pdfv *= pr;
+                    return r;
+                }
+                f3 r = mk(0, 0, 0);
+                if (spec_refract(o, wi)) {
+                    pdfv = 1;
+                    r = spec_trans(wi, o);
+                }
+                pdfv *= (1.f - pr);
+                return r;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------- scene
+struct Scene {
+    const float* tris; int ntris;
+    const cav_mesh_ref* meshes; int nmeshes;
+    const int32_t* tri_mesh;
+    std::vector<int> lights;  // mesh ids of area lights, in mesh order
+    std::vector<float> area;  // per mesh
+    const float* box; const int32_t* left; const int32_t* right; const int32_t* prim;
+};
+
+f3 tri_v(const Scene& s, int t, int j) {
+    const float* p = s.tris + (int64_t)t * 9 + j * 3;
+    return mk(p[0], p[1], p[2]);
+}
+
+Bsdf make_bsdf(const cav_mesh_ref& m) {
+    Bsdf b{};
+    b.type = m.material;
+    b.kd = mk(m.kd[0], m.kd[1], m.kd[2]);
+    b.ks = mk(m.ks[0], m.ks[1], m.ks[2]);
+    b.ggx = GGX{m.roughness, m.roughness};
+    b.kr = m.kr[0];
+    b.refl = mk(m.kr[0], m.kr[1], m.kr[2]);
+    b.trans = mk(m.kt[0], m.kt[1], m.kt[2]);
+    b.ior = m.ior;
+    float eta = m.material == kGlass ? m.ior : 1.5f;  // Materials/Plastic.h:111, Glass.h:238
+    float t = (eta - 1) / (eta + 1);
+    b.f0 = mk(t * t, t * t, t * t);
+    return b;
+}
+
+// single-ray traversal on the reference node arrays (BVH/BVH.h)
+bool box_hit1(const float* b, f3 o, f3 inv, float& mind) {
+    f3 tmin = mul(mk(b[0], b[1], b[2]) - o, inv);
+    f3 tmax = mul(mk(b[3], b[4], b[5]) - o, inv);
+    float maxmin = fmin_fmax(tmin.x, tmax.x, fmin_fmax(tmin.y, tmax.y, fminr(tmin.z, tmax.z)));
+    float minmax = fmax_fmin(tmin.x, tmax.x, fmax_fmin(tmin.y, tmax.y, fmaxr(tmin.z, tmax.z)));
+    if (maxmin <= minmax && minmax > 0) { mind = fmaxr(0.f, maxmin); return true; }
+    mind = -1;
+    return false;
+}
+bool tri_hit1(const Scene& s, int p, f3 o, f3 d, float& t, float& u, float& v) {
+    f3 v0 = tri_v(s, p, 0);
+    f3 e1 = tri_v(s, p, 1) - v0, e2 = tri_v(s, p, 2) - v0;
+    f3 tv = o - v0, pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (det == 0) return false;
+    det = 1.f / det;
+    u = dot(tv, pv) * det;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 qv = cross(tv, e1);
+    v = dot(d, qv) * det;
+    if (v < 0.0f || (u + v) > 1.0f) return false;
+    t = dot(e2, qv) * det;
+    return t > 0.f;
+}
+int closest(const Scene& s, f3 o, f3 d, float& tb) {
+    f3 inv = divv(mk(1, 1, 1), d);
+    float md;
+    tb = FLT_MAX;
+    int bp = -1;
+    if (!box_hit1(s.box, o, inv, md)) return -1;
+    int st[64], top = 0;
+    st[0] = 0;
+    while (top >= 0) {
+        int c = st[top--];
+        if (s.prim[c] >= 0) {
+            float t, u, v;
+            if (tri_hit1(s, s.prim[c], o, d, t, u, v) && (bp < 0 || t < tb)) { tb = t; bp = s.prim[c]; }
+        } else {
+            float dl, dr;
+            int l = s.left[c], r = s.right[c];
+            box_hit1(s.box + (int64_t)l * 6, o, inv, dl);
+            box_hit1(s.box + (int64_t)r * 6, o, inv, dr);
+            bool hl = dl >= 0 && dl < tb, hr = dr >= 0 && dr < tb;
+            if (hl && hr) {
+                if (dl > dr) { if (top < 63) st[++top] = l; if (top < 63) st[++top] = r; }
+                else { if (top < 63) st[++top] = r; if (top < 63) st[++top] = l; }
+            } else if (hl) { if (top < 63) st[++top] = l; }
+            else if (hr) { if (top < 63) st[++top] = r; }
+        }
+    }
+    return bp;
+}
+bool visible(const Scene& s, f3 o, f3 d, float lim, int sm, int dm) {
+    f3 inv = divv(mk(1, 1, 1), d);
+    int st[64], top = 0;
+    st[0] = 0;
+    while (top >= 0) {
+        int c = st[top--];
+        if (s.prim[c] >= 0) {
+            int p = s.prim[c];
+            int m = s.tri_mesh[p];
+            if (m == sm || m == dm) continue;
+            float t, u, v;
+            if (tri_hit1(s, p, o, d, t, u, v) && t < lim) return false;
+        } else {
+            float dl, dr;
+            int l = s.left[c], r = s.right[c];
+            box_hit1(s.box + (int64_t)l * 6, o, inv, dl);
+            box_hit1(s.box + (int64_t)r * 6, o, inv, dr);
+            if (dl >= 0 && dr >= 0) {
+                if (dl > dr) { if (top < 63) st[++top] = l; if (top < 63) st[++top] = r; }
+                else { if (top < 63) st[++top] = r; if (top < 63) st[++top] = l; }
+            } else if (dl >= 0) { if (top < 63) st[++top] = l; }
+            else if (dr >= 0) { if (top < 63) st[++top] = r; }
+        }
+    }
+    return true;
+}
+
+f3 tri_normal(const Scene& s, int p) {  // Core/Triangle.h:211-222 without vertex normals
+    f3 v0 = tri_v(s, p, 0);
+    f3 n = normalize(cross(tri_v(s, p, 1) - v0, tri_v(s, p, 2) - v0));
+    if (s.meshes[s.tri_mesh[p]].reverse_orientation) n = n * -1.f;
+    return n;
+}
+
+float power_heuristic(float a, float b) { a *= a; b *= b; return a / (a + b); }
+
+// glm mat4 * vec4 followed by the divide (Utils/MathsCommons.h:57-60)
+f3 apply(const float* m, f3 v) {
+    float r[4];
+    for (int k = 0; k < 4; ++k)
+        r[k] = (m[0 * 4 + k] * v.x + m[1 * 4 + k] * v.y) + (m[2 * 4 + k] * v.z + m[3 * 4 + k] * 1.f);
+    return mk(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
+}
+
+}  // namespace
+
+extern "C" void ref_halton_table(uint32_t* primes, uint32_t* start, uint16_t* perm, int perm_cap) {
+    const Halton& h = halton();
+    for (int i = 0; i < 1024; ++i) { primes[i] = h.primes[i]; start[i] = h.start[i]; }
+    for (int i = 0; i < perm_cap && i < (int)h.perm.size(); ++i) perm[i] = h.perm[i];
+}
+
+extern "C" float ref_halton(uint64_t index, int dim) {
+    int d = dim;
+    return halton().rand1(index, d);
+}
+
+extern "C" void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, int nmeshes, float* area) {
+    for (int m = 0; m < nmeshes; ++m) {  // TriangleMesh::computeArea (Core/TriangleMesh.cu:189-199)
+        float a = 0;
+        for (int t = meshes[m].first_tri; t < meshes[m].first_tri + meshes[m].num_tris; ++t) {
+            const float* p = tris + (int64_t)t * 9;
+            f3 p0 = mk(p[0], p[1], p[2]), p1 = mk(p[3], p[4], p[5]), p2 = mk(p[6], p[7], p[8]);
+            a += length(cross(p1 - p0, p2 - p0)) * 0.5f;
+        }
+        area[m] = a;
+    }
+}
+
+// Renders with the `path` integrator. The BVH node arrays are the oracle's own
+// (ref_lbvh_build + ref_treelet_optimize). Outputs per-pixel sum (rgb) and
+// weight of the box-filtered film (Core/Film.h:25-49), accumulated in a fixed
+// order: for each pixel, samples of pixels (y-1..y+1, x-1..x+1) in row-major
+// pixel order, then sample order.
+extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh,
+                               const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                               const int32_t* left, const int32_t* right, const int32_t* prim,
+                               const float* cam16, float cam_depth, int width, int height,
+                               int spp, int max_depth, float* film_rgb, float* film_w,
+                               int64_t* rays_traced) {
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    for (int m = 0; m < nmeshes; ++m)
+        if (meshes[m].is_light) s.lights.push_back(m);
+    s.area.resize(nmeshes);
+    ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
+    const Halton& H = halton();
+    const int64_t nsamp = (int64_t)width * height * spp;
+    std::vector<float> res((size_t)nsamp * 3), sx((size_t)nsamp), sy((size_t)nsamp);
+    int64_t traced = 0;
+    const int nl = (int)s.lights.size();
+
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : traced)
+    for (int64_t si = 0; si < nsamp; ++si) {
+        const int64_t pix = si / spp;
+        const int x = (int)(pix % width), y = (int)(pix / width);
+        const uint64_t hidx = (uint64_t)pix * spp + (uint64_t)(si % spp);  // Core/Sampler.cu:29-51
+        int dim = 0;
+        float fx = (float)x, fy = (float)y;
+        fx += H.rand1(hidx, dim);
+        fy += H.rand1(hidx, dim);
+        sx[si] = fx;
+        sy[si] = fy;
+        // PerspectiveCamera::genRay (Cameras/PerspectiveCamera.h:32-48)
+        f3 pl = mk(fx - width / 2.f, height / 2.f - fy, cam_depth);
+        f3 ro = apply(cam16, mk(0, 0, 0));
+        f3 rd = normalize(apply(cam16, pl) - ro);
+        f3 result = mk(0, 0, 0), mult = mk(1, 1, 1);
+        float mat_pdf = 1;
+        bool from_delta = true;
+        for (int depth = 0; depth < max_depth; ++depth) {
+            float t;
+            ++traced;
+            int p = closest(s, ro, rd, t);
+            if (p < 0) break;
+            const int mesh = tri_mesh[p];
+            const cav_mesh_ref& M = meshes[mesh];
+            f3 n = tri_normal(s, p);
+            f3 pos = ro + t * rd;
+            Bsdf bsdf = make_bsdf(M);
+            f3 t0, t1;
+            coordinate_system(n, t0, t1);
+            // computeLighting (PathTracingIntegrator.cu:174-225)
+            if (M.is_light) {
+                f3 Le = dot(rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
+                if (from_delta) {
+                    result = result + mulv(Le, mult);
+                } else if (depth > 0) {
+                    f3 l2s = ro - pos;  // TriangleMesh::pdf (Core/TriangleMesh.h:213-225)
+                    float c = std::fabs(dot(n, normalize(l2s)));
+                    float lpdf = dot(l2s, l2s) / (c * s.area[mesh]);
+                    float w = power_heuristic(mat_pdf, lpdf);
+                    if (std::isfinite(w)) result = result + mulv(Le, mult) * w;
+                }
+            }
+            f3 wo = rd * -1;
+            int li = H.rand_int(nl, hidx, dim);
+            dim += 4;  // rand4, unused by the reference
+            {
+                // DiffuseAreaLight::sampleRayToPoint + TriangleMesh::sample
+                const int lm = s.lights[li];
+                const cav_mesh_ref& L = meshes[lm];
+                int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
+                float tmp = std::sqrt(H.rand1(hidx, dim));
+                float u = 1 - tmp;
+                float v = tmp * H.rand1(hidx, dim);
+                f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+                f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
+                f3 ln = tri_normal(s, tri);
+                f3 l2s = pos - lp;
+                float c = std::fabs(dot(ln, normalize(l2s)));
+                float prob = dot(l2s, l2s) / (c * s.area[lm]);
+                f3 dir = normalize(lp - pos);
+                float lim = length(lp - pos);
+                f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+                // a shadow ray only matters (and is only counted) when prob is finite
+                if (std::isfinite(prob) && (++traced, visible(s, pos, dir, lim, mesh, lm))) {
+                    f3 em = mult * (float)nl / prob;
+                    f3 il = to_local(dir, t0, t1, n), ol = to_local(wo, t0, t1, n);
+                    float mp = bsdf.pdf(il, ol);
+                    float w = power_heuristic(prob, mp);
+                    float cosv = std::fabs(dot(dir, n));  // Material::eval (Core/Material.h:20-26)
+                    f3 contrib = mulv(incident, bsdf.eval(il, ol)) * cosv;
+                    result = result + mulv(contrib, em) * w;
+                }
+            }
+            // genNextRay (PathTracingIntegrator.cu:124-171)
+            if (depth > 3) {
+                float q = 1;
+                q = fminr(q, 1 - mult.x);
+                q = fminr(q, 1 - mult.y);
+                q = fminr(q, 1 - mult.z);
+                q = fmaxr(q, 0.05f);
+                if (H.rand1(hidx, dim) < q) break;
+                mult = divs(mult, 1.f - q);
+            }
+            float ux = H.rand1(hidx, dim);
+            float uy = H.rand1(hidx, dim);
+            f3 wl;
+            float npdf = 0;
+            f3 f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, n), npdf);
+            rd = to_world(wl, t0, t1, n);
+            ro = pos + rd * 0.001f;
+            if (all_zero(f)) mult = mk(0, 0, 0);
+            else mult = divs(mulv(mult, f) * std::fabs(dot(rd, n)), npdf);
+            mat_pdf = npdf;
+            from_delta = bsdf.is_delta();
+        }
+        res[si * 3 + 0] = result.x;
+        res[si * 3 + 1] = result.y;
+        res[si * 3 + 2] = result.z;
+    }
+
+    // box-filtered film (Core/Film.h:25-49, Filters/BoxFilter.h:19-25), gathered
+#pragma omp parallel for schedule(static)
+    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+        const int x = (int)(pix % width), y = (int)(pix / width);
+        float r = 0, g = 0, b = 0, w = 0;
+        for (int py = y - 1; py <= y + 1; ++py)
+            for (int px = x - 1; px <= x + 1; ++px) {
+                if (px < 0 || py < 0 || px >= width || py >= height) continue;
+                for (int k = 0; k < spp; ++k) {
+                    const int64_t si = ((int64_t)py * width + px) * spp + k;
+                    const float* c = &res[si * 3];
+                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
+                    const float fx = sx[si], fy = sy[si];
+                    if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
+                    if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
+                    if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
+                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
+                }
+            }
+        film_rgb[pix * 3 + 0] = r;
+        film_rgb[pix * 3 + 1] = g;
+        film_rgb[pix * 3 + 2] = b;
+        film_w[pix] = w;
+    }
+    if (rays_traced) *rays_traced = traced;
+    return 0;
+}
