@@ -81,6 +81,51 @@ def cpu_baseline(n: int, nrays: int, rounds: int, seed: int) -> dict:
     }
 
 
+def render_bench(renders: int, cpu: bool) -> dict:
+    """Path tracing on the Cornell-box workload: 512x512, 16 spp, max depth 8,
+    no treelet rounds. Mrays/s = (closest + shadow rays) / device time."""
+    import numpy as np
+    from cavalry_amd import Context, render as R, scenes
+    from cavalry_amd.camera import for_scene
+    sc = scenes.cornell_box(seed=1, width=512, height=512, max_depth=8)
+    ctx = Context(0)
+    scene = R.Scene(ctx, sc, bvh_rounds=0)
+    cam = for_scene(sc)
+    R.render(ctx, scene, 16, camera=cam)  # warm-up (allocates)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(renders):
+        _, st = R.render(ctx, scene, 16, camera=cam)
+        stats.append(st)
+    wall = (time.perf_counter() - t0) / renders
+    ms = float(np.mean([st["ms_total"] for st in stats]))
+    rays = stats[-1]["closest_rays"] + stats[-1]["shadow_rays"]
+    out = {
+        "workload": "Cornell box (4014 triangles), 512x512, 16 spp, path, max depth 8, Halton",
+        "mrays_s": round(rays / (ms * 1e-3) / 1e6, 2),
+        "ms_per_render": round(ms, 3),
+        "wall_ms_per_render": round(wall * 1e3, 3),
+        "rays_per_render": int(rays),
+        "renders": renders,
+    }
+    scene.close()
+    ctx.close()
+    if cpu:
+        import oracle
+        small = scenes.cornell_box(seed=1, width=128, height=128, max_depth=8)
+        cam_s = for_scene(small)
+        t0 = time.perf_counter()
+        _, _, rays_c = oracle.render_path(small, 16, cam_s)
+        dt = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {
+            "value": round(rays_c / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"same scene at 128x128, 16 spp (incl. BVH build), {dt:.1f} s on {threads} threads",
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +138,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=1 << 20)
     ap.add_argument("--cpu-rays", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--renders", type=int, default=3)
+    ap.add_argument("--no-render", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -235,6 +282,9 @@ def main() -> int:
     bvh.close()
     ctx.close()
     del tris, rays, srays, lim, hit_prim, hit_t, vis
+    if not args.no_render:
+        log("[bench] path tracing (Cornell box 512x512x16)")
+        out["path_tracing"] = render_bench(args.renders, not args.no_cpu)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)

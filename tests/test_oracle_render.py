@@ -1,0 +1,80 @@
+"""Pins for the CPU oracle's `path` integrator (no GPU).
+
+* Halton sampler: the digit-permutation table and radical inverses equal an
+  independent construction from the published Microsoft C runtime rand()
+  recurrence (tests/golden/halton_fixture.json);
+* furnace: inside a closed box whose every face is a diffuse area light of
+  radiance Le and albedo rho, the radiance is Le / (1 - rho) everywhere; the
+  estimator (NEE + MIS + Russian roulette) must agree to within its noise;
+* film: weights count exactly the samples whose box-filter support covers
+  each pixel (4 per sample away from the edges).
+"""
+import json
+import os
+
+import numpy as np
+
+import oracle
+from cavalry_amd import scenes
+from cavalry_amd.camera import look_at
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_halton_table_and_radical_inverse():
+    with open(os.path.join(GOLDEN, "halton_fixture.json")) as f:
+        fx = json.load(f)
+    primes, start, perm = oracle.halton_table()
+    assert primes[-1] == fx["prime_1023"] and int(primes.sum()) == fx["prime_sum"]
+    for dim, p in enumerate(primes[:6]):
+        head = fx["perm_head"][str(int(p))]
+        np.testing.assert_array_equal(perm[start[dim]:start[dim] + p], head)
+    for c in fx["radical_inverse"]:
+        assert oracle.halton(c["index"], c["dim"]) == np.float32(c["value"]), c
+
+
+def furnace_scene(rho=0.5, le=1.0, n=24):
+    """Unit cube, six light faces with inward normals."""
+    b = scenes._Builder()
+    faces = [
+        scenes.quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)),   # y=0, +y
+        scenes.quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),   # y=1, -y
+        scenes.quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)),   # z=0, +z
+        scenes.quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)),   # z=1, -z
+        scenes.quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)),   # x=0, +x
+        scenes.quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)),   # x=1, -x
+    ]
+    for f in faces:
+        b.mesh(f, kd=(rho, rho, rho), emission=(le, le, le))
+    tris, meshes = b.arrays()
+    return scenes.SceneArrays(tris, meshes, (0.5, 0.5, 0.1), (0.5, 0.5, 0.9), (0, 1, 0), 60.0,
+                              n, n, max_depth=24)
+
+
+def test_face_normals_point_inward():
+    sc = furnace_scene()
+    t = sc.tris.reshape(-1, 3, 3)
+    nrm = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    to_center = 0.5 - t.mean(axis=1)
+    assert (np.einsum("ij,ij->i", nrm, to_center) > 0).all()
+
+
+def test_furnace_radiance_is_le_over_one_minus_rho():
+    for rho in (0.3, 0.6):
+        sc = furnace_scene(rho=rho)
+        cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+        rgb, w, _ = oracle.render_path(sc, 32, cam)
+        img = rgb / w[..., None]
+        expect = 1.0 / (1.0 - rho)
+        mean = float(img.mean())
+        assert abs(mean - expect) < 0.02 * expect, (rho, mean, expect)
+
+
+def test_film_weights_count_box_filter_support():
+    sc = scenes.cornell_box(width=16, height=12)
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    spp = 3
+    _, w, _ = oracle.render_path(sc, spp, cam)
+    # interior pixels: samples of pixels (x-1..x, y-1..y) always cover (x, y)
+    assert (w[1:, 1:] >= 4 * spp).all() and (w[1:, 1:] <= 9 * spp).all()
+    assert w[0, 0] >= spp
