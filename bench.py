@@ -81,47 +81,64 @@ def cpu_baseline(n: int, nrays: int, rounds: int, seed: int) -> dict:
     }
 
 
-def render_bench(renders: int, cpu: bool) -> dict:
-    """Path tracing on the Cornell-box workload: 512x512, 16 spp, max depth 8,
-    no treelet rounds. Mrays/s = (closest + shadow rays) / device time."""
+def _time_renders(ctx, scene, cam, integrator: str, renders: int) -> dict:
     import numpy as np
-    from cavalry_amd import Context, render as R, scenes
-    from cavalry_amd.camera import for_scene
-    sc = scenes.cornell_box(seed=1, width=512, height=512, max_depth=8)
-    ctx = Context(0)
-    scene = R.Scene(ctx, sc, bvh_rounds=0)
-    cam = for_scene(sc)
-    R.render(ctx, scene, 16, camera=cam)  # warm-up (allocates)
+    from cavalry_amd import render as R
+    R.render(ctx, scene, 16, integrator=integrator, camera=cam)  # warm-up (allocates)
     stats = []
     t0 = time.perf_counter()
     for _ in range(renders):
-        _, st = R.render(ctx, scene, 16, camera=cam)
+        _, st = R.render(ctx, scene, 16, integrator=integrator, camera=cam)
         stats.append(st)
     wall = (time.perf_counter() - t0) / renders
     ms = float(np.mean([st["ms_total"] for st in stats]))
     rays = stats[-1]["closest_rays"] + stats[-1]["shadow_rays"]
-    out = {
-        "workload": "Cornell box (4014 triangles), 512x512, 16 spp, path, max depth 8, Halton",
+    return {
         "mrays_s": round(rays / (ms * 1e-3) / 1e6, 2),
         "ms_per_render": round(ms, 3),
         "wall_ms_per_render": round(wall * 1e3, 3),
         "rays_per_render": int(rays),
         "renders": renders,
     }
+
+
+def render_bench(renders: int, cpu: bool) -> dict:
+    """Path tracing on the Cornell-box workload: 512x512, 16 spp, max depth 8,
+    no treelet rounds, with the `path` and `rlpath` integrators.
+    Mrays/s = (closest + shadow rays) / device time."""
+    from cavalry_amd import Context, render as R, scenes
+    from cavalry_amd.camera import for_scene
+    sc = scenes.cornell_box(seed=1, width=512, height=512, max_depth=8)
+    ctx = Context(0)
+    scene = R.Scene(ctx, sc, bvh_rounds=0)
+    cam = for_scene(sc)
+    out = {"workload": "Cornell box (4014 triangles), 512x512, 16 spp, path, max depth 8, Halton"}
+    out.update(_time_renders(ctx, scene, cam, "path", renders))
+    rl = {"workload": "same scene and sampling, rlpath (32^3 Q entries x 128 bins, reference learning rate)"}
+    rl.update(_time_renders(ctx, scene, cam, "rlpath", renders))
+    out["rlpath"] = rl
     scene.close()
     ctx.close()
     if cpu:
         import oracle
-        small = scenes.cornell_box(seed=1, width=128, height=128, max_depth=8)
-        cam_s = for_scene(small)
-        t0 = time.perf_counter()
-        _, _, rays_c = oracle.render_path(small, 16, cam_s)
-        dt = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        _, _, rays_c = oracle.render_path(sc, 16, cam)
+        dt = time.perf_counter() - t0
         out["cpu_baseline"] = {
             "value": round(rays_c / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"same scene at 128x128, 16 spp (incl. BVH build), {dt:.1f} s on {threads} threads",
+            "sample": f"the same 512x512x16 render (incl. BVH build), {dt:.1f} s on {threads} threads",
+        }
+        small = scenes.cornell_box(seed=1, width=128, height=128, max_depth=8)
+        cam_s = for_scene(small)
+        t0 = time.perf_counter()
+        _, _, rays_r = oracle.render_rlpath(small, 16, cam_s)
+        dt = time.perf_counter() - t0
+        rl["cpu_baseline"] = {
+            "value": round(rays_r / dt / 1e6, 3), "unit": "Mrays/s", "cores": 1,
+            "kind": "port",
+            "sample": f"same scene at 128x128, 16 spp, rlpath (incl. BVH build), {dt:.1f} s, single thread",
         }
     return out
 

@@ -41,7 +41,8 @@ class Camera(ctypes.Structure):
 
 class RenderParams(ctypes.Structure):
     _fields_ = [("integrator", ctypes.c_int32), ("spp", ctypes.c_int32),
-                ("max_depth", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("max_depth", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("q_grid", ctypes.c_int32), ("alpha_scale", ctypes.c_float)]
 
 
 class RenderStats(ctypes.Structure):

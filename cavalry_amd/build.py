@@ -24,6 +24,7 @@ SOURCES = [
     "treelet.hip",
     "traverse.hip",
     "render.hip",
+    "rlpath.hip",
     "render_capi.cpp",
     "capi.cpp",
     "camera.cpp",

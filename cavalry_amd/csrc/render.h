@@ -72,6 +72,44 @@ struct RenderArgs {
     float* film_w;             // [w*h]
 };
 
+// extra per-task state of the rlpath ray queues
+struct RlQueue {
+    int32_t* prev_entry;  // Q entry of the previous hit (-1: camera ray)
+    int32_t* prev_cell;   // direction bin taken there
+    int32_t* dim;         // next Halton dimension
+};
+
+// rlpath per-slot results, carried from one kernel of a bounce to the next
+struct RlSlots {
+    int32_t* entry;   // Q entry of the hit
+    int32_t* light;   // chosen light
+    float* sel;       // its selection probability
+    float* lpdf;      // light sample pdf (solid angle)
+    float* emis;      // luminance of MIS-weighted emission found by this ray
+    float* ldir;      // [cap][3] direction to the light sample
+    float* inc;       // [cap][3] unoccluded incident radiance
+    int32_t* sh_idx;  // slot of the shadow task
+    float* ndir;      // [cap][3] Q-sampled next direction
+    float* npdf;
+    int32_t* ncell;   // Q-sampled bin, -1 when the distribution was empty
+};
+
+struct RlArgs {
+    int grid;           // Q entries per axis
+    float alpha_scale;  // 1: the reference's learning rate n/(T+n)
+    float bounds[6];    // scene bounds (min xyz, max xyz)
+    RlQueue q[2];
+    RlSlots slots;
+    float *Q, *newQ, *ldist, *lres, *lnew;
+    int32_t *cnt, *total, *lcount, *lnewc;
+    int32_t* dims;  // [0] synced dimension at the bounce start, [1] any Q sample
+};
+
 hipError_t render_path(const RenderArgs& a, hipStream_t s);
+hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s);
+// camera rays for every sample and the gathered box-filtered film, shared by
+// both integrators
+void render_raygen(const RenderArgs& a, hipStream_t s);
+void render_film(const RenderArgs& a, hipStream_t s);
 
 }  // namespace cav

@@ -31,6 +31,7 @@ struct cav_scene {
     float* uv = nullptr;
     std::vector<cav_mesh> host_meshes;
     std::vector<float> host_area;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};  // Scene::sceneBounds (Core/Scene.cu:131-136)
 };
 
 namespace {
@@ -46,13 +47,24 @@ struct RenderState {
     std::vector<void*> bufs;
     RenderArgs a{};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // rlpath: per-slot state (sized like the sample buffers) and the tables
+    std::vector<void*> rl_bufs, rl_tab;
+    int64_t rl_cap = 0, rl_cells = 0, rl_lcells = 0;
+    RlArgs rl{};
 };
 
+void free_list(std::vector<void*>& v) {
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
 void free_bufs(RenderState* rs) {
-    for (void* p : rs->bufs) (void)hipFree(p);
-    rs->bufs.clear();
+    free_list(rs->bufs);
     rs->cap = 0;
     rs->pix_cap = 0;
+    free_list(rs->rl_bufs);
+    free_list(rs->rl_tab);
+    rs->rl_cap = rs->rl_cells = rs->rl_lcells = 0;
 }
 
 // Halton digit permutations (Samplers/HaltonSampler.cu:7-57): the first 1024
@@ -108,12 +120,70 @@ int ensure_state(cav_ctx* ctx, RenderState*& rs) {
 }
 
 template <typename T>
-int grab(cav_ctx* ctx, RenderState* rs, T*& p, size_t bytes, const char* what) {
+int grab_into(cav_ctx* ctx, std::vector<void*>& list, T*& p, size_t bytes, const char* what) {
     void* q = nullptr;
     const int rc = dev_alloc(ctx, &q, bytes, what);
     if (rc != CAV_OK) return rc;
-    rs->bufs.push_back(q);
+    list.push_back(q);
     p = static_cast<T*>(q);
+    return CAV_OK;
+}
+template <typename T>
+int grab(cav_ctx* ctx, RenderState* rs, T*& p, size_t bytes, const char* what) {
+    return grab_into(ctx, rs->bufs, p, bytes, what);
+}
+
+// rlpath buffers: per-slot state for nsamp tasks, a Q table of `cells` bins
+// and light statistics of `lcells` (entry, light) pairs.
+int ensure_rl(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t cells, int64_t lcells) {
+    RlArgs& r = rs->rl;
+    if (nsamp > rs->rl_cap) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_list(rs->rl_bufs);
+        rs->rl_cap = 0;
+        const size_t S = (size_t)nsamp;
+        auto& L = rs->rl_bufs;
+        int rc = CAV_OK;
+        for (int k = 0; k < 2 && rc == CAV_OK; ++k) {
+            rc = grab_into(ctx, L, r.q[k].prev_entry, 4 * S, "rl previous entry");
+            if (rc == CAV_OK) rc = grab_into(ctx, L, r.q[k].prev_cell, 4 * S, "rl previous bin");
+            if (rc == CAV_OK) rc = grab_into(ctx, L, r.q[k].dim, 4 * S, "rl dimension");
+        }
+        RlSlots& sl = r.slots;
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.entry, 4 * S, "rl entry");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.light, 4 * S, "rl light");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.sel, 4 * S, "rl light selection pdf");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.lpdf, 4 * S, "rl light pdf");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.emis, 4 * S, "rl emission");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.ldir, 12 * S, "rl light direction");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.inc, 12 * S, "rl incident");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.sh_idx, 4 * S, "rl shadow slot");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.ndir, 12 * S, "rl next direction");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.npdf, 4 * S, "rl next pdf");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, sl.ncell, 4 * S, "rl next bin");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.dims, 64, "rl dimensions");
+        if (rc != CAV_OK) { free_list(L); return rc; }
+        rs->rl_cap = nsamp;
+    }
+    if (cells > rs->rl_cells || lcells > rs->rl_lcells) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_list(rs->rl_tab);
+        rs->rl_cells = rs->rl_lcells = 0;
+        auto& L = rs->rl_tab;
+        const size_t C = (size_t)cells, LC = (size_t)lcells;
+        int rc = grab_into(ctx, L, r.Q, 4 * C, "Q table");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.newQ, 4 * C, "Q proposals");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.cnt, 4 * C, "Q proposal counts");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.total, 4 * C, "Q total counts");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.ldist, 4 * LC, "light CDFs");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.lres, 4 * LC, "light results");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.lnew, 4 * LC, "light new results");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.lcount, 4 * LC, "light counts");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.lnewc, 4 * LC, "light new counts");
+        if (rc != CAV_OK) { free_list(L); return rc; }
+        rs->rl_cells = cells;
+        rs->rl_lcells = lcells;
+    }
     return CAV_OK;
 }
 
@@ -195,6 +265,18 @@ int cav_scene_create(cav_ctx* ctx, const float* tris, int n, const cav_mesh* mes
     if (nlights == 0) return fail(ctx, CAV_ERR_INVALID, "scene has no area light");
     RCHK(ctx, hipSetDevice(ctx->device));
     cav_scene* sc = new cav_scene();
+    // union of the triangle boxes, min/max as IEEE minimumNumber/maximumNumber
+    for (int k = 0; k < 3; ++k) {
+        sc->bounds[k] = tris[k];
+        sc->bounds[3 + k] = tris[k];
+    }
+    for (int64_t t = 0; t < n; ++t)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                const float v = tris[t * 9 + j * 3 + k];
+                if (v < sc->bounds[k] || (v == sc->bounds[k] && std::signbit(v))) sc->bounds[k] = v;
+                if (v > sc->bounds[3 + k] || (v == sc->bounds[3 + k] && !std::signbit(v))) sc->bounds[3 + k] = v;
+            }
     sc->ctx = ctx;
     sc->ntris = n;
     sc->nmeshes = nmeshes;
@@ -254,21 +336,32 @@ const cav_bvh* cav_scene_bvh(const cav_scene* sc) { return sc ? sc->bvh : nullpt
 int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_render_params* prm,
                float* film_rgb, float* film_weight, cav_render_stats* stats) {
     if (!ctx || !sc || !cam || !prm || !film_rgb || !film_weight) return CAV_ERR_INVALID;
-    if (prm->integrator != CAV_INTEGRATOR_PATH)
-        return fail(ctx, CAV_ERR_INVALID, "integrator not supported by this build");
+    const bool rl = prm->integrator == CAV_INTEGRATOR_RLPATH;
+    if (prm->integrator != CAV_INTEGRATOR_PATH && !rl)
+        return fail(ctx, CAV_ERR_INVALID, "unknown integrator " + std::to_string(prm->integrator));
+    const int grid = prm->q_grid == 0 ? 32 : prm->q_grid;
+    const float alpha_scale = prm->alpha_scale == 0.f ? 1.f : prm->alpha_scale;
+    if (rl && (grid < 1 || grid > 64))
+        return fail(ctx, CAV_ERR_INVALID, "q_grid must be in [1, 64]");
+    if (rl && !(alpha_scale > 0.f && alpha_scale <= 1.f))
+        return fail(ctx, CAV_ERR_INVALID, "alpha_scale must be in (0, 1]");
     if (prm->spp < 1 || prm->max_depth < 1 || cam->width < 1 || cam->height < 1)
         return fail(ctx, CAV_ERR_INVALID, "spp, max_depth and resolution must be positive");
     const int64_t npix = (int64_t)cam->width * cam->height;
     const int64_t nsamp = npix * prm->spp;
     if (nsamp >= (int64_t)1 << 32) return fail(ctx, CAV_ERR_INVALID, "more than 2^32 samples in one render");
-    // dimensions used: 2 + 10 per bounce + 1 per bounce after the fourth
-    const int64_t dims = 2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0);
+    // dimensions used: path 2 + 10 per bounce + 1 per bounce after the fourth;
+    // rlpath at most 2 + 14 per bounce
+    const int64_t dims = rl ? 2 + 14LL * prm->max_depth
+                            : 2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0);
     if (dims > 1024) return fail(ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
     RCHK(ctx, hipSetDevice(ctx->device));
     RenderState* rs = nullptr;
     int rc = ensure_state(ctx, rs);
     if (rc != CAV_OK) return rc;
     if ((rc = ensure_buffers(ctx, rs, nsamp, npix)) != CAV_OK) return rc;
+    const int64_t nent = (int64_t)grid * grid * grid;
+    if (rl && (rc = ensure_rl(ctx, rs, nsamp, nent * 128, nent * sc->nlights)) != CAV_OK) return rc;
     RenderArgs a = rs->a;
     a.scene = SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
                        sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
@@ -286,7 +379,15 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     RCHK(ctx, hipMemsetAsync(a.rays, 0, 16, s));
     RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));
     RCHK(ctx, hipEventRecord(rs->ev[0], s));
-    RCHK(ctx, render_path(a, s));
+    if (rl) {
+        RlArgs r = rs->rl;
+        r.grid = grid;
+        r.alpha_scale = alpha_scale;
+        std::memcpy(r.bounds, sc->bounds, sizeof r.bounds);
+        RCHK(ctx, render_rlpath(a, r, s));
+    } else {
+        RCHK(ctx, render_path(a, s));
+    }
     RCHK(ctx, hipEventRecord(rs->ev[1], s));
     unsigned long long tally[2] = {0, 0};
     uint32_t tc[2] = {0, 0};

@@ -1,0 +1,701 @@
+// Wavefront `rlpath` integrator: path tracing guided by a Q-table of
+// incident radiance (expected-SARSA, Dahm & Keller) and a learnt per-cell
+// light-selection distribution.
+//
+// Reference: Integrators/ReinforcementLearningPathTracing.cu (QEntry :10-148,
+// findQEntry :155-165, intersectScene :276-302, genNextRay :305-375,
+// computeLighting :388-460, finishLightingTraining :463-514,
+// computeQDistributions :517-606, accumulateLightingContribs :609-647,
+// syncDimension :678-712, Q-table kernels :715-767, render :802-950) and
+// Distribution1D (Utils/RandomUtils.h:120-183).
+//
+// Learning couples every sample of a bounce: the Q proposals and light
+// statistics of all tasks are folded into the tables before any task picks
+// its next direction. One bounce is
+//   closest hits -> k_dim_max (the reference's first syncDimension) ->
+//   k_lighting (emission with MIS, learnt light pick, shadow task) ->
+//   any-hit traversal -> k_qdist (NEE contribution, 128-bin Q distribution,
+//   Q and light proposals, next-bin sample, MIS with the Q pdf) ->
+//   k_finish_q (commit + average per entry) -> k_finish_light (light CDFs) ->
+//   k_next (second syncDimension, Russian roulette, next ray).
+// Halton dimensions follow the reference exactly, including the batch-wide
+// maximum taken by syncDimension, so the device keeps the maximum in a word
+// instead of the host reading it back. Tasks keep their queue slot through a
+// bounce, so no lighting queue is built: per-slot arrays carry the results
+// from one kernel to the next.
+//
+// Proposals are float atomics, as in the reference; the order in which they
+// land varies between runs, so films agree with the CPU restatement exactly
+// only at max depth 1 and statistically beyond.
+#include <cfloat>
+
+#include "../../include/cavalry.h"
+#include "dev_common.h"
+#include "render.h"
+#include "shade.h"
+
+namespace cav {
+namespace {
+
+using namespace shade;
+
+constexpr int kNX = 16, kNY = 8, kXY = kNX * kNY;  // QEntry::NUM_X / NUM_Y / NUM_XY
+constexpr int kEntryWaves = 4;                      // entries per k_finish_q block
+
+// float -> int as the reference's conversion behaves: NaN gives 0, out of
+// range saturates.
+__device__ inline int f2i(float v) {
+    if (isnan(v)) return 0;
+    if (v >= 2147483648.f) return 2147483647;
+    if (v <= -2147483648.f) return (-2147483647 - 1);
+    return (int)v;
+}
+__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// Correctly rounded single-precision acos / cos / sin via binary64.
+__device__ inline float acos_f(float x) { return (float)acos((double)x); }
+__device__ inline float cos_f(float x) { return (float)cos((double)x); }
+__device__ inline float sin_f(float x) { return (float)sin((double)x); }
+
+__device__ inline float luminance(v3 c) {
+    return (c.x * 0.212671f + c.y * 0.715160f) + c.z * 0.072169f;
+}
+
+// QEntry::dirToCellIndex (:113-128)
+__device__ inline int dir_to_cell(v3 d) {
+    const float u = d.z;
+    const float y = (u + 1.f) / 2.f;
+    const int ti = clampi(f2i(y * kNY), 0, kNY - 1);
+    d.x = d.x / sqrtf(1.f - u * u);
+    d.y = d.y / sqrtf(1.f - u * u);
+    float v = acos_f(d.x);
+    if (d.y < 0) v = (float)((2.0 * kPi) - (double)v);
+    v = (float)((double)v / (2.0 * kPi));
+    const int pi = clampi(f2i(v * kNX), 0, kNX - 1);
+    return ti * kNX + pi;
+}
+
+// findQEntry (:155-165)
+struct Grid {
+    float lo[3], size[3];
+    int g;
+};
+__device__ inline int q_entry(const Grid& gr, v3 p) {
+    const float gf = (float)gr.g;
+    const float ex = gr.size[0] / gf, ey = gr.size[1] / gf, ez = gr.size[2] / gf;
+    const int x = clampi(f2i((p.x - gr.lo[0]) / ex), 0, gr.g - 1);
+    const int y = clampi(f2i((p.y - gr.lo[1]) / ey), 0, gr.g - 1);
+    const int z = clampi(f2i((p.z - gr.lo[2]) / ez), 0, gr.g - 1);
+    return (x * gr.g + y) * gr.g + z;
+}
+
+// Distribution1D::getCorrespondingIndex (Utils/RandomUtils.h:130-150)
+__device__ inline int dist_index(const float* cdf, int n, float f) {
+    int l = 0, r = n - 1;
+    while (l < r) {
+        const int m = (l + r) / 2;
+        if (cdf[m] >= f) {
+            if (m == 0) return 0;
+            if (cdf[m - 1] < f) return m;
+            r = m - 1;
+        } else {
+            l = m + 1;
+        }
+    }
+    return l;
+}
+
+// -------------------------------------------------------------- state
+struct RlTables {
+    float* Q;        // [E][128]
+    float* newQ;     // [E][128]
+    int32_t* cnt;    // [E][128] proposals this bounce
+    int32_t* total;  // [E][128]
+    float* ldist;    // [E][L] light-selection CDF
+    float* lres;     // [E][L]
+    float* lnew;     // [E][L]
+    int32_t* lcount; // [E][L]
+    int32_t* lnewc;  // [E][L]
+};
+
+__global__ __launch_bounds__(kBlock) void k_init_tables(RlTables t, int64_t ncells, int64_t nent, int L) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < ncells) {  // QEntry() (:36-41) and prepareForUpdateQ
+        t.Q[i] = 1.f;
+        t.newQ[i] = 0.f;
+        t.cnt[i] = 0;
+        t.total[i] = 0;
+    }
+    if (i < nent * L) {  // initialiseLightSamplingDist (:724-735)
+        const int k = (int)(i % L);
+        t.ldist[i] = ((float)k + 1.f) / (float)L;
+        t.lres[i] = 0.f;
+        t.lnew[i] = 0.f;
+        t.lcount[i] = 0;
+        t.lnewc[i] = 0;
+    }
+}
+
+// genInitialRays' QEntryInfo {-1, -1} and the camera sample's two dimensions
+__global__ __launch_bounds__(kBlock) void k_init_tasks(int64_t nsamp, RlQueue q) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nsamp) return;
+    q.prev_entry[i] = -1;
+    q.prev_cell[i] = -1;
+    q.dim[i] = 2;
+}
+
+__device__ inline int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// syncDimension before computeLighting: maximum over the tasks that hit.
+__global__ __launch_bounds__(kBlock) void k_dim_max(const uint32_t* counts, int qin, RlQueue q,
+                                                    const int32_t* __restrict__ hit_prim, int32_t* dims) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < (int64_t)counts[qin] && hit_prim[i] >= 0;
+    const int m = wave_max(live ? q.dim[i] : 0);
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&dims[0], m);
+}
+
+// computeLighting (:388-460) up to the visibility test: emission with MIS,
+// learnt light choice, point on the light. The shadow task carries the slot.
+__global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, HaltonDev h, Grid gr,
+                                                     Queue qi, RlSlots sl, Shadow sh, RlTables t,
+                                                     const int32_t* __restrict__ hit_prim,
+                                                     const float* __restrict__ hit_t, float* __restrict__ res,
+                                                     uint32_t* counts, const int32_t* dims, int qin) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = i < (int64_t)counts[qin];
+    const int prim = live ? hit_prim[i] : -1;
+    live = live && prim >= 0;
+    v3 pos = mk3(0, 0, 0), dir = pos;
+    float lim = 0.f;
+    int mesh = 0, lm = 0;
+    if (live) {
+        const int D = dims[0];
+        const uint32_t sid = qi.sid[i];
+        const float* r = qi.ray + i * 6;
+        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
+        const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
+        mesh = sc.tri_mesh[prim];
+        const cav_mesh M = sc.meshes[mesh];
+        const v3 nn = tri_normal(sc, prim);
+        pos = add3(ro, scale3(rd, hit_t[i]));
+        float emis = 0.f;
+        if (M.is_light) {
+            const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
+            v3 acc = mk3(res[sid * 3], res[sid * 3 + 1], res[sid * 3 + 2]);
+            if (qi.delta[i]) {
+                acc = add3(acc, mul3(Le, mult));
+            } else if (depth > 0) {
+                const v3 l2s = sub3(ro, pos);
+                const float c = fabsf(dot3(nn, nrm(l2s)));
+                const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[mesh]);
+                const float w = power_heuristic(qi.pdf[i], lpdf);
+                if (isfinite(w)) {
+                    emis = luminance(Le);
+                    acc = add3(acc, scale3(mul3(Le, mult), w));
+                }
+            }
+            res[sid * 3] = acc.x; res[sid * 3 + 1] = acc.y; res[sid * 3 + 2] = acc.z;
+        }
+        const int entry = q_entry(gr, pos);
+        const float* cdf = t.ldist + (int64_t)entry * sc.nlights;
+        const int li = dist_index(cdf, sc.nlights, halton(h, sid, D));
+        const float sel = li == 0 ? cdf[0] : cdf[li] - cdf[li - 1];
+        lm = sc.lights[li];
+        const cav_mesh L = sc.meshes[lm];
+        const int tri = L.first_tri + halton_int(h, sid, D + 5, L.num_tris);
+        const float tmp = sqrtf(halton(h, sid, D + 6));
+        const float u = 1 - tmp;
+        const float v = tmp * halton(h, sid, D + 7);
+        const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
+        const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
+        const v3 ln = tri_normal(sc, tri);
+        const v3 l2s = sub3(pos, lp);
+        const float c = fabsf(dot3(ln, nrm(l2s)));
+        const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
+        dir = nrm(sub3(lp, pos));
+        lim = len3(sub3(lp, pos));
+        const v3 inc = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+        sl.entry[i] = entry;
+        sl.light[i] = li;
+        sl.sel[i] = sel;
+        sl.lpdf[i] = lpdf;
+        sl.emis[i] = emis;
+        sl.ldir[i * 3] = dir.x; sl.ldir[i * 3 + 1] = dir.y; sl.ldir[i * 3 + 2] = dir.z;
+        sl.inc[i * 3] = inc.x; sl.inc[i * 3 + 1] = inc.y; sl.inc[i * 3 + 2] = inc.z;
+    }
+    const uint32_t ss = wave_push(live, &counts[2]);
+    if (live) {
+        float* r = sh.ray + (int64_t)ss * 6;
+        r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
+        sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = lm;
+        sl.sh_idx[i] = (int32_t)ss;
+    }
+}
+
+// Unit direction inside bin `cell` at offset (rx, ry):
+// QEntry::sampleDirectionInCell (:131-147).
+__device__ inline v3 cell_dir(float rx, float ry, int cell) {
+    const int ti = cell / kNX, pi = cell % kNX;
+    float u = ((float)ti + rx) * (1.f / (float)kNY);
+    u = u * 2 - 1.f;
+    const float v = ((float)pi + ry) * (1.f / (float)kNX);
+    const float xy = sqrtf(1.0f - u * u);
+    const float phi = (float)(2 * kPi * (double)v);
+    return mk3(xy * cos_f(phi), xy * sin_f(phi), u);
+}
+
+// Bins of the Q-weighted distribution: qv = |cos| * luminance(f) * Q for
+// the bin directions at offset (px, py). Each thread keeps the cos / sin of
+// its 16 column angles in LDS (`cs`, stride kBlock) so the bin loops need no
+// register arrays.
+struct BinEval {
+    Bsdf bsdf;
+    v3 t0, t1, nn, ol;
+    const float* Qe;
+    const float* cs;
+    float px;
+
+    __device__ inline void row(int ti, float& u, float& xy) const {
+        u = ((float)ti + px) * (1.f / (float)kNY);
+        u = u * 2 - 1.f;
+        xy = sqrtf(1.0f - u * u);
+    }
+    __device__ inline float bin(float u, float xy, int p, int c) const {
+        const v3 d = mk3(xy * cs[p * kBlock], xy * cs[(kNX + p) * kBlock], u);
+        const float scat = luminance(bsdf.eval(to_local(d, t0, t1, nn), ol));
+        return (fabsf(dot3(d, nn)) * scat) * Qe[c];
+    }
+    // prefix[m] and prefix[m - 1] (0 for m = 0), summed bin by bin
+    __device__ inline void prefix(int m, float& pm, float& pm1) const {
+        float s = 0.f;
+        pm1 = 0.f;
+        for (int c = 0; c <= m; ++c) {
+            pm1 = s;
+            float u, xy;
+            row(c / kNX, u, xy);
+            s += bin(u, xy, c % kNX, c);
+        }
+        pm = s;
+    }
+};
+
+// The end of computeLighting (after the visibility test), then
+// computeQDistributions (:517-606) and accumulateLightingContribs (:609-647).
+__global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
+                                                  RlSlots sl, RlTables t, const uint8_t* __restrict__ vis,
+                                                  const int32_t* __restrict__ hit_prim,
+                                                  const float* __restrict__ hit_t, float* __restrict__ res,
+                                                  const uint32_t* counts, int32_t* dims, int qin) {
+    __shared__ float trig[2 * kNX][kBlock];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = i < (int64_t)counts[qin];
+    const int prim = live ? hit_prim[i] : -1;
+    live = live && prim >= 0;
+    bool valid = false;
+    if (live) {
+        const int D = dims[0] + 8;
+        const uint32_t sid = qi.sid[i];
+        const float* r = qi.ray + i * 6;
+        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
+        const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
+        const int mesh = sc.tri_mesh[prim];
+        const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
+        const v3 nn = tri_normal(sc, prim);
+        v3 t0, t1;
+        frame(nn, t0, t1);
+        const v3 ol = to_local(scale3(rd, -1.f), t0, t1, nn);
+        const int entry = sl.entry[i];
+        const int li = sl.light[i];
+        const v3 dir = mk3(sl.ldir[i * 3], sl.ldir[i * 3 + 1], sl.ldir[i * 3 + 2]);
+        v3 inc = mk3(sl.inc[i * 3], sl.inc[i * 3 + 1], sl.inc[i * 3 + 2]);
+        float lpdf = sl.lpdf[i];
+        if (!(vis[sl.sh_idx[i]] && isfinite(lpdf))) {
+            inc = mk3(0, 0, 0);
+            lpdf = 1;
+        }
+        const v3 il = to_local(dir, t0, t1, nn);
+        // Material::eval (Core/Material.h:20-26)
+        v3 contrib = scale3(mul3(inc, bsdf.eval(il, ol)), fabsf(dot3(dir, nn)));
+        contrib = scale3(contrib, 1.f / (lpdf * sl.sel[i]));
+        const float sampled = luminance(inc);
+        atomicAdd(t.lnew + (int64_t)entry * sc.nlights + li, sampled);
+        atomicAdd(t.lnewc + (int64_t)entry * sc.nlights + li, 1);
+
+        // 128-bin distribution of |cos| * luminance(f) * Q at one shared offset.
+        // Bin c = 16 * theta_row + phi_column. The running (unnormalised) sum
+        // is kept only at the end of each row; the row holding the sampled
+        // bin is evaluated again, which reproduces the same partial sums.
+        const float* Qe = t.Q + (int64_t)entry * kXY;
+        const float px = halton(h, sid, D), py = halton(h, sid, D + 1);
+        const int lc = dir_to_cell(dir);
+        float* cs = &trig[0][threadIdx.x];
+        for (int p = 0; p < kNX; ++p) {
+            const float v = ((float)p + py) * (1.f / (float)kNX);
+            const float phi = (float)(2 * kPi * (double)v);
+            cs[p * kBlock] = cos_f(phi);
+            cs[(kNX + p) * kBlock] = sin_f(phi);
+        }
+        const BinEval be{bsdf, t0, t1, nn, ol, Qe, cs, px};
+        float rows[kNY];
+        float sum = 0.f, lc0 = 0.f, lc1 = 0.f;
+        bool mono = true;
+        for (int ti = 0; ti < kNY; ++ti) {
+            float u, xy;
+            be.row(ti, u, xy);
+#pragma unroll 2
+            for (int p = 0; p < kNX; ++p) {
+                const int c = ti * kNX + p;
+                const float q = be.bin(u, xy, p, c);
+                mono = mono && q >= 0.f;
+                sum += q;
+                lc0 = c == lc ? sum : lc0;
+                lc1 = c == lc - 1 ? sum : lc1;
+            }
+#pragma unroll
+            for (int k = 0; k < kNY; ++k) rows[k] = k == ti ? sum : rows[k];
+        }
+        mono = mono && isfinite(sum) && sum > 0.f;
+        const float cell_sa = (float)(4.f * kPi / (double)(float)kXY);
+        const float sumw = sum * cell_sa;
+        const int prev = rq.prev_entry[i];
+        if (prev != -1) {
+            const int64_t k = (int64_t)prev * kXY + rq.prev_cell[i];
+            atomicAdd(t.newQ + k, sumw + sl.emis[i]);
+            atomicAdd(t.cnt + k, 1);
+        }
+        // sample a bin: Distribution1D::sample on cdf[c] = prefix[c] / sum
+        int cell = 0;
+        float b0 = 0.f, b1 = 0.f;  // cdf[cell], cdf[cell - 1]
+        if (sumw != 0) {
+            const float f = halton(h, sid, D + 2);
+            if (mono) {
+                // non-decreasing, NaN-free CDF: the binary search returns the
+                // first bin with cdf >= f (the last bin if none)
+                int r = 0;
+#pragma unroll
+                for (int k = 0; k < kNY - 1; ++k) r += rows[k] / sum < f ? 1 : 0;
+                float pre = 0.f;
+#pragma unroll
+                for (int k = 0; k < kNY - 1; ++k) pre = k == r - 1 ? rows[k] : pre;
+                float u, xy;
+                be.row(r, u, xy);
+                float prevc = r == 0 ? 0.f : pre / sum;
+                int found = kNX - 1;
+                for (int p = 0; p < kNX; ++p) {
+                    pre += be.bin(u, xy, p, r * kNX + p);
+                    const float cm = pre / sum;
+                    if (cm >= f || p == kNX - 1) {
+                        found = p;
+                        b0 = cm;
+                        b1 = prevc;
+                        break;
+                    }
+                    prevc = cm;
+                }
+                cell = r * kNX + found;
+            } else {
+                // the reference's binary search, recomputing the prefix sums
+                int l = 0, rr = kXY - 1;
+                cell = -1;
+                while (l < rr) {
+                    const int m = (l + rr) / 2;
+                    float pm, pm1;
+                    be.prefix(m, pm, pm1);
+                    const float cm = pm / sum, cm1 = pm1 / sum;
+                    if (cm >= f) {
+                        if (m == 0 || cm1 < f) { cell = m; break; }
+                        rr = m - 1;
+                    } else {
+                        l = m + 1;
+                    }
+                }
+                if (cell < 0) cell = l;
+                float pm, pm1;
+                be.prefix(cell, pm, pm1);
+                b0 = pm / sum;
+                b1 = pm1 / sum;
+            }
+        }
+        const float a0 = lc0 / sum, a1 = lc1 / sum;
+        float pdfq = 0.f;
+        if (sumw != 0) pdfq = (lc == 0 ? a0 : a0 - a1) / cell_sa;
+        if (sampled > 0) {  // direct-lighting proposal for this entry (:578-585)
+            const float sa = 1.f / lpdf;
+            const float ratio = fmn(1.f, 0.5f * sa / cell_sa);
+            const float prop = sampled * ratio;
+            if (prop > Qe[lc]) {
+                const int64_t k = (int64_t)entry * kXY + lc;
+                atomicAdd(t.newQ + k, prop);
+                atomicAdd(t.cnt + k, 1);
+            }
+        }
+        if (sumw != 0) {
+            float p = cell == 0 ? b0 : b0 - b1;
+            p = (float)((double)((float)kXY * p) / (4 * kPi));
+            const v3 nd = cell_dir(px, py, cell);
+            sl.ndir[i * 3] = nd.x; sl.ndir[i * 3 + 1] = nd.y; sl.ndir[i * 3 + 2] = nd.z;
+            sl.npdf[i] = p;
+            sl.ncell[i] = cell;
+            valid = true;
+        } else {
+            sl.ncell[i] = -1;
+        }
+        // accumulateLightingContribs
+        const float mp = bsdf.pdf(il, ol);
+        float spdf = 0;
+        if (sumw != 0) spdf = pdfq;
+        if (sumw == 0 || spdf == 0 || bsdf.delta()) spdf = mp;
+        const float w = power_heuristic(lpdf, spdf);
+        const v3 add = scale3(mul3(contrib, mult), w);
+        res[sid * 3] = res[sid * 3] + add.x;
+        res[sid * 3 + 1] = res[sid * 3 + 1] + add.y;
+        res[sid * 3 + 2] = res[sid * 3 + 2] + add.z;
+    }
+    if (__ballot(valid) != 0 && (threadIdx.x & 63) == 0) atomicMax(&dims[1], 1);
+}
+
+// finishUpdateQ (:97-110, 753-762) and averageQ (:52-72, 765-772) for one
+// entry per wave, two bins per lane; also clears newQ / proposalCount for the
+// next bounce (prepareForUpdateQ). Entries without proposals are unchanged by
+// both steps and are skipped.
+__device__ inline float lane_f(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+__global__ __launch_bounds__(64 * kEntryWaves) void k_finish_q(RlTables t, int64_t nent, float alpha_scale) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t e = (int64_t)blockIdx.x * kEntryWaves + wv;
+    if (e >= nent) return;
+    const int64_t base = e * kXY;
+    const int c0 = lane, c1 = lane + 64;
+    const int n0 = t.cnt[base + c0], n1 = t.cnt[base + c1];
+    if (__ballot(n0 != 0 || n1 != 0) == 0) return;
+    float q0 = t.Q[base + c0], q1 = t.Q[base + c1];
+    int tot0 = t.total[base + c0], tot1 = t.total[base + c1];
+    if (n0 != 0) {
+        float a = (float)n0 / (float)(tot0 + n0);
+        a = a * alpha_scale;
+        if (a != 0) {
+            q0 = q0 * (1.f - a) + a * t.newQ[base + c0] / (float)n0;
+            tot0 += n0;
+        }
+        t.newQ[base + c0] = 0.f;
+        t.cnt[base + c0] = 0;
+    }
+    if (n1 != 0) {
+        float a = (float)n1 / (float)(tot1 + n1);
+        a = a * alpha_scale;
+        if (a != 0) {
+            q1 = q1 * (1.f - a) + a * t.newQ[base + c1] / (float)n1;
+            tot1 += n1;
+        }
+        t.newQ[base + c1] = 0.f;
+        t.cnt[base + c1] = 0;
+    }
+    t.total[base + c0] = tot0;
+    t.total[base + c1] = tot1;
+    const uint64_t m0 = __ballot(tot0 > 0), m1 = __ballot(tot1 > 0);
+    const int count = __popcll(m0) + __popcll(m1);
+    if (count > 0) {
+        float s = 0.f;  // in bin order, as averageQ sums (the masks are wave-uniform)
+        for (int k = 0; k < 64; ++k)
+            if ((m0 >> k) & 1) s += lane_f(q0, k);
+        for (int k = 0; k < 64; ++k)
+            if ((m1 >> k) & 1) s += lane_f(q1, k);
+        const float avg = s / (float)count;
+        if (tot0 == 0) q0 = avg;
+        if (tot1 == 0) q1 = avg;
+    }
+    t.Q[base + c0] = q0;
+    t.Q[base + c1] = q1;
+}
+
+// finishLightingTraining (:463-514) per entry, clearing the new statistics
+// (prepareLightingTraining).
+__global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nent, int L) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nent) return;
+    float* res = t.lres + e * L;
+    float* nres = t.lnew + e * L;
+    int32_t* cnt = t.lcount + e * L;
+    int32_t* ncnt = t.lnewc + e * L;
+    float* cdf = t.ldist + e * L;
+    float sum_r = 0, valid = 0;
+    bool any = false;
+    for (int k = 0; k < L; ++k) {
+        const int nc = ncnt[k];
+        if (nc > 0) {
+            const int c = cnt[k];
+            const float tc = (float)(c + nc);
+            const float r = ((float)c / tc) * res[k] + ((float)nc / tc) * (nres[k] / (float)nc);
+            res[k] = r;
+            cnt[k] = c + nc;
+            sum_r += r;
+            valid += 1;
+            ncnt[k] = 0;
+            nres[k] = 0.f;
+            any = true;
+        }
+    }
+    if (!any || !(sum_r > 0)) return;
+    const float avg = sum_r / valid;
+    float dens = 0;
+    for (int k = 0; k < L; ++k) {
+        const float r = res[k];
+        if (r > 0) dens += r;
+        else if (cnt[k] > 0) dens += avg / (float)cnt[k];
+        else dens += avg;
+        cdf[k] = dens;
+    }
+    for (int k = 0; k < L; ++k) cdf[k] = cdf[k] / dens;
+}
+
+// genNextRay (:305-375) after the second syncDimension.
+__global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonDev h, Queue qi, Queue qo,
+                                                 RlQueue rqo, RlSlots sl, const int32_t* __restrict__ hit_prim,
+                                                 const float* __restrict__ hit_t, uint32_t* counts,
+                                                 const int32_t* dims, int qin, int qout) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = i < (int64_t)counts[qin];
+    const int prim = live ? hit_prim[i] : -1;
+    live = live && prim >= 0;
+    bool push = false;
+    uint32_t sid = 0;
+    v3 nro = mk3(0, 0, 0), nrd = nro, mult = nro;
+    float npdf = 0.f;
+    int cell = 0, entry = 0, dim = 0;
+    bool ndelta = false;
+    if (live) {
+        dim = dims[0] + 10 + dims[1];
+        sid = qi.sid[i];
+        const float* r = qi.ray + i * 6;
+        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
+        mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
+        bool alive = true;
+        if (depth > 3) {
+            float q = 1;
+            q = fmn(q, 1 - mult.x);
+            q = fmn(q, 1 - mult.y);
+            q = fmn(q, 1 - mult.z);
+            q = fmx(q, 0.05f);
+            if (halton(h, sid, dim) < q) alive = false;
+            else mult = divs3(mult, 1.f - q);
+            ++dim;
+        }
+        if (alive) {
+            const int mesh = sc.tri_mesh[prim];
+            const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
+            const v3 nn = tri_normal(sc, prim);
+            v3 t0, t1;
+            frame(nn, t0, t1);
+            const v3 pos = add3(ro, scale3(rd, hit_t[i]));
+            const v3 wo = scale3(rd, -1.f);
+            entry = sl.entry[i];
+            const int qc = sl.ncell[i];
+            v3 f;
+            if (bsdf.delta() || qc < 0) {
+                const float ux = halton(h, sid, dim), uy = halton(h, sid, dim + 1);
+                dim += 2;
+                v3 wl;
+                float p = 0.f;
+                f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, nn), p);
+                nrd = to_world(wl, t0, t1, nn);
+                npdf = p;
+                cell = dir_to_cell(nrd);
+            } else {
+                cell = qc;
+                npdf = sl.npdf[i];
+                nrd = mk3(sl.ndir[i * 3], sl.ndir[i * 3 + 1], sl.ndir[i * 3 + 2]);
+                f = bsdf.eval(to_local(nrd, t0, t1, nn), to_local(wo, t0, t1, nn));
+            }
+            nro = add3(pos, scale3(nrd, 0.001f));
+            if (zero3(f)) mult = mk3(0, 0, 0);
+            else mult = divs3(scale3(mul3(mult, f), fabsf(dot3(nrd, nn))), npdf);
+            ndelta = bsdf.delta();
+            push = true;
+        }
+    }
+    const uint32_t ns = wave_push(push, &counts[qout]);
+    if (push) {
+        float* r = qo.ray + (int64_t)ns * 6;
+        r[0] = nro.x; r[1] = nro.y; r[2] = nro.z; r[3] = nrd.x; r[4] = nrd.y; r[5] = nrd.z;
+        qo.sid[ns] = sid;
+        qo.mult[(int64_t)ns * 3] = mult.x;
+        qo.mult[(int64_t)ns * 3 + 1] = mult.y;
+        qo.mult[(int64_t)ns * 3 + 2] = mult.z;
+        qo.pdf[ns] = npdf;
+        qo.delta[ns] = ndelta ? 1 : 0;
+        rqo.prev_entry[ns] = entry;
+        rqo.prev_cell[ns] = cell;
+        rqo.dim[ns] = dim;
+    }
+}
+
+__global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims, int qin) {
+    if (threadIdx.x == 0) {
+        rays[0] += counts[qin];
+        rays[1] += counts[2];
+        counts[qin] = 0;
+        counts[2] = 0;
+        dims[0] = 0;
+        dims[1] = 0;
+    }
+}
+
+}  // namespace
+
+hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s) {
+    const int64_t nsamp = a.nsamp;
+    const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
+    const int L = a.scene.nlights;
+    const int64_t nent = (int64_t)rl.grid * rl.grid * rl.grid;
+    const int64_t ncells = nent * kXY;
+    Grid gr;
+    for (int k = 0; k < 3; ++k) {
+        gr.lo[k] = rl.bounds[k];
+        gr.size[k] = rl.bounds[3 + k] - rl.bounds[k];
+    }
+    gr.g = rl.grid;
+    RlTables t{rl.Q, rl.newQ, rl.cnt, rl.total, rl.ldist, rl.lres, rl.lnew, rl.lcount, rl.lnewc};
+    hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemsetAsync(rl.dims, 0, 16, s)) != hipSuccess) return e;
+    const int64_t ninit = ncells > nent * L ? ncells : nent * L;
+    hipLaunchKernelGGL(k_init_tables, dim3((unsigned)((ninit + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
+                       ncells, nent, L);
+    render_raygen(a, s);
+    hipLaunchKernelGGL(k_init_tasks, dim3(nb), dim3(kBlock), 0, s, nsamp, rl.q[0]);
+    int qin = 0;
+    for (int depth = 0; depth < a.max_depth; ++depth) {
+        const int qout = qin ^ 1;
+        if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[qin].ray,
+                                     a.counts + qin, nsamp, a.hit_prim, a.hit_t, nullptr, nullptr,
+                                     a.trace_counters, s)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_dim_max, dim3(nb), dim3(kBlock), 0, s, a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
+        hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
+                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin);
+        if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
+                                      a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
+                           t, a.vis, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin);
+        hipLaunchKernelGGL(k_finish_q, dim3((unsigned)((nent + kEntryWaves - 1) / kEntryWaves)),
+                           dim3(64 * kEntryWaves), 0, s, t, nent, rl.alpha_scale);
+        hipLaunchKernelGGL(k_finish_light, dim3((unsigned)((nent + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
+                           nent, L);
+        hipLaunchKernelGGL(k_next, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, a.q[qin], a.q[qout],
+                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, a.counts, rl.dims, qin, qout);
+        hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
+        qin = qout;
+    }
+    render_film(a, s);
+    return hipGetLastError();
+}
+
+}  // namespace cav

@@ -1,0 +1,343 @@
+// Device code shared by the wavefront integrators: the Halton sampler, frames,
+// the matte / plastic / mirror / glass BSDFs, triangle helpers and the
+// wave-aggregated queue append.
+//
+// Reference: Samplers/HaltonSampler.h, Utils/MathsCommons.h,
+// Utils/RandomUtils.h, BSDFs/*.h, Materials/*.h, Core/TriangleMesh.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cavalry.h"
+#include "dev_common.h"
+#include "render.h"
+
+namespace cav {
+namespace shade {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- Halton
+__device__ inline float halton(const HaltonDev& h, uint32_t index, int dim) {
+    const uint32_t base = h.primes[dim];
+    const uint16_t* pm = h.perm + h.start[dim];
+    float r = 0.f, f = 1.f;
+    const float fb = (float)base;
+    uint32_t i = index;
+    while (i > 0) {
+        const uint32_t q = i / base;
+        f = f / fb;
+        r = r + f * (float)pm[i - q * base];
+        i = q;
+    }
+    return r;
+}
+__device__ inline int halton_int(const HaltonDev& h, uint32_t index, int dim, int n) {
+    const float f = halton(h, index, dim);
+    int r = (int)(f * (float)n);
+    if (r >= n) r = n - 1;
+    return r;
+}
+
+// ---------------------------------------------------------------- vectors
+__device__ inline v3 nrm(v3 v) {
+    const float inv = 1.0f / sqrtf(dot3(v, v));
+    return scale3(v, inv);
+}
+__device__ inline float len3(v3 v) { return sqrtf(dot3(v, v)); }
+__device__ inline v3 divs3(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ inline bool zero3(v3 c) { return c.x == 0 && c.y == 0 && c.z == 0; }
+__device__ inline float clampf(float x, float lo, float hi) { return fmx(lo, fmn(x, hi)); }
+
+__device__ inline void frame(v3 n, v3& t0, v3& t1) {  // Utils/MathsCommons.h:193-203
+    if (fabsf(n.x) < 0.5) t0 = cross3(mk3(1, 0, 0), n);
+    else t0 = cross3(mk3(0, 1, 0), n);
+    t0 = nrm(t0);
+    t1 = nrm(cross3(n, t0));
+}
+__device__ inline v3 to_local(v3 v, v3 t0, v3 t1, v3 n) {
+    return mk3(dot3(t0, v), dot3(t1, v), dot3(n, v));
+}
+__device__ inline v3 to_world(v3 v, v3 t0, v3 t1, v3 n) {
+    return mk3(t0.x * v.x + t1.x * v.y + n.x * v.z, t0.y * v.x + t1.y * v.y + n.y * v.z,
+               t0.z * v.x + t1.z * v.y + n.z * v.z);
+}
+
+// Utils/MathsCommons.h:109-162; the reference's `sinZenith == 0` guard never
+// fires, so the clamped quotient is always used.
+__device__ inline float c2z(v3 w) { return w.z * w.z; }
+__device__ inline float s2z(v3 w) { return fmx(0.f, 1.f - c2z(w)); }
+__device__ inline float sz(v3 w) { return sqrtf(s2z(w)); }
+__device__ inline float tz(v3 w) { return sz(w) / w.z; }
+__device__ inline float t2z(v3 w) { return s2z(w) / c2z(w); }
+__device__ inline float caz(v3 w) { return clampf(w.x / sz(w), -1.f, 1.f); }
+__device__ inline float saz(v3 w) { return clampf(w.y / sz(w), -1.f, 1.f); }
+__device__ inline bool same_hemi(v3 a, v3 b) { return a.z * b.z > 0; }
+__device__ inline float pow5(float f) { return (f * f) * (f * f) * f; }
+__device__ inline v3 reflect3(v3 i, v3 n) { return sub3(scale3(scale3(n, 2.0f), dot3(n, i)), i); }
+__device__ inline bool refract3(v3 i, v3 n, float eta, v3& out) {
+    const float c = dot3(n, i);
+    const float s2 = fmx(0.f, 1.f - c * c);
+    const float s2t = eta * eta * s2;
+    if (s2t >= 1) return false;
+    const float ct = sqrtf(1 - s2t);
+    out = add3(scale3(scale3(i, -1.f), eta), scale3(n, eta * c - ct));
+    return true;
+}
+__device__ inline bool eq3(v3 a, v3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// Utils/RandomUtils.h:20-41: theta is a double (M_PI), so are cos/sin.
+__device__ inline v3 cos_hemi(float ux, float uy) {
+    const float ox = 2.f * ux - 1.f, oy = 2.f * uy - 1.f;
+    float dx = 0.f, dy = 0.f;
+    if (!(ox == 0 && oy == 0)) {
+        double th;
+        float r;
+        if (fabsf(ox) > fabsf(oy)) { r = ox; th = (kPi / 4.f) * (double)(oy / ox); }
+        else { r = oy; th = (kPi / 2.f) - (kPi / 4.f) * (double)(ox / oy); }
+        dx = r * (float)cos(th);
+        dy = r * (float)sin(th);
+    }
+    return mk3(dx, dy, sqrtf(fmx(0.f, 1 - dx * dx - dy * dy)));
+}
+__device__ inline float cos_pdf(float z) { return (float)fabs((double)z / kPi); }
+
+// ---------------------------------------------------------------- GGX
+struct Ggx {
+    float ax, ay;
+    __device__ float D(v3 h) const {
+        const float t2 = t2z(h);
+        if (isinf(t2)) return 0;
+        const float c4 = c2z(h) * c2z(h);
+        const float e = (caz(h) * caz(h) / (ax * ax) + saz(h) * saz(h) / (ay * ay)) * t2;
+        return (float)(1 / (kPi * ax * ay * c4 * (1 + e) * (1 + e)));
+    }
+    __device__ float lambda(v3 w) const {
+        if (dot3(w, mk3(0, 0, 1)) == 0) return 0;
+        const float at = fabsf(tz(w));
+        const float a = sqrtf(caz(w) * caz(w) * ax * ax + saz(w) * saz(w) * ay * ay);
+        const float a2 = (a * at) * (a * at);
+        return (-1 + sqrtf(1.f + a2)) / 2;
+    }
+    __device__ float G1(v3 w) const { return 1 / (1 + lambda(w)); }
+    __device__ float G(v3 i, v3 o) const { return 1.f / (1.f + lambda(i) + lambda(o)); }
+    __device__ float pdf(v3 h, v3 o) const { return D(h) * G1(o) * fabsf(dot3(o, h)) / fabsf(o.z); }
+    __device__ static void slopes(float cosT, float u1, float u2, float& sx, float& sy) {
+        if (cosT > .9999) {
+            const float r = sqrtf(u1 / (1 - u1));
+            const float phi = (float)(6.28318530718 * u2);
+            sx = r * cosf(phi);
+            sy = r * sinf(phi);
+            return;
+        }
+        const float sinT = sqrtf(fmx(0.f, 1.f - cosT * cosT));
+        const float tanT = sinT / cosT;
+        const float a = 1 / tanT;
+        const float g1 = 2 / (1 + sqrtf(1.f + 1.f / (a * a)));
+        const float A = 2 * u1 / g1 - 1;
+        float tmp = 1.f / (A * A - 1.f);
+        if (tmp > 1e10) tmp = 1e10;
+        const float B = tanT;
+        const float Dd = sqrtf(fmx(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
+        const float s1 = B * tmp - Dd, s2 = B * tmp + Dd;
+        sx = (A < 0 || s2 > 1.f / tanT) ? s1 : s2;
+        float S;
+        if (u2 > 0.5f) { S = 1.f; u2 = 2.f * (u2 - .5f); }
+        else { S = -1.f; u2 = 2.f * (.5f - u2); }
+        const float z = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f)) /
+                        (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
+        sy = S * z * sqrtf(1.f + sx * sx);
+    }
+    __device__ v3 sample(float u1, float u2, v3 o) const {
+        const bool flip = o.z < 0;
+        if (flip) o = scale3(o, -1.f);
+        const v3 os = nrm(mk3(ax * o.x, ay * o.y, o.z));
+        float sx, sy;
+        slopes(os.z, u1, u2, sx, sy);
+        const float t = caz(os) * sx - saz(os) * sy;
+        sy = saz(os) * sx + caz(os) * sy;
+        sx = t;
+        sx = ax * sx;
+        sy = ay * sy;
+        v3 r = nrm(mk3(-sx, -sy, 1.f));
+        if (flip) r = scale3(r, -1.f);
+        return r;
+    }
+};
+
+__device__ inline v3 schlick(float c, v3 f0) {
+    const float t = pow5(1 - fabsf(c));
+    return add3(f0, scale3(sub3(mk3(1, 1, 1), f0), t));
+}
+
+// ---------------------------------------------------------------- BSDF
+struct Bsdf {
+    int type;
+    v3 kd, ks, f0, refl, trans;
+    Ggx ggx;
+    float kr, ior;
+
+    __device__ bool delta() const { return type == CAV_MAT_MIRROR || type == CAV_MAT_GLASS; }
+    __device__ v3 lam(v3 i, v3 o) const {
+        if (!same_hemi(i, o)) return mk3(0, 0, 0);
+        return divs3(kd, (float)kPi);
+    }
+    __device__ v3 mf(v3 i, v3 o) const {
+        if (!same_hemi(i, o)) return mk3(0, 0, 0);
+        const float co = fabsf(o.z), ci = fabsf(i.z);
+        v3 h = add3(i, o);
+        if (ci == 0 || co == 0) return mk3(0, 0, 0);
+        if (h.x == 0 && h.y == 0 && h.z == 0) return mk3(0, 0, 0);
+        h = nrm(h);
+        const v3 F = schlick(dot3(o, h), f0);
+        return divs3(mul3(scale3(scale3(ks, ggx.D(h)), ggx.G(o, i)), F), 4 * ci * co);
+    }
+    __device__ float mf_pdf(v3 i, v3 o) const {
+        const v3 h = nrm(add3(i, o));
+        return ggx.pdf(h, o) / (4.f * dot3(o, h));
+    }
+    __device__ v3 srefl(v3 i, v3 o) const {
+        if (!same_hemi(i, o)) return mk3(0, 0, 0);
+        return divs3(mul3(refl, schlick(fabsf(i.z), f0)), fabsf(i.z));
+    }
+    __device__ v3 strans(v3 i, v3 o) const {
+        if (same_hemi(i, o)) return mk3(0, 0, 0);
+        return divs3(mul3(trans, sub3(mk3(1, 1, 1), schlick(fabsf(i.z), f0))), fabsf(i.z));
+    }
+    __device__ bool srefract(v3 o, v3& out) const {
+        const float eta = o.z > 0 ? (1.f / ior) : (ior / 1.f);
+        v3 n = mk3(0, 0, 1);
+        if (o.z < 0) n.z = -1;
+        return refract3(o, n, eta, out);
+    }
+    __device__ float sprob(v3 o) const {
+        float p = schlick(fabsf(o.z), f0).x;
+        if (zero3(refl)) p = 0;
+        else if (zero3(trans)) p = 1;
+        return p;
+    }
+    __device__ v3 eval(v3 i, v3 o) const {
+        if (type == CAV_MAT_MATTE) return lam(i, o);
+        if (type == CAV_MAT_PLASTIC) return add3(lam(i, o), mf(i, o));
+        if (type == CAV_MAT_MIRROR) {
+            if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk3(0, 0, 0);
+            return scale3(mk3(1, 1, 1), kr / fabsf(i.z));
+        }
+        if (same_hemi(i, o)) {
+            if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk3(0, 0, 0);
+            return srefl(i, o);
+        }
+        v3 r;
+        if (!srefract(o, r) || !eq3(r, i)) return mk3(0, 0, 0);
+        return strans(i, o);
+    }
+    __device__ float pdf(v3 i, v3 o) const {
+        if (type == CAV_MAT_MATTE) return cos_pdf(i.z);
+        if (type == CAV_MAT_PLASTIC) return 0.5f * cos_pdf(i.z) + (1.f - 0.5f) * mf_pdf(i, o);
+        if (type == CAV_MAT_MIRROR) return (i.x == -o.x && i.y == -o.y && i.z == o.z) ? 1.f : 0.f;
+        if (same_hemi(i, o)) {
+            if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return 0;
+            return sprob(o);
+        }
+        v3 r;
+        if (!srefract(o, r) || !eq3(r, i)) return 0;
+        return 1.f - sprob(o);
+    }
+    __device__ v3 sample(float ux, float uy, v3& wi, v3 o, float& p) const {
+        if (type == CAV_MAT_MATTE) {
+            wi = cos_hemi(ux, uy);
+            p = cos_pdf(wi.z);
+            if (o.z < 0) wi.z *= -1;
+            return lam(wi, o);
+        }
+        if (type == CAV_MAT_PLASTIC) {
+            float q = ux;
+            if (q < 0.5f) {
+                q = q * (1.f / 0.5f);
+                wi = cos_hemi(q, uy);
+                if (o.z < 0) wi.z *= -1;
+            } else {
+                q = (q - 0.5f) * (1.f / (1.f - 0.5f));
+                wi = reflect3(o, ggx.sample(q, uy, o));
+            }
+            p = pdf(wi, o);
+            return eval(wi, o);
+        }
+        if (type == CAV_MAT_MIRROR) {
+            wi = scale3(o, -1);
+            wi.z = o.z;
+            p = 1;
+            return eval(wi, o);
+        }
+        const float pr = sprob(o);
+        if (ux < pr) {
+            wi = mk3(-o.x, -o.y, o.z);
+            p = 1;
+            const v3 r = srefl(wi, o);
+            p *= pr;
+            return r;
+        }
+        v3 r = mk3(0, 0, 0);
+        if (srefract(o, wi)) { p = 1; r = strans(wi, o); }
+        p *= (1.f - pr);
+        return r;
+    }
+};
+
+__device__ inline Bsdf make_bsdf(const cav_mesh& m) {
+    Bsdf b;
+    b.type = m.material;
+    b.kd = mk3(m.kd[0], m.kd[1], m.kd[2]);
+    b.ks = mk3(m.ks[0], m.ks[1], m.ks[2]);
+    b.refl = mk3(m.kr[0], m.kr[1], m.kr[2]);
+    b.trans = mk3(m.kt[0], m.kt[1], m.kt[2]);
+    b.ggx = Ggx{m.roughness, m.roughness};
+    b.kr = m.kr[0];
+    b.ior = m.ior;
+    const float eta = m.material == CAV_MAT_GLASS ? m.ior : 1.5f;  // Materials/Plastic.h:111, Glass.h:238
+    const float t = (eta - 1) / (eta + 1);
+    b.f0 = mk3(t * t, t * t, t * t);
+    return b;
+}
+
+__device__ inline v3 soup_v(const float* soup, int t, int j) {
+    const float* p = soup + (int64_t)t * 9 + j * 3;
+    return mk3(p[0], p[1], p[2]);
+}
+__device__ inline v3 tri_normal(const SceneDev& sc, int t) {
+    const v3 v0 = soup_v(sc.soup, t, 0);
+    v3 n = nrm(cross3(sub3(soup_v(sc.soup, t, 1), v0), sub3(soup_v(sc.soup, t, 2), v0)));
+    if (sc.meshes[sc.tri_mesh[t]].reverse_orientation) n = scale3(n, -1.f);
+    return n;
+}
+__device__ inline float power_heuristic(float a, float b) {
+    a *= a;
+    b *= b;
+    return a / (a + b);
+}
+// glm mat4 * vec4 and the divide (Utils/MathsCommons.h:57-60)
+__device__ inline v3 apply_m(const float* m, v3 v) {
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r[k] = (m[0 * 4 + k] * v.x + m[1 * 4 + k] * v.y) + (m[2 * 4 + k] * v.z + m[3 * 4 + k] * 1.f);
+    return mk3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
+}
+
+// Appends `count` lanes of the wave (those with `push`) to a queue; returns
+// each pushing lane's slot. All lanes of the wave must call it.
+__device__ inline uint32_t wave_push(bool push, uint32_t* count) {
+    const uint64_t m = __ballot(push);
+    if (m == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, first, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+
+}  // namespace shade
+}  // namespace cav

@@ -63,12 +63,16 @@ class Scene:
 
 
 def render(ctx: Context, scene: Scene, spp: int, integrator: str = "path", camera=None,
-           max_depth: int | None = None):
-    """Renders the scene; returns (Film, stats dict)."""
+           max_depth: int | None = None, q_grid: int = 32, alpha_scale: float = 1.0):
+    """Renders the scene; returns (Film, stats dict). q_grid and alpha_scale
+    apply to rlpath (Q entries per axis; scale of the n/(T+n) learning rate)."""
     a = scene.arrays
     cam = camera if camera is not None else for_scene(a)
+    if integrator not in INTEGRATORS:
+        raise ValueError(f"unknown integrator {integrator!r}")
     prm = _lib.RenderParams(INTEGRATORS[integrator], int(spp),
-                            int(max_depth if max_depth is not None else a.max_depth), 0)
+                            int(max_depth if max_depth is not None else a.max_depth), 0,
+                            int(q_grid), float(alpha_scale))
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
     w = np.zeros((cam.height, cam.width), np.float32)
     st = _lib.RenderStats()

@@ -171,6 +171,12 @@ typedef struct {
     int32_t spp;         /* samples per pixel */
     int32_t max_depth;   /* bounces (the scene's "integer maxdepth") */
     int32_t flags;       /* reserved, 0 */
+    /* rlpath only. Q entries per axis: 0 = 32, the reference's Q_TABLE_SIZE
+     * (Integrators/ReinforcementLearningPathTracing.cu:151); at most 64. */
+    int32_t q_grid;
+    /* rlpath only. Scale of the learning rate n/(T+n) (QEntry::alpha, :44-50):
+     * 0 or 1 = the reference's rule. */
+    float alpha_scale;
 } cav_render_params;
 
 typedef struct {

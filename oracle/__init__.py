@@ -173,3 +173,58 @@ def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int =
     if rc != 0:
         raise ValueError(f"ref_render_path failed ({rc})")
     return rgb, wt, int(rays[0])
+
+
+def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30,
+                  q_grid: int = 32, alpha_scale: float = 1.0, max_depth: int | None = None):
+    """`rlpath` integrator on the CPU, bounce by bounce over the samples in
+    sample order. Returns (film_rgb [H,W,3], film_w [H,W], rays_traced)."""
+    tris = np.ascontiguousarray(scene.tris, np.float32)
+    _, _, nd = lbvh_build(tris, morton_bits)
+    if bvh_rounds:
+        treelet_optimize(nd, bvh_rounds)
+    w, h = scene.width, scene.height
+    rgb = np.zeros((h, w, 3), np.float32)
+    wt = np.zeros((h, w), np.float32)
+    rays = np.zeros(1, np.int64)
+    cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
+    s6 = scene_bounds(tris)
+    depth = scene.max_depth if max_depth is None else max_depth
+    rc = lib().ref_render_rlpath(
+        _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(s6), ctypes.c_int(q_grid),
+        ctypes.c_float(alpha_scale), _p(cam16), ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h),
+        ctypes.c_int(spp), ctypes.c_int(depth), _p(rgb), _p(wt), _p(rays))
+    if rc != 0:
+        raise ValueError(f"ref_render_rlpath failed ({rc})")
+    return rgb, wt, int(rays[0])
+
+
+def rl_dir_to_cell(d) -> int:
+    """QEntry::dirToCellIndex of one direction."""
+    return int(lib().ref_rl_dir_to_cell(_p(np.ascontiguousarray(d, np.float32))))
+
+
+def rl_cell_dir(rx: float, ry: float, cell: int) -> np.ndarray:
+    """QEntry::sampleDirectionInCell."""
+    out = np.empty(3, np.float32)
+    lib().ref_rl_cell_dir(ctypes.c_float(rx), ctypes.c_float(ry), ctypes.c_int(cell), _p(out))
+    return out
+
+
+def rl_finish_q(Q, newQ, cnt, total, alpha_scale: float = 1.0) -> None:
+    """finishUpdateQ + averageQ in place on [E, 128] arrays (float32, float32,
+    int32, int32); newQ and cnt are cleared."""
+    for a, dt in ((Q, np.float32), (newQ, np.float32), (cnt, np.int32), (total, np.int32)):
+        assert a.dtype == dt and a.flags.c_contiguous and a.shape == Q.shape and a.shape[1] == 128
+    lib().ref_rl_finish_q(_p(Q), _p(newQ), _p(cnt), _p(total), ctypes.c_int(Q.shape[0]),
+                          ctypes.c_float(alpha_scale))
+
+
+def rl_finish_light(ldist, lres, lcount, lnew, lnewc) -> None:
+    """finishLightingTraining in place on [E, L] arrays."""
+    for a, dt in ((ldist, np.float32), (lres, np.float32), (lcount, np.int32), (lnew, np.float32),
+                  (lnewc, np.int32)):
+        assert a.dtype == dt and a.flags.c_contiguous and a.shape == ldist.shape
+    lib().ref_rl_finish_light(_p(ldist), _p(lres), _p(lcount), _p(lnew), _p(lnewc),
+                              ctypes.c_int(ldist.shape[0]), ctypes.c_int(ldist.shape[1]))

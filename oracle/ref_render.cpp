@@ -669,3 +669,453 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
     if (rays_traced) *rays_traced = traced;
     return 0;
 }
+
+
+// ====================================================================== rlpath
+// Integrators/ReinforcementLearningPathTracing.cu, bounce by bounce over the
+// live samples (in sample order), because the Q-table and the per-cell light
+// distributions learnt in one bounce are shared by every sample. The
+// reference's syncDimension is applied as written: each sync sets every task
+// of the queue to the largest Halton dimension in the queue.
+namespace {
+
+struct QTab {
+    int g, E, L;
+    std::vector<float> Q, newQ;
+    std::vector<int> cnt, total;
+    std::vector<float> ldist, lres, lnew;
+    std::vector<int> lcount, lnewc;
+};
+
+const int kNumX = 16, kNumY = 8, kNumXY = 128;
+
+// QEntry::dirToCellIndex (RL .cu:113-128); (int)NaN is taken as 0, the value
+// the reference's float->int conversion gives.
+int f2i(float v) { return std::isnan(v) ? 0 : (v >= 2147483648.f ? 2147483647 : (v <= -2147483648.f ? (-2147483647 - 1) : (int)v)); }
+int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+int dir_to_cell(f3 d) {
+    float u = d.z;
+    float y = (u + 1.f) / 2.f;
+    int ti = clampi(f2i(y * kNumY), 0, kNumY - 1);
+    d.x /= std::sqrt(1.f - u * u);
+    d.y /= std::sqrt(1.f - u * u);
+    // acos / cos / sin are evaluated in binary64 and rounded once, so the
+    // single-precision results are correctly rounded (the HIP kernels do the
+    // same) rather than depending on a libm's float routines.
+    float v = (float)std::acos((double)d.x);
+    if (d.y < 0) v = (float)((2.0 * kPi) - (double)v);
+    v = (float)((double)v / (2.0 * kPi));
+    int pi = clampi(f2i(v * kNumX), 0, kNumX - 1);
+    return ti * kNumX + pi;
+}
+// QEntry::sampleDirectionInCell (RL .cu:131-147)
+f3 cell_dir(float rx, float ry, int cell) {
+    int ti = cell / kNumX, pi = cell % kNumX;
+    float u = ((float)ti + rx) * (1.f / (float)kNumY);
+    u = u * 2 - 1.f;
+    float v = ((float)pi + ry) * (1.f / (float)kNumX);
+    float xy = std::sqrt(1.0f - u * u);
+    float phi = (float)(2 * kPi * v);
+    return mk(xy * (float)std::cos((double)phi), xy * (float)std::sin((double)phi), u);
+}
+// findQEntry (RL .cu:155-165)
+int find_entry(const float* b6, int g, f3 p) {
+    f3 size = mk(b6[3], b6[4], b6[5]) - mk(b6[0], b6[1], b6[2]);
+    f3 es = divv(size, mk((float)g, (float)g, (float)g));
+    p = p - mk(b6[0], b6[1], b6[2]);
+    int x = clampi(f2i(p.x / es.x), 0, g - 1);
+    int y = clampi(f2i(p.y / es.y), 0, g - 1);
+    int z = clampi(f2i(p.z / es.z), 0, g - 1);
+    return x * g * g + y * g + z;
+}
+float luminance(f3 c) { return dot(c, mk(0.212671f, 0.715160f, 0.072169f)); }
+
+// Distribution1D::getCorrespondingIndex / sample (Utils/RandomUtils.h:120-160)
+int dist_index(const float* cdf, int n, float f) {
+    int l = 0, r = n - 1;
+    while (l < r) {
+        int m = (l + r) / 2;
+        if (cdf[m] >= f) {
+            if (m == 0) return 0;
+            if (cdf[m - 1] < f) return m;
+            r = m - 1;
+        } else {
+            l = m + 1;
+        }
+    }
+    return l;
+}
+float dist_pdf(const float* cdf, int i) { return i == 0 ? cdf[0] : cdf[i] - cdf[i - 1]; }
+
+struct Task {
+    uint32_t sid;
+    f3 ro, rd, mult;
+    float pdf;
+    bool delta;
+    int prev_entry, prev_cell;
+    int dim;
+};
+
+}  // namespace
+
+// finishUpdateQ + averageQ (RL .cu:44-110, 738-767) over `nent` entries of
+// 128 bins, then prepareForUpdateQ (newQ and the proposal counts cleared).
+extern "C" void ref_rl_finish_q(float* Q, float* newQ, int32_t* cnt, int32_t* total, int nent,
+                                float alpha_scale) {
+    for (int64_t i = 0; i < (int64_t)nent * kNumXY; ++i) {
+        if (cnt[i] != 0) {
+            float a = (float)cnt[i] / (float)(total[i] + cnt[i]);
+            a *= alpha_scale;
+            if (a != 0) {
+                Q[i] = Q[i] * (1.f - a) + a * newQ[i] / (float)cnt[i];
+                total[i] += cnt[i];
+            }
+        }
+        newQ[i] = 0.f;
+        cnt[i] = 0;
+    }
+    for (int e = 0; e < nent; ++e) {
+        float* Qe = Q + (int64_t)e * kNumXY;
+        const int32_t* tot = total + (int64_t)e * kNumXY;
+        float sq = 0;
+        int c = 0;
+        for (int i = 0; i < kNumXY; ++i)
+            if (tot[i] > 0) { sq += Qe[i]; c += 1; }
+        if (c == 0) continue;
+        const float avg = sq / (float)c;
+        for (int i = 0; i < kNumXY; ++i)
+            if (tot[i] == 0) Qe[i] = avg;
+    }
+}
+
+// finishLightingTraining (RL .cu:463-514) over `nent` entries of L lights;
+// the new results and counts are cleared (prepareLightingTraining).
+extern "C" void ref_rl_finish_light(float* ldist, float* lres, int32_t* lcount, float* lnew,
+                                    int32_t* lnewc, int nent, int L) {
+    for (int e = 0; e < nent; ++e) {
+        float sum_r = 0, valid = 0;
+        for (int i = 0; i < L; ++i) {
+            const size_t j = (size_t)e * L + i;
+            if (lnewc[j] > 0) {
+                float tc = (float)(lcount[j] + lnewc[j]);
+                lres[j] = ((float)lcount[j] / tc) * lres[j] + ((float)lnewc[j] / tc) * (lnew[j] / (float)lnewc[j]);
+                lcount[j] += lnewc[j];
+                sum_r += lres[j];
+                valid += 1;
+            }
+            lnewc[j] = 0;
+            lnew[j] = 0;
+        }
+        if (sum_r > 0) {
+            float avg = sum_r / (float)valid;
+            float dens = 0;
+            for (int i = 0; i < L; ++i) {
+                const size_t j = (size_t)e * L + i;
+                if (lres[j] > 0) dens += lres[j];
+                else if (lcount[j] > 0) dens += avg / (float)lcount[j];
+                else dens += avg;
+                ldist[j] = dens;
+            }
+            for (int i = 0; i < L; ++i) ldist[(size_t)e * L + i] /= dens;
+        }
+    }
+}
+
+// rlpath render. scene6 = scene bounds (union of triangle boxes); q_grid = Q
+// entries per axis (the reference's Q_TABLE_SIZE is 32); alpha_scale scales
+// the learning rate n/(T+n) (1 = the reference's rule).
+extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
+                                 const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim,
+                                 const float* scene6, int q_grid, float alpha_scale,
+                                 const float* cam16, float cam_depth, int width, int height,
+                                 int spp, int max_depth, float* film_rgb, float* film_w,
+                                 int64_t* rays_traced) {
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    for (int m = 0; m < nmeshes; ++m)
+        if (meshes[m].is_light) s.lights.push_back(m);
+    s.area.resize(nmeshes);
+    ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
+    const Halton& H = halton();
+    const int L = (int)s.lights.size();
+    QTab T;
+    T.g = q_grid;
+    T.E = q_grid * q_grid * q_grid;
+    T.L = L;
+    T.Q.assign((size_t)T.E * kNumXY, 1.f);  // QEntry(): Q = defaultQ = 1
+    T.newQ.assign((size_t)T.E * kNumXY, 0.f);
+    T.cnt.assign((size_t)T.E * kNumXY, 0);
+    T.total.assign((size_t)T.E * kNumXY, 0);
+    T.ldist.resize((size_t)T.E * L);
+    for (int e = 0; e < T.E; ++e)  // initialiseLightSamplingDist
+        for (int i = 0; i < L; ++i) T.ldist[(size_t)e * L + i] = ((float)i + 1.f) / (float)L;
+    T.lres.assign((size_t)T.E * L, 0.f);
+    T.lnew.assign((size_t)T.E * L, 0.f);
+    T.lcount.assign((size_t)T.E * L, 0);
+    T.lnewc.assign((size_t)T.E * L, 0);
+    const float cell_sa = (float)(4.f * kPi / (float)kNumXY);
+
+    const int64_t nsamp = (int64_t)width * height * spp;
+    std::vector<float> res((size_t)nsamp * 3, 0.f), sx((size_t)nsamp), sy((size_t)nsamp);
+    std::vector<Task> q;
+    q.reserve(nsamp);
+    for (int64_t si = 0; si < nsamp; ++si) {
+        const int64_t pix = si / spp;
+        const int x = (int)(pix % width), y = (int)(pix / width);
+        int dim = 0;
+        float fx = (float)x + H.rand1((uint64_t)si, dim);
+        float fy = (float)y + H.rand1((uint64_t)si, dim);
+        sx[si] = fx;
+        sy[si] = fy;
+        f3 pl = mk(fx - width / 2.f, height / 2.f - fy, cam_depth);
+        f3 ro = apply(cam16, mk(0, 0, 0));
+        f3 rd = normalize(apply(cam16, pl) - ro);
+        q.push_back(Task{(uint32_t)si, ro, rd, mk(1, 1, 1), 1.f, true, -1, -1, dim});
+    }
+    int64_t traced = 0;
+    struct Hit { Task t; int prim; float dist; };
+    for (int depth = 0; depth < max_depth && !q.empty(); ++depth) {
+        std::vector<Hit> hits;
+        hits.reserve(q.size());
+        for (const Task& t : q) {
+            float d;
+            ++traced;
+            int p = closest(s, t.ro, t.rd, d);
+            if (p >= 0) hits.push_back(Hit{t, p, d});
+        }
+        q.clear();
+        if (hits.empty()) break;
+        int mx = 0;
+        for (auto& h : hits) mx = std::max(mx, h.t.dim);
+        for (auto& h : hits) h.t.dim = mx;  // syncDimension
+
+        struct LRes { f3 contrib; f3 dir; float emis, sampled, lpdf; };
+        struct QRes { float sum, pdfq; };
+        struct NInfo { f3 dir; float pdf; int cell; bool valid; };
+        std::vector<LRes> lr(hits.size());
+        std::vector<QRes> qr(hits.size());
+        std::vector<NInfo> ni(hits.size());
+        // computeLighting (RL .cu:388-460)
+        for (size_t k = 0; k < hits.size(); ++k) {
+            Hit& h = hits[k];
+            Task& t = h.t;
+            const int mesh = tri_mesh[h.prim];
+            const cav_mesh_ref& M = meshes[mesh];
+            f3 n = tri_normal(s, h.prim);
+            f3 pos = t.ro + h.dist * t.rd;
+            Bsdf bsdf = make_bsdf(M);
+            f3 t0, t1;
+            coordinate_system(n, t0, t1);
+            float* acc = &res[(size_t)t.sid * 3];
+            f3 emis = mk(0, 0, 0);
+            if (M.is_light) {
+                f3 Le = dot(t.rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
+                if (t.delta) {
+                    f3 a = mulv(Le, t.mult);
+                    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+                } else if (depth > 0) {
+                    f3 l2s = t.ro - pos;
+                    float c = std::fabs(dot(n, normalize(l2s)));
+                    float lp = dot(l2s, l2s) / (c * s.area[mesh]);
+                    float w = power_heuristic(t.pdf, lp);
+                    if (std::isfinite(w)) {
+                        emis = Le;
+                        f3 a = mulv(Le, t.mult) * w;
+                        acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+                    }
+                }
+            }
+            const int entry = find_entry(scene6, T.g, pos);
+            const float* cdf = &T.ldist[(size_t)entry * L];
+            const int li = dist_index(cdf, L, H.rand1(t.sid, t.dim));
+            const float sel = dist_pdf(cdf, li);
+            t.dim += 4;  // rand4
+            const int lm = s.lights[li];
+            const cav_mesh_ref& Lm = meshes[lm];
+            int tri = Lm.first_tri + H.rand_int(Lm.num_tris, t.sid, t.dim);
+            float tmp = std::sqrt(H.rand1(t.sid, t.dim));
+            float u = 1 - tmp;
+            float v = tmp * H.rand1(t.sid, t.dim);
+            f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+            f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
+            f3 ln = tri_normal(s, tri);
+            f3 l2s = pos - lp;
+            float c = std::fabs(dot(ln, normalize(l2s)));
+            float lpdf = dot(l2s, l2s) / (c * s.area[lm]);
+            f3 dir = normalize(lp - pos);
+            float lim = length(lp - pos);
+            f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(Lm.emission[0], Lm.emission[1], Lm.emission[2]);
+            ++traced;
+            if (!(visible(s, pos, dir, lim, mesh, lm) && std::isfinite(lpdf))) {
+                incident = mk(0, 0, 0);
+                lpdf = 1;
+            }
+            f3 wo = t.rd * -1;
+            f3 il = to_local(dir, t0, t1, n), ol = to_local(wo, t0, t1, n);
+            f3 contrib = mulv(incident, bsdf.eval(il, ol)) * std::fabs(dot(dir, n));
+            contrib = contrib * (1.f / (lpdf * sel));
+            T.lnew[(size_t)entry * L + li] += luminance(incident);
+            T.lnewc[(size_t)entry * L + li] += 1;
+            lr[k] = LRes{contrib, dir, luminance(emis), luminance(incident), lpdf};
+        }
+        // computeQDistributions (RL .cu:517-606)
+        bool any_valid = false;
+        for (size_t k = 0; k < hits.size(); ++k) {
+            Hit& h = hits[k];
+            Task& t = h.t;
+            const int mesh = tri_mesh[h.prim];
+            f3 n = tri_normal(s, h.prim);
+            f3 pos = t.ro + h.dist * t.rd;
+            Bsdf bsdf = make_bsdf(meshes[mesh]);
+            f3 t0, t1;
+            coordinate_system(n, t0, t1);
+            const int entry = find_entry(scene6, T.g, pos);
+            float* Qe = &T.Q[(size_t)entry * kNumXY];
+            f3 ol = to_local(t.rd * -1.f, t0, t1, n);
+            float px = H.rand1(t.sid, t.dim), py = H.rand1(t.sid, t.dim);
+            float cdf[kNumXY];
+            float sum = 0.f;
+            for (int c = 0; c < kNumXY; ++c) {
+                f3 d = cell_dir(px, py, c);
+                float sc = luminance(bsdf.eval(to_local(d, t0, t1, n), ol));
+                float qv = std::fabs(dot(d, n)) * sc * Qe[c];
+                sum += qv;
+                cdf[c] = sum;
+            }
+            for (int c = 0; c < kNumXY; ++c) cdf[c] /= sum;
+            sum *= cell_sa;
+            qr[k].sum = sum;
+            qr[k].pdfq = 0.f;
+            if (t.prev_entry != -1) {
+                const size_t idx = (size_t)t.prev_entry * kNumXY + t.prev_cell;
+                T.newQ[idx] += sum + lr[k].emis;
+                T.cnt[idx] += 1;
+            }
+            const int lc = dir_to_cell(lr[k].dir);
+            if (sum != 0) qr[k].pdfq = dist_pdf(cdf, lc) / cell_sa;
+            if (lr[k].sampled > 0) {
+                float sa = 1.f / lr[k].lpdf;
+                float ratio = fminr(1.f, 0.5f * sa / cell_sa);
+                float prop = lr[k].sampled * ratio;
+                if (prop > Qe[lc]) {
+                    T.newQ[(size_t)entry * kNumXY + lc] += prop;
+                    T.cnt[(size_t)entry * kNumXY + lc] += 1;
+                }
+            }
+            if (sum != 0) {
+                int cell = dist_index(cdf, kNumXY, H.rand1(t.sid, t.dim));
+                float p = dist_pdf(cdf, cell);
+                p = (float)(kNumXY * p / (4 * kPi));
+                ni[k] = NInfo{cell_dir(px, py, cell), p, cell, true};
+                any_valid = true;
+            } else {
+                ni[k].valid = false;
+            }
+        }
+        // accumulateLightingContribs (RL .cu:609-647)
+        for (size_t k = 0; k < hits.size(); ++k) {
+            Hit& h = hits[k];
+            Task& t = h.t;
+            const int mesh = tri_mesh[h.prim];
+            f3 n = tri_normal(s, h.prim);
+            Bsdf bsdf = make_bsdf(meshes[mesh]);
+            f3 t0, t1;
+            coordinate_system(n, t0, t1);
+            f3 il = to_local(lr[k].dir, t0, t1, n), ol = to_local(t.rd * -1.f, t0, t1, n);
+            float mp = bsdf.pdf(il, ol);
+            float sp = 0;
+            if (qr[k].sum != 0) sp = qr[k].pdfq;
+            if (qr[k].sum == 0 || sp == 0 || bsdf.is_delta()) sp = mp;
+            float w = power_heuristic(lr[k].lpdf, sp);
+            f3 a = mulv(lr[k].contrib, t.mult) * w;
+            float* acc = &res[(size_t)t.sid * 3];
+            acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+        }
+        ref_rl_finish_q(T.Q.data(), T.newQ.data(), T.cnt.data(), T.total.data(), T.E, alpha_scale);
+        ref_rl_finish_light(T.ldist.data(), T.lres.data(), T.lcount.data(), T.lnew.data(), T.lnewc.data(), T.E, L);
+        (void)any_valid;
+        mx = 0;
+        for (auto& h : hits) mx = std::max(mx, h.t.dim);
+        for (auto& h : hits) h.t.dim = mx;  // syncDimension
+        // genNextRay (RL .cu:305-375)
+        for (size_t k = 0; k < hits.size(); ++k) {
+            Hit& h = hits[k];
+            Task t = h.t;
+            if (depth > 3) {
+                float qq = 1;
+                qq = fminr(qq, 1 - t.mult.x);
+                qq = fminr(qq, 1 - t.mult.y);
+                qq = fminr(qq, 1 - t.mult.z);
+                qq = fmaxr(qq, 0.05f);
+                if (H.rand1(t.sid, t.dim) < qq) continue;
+                t.mult = t.mult / (1.f - qq);
+            }
+            const int mesh = tri_mesh[h.prim];
+            f3 n = tri_normal(s, h.prim);
+            f3 pos = t.ro + h.dist * t.rd;
+            Bsdf bsdf = make_bsdf(meshes[mesh]);
+            f3 t0, t1;
+            coordinate_system(n, t0, t1);
+            f3 wo = t.rd * -1.f;
+            const int entry = find_entry(scene6, T.g, pos);
+            f3 f, nd;
+            float npdf;
+            int cell;
+            if (bsdf.is_delta() || !ni[k].valid) {
+                float ux = H.rand1(t.sid, t.dim), uy = H.rand1(t.sid, t.dim);
+                f3 wl;
+                npdf = 0;
+                f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, n), npdf);
+                nd = to_world(wl, t0, t1, n);
+                cell = dir_to_cell(nd);
+            } else {
+                cell = ni[k].cell;
+                npdf = ni[k].pdf;
+                nd = ni[k].dir;
+                f = bsdf.eval(to_local(nd, t0, t1, n), to_local(wo, t0, t1, n));
+            }
+            f3 no = pos + nd * 0.001f;
+            if (all_zero(f)) t.mult = mk(0, 0, 0);
+            else t.mult = mulv(t.mult, f) * std::fabs(dot(nd, n)) / npdf;
+            t.ro = no;
+            t.rd = nd;
+            t.pdf = npdf;
+            t.delta = bsdf.is_delta();
+            t.prev_entry = entry;
+            t.prev_cell = cell;
+            q.push_back(t);
+        }
+    }
+
+#pragma omp parallel for schedule(static)
+    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+        const int x = (int)(pix % width), y = (int)(pix / width);
+        float r = 0, g = 0, b = 0, w = 0;
+        for (int py = y - 1; py <= y + 1; ++py)
+            for (int px = x - 1; px <= x + 1; ++px) {
+                if (px < 0 || py < 0 || px >= width || py >= height) continue;
+                for (int k = 0; k < spp; ++k) {
+                    const int64_t si = ((int64_t)py * width + px) * spp + k;
+                    const float* c = &res[si * 3];
+                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
+                    const float fx = sx[si], fy = sy[si];
+                    if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
+                    if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
+                    if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
+                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
+                }
+            }
+        film_rgb[pix * 3 + 0] = r;
+        film_rgb[pix * 3 + 1] = g;
+        film_rgb[pix * 3 + 2] = b;
+        film_w[pix] = w;
+    }
+    if (rays_traced) *rays_traced = traced;
+    return 0;
+}
+
+extern "C" int ref_rl_dir_to_cell(const float* dir3) { return dir_to_cell(mk(dir3[0], dir3[1], dir3[2])); }
+extern "C" void ref_rl_cell_dir(float rx, float ry, int cell, float* out3) {
+    f3 d = cell_dir(rx, ry, cell);
+    out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+}
