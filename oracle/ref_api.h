@@ -32,4 +32,24 @@ int ref_trace_occluded(const float* box, const int32_t* left, const int32_t* rig
                        const int32_t* prim, const float* tris, const int32_t* mesh,
                        const float* rays, const float* limit, const int32_t* src_mesh,
                        const int32_t* dst_mesh, int64_t nrays, uint8_t* visible);
+
+// render oracles (ref_render.cpp)
+void ref_halton_table(uint32_t* primes, uint32_t* start, uint16_t* perm, int perm_cap);
+float ref_halton(uint64_t index, int dim);
+void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, int nmeshes, float* area);
+int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
+                    int nmeshes, const float* box, const int32_t* left, const int32_t* right,
+                    const int32_t* prim, const float* cam16, float cam_depth, int width, int height,
+                    int spp, int max_depth, float* film_rgb, float* film_w, int64_t* rays_traced);
+int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
+                      const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                      const int32_t* left, const int32_t* right, const int32_t* prim,
+                      const float* scene6, int q_grid, float alpha_scale, const float* cam16,
+                      float cam_depth, int width, int height, int spp, int max_depth,
+                      float* film_rgb, float* film_w, int64_t* rays_traced);
+int ref_rl_dir_to_cell(const float* dir3);
+void ref_rl_cell_dir(float rx, float ry, int cell, float* out3);
+void ref_rl_finish_q(float* Q, float* newQ, int32_t* cnt, int32_t* total, int nent, float alpha_scale);
+void ref_rl_finish_light(float* ldist, float* lres, int32_t* lcount, float* lnew, int32_t* lnewc,
+                         int nent, int L);
 }
