@@ -42,12 +42,14 @@ class Camera(ctypes.Structure):
 class RenderParams(ctypes.Structure):
     _fields_ = [("integrator", ctypes.c_int32), ("spp", ctypes.c_int32),
                 ("max_depth", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("q_grid", ctypes.c_int32), ("alpha_scale", ctypes.c_float)]
+                ("q_grid", ctypes.c_int32), ("alpha_scale", ctypes.c_float),
+                ("spp_per_round", ctypes.c_int32)]
 
 
 class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("closest_rays", ctypes.c_int64),
-                ("shadow_rays", ctypes.c_int64), ("mrays_per_s", ctypes.c_double)]
+                ("shadow_rays", ctypes.c_int64), ("mrays_per_s", ctypes.c_double),
+                ("rounds", ctypes.c_int32)]
 
 
 _lib = None
@@ -82,6 +84,9 @@ _SIGS = {
     "cav_scene_bvh": ([P], P),
     "cav_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
                     ctypes.POINTER(RenderStats)], I),
+    "cav_film_to_srgb8": ([P, P, I, I, P], I),
+    "cav_write_png": ([ctypes.c_char_p, P, I, I], I),
+    "cav_write_pfm": ([ctypes.c_char_p, P, P, I, I], I),
 }
 
 

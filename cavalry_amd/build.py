@@ -26,6 +26,7 @@ SOURCES = [
     "render.hip",
     "rlpath.hip",
     "render_capi.cpp",
+    "film_io.cpp",
     "capi.cpp",
     "camera.cpp",
     "ctx_release.cpp",

@@ -57,7 +57,9 @@ struct RenderArgs {
     HaltonDev halton;
     CameraDev cam;
     int width, height, spp, max_depth;
-    int64_t nsamp;
+    int64_t nsamp;           // samples of the whole render (width * height * spp)
+    int64_t nq;              // samples of this round (width * height * spp_round): queue capacity
+    int spp_round, spp_done; // this round's samples per pixel and those done before it
     Queue q[2];
     Shadow sh;
     uint8_t* vis;
@@ -105,8 +107,11 @@ struct RlArgs {
     int32_t* dims;  // [0] synced dimension at the bounce start, [1] any Q sample
 };
 
+// one spp round (a.spp_done .. a.spp_done + a.spp_round) of each integrator;
+// rlpath's Q table and light statistics persist across the rounds of a render
+// and are initialised when init_tables is set
 hipError_t render_path(const RenderArgs& a, hipStream_t s);
-hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s);
+hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s);
 // camera rays for every sample and the gathered box-filtered film, shared by
 // both integrators
 void render_raygen(const RenderArgs& a, hipStream_t s);

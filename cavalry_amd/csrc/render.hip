@@ -30,14 +30,19 @@ namespace {
 using namespace shade;
 
 // genCameraSampleSPPs (Core/Sampler.cu:29-51) + genInitialRays
-// (PathTracingIntegrator.cu:252-266): sample si of pixel si / spp.
-__global__ __launch_bounds__(kBlock) void k_raygen(int64_t nsamp, int width, int spp, CameraDev cam,
-                                                   HaltonDev h, Queue q, float* __restrict__ res,
-                                                   float* __restrict__ sxy, uint32_t* counts) {
-    const int64_t si = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (si == 0) counts[0] = (uint32_t)nsamp;
-    if (si >= nsamp) return;
-    const int64_t pix = si / spp;
+// (PathTracingIntegrator.cu:252-266) for one round of `spp_round` samples per
+// pixel starting at `spp_done`: queue slot j is sample k = spp_done +
+// j % spp_round of pixel j / spp_round, whose Halton index (and sample id) is
+// pixel * spp + k.
+__global__ __launch_bounds__(kBlock) void k_raygen(int64_t nq, int width, int spp, int spp_round,
+                                                   int spp_done, CameraDev cam, HaltonDev h, Queue q,
+                                                   float* __restrict__ res, float* __restrict__ sxy,
+                                                   uint32_t* counts) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j == 0) counts[0] = (uint32_t)nq;
+    if (j >= nq) return;
+    const int64_t pix = j / spp_round;
+    const int64_t si = pix * spp + spp_done + j % spp_round;
     const int x = (int)(pix % width), y = (int)(pix / width);
     float fx = (float)x, fy = (float)y;
     fx += halton(h, (uint32_t)si, 0);
@@ -47,12 +52,12 @@ __global__ __launch_bounds__(kBlock) void k_raygen(int64_t nsamp, int width, int
     const v3 pl = mk3(fx - cam.width / 2.f, cam.height / 2.f - fy, cam.depth);
     const v3 ro = apply_m(cam.m, mk3(0, 0, 0));
     const v3 rd = nrm(sub3(apply_m(cam.m, pl), ro));
-    float* r = q.ray + si * 6;
+    float* r = q.ray + j * 6;
     r[0] = ro.x; r[1] = ro.y; r[2] = ro.z; r[3] = rd.x; r[4] = rd.y; r[5] = rd.z;
-    q.sid[si] = (uint32_t)si;
-    q.mult[si * 3] = 1.f; q.mult[si * 3 + 1] = 1.f; q.mult[si * 3 + 2] = 1.f;
-    q.pdf[si] = 1.f;
-    q.delta[si] = 1;
+    q.sid[j] = (uint32_t)si;
+    q.mult[j * 3] = 1.f; q.mult[j * 3 + 1] = 1.f; q.mult[j * 3 + 2] = 1.f;
+    q.pdf[j] = 1.f;
+    q.delta[j] = 1;
     res[si * 3] = 0.f; res[si * 3 + 1] = 0.f; res[si * 3 + 2] = 0.f;
 }
 
@@ -239,9 +244,9 @@ __global__ __launch_bounds__(kBlock) void k_film(int width, int height, int spp,
 }  // namespace
 
 void render_raygen(const RenderArgs& a, hipStream_t s) {
-    const unsigned nb = (unsigned)((a.nsamp + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_raygen, dim3(nb), dim3(kBlock), 0, s, a.nsamp, a.width, a.spp, a.cam, a.halton,
-                       a.q[0], a.res, a.sxy, a.counts);
+    const unsigned nb = (unsigned)((a.nq + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_raygen, dim3(nb), dim3(kBlock), 0, s, a.nq, a.width, a.spp, a.spp_round,
+                       a.spp_done, a.cam, a.halton, a.q[0], a.res, a.sxy, a.counts);
 }
 
 void render_film(const RenderArgs& a, hipStream_t s) {
@@ -251,7 +256,7 @@ void render_film(const RenderArgs& a, hipStream_t s) {
 }
 
 hipError_t render_path(const RenderArgs& a, hipStream_t s) {
-    const int64_t nsamp = a.nsamp;
+    const int64_t nsamp = a.nq;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
@@ -274,7 +279,6 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
         dim0 += 10 + (depth > 3 ? 1 : 0);
         qin = qout;
     }
-    render_film(a, s);
     return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 // include/cavalry.h for the reference counterparts.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -36,13 +37,20 @@ struct cav_scene {
 
 namespace {
 
+// device bytes per sample of a round (queues, shadow tasks, hits) and per
+// sample of the whole render (radiance + film position)
+constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 8;
+constexpr double kRlQueueBytes = 2 * 12 + (4 * 5 + 12 * 3 + 4 + 4 + 4);
+constexpr double kResultBytes = 12 + 8;
+
 // Per-context render state: Halton tables and the sample/queue buffers,
 // grown on demand and kept between renders.
 struct RenderState {
     uint32_t* primes = nullptr;
     uint32_t* start = nullptr;
     uint16_t* perm = nullptr;
-    int64_t cap = 0;      // samples the buffers hold
+    int64_t cap = 0;      // samples per round the queue buffers hold
+    int64_t tot_cap = 0;  // samples the per-sample result buffers hold
     int64_t pix_cap = 0;  // pixels the film buffers hold
     std::vector<void*> bufs;
     RenderArgs a{};
@@ -61,6 +69,7 @@ void free_list(std::vector<void*>& v) {
 void free_bufs(RenderState* rs) {
     free_list(rs->bufs);
     rs->cap = 0;
+    rs->tot_cap = 0;
     rs->pix_cap = 0;
     free_list(rs->rl_bufs);
     free_list(rs->rl_tab);
@@ -187,11 +196,13 @@ int ensure_rl(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t cells, int64
     return CAV_OK;
 }
 
-int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t npix) {
-    if (nsamp <= rs->cap && npix <= rs->pix_cap) return CAV_OK;
+// Queue-sized buffers for nq samples per round; per-sample results for the
+// whole render (ntot samples) and the film (npix pixels).
+int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int64_t npix) {
+    if (nq <= rs->cap && ntot <= rs->tot_cap && npix <= rs->pix_cap) return CAV_OK;
     (void)hipStreamSynchronize(ctx->stream);
     free_bufs(rs);
-    const size_t S = (size_t)nsamp, P = (size_t)npix;
+    const size_t S = (size_t)nq, T = (size_t)ntot, P = (size_t)npix;
     RenderArgs& a = rs->a;
     int rc = CAV_OK;
     for (int k = 0; k < 2 && rc == CAV_OK; ++k) {
@@ -210,15 +221,16 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t npix) {
     if (rc == CAV_OK) rc = grab(ctx, rs, a.vis, S, "visibility");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_prim, 4 * S, "hit prim");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_t, 4 * S, "hit t");
-    if (rc == CAV_OK) rc = grab(ctx, rs, a.res, 12 * S, "sample radiance");
-    if (rc == CAV_OK) rc = grab(ctx, rs, a.sxy, 8 * S, "sample positions");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.res, 12 * T, "sample radiance");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sxy, 8 * T, "sample positions");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.counts, 64, "queue counts");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.rays, 64, "ray tallies");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 64, "trace counters");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
     if (rc != CAV_OK) { free_bufs(rs); return rc; }
-    rs->cap = nsamp;
+    rs->cap = nq;
+    rs->tot_cap = ntot;
     rs->pix_cap = npix;
     return CAV_OK;
 }
@@ -355,13 +367,31 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     const int64_t dims = rl ? 2 + 14LL * prm->max_depth
                             : 2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0);
     if (dims > 1024) return fail(ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
+    if (prm->spp_per_round < 0) return fail(ctx, CAV_ERR_INVALID, "spp_per_round must be >= 0");
     RCHK(ctx, hipSetDevice(ctx->device));
     RenderState* rs = nullptr;
     int rc = ensure_state(ctx, rs);
     if (rc != CAV_OK) return rc;
-    if ((rc = ensure_buffers(ctx, rs, nsamp, npix)) != CAV_OK) return rc;
+    // spp per round (Samplers/DecideSampleCount.cu:7-23, sppBySpp order): as
+    // many as the free device memory holds with 10% slack, or the caller's
+    // fixed count
+    int spp_round = prm->spp;
+    if (prm->spp_per_round > 0) {
+        spp_round = std::min(prm->spp_per_round, prm->spp);
+    } else {
+        size_t fr = 0, tot = 0;
+        RCHK(ctx, hipMemGetInfo(&fr, &tot));
+        const double held = (double)rs->cap * kQueueBytes + (rl ? (double)rs->rl_cap * kRlQueueBytes : 0.0);
+        const double per_spp = (double)npix * (kQueueBytes + (rl ? kRlQueueBytes : 0)) * 1.1;
+        const double avail = (double)fr + held - (double)nsamp * kResultBytes - (512.0 * (1 << 20));
+        const double fit = avail / per_spp;
+        if (fit < 1.0) return fail(ctx, CAV_ERR_OOM, "one sample per pixel does not fit in device memory");
+        spp_round = (int)std::min<double>(prm->spp, fit);
+    }
+    const int64_t nq = npix * spp_round;
+    if ((rc = ensure_buffers(ctx, rs, nq, nsamp, npix)) != CAV_OK) return rc;
     const int64_t nent = (int64_t)grid * grid * grid;
-    if (rl && (rc = ensure_rl(ctx, rs, nsamp, nent * 128, nent * sc->nlights)) != CAV_OK) return rc;
+    if (rl && (rc = ensure_rl(ctx, rs, nq, nent * 128, nent * sc->nlights)) != CAV_OK) return rc;
     RenderArgs a = rs->a;
     a.scene = SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
                        sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
@@ -379,15 +409,20 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     RCHK(ctx, hipMemsetAsync(a.rays, 0, 16, s));
     RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));
     RCHK(ctx, hipEventRecord(rs->ev[0], s));
-    if (rl) {
-        RlArgs r = rs->rl;
-        r.grid = grid;
-        r.alpha_scale = alpha_scale;
-        std::memcpy(r.bounds, sc->bounds, sizeof r.bounds);
-        RCHK(ctx, render_rlpath(a, r, s));
-    } else {
-        RCHK(ctx, render_path(a, s));
+    RlArgs r = rs->rl;
+    r.grid = grid;
+    r.alpha_scale = alpha_scale;
+    std::memcpy(r.bounds, sc->bounds, sizeof r.bounds);
+    int rounds = 0;
+    for (int done = 0; done < prm->spp; done += spp_round, ++rounds) {
+        a.spp_done = done;
+        a.spp_round = std::min(spp_round, prm->spp - done);
+        a.nq = npix * a.spp_round;
+        if (rl) RCHK(ctx, render_rlpath(a, r, done == 0, s));
+        else RCHK(ctx, render_path(a, s));
     }
+    render_film(a, s);
+    RCHK(ctx, hipGetLastError());
     RCHK(ctx, hipEventRecord(rs->ev[1], s));
     unsigned long long tally[2] = {0, 0};
     uint32_t tc[2] = {0, 0};
@@ -403,6 +438,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
         stats->closest_rays = (int64_t)tally[0];
         stats->shadow_rays = (int64_t)tally[1];
         stats->mrays_per_s = ms > 0 ? (double)(tally[0] + tally[1]) / (ms * 1e3) : 0.0;
+        stats->rounds = rounds;
     }
     if (tc[0] || tc[1]) return fail(ctx, CAV_ERR_OVERFLOW, "traversal stack overflow while rendering");
     return CAV_OK;

@@ -649,8 +649,8 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
 
 }  // namespace
 
-hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s) {
-    const int64_t nsamp = a.nsamp;
+hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s) {
+    const int64_t nsamp = a.nq;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
     const int L = a.scene.nlights;
     const int64_t nent = (int64_t)rl.grid * rl.grid * rl.grid;
@@ -666,7 +666,8 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s) {
     if (e != hipSuccess) return e;
     if ((e = hipMemsetAsync(rl.dims, 0, 16, s)) != hipSuccess) return e;
     const int64_t ninit = ncells > nent * L ? ncells : nent * L;
-    hipLaunchKernelGGL(k_init_tables, dim3((unsigned)((ninit + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
+    if (init_tables)
+        hipLaunchKernelGGL(k_init_tables, dim3((unsigned)((ninit + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
                        ncells, nent, L);
     render_raygen(a, s);
     hipLaunchKernelGGL(k_init_tasks, dim3(nb), dim3(kBlock), 0, s, nsamp, rl.q[0]);
@@ -694,7 +695,6 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, hipStream_t s) {
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
         qin = qout;
     }
-    render_film(a, s);
     return hipGetLastError();
 }
 

@@ -26,13 +26,54 @@ def default_bvh_rounds(spp: int) -> int:
 
 
 class Film:
+    """Per-pixel sample sums (rgb, [H, W, 3]) and filter weights ([H, W])."""
+
     def __init__(self, rgb: np.ndarray, weight: np.ndarray):
-        self.rgb = rgb
-        self.weight = weight
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        self.weight = np.ascontiguousarray(weight, np.float32)
 
     @property
     def image(self) -> np.ndarray:
         return self.rgb / np.maximum(self.weight, 1e-30)[..., None]
+
+    def to_srgb8(self) -> np.ndarray:
+        """Film::readCurrentResult: clamp to [0, 1], sRGB gamma, u8."""
+        h, w = self.weight.shape
+        out = np.empty((h, w, 3), np.uint8)
+        check(None, lib().cav_film_to_srgb8(self.rgb.ctypes.data_as(ctypes.c_void_p),
+                                            self.weight.ctypes.data_as(ctypes.c_void_p), w, h,
+                                            out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def save_png(self, path: str) -> None:
+        """RenderResult::saveToPNG of the resolved film."""
+        img = np.ascontiguousarray(self.to_srgb8())
+        h, w = self.weight.shape
+        check(None, lib().cav_write_png(path.encode(), img.ctypes.data_as(ctypes.c_void_p), w, h))
+
+    def save_pfm(self, path: str) -> None:
+        """FP32 colour PFM of rgb / weight."""
+        h, w = self.weight.shape
+        check(None, lib().cav_write_pfm(path.encode(), self.rgb.ctypes.data_as(ctypes.c_void_p),
+                                        self.weight.ctypes.data_as(ctypes.c_void_p), w, h))
+
+
+def read_pfm(path: str) -> np.ndarray:
+    """Colour PFM -> float32 [H, W, 3], top row first."""
+    with open(path, "rb") as f:
+        if f.readline().strip() != b"PF":
+            raise ValueError(f"{path}: not a colour PFM")
+        w, h = (int(v) for v in f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(), "<f4" if scale < 0 else ">f4", count=w * h * 3)
+    return data.reshape(h, w, 3)[::-1].astype(np.float32)
+
+
+def rel_mse(img: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
+    """Relative MSE, mean over pixels and channels of (img - ref)^2 / (ref^2 + eps)."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
 
 
 class Scene:
@@ -63,16 +104,19 @@ class Scene:
 
 
 def render(ctx: Context, scene: Scene, spp: int, integrator: str = "path", camera=None,
-           max_depth: int | None = None, q_grid: int = 32, alpha_scale: float = 1.0):
+           max_depth: int | None = None, q_grid: int = 32, alpha_scale: float = 1.0,
+           spp_per_round: int = 0):
     """Renders the scene; returns (Film, stats dict). q_grid and alpha_scale
-    apply to rlpath (Q entries per axis; scale of the n/(T+n) learning rate)."""
+    apply to rlpath (Q entries per axis; scale of the n/(T+n) learning rate);
+    spp_per_round fixes the samples per pixel of each round (0: as many as
+    device memory holds)."""
     a = scene.arrays
     cam = camera if camera is not None else for_scene(a)
     if integrator not in INTEGRATORS:
         raise ValueError(f"unknown integrator {integrator!r}")
     prm = _lib.RenderParams(INTEGRATORS[integrator], int(spp),
                             int(max_depth if max_depth is not None else a.max_depth), 0,
-                            int(q_grid), float(alpha_scale))
+                            int(q_grid), float(alpha_scale), int(spp_per_round))
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
     w = np.zeros((cam.height, cam.width), np.float32)
     st = _lib.RenderStats()

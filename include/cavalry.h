@@ -177,6 +177,11 @@ typedef struct {
     /* rlpath only. Scale of the learning rate n/(T+n) (QEntry::alpha, :44-50):
      * 0 or 1 = the reference's rule. */
     float alpha_scale;
+    /* Samples per pixel per round (Sampler::genAllCameraSamples, sppBySpp
+     * order, Core/Sampler.cu:77-96): 0 = as many as free device memory holds
+     * (Samplers/DecideSampleCount.cu:7-23). Rounds do not change a path film;
+     * rlpath keeps learning across them as the reference does. */
+    int32_t spp_per_round;
 } cav_render_params;
 
 typedef struct {
@@ -184,6 +189,7 @@ typedef struct {
     int64_t closest_rays;       /* camera + continuation rays traced */
     int64_t shadow_rays;        /* shadow rays traced */
     double mrays_per_s;         /* (closest + shadow) / ms_total */
+    int32_t rounds;             /* spp rounds the render took */
 } cav_render_stats;
 
 /* Replaces Renderer::render -> Integrator::render (Core/Renderer.cu:3-6,
@@ -193,6 +199,21 @@ typedef struct {
 int cav_render(cav_ctx* ctx, cav_scene* scene, const cav_camera* camera,
                const cav_render_params* params, float* film_rgb, float* film_weight,
                cav_render_stats* stats);
+
+/* ---------------------------------------------------------------- film output */
+
+/* Replaces Film::readCurrentResult + apply + writeColorAt (Core/Film.cu:20-46,
+ * Core/Color.h:53-81): per pixel rgb / weight, clamped to [0, 1], sRGB gamma,
+ * times 255 truncated to u8. Host only; out is uint8[h][w][3]. */
+int cav_film_to_srgb8(const float* film_rgb, const float* film_weight, int width, int height,
+                      uint8_t* out);
+/* Replaces RenderResult::saveToPNG (Core/RenderResult.cu:9-12): 8-bit RGB PNG,
+ * stored (uncompressed) deflate blocks. */
+int cav_write_png(const char* path, const uint8_t* rgb8, int width, int height);
+/* No reference counterpart (the reference writes 8-bit PNG only): the film as
+ * little-endian colour PFM of rgb / weight (weight may be NULL: rgb as is). */
+int cav_write_pfm(const char* path, const float* film_rgb, const float* film_weight, int width,
+                  int height);
 
 #ifdef __cplusplus
 }

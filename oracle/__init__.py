@@ -176,9 +176,11 @@ def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int =
 
 
 def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30,
-                  q_grid: int = 32, alpha_scale: float = 1.0, max_depth: int | None = None):
+                  q_grid: int = 32, alpha_scale: float = 1.0, max_depth: int | None = None,
+                  spp_per_round: int = 0):
     """`rlpath` integrator on the CPU, bounce by bounce over the samples in
-    sample order. Returns (film_rgb [H,W,3], film_w [H,W], rays_traced)."""
+    sample order, in rounds of spp_per_round samples per pixel (0: one round).
+    Returns (film_rgb [H,W,3], film_w [H,W], rays_traced)."""
     tris = np.ascontiguousarray(scene.tris, np.float32)
     _, _, nd = lbvh_build(tris, morton_bits)
     if bvh_rounds:
@@ -194,7 +196,7 @@ def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
         _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
         _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(s6), ctypes.c_int(q_grid),
         ctypes.c_float(alpha_scale), _p(cam16), ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h),
-        ctypes.c_int(spp), ctypes.c_int(depth), _p(rgb), _p(wt), _p(rays))
+        ctypes.c_int(spp), ctypes.c_int(depth), ctypes.c_int(spp_per_round), _p(rgb), _p(wt), _p(rays))
     if rc != 0:
         raise ValueError(f"ref_render_rlpath failed ({rc})")
     return rgb, wt, int(rays[0])

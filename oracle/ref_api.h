@@ -46,7 +46,7 @@ int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                       const int32_t* left, const int32_t* right, const int32_t* prim,
                       const float* scene6, int q_grid, float alpha_scale, const float* cam16,
                       float cam_depth, int width, int height, int spp, int max_depth,
-                      float* film_rgb, float* film_w, int64_t* rays_traced);
+                      int spp_per_round, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_rl_dir_to_cell(const float* dir3);
 void ref_rl_cell_dir(float rx, float ry, int cell, float* out3);
 void ref_rl_finish_q(float* Q, float* newQ, int32_t* cnt, int32_t* total, int nent, float alpha_scale);

@@ -17,6 +17,7 @@
 #include "ref_api.h"
 #include "ref_math.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -829,8 +830,8 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                                  const int32_t* left, const int32_t* right, const int32_t* prim,
                                  const float* scene6, int q_grid, float alpha_scale,
                                  const float* cam16, float cam_depth, int width, int height,
-                                 int spp, int max_depth, float* film_rgb, float* film_w,
-                                 int64_t* rays_traced) {
+                                 int spp, int max_depth, int spp_per_round, float* film_rgb,
+                                 float* film_w, int64_t* rays_traced) {
     Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
@@ -857,235 +858,243 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
 
     const int64_t nsamp = (int64_t)width * height * spp;
     std::vector<float> res((size_t)nsamp * 3, 0.f), sx((size_t)nsamp), sy((size_t)nsamp);
-    std::vector<Task> q;
-    q.reserve(nsamp);
-    for (int64_t si = 0; si < nsamp; ++si) {
-        const int64_t pix = si / spp;
-        const int x = (int)(pix % width), y = (int)(pix / width);
-        int dim = 0;
-        float fx = (float)x + H.rand1((uint64_t)si, dim);
-        float fy = (float)y + H.rand1((uint64_t)si, dim);
-        sx[si] = fx;
-        sy[si] = fy;
-        f3 pl = mk(fx - width / 2.f, height / 2.f - fy, cam_depth);
-        f3 ro = apply(cam16, mk(0, 0, 0));
-        f3 rd = normalize(apply(cam16, pl) - ro);
-        q.push_back(Task{(uint32_t)si, ro, rd, mk(1, 1, 1), 1.f, true, -1, -1, dim});
-    }
     int64_t traced = 0;
-    struct Hit { Task t; int prim; float dist; };
-    for (int depth = 0; depth < max_depth && !q.empty(); ++depth) {
-        std::vector<Hit> hits;
-        hits.reserve(q.size());
-        for (const Task& t : q) {
-            float d;
-            ++traced;
-            int p = closest(s, t.ro, t.rd, d);
-            if (p >= 0) hits.push_back(Hit{t, p, d});
+    // spp rounds in the reference's sppBySpp order (Core/Sampler.cu:29-51,
+    // 77-96): round slot j is sample done + j % round of pixel j / round; the
+    // Q table and light statistics carry over from round to round.
+    const int round_spp = spp_per_round > 0 && spp_per_round < spp ? spp_per_round : spp;
+    for (int done = 0; done < spp; done += round_spp) {
+        const int this_spp = std::min(round_spp, spp - done);
+        std::vector<Task> q;
+        q.reserve((size_t)width * height * this_spp);
+        for (int64_t j = 0; j < (int64_t)width * height * this_spp; ++j) {
+            const int64_t pix = j / this_spp;
+            const int64_t si = pix * spp + done + j % this_spp;
+            const int x = (int)(pix % width), y = (int)(pix / width);
+            int dim = 0;
+            float fx = (float)x + H.rand1((uint64_t)si, dim);
+            float fy = (float)y + H.rand1((uint64_t)si, dim);
+            sx[si] = fx;
+            sy[si] = fy;
+            f3 pl = mk(fx - width / 2.f, height / 2.f - fy, cam_depth);
+            f3 ro = apply(cam16, mk(0, 0, 0));
+            f3 rd = normalize(apply(cam16, pl) - ro);
+            q.push_back(Task{(uint32_t)si, ro, rd, mk(1, 1, 1), 1.f, true, -1, -1, dim});
         }
-        q.clear();
-        if (hits.empty()) break;
-        int mx = 0;
-        for (auto& h : hits) mx = std::max(mx, h.t.dim);
-        for (auto& h : hits) h.t.dim = mx;  // syncDimension
+        struct Hit { Task t; int prim; float dist; };
+        for (int depth = 0; depth < max_depth && !q.empty(); ++depth) {
+            std::vector<Hit> hits;
+            hits.reserve(q.size());
+            for (const Task& t : q) {
+                float d;
+                ++traced;
+                int p = closest(s, t.ro, t.rd, d);
+                if (p >= 0) hits.push_back(Hit{t, p, d});
+            }
+            q.clear();
+            if (hits.empty()) break;
+            int mx = 0;
+            for (auto& h : hits) mx = std::max(mx, h.t.dim);
+            for (auto& h : hits) h.t.dim = mx;  // syncDimension
 
-        struct LRes { f3 contrib; f3 dir; float emis, sampled, lpdf; };
-        struct QRes { float sum, pdfq; };
-        struct NInfo { f3 dir; float pdf; int cell; bool valid; };
-        std::vector<LRes> lr(hits.size());
-        std::vector<QRes> qr(hits.size());
-        std::vector<NInfo> ni(hits.size());
-        // computeLighting (RL .cu:388-460)
-        for (size_t k = 0; k < hits.size(); ++k) {
-            Hit& h = hits[k];
-            Task& t = h.t;
-            const int mesh = tri_mesh[h.prim];
-            const cav_mesh_ref& M = meshes[mesh];
-            f3 n = tri_normal(s, h.prim);
-            f3 pos = t.ro + h.dist * t.rd;
-            Bsdf bsdf = make_bsdf(M);
-            f3 t0, t1;
-            coordinate_system(n, t0, t1);
-            float* acc = &res[(size_t)t.sid * 3];
-            f3 emis = mk(0, 0, 0);
-            if (M.is_light) {
-                f3 Le = dot(t.rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
-                if (t.delta) {
-                    f3 a = mulv(Le, t.mult);
-                    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
-                } else if (depth > 0) {
-                    f3 l2s = t.ro - pos;
-                    float c = std::fabs(dot(n, normalize(l2s)));
-                    float lp = dot(l2s, l2s) / (c * s.area[mesh]);
-                    float w = power_heuristic(t.pdf, lp);
-                    if (std::isfinite(w)) {
-                        emis = Le;
-                        f3 a = mulv(Le, t.mult) * w;
+            struct LRes { f3 contrib; f3 dir; float emis, sampled, lpdf; };
+            struct QRes { float sum, pdfq; };
+            struct NInfo { f3 dir; float pdf; int cell; bool valid; };
+            std::vector<LRes> lr(hits.size());
+            std::vector<QRes> qr(hits.size());
+            std::vector<NInfo> ni(hits.size());
+            // computeLighting (RL .cu:388-460)
+            for (size_t k = 0; k < hits.size(); ++k) {
+                Hit& h = hits[k];
+                Task& t = h.t;
+                const int mesh = tri_mesh[h.prim];
+                const cav_mesh_ref& M = meshes[mesh];
+                f3 n = tri_normal(s, h.prim);
+                f3 pos = t.ro + h.dist * t.rd;
+                Bsdf bsdf = make_bsdf(M);
+                f3 t0, t1;
+                coordinate_system(n, t0, t1);
+                float* acc = &res[(size_t)t.sid * 3];
+                f3 emis = mk(0, 0, 0);
+                if (M.is_light) {
+                    f3 Le = dot(t.rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
+                    if (t.delta) {
+                        f3 a = mulv(Le, t.mult);
                         acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+                    } else if (depth > 0) {
+                        f3 l2s = t.ro - pos;
+                        float c = std::fabs(dot(n, normalize(l2s)));
+                        float lp = dot(l2s, l2s) / (c * s.area[mesh]);
+                        float w = power_heuristic(t.pdf, lp);
+                        if (std::isfinite(w)) {
+                            emis = Le;
+                            f3 a = mulv(Le, t.mult) * w;
+                            acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+                        }
                     }
                 }
+                const int entry = find_entry(scene6, T.g, pos);
+                const float* cdf = &T.ldist[(size_t)entry * L];
+                const int li = dist_index(cdf, L, H.rand1(t.sid, t.dim));
+                const float sel = dist_pdf(cdf, li);
+                t.dim += 4;  // rand4
+                const int lm = s.lights[li];
+                const cav_mesh_ref& Lm = meshes[lm];
+                int tri = Lm.first_tri + H.rand_int(Lm.num_tris, t.sid, t.dim);
+                float tmp = std::sqrt(H.rand1(t.sid, t.dim));
+                float u = 1 - tmp;
+                float v = tmp * H.rand1(t.sid, t.dim);
+                f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+                f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
+                f3 ln = tri_normal(s, tri);
+                f3 l2s = pos - lp;
+                float c = std::fabs(dot(ln, normalize(l2s)));
+                float lpdf = dot(l2s, l2s) / (c * s.area[lm]);
+                f3 dir = normalize(lp - pos);
+                float lim = length(lp - pos);
+                f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(Lm.emission[0], Lm.emission[1], Lm.emission[2]);
+                ++traced;
+                if (!(visible(s, pos, dir, lim, mesh, lm) && std::isfinite(lpdf))) {
+                    incident = mk(0, 0, 0);
+                    lpdf = 1;
+                }
+                f3 wo = t.rd * -1;
+                f3 il = to_local(dir, t0, t1, n), ol = to_local(wo, t0, t1, n);
+                f3 contrib = mulv(incident, bsdf.eval(il, ol)) * std::fabs(dot(dir, n));
+                contrib = contrib * (1.f / (lpdf * sel));
+                T.lnew[(size_t)entry * L + li] += luminance(incident);
+                T.lnewc[(size_t)entry * L + li] += 1;
+                lr[k] = LRes{contrib, dir, luminance(emis), luminance(incident), lpdf};
             }
-            const int entry = find_entry(scene6, T.g, pos);
-            const float* cdf = &T.ldist[(size_t)entry * L];
-            const int li = dist_index(cdf, L, H.rand1(t.sid, t.dim));
-            const float sel = dist_pdf(cdf, li);
-            t.dim += 4;  // rand4
-            const int lm = s.lights[li];
-            const cav_mesh_ref& Lm = meshes[lm];
-            int tri = Lm.first_tri + H.rand_int(Lm.num_tris, t.sid, t.dim);
-            float tmp = std::sqrt(H.rand1(t.sid, t.dim));
-            float u = 1 - tmp;
-            float v = tmp * H.rand1(t.sid, t.dim);
-            f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
-            f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-            f3 ln = tri_normal(s, tri);
-            f3 l2s = pos - lp;
-            float c = std::fabs(dot(ln, normalize(l2s)));
-            float lpdf = dot(l2s, l2s) / (c * s.area[lm]);
-            f3 dir = normalize(lp - pos);
-            float lim = length(lp - pos);
-            f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(Lm.emission[0], Lm.emission[1], Lm.emission[2]);
-            ++traced;
-            if (!(visible(s, pos, dir, lim, mesh, lm) && std::isfinite(lpdf))) {
-                incident = mk(0, 0, 0);
-                lpdf = 1;
-            }
-            f3 wo = t.rd * -1;
-            f3 il = to_local(dir, t0, t1, n), ol = to_local(wo, t0, t1, n);
-            f3 contrib = mulv(incident, bsdf.eval(il, ol)) * std::fabs(dot(dir, n));
-            contrib = contrib * (1.f / (lpdf * sel));
-            T.lnew[(size_t)entry * L + li] += luminance(incident);
-            T.lnewc[(size_t)entry * L + li] += 1;
-            lr[k] = LRes{contrib, dir, luminance(emis), luminance(incident), lpdf};
-        }
-        // computeQDistributions (RL .cu:517-606)
-        bool any_valid = false;
-        for (size_t k = 0; k < hits.size(); ++k) {
-            Hit& h = hits[k];
-            Task& t = h.t;
-            const int mesh = tri_mesh[h.prim];
-            f3 n = tri_normal(s, h.prim);
-            f3 pos = t.ro + h.dist * t.rd;
-            Bsdf bsdf = make_bsdf(meshes[mesh]);
-            f3 t0, t1;
-            coordinate_system(n, t0, t1);
-            const int entry = find_entry(scene6, T.g, pos);
-            float* Qe = &T.Q[(size_t)entry * kNumXY];
-            f3 ol = to_local(t.rd * -1.f, t0, t1, n);
-            float px = H.rand1(t.sid, t.dim), py = H.rand1(t.sid, t.dim);
-            float cdf[kNumXY];
-            float sum = 0.f;
-            for (int c = 0; c < kNumXY; ++c) {
-                f3 d = cell_dir(px, py, c);
-                float sc = luminance(bsdf.eval(to_local(d, t0, t1, n), ol));
-                float qv = std::fabs(dot(d, n)) * sc * Qe[c];
-                sum += qv;
-                cdf[c] = sum;
-            }
-            for (int c = 0; c < kNumXY; ++c) cdf[c] /= sum;
-            sum *= cell_sa;
-            qr[k].sum = sum;
-            qr[k].pdfq = 0.f;
-            if (t.prev_entry != -1) {
-                const size_t idx = (size_t)t.prev_entry * kNumXY + t.prev_cell;
-                T.newQ[idx] += sum + lr[k].emis;
-                T.cnt[idx] += 1;
-            }
-            const int lc = dir_to_cell(lr[k].dir);
-            if (sum != 0) qr[k].pdfq = dist_pdf(cdf, lc) / cell_sa;
-            if (lr[k].sampled > 0) {
-                float sa = 1.f / lr[k].lpdf;
-                float ratio = fminr(1.f, 0.5f * sa / cell_sa);
-                float prop = lr[k].sampled * ratio;
-                if (prop > Qe[lc]) {
-                    T.newQ[(size_t)entry * kNumXY + lc] += prop;
-                    T.cnt[(size_t)entry * kNumXY + lc] += 1;
+            // computeQDistributions (RL .cu:517-606)
+            bool any_valid = false;
+            for (size_t k = 0; k < hits.size(); ++k) {
+                Hit& h = hits[k];
+                Task& t = h.t;
+                const int mesh = tri_mesh[h.prim];
+                f3 n = tri_normal(s, h.prim);
+                f3 pos = t.ro + h.dist * t.rd;
+                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                f3 t0, t1;
+                coordinate_system(n, t0, t1);
+                const int entry = find_entry(scene6, T.g, pos);
+                float* Qe = &T.Q[(size_t)entry * kNumXY];
+                f3 ol = to_local(t.rd * -1.f, t0, t1, n);
+                float px = H.rand1(t.sid, t.dim), py = H.rand1(t.sid, t.dim);
+                float cdf[kNumXY];
+                float sum = 0.f;
+                for (int c = 0; c < kNumXY; ++c) {
+                    f3 d = cell_dir(px, py, c);
+                    float sc = luminance(bsdf.eval(to_local(d, t0, t1, n), ol));
+                    float qv = std::fabs(dot(d, n)) * sc * Qe[c];
+                    sum += qv;
+                    cdf[c] = sum;
+                }
+                for (int c = 0; c < kNumXY; ++c) cdf[c] /= sum;
+                sum *= cell_sa;
+                qr[k].sum = sum;
+                qr[k].pdfq = 0.f;
+                if (t.prev_entry != -1) {
+                    const size_t idx = (size_t)t.prev_entry * kNumXY + t.prev_cell;
+                    T.newQ[idx] += sum + lr[k].emis;
+                    T.cnt[idx] += 1;
+                }
+                const int lc = dir_to_cell(lr[k].dir);
+                if (sum != 0) qr[k].pdfq = dist_pdf(cdf, lc) / cell_sa;
+                if (lr[k].sampled > 0) {
+                    float sa = 1.f / lr[k].lpdf;
+                    float ratio = fminr(1.f, 0.5f * sa / cell_sa);
+                    float prop = lr[k].sampled * ratio;
+                    if (prop > Qe[lc]) {
+                        T.newQ[(size_t)entry * kNumXY + lc] += prop;
+                        T.cnt[(size_t)entry * kNumXY + lc] += 1;
+                    }
+                }
+                if (sum != 0) {
+                    int cell = dist_index(cdf, kNumXY, H.rand1(t.sid, t.dim));
+                    float p = dist_pdf(cdf, cell);
+                    p = (float)(kNumXY * p / (4 * kPi));
+                    ni[k] = NInfo{cell_dir(px, py, cell), p, cell, true};
+                    any_valid = true;
+                } else {
+                    ni[k].valid = false;
                 }
             }
-            if (sum != 0) {
-                int cell = dist_index(cdf, kNumXY, H.rand1(t.sid, t.dim));
-                float p = dist_pdf(cdf, cell);
-                p = (float)(kNumXY * p / (4 * kPi));
-                ni[k] = NInfo{cell_dir(px, py, cell), p, cell, true};
-                any_valid = true;
-            } else {
-                ni[k].valid = false;
+            // accumulateLightingContribs (RL .cu:609-647)
+            for (size_t k = 0; k < hits.size(); ++k) {
+                Hit& h = hits[k];
+                Task& t = h.t;
+                const int mesh = tri_mesh[h.prim];
+                f3 n = tri_normal(s, h.prim);
+                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                f3 t0, t1;
+                coordinate_system(n, t0, t1);
+                f3 il = to_local(lr[k].dir, t0, t1, n), ol = to_local(t.rd * -1.f, t0, t1, n);
+                float mp = bsdf.pdf(il, ol);
+                float sp = 0;
+                if (qr[k].sum != 0) sp = qr[k].pdfq;
+                if (qr[k].sum == 0 || sp == 0 || bsdf.is_delta()) sp = mp;
+                float w = power_heuristic(lr[k].lpdf, sp);
+                f3 a = mulv(lr[k].contrib, t.mult) * w;
+                float* acc = &res[(size_t)t.sid * 3];
+                acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+            }
+            ref_rl_finish_q(T.Q.data(), T.newQ.data(), T.cnt.data(), T.total.data(), T.E, alpha_scale);
+            ref_rl_finish_light(T.ldist.data(), T.lres.data(), T.lcount.data(), T.lnew.data(), T.lnewc.data(), T.E, L);
+            (void)any_valid;
+            mx = 0;
+            for (auto& h : hits) mx = std::max(mx, h.t.dim);
+            for (auto& h : hits) h.t.dim = mx;  // syncDimension
+            // genNextRay (RL .cu:305-375)
+            for (size_t k = 0; k < hits.size(); ++k) {
+                Hit& h = hits[k];
+                Task t = h.t;
+                if (depth > 3) {
+                    float qq = 1;
+                    qq = fminr(qq, 1 - t.mult.x);
+                    qq = fminr(qq, 1 - t.mult.y);
+                    qq = fminr(qq, 1 - t.mult.z);
+                    qq = fmaxr(qq, 0.05f);
+                    if (H.rand1(t.sid, t.dim) < qq) continue;
+                    t.mult = t.mult / (1.f - qq);
+                }
+                const int mesh = tri_mesh[h.prim];
+                f3 n = tri_normal(s, h.prim);
+                f3 pos = t.ro + h.dist * t.rd;
+                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                f3 t0, t1;
+                coordinate_system(n, t0, t1);
+                f3 wo = t.rd * -1.f;
+                const int entry = find_entry(scene6, T.g, pos);
+                f3 f, nd;
+                float npdf;
+                int cell;
+                if (bsdf.is_delta() || !ni[k].valid) {
+                    float ux = H.rand1(t.sid, t.dim), uy = H.rand1(t.sid, t.dim);
+                    f3 wl;
+                    npdf = 0;
+                    f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, n), npdf);
+                    nd = to_world(wl, t0, t1, n);
+                    cell = dir_to_cell(nd);
+                } else {
+                    cell = ni[k].cell;
+                    npdf = ni[k].pdf;
+                    nd = ni[k].dir;
+                    f = bsdf.eval(to_local(nd, t0, t1, n), to_local(wo, t0, t1, n));
+                }
+                f3 no = pos + nd * 0.001f;
+                if (all_zero(f)) t.mult = mk(0, 0, 0);
+                else t.mult = mulv(t.mult, f) * std::fabs(dot(nd, n)) / npdf;
+                t.ro = no;
+                t.rd = nd;
+                t.pdf = npdf;
+                t.delta = bsdf.is_delta();
+                t.prev_entry = entry;
+                t.prev_cell = cell;
+                q.push_back(t);
             }
         }
-        // accumulateLightingContribs (RL .cu:609-647)
-        for (size_t k = 0; k < hits.size(); ++k) {
-            Hit& h = hits[k];
-            Task& t = h.t;
-            const int mesh = tri_mesh[h.prim];
-            f3 n = tri_normal(s, h.prim);
-            Bsdf bsdf = make_bsdf(meshes[mesh]);
-            f3 t0, t1;
-            coordinate_system(n, t0, t1);
-            f3 il = to_local(lr[k].dir, t0, t1, n), ol = to_local(t.rd * -1.f, t0, t1, n);
-            float mp = bsdf.pdf(il, ol);
-            float sp = 0;
-            if (qr[k].sum != 0) sp = qr[k].pdfq;
-            if (qr[k].sum == 0 || sp == 0 || bsdf.is_delta()) sp = mp;
-            float w = power_heuristic(lr[k].lpdf, sp);
-            f3 a = mulv(lr[k].contrib, t.mult) * w;
-            float* acc = &res[(size_t)t.sid * 3];
-            acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
-        }
-        ref_rl_finish_q(T.Q.data(), T.newQ.data(), T.cnt.data(), T.total.data(), T.E, alpha_scale);
-        ref_rl_finish_light(T.ldist.data(), T.lres.data(), T.lcount.data(), T.lnew.data(), T.lnewc.data(), T.E, L);
-        (void)any_valid;
-        mx = 0;
-        for (auto& h : hits) mx = std::max(mx, h.t.dim);
-        for (auto& h : hits) h.t.dim = mx;  // syncDimension
-        // genNextRay (RL .cu:305-375)
-        for (size_t k = 0; k < hits.size(); ++k) {
-            Hit& h = hits[k];
-            Task t = h.t;
-            if (depth > 3) {
-                float qq = 1;
-                qq = fminr(qq, 1 - t.mult.x);
-                qq = fminr(qq, 1 - t.mult.y);
-                qq = fminr(qq, 1 - t.mult.z);
-                qq = fmaxr(qq, 0.05f);
-                if (H.rand1(t.sid, t.dim) < qq) continue;
-                t.mult = t.mult / (1.f - qq);
-            }
-            const int mesh = tri_mesh[h.prim];
-            f3 n = tri_normal(s, h.prim);
-            f3 pos = t.ro + h.dist * t.rd;
-            Bsdf bsdf = make_bsdf(meshes[mesh]);
-            f3 t0, t1;
-            coordinate_system(n, t0, t1);
-            f3 wo = t.rd * -1.f;
-            const int entry = find_entry(scene6, T.g, pos);
-            f3 f, nd;
-            float npdf;
-            int cell;
-            if (bsdf.is_delta() || !ni[k].valid) {
-                float ux = H.rand1(t.sid, t.dim), uy = H.rand1(t.sid, t.dim);
-                f3 wl;
-                npdf = 0;
-                f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, n), npdf);
-                nd = to_world(wl, t0, t1, n);
-                cell = dir_to_cell(nd);
-            } else {
-                cell = ni[k].cell;
-                npdf = ni[k].pdf;
-                nd = ni[k].dir;
-                f = bsdf.eval(to_local(nd, t0, t1, n), to_local(wo, t0, t1, n));
-            }
-            f3 no = pos + nd * 0.001f;
-            if (all_zero(f)) t.mult = mk(0, 0, 0);
-            else t.mult = mulv(t.mult, f) * std::fabs(dot(nd, n)) / npdf;
-            t.ro = no;
-            t.rd = nd;
-            t.pdf = npdf;
-            t.delta = bsdf.is_delta();
-            t.prev_entry = entry;
-            t.prev_cell = cell;
-            q.push_back(t);
-        }
-    }
+    }  // rounds
 
 #pragma omp parallel for schedule(static)
     for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {

@@ -52,3 +52,22 @@ def test_render_errors(ctx):
         R.render(ctx, scene, 0)
     with pytest.raises(CavError):
         R.render(ctx, scene, 1, max_depth=200)
+
+
+def test_path_rounds_equal_single_round_oracle(ctx):
+    """spp rounds (sppBySpp order) keep each sample's Halton index, so the
+    film does not depend on the split."""
+    print("[render] cornell matte spp=5 in rounds of 2", flush=True)
+    sc = scenes.cornell_box(seed=5, width=40, height=30)
+    cam = for_scene(sc)
+    rgb, w, rays = oracle.render_path(sc, 5, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 5, camera=cam, spp_per_round=2)
+    assert st["rounds"] == 3
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    film1, st1 = R.render(ctx, scene, 5, camera=cam)
+    assert st1["rounds"] == 1
+    np.testing.assert_array_equal(film1.rgb, rgb)
+    scene.close()

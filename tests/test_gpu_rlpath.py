@@ -93,3 +93,22 @@ def test_rlpath_errors(ctx):
     with pytest.raises(ValueError):
         R.render(ctx, scene, 1, integrator="bdpt")
     scene.close()
+
+
+def test_rlpath_rounds_match_oracle_rounds(ctx):
+    """Learning carries over from round to round, on the GPU as in the oracle."""
+    print("[rlpath] spp=8 in rounds of 2", flush=True)
+    sc = scenes.cornell_box(seed=6, width=48, height=40)
+    cam = for_scene(sc)
+    rgb, w, _ = oracle.render_rlpath(sc, 8, cam, spp_per_round=2)
+    ora = rgb / np.maximum(w, 1e-30)[..., None]
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 8, integrator="rlpath", camera=cam, spp_per_round=2)
+    assert st["rounds"] == 4
+    np.testing.assert_array_equal(film.weight, w)
+    err = rel_mse(film.image, ora)
+    print(f"[rlpath] rounds: gpu-vs-oracle rel_mse={err:.3e} mean gpu={float(film.image.mean()):.5f} "
+          f"oracle={float(ora.mean()):.5f}", flush=True)
+    assert abs(float(film.image.mean()) - float(ora.mean())) < 0.01 * float(ora.mean())
+    assert err < 1e-3
+    scene.close()
