@@ -84,6 +84,11 @@ _SIGS = {
     "cav_scene_bvh": ([P], P),
     "cav_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
                     ctypes.POINTER(RenderStats)], I),
+    "cav_camera_from_matrix": ([P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
+    "cav_pbrt_parse": ([ctypes.c_char_p, P, ctypes.POINTER(P), ctypes.c_char_p, I], I),
+    "cav_pbrt_get": ([P, P, P, P, P], I),
+    "cav_pbrt_destroy": ([P], None),
+    "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
     "cav_film_to_srgb8": ([P, P, I, I, P], I),
     "cav_write_png": ([ctypes.c_char_p, P, I, I], I),
     "cav_write_pfm": ([ctypes.c_char_p, P, P, I, I], I),

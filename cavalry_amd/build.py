@@ -27,6 +27,7 @@ SOURCES = [
     "rlpath.hip",
     "render_capi.cpp",
     "film_io.cpp",
+    "pbrt.cpp",
     "capi.cpp",
     "camera.cpp",
     "ctx_release.cpp",
@@ -64,8 +65,17 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line renderer, linked against the library next to it
+    cli = os.path.join(OUT_DIR, "cavalry")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "host", "main.cpp"),
+           "-I" + os.path.join(os.path.dirname(HERE), "include"), "-L" + OUT_DIR, "-lcavalry_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", cli + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(cli + ".tmp", cli)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {cli}")
     return LIB
 
 

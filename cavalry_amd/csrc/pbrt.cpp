@@ -1,0 +1,832 @@
+// PBRT scene front end on the host: lexer, parser, shape builders and PLY
+// reader, producing the triangle soup + mesh table that cav_scene_create
+// takes, the camera and the render settings (cav_pbrt_*,
+// cav_scene_create_from_pbrt).
+//
+// Reference: SceneLoading/Lexing.cu (tokens, recognised keywords),
+// SceneLoading/Parsing.cu (scene-wide options, subsections, Include,
+// ReverseOrientation, AreaLightSource binding, named materials), the factories
+// TriangleMesh::createFromParams / createFromPLY (Core/TriangleMesh.cu:52-183),
+// ShapeMesh sphere / disk (ShapeMesh/Sphere.cu, Disk.cu), the material,
+// light, camera, sampler and integrator factories (Materials/*.h,
+// Lights/LightObject.h:89-125, Cameras/PerspectiveCamera.cu:22-26,
+// Samplers/SamplerObject.h:157-168, Integrators/CreateIntegrator.cu:10-25) and
+// the CLI overrides of main.cu:25-97.
+//
+// Differences, each reported through cav_render_setup::warnings and the
+// message list: the device BSDFs cover matte / plastic / mirror / specular
+// glass, so metal, substrate and rough glass are replaced (by matte with the
+// material's Kd, and by specular glass); image textures, non-area lights and
+// the environment map are ignored; the "random" sampler renders with Halton;
+// "directlighting" renders with path at depth 1. Unlike the reference, which
+// exits, every error returns CAV_ERR_PARSE with a message.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/cavalry.h"
+#include "capi_internal.h"
+#include "mat4.h"
+
+using namespace cav::m4;
+
+namespace {
+
+struct ParseError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// ------------------------------------------------------------------ lexing
+enum class Tok { Keyword, String, Num, LBracket, RBracket };
+
+struct Token {
+    Tok type;
+    std::string text;  // keyword or string contents
+    float num = 0.f;
+};
+
+const std::unordered_set<std::string>& keywords() {
+    static const std::unordered_set<std::string> k = {
+        "ReverseOrientation", "Material", "MakeNamedMedium", "MediumInterface", "NamedMedium",
+        "Include", "PixelFilter", "Transform", "NamedMaterial", "TransformBegin", "TransformEnd",
+        "AreaLightSource", "MakeNamedMaterial", "Texture", "Scale", "Rotate", "Translate",
+        "LightSource", "LookAt", "Camera", "Film", "WorldBegin", "WorldEnd", "AttributeBegin",
+        "AttributeEnd", "Sampler", "Shape", "Integrator"};
+    return k;
+}
+
+bool is_letter(char c) { return ('a' <= c && c <= 'z') || ('A' <= c && c <= 'Z'); }
+bool is_digit(char c) { return '0' <= c && c <= '9'; }
+
+std::string read_text(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw ParseError("cannot open " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+// SceneLoading/Lexing.cu:8-101. A number is a run of digits, '-', '.', 'e'
+// (also 'E' and '+' here, so exponents like 1e+05 lex); a leading '.' reads
+// as "0.". Carriage returns count as white space.
+std::vector<Token> lex(const std::string& path) {
+    const std::string in = read_text(path);
+    std::vector<Token> out;
+    size_t pos = 0;
+    while (pos < in.size()) {
+        const char c = in[pos];
+        if (c == ' ' || c == '\n' || c == '\t' || c == '\r') {
+            ++pos;
+        } else if (c == '#') {
+            while (pos < in.size() && in[pos] != '\n') ++pos;
+            ++pos;
+        } else if (c == '[') {
+            out.push_back({Tok::LBracket, "["});
+            ++pos;
+        } else if (c == ']') {
+            out.push_back({Tok::RBracket, "]"});
+            ++pos;
+        } else if (is_letter(c)) {
+            std::string w;
+            while (pos < in.size() && is_letter(in[pos])) w += in[pos++];
+            if (!keywords().count(w)) throw ParseError("Unrecognized Keyword: " + w + " in " + path);
+            out.push_back({Tok::Keyword, w});
+        } else if (c == '"') {
+            ++pos;
+            std::string s;
+            while (pos < in.size() && in[pos] != '"') s += in[pos++];
+            if (pos == in.size()) throw ParseError("Lexing Error: Didn't find ending quotation in " + path);
+            ++pos;
+            out.push_back({Tok::String, s});
+        } else if (is_digit(c) || c == '-' || c == '.') {
+            std::string s;
+            while (pos < in.size() && (is_digit(in[pos]) || in[pos] == '-' || in[pos] == '.' || in[pos] == 'e' ||
+                                       in[pos] == 'E' || in[pos] == '+'))
+                s += in[pos++];
+            if (s[0] == '.') s = "0" + s;
+            Token t{Tok::Num, s};
+            try {
+                size_t used = 0;
+                t.num = std::stof(s, &used);
+                if (used != s.size()) throw std::invalid_argument(s);
+            } catch (const std::exception&) {
+                throw ParseError("Lexing Error: bad number '" + s + "' in " + path);
+            }
+            out.push_back(t);
+        } else {
+            throw ParseError(std::string("unrecognized character '") + c + "' in " + path);
+        }
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ params
+struct Params {
+    std::unordered_map<std::string, std::string> strings;
+    std::unordered_map<std::string, std::vector<std::string>> string_lists;
+    std::unordered_map<std::string, float> nums;
+    std::unordered_map<std::string, std::vector<float>> num_lists;
+
+    // Core/Parameters.h:16-70: a one-element list also answers the scalar
+    // queries, with its first element
+    bool has_num(const std::string& k) const { return nums.count(k) || num_lists.count(k); }
+    float num(const std::string& k) const {
+        if (nums.count(k)) return nums.at(k);
+        if (num_lists.count(k) && !num_lists.at(k).empty()) return num_lists.at(k)[0];
+        throw ParseError("Params Num field not found: " + k);
+    }
+    bool has_string(const std::string& k) const { return strings.count(k) || string_lists.count(k); }
+    std::string str(const std::string& k) const {
+        if (strings.count(k)) return strings.at(k);
+        if (string_lists.count(k) && !string_lists.at(k).empty()) return string_lists.at(k)[0];
+        throw ParseError("Params String field not found: " + k);
+    }
+    bool has_list(const std::string& k) const { return num_lists.count(k) > 0; }
+    const std::vector<float>& list(const std::string& k) const {
+        if (!num_lists.count(k)) throw ParseError("Params NumList field not found: " + k);
+        return num_lists.at(k);
+    }
+};
+
+struct ObjectDef {
+    std::string keyword, name;
+    Params params;
+    bool defined = false;
+};
+
+struct Buf {
+    std::vector<Token> toks;
+    size_t i = 0;
+    const Token& peek(size_t off = 0) const {
+        if (i + off >= toks.size()) throw ParseError("unexpected end of scene file");
+        return toks[i + off];
+    }
+    const Token& pop(Tok t, const char* what) {
+        const Token& k = peek();
+        if (k.type != t)
+            throw ParseError(std::string("Parsing Error: ") + what + " expected at token " + std::to_string(i) +
+                             ": " + k.text);
+        ++i;
+        return k;
+    }
+};
+
+std::vector<std::string> split_spaces(const std::string& s) {
+    std::vector<std::string> r;
+    size_t a = 0, b;
+    while ((b = s.find(' ', a)) != std::string::npos) {
+        r.push_back(s.substr(a, b - a));
+        a = b + 1;
+    }
+    r.push_back(s.substr(a));
+    return r;
+}
+
+std::vector<float> read_num_list(Buf& b) {
+    std::vector<float> r;
+    b.pop(Tok::LBracket, "[");
+    while (b.peek().type == Tok::Num) r.push_back(b.pop(Tok::Num, "number").num);
+    b.pop(Tok::RBracket, "]");
+    return r;
+}
+
+std::vector<std::string> read_string_list(Buf& b) {
+    std::vector<std::string> r;
+    b.pop(Tok::LBracket, "[");
+    while (b.peek().type == Tok::String) r.push_back(b.pop(Tok::String, "string").text);
+    b.pop(Tok::RBracket, "]");
+    return r;
+}
+
+// Parsing.cu:66-110: Keyword "name" then "type field" value pairs; the field
+// is the last word of its declaration, the object name the first word.
+ObjectDef read_object(Buf& b) {
+    ObjectDef d;
+    d.keyword = b.pop(Tok::Keyword, "keyword").text;
+    d.name = split_spaces(b.pop(Tok::String, "object name").text)[0];
+    while (b.i < b.toks.size() && b.peek().type == Tok::String) {
+        const std::vector<std::string> w = split_spaces(b.pop(Tok::String, "field").text);
+        const std::string field = w.back();
+        const Token& v = b.peek();
+        if (v.type == Tok::String) {
+            d.params.strings[field] = b.pop(Tok::String, "string").text;
+        } else if (v.type == Tok::Num) {
+            d.params.nums[field] = b.pop(Tok::Num, "number").num;
+        } else if (v.type == Tok::LBracket) {
+            const Token& n = b.peek(1);
+            if (n.type == Tok::Num || n.type == Tok::RBracket) d.params.num_lists[field] = read_num_list(b);
+            else if (n.type == Tok::String) d.params.string_lists[field] = read_string_list(b);
+            else throw ParseError("Unaccepted List Element at token " + std::to_string(b.i + 1));
+        } else {
+            throw ParseError("Read Object failed at token " + std::to_string(b.i) + ": " + v.text);
+        }
+    }
+    d.defined = true;
+    return d;
+}
+
+// skips an unsupported directive and its arguments
+void skip_object(Buf& b) {
+    b.pop(Tok::Keyword, "keyword");
+    while (b.i < b.toks.size() && b.peek().type != Tok::Keyword) ++b.i;
+}
+
+// Parsing.cu:119-169. Returns false when the next keyword is not a transform.
+bool read_transform(Buf& b, M4& t) {
+    const Token& k = b.peek();
+    if (k.type != Tok::Keyword) throw ParseError("Parsing Error: Keyword expected at token " + std::to_string(b.i));
+    auto num = [&]() { return b.pop(Tok::Num, "number").num; };
+    if (k.text == "Translate") {
+        ++b.i;
+        const float x = num(), y = num(), z = num();
+        t = translate(t, {x, y, z});
+        return true;
+    }
+    if (k.text == "Rotate") {
+        ++b.i;
+        const float a = num(), x = num(), y = num(), z = num();
+        t = rotate(t, radians(a), {x, y, z});
+        return true;
+    }
+    if (k.text == "Scale") {
+        ++b.i;
+        const float x = num(), y = num(), z = num();
+        t = scale(t, {x, y, z});
+        return true;
+    }
+    if (k.text == "Transform") {
+        ++b.i;
+        const std::vector<float> v = read_num_list(b);
+        if (v.size() != 16) throw ParseError("Transform needs 16 numbers");
+        M4 m;
+        for (int j = 0; j < 16; ++j) m.m[j] = v[j];
+        t = mul(m, t);  // pre-multiplied, as the reference does
+        return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------ scene
+struct Mesh {
+    std::vector<V3> pos;
+    std::vector<int> idx;  // 3 per triangle
+};
+
+struct Material {
+    cav_mesh m{};  // material fields only
+};
+
+struct Builder {
+    std::vector<float> soup;
+    std::vector<cav_mesh> meshes;
+    std::vector<std::string> messages;
+    int warnings = 0;
+    std::unordered_map<std::string, Material> named;
+    std::unordered_set<std::string> textures;
+    // pending AreaLightSource (binds to the next shape, across subsections)
+    bool next_is_light = false;
+    float light_L[3] = {1, 1, 1};
+
+    void warn(const std::string& s) {
+        ++warnings;
+        messages.push_back(s);
+    }
+};
+
+Material default_matte() {  // Parsing.cu:290-292: MatteMaterial(1, 1, 1)
+    Material r;
+    r.m.material = CAV_MAT_MATTE;
+    r.m.texture = -1;
+    for (int c = 0; c < 3; ++c) r.m.kd[c] = 1.f;
+    r.m.ior = 1.5f;
+    return r;
+}
+
+// GGX::roughnessToAlpha (BSDFs/MicrofacetDistribution.h:57-62)
+float roughness_to_alpha(float r) {
+    r = std::fmax(r, 1e-3f);
+    const float x = std::log(r);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
+void rgb_param(const Params& p, const char* k, float* out, Builder& bd, const std::string& what) {
+    if (p.has_list(k)) {
+        const auto& v = p.list(k);
+        if (v.size() < 3) throw ParseError(what + ": " + k + " needs 3 values");
+        for (int c = 0; c < 3; ++c) out[c] = v[c];
+    } else if (p.strings.count(k)) {
+        bd.warn(what + ": texture '" + p.strings.at(k) + "' for " + k + " is not supported; using the constant");
+    }
+}
+
+// MaterialObject::createFromObjectDefinition (Materials/MaterialObject.h:82-104)
+Material make_material(const Params& p, Builder& bd) {
+    const std::string type = p.str("type");
+    Material r = default_matte();
+    cav_mesh& m = r.m;
+    if (type == "matte") {  // MatteMaterial.h:41-52
+        rgb_param(p, "Kd", m.kd, bd, "matte");
+    } else if (type == "plastic") {  // Plastic.h:58-85
+        m.material = CAV_MAT_PLASTIC;
+        float rough = 0.1f;
+        bool remap = true;
+        if (p.has_num("roughness")) rough = p.num("roughness");
+        if (p.has_string("remaproughness")) remap = p.str("remaproughness") == "true";
+        for (int c = 0; c < 3; ++c) m.kd[c] = m.ks[c] = 0.25f;
+        rgb_param(p, "Kd", m.kd, bd, "plastic");
+        rgb_param(p, "Ks", m.ks, bd, "plastic");
+        m.roughness = remap ? roughness_to_alpha(rough) : rough;
+    } else if (type == "mirror") {  // Mirror.h:32-38
+        m.material = CAV_MAT_MIRROR;
+        const float kr = p.has_num("Kr") ? p.num("Kr") : 0.9f;
+        for (int c = 0; c < 3; ++c) m.kr[c] = kr;
+    } else if (type == "glass") {  // Glass.h:44-82
+        m.material = CAV_MAT_GLASS;
+        for (int c = 0; c < 3; ++c) m.kr[c] = m.kt[c] = 1.f;
+        rgb_param(p, "Kr", m.kr, bd, "glass");
+        rgb_param(p, "Kt", m.kt, bd, "glass");
+        m.ior = 1.5f;
+        if (p.has_num("eta")) m.ior = p.num("eta");
+        else if (p.has_num("index")) m.ior = p.num("index");
+        float ur = 0.f, vr = 0.f;
+        if (p.has_num("uroughness")) ur = p.num("uroughness");
+        if (p.has_num("vroughness")) vr = p.num("vroughness");
+        if (p.has_num("roughness")) ur = vr = p.num("roughness");
+        if (ur != 0.f || vr != 0.f) bd.warn("glass: rough glass is rendered as specular glass");
+    } else if (type == "metal" || type == "substrate") {
+        rgb_param(p, "Kd", m.kd, bd, type);
+        bd.warn(type + ": not supported by the device BSDFs; rendered as matte");
+    } else {
+        bd.warn("unsupported material type: " + type + "; using matte");
+        rgb_param(p, "Kd", m.kd, bd, type);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- PLY
+// Minimal reader for the element / property layouts of pbrt scenes: ASCII or
+// binary little-endian, scalar properties of any PLY type, one list property
+// on faces. TriangleMesh::createFromPLY keeps the first three indices of
+// every face.
+struct PlyProp {
+    std::string name, type, count_type;  // count_type empty unless a list
+};
+struct PlyElem {
+    std::string name;
+    int64_t count = 0;
+    std::vector<PlyProp> props;
+};
+
+int ply_size(const std::string& t) {
+    if (t == "char" || t == "uchar" || t == "int8" || t == "uint8") return 1;
+    if (t == "short" || t == "ushort" || t == "int16" || t == "uint16") return 2;
+    if (t == "int" || t == "uint" || t == "float" || t == "int32" || t == "uint32" || t == "float32") return 4;
+    if (t == "double" || t == "float64") return 8;
+    throw ParseError("PLY: unknown type " + t);
+}
+
+double ply_bin(const std::string& t, const unsigned char* p) {
+    if (t == "char" || t == "int8") return (double)*(const int8_t*)p;
+    if (t == "uchar" || t == "uint8") return (double)*p;
+    int16_t s; uint16_t us; int32_t i; uint32_t u; float f; double d;
+    if (t == "short" || t == "int16") { std::memcpy(&s, p, 2); return s; }
+    if (t == "ushort" || t == "uint16") { std::memcpy(&us, p, 2); return us; }
+    if (t == "int" || t == "int32") { std::memcpy(&i, p, 4); return i; }
+    if (t == "uint" || t == "uint32") { std::memcpy(&u, p, 4); return u; }
+    if (t == "float" || t == "float32") { std::memcpy(&f, p, 4); return f; }
+    std::memcpy(&d, p, 8);
+    return d;
+}
+
+Mesh read_ply(const std::string& path, const M4& t) {
+    const std::string data = read_text(path);
+    size_t pos = 0;
+    auto line = [&]() {
+        const size_t e = data.find('\n', pos);
+        if (e == std::string::npos) throw ParseError("PLY: truncated header in " + path);
+        std::string l = data.substr(pos, e - pos);
+        if (!l.empty() && l.back() == '\r') l.pop_back();
+        pos = e + 1;
+        return l;
+    };
+    if (line() != "ply") throw ParseError("PLY: bad magic in " + path);
+    std::string format;
+    std::vector<PlyElem> elems;
+    for (;;) {
+        std::istringstream ls(line());
+        std::string w;
+        ls >> w;
+        if (w == "format") ls >> format;
+        else if (w == "element") {
+            PlyElem e;
+            ls >> e.name >> e.count;
+            elems.push_back(e);
+        } else if (w == "property") {
+            if (elems.empty()) throw ParseError("PLY: property before element in " + path);
+            PlyProp p;
+            std::string ty;
+            ls >> ty;
+            if (ty == "list") ls >> p.count_type >> p.type >> p.name;
+            else { p.type = ty; ls >> p.name; }
+            elems.back().props.push_back(p);
+        } else if (w == "end_header") break;
+    }
+    const bool ascii = format == "ascii";
+    if (!ascii && format != "binary_little_endian") throw ParseError("PLY: unsupported format " + format);
+    Mesh m;
+    std::istringstream as(ascii ? data.substr(pos) : std::string());
+    const unsigned char* bp = (const unsigned char*)data.data() + pos;
+    const unsigned char* be = (const unsigned char*)data.data() + data.size();
+    auto next = [&](const std::string& ty) -> double {
+        if (ascii) {
+            double v;
+            if (!(as >> v)) throw ParseError("PLY: truncated body in " + path);
+            return v;
+        }
+        const int n = ply_size(ty);
+        if (bp + n > be) throw ParseError("PLY: truncated body in " + path);
+        const double v = ply_bin(ty, bp);
+        bp += n;
+        return v;
+    };
+    for (const PlyElem& e : elems) {
+        int ix = -1, iy = -1, iz = -1, ivi = -1;
+        for (size_t k = 0; k < e.props.size(); ++k) {
+            if (e.props[k].name == "x") ix = (int)k;
+            if (e.props[k].name == "y") iy = (int)k;
+            if (e.props[k].name == "z") iz = (int)k;
+            if (e.props[k].name == "vertex_indices" || e.props[k].name == "vertex_index") ivi = (int)k;
+        }
+        std::vector<double> vals(e.props.size());
+        for (int64_t r = 0; r < e.count; ++r) {
+            for (size_t k = 0; k < e.props.size(); ++k) {
+                const PlyProp& p = e.props[k];
+                if (p.count_type.empty()) {
+                    vals[k] = next(p.type);
+                } else {
+                    const int64_t n = (int64_t)next(p.count_type);
+                    std::vector<int> li((size_t)std::max<int64_t>(n, 0));
+                    for (int64_t q = 0; q < n; ++q) li[(size_t)q] = (int)next(p.type);
+                    if (e.name == "face" && (int)k == ivi) {
+                        if (n < 3) throw ParseError("PLY: face with fewer than 3 vertices in " + path);
+                        m.idx.insert(m.idx.end(), li.begin(), li.begin() + 3);
+                    }
+                }
+            }
+            if (e.name == "vertex") {
+                if (ix < 0 || iy < 0 || iz < 0) throw ParseError("PLY: vertex without x, y, z in " + path);
+                m.pos.push_back(apply(t, {(float)vals[ix], (float)vals[iy], (float)vals[iz]}));
+            }
+        }
+    }
+    return m;
+}
+
+// ShapeMesh/Sphere.cu: octahedron, four midpoint subdivisions, scaled by the
+// radius then transformed
+Mesh sphere_mesh(float radius, const M4& t) {
+    std::vector<V3> p = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    std::vector<int> cur = {0, 2, 4, 0, 2, 5, 0, 3, 4, 0, 3, 5, 1, 2, 4, 1, 2, 5, 1, 3, 4, 1, 3, 5};
+    auto mid = [&](int a, int b) {
+        const V3 s{(p[a].x + p[b].x) * 0.5f, (p[a].y + p[b].y) * 0.5f, (p[a].z + p[b].z) * 0.5f};
+        p.push_back(normalize(s));
+        return (int)p.size() - 1;
+    };
+    for (int it = 0; it < 4; ++it) {
+        std::vector<int> nxt;
+        for (size_t k = 0; k < cur.size(); k += 3) {
+            const int i0 = cur[k], i1 = cur[k + 1], i2 = cur[k + 2];
+            const int i01 = mid(i0, i1), i02 = mid(i0, i2), i12 = mid(i1, i2);
+            nxt.insert(nxt.end(), {i01, i02, i12, i0, i01, i02, i1, i01, i12, i2, i02, i12});
+        }
+        cur.swap(nxt);
+    }
+    Mesh m;
+    for (const V3& v : p) m.pos.push_back(apply(t, {v.x * radius, v.y * radius, v.z * radius}));
+    m.idx = cur;
+    return m;
+}
+
+// ShapeMesh/Disk.cu: 360 segments around the centre at the given height
+Mesh disk_mesh(float radius, float height, const M4& t) {
+    const int seg = 360;
+    Mesh m;
+    for (int s = 0; s < seg; ++s) {
+        const float th = (float)(2.f * M_PI * (double)((float)s / (float)seg));
+        m.pos.push_back(apply(t, {std::sin(th) * radius, std::cos(th) * radius, height}));
+    }
+    m.pos.push_back(apply(t, {0, 0, height}));
+    for (int s = 0; s < seg; ++s) m.idx.insert(m.idx.end(), {(s + 1) % seg, s, seg});
+    return m;
+}
+
+std::string join(const std::string& base, const std::string& rel) {
+    if (!rel.empty() && rel[0] == '/') return rel;
+    if (base.empty()) return rel;
+    return base + "/" + rel;
+}
+
+std::string dir_of(const std::string& path) {
+    const size_t s = path.find_last_of('/');
+    return s == std::string::npos ? std::string() : path.substr(0, s);
+}
+
+// TriangleMesh::createFromObjectDefinition (Core/TriangleMesh.cu:111-183)
+Mesh make_shape(const ObjectDef& d, const M4& t, const std::string& base) {
+    if (d.name == "trianglemesh" || d.name == "plymesh") {
+        if (d.params.has_string("filename")) return read_ply(join(base, d.params.str("filename")), t);
+        if (d.params.has_list("P") && d.params.has_list("indices")) {
+            const auto& P = d.params.list("P");
+            const auto& I = d.params.list("indices");
+            Mesh m;
+            for (size_t k = 0; k + 2 < P.size(); k += 3) m.pos.push_back(apply(t, {P[k], P[k + 1], P[k + 2]}));
+            for (size_t k = 0; k + 2 < I.size(); k += 3)
+                m.idx.insert(m.idx.end(), {(int)I[k], (int)I[k + 1], (int)I[k + 2]});
+            return m;
+        }
+        throw ParseError("No valid inputs for triangle mesh");
+    }
+    if (d.name == "sphere") return sphere_mesh(d.params.num("radius"), t);
+    if (d.name == "disk") return disk_mesh(d.params.num("radius"), d.params.has_num("height") ? d.params.num("height") : 0.f, t);
+    throw ParseError("ShapeMesh type not supported: " + d.name);
+}
+
+struct Setup {
+    ObjectDef camera, film, integrator, sampler, filter;
+    M4 view = identity();
+};
+
+// Parsing.cu:172-268
+void parse_options(Buf& b, Setup& su, const cav_pbrt_overrides* ov, Builder& bd) {
+    su.filter.defined = true;
+    su.filter.name = "triangle";
+    for (;;) {
+        const Token& k = b.peek();
+        if (k.type != Tok::Keyword)
+            throw ParseError("Parsing Error: Keyword expected at token " + std::to_string(b.i) + ": " + k.text);
+        if (k.text == "WorldBegin") break;
+        if (k.text == "LookAt") {
+            ++b.i;
+            float v[9];
+            for (float& x : v) x = b.pop(Tok::Num, "number").num;
+            V3 eye = apply(su.view, {v[0], v[1], v[2]});
+            V3 center = apply(su.view, {v[3], v[4], v[5]});
+            V3 up = apply3(su.view, {v[6], v[7], v[8]});
+            su.view = mul(look_at_lh(eye, center, up), su.view);
+        } else if (k.text == "Camera") {
+            su.camera = read_object(b);
+        } else if (k.text == "Film") {
+            su.film = read_object(b);
+            if (ov && ov->output && ov->output[0]) su.film.params.strings["filename"] = ov->output;
+        } else if (k.text == "Sampler") {
+            su.sampler = read_object(b);
+            if (ov && ov->spp > 0) su.sampler.params.nums["pixelsamples"] = (float)ov->spp;
+        } else if (read_transform(b, su.view)) {
+        } else if (k.text == "Integrator") {
+            su.integrator = read_object(b);
+            if (ov && ov->integrator && ov->integrator[0]) su.integrator.name = ov->integrator;
+        } else if (k.text == "PixelFilter") {
+            su.filter = read_object(b);
+        } else {
+            bd.warn("ignored scene-wide directive " + k.text);
+            skip_object(b);
+        }
+    }
+    if (!(su.camera.defined && su.film.defined && su.integrator.defined && su.sampler.defined))
+        throw ParseError("incomplete scene-wide options: requires Camera, Film, Integrator, Sampler, PixelFilter");
+}
+
+struct State {
+    M4 t;
+    std::shared_ptr<Material> mat;  // null: default matte
+};
+
+void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
+    cav_mesh c = mat.m;
+    const int64_t first = (int64_t)bd.soup.size() / 9;
+    const int64_t nv = (int64_t)m.pos.size();
+    int ntris = 0;
+    for (size_t k = 0; k + 2 < m.idx.size(); k += 3) {
+        for (int j = 0; j < 3; ++j) {
+            const int vi = m.idx[k + j];
+            if (vi < 0 || vi >= nv) throw ParseError("triangle index out of range");
+            bd.soup.insert(bd.soup.end(), {m.pos[vi].x, m.pos[vi].y, m.pos[vi].z});
+        }
+        ++ntris;
+    }
+    if (first + ntris > INT32_MAX) throw ParseError("more than 2^31 triangles");
+    c.first_tri = (int32_t)first;
+    c.num_tris = ntris;
+    c.reverse_orientation = reverse ? 1 : 0;
+    c.is_light = 0;
+    if (bd.next_is_light) {
+        c.is_light = 1;
+        for (int q = 0; q < 3; ++q) c.emission[q] = bd.light_L[q];
+        bd.next_is_light = false;
+    } else {
+        for (int q = 0; q < 3; ++q) c.emission[q] = 0.f;
+    }
+    if (ntris == 0) {
+        bd.warn("shape without triangles skipped");
+        return;
+    }
+    bd.meshes.push_back(c);
+}
+
+// Parsing.cu:272-388: one XBegin ... XEnd section
+void parse_section(Buf& b, State st, const std::string& base, Builder& bd) {
+    const Token& begin = b.pop(Tok::Keyword, "XBegin");
+    const std::string& w = begin.text;
+    if (w.size() < 5 || w.compare(w.size() - 5, 5, "Begin") != 0)
+        throw ParseError("Parsing Error: XXXBegin expected at token " + std::to_string(b.i - 1) + ": " + w);
+    const std::string name = w.substr(0, w.size() - 5);
+    bool reverse = false;
+    for (;;) {
+        const Token& k = b.peek();
+        if (k.type != Tok::Keyword)
+            throw ParseError("Parsing Error: Keyword expected at token " + std::to_string(b.i) + ": " + k.text);
+        const std::string& kw = k.text;
+        if (kw.size() >= 3 && kw.compare(kw.size() - 3, 3, "End") == 0) {
+            if (kw.substr(0, kw.size() - 3) != name)
+                throw ParseError("Parsing Error: Mismatching subsection names " + name + " / " + kw);
+            break;
+        } else if (kw.size() >= 5 && kw.compare(kw.size() - 5, 5, "Begin") == 0) {
+            parse_section(b, st, base, bd);
+        } else if (kw == "Shape") {
+            const ObjectDef d = read_object(b);
+            const Mesh m = make_shape(d, st.t, base);
+            add_shape(bd, m, st.mat ? *st.mat : default_matte(), reverse);
+        } else if (kw == "AreaLightSource") {
+            const ObjectDef d = read_object(b);
+            float L[3] = {1, 1, 1};
+            const char* key = d.params.has_list("L") ? "L" : (d.params.has_list("l") ? "l" : nullptr);
+            if (key) {
+                const auto& v = d.params.list(key);
+                for (int q = 0; q < 3; ++q) L[q] = v.size() >= 3 ? v[q] : (v.empty() ? 1.f : v[0]);
+            }
+            if (d.name == "diffuse") {
+                bd.next_is_light = true;
+                for (int q = 0; q < 3; ++q) bd.light_L[q] = L[q];
+            } else {
+                bd.warn("area light '" + d.name + "' is not supported");
+            }
+        } else if (kw == "LightSource") {
+            const ObjectDef d = read_object(b);
+            bd.warn("light source '" + d.name + "' is not supported (area lights only)");
+        } else if (read_transform(b, st.t)) {
+        } else if (kw == "MakeNamedMaterial") {
+            const ObjectDef d = read_object(b);
+            bd.named[d.name] = make_material(d.params, bd);
+        } else if (kw == "Material") {
+            ObjectDef d = read_object(b);
+            d.params.strings["type"] = d.name;
+            st.mat = std::make_shared<Material>(make_material(d.params, bd));
+        } else if (kw == "Texture") {
+            const ObjectDef d = read_object(b);
+            bd.textures.insert(d.name);
+            bd.warn("texture '" + d.name + "' is not supported");
+        } else if (kw == "NamedMaterial") {
+            ++b.i;
+            const std::string n = b.pop(Tok::String, "material name").text;
+            if (!bd.named.count(n)) throw ParseError("Named material not found: " + n);
+            st.mat = std::make_shared<Material>(bd.named.at(n));
+        } else if (kw == "Include") {
+            ++b.i;
+            const std::string file = b.pop(Tok::String, "file name").text;
+            std::vector<Token> inc = lex(join(base, file));
+            b.toks.insert(b.toks.begin() + (std::ptrdiff_t)b.i, inc.begin(), inc.end());
+        } else if (kw == "ReverseOrientation") {
+            ++b.i;
+            reverse = true;
+        } else {
+            bd.warn("ignored directive " + kw);
+            skip_object(b);
+        }
+    }
+    b.pop(Tok::Keyword, "XEnd");
+}
+
+}  // namespace
+
+struct cav_pbrt {
+    std::vector<float> soup;
+    std::vector<cav_mesh> meshes;
+    cav_render_setup setup{};
+    std::string messages;
+};
+
+namespace {
+
+int integrator_id(const std::string& n, int& depth, const Params& p, Builder& bd) {
+    // Integrators/CreateIntegrator.cu:10-25
+    if (n == "path" || n == "rlpath") {
+        depth = (int)p.num("maxdepth");
+        return n == "path" ? CAV_INTEGRATOR_PATH : CAV_INTEGRATOR_RLPATH;
+    }
+    if (n == "directlighting") {
+        bd.warn("directlighting is rendered by path with max depth 1");
+        depth = 1;
+        return CAV_INTEGRATOR_PATH;
+    }
+    bd.warn("unrecognized integrator name. using pathtracer as default.");
+    depth = 8;
+    return CAV_INTEGRATOR_PATH;
+}
+
+void copy_err(char* err, int err_len, const std::string& s) {
+    if (err && err_len > 0) {
+        std::snprintf(err, (size_t)err_len, "%s", s.c_str());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** out, char* err, int err_len) {
+    if (!path || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    try {
+        Buf b;
+        b.toks = lex(path);
+        Builder bd;
+        Setup su;
+        parse_options(b, su, ov, bd);
+        State st{identity(), nullptr};
+        parse_section(b, st, dir_of(path), bd);
+        if (bd.meshes.empty()) throw ParseError("scene has no shapes");
+        std::unique_ptr<cav_pbrt> r(new cav_pbrt());
+        cav_render_setup& s = r->setup;
+        s.width = (int)su.film.params.num("xresolution");
+        s.height = (int)su.film.params.num("yresolution");
+        if (s.width < 1 || s.height < 1) throw ParseError("film resolution must be positive");
+        float c2w[16];
+        if (!invert(su.view.m, c2w)) throw ParseError("camera transform is singular");
+        // Cameras/CameraObject.h:91-93: only the perspective camera exists
+        if (su.camera.name != "perspective") bd.warn("camera '" + su.camera.name + "' rendered as perspective");
+        if (cav_camera_from_matrix(c2w, su.camera.params.num("fov"), s.width, s.height, &s.camera) != CAV_OK)
+            throw ParseError("bad camera");
+        s.spp = (int)su.sampler.params.num("pixelsamples");
+        if (s.spp < 1) throw ParseError("pixelsamples must be positive");
+        if (su.sampler.name != "halton") bd.warn("sampler '" + su.sampler.name + "' rendered with halton");
+        s.integrator = integrator_id(su.integrator.name, s.max_depth, su.integrator.params, bd);
+        // Scene::prepareForRender (Core/Scene.cu:141-146)
+        s.bvh_opt_rounds = ov ? ov->bvh_opt_rounds : -1;
+        if (s.bvh_opt_rounds < 0) s.bvh_opt_rounds = s.spp >= 64 ? 10 : 3;
+        s.morton_bits = (ov && ov->morton_bits) ? ov->morton_bits : 63;
+        const std::string fn =
+            su.film.params.has_string("filename") ? su.film.params.str("filename") : std::string("output.png");
+        std::snprintf(s.output, sizeof s.output, "%s", fn.c_str());
+        s.warnings = bd.warnings;
+        s.ntris = (int64_t)bd.soup.size() / 9;
+        s.nmeshes = (int)bd.meshes.size();
+        for (const cav_mesh& m : bd.meshes) s.nlights += m.is_light ? 1 : 0;
+        r->soup.swap(bd.soup);
+        r->meshes.swap(bd.meshes);
+        for (const std::string& m : bd.messages) r->messages += m + "\n";
+        *out = r.release();
+        return CAV_OK;
+    } catch (const std::exception& e) {
+        copy_err(err, err_len, e.what());
+        return CAV_ERR_PARSE;
+    }
+}
+
+int cav_pbrt_get(const cav_pbrt* p, const float** tris, const cav_mesh** meshes, cav_render_setup* setup,
+                 const char** messages) {
+    if (!p) return CAV_ERR_INVALID;
+    if (tris) *tris = p->soup.data();
+    if (meshes) *meshes = p->meshes.data();
+    if (setup) *setup = p->setup;
+    if (messages) *messages = p->messages.c_str();
+    return CAV_OK;
+}
+
+void cav_pbrt_destroy(cav_pbrt* p) { delete p; }
+
+int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* ov, cav_scene** out,
+                               cav_render_setup* setup) {
+    if (!ctx || !path || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    cav_pbrt* p = nullptr;
+    char err[1024] = {0};
+    int rc = cav_pbrt_parse(path, ov, &p, err, sizeof err);
+    if (rc != CAV_OK) return cav::fail(ctx, rc, err);
+    const cav_render_setup& s = p->setup;
+    rc = cav_scene_create(ctx, p->soup.data(), (int)s.ntris, p->meshes.data(), s.nmeshes, nullptr, s.bvh_opt_rounds,
+                          s.morton_bits, out);
+    if (rc == CAV_OK && setup) *setup = s;
+    cav_pbrt_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

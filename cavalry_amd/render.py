@@ -91,6 +91,23 @@ class Scene:
             int(morton_bits), ctypes.byref(h)))
         self.handle = h
 
+    @classmethod
+    def from_pbrt(cls, ctx: Context, path: str, **overrides):
+        """Scene::prepareForRender of a parsed .pbrt file (overrides as
+        cavalry_amd.pbrt.overrides). Returns (Scene, RenderSetup)."""
+        from . import pbrt
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.arrays = None
+        ov = pbrt.overrides(**overrides)
+        st = pbrt.RenderSetup()
+        h = ctypes.c_void_p()
+        check(ctx.handle, lib().cav_scene_create_from_pbrt(ctx.handle, path.encode(), ctypes.byref(ov),
+                                                           ctypes.byref(h), ctypes.byref(st)))
+        self.handle = h
+        self.setup = st
+        return self, st
+
     def close(self) -> None:
         if self.handle:
             lib().cav_scene_destroy(self.handle)
@@ -111,7 +128,12 @@ def render(ctx: Context, scene: Scene, spp: int, integrator: str = "path", camer
     spp_per_round fixes the samples per pixel of each round (0: as many as
     device memory holds)."""
     a = scene.arrays
-    cam = camera if camera is not None else for_scene(a)
+    if a is None:  # a scene parsed from a file: its setup gives the defaults
+        st = scene.setup
+        cam = camera if camera is not None else st.camera
+        max_depth = max_depth if max_depth is not None else st.max_depth
+    else:
+        cam = camera if camera is not None else for_scene(a)
     if integrator not in INTEGRATORS:
         raise ValueError(f"unknown integrator {integrator!r}")
     prm = _lib.RenderParams(INTEGRATORS[integrator], int(spp),

@@ -150,6 +150,10 @@ typedef struct {
  * (SceneLoading/Parsing.cu:196-204,257; Cameras/PerspectiveCamera.cu:6-20). */
 int cav_camera_look_at(const float* eye3, const float* center3, const float* up3,
                        float fov_degrees, int width, int height, cav_camera* out);
+/* Camera-to-world matrix (column-major, as glm) + fov -> camera: the tail of
+ * PerspectiveCamera::createFromParams (Cameras/PerspectiveCamera.cu:22-26). */
+int cav_camera_from_matrix(const float* cam_to_world16, float fov_degrees, int width, int height,
+                           cav_camera* out);
 
 typedef struct cav_scene cav_scene;
 
@@ -199,6 +203,51 @@ typedef struct {
 int cav_render(cav_ctx* ctx, cav_scene* scene, const cav_camera* camera,
                const cav_render_params* params, float* film_rgb, float* film_weight,
                cav_render_stats* stats);
+
+/* ---------------------------------------------------------------- PBRT front end */
+
+/* The CLI overrides of main.cu:25-97 (cxxopts --output, --integrator, --spp,
+ * --bvh-opt-rounds), applied while parsing as SceneLoading/Parsing.cu does. */
+typedef struct {
+    const char* output;      /* Film "filename" (NULL or "": keep the file's) */
+    const char* integrator;  /* Integrator name (NULL or "": keep the file's) */
+    int32_t spp;             /* Sampler "pixelsamples" (0: keep the file's) */
+    int32_t bvh_opt_rounds;  /* -1: 3, or 10 when spp >= 64 (Core/Scene.cu:141-146) */
+    int32_t morton_bits;     /* 0: 63, the reference's compiled default (BVH/BVH.cu:14) */
+} cav_pbrt_overrides;
+
+/* What RenderSetup holds after parsing (Core/RenderSetup.h), resolved. */
+typedef struct {
+    cav_camera camera;
+    int32_t width, height;
+    int32_t integrator;      /* CAV_INTEGRATOR_*; unknown names give path, depth 8 */
+    int32_t max_depth;
+    int32_t spp;
+    int32_t bvh_opt_rounds;
+    int32_t morton_bits;
+    int32_t warnings;        /* unsupported features replaced or ignored */
+    int64_t ntris;
+    int32_t nmeshes, nlights;
+    char output[1024];       /* output file name ("output.png" by default) */
+} cav_render_setup;
+
+typedef struct cav_pbrt cav_pbrt;
+
+/* Replaces runLexing + runParsing (SceneLoading/Lexing.cu:8-50,
+ * SceneLoading/Parsing.cu:391-405) on the host: the scene as a triangle soup
+ * and mesh table (the input of cav_scene_create) plus the render setup.
+ * CAV_ERR_PARSE with the message in err on failure. No device work. */
+int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* overrides, cav_pbrt** out, char* err,
+                   int err_len);
+/* Views into a parsed scene (owned by it): float[ntris][9], cav_mesh[nmeshes],
+ * the setup and the warnings, one per line. */
+int cav_pbrt_get(const cav_pbrt* p, const float** tris, const cav_mesh** meshes, cav_render_setup* setup,
+                 const char** messages);
+void cav_pbrt_destroy(cav_pbrt* p);
+/* Replaces readRenderSetup + Scene::prepareForRender (main.cu:10-15):
+ * parse, then cav_scene_create with the setup's BVH rounds and Morton width. */
+int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* overrides,
+                               cav_scene** out, cav_render_setup* setup);
 
 /* ---------------------------------------------------------------- film output */
 
