@@ -17,9 +17,9 @@
 // message list: the device BSDFs cover matte / plastic / mirror / specular
 // glass, so metal, substrate and rough glass are replaced (by matte with the
 // material's Kd, and by specular glass); image textures, non-area lights and
-// the environment map are ignored; the "random" sampler renders with Halton;
-// "directlighting" renders with path at depth 1. Unlike the reference, which
-// exits, every error returns CAV_ERR_PARSE with a message.
+// the environment map are ignored; the "random" sampler renders with Halton.
+// Unlike the reference, which exits, every error returns CAV_ERR_PARSE with a
+// message.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -733,9 +733,8 @@ int integrator_id(const std::string& n, int& depth, const Params& p, Builder& bd
         return n == "path" ? CAV_INTEGRATOR_PATH : CAV_INTEGRATOR_RLPATH;
     }
     if (n == "directlighting") {
-        bd.warn("directlighting is rendered by path with max depth 1");
         depth = 1;
-        return CAV_INTEGRATOR_PATH;
+        return CAV_INTEGRATOR_DIRECT;
     }
     bd.warn("unrecognized integrator name. using pathtracer as default.");
     depth = 8;

@@ -111,6 +111,7 @@ struct RlArgs {
 // rlpath's Q table and light statistics persist across the rounds of a render
 // and are initialised when init_tables is set
 hipError_t render_path(const RenderArgs& a, hipStream_t s);
+hipError_t render_direct(const RenderArgs& a, hipStream_t s);
 hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s);
 // camera rays for every sample and the gathered box-filtered film, shared by
 // both integrators

@@ -189,6 +189,70 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
     }
 }
 
+// DirectLighting::renderRay (Integrators/DirectLightingIntegrator.cu:22-63):
+// emission of the hit light without MIS, one light sample, and a shadow task
+// for every hit carrying Material::eval * lightsCount / pdf (no MIS weight).
+__global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Queue qi, Shadow sh,
+                                                   const int32_t* __restrict__ hit_prim,
+                                                   const float* __restrict__ hit_t, float* __restrict__ res,
+                                                   uint32_t* counts) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = i < (int64_t)counts[0];
+    const int prim = live ? hit_prim[i] : -1;
+    live = live && prim >= 0;
+    uint32_t sid = 0;
+    v3 pos = mk3(0, 0, 0), dir = pos, contrib = pos;
+    float lim = 0.f;
+    int mesh = 0, lm = 0;
+    if (live) {
+        sid = qi.sid[i];
+        const float* r = qi.ray + i * 6;
+        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
+        mesh = sc.tri_mesh[prim];
+        const cav_mesh M = sc.meshes[mesh];
+        const v3 nn = tri_normal(sc, prim);
+        pos = add3(ro, scale3(rd, hit_t[i]));
+        if (M.is_light) {
+            const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
+            res[sid * 3] = res[sid * 3] + Le.x;
+            res[sid * 3 + 1] = res[sid * 3 + 1] + Le.y;
+            res[sid * 3 + 2] = res[sid * 3 + 2] + Le.z;
+        }
+        const int dim0 = 2;
+        const int li = halton_int(h, sid, dim0, sc.nlights);
+        lm = sc.lights[li];
+        const cav_mesh L = sc.meshes[lm];
+        const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
+        const float tmp = sqrtf(halton(h, sid, dim0 + 6));
+        const float u = 1 - tmp;
+        const float v = tmp * halton(h, sid, dim0 + 7);
+        const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
+        const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
+        const v3 ln = tri_normal(sc, tri);
+        const v3 l2s = sub3(pos, lp);
+        const float c = fabsf(dot3(ln, nrm(l2s)));
+        const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
+        dir = nrm(sub3(lp, pos));
+        lim = len3(sub3(lp, pos));
+        const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+        const Bsdf bsdf = make_bsdf(M);
+        v3 t0, t1;
+        frame(nn, t0, t1);
+        const v3 il = to_local(dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);
+        const float mult = (float)sc.nlights / prob;
+        contrib = scale3(scale3(mul3(incident, bsdf.eval(il, ol)), fabsf(dot3(dir, nn))), mult);
+    }
+    const uint32_t ss = wave_push(live, &counts[2]);
+    if (live) {
+        float* r = sh.ray + (int64_t)ss * 6;
+        r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
+        sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = lm; sh.sid[ss] = sid;
+        sh.contrib[(int64_t)ss * 3] = contrib.x;
+        sh.contrib[(int64_t)ss * 3 + 1] = contrib.y;
+        sh.contrib[(int64_t)ss * 3 + 2] = contrib.z;
+    }
+}
+
 // evalMaterial's accumulation for unoccluded shadow rays.
 __global__ __launch_bounds__(kBlock) void k_nee(Shadow sh, const uint8_t* __restrict__ vis,
                                                 float* __restrict__ res, const uint32_t* counts) {
@@ -253,6 +317,25 @@ void render_film(const RenderArgs& a, hipStream_t s) {
     const int64_t npix = (int64_t)a.width * a.height;
     hipLaunchKernelGGL(k_film, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        a.width, a.height, a.spp, a.res, a.sxy, a.film_rgb, a.film_w);
+}
+
+hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
+    const int64_t nq = a.nq;
+    const unsigned nb = (unsigned)((nq + kBlock - 1) / kBlock);
+    hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
+    if (e != hipSuccess) return e;
+    render_raygen(a, s);
+    if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[0].ray, a.counts, nq,
+                                 a.hit_prim, a.hit_t, nullptr, nullptr, a.trace_counters, s)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_direct, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[0], a.sh, a.hit_prim, a.hit_t,
+                       a.res, a.counts);
+    if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nq, a.sh.limit,
+                                  a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_nee, dim3(nb), dim3(kBlock), 0, s, a.sh, a.vis, a.res, a.counts);
+    hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, 0);
+    return hipGetLastError();
 }
 
 hipError_t render_path(const RenderArgs& a, hipStream_t s) {

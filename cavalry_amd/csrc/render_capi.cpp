@@ -349,7 +349,8 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
                float* film_rgb, float* film_weight, cav_render_stats* stats) {
     if (!ctx || !sc || !cam || !prm || !film_rgb || !film_weight) return CAV_ERR_INVALID;
     const bool rl = prm->integrator == CAV_INTEGRATOR_RLPATH;
-    if (prm->integrator != CAV_INTEGRATOR_PATH && !rl)
+    const bool direct = prm->integrator == CAV_INTEGRATOR_DIRECT;
+    if (prm->integrator != CAV_INTEGRATOR_PATH && !rl && !direct)
         return fail(ctx, CAV_ERR_INVALID, "unknown integrator " + std::to_string(prm->integrator));
     const int grid = prm->q_grid == 0 ? 32 : prm->q_grid;
     const float alpha_scale = prm->alpha_scale == 0.f ? 1.f : prm->alpha_scale;
@@ -357,7 +358,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
         return fail(ctx, CAV_ERR_INVALID, "q_grid must be in [1, 64]");
     if (rl && !(alpha_scale > 0.f && alpha_scale <= 1.f))
         return fail(ctx, CAV_ERR_INVALID, "alpha_scale must be in (0, 1]");
-    if (prm->spp < 1 || prm->max_depth < 1 || cam->width < 1 || cam->height < 1)
+    if (prm->spp < 1 || (prm->max_depth < 1 && !direct) || cam->width < 1 || cam->height < 1)
         return fail(ctx, CAV_ERR_INVALID, "spp, max_depth and resolution must be positive");
     const int64_t npix = (int64_t)cam->width * cam->height;
     const int64_t nsamp = npix * prm->spp;
@@ -419,6 +420,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
         a.spp_round = std::min(spp_round, prm->spp - done);
         a.nq = npix * a.spp_round;
         if (rl) RCHK(ctx, render_rlpath(a, r, done == 0, s));
+        else if (direct) RCHK(ctx, render_direct(a, s));
         else RCHK(ctx, render_path(a, s));
     }
     render_film(a, s);

@@ -84,7 +84,10 @@ int main(int argc, char** argv) {
         const cav_render_stats& st = setup.renderer.integrator->stats;
         std::printf("%d x %d, %d spp, %s, max depth %d, %lld triangles, %d lights, %d BVH rounds\n",
                     setup.setup.width, setup.setup.height, setup.setup.spp,
-                    setup.setup.integrator == CAV_INTEGRATOR_RLPATH ? "rlpath" : "path", setup.setup.max_depth,
+                    setup.setup.integrator == CAV_INTEGRATOR_RLPATH   ? "rlpath"
+                    : setup.setup.integrator == CAV_INTEGRATOR_DIRECT ? "directlighting"
+                                                                      : "path",
+                    setup.setup.max_depth,
                     (long long)setup.setup.ntris, setup.setup.nlights, setup.setup.bvh_opt_rounds);
         if (setup.setup.warnings) std::printf("%d unsupported feature(s) replaced or ignored\n", setup.setup.warnings);
         std::printf("device time %.3f ms, %.2f Mrays/s (%lld closest + %lld shadow rays)\n", st.ms_total,

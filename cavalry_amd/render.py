@@ -17,7 +17,7 @@ from ._lib import check, lib
 from .bvh import Context
 from .camera import for_scene
 
-INTEGRATORS = {"path": 0, "rlpath": 1}
+INTEGRATORS = {"path": 0, "rlpath": 1, "directlighting": 2}
 
 
 def default_bvh_rounds(spp: int) -> int:

@@ -120,7 +120,10 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays
 
 /* ---- scenes and rendering ---------------------------------------------- */
 enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_GLASS = 4, CAV_MAT_PLASTIC = 5 };
-enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1 };
+/* path (Integrators/PathTracingIntegrator.cu), rlpath
+ * (Integrators/ReinforcementLearningPathTracing.cu), directlighting
+ * (Integrators/DirectLightingIntegrator.cu: one bounce, NEE without MIS) */
+enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1, CAV_INTEGRATOR_DIRECT = 2 };
 
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),
@@ -171,9 +174,9 @@ int cav_scene_destroy(cav_scene* scene);
 const cav_bvh* cav_scene_bvh(const cav_scene* scene);
 
 typedef struct {
-    int32_t integrator;  /* CAV_INTEGRATOR_PATH or CAV_INTEGRATOR_RLPATH */
+    int32_t integrator;  /* CAV_INTEGRATOR_PATH, _RLPATH or _DIRECT */
     int32_t spp;         /* samples per pixel */
-    int32_t max_depth;   /* bounces (the scene's "integer maxdepth") */
+    int32_t max_depth;   /* bounces (the scene's "integer maxdepth"); ignored by _DIRECT */
     int32_t flags;       /* reserved, 0 */
     /* rlpath only. Q entries per axis: 0 = 32, the reference's Q_TABLE_SIZE
      * (Integrators/ReinforcementLearningPathTracing.cu:151); at most 64. */

@@ -131,10 +131,22 @@ public:
     }
 };
 
+class DirectLightingIntegrator : public Integrator {
+public:
+    void render(const Scene& scene, const cav_camera& camera, Film& film) override {
+        cav_render_params p{};
+        p.integrator = CAV_INTEGRATOR_DIRECT;
+        p.spp = spp;
+        p.max_depth = 1;
+        run(scene, camera, film, p);
+    }
+};
+
 // CreateIntegrator::createFromObjectDefinition, from a parsed setup
 inline std::unique_ptr<Integrator> createIntegrator(const cav_render_setup& s) {
     std::unique_ptr<Integrator> r;
     if (s.integrator == CAV_INTEGRATOR_RLPATH) r = std::make_unique<RLPTIntegrator>(s.max_depth);
+    else if (s.integrator == CAV_INTEGRATOR_DIRECT) r = std::make_unique<DirectLightingIntegrator>();
     else r = std::make_unique<PathTracingIntegrator>(s.max_depth);
     r->spp = s.spp;
     return r;

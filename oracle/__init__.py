@@ -175,6 +175,26 @@ def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int =
     return rgb, wt, int(rays[0])
 
 
+def render_direct(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30):
+    """`directlighting` integrator on the CPU. Returns (film_rgb, film_w, rays)."""
+    tris = np.ascontiguousarray(scene.tris, np.float32)
+    _, _, nd = lbvh_build(tris, morton_bits)
+    if bvh_rounds:
+        treelet_optimize(nd, bvh_rounds)
+    w, h = scene.width, scene.height
+    rgb = np.zeros((h, w, 3), np.float32)
+    wt = np.zeros((h, w), np.float32)
+    rays = np.zeros(1, np.int64)
+    cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
+    rc = lib().ref_render_direct(
+        _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(cam16), ctypes.c_float(camera.depth),
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), _p(rgb), _p(wt), _p(rays))
+    if rc != 0:
+        raise ValueError(f"ref_render_direct failed ({rc})")
+    return rgb, wt, int(rays[0])
+
+
 def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30,
                   q_grid: int = 32, alpha_scale: float = 1.0, max_depth: int | None = None,
                   spp_per_round: int = 0):

@@ -41,6 +41,10 @@ int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh, const
                     int nmeshes, const float* box, const int32_t* left, const int32_t* right,
                     const int32_t* prim, const float* cam16, float cam_depth, int width, int height,
                     int spp, int max_depth, float* film_rgb, float* film_w, int64_t* rays_traced);
+int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
+                      int nmeshes, const float* box, const int32_t* left, const int32_t* right,
+                      const int32_t* prim, const float* cam16, float cam_depth, int width, int height,
+                      int spp, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                       const cav_mesh_ref* meshes, int nmeshes, const float* box,
                       const int32_t* left, const int32_t* right, const int32_t* prim,

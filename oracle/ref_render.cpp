@@ -520,6 +520,36 @@ extern "C" void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, in
     }
 }
 
+// box-filtered film (Core/Film.h:25-49, Filters/BoxFilter.h:19-25), gathered
+// per pixel in a fixed order: samples of pixels (y-1..y+1, x-1..x+1) in
+// row-major pixel order, then sample order; non-finite samples are skipped
+void gather_film(int width, int height, int spp, const std::vector<float>& res, const std::vector<float>& sx,
+                 const std::vector<float>& sy, float* film_rgb, float* film_w) {
+#pragma omp parallel for schedule(static)
+    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+        const int x = (int)(pix % width), y = (int)(pix / width);
+        float r = 0, g = 0, b = 0, w = 0;
+        for (int py = y - 1; py <= y + 1; ++py)
+            for (int px = x - 1; px <= x + 1; ++px) {
+                if (px < 0 || py < 0 || px >= width || py >= height) continue;
+                for (int k = 0; k < spp; ++k) {
+                    const int64_t si = ((int64_t)py * width + px) * spp + k;
+                    const float* c = &res[si * 3];
+                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
+                    const float fx = sx[si], fy = sy[si];
+                    if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
+                    if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
+                    if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
+                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
+                }
+            }
+        film_rgb[pix * 3 + 0] = r;
+        film_rgb[pix * 3 + 1] = g;
+        film_rgb[pix * 3 + 2] = b;
+        film_w[pix] = w;
+    }
+}
+
 // Renders with the `path` integrator. The BVH node arrays are the oracle's own
 // (ref_lbvh_build + ref_treelet_optimize). Outputs per-pixel sum (rgb) and
 // weight of the box-filtered film (Core/Film.h:25-49), accumulated in a fixed
@@ -643,30 +673,91 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
         res[si * 3 + 2] = result.z;
     }
 
-    // box-filtered film (Core/Film.h:25-49, Filters/BoxFilter.h:19-25), gathered
-#pragma omp parallel for schedule(static)
-    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
+    if (rays_traced) *rays_traced = traced;
+    return 0;
+}
+
+
+// Renders with the `directlighting` integrator
+// (Integrators/DirectLightingIntegrator.cu:22-140): the camera ray's hit adds
+// the light's emission (no MIS), then one light sample whose unoccluded
+// contribution is Material::eval * lightsCount / pdf (no MIS weight).
+extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh,
+                                 const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim,
+                                 const float* cam16, float cam_depth, int width, int height, int spp,
+                                 float* film_rgb, float* film_w, int64_t* rays_traced) {
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    for (int m = 0; m < nmeshes; ++m)
+        if (meshes[m].is_light) s.lights.push_back(m);
+    s.area.resize(nmeshes);
+    ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
+    const Halton& H = halton();
+    const int64_t nsamp = (int64_t)width * height * spp;
+    std::vector<float> res((size_t)nsamp * 3), sx((size_t)nsamp), sy((size_t)nsamp);
+    int64_t traced = 0;
+    const int nl = (int)s.lights.size();
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : traced)
+    for (int64_t si = 0; si < nsamp; ++si) {
+        const int64_t pix = si / spp;
         const int x = (int)(pix % width), y = (int)(pix / width);
-        float r = 0, g = 0, b = 0, w = 0;
-        for (int py = y - 1; py <= y + 1; ++py)
-            for (int px = x - 1; px <= x + 1; ++px) {
-                if (px < 0 || py < 0 || px >= width || py >= height) continue;
-                for (int k = 0; k < spp; ++k) {
-                    const int64_t si = ((int64_t)py * width + px) * spp + k;
-                    const float* c = &res[si * 3];
-                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
-                    const float fx = sx[si], fy = sy[si];
-                    if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
-                    if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
-                    if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
-                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
-                }
+        const uint64_t hidx = (uint64_t)si;
+        int dim = 0;
+        float fx = (float)x, fy = (float)y;
+        fx += H.rand1(hidx, dim);
+        fy += H.rand1(hidx, dim);
+        sx[si] = fx;
+        sy[si] = fy;
+        f3 pl = mk(fx - width / 2.f, height / 2.f - fy, cam_depth);
+        f3 ro = apply(cam16, mk(0, 0, 0));
+        f3 rd = normalize(apply(cam16, pl) - ro);
+        f3 result = mk(0, 0, 0);
+        float t;
+        ++traced;
+        const int p = closest(s, ro, rd, t);
+        if (p >= 0) {
+            const int mesh = tri_mesh[p];
+            const cav_mesh_ref& M = meshes[mesh];
+            f3 n = tri_normal(s, p);
+            f3 pos = ro + t * rd;
+            if (M.is_light) {
+                f3 Le = dot(rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
+                result = result + Le;
             }
-        film_rgb[pix * 3 + 0] = r;
-        film_rgb[pix * 3 + 1] = g;
-        film_rgb[pix * 3 + 2] = b;
-        film_w[pix] = w;
+            const int li = H.rand_int(nl, hidx, dim);
+            dim += 4;  // rand4, unused by the reference
+            const int lm = s.lights[li];
+            const cav_mesh_ref& L = meshes[lm];
+            int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
+            float tmp = std::sqrt(H.rand1(hidx, dim));
+            float u = 1 - tmp;
+            float v = tmp * H.rand1(hidx, dim);
+            f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+            f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
+            f3 ln = tri_normal(s, tri);
+            f3 l2s = pos - lp;
+            float c = std::fabs(dot(ln, normalize(l2s)));
+            float prob = dot(l2s, l2s) / (c * s.area[lm]);
+            f3 dir = normalize(lp - pos);
+            float lim = length(lp - pos);
+            f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+            ++traced;
+            if (visible(s, pos, dir, lim, mesh, lm)) {
+                Bsdf bsdf = make_bsdf(M);
+                f3 t0, t1;
+                coordinate_system(n, t0, t1);
+                f3 il = to_local(dir, t0, t1, n), ol = to_local(rd * -1, t0, t1, n);
+                const float mult = (float)nl / prob;
+                f3 contrib = (mulv(incident, bsdf.eval(il, ol)) * std::fabs(dot(dir, n))) * mult;
+                result = result + contrib;
+            }
+        }
+        res[si * 3 + 0] = result.x;
+        res[si * 3 + 1] = result.y;
+        res[si * 3 + 2] = result.z;
     }
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
     if (rays_traced) *rays_traced = traced;
     return 0;
 }
@@ -1096,29 +1187,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
         }
     }  // rounds
 
-#pragma omp parallel for schedule(static)
-    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
-        const int x = (int)(pix % width), y = (int)(pix / width);
-        float r = 0, g = 0, b = 0, w = 0;
-        for (int py = y - 1; py <= y + 1; ++py)
-            for (int px = x - 1; px <= x + 1; ++px) {
-                if (px < 0 || py < 0 || px >= width || py >= height) continue;
-                for (int k = 0; k < spp; ++k) {
-                    const int64_t si = ((int64_t)py * width + px) * spp + k;
-                    const float* c = &res[si * 3];
-                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
-                    const float fx = sx[si], fy = sy[si];
-                    if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
-                    if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
-                    if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
-                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
-                }
-            }
-        film_rgb[pix * 3 + 0] = r;
-        film_rgb[pix * 3 + 1] = g;
-        film_rgb[pix * 3 + 2] = b;
-        film_w[pix] = w;
-    }
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
     if (rays_traced) *rays_traced = traced;
     return 0;
 }

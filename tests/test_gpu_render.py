@@ -71,3 +71,17 @@ def test_path_rounds_equal_single_round_oracle(ctx):
     assert st1["rounds"] == 1
     np.testing.assert_array_equal(film1.rgb, rgb)
     scene.close()
+
+
+@pytest.mark.parametrize("materials", ["matte", "mixed"])
+def test_directlighting_matches_oracle(ctx, materials):
+    print(f"[render] directlighting {materials}", flush=True)
+    sc = scenes.cornell_box(seed=7, width=48, height=40, materials=materials)
+    cam = for_scene(sc)
+    rgb, w, rays = oracle.render_direct(sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator="directlighting", camera=cam)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()

@@ -78,3 +78,15 @@ def test_film_weights_count_box_filter_support():
     # interior pixels: samples of pixels (x-1..x, y-1..y) always cover (x, y)
     assert (w[1:, 1:] >= 4 * spp).all() and (w[1:, 1:] <= 9 * spp).all()
     assert w[0, 0] >= spp
+
+
+def test_directlighting_furnace_is_le_times_one_plus_rho():
+    """directlighting = emission + one unweighted light sample: inside the
+    furnace every surface sees radiance Le from all directions, so the
+    estimate converges to Le (1 + rho)."""
+    for rho in (0.3, 0.7):
+        sc = furnace_scene(rho=rho)
+        cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+        rgb, w, _ = oracle.render_direct(sc, 32, cam)
+        mean = float((rgb / w[..., None]).mean())
+        assert abs(mean - (1.0 + rho)) < 0.02 * (1.0 + rho), (rho, mean)

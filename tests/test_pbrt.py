@@ -80,11 +80,11 @@ def test_overrides_and_bvh_round_rule(tmp_path):
 
 
 def test_integrator_factory(tmp_path):
-    for name, expect in (("directlighting", (0, 1)), ("bdpt", (0, 8))):
+    for name, expect, warn in (("directlighting", (2, 1), 0), ("bdpt", (0, 8), 1)):
         head = HEAD.replace('Integrator "path" "integer maxdepth" [ 5 ]', f'Integrator "{name}"')
         p = write(tmp_path, 'WorldBegin\nShape "trianglemesh" ' + QUAD + '\nWorldEnd\n', head=head)
         _, _, st, msgs = pbrt.parse(p)
-        assert (st.integrator, st.max_depth) == expect and st.warnings == 1 and msgs
+        assert (st.integrator, st.max_depth) == expect and st.warnings == warn and len(msgs) == warn
 
 
 def test_transforms(tmp_path):
