@@ -25,6 +25,7 @@ SOURCES = [
     "traverse.hip",
     "render.hip",
     "rlpath.hip",
+    "queue_sort.hip",
     "render_capi.cpp",
     "film_io.cpp",
     "pbrt.cpp",

@@ -72,6 +72,9 @@ struct RenderArgs {
     uint32_t* trace_counters;  // traversal counters (8 words)
     float* film_rgb;           // [w*h][3]
     float* film_w;             // [w*h]
+    float4* film_partial;      // [9][w*h] per-source-pixel partial sums
+    uint32_t* sort_perm;       // [cap] hit queue order by material (null: queue order)
+    uint32_t* sort_hist;       // sort_hist_words(cap)
 };
 
 // extra per-task state of the rlpath ray queues
@@ -110,6 +113,11 @@ struct RlArgs {
 // one spp round (a.spp_done .. a.spp_done + a.spp_round) of each integrator;
 // rlpath's Q table and light statistics persist across the rounds of a render
 // and are initialised when init_tables is set
+// stable counting sort of the first *count hits by material (misses last)
+hipError_t sort_hits_by_material(const SceneDev& sc, const int32_t* hit_prim, const uint32_t* count, int64_t cap,
+                                 uint32_t* hist, uint32_t* perm, hipStream_t s);
+int64_t sort_hist_words(int64_t cap);
+
 hipError_t render_path(const RenderArgs& a, hipStream_t s);
 hipError_t render_direct(const RenderArgs& a, hipStream_t s);
 hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s);

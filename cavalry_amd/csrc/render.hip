@@ -68,10 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t,
                                                   float* __restrict__ res, uint32_t* counts,
-                                                  int qin, int qout) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                  int qin, int qout, const uint32_t* __restrict__ perm) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counts[qin];
-    bool live = i < (int64_t)n;
+    bool live = j < (int64_t)n;
+    const int64_t i = live ? task_slot(perm, j) : 0;
     int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
 
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
             push_next = true;
         }
     }
-    const uint32_t ss = wave_push(push_sh, &counts[2]);
+    const uint32_t ss = block_push<kBlock>(push_sh, &counts[2]);
     if (push_sh) {
         float* r = sh.ray + (int64_t)ss * 6;
         r[0] = sro.x; r[1] = sro.y; r[2] = sro.z; r[3] = srd.x; r[4] = srd.y; r[5] = srd.z;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
         sh.contrib[(int64_t)ss * 3 + 1] = contrib.y;
         sh.contrib[(int64_t)ss * 3 + 2] = contrib.z;
     }
-    const uint32_t ns = wave_push(push_next, &counts[qout]);
+    const uint32_t ns = block_push<kBlock>(push_next, &counts[qout]);
     if (push_next) {
         float* r = qo.ray + (int64_t)ns * 6;
         r[0] = nro.x; r[1] = nro.y; r[2] = nro.z; r[3] = nrd.x; r[4] = nrd.y; r[5] = nrd.z;
@@ -195,9 +196,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
 __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Queue qi, Shadow sh,
                                                    const int32_t* __restrict__ hit_prim,
                                                    const float* __restrict__ hit_t, float* __restrict__ res,
-                                                   uint32_t* counts) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool live = i < (int64_t)counts[0];
+                                                   uint32_t* counts, const uint32_t* __restrict__ perm) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = j < (int64_t)counts[0];
+    const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
     uint32_t sid = 0;
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Que
         const float mult = (float)sc.nlights / prob;
         contrib = scale3(scale3(mul3(incident, bsdf.eval(il, ol)), fabsf(dot3(dir, nn))), mult);
     }
-    const uint32_t ss = wave_push(live, &counts[2]);
+    const uint32_t ss = block_push<kBlock>(live, &counts[2]);
     if (live) {
         float* r = sh.ray + (int64_t)ss * 6;
         r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
@@ -274,30 +276,57 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int qin) {
     }
 }
 
-// addSamplesToFilm with the box filter, gathered: pixel (x, y) takes the
-// samples of pixels (x-1..x+1, y-1..y+1) in row-major order whose 2x2+ support
-// covers it.
-__global__ __launch_bounds__(kBlock) void k_film(int width, int height, int spp,
-                                                 const float* __restrict__ res,
-                                                 const float* __restrict__ sxy,
-                                                 float* __restrict__ rgb, float* __restrict__ wt) {
+// addSamplesToFilm with the box filter (Core/Film.h:25-49,
+// Filters/BoxFilter.h:19-25) in a fixed order instead of float atomics, in
+// two passes. k_film_partial: for source pixel p and each of the 9 target
+// offsets (dy, dx) in row-major order, the sum over p's samples k = 0..spp-1
+// (in order) of those whose filter support covers pixel p + (dx, dy).
+// k_film_final: each target pixel adds the partials of its 3x3 neighbours in
+// row-major neighbour order. Samples with a non-finite value are skipped.
+__global__ __launch_bounds__(kBlock) void k_film_partial(int width, int height, int spp,
+                                                         const float* __restrict__ res,
+                                                         const float* __restrict__ sxy,
+                                                         float4* __restrict__ partial) {
     const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= (int64_t)width * height) return;
+    const int64_t npix = (int64_t)width * height;
+    if (pix >= npix) return;
+    const int sx0 = (int)(pix % width), sy0 = (int)(pix / width);
+    float4 acc[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) acc[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < spp; ++k) {
+        const int64_t si = pix * spp + k;
+        const float c0 = res[si * 3], c1 = res[si * 3 + 1], c2 = res[si * 3 + 2];
+        if (!isfinite(c0) || !isfinite(c1) || !isfinite(c2)) continue;
+        const float fx = sxy[si * 2], fy = sxy[si * 2 + 1];
+        const int x_lo = (int)floorf(fx - 1.f), x_hi = (int)ceilf(fx + 1.f);
+        const int y_lo = (int)floorf(fy - 1.f), y_hi = (int)ceilf(fy + 1.f);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const int x = sx0 + d % 3 - 1, y = sy0 + d / 3 - 1;
+            if (x < x_lo || x > x_hi || y < y_lo || y > y_hi) continue;
+            if (fabsf((float)x - fx) > 1.f || fabsf((float)y - fy) > 1.f) continue;
+            acc[d].x += c0; acc[d].y += c1; acc[d].z += c2; acc[d].w += 1.f;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 9; ++d) partial[d * npix + pix] = acc[d];
+}
+
+__global__ __launch_bounds__(kBlock) void k_film_final(int width, int height,
+                                                       const float4* __restrict__ partial,
+                                                       float* __restrict__ rgb, float* __restrict__ wt) {
+    const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t npix = (int64_t)width * height;
+    if (pix >= npix) return;
     const int x = (int)(pix % width), y = (int)(pix / width);
     float r = 0, g = 0, b = 0, w = 0;
     for (int py = y - 1; py <= y + 1; ++py)
         for (int px = x - 1; px <= x + 1; ++px) {
             if (px < 0 || py < 0 || px >= width || py >= height) continue;
-            for (int k = 0; k < spp; ++k) {
-                const int64_t si = ((int64_t)py * width + px) * spp + k;
-                const float c0 = res[si * 3], c1 = res[si * 3 + 1], c2 = res[si * 3 + 2];
-                if (!isfinite(c0) || !isfinite(c1) || !isfinite(c2)) continue;
-                const float fx = sxy[si * 2], fy = sxy[si * 2 + 1];
-                if (x < (int)floorf(fx - 1.f) || x > (int)ceilf(fx + 1.f)) continue;
-                if (y < (int)floorf(fy - 1.f) || y > (int)ceilf(fy + 1.f)) continue;
-                if (fabsf((float)x - fx) > 1.f || fabsf((float)y - fy) > 1.f) continue;
-                r += c0; g += c1; b += c2; w += 1.f;
-            }
+            const int d = (y - py + 1) * 3 + (x - px + 1);  // offset from the source to this pixel
+            const float4 v = partial[d * npix + (int64_t)py * width + px];
+            r += v.x; g += v.y; b += v.z; w += v.w;
         }
     rgb[pix * 3] = r;
     rgb[pix * 3 + 1] = g;
@@ -315,8 +344,11 @@ void render_raygen(const RenderArgs& a, hipStream_t s) {
 
 void render_film(const RenderArgs& a, hipStream_t s) {
     const int64_t npix = (int64_t)a.width * a.height;
-    hipLaunchKernelGGL(k_film, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       a.width, a.height, a.spp, a.res, a.sxy, a.film_rgb, a.film_w);
+    const dim3 grid((unsigned)((npix + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_film_partial, grid, dim3(kBlock), 0, s, a.width, a.height, a.spp, a.res, a.sxy,
+                       a.film_partial);
+    hipLaunchKernelGGL(k_film_final, grid, dim3(kBlock), 0, s, a.width, a.height, a.film_partial, a.film_rgb,
+                       a.film_w);
 }
 
 hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
@@ -328,8 +360,11 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
     if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[0].ray, a.counts, nq,
                                  a.hit_prim, a.hit_t, nullptr, nullptr, a.trace_counters, s)) != hipSuccess)
         return e;
+    if (a.sort_perm && (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts, nq, a.sort_hist, a.sort_perm, s)) !=
+                           hipSuccess)
+        return e;
     hipLaunchKernelGGL(k_direct, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[0], a.sh, a.hit_prim, a.hit_t,
-                       a.res, a.counts);
+                       a.res, a.counts, a.sort_perm);
     if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nq, a.sh.limit,
                                   a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
         return e;
@@ -351,8 +386,12 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, nullptr, nullptr,
                                      a.trace_counters, s)) != hipSuccess)
             return e;
+        if (a.sort_perm &&
+            (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts + qin, nsamp, a.sort_hist, a.sort_perm, s)) !=
+                hipSuccess)
+            return e;
         hipLaunchKernelGGL(k_shade, dim3(nb), dim3(kBlock), 0, s, depth, dim0, a.scene, a.halton,
-                           a.q[qin], a.q[qout], a.sh, a.hit_prim, a.hit_t, a.res, a.counts, qin, qout);
+                           a.q[qin], a.q[qout], a.sh, a.hit_prim, a.hit_t, a.res, a.counts, qin, qout, a.sort_perm);
         if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2,
                                       nsamp, a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters,
                                       s)) != hipSuccess)

@@ -39,7 +39,7 @@ namespace {
 
 // device bytes per sample of a round (queues, shadow tasks, hits) and per
 // sample of the whole render (radiance + film position)
-constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 8;
+constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 8 + 4;
 constexpr double kRlQueueBytes = 2 * 12 + (4 * 5 + 12 * 3 + 4 + 4 + 4);
 constexpr double kResultBytes = 12 + 8;
 
@@ -228,6 +228,9 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
     if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 64, "trace counters");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.film_partial, 9 * 16 * P, "film partial sums");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sort_perm, 4 * S, "material order");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.sort_hist, 4 * (size_t)sort_hist_words(nq), "material histogram");
     if (rc != CAV_OK) { free_bufs(rs); return rc; }
     rs->cap = nq;
     rs->tot_cap = ntot;
@@ -406,6 +409,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     a.spp = prm->spp;
     a.max_depth = prm->max_depth;
     a.nsamp = nsamp;
+    if (!(prm->flags & CAV_RENDER_SORT_BY_MATERIAL)) a.sort_perm = nullptr;
     hipStream_t s = ctx->stream;
     RCHK(ctx, hipMemsetAsync(a.rays, 0, 16, s));
     RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));

@@ -150,13 +150,23 @@ __device__ inline int wave_max(int v) {
     return v;
 }
 
-// syncDimension before computeLighting: maximum over the tasks that hit.
-__global__ __launch_bounds__(kBlock) void k_dim_max(const uint32_t* counts, int qin, RlQueue q,
-                                                    const int32_t* __restrict__ hit_prim, int32_t* dims) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+// syncDimension before computeLighting: maximum over the tasks that hit, one
+// atomic per 1024-thread block at most (almost every block finds the value
+// already stored and skips it).
+constexpr int kMaxBlock = 1024;
+__global__ __launch_bounds__(kMaxBlock) void k_dim_max(const uint32_t* counts, int qin, RlQueue q,
+                                                       const int32_t* __restrict__ hit_prim, int32_t* dims) {
+    __shared__ int wmax[kMaxBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kMaxBlock + threadIdx.x;
     const bool live = i < (int64_t)counts[qin] && hit_prim[i] >= 0;
     const int m = wave_max(live ? q.dim[i] : 0);
-    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&dims[0], m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = 0;
+        for (int w = 0; w < kMaxBlock / 64; ++w) b = max(b, wmax[w]);
+        if (b > __hip_atomic_load(&dims[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&dims[0], b);
+    }
 }
 
 // computeLighting (:388-460) up to the visibility test: emission with MIS,
@@ -165,9 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
                                                      Queue qi, RlSlots sl, Shadow sh, RlTables t,
                                                      const int32_t* __restrict__ hit_prim,
                                                      const float* __restrict__ hit_t, float* __restrict__ res,
-                                                     uint32_t* counts, const int32_t* dims, int qin) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool live = i < (int64_t)counts[qin];
+                                                     uint32_t* counts, const int32_t* dims, int qin,
+                                                     const uint32_t* __restrict__ perm) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = j < (int64_t)counts[qin];
+    const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
     v3 pos = mk3(0, 0, 0), dir = pos;
@@ -228,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
         sl.ldir[i * 3] = dir.x; sl.ldir[i * 3 + 1] = dir.y; sl.ldir[i * 3 + 2] = dir.z;
         sl.inc[i * 3] = inc.x; sl.inc[i * 3 + 1] = inc.y; sl.inc[i * 3 + 2] = inc.z;
     }
-    const uint32_t ss = wave_push(live, &counts[2]);
+    const uint32_t ss = block_push<kBlock>(live, &counts[2]);
     if (live) {
         float* r = sh.ray + (int64_t)ss * 6;
         r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
@@ -290,10 +302,12 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
                                                   RlSlots sl, RlTables t, const uint8_t* __restrict__ vis,
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t, float* __restrict__ res,
-                                                  const uint32_t* counts, int32_t* dims, int qin) {
+                                                  const uint32_t* counts, int32_t* dims, int qin,
+                                                  const uint32_t* __restrict__ perm) {
     __shared__ float trig[2 * kNX][kBlock];
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool live = i < (int64_t)counts[qin];
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = j < (int64_t)counts[qin];
+    const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
     bool valid = false;
@@ -456,7 +470,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         res[sid * 3 + 1] = res[sid * 3 + 1] + add.y;
         res[sid * 3 + 2] = res[sid * 3 + 2] + add.z;
     }
-    if (__ballot(valid) != 0 && (threadIdx.x & 63) == 0) atomicMax(&dims[1], 1);
+    if (__ballot(valid) != 0 && (threadIdx.x & 63) == 0 &&
+        __hip_atomic_load(&dims[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        atomicMax(&dims[1], 1);
 }
 
 // finishUpdateQ (:97-110, 753-762) and averageQ (:52-72, 765-772) for one
@@ -559,9 +575,11 @@ __global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nen
 __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonDev h, Queue qi, Queue qo,
                                                  RlQueue rqo, RlSlots sl, const int32_t* __restrict__ hit_prim,
                                                  const float* __restrict__ hit_t, uint32_t* counts,
-                                                 const int32_t* dims, int qin, int qout) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool live = i < (int64_t)counts[qin];
+                                                 const int32_t* dims, int qin, int qout,
+                                                 const uint32_t* __restrict__ perm) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool live = j < (int64_t)counts[qin];
+    const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
     bool push = false;
@@ -620,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
             push = true;
         }
     }
-    const uint32_t ns = wave_push(push, &counts[qout]);
+    const uint32_t ns = block_push<kBlock>(push, &counts[qout]);
     if (push) {
         float* r = qo.ray + (int64_t)ns * 6;
         r[0] = nro.x; r[1] = nro.y; r[2] = nro.z; r[3] = nrd.x; r[4] = nrd.y; r[5] = nrd.z;
@@ -678,20 +696,25 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, nullptr, nullptr,
                                      a.trace_counters, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(k_dim_max, dim3(nb), dim3(kBlock), 0, s, a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
+        hipLaunchKernelGGL(k_dim_max, dim3((unsigned)((nsamp + kMaxBlock - 1) / kMaxBlock)), dim3(kMaxBlock), 0, s,
+                           a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
+        if (a.sort_perm &&
+            (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts + qin, nsamp, a.sort_hist, a.sort_perm, s)) !=
+                hipSuccess)
+            return e;
         hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
-                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin);
+                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin, a.sort_perm);
         if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
-                           t, a.vis, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin);
+                           t, a.vis, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin, a.sort_perm);
         hipLaunchKernelGGL(k_finish_q, dim3((unsigned)((nent + kEntryWaves - 1) / kEntryWaves)),
                            dim3(64 * kEntryWaves), 0, s, t, nent, rl.alpha_scale);
         hipLaunchKernelGGL(k_finish_light, dim3((unsigned)((nent + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
                            nent, L);
         hipLaunchKernelGGL(k_next, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, a.q[qin], a.q[qout],
-                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, a.counts, rl.dims, qin, qout);
+                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, a.counts, rl.dims, qin, qout, a.sort_perm);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
         qin = qout;
     }

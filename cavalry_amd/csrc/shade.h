@@ -325,17 +325,37 @@ __device__ inline v3 apply_m(const float* m, v3 v) {
     return mk3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
 }
 
-// Appends `count` lanes of the wave (those with `push`) to a queue; returns
-// each pushing lane's slot. All lanes of the wave must call it.
-__device__ inline uint32_t wave_push(bool push, uint32_t* count) {
+// Queue slot handled by thread j: through the material order when the queue
+// was sorted, else j itself.
+__device__ inline int64_t task_slot(const uint32_t* perm, int64_t j) { return perm ? (int64_t)perm[j] : j; }
+
+// Appends the threads of the block that have `push` set to a queue and
+// returns each pushing thread's slot. One atomic per block: with one atomic
+// per wave, the ~65K waves of a 4M-task launch serialise on the counter's L2
+// channel for ~0.35 ms. Every thread of the block must call it (it contains
+// barriers); slots within the block follow thread order.
+template <int kThreads>
+__device__ inline uint32_t block_push(bool push, uint32_t* count) {
+    constexpr int kWaves = kThreads / 64;
+    __shared__ uint32_t wave_base[kWaves];
+    __shared__ uint32_t block_base;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t m = __ballot(push);
-    if (m == 0) return 0;
-    const int lane = threadIdx.x & 63;
-    const int first = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, first, 64);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_base[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = wave_base[w];
+            wave_base[w] = total;
+            total += c;
+        }
+        block_base = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = block_base + wave_base[wv] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();  // the next call may reuse wave_base / block_base
+    return slot;
 }
 
 

@@ -124,6 +124,11 @@ enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_GLASS = 4, CAV_MAT_PLASTIC
  * (Integrators/ReinforcementLearningPathTracing.cu), directlighting
  * (Integrators/DirectLightingIntegrator.cu: one bounce, NEE without MIS) */
 enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1, CAV_INTEGRATOR_DIRECT = 2 };
+/* cav_render_params.flags. SORT_BY_MATERIAL shades each bounce's hits in
+ * material order (sortLightingQueue, Integrators/PathTracingIntegrator.cu:50-92
+ * and ReinforcementLearningPathTracing.cu:226-272, by material only): the
+ * film does not change, only the speed. */
+enum { CAV_RENDER_SORT_BY_MATERIAL = 1 };
 
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),
@@ -177,7 +182,7 @@ typedef struct {
     int32_t integrator;  /* CAV_INTEGRATOR_PATH, _RLPATH or _DIRECT */
     int32_t spp;         /* samples per pixel */
     int32_t max_depth;   /* bounces (the scene's "integer maxdepth"); ignored by _DIRECT */
-    int32_t flags;       /* reserved, 0 */
+    int32_t flags;       /* CAV_RENDER_* bits */
     /* rlpath only. Q entries per axis: 0 = 32, the reference's Q_TABLE_SIZE
      * (Integrators/ReinforcementLearningPathTracing.cu:151); at most 64. */
     int32_t q_grid;

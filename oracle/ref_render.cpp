@@ -520,9 +520,10 @@ extern "C" void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, in
     }
 }
 
-// box-filtered film (Core/Film.h:25-49, Filters/BoxFilter.h:19-25), gathered
-// per pixel in a fixed order: samples of pixels (y-1..y+1, x-1..x+1) in
-// row-major pixel order, then sample order; non-finite samples are skipped
+// box-filtered film (Core/Film.h:25-49, Filters/BoxFilter.h:19-25) in a
+// fixed order: for each pixel, its 3x3 neighbours in row-major order each
+// contribute the sum (in sample order) of their samples whose filter support
+// covers the pixel; non-finite samples are skipped
 void gather_film(int width, int height, int spp, const std::vector<float>& res, const std::vector<float>& sx,
                  const std::vector<float>& sy, float* film_rgb, float* film_w) {
 #pragma omp parallel for schedule(static)
@@ -532,6 +533,7 @@ void gather_film(int width, int height, int spp, const std::vector<float>& res, 
         for (int py = y - 1; py <= y + 1; ++py)
             for (int px = x - 1; px <= x + 1; ++px) {
                 if (px < 0 || py < 0 || px >= width || py >= height) continue;
+                float pr = 0, pg = 0, pb = 0, pw = 0;
                 for (int k = 0; k < spp; ++k) {
                     const int64_t si = ((int64_t)py * width + px) * spp + k;
                     const float* c = &res[si * 3];
@@ -540,8 +542,9 @@ void gather_film(int width, int height, int spp, const std::vector<float>& res, 
                     if (x < (int)std::floor(fx - 1.f) || x > (int)std::ceil(fx + 1.f)) continue;
                     if (y < (int)std::floor(fy - 1.f) || y > (int)std::ceil(fy + 1.f)) continue;
                     if (std::fabs((float)x - fx) > 1.f || std::fabs((float)y - fy) > 1.f) continue;
-                    r += c[0]; g += c[1]; b += c[2]; w += 1.f;
+                    pr += c[0]; pg += c[1]; pb += c[2]; pw += 1.f;
                 }
+                r += pr; g += pg; b += pb; w += pw;
             }
         film_rgb[pix * 3 + 0] = r;
         film_rgb[pix * 3 + 1] = g;

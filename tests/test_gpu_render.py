@@ -85,3 +85,18 @@ def test_directlighting_matches_oracle(ctx, materials):
     np.testing.assert_array_equal(film.rgb, rgb)
     assert st["closest_rays"] + st["shadow_rays"] == rays
     scene.close()
+
+
+@pytest.mark.parametrize("integrator,depth", [("path", 8), ("directlighting", 1), ("rlpath", 1)])
+def test_material_sort_keeps_the_film(ctx, integrator, depth):
+    """Shading hits in material order changes only the speed."""
+    print(f"[render] material sort {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=8, width=48, height=40, materials="mixed")
+    cam = for_scene(sc)
+    scene = R.Scene(ctx, sc)
+    a, sa = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    b, sb = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth, sort_by_material=True)
+    np.testing.assert_array_equal(a.rgb, b.rgb)
+    np.testing.assert_array_equal(a.weight, b.weight)
+    assert sa["closest_rays"] == sb["closest_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
+    scene.close()
