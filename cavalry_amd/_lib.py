@@ -52,6 +52,24 @@ class RenderStats(ctypes.Structure):
                 ("rounds", ctypes.c_int32)]
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("arena_bytes", ctypes.c_int64), ("host_arena_bytes", ctypes.c_int64),
+                ("pool_tasks", ctypes.c_int64), ("chunk_samples", ctypes.c_int64),
+                ("max_steps_ahead", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("error", ctypes.c_int32), ("submit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("closest_rays", ctypes.c_int64), ("shadow_rays", ctypes.c_int64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("pool_closest_rays", ctypes.c_int64), ("pool_shadow_rays", ctypes.c_int64),
+                ("rl_closest_rays", ctypes.c_int64), ("rl_shadow_rays", ctypes.c_int64),
+                ("steps", ctypes.c_int64), ("pools", ctypes.c_int32)]
+
+
 _lib = None
 
 P = ctypes.c_void_p
@@ -89,6 +107,12 @@ _SIGS = {
     "cav_pbrt_get": ([P, P, P, P, P], I),
     "cav_pbrt_destroy": ([P], None),
     "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
+    "cav_serve_open": ([P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P)], I),
+    "cav_serve_submit": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
+                          ctypes.POINTER(I64)], I),
+    "cav_serve_poll": ([P, I64, ctypes.c_int32, ctypes.POINTER(ServeResult)], I),
+    "cav_serve_cancel": ([P, I64], I),
+    "cav_serve_close": ([P, ctypes.POINTER(ServeStats)], I),
     "cav_film_to_srgb8": ([P, P, I, I, P], I),
     "cav_write_png": ([ctypes.c_char_p, P, I, I], I),
     "cav_write_pfm": ([ctypes.c_char_p, P, P, I, I], I),

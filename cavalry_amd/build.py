@@ -26,9 +26,11 @@ SOURCES = [
     "render.hip",
     "rlpath.hip",
     "queue_sort.hip",
+    "serve.hip",
     "render_capi.cpp",
     "film_io.cpp",
     "pbrt.cpp",
+    "server.cpp",
     "capi.cpp",
     "camera.cpp",
     "ctx_release.cpp",

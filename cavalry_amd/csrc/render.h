@@ -125,5 +125,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
 // both integrators
 void render_raygen(const RenderArgs& a, hipStream_t s);
 void render_film(const RenderArgs& a, hipStream_t s);
+void film_launch(int width, int height, int spp, const float* res, const float* sxy, float4* partial, float* rgb,
+                 float* wt, hipStream_t s);
 
 }  // namespace cav

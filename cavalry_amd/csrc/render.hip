@@ -62,7 +62,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(int64_t nq, int width, int sp
 }
 
 // computeLighting + the shading half of evalMaterial + genNextRay for one
-// bounce (PathTracingIntegrator.cu:124-249).
+// bounce (PathTracingIntegrator.cu:124-249); the per-task work is
+// shade::path_shade.
 __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev sc, HaltonDev h,
                                                   Queue qi, Queue qo, Shadow sh,
                                                   const int32_t* __restrict__ hit_prim,
@@ -75,124 +76,29 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
     const int64_t i = live ? task_slot(perm, j) : 0;
     int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
-
-    bool push_sh = false, push_next = false;
+    ShadeOut o;
     uint32_t sid = 0;
-    v3 sro = mk3(0, 0, 0), srd = sro, contrib = sro, nro = sro, nrd = sro, nmult = sro;
-    float slim = 0.f, npdf = 0.f;
-    int ssrc = 0, sdst = 0;
-    bool ndelta = false;
-
     if (live) {
         sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
-        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
-        v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
-        const float mat_pdf = qi.pdf[i];
-        const bool from_delta = qi.delta[i] != 0;
-        const float t = hit_t[i];
-        const int mesh = sc.tri_mesh[prim];
-        const cav_mesh M = sc.meshes[mesh];
-        const v3 nn = tri_normal(sc, prim);
-        const v3 pos = add3(ro, scale3(rd, t));
-        const Bsdf bsdf = make_bsdf(M);
-        v3 t0, t1;
-        frame(nn, t0, t1);
-        v3 acc = mk3(res[sid * 3], res[sid * 3 + 1], res[sid * 3 + 2]);
-        if (M.is_light) {
-            const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
-            if (from_delta) {
-                acc = add3(acc, mul3(Le, mult));
-            } else if (depth > 0) {
-                const v3 l2s = sub3(ro, pos);
-                const float c = fabsf(dot3(nn, nrm(l2s)));
-                const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[mesh]);
-                const float w = power_heuristic(mat_pdf, lpdf);
-                if (isfinite(w)) acc = add3(acc, scale3(mul3(Le, mult), w));
-            }
-            res[sid * 3] = acc.x; res[sid * 3 + 1] = acc.y; res[sid * 3 + 2] = acc.z;
-        }
-        const v3 wo = scale3(rd, -1);
-        // light sample: DiffuseAreaLight::sampleRayToPoint / TriangleMesh::sample
-        const int li = halton_int(h, sid, dim0, sc.nlights);
-        const int lm = sc.lights[li];
-        const cav_mesh L = sc.meshes[lm];
-        const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
-        const float tmp = sqrtf(halton(h, sid, dim0 + 6));
-        const float u = 1 - tmp;
-        const float v = tmp * halton(h, sid, dim0 + 7);
-        const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
-        const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-        const v3 ln = tri_normal(sc, tri);
-        const v3 l2s = sub3(pos, lp);
-        const float c = fabsf(dot3(ln, nrm(l2s)));
-        const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
-        const v3 dir = nrm(sub3(lp, pos));
-        const float lim = len3(sub3(lp, pos));
-        const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
-        if (isfinite(prob)) {
-            const v3 em = divs3(scale3(mult, (float)sc.nlights), prob);
-            const v3 il = to_local(dir, t0, t1, nn), ol = to_local(wo, t0, t1, nn);
-            const float mp = bsdf.pdf(il, ol);
-            const float w = power_heuristic(prob, mp);
-            const float cosv = fabsf(dot3(dir, nn));
-            const v3 e = scale3(mul3(incident, bsdf.eval(il, ol)), cosv);
-            contrib = scale3(mul3(e, em), w);
-            sro = pos; srd = dir; slim = lim; ssrc = mesh; sdst = lm;
-            push_sh = true;
-        }
-        // genNextRay
-        int dim = dim0 + 8;
-        bool alive = true;
-        if (depth > 3) {
-            float q = 1;
-            q = fmn(q, 1 - mult.x);
-            q = fmn(q, 1 - mult.y);
-            q = fmn(q, 1 - mult.z);
-            q = fmx(q, 0.05f);
-            if (halton(h, sid, dim) < q) alive = false;
-            else mult = divs3(mult, 1.f - q);
-            ++dim;
-        }
-        if (alive) {
-            const float ux = halton(h, sid, dim), uy = halton(h, sid, dim + 1);
-            v3 wl;
-            float p = 0.f;
-            const v3 f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, nn), p);
-            nrd = to_world(wl, t0, t1, nn);
-            nro = add3(pos, scale3(nrd, 0.001f));
-            if (zero3(f)) nmult = mk3(0, 0, 0);
-            else nmult = divs3(scale3(mul3(mult, f), fabsf(dot3(nrd, nn))), p);
-            npdf = p;
-            ndelta = bsdf.delta();
-            push_next = true;
-        }
+        path_shade(depth, dim0, sc, h, sid, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]),
+                   mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]), qi.pdf[i], qi.delta[i] != 0, prim,
+                   hit_t[i], res + (int64_t)sid * 3, o);
     }
-    const uint32_t ss = block_push<kBlock>(push_sh, &counts[2]);
-    if (push_sh) {
-        float* r = sh.ray + (int64_t)ss * 6;
-        r[0] = sro.x; r[1] = sro.y; r[2] = sro.z; r[3] = srd.x; r[4] = srd.y; r[5] = srd.z;
-        sh.limit[ss] = slim; sh.src[ss] = ssrc; sh.dst[ss] = sdst; sh.sid[ss] = sid;
-        sh.contrib[(int64_t)ss * 3] = contrib.x;
-        sh.contrib[(int64_t)ss * 3 + 1] = contrib.y;
-        sh.contrib[(int64_t)ss * 3 + 2] = contrib.z;
+    const uint32_t ss = block_push<kBlock>(o.push_sh, &counts[2]);
+    if (o.push_sh) {
+        write_shadow(sh, ss, o);
+        sh.sid[ss] = sid;
     }
-    const uint32_t ns = block_push<kBlock>(push_next, &counts[qout]);
-    if (push_next) {
-        float* r = qo.ray + (int64_t)ns * 6;
-        r[0] = nro.x; r[1] = nro.y; r[2] = nro.z; r[3] = nrd.x; r[4] = nrd.y; r[5] = nrd.z;
+    const uint32_t ns = block_push<kBlock>(o.push_next, &counts[qout]);
+    if (o.push_next) {
+        write_next(qo, ns, o);
         qo.sid[ns] = sid;
-        qo.mult[(int64_t)ns * 3] = nmult.x;
-        qo.mult[(int64_t)ns * 3 + 1] = nmult.y;
-        qo.mult[(int64_t)ns * 3 + 2] = nmult.z;
-        qo.pdf[ns] = npdf;
-        qo.delta[ns] = ndelta ? 1 : 0;
     }
 }
 
-// DirectLighting::renderRay (Integrators/DirectLightingIntegrator.cu:22-63):
-// emission of the hit light without MIS, one light sample, and a shadow task
-// for every hit carrying Material::eval * lightsCount / pdf (no MIS weight).
+// DirectLighting::renderRay (Integrators/DirectLightingIntegrator.cu:22-63);
+// the per-task work is shade::direct_shade.
 __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Queue qi, Shadow sh,
                                                    const int32_t* __restrict__ hit_prim,
                                                    const float* __restrict__ hit_t, float* __restrict__ res,
@@ -202,56 +108,18 @@ __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Que
     const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
+    ShadeOut o;
     uint32_t sid = 0;
-    v3 pos = mk3(0, 0, 0), dir = pos, contrib = pos;
-    float lim = 0.f;
-    int mesh = 0, lm = 0;
     if (live) {
         sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
-        const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
-        mesh = sc.tri_mesh[prim];
-        const cav_mesh M = sc.meshes[mesh];
-        const v3 nn = tri_normal(sc, prim);
-        pos = add3(ro, scale3(rd, hit_t[i]));
-        if (M.is_light) {
-            const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
-            res[sid * 3] = res[sid * 3] + Le.x;
-            res[sid * 3 + 1] = res[sid * 3 + 1] + Le.y;
-            res[sid * 3 + 2] = res[sid * 3 + 2] + Le.z;
-        }
-        const int dim0 = 2;
-        const int li = halton_int(h, sid, dim0, sc.nlights);
-        lm = sc.lights[li];
-        const cav_mesh L = sc.meshes[lm];
-        const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
-        const float tmp = sqrtf(halton(h, sid, dim0 + 6));
-        const float u = 1 - tmp;
-        const float v = tmp * halton(h, sid, dim0 + 7);
-        const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
-        const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-        const v3 ln = tri_normal(sc, tri);
-        const v3 l2s = sub3(pos, lp);
-        const float c = fabsf(dot3(ln, nrm(l2s)));
-        const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
-        dir = nrm(sub3(lp, pos));
-        lim = len3(sub3(lp, pos));
-        const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
-        const Bsdf bsdf = make_bsdf(M);
-        v3 t0, t1;
-        frame(nn, t0, t1);
-        const v3 il = to_local(dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);
-        const float mult = (float)sc.nlights / prob;
-        contrib = scale3(scale3(mul3(incident, bsdf.eval(il, ol)), fabsf(dot3(dir, nn))), mult);
+        direct_shade(sc, h, sid, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), prim, hit_t[i],
+                     res + (int64_t)sid * 3, o);
     }
-    const uint32_t ss = block_push<kBlock>(live, &counts[2]);
-    if (live) {
-        float* r = sh.ray + (int64_t)ss * 6;
-        r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
-        sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = lm; sh.sid[ss] = sid;
-        sh.contrib[(int64_t)ss * 3] = contrib.x;
-        sh.contrib[(int64_t)ss * 3 + 1] = contrib.y;
-        sh.contrib[(int64_t)ss * 3 + 2] = contrib.z;
+    const uint32_t ss = block_push<kBlock>(o.push_sh, &counts[2]);
+    if (o.push_sh) {
+        write_shadow(sh, ss, o);
+        sh.sid[ss] = sid;
     }
 }
 
@@ -342,13 +210,16 @@ void render_raygen(const RenderArgs& a, hipStream_t s) {
                        a.spp_done, a.cam, a.halton, a.q[0], a.res, a.sxy, a.counts);
 }
 
-void render_film(const RenderArgs& a, hipStream_t s) {
-    const int64_t npix = (int64_t)a.width * a.height;
+void film_launch(int width, int height, int spp, const float* res, const float* sxy, float4* partial, float* rgb,
+                 float* wt, hipStream_t s) {
+    const int64_t npix = (int64_t)width * height;
     const dim3 grid((unsigned)((npix + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_film_partial, grid, dim3(kBlock), 0, s, a.width, a.height, a.spp, a.res, a.sxy,
-                       a.film_partial);
-    hipLaunchKernelGGL(k_film_final, grid, dim3(kBlock), 0, s, a.width, a.height, a.film_partial, a.film_rgb,
-                       a.film_w);
+    hipLaunchKernelGGL(k_film_partial, grid, dim3(kBlock), 0, s, width, height, spp, res, sxy, partial);
+    hipLaunchKernelGGL(k_film_final, grid, dim3(kBlock), 0, s, width, height, partial, rgb, wt);
+}
+
+void render_film(const RenderArgs& a, hipStream_t s) {
+    film_launch(a.width, a.height, a.spp, a.res, a.sxy, a.film_partial, a.film_rgb, a.film_w, s);
 }
 
 hipError_t render_direct(const RenderArgs& a, hipStream_t s) {

@@ -241,6 +241,18 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 }  // namespace
 
 namespace cav {
+SceneDev scene_dev(const cav_scene* sc) {
+    return SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
+                    sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
+}
+
+int halton_dev(cav_ctx* ctx, HaltonDev* out) {
+    RenderState* rs = nullptr;
+    const int rc = ensure_state(ctx, rs);
+    if (rc == CAV_OK) *out = HaltonDev{rs->primes, rs->start, rs->perm};
+    return rc;
+}
+
 void render_release(cav_ctx* ctx) {
     RenderState* rs = static_cast<RenderState*>(ctx->render);
     if (!rs) return;
@@ -397,8 +409,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     const int64_t nent = (int64_t)grid * grid * grid;
     if (rl && (rc = ensure_rl(ctx, rs, nq, nent * 128, nent * sc->nlights)) != CAV_OK) return rc;
     RenderArgs a = rs->a;
-    a.scene = SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
-                       sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
+    a.scene = scene_dev(sc);
     a.halton = HaltonDev{rs->primes, rs->start, rs->perm};
     std::memcpy(a.cam.m, cam->cam_to_world, sizeof a.cam.m);
     a.cam.depth = cam->depth;

@@ -1,0 +1,662 @@
+// Render server: continuous batching of render requests over resident scenes
+// (cav_serve_*). No counterpart in the reference, which renders one scene
+// per process; this is the serving layer around its integrators.
+//
+// One worker thread owns a persistent ray pool per scene. Every step it
+// admits camera-ray chunks of the scene's pending `path` / `directlighting`
+// requests into the pool (round-robin, a fixed quantum per request, within
+// the pool's free task slots and the device arena), then enqueues one bounce
+// of every task in the pool. Tasks of different requests and different
+// depths share every launch. A chunk admitted at step s has finished after
+// step s + max_depth - 1 (step s for directlighting), so completion is known
+// on the host by counting steps; the request's film is then enqueued and
+// copied to pinned staging, and an event says when it is ready. The host runs
+// at most `max_steps_ahead` steps ahead of the device. Requests whose buffers
+// do not fit the arena wait (admission by memory).
+//
+// rlpath requests learn from every sample of a bounce of their own render,
+// so they are not mixed: a second worker renders them one at a time through
+// cav_render on a context of its own; their kernels overlap the pools' on the
+// device (separate streams, default priority).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cavalry.h"
+#include "capi_internal.h"
+#include "render.h"
+#include "serve.h"
+
+namespace cav {
+SceneDev scene_dev(const cav_scene* sc);
+int halton_dev(cav_ctx* ctx, HaltonDev* out);
+}  // namespace cav
+
+using namespace cav;
+
+namespace {
+
+constexpr int kSlots = 256;  // requests in flight per pool
+
+double now_ms(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// first-fit sub-allocator over one pre-allocated block (no allocation calls
+// while serving)
+class Arena {
+public:
+    void init(char* base, size_t size) {
+        base_ = base;
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool alloc(size_t n, size_t& off) {
+        n = (n + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = (n + 255) & ~(size_t)255;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char* at(size_t off) const { return base_ + off; }
+
+private:
+    char* base_ = nullptr;
+    std::map<size_t, size_t> free_;
+};
+
+struct Request {
+    int64_t id = 0;
+    cav_scene* scene = nullptr;
+    cav_camera cam{};
+    cav_render_params prm{};
+    float* out_rgb = nullptr;
+    float* out_w = nullptr;
+    int64_t npix = 0, nsamp = 0;
+    int32_t status = CAV_SERVE_QUEUED;
+    int32_t error = CAV_OK;
+    double t_submit = 0, t_start = -1, t_done = -1;
+    int64_t closest = 0, shadow = 0;
+    // pooled requests
+    bool active = false;
+    int slot = -1;
+    int64_t next_sample = 0;
+    int64_t outstanding = 0;  // chunks not yet finished
+    size_t dev_off = 0, dev_len = 0, host_off = 0, host_len = 0;
+    float *res = nullptr, *sxy = nullptr, *rgb = nullptr, *w = nullptr;
+    float4* partial = nullptr;
+    float *h_rgb = nullptr, *h_w = nullptr;
+    hipEvent_t ready = nullptr;
+    bool finishing = false;
+    bool cancel = false;
+};
+
+struct LiveChunk {
+    Request* req;
+    int64_t done_step;
+};
+
+struct Pool {
+    cav_scene* scene = nullptr;
+    PoolDev dev{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> bufs;
+    int qin = 0;
+    int64_t step = 0;
+    int64_t reserved = 0;
+    std::vector<std::pair<LiveChunk, int64_t>> live;  // chunk, samples
+    std::deque<Request*> pending;
+    Request* slots[kSlots] = {};
+    ReqDev* d_reqs = nullptr;
+    ReqDev* h_reqs = nullptr;  // pinned staging of request entries, [ring][kSlots]
+    Chunk* d_chunks = nullptr;  // [ring][kSlots]
+    Chunk* h_chunks = nullptr;  // pinned
+    std::vector<hipEvent_t> ring;
+    int64_t done_upto = 0;      // steps known complete on the device
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+}  // namespace
+
+struct cav_server {
+    cav_ctx* ctx = nullptr;
+    cav_serve_config cfg{};
+    std::chrono::steady_clock::time_point t0;
+    HaltonDev halton{};
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    bool stop = false;
+    int64_t next_id = 1;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+    std::deque<Request*> incoming;
+    std::deque<Request*> rl_queue;
+    std::vector<std::unique_ptr<Pool>> pools;
+    Arena dev_arena, host_arena;
+    char* dev_base = nullptr;
+    char* host_base = nullptr;
+    std::thread worker, rl_worker;
+    cav_ctx* rl_ctx = nullptr;
+    unsigned long long pool_rays[2] = {0, 0};
+    int64_t rl_rays[2] = {0, 0};
+    std::string error;
+};
+
+namespace {
+
+int alloc_dev(std::vector<void*>& list, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return CAV_ERR_OOM;
+    list.push_back(*p);
+    return CAV_OK;
+}
+
+int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
+    std::unique_ptr<Pool> p(new Pool());
+    p->scene = sc;
+    const size_t C = (size_t)sv->cfg.pool_tasks;
+    PoolDev& d = p->dev;
+    d.scene = scene_dev(sc);
+    d.halton = sv->halton;
+    d.cap = (int64_t)C;
+    auto& L = p->bufs;
+    int rc = CAV_OK;
+    auto A = [&](auto*& ptr, size_t bytes) {
+        if (rc == CAV_OK) rc = alloc_dev(L, reinterpret_cast<void**>(&ptr), bytes);
+    };
+    for (int k = 0; k < 2; ++k) {
+        A(d.q[k].ray, 24 * C); A(d.q[k].sid, 4 * C); A(d.q[k].mult, 12 * C); A(d.q[k].pdf, 4 * C);
+        A(d.q[k].delta, C); A(d.rid[k], 4 * C); A(d.depth[k], C);
+    }
+    A(d.sh.ray, 24 * C); A(d.sh.limit, 4 * C); A(d.sh.src, 4 * C); A(d.sh.dst, 4 * C); A(d.sh.sid, 4 * C);
+    A(d.sh.contrib, 12 * C); A(d.sh_rid, 4 * C); A(d.vis, C); A(d.hit_prim, 4 * C); A(d.hit_t, 4 * C);
+    A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 64);
+    A(p->d_reqs, sizeof(ReqDev) * kSlots);
+    const int ring = sv->cfg.max_steps_ahead + 2;
+    A(p->d_chunks, sizeof(Chunk) * kSlots * ring);
+    if (rc != CAV_OK) {
+        for (void* q : L) (void)hipFree(q);
+        return rc;
+    }
+    d.reqs = p->d_reqs;
+    bool ok = hipHostMalloc(reinterpret_cast<void**>(&p->h_reqs), sizeof(ReqDev) * kSlots * ring, 0) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void**>(&p->h_chunks), sizeof(Chunk) * kSlots * ring, 0) == hipSuccess &&
+              hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+    p->ring.assign(ring, nullptr);
+    for (auto& e : p->ring) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.counts, 0, 64, p->stream) == hipSuccess &&
+         hipMemsetAsync(d.rays, 0, 64, p->stream) == hipSuccess &&
+         hipMemsetAsync(d.trace_counters, 0, 64, p->stream) == hipSuccess &&
+         hipStreamSynchronize(p->stream) == hipSuccess;
+    if (!ok) return CAV_ERR_HIP;
+    out = p.get();
+    sv->pools.push_back(std::move(p));
+    return CAV_OK;
+}
+
+void destroy_pool(Pool* p) {
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (int k = 0; k < 2; ++k)
+        if (p->graph[k]) (void)hipGraphExecDestroy(p->graph[k]);
+    for (auto e : p->ring)
+        if (e) (void)hipEventDestroy(e);
+    for (void* q : p->bufs) (void)hipFree(q);
+    if (p->h_reqs) (void)hipHostFree(p->h_reqs);
+    if (p->h_chunks) (void)hipHostFree(p->h_chunks);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+}
+
+Pool* pool_for(cav_server* sv, cav_scene* sc) {
+    for (auto& p : sv->pools)
+        if (p->scene == sc) return p.get();
+    return nullptr;
+}
+
+// buffers and a slot for a request's first admission; false if they do not
+// fit now
+bool activate(cav_server* sv, Pool* p, Request* r) {
+    int slot = -1;
+    for (int k = 0; k < kSlots; ++k)
+        if (!p->slots[k]) { slot = k; break; }
+    if (slot < 0) return false;
+    const size_t S = (size_t)r->nsamp, P = (size_t)r->npix;
+    const size_t dlen = 12 * S + 8 * S + 144 * P + 12 * P + 4 * P + 5 * 256;
+    const size_t hlen = 16 * P + 2 * 256;
+    size_t doff = 0, hoff = 0;
+    if (!sv->dev_arena.alloc(dlen, doff)) return false;
+    if (!sv->host_arena.alloc(hlen, hoff)) {
+        sv->dev_arena.release(doff, dlen);
+        return false;
+    }
+    auto carve = [](char*& c, size_t bytes) {
+        char* q = c;
+        c += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    char* c = sv->dev_arena.at(doff);
+    r->res = reinterpret_cast<float*>(carve(c, 12 * S));
+    r->sxy = reinterpret_cast<float*>(carve(c, 8 * S));
+    r->partial = reinterpret_cast<float4*>(carve(c, 144 * P));
+    r->rgb = reinterpret_cast<float*>(carve(c, 12 * P));
+    r->w = reinterpret_cast<float*>(carve(c, 4 * P));
+    char* h = sv->host_arena.at(hoff);
+    r->h_rgb = reinterpret_cast<float*>(carve(h, 12 * P));
+    r->h_w = reinterpret_cast<float*>(carve(h, 4 * P));
+    r->dev_off = doff; r->dev_len = dlen; r->host_off = hoff; r->host_len = hlen;
+    r->slot = slot;
+    p->slots[slot] = r;
+    // staged per step: an entry copied at an earlier step may still be queued
+    ReqDev& e = p->h_reqs[(size_t)(p->step % (int64_t)p->ring.size()) * kSlots + slot];
+    std::memcpy(e.cam.m, r->cam.cam_to_world, sizeof e.cam.m);
+    e.cam.depth = r->cam.depth;
+    e.cam.width = r->cam.width;
+    e.cam.height = r->cam.height;
+    e.spp = r->prm.spp;
+    e.max_depth = r->prm.integrator == CAV_INTEGRATOR_DIRECT ? 1 : r->prm.max_depth;
+    e.integrator = r->prm.integrator;
+    e.res = r->res;
+    e.sxy = r->sxy;
+    // the pool's earlier steps no longer read this slot (its previous owner
+    // finished), so the stream-ordered copy is safe
+    (void)hipMemcpyAsync(p->d_reqs + slot, &e, sizeof(ReqDev), hipMemcpyHostToDevice, p->stream);
+    r->active = true;
+    return true;
+}
+
+void release_request_memory(cav_server* sv, Request* r) {
+    if (r->dev_len) sv->dev_arena.release(r->dev_off, r->dev_len);
+    if (r->host_len) sv->host_arena.release(r->host_off, r->host_len);
+    r->dev_len = r->host_len = 0;
+}
+
+int capture_graphs(Pool* p) {
+    for (int k = 0; k < 2; ++k) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return CAV_ERR_HIP;
+        const hipError_t e = pool_bounce(p->dev, k, p->stream);
+        if (hipStreamEndCapture(p->stream, &g) != hipSuccess || e != hipSuccess) return CAV_ERR_HIP;
+        const hipError_t ie = hipGraphInstantiate(&p->graph[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return CAV_ERR_HIP;
+    }
+    return CAV_OK;
+}
+
+// one step of one pool; called with sv->mu held (it touches request state),
+// enqueues work only
+int pool_step(cav_server* sv, Pool* p) {
+    const int ring = (int)p->ring.size();
+    // wait until the chunk/event ring slot of step - ring + 1 is free
+    while (p->step - p->done_upto >= sv->cfg.max_steps_ahead) {
+        const int64_t s = p->done_upto;
+        if (hipEventSynchronize(p->ring[s % ring]) != hipSuccess) return CAV_ERR_HIP;
+        p->done_upto = s + 1;
+    }
+    // admission: round-robin quantum over the scene's pending requests
+    Chunk* hc = p->h_chunks + (size_t)(p->step % ring) * kSlots;
+    int nch = 0;
+    int64_t total = 0;
+    const int64_t cap = p->dev.cap;
+    for (size_t k = 0; k < p->pending.size() && nch < kSlots;) {
+        Request* r = p->pending[k];
+        if (r->cancel && !r->active) {
+            r->status = CAV_SERVE_CANCELLED;
+            p->pending.erase(p->pending.begin() + (std::ptrdiff_t)k);
+            sv->done_cv.notify_all();
+            continue;
+        }
+        const int64_t room = cap - p->reserved;
+        if (room <= 0) break;
+        if (!r->active && !activate(sv, p, r)) { ++k; continue; }
+        const int64_t n = std::min({room, sv->cfg.chunk_samples, r->nsamp - r->next_sample});
+        if (r->t_start < 0) {
+            r->t_start = now_ms(sv->t0);
+            r->status = CAV_SERVE_RUNNING;
+        }
+        hc[nch] = Chunk{r->slot, 0, r->next_sample, n, total};
+        ++nch;
+        total += n;
+        r->next_sample += n;
+        r->outstanding += 1;
+        p->reserved += n;
+        const int64_t last = r->prm.integrator == CAV_INTEGRATOR_DIRECT ? 0 : r->prm.max_depth - 1;
+        p->live.push_back({LiveChunk{r, p->step + last}, n});
+        if (r->next_sample == r->nsamp) p->pending.erase(p->pending.begin() + (std::ptrdiff_t)k);
+        else ++k;
+    }
+    if (nch) {
+        Chunk* dc = p->d_chunks + (size_t)(p->step % ring) * kSlots;
+        if (hipMemcpyAsync(dc, hc, sizeof(Chunk) * nch, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+            return CAV_ERR_HIP;
+        pool_raygen(p->dev, dc, nch, total, p->qin, p->stream);
+    }
+    if (p->graph[p->qin]) {
+        if (hipGraphLaunch(p->graph[p->qin], p->stream) != hipSuccess) return CAV_ERR_HIP;
+    } else if (pool_bounce(p->dev, p->qin, p->stream) != hipSuccess) {
+        return CAV_ERR_HIP;
+    }
+    // chunks whose last bounce was this step
+    for (size_t k = 0; k < p->live.size();) {
+        if (p->live[k].first.done_step != p->step) { ++k; continue; }
+        Request* r = p->live[k].first.req;
+        p->reserved -= p->live[k].second;
+        p->live[k] = p->live.back();
+        p->live.pop_back();
+        if (--r->outstanding == 0 && r->next_sample == r->nsamp) {
+            film_launch(r->cam.width, r->cam.height, r->prm.spp, r->res, r->sxy, r->partial, r->rgb, r->w, p->stream);
+            (void)hipMemcpyAsync(r->h_rgb, r->rgb, 12 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
+            (void)hipMemcpyAsync(r->h_w, r->w, 4 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
+            if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(r->ready, p->stream) != hipSuccess)
+                return CAV_ERR_HIP;
+            r->finishing = true;
+            p->slots[r->slot] = nullptr;
+        }
+    }
+    if (hipEventRecord(p->ring[p->step % ring], p->stream) != hipSuccess) return CAV_ERR_HIP;
+    ++p->step;
+    p->qin ^= 1;
+    return CAV_OK;
+}
+
+void finish_ready(cav_server* sv) {
+    for (auto& kv : sv->reqs) {
+        Request* r = kv.second.get();
+        if (!r->finishing) continue;
+        const hipError_t q = hipEventQuery(r->ready);
+        if (q == hipErrorNotReady) continue;
+        r->finishing = false;
+        (void)hipEventDestroy(r->ready);
+        r->ready = nullptr;
+        if (q != hipSuccess) {
+            r->status = CAV_SERVE_FAILED;
+            r->error = CAV_ERR_HIP;
+        } else if (r->cancel) {
+            r->status = CAV_SERVE_CANCELLED;
+        } else {
+            std::memcpy(r->out_rgb, r->h_rgb, 12 * (size_t)r->npix);
+            std::memcpy(r->out_w, r->h_w, 4 * (size_t)r->npix);
+            r->status = CAV_SERVE_DONE;
+        }
+        r->t_done = now_ms(sv->t0);
+        release_request_memory(sv, r);
+        sv->done_cv.notify_all();
+    }
+}
+
+void worker_main(cav_server* sv) {
+    (void)hipSetDevice(sv->ctx->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        while (!sv->incoming.empty()) {
+            Request* r = sv->incoming.front();
+            sv->incoming.pop_front();
+            Pool* p = pool_for(sv, r->scene);
+            int rc = CAV_OK;
+            if (!p) {
+                rc = make_pool(sv, r->scene, p);
+                if (rc == CAV_OK && sv->cfg.use_graphs) rc = capture_graphs(p);
+            }
+            if (rc != CAV_OK) {
+                r->status = CAV_SERVE_FAILED;
+                r->error = rc;
+                r->t_done = now_ms(sv->t0);
+                sv->done_cv.notify_all();
+                continue;
+            }
+            p->pending.push_back(r);
+        }
+        bool busy = false;
+        for (auto& up : sv->pools) {
+            Pool* p = up.get();
+            if (p->live.empty() && p->pending.empty()) continue;
+            busy = true;
+            if (pool_step(sv, p) != CAV_OK) {
+                sv->error = "a HIP call failed while serving";
+                for (auto& kv : sv->reqs)
+                    if (kv.second->status == CAV_SERVE_QUEUED || kv.second->status == CAV_SERVE_RUNNING) {
+                        kv.second->status = CAV_SERVE_FAILED;
+                        kv.second->error = CAV_ERR_HIP;
+                    }
+                sv->done_cv.notify_all();
+                return;
+            }
+        }
+        finish_ready(sv);
+        bool finishing = false;
+        for (auto& kv : sv->reqs) finishing = finishing || kv.second->finishing;
+        if (sv->stop && !busy && !finishing && sv->incoming.empty()) break;
+        if (busy) {
+            // let submitters and pollers in between steps
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        } else if (finishing) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+            lk.lock();
+        } else {
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->incoming.empty(); });
+        }
+    }
+    for (auto& up : sv->pools) {
+        unsigned long long t[2] = {0, 0};
+        (void)hipStreamSynchronize(up->stream);
+        if (hipMemcpy(t, up->dev.rays, 16, hipMemcpyDeviceToHost) == hipSuccess) {
+            sv->pool_rays[0] += t[0];
+            sv->pool_rays[1] += t[1];
+        }
+    }
+}
+
+void rl_main(cav_server* sv) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        sv->cv.wait(lk, [&] { return sv->stop || !sv->rl_queue.empty(); });
+        if (sv->rl_queue.empty()) {
+            if (sv->stop) break;
+            continue;
+        }
+        Request* r = sv->rl_queue.front();
+        sv->rl_queue.pop_front();
+        if (r->cancel) {
+            r->status = CAV_SERVE_CANCELLED;
+            sv->done_cv.notify_all();
+            continue;
+        }
+        r->status = CAV_SERVE_RUNNING;
+        r->t_start = now_ms(sv->t0);
+        lk.unlock();
+        cav_render_stats st{};
+        const int rc = cav_render(sv->rl_ctx, r->scene, &r->cam, &r->prm, r->out_rgb, r->out_w, &st);
+        lk.lock();
+        r->t_done = now_ms(sv->t0);
+        r->closest = st.closest_rays;
+        r->shadow = st.shadow_rays;
+        sv->rl_rays[0] += st.closest_rays;
+        sv->rl_rays[1] += st.shadow_rays;
+        if (rc != CAV_OK) {
+            r->status = CAV_SERVE_FAILED;
+            r->error = rc;
+        } else {
+            r->status = r->cancel ? CAV_SERVE_CANCELLED : CAV_SERVE_DONE;
+        }
+        sv->done_cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) {
+    if (!ctx || !out) return CAV_ERR_INVALID;
+    *out = nullptr;
+    std::unique_ptr<cav_server> sv(new cav_server());
+    sv->ctx = ctx;
+    if (cfg) sv->cfg = *cfg;
+    cav_serve_config& c = sv->cfg;
+    if (c.arena_bytes <= 0) c.arena_bytes = (int64_t)16 << 30;
+    if (c.pool_tasks <= 0) c.pool_tasks = (int64_t)4 << 20;
+    if (c.max_steps_ahead <= 0) c.max_steps_ahead = 4;
+    if (c.chunk_samples <= 0) c.chunk_samples = c.pool_tasks / 4;
+    if (c.host_arena_bytes <= 0) c.host_arena_bytes = (int64_t)1 << 30;
+    if (c.pool_tasks >= ((int64_t)1 << 31)) return fail(ctx, CAV_ERR_INVALID, "pool_tasks must be below 2^31");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CAV_ERR_HIP, "hipSetDevice");
+    int rc = halton_dev(ctx, &sv->halton);
+    if (rc != CAV_OK) return rc;
+    if (hipMalloc(reinterpret_cast<void**>(&sv->dev_base), (size_t)c.arena_bytes) != hipSuccess)
+        return fail(ctx, CAV_ERR_OOM, "server device arena of " + std::to_string(c.arena_bytes) + " bytes");
+    if (hipHostMalloc(reinterpret_cast<void**>(&sv->host_base), (size_t)c.host_arena_bytes, 0) != hipSuccess) {
+        (void)hipFree(sv->dev_base);
+        return fail(ctx, CAV_ERR_OOM, "server pinned host arena");
+    }
+    sv->dev_arena.init(sv->dev_base, (size_t)c.arena_bytes);
+    sv->host_arena.init(sv->host_base, (size_t)c.host_arena_bytes);
+    if ((rc = cav_ctx_create(ctx->device, &sv->rl_ctx)) != CAV_OK) {
+        (void)hipFree(sv->dev_base);
+        (void)hipHostFree(sv->host_base);
+        return fail(ctx, rc, "server rlpath context");
+    }
+    sv->t0 = std::chrono::steady_clock::now();
+    cav_server* raw = sv.get();
+    raw->worker = std::thread(worker_main, raw);
+    raw->rl_worker = std::thread(rl_main, raw);
+    *out = sv.release();
+    return CAV_OK;
+}
+
+int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, const cav_render_params* prm,
+                     float* film_rgb, float* film_weight, int64_t* id) {
+    if (!sv || !scene || !cam || !prm || !film_rgb || !film_weight || !id) return CAV_ERR_INVALID;
+    if (prm->integrator != CAV_INTEGRATOR_PATH && prm->integrator != CAV_INTEGRATOR_RLPATH &&
+        prm->integrator != CAV_INTEGRATOR_DIRECT)
+        return fail(sv->ctx, CAV_ERR_INVALID, "unknown integrator");
+    if (prm->spp < 1 || cam->width < 1 || cam->height < 1 ||
+        (prm->integrator != CAV_INTEGRATOR_DIRECT && prm->max_depth < 1))
+        return fail(sv->ctx, CAV_ERR_INVALID, "spp, max_depth and resolution must be positive");
+    const int64_t npix = (int64_t)cam->width * cam->height;
+    if (npix * prm->spp >= ((int64_t)1 << 32)) return fail(sv->ctx, CAV_ERR_INVALID, "more than 2^32 samples");
+    if (prm->integrator == CAV_INTEGRATOR_PATH &&
+        2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0) > 1024)
+        return fail(sv->ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
+    if (prm->integrator == CAV_INTEGRATOR_PATH && prm->max_depth > 255)
+        return fail(sv->ctx, CAV_ERR_INVALID, "max_depth above 255");
+    std::unique_ptr<Request> r(new Request());
+    r->scene = scene;
+    r->cam = *cam;
+    r->prm = *prm;
+    r->out_rgb = film_rgb;
+    r->out_w = film_weight;
+    r->npix = npix;
+    r->nsamp = npix * prm->spp;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop || !sv->error.empty()) return fail(sv->ctx, CAV_ERR_INVALID, "server is closing or failed");
+    if (prm->integrator != CAV_INTEGRATOR_RLPATH) {
+        const size_t need = 20 * (size_t)r->nsamp + 160 * (size_t)npix + 5 * 256;
+        if (need > (size_t)sv->cfg.arena_bytes || 16 * (size_t)npix + 512 > (size_t)sv->cfg.host_arena_bytes)
+            return fail(sv->ctx, CAV_ERR_OOM, "request larger than the server's arenas");
+    }
+    r->id = sv->next_id++;
+    r->t_submit = now_ms(sv->t0);
+    *id = r->id;
+    Request* raw = r.get();
+    sv->reqs.emplace(raw->id, std::move(r));
+    if (prm->integrator == CAV_INTEGRATOR_RLPATH) sv->rl_queue.push_back(raw);
+    else sv->incoming.push_back(raw);
+    sv->cv.notify_all();
+    return CAV_OK;
+}
+
+int cav_serve_poll(cav_server* sv, int64_t id, int32_t wait, cav_serve_result* out) {
+    if (!sv || !out) return CAV_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return CAV_ERR_NOT_FOUND;
+    Request* r = it->second.get();
+    auto final_state = [&] {
+        return r->status == CAV_SERVE_DONE || r->status == CAV_SERVE_CANCELLED || r->status == CAV_SERVE_FAILED;
+    };
+    if (wait) sv->done_cv.wait(lk, final_state);
+    out->status = r->status;
+    out->error = r->error;
+    out->submit_ms = r->t_submit;
+    out->start_ms = r->t_start;
+    out->done_ms = r->t_done;
+    out->closest_rays = r->closest;
+    out->shadow_rays = r->shadow;
+    if (final_state()) sv->reqs.erase(it);  // the id is released with its final status
+    return CAV_OK;
+}
+
+int cav_serve_cancel(cav_server* sv, int64_t id) {
+    if (!sv) return CAV_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return CAV_ERR_NOT_FOUND;
+    it->second->cancel = true;
+    sv->cv.notify_all();
+    return CAV_OK;
+}
+
+int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
+    if (!sv) return CAV_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        sv->cv.notify_all();
+    }
+    if (sv->worker.joinable()) sv->worker.join();
+    if (sv->rl_worker.joinable()) sv->rl_worker.join();
+    if (stats) {
+        stats->pool_closest_rays = (int64_t)sv->pool_rays[0];
+        stats->pool_shadow_rays = (int64_t)sv->pool_rays[1];
+        stats->rl_closest_rays = sv->rl_rays[0];
+        stats->rl_shadow_rays = sv->rl_rays[1];
+        int64_t steps = 0;
+        for (auto& p : sv->pools) steps += p->step;
+        stats->steps = steps;
+        stats->pools = (int32_t)sv->pools.size();
+    }
+    for (auto& p : sv->pools) destroy_pool(p.get());
+    for (auto& kv : sv->reqs)
+        if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
+    if (sv->rl_ctx) cav_ctx_destroy(sv->rl_ctx);
+    if (sv->dev_base) (void)hipFree(sv->dev_base);
+    if (sv->host_base) (void)hipHostFree(sv->host_base);
+    delete sv;
+    return CAV_OK;
+}
+
+}  // extern "C"

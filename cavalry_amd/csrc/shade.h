@@ -325,6 +325,162 @@ __device__ inline v3 apply_m(const float* m, v3 v) {
     return mk3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
 }
 
+// What shading one task produces: an optional shadow task and an optional
+// continuation ray.
+struct ShadeOut {
+    bool push_sh = false, push_next = false;
+    v3 sro, srd, contrib, nro, nrd, nmult;
+    float slim = 0.f, npdf = 0.f;
+    int ssrc = 0, sdst = 0;
+    bool ndelta = false;
+};
+
+// One `path` bounce of one task (PathTracingIntegrator.cu:124-249):
+// emission with MIS (added to acc3, the sample's radiance), the light sample
+// (a shadow task carrying the unoccluded contribution), Russian roulette and
+// the BSDF sample of the next ray. dim0 = the task's first Halton dimension
+// of this bounce.
+__device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const HaltonDev& h, uint32_t sid, v3 ro,
+                                  v3 rd, v3 mult, float mat_pdf, bool from_delta, int prim, float t, float* acc3,
+                                  ShadeOut& o) {
+    const int mesh = sc.tri_mesh[prim];
+    const cav_mesh M = sc.meshes[mesh];
+    const v3 nn = tri_normal(sc, prim);
+    const v3 pos = add3(ro, scale3(rd, t));
+    const Bsdf bsdf = make_bsdf(M);
+    v3 t0, t1;
+    frame(nn, t0, t1);
+    if (M.is_light) {
+        v3 acc = mk3(acc3[0], acc3[1], acc3[2]);
+        const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
+        if (from_delta) {
+            acc = add3(acc, mul3(Le, mult));
+        } else if (depth > 0) {
+            const v3 l2s = sub3(ro, pos);
+            const float c = fabsf(dot3(nn, nrm(l2s)));
+            const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[mesh]);
+            const float w = power_heuristic(mat_pdf, lpdf);
+            if (isfinite(w)) acc = add3(acc, scale3(mul3(Le, mult), w));
+        }
+        acc3[0] = acc.x; acc3[1] = acc.y; acc3[2] = acc.z;
+    }
+    const v3 wo = scale3(rd, -1);
+    // light sample: DiffuseAreaLight::sampleRayToPoint / TriangleMesh::sample
+    const int li = halton_int(h, sid, dim0, sc.nlights);
+    const int lm = sc.lights[li];
+    const cav_mesh L = sc.meshes[lm];
+    const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
+    const float tmp = sqrtf(halton(h, sid, dim0 + 6));
+    const float u = 1 - tmp;
+    const float v = tmp * halton(h, sid, dim0 + 7);
+    const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
+    const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
+    const v3 ln = tri_normal(sc, tri);
+    const v3 l2s = sub3(pos, lp);
+    const float c = fabsf(dot3(ln, nrm(l2s)));
+    const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
+    const v3 dir = nrm(sub3(lp, pos));
+    const float lim = len3(sub3(lp, pos));
+    const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+    if (isfinite(prob)) {
+        const v3 em = divs3(scale3(mult, (float)sc.nlights), prob);
+        const v3 il = to_local(dir, t0, t1, nn), ol = to_local(wo, t0, t1, nn);
+        const float mp = bsdf.pdf(il, ol);
+        const float w = power_heuristic(prob, mp);
+        const float cosv = fabsf(dot3(dir, nn));
+        const v3 e = scale3(mul3(incident, bsdf.eval(il, ol)), cosv);
+        o.contrib = scale3(mul3(e, em), w);
+        o.sro = pos; o.srd = dir; o.slim = lim; o.ssrc = mesh; o.sdst = lm;
+        o.push_sh = true;
+    }
+    // genNextRay
+    int dim = dim0 + 8;
+    bool alive = true;
+    if (depth > 3) {
+        float q = 1;
+        q = fmn(q, 1 - mult.x);
+        q = fmn(q, 1 - mult.y);
+        q = fmn(q, 1 - mult.z);
+        q = fmx(q, 0.05f);
+        if (halton(h, sid, dim) < q) alive = false;
+        else mult = divs3(mult, 1.f - q);
+        ++dim;
+    }
+    if (alive) {
+        const float ux = halton(h, sid, dim), uy = halton(h, sid, dim + 1);
+        v3 wl;
+        float p = 0.f;
+        const v3 f = bsdf.sample(ux, uy, wl, to_local(wo, t0, t1, nn), p);
+        o.nrd = to_world(wl, t0, t1, nn);
+        o.nro = add3(pos, scale3(o.nrd, 0.001f));
+        if (zero3(f)) o.nmult = mk3(0, 0, 0);
+        else o.nmult = divs3(scale3(mul3(mult, f), fabsf(dot3(o.nrd, nn))), p);
+        o.npdf = p;
+        o.ndelta = bsdf.delta();
+        o.push_next = true;
+    }
+}
+
+// DirectLighting::renderRay for one camera-ray hit: the light's emission
+// without MIS (added to acc3) and one light sample weighted by
+// lightsCount / pdf without MIS; the shadow task carries it.
+__device__ inline void direct_shade(const SceneDev& sc, const HaltonDev& h, uint32_t sid, v3 ro, v3 rd, int prim,
+                                    float t, float* acc3, ShadeOut& o) {
+    const int mesh = sc.tri_mesh[prim];
+    const cav_mesh M = sc.meshes[mesh];
+    const v3 nn = tri_normal(sc, prim);
+    const v3 pos = add3(ro, scale3(rd, t));
+    if (M.is_light) {
+        const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
+        acc3[0] = acc3[0] + Le.x;
+        acc3[1] = acc3[1] + Le.y;
+        acc3[2] = acc3[2] + Le.z;
+    }
+    const int dim0 = 2;
+    const int li = halton_int(h, sid, dim0, sc.nlights);
+    const int lm = sc.lights[li];
+    const cav_mesh L = sc.meshes[lm];
+    const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
+    const float tmp = sqrtf(halton(h, sid, dim0 + 6));
+    const float u = 1 - tmp;
+    const float v = tmp * halton(h, sid, dim0 + 7);
+    const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
+    const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
+    const v3 ln = tri_normal(sc, tri);
+    const v3 l2s = sub3(pos, lp);
+    const float c = fabsf(dot3(ln, nrm(l2s)));
+    const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
+    const v3 dir = nrm(sub3(lp, pos));
+    const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+    const Bsdf bsdf = make_bsdf(M);
+    v3 t0, t1;
+    frame(nn, t0, t1);
+    const v3 il = to_local(dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);
+    const float m = (float)sc.nlights / prob;
+    o.contrib = scale3(scale3(mul3(incident, bsdf.eval(il, ol)), fabsf(dot3(dir, nn))), m);
+    o.sro = pos; o.srd = dir; o.slim = len3(sub3(lp, pos)); o.ssrc = mesh; o.sdst = lm;
+    o.push_sh = true;
+}
+
+__device__ inline void write_shadow(const Shadow& sh, uint32_t ss, const ShadeOut& o) {
+    float* r = sh.ray + (int64_t)ss * 6;
+    r[0] = o.sro.x; r[1] = o.sro.y; r[2] = o.sro.z; r[3] = o.srd.x; r[4] = o.srd.y; r[5] = o.srd.z;
+    sh.limit[ss] = o.slim; sh.src[ss] = o.ssrc; sh.dst[ss] = o.sdst;
+    sh.contrib[(int64_t)ss * 3] = o.contrib.x;
+    sh.contrib[(int64_t)ss * 3 + 1] = o.contrib.y;
+    sh.contrib[(int64_t)ss * 3 + 2] = o.contrib.z;
+}
+
+__device__ inline void write_next(const Queue& qo, uint32_t ns, const ShadeOut& o) {
+    float* r = qo.ray + (int64_t)ns * 6;
+    r[0] = o.nro.x; r[1] = o.nro.y; r[2] = o.nro.z; r[3] = o.nrd.x; r[4] = o.nrd.y; r[5] = o.nrd.z;
+    qo.mult[(int64_t)ns * 3] = o.nmult.x;
+    qo.mult[(int64_t)ns * 3 + 1] = o.nmult.y;
+    qo.mult[(int64_t)ns * 3 + 2] = o.nmult.z;
+    qo.pdf[ns] = o.npdf;
+    qo.delta[ns] = o.ndelta ? 1 : 0;
+}
+
 // Queue slot handled by thread j: through the material order when the queue
 // was sorted, else j itself.
 __device__ inline int64_t task_slot(const uint32_t* perm, int64_t j) { return perm ? (int64_t)perm[j] : j; }

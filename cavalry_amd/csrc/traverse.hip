@@ -223,6 +223,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
     if (ovf) atomicAdd(overflow, ovf);
 }
 
+// zeroes n words; a kernel rather than hipMemsetAsync so the launch sequence
+// can be captured into a hipGraph
+__global__ void k_clear_words(uint32_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
 int persistent_blocks(const void* kernel) {
     static int cus = 0;
     if (cus == 0) {
@@ -276,8 +282,7 @@ hipError_t trace_closest_queue(const TNode* tn, const TTri* tt, int ntris, const
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
                                hipStream_t s) {
-    hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 2, 2);
     const void* k = reinterpret_cast<const void*>(&k_trace<true>);
     int64_t blocks = persistent_blocks(k);
     const int64_t need = (capacity + kBlock - 1) / kBlock;
@@ -293,8 +298,7 @@ hipError_t trace_occluded_queue(const TNode* tn, const TTri* tt, int ntris, cons
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
                                 uint32_t* counters, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 4, 2);
     const void* k = reinterpret_cast<const void*>(&k_trace<false>);
     int64_t blocks = persistent_blocks(k);
     const int64_t need = (capacity + kBlock - 1) / kBlock;

@@ -206,3 +206,111 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
     tris, meshes = b.arrays()
     return SceneArrays(tris, meshes, (0.5, 0.5, -1.3), (0.5, 0.5, 0.5), (0, 1, 0), 40.0,
                        width, height, max_depth)
+
+
+def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere subdivided `level` times: (vertices [V, 3] float64,
+    faces [20 * 4^level, 3] int64), faces wound outward."""
+    t = (1.0 + 5 ** 0.5) / 2
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        inv = inv.reshape(3, -1) + base
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = inv[0], inv[1], inv[2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+def _value_noise(p: np.ndarray, seed: int) -> np.ndarray:
+    """Trilinear value noise in [-1, 1] on the integer lattice (smoothstep
+    weights, lattice values from a seeded hash)."""
+    i = np.floor(p).astype(np.int64)
+    fr = p - i
+    w = fr * fr * (3 - 2 * fr)
+
+    def h(ix, iy, iz):
+        x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        x = (x ^ (x >> 13)) * 1274126177
+        return ((x ^ (x >> 16)) & 0xFFFF) / 32767.5 - 1.0
+
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                wt = (w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) * (
+                    w[:, 2] if dz else 1 - w[:, 2])
+                out = out + wt * h(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz)
+    return out
+
+
+def interior(n_tris: int, seed: int = 1, width: int = 1920, height: int = 1080, max_depth: int = 16) -> SceneArrays:
+    """Procedural interior (the wavefront-at-scale scene family): a 10 x 4 x 10
+    room, 64 ceiling quad lights (two equal triangles each, L ~ U[5, 20]),
+    displaced icospheres (level 6-8, radial value-noise displacement of
+    amplitude U[0.02, 0.1] r) and boxes, about n_tris triangles in total.
+    Materials per object: 40% matte, 30% plastic (alpha ~ U[0.05, 0.5]),
+    15% mirror, 15% glass (index 1.5). No textures."""
+    rng = _rng(seed, 7)
+    b = _Builder()
+    X, Y, Z = 10.0, 4.0, 10.0
+    # the room's faces, normals pointing inward
+    b.mesh(split_quad((0, 0, 0), (0, 0, Z), (X, 0, 0), 4), kd=(0.6, 0.6, 0.6))        # floor, +y
+    b.mesh(split_quad((0, Y, 0), (X, 0, 0), (0, 0, Z), 4), kd=(0.7, 0.7, 0.7))        # ceiling, -y
+    b.mesh(split_quad((0, 0, 0), (X, 0, 0), (0, Y, 0), 2), kd=(0.5, 0.3, 0.3))        # z = 0, +z
+    b.mesh(split_quad((0, 0, Z), (0, Y, 0), (X, 0, 0), 2), kd=(0.3, 0.5, 0.3))        # z = Z, -z
+    b.mesh(split_quad((0, 0, 0), (0, Y, 0), (0, 0, Z), 2), kd=(0.3, 0.3, 0.5))        # x = 0, +x
+    b.mesh(split_quad((X, 0, 0), (0, 0, Z), (0, Y, 0), 2), kd=(0.5, 0.5, 0.3))        # x = X, -x
+    for i in range(8):
+        for j in range(8):
+            cx, cz = (i + 0.5) * X / 8, (j + 0.5) * Z / 8
+            s = 0.2
+            L = float(rng.uniform(5, 20))
+            b.mesh(quad((cx - s, Y - 0.01, cz - s), (cx + s, Y - 0.01, cz - s), (cx + s, Y - 0.01, cz + s),
+                        (cx - s, Y - 0.01, cz + s)), kd=(0.5, 0.5, 0.5), emission=(L, L, L))
+
+    def material():
+        u = float(rng.uniform())
+        kd = tuple(float(x) for x in rng.uniform(0.1, 0.9, 3))
+        if u < 0.4:
+            return dict(material=MATTE, kd=kd)
+        if u < 0.7:
+            return dict(material=PLASTIC, kd=kd, ks=(0.25, 0.25, 0.25), roughness=float(rng.uniform(0.05, 0.5)))
+        if u < 0.85:
+            return dict(material=MIRROR, kr=(0.9, 0.9, 0.9))
+        return dict(material=GLASS, kr=(1, 1, 1), kt=(1, 1, 1), ior=1.5)
+
+    budget = n_tris - 20 - 128
+    k = 0
+    while budget > 20 * 4 ** 6 or k == 0:
+        level = 8 if budget >= 20 * 4 ** 8 else (7 if budget >= 20 * 4 ** 7 else 6)
+        v, f = icosphere(level)
+        r = float(rng.uniform(0.3, 0.8))
+        amp = float(rng.uniform(0.02, 0.1)) * r
+        d = 1.0 + amp / r * _value_noise(v * 3.0 + k * 17.0, seed + k)
+        c = np.array([rng.uniform(1.5, X - 1.5), r + 0.05, rng.uniform(2.5, Z - 1.5)])
+        pts = c + v * (r * d)[:, None]
+        b.mesh(pts[f].reshape(-1, 9), **material())
+        budget -= len(f)
+        k += 1
+    for _ in range(max(4, budget // 12)):
+        lo = np.array([rng.uniform(0.5, X - 1.5), 0.0, rng.uniform(1.5, Z - 1.5)])
+        hi = lo + rng.uniform(0.2, 1.0, 3)
+        b.mesh(box_tris(lo, hi), **material())
+        if len(b.meshes) > 4000:
+            break
+    tris, meshes = b.arrays()
+    return SceneArrays(tris, meshes, (X / 2, 2.0, 0.3), (X / 2, 1.2, Z / 2), (0, 1, 0), 60.0, width, height,
+                       max_depth=max_depth)

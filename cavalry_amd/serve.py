@@ -1,0 +1,174 @@
+"""Render server over the C ABI (cav_serve_*) and a load harness.
+
+``Server`` submits render requests to resident scenes and polls them;
+``run_served`` / ``run_plain`` replay one seeded Poisson arrival trace through
+the server (continuous batching) and through the plain per-request renderer
+(cav_render, one request at a time in arrival order), and report latency
+percentiles and throughput for the comparison.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import RenderParams, ServeConfig, ServeResult, ServeStats, check, lib
+from .render import INTEGRATORS, Film, Scene, render
+
+QUEUED, RUNNING, DONE, CANCELLED, FAILED = range(5)  # CAV_SERVE_*
+
+
+@dataclass
+class Ticket:
+    id: int
+    film: Film
+    integrator: str
+
+
+class Server:
+    def __init__(self, ctx, arena_bytes=0, host_arena_bytes=0, pool_tasks=0, chunk_samples=0,
+                 max_steps_ahead=0, use_graphs=False):
+        self.ctx = ctx
+        cfg = ServeConfig(int(arena_bytes), int(host_arena_bytes), int(pool_tasks), int(chunk_samples),
+                          int(max_steps_ahead), 1 if use_graphs else 0)
+        h = ctypes.c_void_p()
+        check(ctx.handle, lib().cav_serve_open(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+
+    def submit(self, scene: Scene, spp: int, integrator: str = "path", camera=None, max_depth=None,
+               q_grid: int = 32, alpha_scale: float = 1.0) -> Ticket:
+        cam = camera if camera is not None else scene.setup.camera
+        depth = max_depth if max_depth is not None else (scene.arrays.max_depth if scene.arrays is not None
+                                                         else scene.setup.max_depth)
+        prm = RenderParams(INTEGRATORS[integrator], int(spp), int(depth), 0, int(q_grid), float(alpha_scale), 0)
+        film = Film(np.zeros((cam.height, cam.width, 3), np.float32), np.zeros((cam.height, cam.width), np.float32))
+        rid = ctypes.c_int64()
+        check(self.ctx.handle, lib().cav_serve_submit(
+            self.handle, scene.handle, ctypes.byref(cam), ctypes.byref(prm),
+            film.rgb.ctypes.data_as(ctypes.c_void_p), film.weight.ctypes.data_as(ctypes.c_void_p),
+            ctypes.byref(rid)))
+        return Ticket(int(rid.value), film, integrator)
+
+    def poll(self, t: Ticket, wait: bool = True) -> dict:
+        r = ServeResult()
+        check(self.ctx.handle, lib().cav_serve_poll(self.handle, t.id, 1 if wait else 0, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
+    def cancel(self, t: Ticket) -> None:
+        check(self.ctx.handle, lib().cav_serve_cancel(self.handle, t.id))
+
+    def close(self) -> dict:
+        st = ServeStats()
+        if self.handle:
+            check(None, lib().cav_serve_close(self.handle, ctypes.byref(st)))
+            self.handle = None
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ harness
+@dataclass
+class Req:
+    scene: int        # index into the resident scene list
+    camera: object    # cav_camera
+    spp: int
+    integrator: str
+    max_depth: int
+    arrival: float    # seconds from the start of the trace
+
+
+@dataclass
+class TraceResult:
+    latencies_ms: list = field(default_factory=list)
+    makespan_s: float = 0.0
+    rays: int = 0
+    films: list = field(default_factory=list)
+
+    def summary(self) -> dict:
+        lat = np.asarray(self.latencies_ms)
+        return {"requests": len(lat), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2), "mean_ms": round(float(lat.mean()), 2),
+                "makespan_s": round(self.makespan_s, 3), "rays": int(self.rays),
+                "mrays_s": round(self.rays / self.makespan_s / 1e6, 2)}
+
+
+def poisson_trace(rng: np.random.Generator, n: int, rate: float, scenes: list, resolutions: list, spps: list,
+                  integrators: list, max_depth: int, camera_fn) -> list:
+    """n requests with exponential inter-arrival times of mean 1/rate; each
+    picks a scene, resolution, spp and integrator uniformly, and a camera
+    from camera_fn(rng, scene_index, width, height)."""
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        s = int(rng.integers(len(scenes)))
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        out.append(Req(s, camera_fn(rng, s, w, h), int(spps[int(rng.integers(len(spps)))]),
+                       integrators[int(rng.integers(len(integrators)))], max_depth, t))
+    return out
+
+
+def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
+    """Submits each request at its arrival time (from a thread), waits for
+    all, and measures arrival -> film-ready latency on the server's clock.
+    `rays` counts the rlpath requests' rays only: pooled requests share the
+    pools' tallies, which cav_serve_close reports."""
+    tickets = [None] * len(trace)
+    start = time.perf_counter()
+
+    def submitter():
+        for k, r in enumerate(trace):
+            delay = start + r.arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[k] = server.submit(scenes[r.scene], r.spp, r.integrator, camera=r.camera,
+                                       max_depth=r.max_depth)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    res = TraceResult()
+    first_submit, last_done = None, 0.0
+    rl_rays = 0
+    for t in tickets:
+        out = server.poll(t, wait=True)
+        if out["status"] != DONE:
+            raise RuntimeError(f"request {t.id} ended with status {out['status']} (error {out['error']})")
+        res.latencies_ms.append(out["done_ms"] - out["submit_ms"])
+        first_submit = out["submit_ms"] if first_submit is None else min(first_submit, out["submit_ms"])
+        last_done = max(last_done, out["done_ms"])
+        rl_rays += out["closest_rays"] + out["shadow_rays"]
+        res.films.append(t.film)
+    res.makespan_s = (last_done - first_submit) / 1e3
+    res.rays = rl_rays
+    return res
+
+
+def run_plain(ctx, scenes: list, trace: list) -> TraceResult:
+    """The plain per-request renderer: one cav_render at a time, in arrival
+    order, each starting when it has arrived and the previous one is done."""
+    res = TraceResult()
+    start = time.perf_counter()
+    first = None
+    for r in trace:
+        now = time.perf_counter() - start
+        if now < r.arrival:
+            time.sleep(r.arrival - now)
+        arrived = start + r.arrival
+        film, st = render(ctx, scenes[r.scene], r.spp, integrator=r.integrator, camera=r.camera,
+                          max_depth=r.max_depth)
+        done = time.perf_counter()
+        first = arrived if first is None else first
+        res.latencies_ms.append((done - arrived) * 1e3)
+        res.rays += st["closest_rays"] + st["shadow_rays"]
+        res.films.append(film)
+    res.makespan_s = time.perf_counter() - first
+    return res

@@ -257,6 +257,57 @@ void cav_pbrt_destroy(cav_pbrt* p);
 int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* overrides,
                                cav_scene** out, cav_render_setup* setup);
 
+/* ---------------------------------------------------------------- serving */
+
+/* No counterpart in the reference (it renders one scene per process): a
+ * render server over resident scenes with continuous batching. path and
+ * directlighting requests of one scene share a persistent ray pool: camera
+ * rays of new requests join at every bounce step, tasks of many requests and
+ * depths share each launch, and each film is bit-identical to the same
+ * request rendered alone by cav_render. rlpath requests (batch-coupled
+ * learning) run one at a time on a second stream, overlapping the pools.
+ * Admission is bounded by the pool's task slots and by a device arena for
+ * per-request sample and film buffers; the server allocates nothing while
+ * serving. All functions are thread-safe. */
+typedef struct cav_server cav_server;
+typedef struct {
+    int64_t arena_bytes;       /* device arena for admitted requests (0: 16 GiB); 20 B/sample + 160 B/pixel */
+    int64_t host_arena_bytes;  /* pinned staging for finished films (0: 1 GiB); 16 B/pixel */
+    int64_t pool_tasks;        /* ray-pool capacity per scene (0: 4M tasks, ~170 B each) */
+    int64_t chunk_samples;     /* admission quantum per request per step (0: pool_tasks / 4) */
+    int32_t max_steps_ahead;   /* bounce steps the host may enqueue ahead of the device (0: 4) */
+    int32_t use_graphs;        /* replay each pool's bounce sequence from a captured hipGraph */
+} cav_serve_config;
+enum { CAV_SERVE_QUEUED = 0, CAV_SERVE_RUNNING = 1, CAV_SERVE_DONE = 2, CAV_SERVE_CANCELLED = 3,
+       CAV_SERVE_FAILED = 4 };
+typedef struct {
+    int32_t status;                   /* CAV_SERVE_* */
+    int32_t error;                    /* cav_status when FAILED */
+    double submit_ms, start_ms, done_ms;  /* server clock (ms since cav_serve_open); -1 if not reached */
+    int64_t closest_rays, shadow_rays;    /* rlpath requests only (pooled requests share the pool's tallies) */
+} cav_serve_result;
+typedef struct {
+    int64_t pool_closest_rays, pool_shadow_rays, rl_closest_rays, rl_shadow_rays;
+    int64_t steps;                    /* bounce steps over all pools */
+    int32_t pools;
+} cav_serve_stats;
+
+int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out);
+/* Queues a render of `scene` (created by cav_scene_create on the same
+ * device) with the camera and parameters of cav_render. The film buffers
+ * (float[h][w][3] sums and float[h][w] weights) must stay valid until poll
+ * returns a final status. */
+int cav_serve_submit(cav_server* s, cav_scene* scene, const cav_camera* camera, const cav_render_params* params,
+                     float* film_rgb, float* film_weight, int64_t* id);
+/* Status of a request; wait != 0 blocks until it is final. Returning a final
+ * status (DONE, CANCELLED, FAILED) releases the id. */
+int cav_serve_poll(cav_server* s, int64_t id, int32_t wait, cav_serve_result* out);
+/* Cancels a request: queued ones are dropped, running ones finish without
+ * writing the caller's film. */
+int cav_serve_cancel(cav_server* s, int64_t id);
+/* Drains outstanding work, then frees the server; stats may be NULL. */
+int cav_serve_close(cav_server* s, cav_serve_stats* stats);
+
 /* ---------------------------------------------------------------- film output */
 
 /* Replaces Film::readCurrentResult + apply + writeColorAt (Core/Film.cu:20-46,

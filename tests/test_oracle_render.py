@@ -90,3 +90,21 @@ def test_directlighting_furnace_is_le_times_one_plus_rho():
         rgb, w, _ = oracle.render_direct(sc, 32, cam)
         mean = float((rgb / w[..., None]).mean())
         assert abs(mean - (1.0 + rho)) < 0.02 * (1.0 + rho), (rho, mean)
+
+
+def test_interior_generator():
+    """The wavefront-at-scale scene family: triangle budget, 64 equal-area
+    downward ceiling lights, inward room faces, the material mix present."""
+    sc = scenes.interior(120_000, seed=3, width=32, height=18)
+    assert abs(len(sc.tris) - 120_000) < 200
+    lights = sc.meshes[sc.meshes["is_light"] == 1]
+    assert len(lights) == 64 and (lights["num_tris"] == 2).all()
+    for m in lights[:4]:
+        a, b = sc.tris[m["first_tri"]].reshape(3, 3), sc.tris[m["first_tri"] + 1].reshape(3, 3)
+        na, nb = np.cross(a[1] - a[0], a[2] - a[0]), np.cross(b[1] - b[0], b[2] - b[0])
+        assert na[1] < 0 and nb[1] < 0 and abs(np.linalg.norm(na) - np.linalg.norm(nb)) < 1e-6
+    centre = np.array([5.0, 2.0, 5.0])
+    for m in sc.meshes[:6]:
+        t = sc.tris[m["first_tri"]].reshape(3, 3).astype(np.float64)
+        assert np.dot(np.cross(t[1] - t[0], t[2] - t[0]), centre - t.mean(0)) > 0
+    assert set(np.unique(sc.meshes["material"])) == {scenes.MATTE, scenes.PLASTIC, scenes.MIRROR, scenes.GLASS}
