@@ -26,19 +26,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
-# Algorithmic bytes per triangle of the LBVH stages (30-bit keys, K = 4 B):
-# Morton reads the 36 B soup and writes key + index + a 24 B box; each of the
-# four 8-bit sort passes reads and writes key + value; the histogram reads the
-# keys once; Karras reads a key and writes 8 B of children and 8 B of parents;
-# refit reads the parent chain, the gate, both child boxes and writes a box.
-ALG_BYTES_30 = {
-    "morton": 36 + 4 + 4 + 24,
-    "sort_hist": 4,
-    "sort_downsweep": 2 * (4 + 4),  # per pass
-    "karras": 4 + 8 + 8,
-    "refit": 4 + 24 + 24 + 8 + 8 + 24,
+# Algorithmic bytes of one launch of each single-launch LBVH build kernel
+# (30-bit keys, K = 4 B), as functions of N:
+#   bounds        reads the 36 B soup;
+#   morton        reads the soup, writes key + index (+ the 24 B leaf box of
+#                 the canonical model);
+#   karras        reads a key, writes 8 B of children and 8 B of parents;
+#   refit_leaves  (k_leaves) reads the sorted index and the triangle, writes
+#                 the 32 B leaf box/area/cost record, then climbs: each of the
+#                 2(N-1) arrivals at an internal node reads the child's parent
+#                 id (4 B) and does one 4 B atomic read-modify-write (8 B).
+ALG_BYTES_LAUNCH_30 = {
+    "bounds": lambda n: 36 * n,
+    "morton": lambda n: (36 + 4 + 4 + 24) * n,
+    "karras": lambda n: (4 + 8 + 8) * n,
+    "refit_leaves": lambda n: (4 + 36 + 32) * n + 2 * (n - 1) * (4 + 8),
 }
-ALG_BYTES_PER_TRI_30 = 248  # S1 68 + S2 68 + S3 20 + S4 92
+ALG_BYTES_PER_TRI_30 = 248  # whole build, canonical model: S1 68 + S2 68 + S3 20 + S4 92
 
 
 def log(msg: str) -> None:
@@ -143,6 +147,66 @@ def render_bench(renders: int, cpu: bool) -> dict:
     return out
 
 
+def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
+    """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
+    triangles): one seeded Poisson trace of render requests (resolution from
+    256^2, 512^2, 1280x720, 1920x1080; spp from 1, 4, 16; path / rlpath
+    50/50; max depth 8) replayed through the plain per-request renderer (one
+    cav_render at a time, in arrival order) and through the server
+    (continuous batching). Latency = arrival -> film ready. Throughput =
+    traced rays / (first arrival -> last film)."""
+    import numpy as np
+    from cavalry_amd import Context, render as R, scenes
+    from cavalry_amd.camera import look_at
+    from cavalry_amd.serve import Server, poisson_trace, run_plain, run_served
+
+    sizes = [100_000, 500_000, 2_000_000, 8_000_000]
+    ctx = Context(0)
+    log("[bench] serving: generating 4 resident scenes")
+    arrays = [scenes.interior(n, seed=seed + k, width=64, height=36) for k, n in enumerate(sizes)]
+    res_scenes = [R.Scene(ctx, a, bvh_rounds=3) for a in arrays]
+
+    def camera(rng, s, w, h):
+        eye = np.float32([rng.uniform(1, 9), rng.uniform(1, 3), rng.uniform(0.5, 2.0)])
+        at = np.float32([rng.uniform(2, 8), rng.uniform(0.3, 1.5), rng.uniform(4, 9)])
+        return look_at(eye, at, (0, 1, 0), 60.0, w, h)
+
+    rng = np.random.default_rng(seed)
+    trace = poisson_trace(rng, n_requests, rate, res_scenes,
+                          [(256, 256), (512, 512), (1280, 720), (1920, 1080)], [1, 4, 16],
+                          ["path", "rlpath"], 8, camera)
+    log(f"[bench] serving: plain per-request renderer, {n_requests} requests at {rate}/s")
+    plain = run_plain(ctx, res_scenes, trace)
+    log("[bench] serving: server (continuous batching)")
+    srv = Server(ctx, arena_bytes=48 << 30, host_arena_bytes=4 << 30, pool_tasks=8 << 20)
+    served = run_served(srv, res_scenes, trace)
+    st = srv.close()
+    served.rays += st["pool_closest_rays"] + st["pool_shadow_rays"]
+    same = True
+    worst_rl = 0.0
+    for r, a, b in zip(trace, plain.films, served.films):
+        if r.integrator == "rlpath":
+            ia, ib = a.image.astype(np.float64), b.image.astype(np.float64)
+            worst_rl = max(worst_rl, float(np.mean((ib - ia) ** 2 / (ia * ia + 1e-2))))
+        else:
+            same = same and np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
+    for sc in res_scenes:
+        sc.close()
+    ctx.close()
+    p, s = plain.summary(), served.summary()
+    return {
+        "workload": f"{n_requests} requests, Poisson {rate}/s, 4 resident interior scenes "
+                    f"({', '.join(str(len(a.tris)) for a in arrays)} tris), res 256^2..1920x1080, spp 1/4/16, "
+                    "path/rlpath 50/50, max depth 8",
+        "plain": p,
+        "served": s,
+        "speedup_mrays": round(s["mrays_s"] / p["mrays_s"], 3),
+        "path_films_identical": bool(same),
+        "rlpath_max_rel_mse_vs_plain": worst_rl,
+        "server_steps": st["steps"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +221,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--renders", type=int, default=3)
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--probe", action="store_true",
+                    help="one LBVH build and one 256 MiB device copy, then exit (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -192,6 +261,14 @@ def main() -> int:
     stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
     log("[bench] first build (allocates)")
     bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
+    if args.probe:
+        # the copy calibrates HBM byte counters: it moves exactly 2^28 B each way
+        src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] probe done: one build, one 2^28-byte device copy")
+        return 0
 
     def step(record: bool):
         bvh.rebuild(tris, optimization_rounds=args.rounds, morton_bits=30)
@@ -240,10 +317,10 @@ def main() -> int:
     launches = {k: len(v) // args.steps for k, v in kernels.items()}
     totals = {k: per_launch[k] * launches[k] for k in per_launch}
 
-    # roofline of the dominant HBM-bound LBVH kernel
-    lbvh_kernels = [k for k in ALG_BYTES_30 if k in totals]
+    # roofline of the dominant single-launch LBVH build kernel (HBM-bound model)
+    lbvh_kernels = [k for k in ALG_BYTES_LAUNCH_30 if k in totals and launches.get(k) == 1]
     dom = max(lbvh_kernels, key=lambda k: totals[k])
-    alg = ALG_BYTES_30[dom] * n
+    alg = ALG_BYTES_LAUNCH_30[dom](n)
     achieved = alg / (per_launch[dom] * 1e-3) / 1e9
     roofline = {
         "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -302,6 +379,9 @@ def main() -> int:
     if not args.no_render:
         log("[bench] path tracing (Cornell box 512x512x16)")
         out["path_tracing"] = render_bench(args.renders, not args.no_cpu)
+    if not args.no_serve:
+        log("[bench] serving")
+        out["serving"] = serving_bench(args.serve_requests, args.serve_rate, args.seed)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)

@@ -52,6 +52,13 @@ class RenderStats(ctypes.Structure):
                 ("rounds", ctypes.c_int32)]
 
 
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int32), ("meshes", ctypes.c_void_p),
+                ("nmeshes", ctypes.c_int32), ("tri_uv", ctypes.c_void_p), ("tri_normals", ctypes.c_void_p),
+                ("mesh_normals", ctypes.c_void_p), ("bvh_opt_rounds", ctypes.c_int32),
+                ("morton_bits", ctypes.c_int32)]
+
+
 class ServeConfig(ctypes.Structure):
     _fields_ = [("arena_bytes", ctypes.c_int64), ("host_arena_bytes", ctypes.c_int64),
                 ("pool_tasks", ctypes.c_int64), ("chunk_samples", ctypes.c_int64),
@@ -98,6 +105,7 @@ _SIGS = {
     "cav_trace_occluded_async": ([P, P, P, P, I64, P], I),
     "cav_camera_look_at": ([P, P, P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
     "cav_scene_create": ([P, P, I, P, I, P, I, I, ctypes.POINTER(P)], I),
+    "cav_scene_create_desc": ([P, ctypes.POINTER(SceneDesc), ctypes.POINTER(P)], I),
     "cav_scene_destroy": ([P], I),
     "cav_scene_bvh": ([P], P),
     "cav_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
@@ -105,6 +113,7 @@ _SIGS = {
     "cav_camera_from_matrix": ([P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
     "cav_pbrt_parse": ([ctypes.c_char_p, P, ctypes.POINTER(P), ctypes.c_char_p, I], I),
     "cav_pbrt_get": ([P, P, P, P, P], I),
+    "cav_pbrt_get_normals": ([P, P, P], I),
     "cav_pbrt_destroy": ([P], None),
     "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
     "cav_serve_open": ([P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P)], I),

@@ -141,6 +141,38 @@ inline bool invert(const float* m, float* out) {
     return true;
 }
 
+// transpose(inverse(upper 3x3 of m)) (getTransformForNormal,
+// Utils/MathsCommons.h:80-82), as a row-major 3x3 r[row][col]; false when
+// singular
+inline bool normal_matrix(const M4& m, float r[3][3]) {
+    // a[row][col] of the upper 3x3
+    float a[3][3];
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) a[row][col] = m.at(col, row);
+    const float c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    const float c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+    const float c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    const float det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+    if (det == 0 || !std::isfinite(det)) return false;
+    const float inv = 1.0f / det;
+    // inverse(a)[i][j] = cofactor[j][i] / det, so its transpose is cofactor[i][j] / det
+    r[0][0] = c00 * inv;
+    r[0][1] = c01 * inv;
+    r[0][2] = c02 * inv;
+    r[1][0] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * inv;
+    r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * inv;
+    r[1][2] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * inv;
+    r[2][0] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * inv;
+    r[2][1] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * inv;
+    r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * inv;
+    return true;
+}
+
+inline V3 mul3(const float r[3][3], V3 v) {
+    return {(r[0][0] * v.x + r[0][1] * v.y) + r[0][2] * v.z, (r[1][0] * v.x + r[1][1] * v.y) + r[1][2] * v.z,
+            (r[2][0] * v.x + r[2][1] * v.y) + r[2][2] * v.z};
+}
+
 // glm::radians for float
 inline float radians(float deg) { return deg * 0.01745329251994329576923690768489f; }
 

@@ -278,8 +278,19 @@ bool read_transform(Buf& b, M4& t) {
 // ------------------------------------------------------------------ scene
 struct Mesh {
     std::vector<V3> pos;
+    std::vector<V3> nrm;   // per vertex, empty when the mesh has none
     std::vector<int> idx;  // 3 per triangle
 };
+
+// normalize(normalMatrix * n) for each given normal
+std::vector<V3> transform_normals(const std::vector<V3>& n, const M4& t) {
+    float r[3][3];
+    if (!normal_matrix(t, r)) throw ParseError("singular transform for normals");
+    std::vector<V3> out;
+    out.reserve(n.size());
+    for (const V3& v : n) out.push_back(normalize(mul3(r, v)));
+    return out;
+}
 
 struct Material {
     cav_mesh m{};  // material fields only
@@ -287,6 +298,9 @@ struct Material {
 
 struct Builder {
     std::vector<float> soup;
+    std::vector<float> normals;  // per triangle, 9 floats (zeros for meshes without normals)
+    std::vector<uint8_t> mesh_normals;
+    bool any_normals = false;
     std::vector<cav_mesh> meshes;
     std::vector<std::string> messages;
     int warnings = 0;
@@ -459,11 +473,14 @@ Mesh read_ply(const std::string& path, const M4& t) {
         return v;
     };
     for (const PlyElem& e : elems) {
-        int ix = -1, iy = -1, iz = -1, ivi = -1;
+        int ix = -1, iy = -1, iz = -1, ivi = -1, inx = -1, iny = -1, inz = -1;
         for (size_t k = 0; k < e.props.size(); ++k) {
             if (e.props[k].name == "x") ix = (int)k;
             if (e.props[k].name == "y") iy = (int)k;
             if (e.props[k].name == "z") iz = (int)k;
+            if (e.props[k].name == "nx") inx = (int)k;
+            if (e.props[k].name == "ny") iny = (int)k;
+            if (e.props[k].name == "nz") inz = (int)k;
             if (e.props[k].name == "vertex_indices" || e.props[k].name == "vertex_index") ivi = (int)k;
         }
         std::vector<double> vals(e.props.size());
@@ -485,9 +502,12 @@ Mesh read_ply(const std::string& path, const M4& t) {
             if (e.name == "vertex") {
                 if (ix < 0 || iy < 0 || iz < 0) throw ParseError("PLY: vertex without x, y, z in " + path);
                 m.pos.push_back(apply(t, {(float)vals[ix], (float)vals[iy], (float)vals[iz]}));
+                if (inx >= 0 && iny >= 0 && inz >= 0)
+                    m.nrm.push_back({(float)vals[inx], (float)vals[iny], (float)vals[inz]});
             }
         }
     }
+    if (!m.nrm.empty()) m.nrm = transform_normals(m.nrm, t);
     return m;
 }
 
@@ -512,6 +532,7 @@ Mesh sphere_mesh(float radius, const M4& t) {
     }
     Mesh m;
     for (const V3& v : p) m.pos.push_back(apply(t, {v.x * radius, v.y * radius, v.z * radius}));
+    m.nrm = transform_normals(p, t);  // the unit-sphere points are their own normals
     m.idx = cur;
     return m;
 }
@@ -551,6 +572,13 @@ Mesh make_shape(const ObjectDef& d, const M4& t, const std::string& base) {
             for (size_t k = 0; k + 2 < P.size(); k += 3) m.pos.push_back(apply(t, {P[k], P[k + 1], P[k + 2]}));
             for (size_t k = 0; k + 2 < I.size(); k += 3)
                 m.idx.insert(m.idx.end(), {(int)I[k], (int)I[k + 1], (int)I[k + 2]});
+            if (d.params.has_list("N")) {  // Core/TriangleMesh.cu:152-166
+                const auto& N = d.params.list("N");
+                if (N.size() != P.size()) throw ParseError("trianglemesh: N must give one normal per vertex");
+                std::vector<V3> n;
+                for (size_t k = 0; k + 2 < N.size(); k += 3) n.push_back({N[k], N[k + 1], N[k + 2]});
+                m.nrm = transform_normals(n, t);
+            }
             return m;
         }
         throw ParseError("No valid inputs for triangle mesh");
@@ -615,11 +643,15 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
     const int64_t first = (int64_t)bd.soup.size() / 9;
     const int64_t nv = (int64_t)m.pos.size();
     int ntris = 0;
+    const bool has_n = !m.nrm.empty();
+    if (has_n && (int64_t)m.nrm.size() != nv) throw ParseError("normals and positions differ in count");
     for (size_t k = 0; k + 2 < m.idx.size(); k += 3) {
         for (int j = 0; j < 3; ++j) {
             const int vi = m.idx[k + j];
             if (vi < 0 || vi >= nv) throw ParseError("triangle index out of range");
             bd.soup.insert(bd.soup.end(), {m.pos[vi].x, m.pos[vi].y, m.pos[vi].z});
+            if (has_n) bd.normals.insert(bd.normals.end(), {m.nrm[vi].x, m.nrm[vi].y, m.nrm[vi].z});
+            else bd.normals.insert(bd.normals.end(), {0.f, 0.f, 0.f});
         }
         ++ntris;
     }
@@ -640,6 +672,8 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
         return;
     }
     bd.meshes.push_back(c);
+    bd.mesh_normals.push_back(has_n ? 1 : 0);
+    bd.any_normals = bd.any_normals || has_n;
 }
 
 // Parsing.cu:272-388: one XBegin ... XEnd section
@@ -719,6 +753,8 @@ void parse_section(Buf& b, State st, const std::string& base, Builder& bd) {
 
 struct cav_pbrt {
     std::vector<float> soup;
+    std::vector<float> normals;        // empty when no mesh has vertex normals
+    std::vector<uint8_t> mesh_normals;
     std::vector<cav_mesh> meshes;
     cav_render_setup setup{};
     std::string messages;
@@ -791,6 +827,10 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** ou
         for (const cav_mesh& m : bd.meshes) s.nlights += m.is_light ? 1 : 0;
         r->soup.swap(bd.soup);
         r->meshes.swap(bd.meshes);
+        if (bd.any_normals) {
+            r->normals.swap(bd.normals);
+            r->mesh_normals.swap(bd.mesh_normals);
+        }
         for (const std::string& m : bd.messages) r->messages += m + "\n";
         *out = r.release();
         return CAV_OK;
@@ -810,6 +850,13 @@ int cav_pbrt_get(const cav_pbrt* p, const float** tris, const cav_mesh** meshes,
     return CAV_OK;
 }
 
+int cav_pbrt_get_normals(const cav_pbrt* p, const float** tri_normals, const uint8_t** mesh_normals) {
+    if (!p) return CAV_ERR_INVALID;
+    if (tri_normals) *tri_normals = p->normals.empty() ? nullptr : p->normals.data();
+    if (mesh_normals) *mesh_normals = p->normals.empty() ? nullptr : p->mesh_normals.data();
+    return CAV_OK;
+}
+
 void cav_pbrt_destroy(cav_pbrt* p) { delete p; }
 
 int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* ov, cav_scene** out,
@@ -821,8 +868,16 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
     int rc = cav_pbrt_parse(path, ov, &p, err, sizeof err);
     if (rc != CAV_OK) return cav::fail(ctx, rc, err);
     const cav_render_setup& s = p->setup;
-    rc = cav_scene_create(ctx, p->soup.data(), (int)s.ntris, p->meshes.data(), s.nmeshes, nullptr, s.bvh_opt_rounds,
-                          s.morton_bits, out);
+    cav_scene_desc d{};
+    d.tris = p->soup.data();
+    d.ntris = (int32_t)s.ntris;
+    d.meshes = p->meshes.data();
+    d.nmeshes = s.nmeshes;
+    d.tri_normals = p->normals.empty() ? nullptr : p->normals.data();
+    d.mesh_normals = p->normals.empty() ? nullptr : p->mesh_normals.data();
+    d.bvh_opt_rounds = s.bvh_opt_rounds;
+    d.morton_bits = s.morton_bits;
+    rc = cav_scene_create_desc(ctx, &d, out);
     if (rc == CAV_OK && setup) *setup = s;
     cav_pbrt_destroy(p);
     return rc;

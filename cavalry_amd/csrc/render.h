@@ -26,6 +26,8 @@ struct SceneDev {
     const float* mesh_area;
     const int32_t* lights;     // mesh ids of area lights
     int nmeshes, nlights;
+    const float* normals;        // float[ntris][9] per-vertex normals, or null
+    const uint8_t* mesh_normals; // per mesh: 1 = shade with the interpolated normals
 };
 
 struct CameraDev {
@@ -65,6 +67,8 @@ struct RenderArgs {
     uint8_t* vis;
     int32_t* hit_prim;
     float* hit_t;
+    float* hit_u;              // hit barycentrics, written when the scene has vertex normals
+    float* hit_v;
     float* res;                // [nsamp][3]
     float* sxy;                // [nsamp][2]
     uint32_t* counts;          // queue counts (16 words)

@@ -68,6 +68,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
                                                   Queue qi, Queue qo, Shadow sh,
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t,
+                                                  const float* __restrict__ hit_u,
+                                                  const float* __restrict__ hit_v,
                                                   float* __restrict__ res, uint32_t* counts,
                                                   int qin, int qout, const uint32_t* __restrict__ perm) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
         const float* r = qi.ray + i * 6;
         path_shade(depth, dim0, sc, h, sid, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]),
                    mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]), qi.pdf[i], qi.delta[i] != 0, prim,
-                   hit_t[i], res + (int64_t)sid * 3, o);
+                   hit_t[i], hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f, res + (int64_t)sid * 3, o);
     }
     const uint32_t ss = block_push<kBlock>(o.push_sh, &counts[2]);
     if (o.push_sh) {
@@ -101,7 +103,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
 // the per-task work is shade::direct_shade.
 __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Queue qi, Shadow sh,
                                                    const int32_t* __restrict__ hit_prim,
-                                                   const float* __restrict__ hit_t, float* __restrict__ res,
+                                                   const float* __restrict__ hit_t,
+                                                   const float* __restrict__ hit_u,
+                                                   const float* __restrict__ hit_v, float* __restrict__ res,
                                                    uint32_t* counts, const uint32_t* __restrict__ perm) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = j < (int64_t)counts[0];
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Que
         sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
         direct_shade(sc, h, sid, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), prim, hit_t[i],
-                     res + (int64_t)sid * 3, o);
+                     hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f, res + (int64_t)sid * 3, o);
     }
     const uint32_t ss = block_push<kBlock>(o.push_sh, &counts[2]);
     if (o.push_sh) {
@@ -228,14 +232,16 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
     render_raygen(a, s);
+    float* hu = a.scene.normals ? a.hit_u : nullptr;
+    float* hv = a.scene.normals ? a.hit_v : nullptr;
     if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[0].ray, a.counts, nq,
-                                 a.hit_prim, a.hit_t, nullptr, nullptr, a.trace_counters, s)) != hipSuccess)
+                                 a.hit_prim, a.hit_t, hu, hv, a.trace_counters, s)) != hipSuccess)
         return e;
     if (a.sort_perm && (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts, nq, a.sort_hist, a.sort_perm, s)) !=
                            hipSuccess)
         return e;
     hipLaunchKernelGGL(k_direct, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[0], a.sh, a.hit_prim, a.hit_t,
-                       a.res, a.counts, a.sort_perm);
+                       hu, hv, a.res, a.counts, a.sort_perm);
     if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nq, a.sh.limit,
                                   a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
         return e;
@@ -247,6 +253,8 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
 hipError_t render_path(const RenderArgs& a, hipStream_t s) {
     const int64_t nsamp = a.nq;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
+    float* hu = a.scene.normals ? a.hit_u : nullptr;
+    float* hv = a.scene.normals ? a.hit_v : nullptr;
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
     render_raygen(a, s);
@@ -254,7 +262,7 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
     for (int depth = 0; depth < a.max_depth; ++depth) {
         const int qout = qin ^ 1;
         if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[qin].ray,
-                                     a.counts + qin, nsamp, a.hit_prim, a.hit_t, nullptr, nullptr,
+                                     a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
                                      a.trace_counters, s)) != hipSuccess)
             return e;
         if (a.sort_perm &&
@@ -262,7 +270,8 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
                 hipSuccess)
             return e;
         hipLaunchKernelGGL(k_shade, dim3(nb), dim3(kBlock), 0, s, depth, dim0, a.scene, a.halton,
-                           a.q[qin], a.q[qout], a.sh, a.hit_prim, a.hit_t, a.res, a.counts, qin, qout, a.sort_perm);
+                           a.q[qin], a.q[qout], a.sh, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, qin, qout,
+                           a.sort_perm);
         if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2,
                                       nsamp, a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters,
                                       s)) != hipSuccess)

@@ -30,6 +30,8 @@ struct cav_scene {
     float* area = nullptr;
     int32_t* lights = nullptr;
     float* uv = nullptr;
+    float* normals = nullptr;        // float[ntris][9] or null
+    uint8_t* mesh_normals = nullptr; // per mesh
     std::vector<cav_mesh> host_meshes;
     std::vector<float> host_area;
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // Scene::sceneBounds (Core/Scene.cu:131-136)
@@ -39,7 +41,7 @@ namespace {
 
 // device bytes per sample of a round (queues, shadow tasks, hits) and per
 // sample of the whole render (radiance + film position)
-constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 8 + 4;
+constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 16 + 4;
 constexpr double kRlQueueBytes = 2 * 12 + (4 * 5 + 12 * 3 + 4 + 4 + 4);
 constexpr double kResultBytes = 12 + 8;
 
@@ -221,6 +223,8 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
     if (rc == CAV_OK) rc = grab(ctx, rs, a.vis, S, "visibility");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_prim, 4 * S, "hit prim");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_t, 4 * S, "hit t");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_u, 4 * S, "hit u");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.hit_v, 4 * S, "hit v");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.res, 12 * T, "sample radiance");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.sxy, 8 * T, "sample positions");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.counts, 64, "queue counts");
@@ -242,8 +246,9 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 
 namespace cav {
 SceneDev scene_dev(const cav_scene* sc) {
-    return SceneDev{sc->bvh->tn, sc->bvh->tt, sc->bvh->root, sc->ntris, sc->soup, sc->tri_mesh,
-                    sc->meshes, sc->area, sc->lights, sc->nmeshes, sc->nlights};
+    return SceneDev{sc->bvh->tn, sc->bvh->tt,   sc->bvh->root, sc->ntris,   sc->soup,
+                    sc->tri_mesh, sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
+                    sc->nlights,  sc->normals,  sc->mesh_normals};
 }
 
 int halton_dev(cav_ctx* ctx, HaltonDev* out) {
@@ -270,8 +275,25 @@ extern "C" {
 
 int cav_scene_create(cav_ctx* ctx, const float* tris, int n, const cav_mesh* meshes, int nmeshes,
                      const float* tri_uv, int bvh_opt_rounds, int morton_bits, cav_scene** out) {
-    if (!ctx || !out) return CAV_ERR_INVALID;
+    cav_scene_desc d{};
+    d.tris = tris;
+    d.ntris = n;
+    d.meshes = meshes;
+    d.nmeshes = nmeshes;
+    d.tri_uv = tri_uv;
+    d.bvh_opt_rounds = bvh_opt_rounds;
+    d.morton_bits = morton_bits;
+    return cav_scene_create_desc(ctx, &d, out);
+}
+
+int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** out) {
+    if (!ctx || !out || !desc) return CAV_ERR_INVALID;
     *out = nullptr;
+    const float* tris = desc->tris;
+    const int n = desc->ntris;
+    const cav_mesh* meshes = desc->meshes;
+    const int nmeshes = desc->nmeshes;
+    const float* tri_uv = desc->tri_uv;
     if (!tris || !meshes || n < 2 || nmeshes < 1) return fail(ctx, CAV_ERR_INVALID, "scene needs >= 2 triangles and >= 1 mesh");
     // meshes must tile the soup in order
     std::vector<int32_t> tri_mesh(n, -1);
@@ -330,6 +352,13 @@ int cav_scene_create(cav_ctx* ctx, const float* tris, int n, const cav_mesh* mes
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->area, 4 * (size_t)nmeshes, "mesh areas");
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->lights, 4 * (size_t)nlights, "lights");
     if (rc == CAV_OK && tri_uv) rc = dev_alloc(ctx, (void**)&sc->uv, 24 * (size_t)n, "uv");
+    std::vector<uint8_t> mflags;
+    if (desc->tri_normals) {
+        mflags.assign(nmeshes, 1);
+        if (desc->mesh_normals) mflags.assign(desc->mesh_normals, desc->mesh_normals + nmeshes);
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->normals, 36 * (size_t)n, "vertex normals");
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->mesh_normals, (size_t)nmeshes, "mesh normal flags");
+    }
     hipError_t e = hipSuccess;
     if (rc == CAV_OK) {
         e = hipMemcpy(sc->soup, tris, 36 * (size_t)n, hipMemcpyHostToDevice);
@@ -338,10 +367,15 @@ int cav_scene_create(cav_ctx* ctx, const float* tris, int n, const cav_mesh* mes
         if (e == hipSuccess) e = hipMemcpy(sc->area, sc->host_area.data(), 4 * (size_t)nmeshes, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(sc->lights, lights.data(), 4 * (size_t)nlights, hipMemcpyHostToDevice);
         if (e == hipSuccess && tri_uv) e = hipMemcpy(sc->uv, tri_uv, 24 * (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && desc->tri_normals)
+            e = hipMemcpy(sc->normals, desc->tri_normals, 36 * (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && desc->tri_normals)
+            e = hipMemcpy(sc->mesh_normals, mflags.data(), (size_t)nmeshes, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "upload scene");
     }
     if (rc == CAV_OK)
-        rc = cav_bvh_build_device(ctx, sc->soup, n, nullptr, sc->tri_mesh, morton_bits, bvh_opt_rounds, &sc->bvh);
+        rc = cav_bvh_build_device(ctx, sc->soup, n, nullptr, sc->tri_mesh, desc->morton_bits, desc->bvh_opt_rounds,
+                                  &sc->bvh);
     if (rc != CAV_OK) { cav_scene_destroy(sc); return rc; }
     *out = sc;
     return CAV_OK;
@@ -352,7 +386,7 @@ int cav_scene_destroy(cav_scene* sc) {
     if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
     if (sc->bvh) cav_bvh_destroy(sc->bvh);
     for (void* p : {(void*)sc->soup, (void*)sc->tri_mesh, (void*)sc->meshes, (void*)sc->area,
-                    (void*)sc->lights, (void*)sc->uv})
+                    (void*)sc->lights, (void*)sc->uv, (void*)sc->normals, (void*)sc->mesh_normals})
         if (p) (void)hipFree(p);
     delete sc;
     return CAV_OK;

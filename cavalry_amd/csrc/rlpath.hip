@@ -174,7 +174,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_dim_max(const uint32_t* counts, i
 __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, HaltonDev h, Grid gr,
                                                      Queue qi, RlSlots sl, Shadow sh, RlTables t,
                                                      const int32_t* __restrict__ hit_prim,
-                                                     const float* __restrict__ hit_t, float* __restrict__ res,
+                                                     const float* __restrict__ hit_t,
+                                                     const float* __restrict__ hit_u,
+                                                     const float* __restrict__ hit_v, float* __restrict__ res,
                                                      uint32_t* counts, const int32_t* dims, int qin,
                                                      const uint32_t* __restrict__ perm) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
         const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
         mesh = sc.tri_mesh[prim];
         const cav_mesh M = sc.meshes[mesh];
-        const v3 nn = tri_normal(sc, prim);
+        const v3 nn = shading_normal(sc, prim, hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f);
         pos = add3(ro, scale3(rd, hit_t[i]));
         float emis = 0.f;
         if (M.is_light) {
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
         const float v = tmp * halton(h, sid, D + 7);
         const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
         const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-        const v3 ln = tri_normal(sc, tri);
+        const v3 ln = shading_normal(sc, tri, u, v);
         const v3 l2s = sub3(pos, lp);
         const float c = fabsf(dot3(ln, nrm(l2s)));
         const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
@@ -301,7 +303,9 @@ struct BinEval {
 __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
                                                   RlSlots sl, RlTables t, const uint8_t* __restrict__ vis,
                                                   const int32_t* __restrict__ hit_prim,
-                                                  const float* __restrict__ hit_t, float* __restrict__ res,
+                                                  const float* __restrict__ hit_t,
+                                                  const float* __restrict__ hit_u,
+                                                  const float* __restrict__ hit_v, float* __restrict__ res,
                                                   const uint32_t* counts, int32_t* dims, int qin,
                                                   const uint32_t* __restrict__ perm) {
     __shared__ float trig[2 * kNX][kBlock];
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
         const int mesh = sc.tri_mesh[prim];
         const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
-        const v3 nn = tri_normal(sc, prim);
+        const v3 nn = shading_normal(sc, prim, hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f);
         v3 t0, t1;
         frame(nn, t0, t1);
         const v3 ol = to_local(scale3(rd, -1.f), t0, t1, nn);
@@ -574,7 +578,9 @@ __global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nen
 // genNextRay (:305-375) after the second syncDimension.
 __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonDev h, Queue qi, Queue qo,
                                                  RlQueue rqo, RlSlots sl, const int32_t* __restrict__ hit_prim,
-                                                 const float* __restrict__ hit_t, uint32_t* counts,
+                                                 const float* __restrict__ hit_t,
+                                                 const float* __restrict__ hit_u,
+                                                 const float* __restrict__ hit_v, uint32_t* counts,
                                                  const int32_t* dims, int qin, int qout,
                                                  const uint32_t* __restrict__ perm) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -608,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
         if (alive) {
             const int mesh = sc.tri_mesh[prim];
             const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
-            const v3 nn = tri_normal(sc, prim);
+            const v3 nn = shading_normal(sc, prim, hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f);
             v3 t0, t1;
             frame(nn, t0, t1);
             const v3 pos = add3(ro, scale3(rd, hit_t[i]));
@@ -669,6 +675,8 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
 
 hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s) {
     const int64_t nsamp = a.nq;
+    float* hu = a.scene.normals ? a.hit_u : nullptr;
+    float* hv = a.scene.normals ? a.hit_v : nullptr;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
     const int L = a.scene.nlights;
     const int64_t nent = (int64_t)rl.grid * rl.grid * rl.grid;
@@ -693,7 +701,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
     for (int depth = 0; depth < a.max_depth; ++depth) {
         const int qout = qin ^ 1;
         if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[qin].ray,
-                                     a.counts + qin, nsamp, a.hit_prim, a.hit_t, nullptr, nullptr,
+                                     a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
                                      a.trace_counters, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_dim_max, dim3((unsigned)((nsamp + kMaxBlock - 1) / kMaxBlock)), dim3(kMaxBlock), 0, s,
@@ -703,18 +711,20 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
                 hipSuccess)
             return e;
         hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
-                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin, a.sort_perm);
+                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin,
+                           a.sort_perm);
         if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
-                           t, a.vis, a.hit_prim, a.hit_t, a.res, a.counts, rl.dims, qin, a.sort_perm);
+                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, a.sort_perm);
         hipLaunchKernelGGL(k_finish_q, dim3((unsigned)((nent + kEntryWaves - 1) / kEntryWaves)),
                            dim3(64 * kEntryWaves), 0, s, t, nent, rl.alpha_scale);
         hipLaunchKernelGGL(k_finish_light, dim3((unsigned)((nent + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
                            nent, L);
         hipLaunchKernelGGL(k_next, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, a.q[qin], a.q[qout],
-                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, a.counts, rl.dims, qin, qout, a.sort_perm);
+                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, hu, hv, a.counts, rl.dims, qin, qout,
+                           a.sort_perm);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
         qin = qout;
     }

@@ -201,6 +201,7 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     }
     A(d.sh.ray, 24 * C); A(d.sh.limit, 4 * C); A(d.sh.src, 4 * C); A(d.sh.dst, 4 * C); A(d.sh.sid, 4 * C);
     A(d.sh.contrib, 12 * C); A(d.sh_rid, 4 * C); A(d.vis, C); A(d.hit_prim, 4 * C); A(d.hit_t, 4 * C);
+    A(d.hit_u, 4 * C); A(d.hit_v, 4 * C);
     A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 64);
     A(p->d_reqs, sizeof(ReqDev) * kSlots);
     const int ring = sv->cfg.max_steps_ahead + 2;

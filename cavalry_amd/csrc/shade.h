@@ -311,6 +311,21 @@ __device__ inline v3 tri_normal(const SceneDev& sc, int t) {
     if (sc.meshes[sc.tri_mesh[t]].reverse_orientation) n = scale3(n, -1.f);
     return n;
 }
+
+// Shading normal at barycentrics (u, v) of triangle t
+// (Triangle::fillIntersectionInformation, Core/Triangle.h:94-136, and
+// TriangleMesh::getNormal, Core/TriangleMesh.h:76-104): with vertex normals
+// normalize(n0 (1 - u - v) + u n1 + v n2), else the geometric normal; flipped
+// for ReverseOrientation.
+__device__ inline v3 shading_normal(const SceneDev& sc, int t, float u, float v) {
+    const int mesh = sc.tri_mesh[t];
+    if (!(sc.normals && sc.mesh_normals[mesh])) return tri_normal(sc, t);
+    const float* q = sc.normals + (int64_t)t * 9;
+    const v3 n0 = mk3(q[0], q[1], q[2]), n1 = mk3(q[3], q[4], q[5]), n2 = mk3(q[6], q[7], q[8]);
+    v3 n = nrm(add3(add3(scale3(n0, 1.f - u - v), scale3(n1, u)), scale3(n2, v)));
+    if (sc.meshes[mesh].reverse_orientation) n = scale3(n, -1.f);
+    return n;
+}
 __device__ inline float power_heuristic(float a, float b) {
     a *= a;
     b *= b;
@@ -341,11 +356,11 @@ struct ShadeOut {
 // the BSDF sample of the next ray. dim0 = the task's first Halton dimension
 // of this bounce.
 __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const HaltonDev& h, uint32_t sid, v3 ro,
-                                  v3 rd, v3 mult, float mat_pdf, bool from_delta, int prim, float t, float* acc3,
-                                  ShadeOut& o) {
+                                  v3 rd, v3 mult, float mat_pdf, bool from_delta, int prim, float t, float hu,
+                                  float hv, float* acc3, ShadeOut& o) {
     const int mesh = sc.tri_mesh[prim];
     const cav_mesh M = sc.meshes[mesh];
-    const v3 nn = tri_normal(sc, prim);
+    const v3 nn = shading_normal(sc, prim, hu, hv);
     const v3 pos = add3(ro, scale3(rd, t));
     const Bsdf bsdf = make_bsdf(M);
     v3 t0, t1;
@@ -375,7 +390,7 @@ __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const
     const float v = tmp * halton(h, sid, dim0 + 7);
     const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
     const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-    const v3 ln = tri_normal(sc, tri);
+    const v3 ln = shading_normal(sc, tri, u, v);
     const v3 l2s = sub3(pos, lp);
     const float c = fabsf(dot3(ln, nrm(l2s)));
     const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
@@ -425,10 +440,10 @@ __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const
 // without MIS (added to acc3) and one light sample weighted by
 // lightsCount / pdf without MIS; the shadow task carries it.
 __device__ inline void direct_shade(const SceneDev& sc, const HaltonDev& h, uint32_t sid, v3 ro, v3 rd, int prim,
-                                    float t, float* acc3, ShadeOut& o) {
+                                    float t, float hu, float hv, float* acc3, ShadeOut& o) {
     const int mesh = sc.tri_mesh[prim];
     const cav_mesh M = sc.meshes[mesh];
-    const v3 nn = tri_normal(sc, prim);
+    const v3 nn = shading_normal(sc, prim, hu, hv);
     const v3 pos = add3(ro, scale3(rd, t));
     if (M.is_light) {
         const v3 Le = dot3(rd, nn) >= 0 ? mk3(0, 0, 0) : mk3(M.emission[0], M.emission[1], M.emission[2]);
@@ -446,7 +461,7 @@ __device__ inline void direct_shade(const SceneDev& sc, const HaltonDev& h, uint
     const float v = tmp * halton(h, sid, dim0 + 7);
     const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
     const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-    const v3 ln = tri_normal(sc, tri);
+    const v3 ln = shading_normal(sc, tri, u, v);
     const v3 l2s = sub3(pos, lp);
     const float c = fabsf(dot3(ln, nrm(l2s)));
     const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);

@@ -39,9 +39,8 @@ def overrides(output=None, integrator=None, spp=0, bvh_opt_rounds=-1, morton_bit
                      int(spp), int(bvh_opt_rounds), int(morton_bits))
 
 
-def parse(path: str, **kw):
-    """Parses a scene on the host. Returns (tris [n, 9] float32, meshes
-    (MESH_DTYPE), setup (RenderSetup), messages list)."""
+def _parse(path: str, **kw):
+    """(tris, meshes, setup, messages, normals or None, mesh flags or None)"""
     ov = overrides(**kw)
     h = ctypes.c_void_p()
     err = ctypes.create_string_buffer(1024)
@@ -57,16 +56,28 @@ def parse(path: str, **kw):
         raw = (ctypes.c_char * (MESH_DTYPE.itemsize * m)).from_address(mp.value)
         meshes = np.frombuffer(bytes(raw), MESH_DTYPE).copy()
         messages = [s for s in (msg.value or b"").decode().split("\n") if s]
-        return tris, meshes, st, messages
+        nptr, fptr = ctypes.c_void_p(), ctypes.c_void_p()
+        check(None, lib().cav_pbrt_get_normals(h, ctypes.byref(nptr), ctypes.byref(fptr)))
+        normals = flags = None
+        if nptr.value:
+            normals = np.ctypeslib.as_array((ctypes.c_float * (9 * n)).from_address(nptr.value)).reshape(n, 9).copy()
+            flags = np.ctypeslib.as_array((ctypes.c_uint8 * m).from_address(fptr.value)).copy()
+        return tris, meshes, st, messages, normals, flags
     finally:
         lib().cav_pbrt_destroy(h)
 
 
-def to_arrays(path: str, **kw):
-    """Parses a scene into SceneArrays-compatible data plus the setup: returns
-    (tris, meshes, setup)."""
-    tris, meshes, st, _ = parse(path, **kw)
-    return tris, meshes, st
+def parse(path: str, **kw):
+    """Parses a scene on the host. Returns (tris [n, 9] float32, meshes
+    (MESH_DTYPE), setup (RenderSetup), messages list)."""
+    return _parse(path, **kw)[:4]
+
+
+def parse_normals(path: str, **kw):
+    """Per-vertex normals of a parsed scene, gathered per triangle corner:
+    (normals [n, 9] float32 or None, one flag per mesh or None). None when
+    no shape gave normals ("normal N", PLY nx/ny/nz, spheres)."""
+    return _parse(path, **kw)[4:]
 
 
 def _f(x) -> str:
@@ -81,7 +92,7 @@ def _list(v) -> str:
 def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", filename: str = "output.png",
           sampler: str = "halton") -> None:
     """Writes a generated scene as a .pbrt file the parser reads back into the
-    same triangles, meshes and camera."""
+    same triangles, meshes, camera and vertex normals."""
     out = [
         "# generated scene",
         f"LookAt {' '.join(_f(x) for x in arr.eye)} {' '.join(_f(x) for x in arr.center)} "
@@ -94,7 +105,7 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         'PixelFilter "box"',
         "WorldBegin",
     ]
-    for m in arr.meshes:
+    for mi, m in enumerate(arr.meshes):
         out.append("AttributeBegin")
         mat = int(m["material"])
         if mat == MATTE:
@@ -115,7 +126,10 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
             out.append("  ReverseOrientation")
         t = arr.tris[m["first_tri"]:m["first_tri"] + m["num_tris"]]
         idx = " ".join(str(i) for i in range(3 * len(t)))
-        out.append(f'  Shape "trianglemesh" "integer indices" [ {idx} ] "point P" {_list(t)}')
+        normals = ""
+        if getattr(arr, "normals", None) is not None and (arr.mesh_normals is None or arr.mesh_normals[mi]):
+            normals = f' "normal N" {_list(arr.normals[m["first_tri"]:m["first_tri"] + m["num_tris"]])}'
+        out.append(f'  Shape "trianglemesh" "integer indices" [ {idx} ] "point P" {_list(t)}{normals}')
         out.append("AttributeEnd")
     out.append("WorldEnd")
     with open(path, "w") as f:

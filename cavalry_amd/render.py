@@ -86,10 +86,13 @@ class Scene:
         h = ctypes.c_void_p()
         tris = np.ascontiguousarray(arrays.tris, np.float32)
         meshes = np.ascontiguousarray(arrays.meshes)
-        check(ctx.handle, lib().cav_scene_create(
-            ctx.handle, tris.ctypes.data_as(ctypes.c_void_p), len(tris),
-            meshes.ctypes.data_as(ctypes.c_void_p), len(meshes), None, int(bvh_rounds),
-            int(morton_bits), ctypes.byref(h)))
+        normals = getattr(arrays, "normals", None)
+        flags = getattr(arrays, "mesh_normals", None)
+        self._keep = (tris, meshes, normals, flags)
+        d = _lib.SceneDesc(tris.ctypes.data, len(tris), meshes.ctypes.data, len(meshes), None,
+                           None if normals is None else normals.ctypes.data,
+                           None if flags is None else flags.ctypes.data, int(bvh_rounds), int(morton_bits))
+        check(ctx.handle, lib().cav_scene_create_desc(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
 
     @classmethod

@@ -8,6 +8,8 @@ Triangle soups are ``float32[N, 9]`` rows ``v0.xyz, v1.xyz, v2.xyz``; rays are
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 
@@ -76,8 +78,12 @@ class SceneArrays:
     """A renderable scene: triangle soup, meshes (contiguous triangle ranges)
     and a LookAt camera."""
 
-    def __init__(self, tris, meshes, eye, center, up, fov, width, height, max_depth=8):
+    def __init__(self, tris, meshes, eye, center, up, fov, width, height, max_depth=8, normals=None,
+                 mesh_normals=None):
         self.tris = np.ascontiguousarray(tris, np.float32)
+        # optional per-vertex shading normals float32[N, 9] and per-mesh use flags
+        self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        self.mesh_normals = None if mesh_normals is None else np.ascontiguousarray(mesh_normals, np.uint8)
         self.meshes = meshes
         self.eye = np.asarray(eye, np.float32)
         self.center = np.asarray(center, np.float32)
@@ -173,11 +179,12 @@ def uv_sphere(center, r, nphi=64, ntheta=32):
 
 
 def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: int = 8,
-                materials: str = "matte") -> SceneArrays:
+                materials: str = "matte", smooth: bool = False) -> SceneArrays:
     """Cornell-style box: unit box open at z=0 (5 walls x 2 quads), a tall and a
     short box, a 64x32 UV sphere (3968 triangles) and a 0.25x0.25 ceiling light
     of radiance 15 facing down. Albedos U[0.1,0.9] per mesh from the seed.
-    materials="mixed" makes the boxes plastic/mirror and the sphere glass."""
+    materials="mixed" makes the boxes plastic/mirror and the sphere glass.
+    smooth=True gives the sphere radial per-vertex normals (smooth shading)."""
     rng = _rng(seed, 7)
     b = _Builder()
 
@@ -204,13 +211,25 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
                  (0.375, 0.999, 0.625))
     b.mesh(light, kd=albedo(), emission=(15.0, 15.0, 15.0))
     tris, meshes = b.arrays()
+    normals = flags = None
+    if smooth:
+        normals = np.zeros_like(tris)
+        flags = np.zeros(len(meshes), np.uint8)
+        m = meshes[7]  # the sphere
+        t = tris[m["first_tri"]:m["first_tri"] + m["num_tris"]].reshape(-1, 3, 3).astype(np.float64)
+        n = t - np.array([0.5, 0.72, 0.3])
+        n /= np.linalg.norm(n, axis=2, keepdims=True)
+        normals[m["first_tri"]:m["first_tri"] + m["num_tris"]] = n.reshape(-1, 9)
+        flags[7] = 1
     return SceneArrays(tris, meshes, (0.5, 0.5, -1.3), (0.5, 0.5, 0.5), (0, 1, 0), 40.0,
-                       width, height, max_depth)
+                       width, height, max_depth, normals=normals, mesh_normals=flags)
 
 
+@functools.lru_cache(maxsize=4)
 def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
     """Unit icosphere subdivided `level` times: (vertices [V, 3] float64,
-    faces [20 * 4^level, 3] int64), faces wound outward."""
+    faces [20 * 4^level, 3] int64), faces wound outward. Cached: do not
+    modify the returned arrays."""
     t = (1.0 + 5 ** 0.5) / 2
     v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
                   [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
@@ -221,7 +240,9 @@ def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
     for _ in range(level):
         e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
         e.sort(axis=1)
-        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        key = e[:, 0] * (len(v) + 1) + e[:, 1]
+        uk, first, inv = np.unique(key, return_index=True, return_inverse=True)
+        uniq = e[first]
         mid = v[uniq[:, 0]] + v[uniq[:, 1]]
         mid /= np.linalg.norm(mid, axis=1, keepdims=True)
         base = len(v)

@@ -174,6 +174,22 @@ typedef struct cav_scene cav_scene;
 int cav_scene_create(cav_ctx* ctx, const float* triangles, int n, const cav_mesh* meshes,
                      int nmeshes, const float* tri_uv, int bvh_opt_rounds, int morton_bits,
                      cav_scene** out);
+/* The same with every scene input in one descriptor, including per-vertex
+ * shading normals: Triangle::fillIntersectionInformation and
+ * TriangleMesh::getNormal (Core/Triangle.h:94-136, Core/TriangleMesh.h:76-104)
+ * shade a mesh that has them with normalize(n0 (1-u-v) + u n1 + v n2). */
+typedef struct {
+    const float* tris;            /* float[ntris][9] */
+    int32_t ntris;
+    const cav_mesh* meshes;
+    int32_t nmeshes;
+    const float* tri_uv;          /* float[ntris][6] or NULL */
+    const float* tri_normals;     /* float[ntris][9] per-vertex normals, or NULL */
+    const uint8_t* mesh_normals;  /* per mesh 1 = shade with tri_normals (NULL: every mesh) */
+    int32_t bvh_opt_rounds;
+    int32_t morton_bits;
+} cav_scene_desc;
+int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** out);
 int cav_scene_destroy(cav_scene* scene);
 /* the scene's BVH (owned by the scene) */
 const cav_bvh* cav_scene_bvh(const cav_scene* scene);
@@ -251,6 +267,9 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* overrides, cav_pb
  * the setup and the warnings, one per line. */
 int cav_pbrt_get(const cav_pbrt* p, const float** tris, const cav_mesh** meshes, cav_render_setup* setup,
                  const char** messages);
+/* Per-vertex normals of a parsed scene (float[ntris][9] and one flag per
+ * mesh), or NULLs when no mesh has any ("normal N", PLY nx/ny/nz, spheres). */
+int cav_pbrt_get_normals(const cav_pbrt* p, const float** tri_normals, const uint8_t** mesh_normals);
 void cav_pbrt_destroy(cav_pbrt* p);
 /* Replaces readRenderSetup + Scene::prepareForRender (main.cu:10-15):
  * parse, then cav_scene_create with the setup's BVH rounds and Morton width. */

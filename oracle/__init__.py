@@ -152,6 +152,18 @@ def mesh_areas(tris, meshes) -> np.ndarray:
     return out
 
 
+def _normals(scene):
+    """(tri_normals, mesh_normals) pointers of a scene, or two NULLs."""
+    nrm = getattr(scene, "normals", None)
+    if nrm is None:
+        return None, None
+    flags = getattr(scene, "mesh_normals", None)
+    if flags is None:
+        flags = np.ones(len(scene.meshes), np.uint8)
+    scene._oracle_keep = (np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(flags, np.uint8))
+    return _p(scene._oracle_keep[0]), _p(scene._oracle_keep[1])
+
+
 def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30):
     """`path` integrator on the CPU. Returns (film_rgb [H,W,3], film_w [H,W],
     rays_traced). camera is a cavalry_amd Camera (cav_camera)."""
@@ -167,9 +179,9 @@ def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int =
     tri_mesh = scene.tri_mesh
     rc = lib().ref_render_path(
         _p(tris), ctypes.c_int(len(tris)), _p(tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(cam16), ctypes.c_float(camera.depth),
-        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), ctypes.c_int(scene.max_depth),
-        _p(rgb), _p(wt), _p(rays))
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(cam16),
+        ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp),
+        ctypes.c_int(scene.max_depth), _p(rgb), _p(wt), _p(rays))
     if rc != 0:
         raise ValueError(f"ref_render_path failed ({rc})")
     return rgb, wt, int(rays[0])
@@ -188,8 +200,9 @@ def render_direct(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
     cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
     rc = lib().ref_render_direct(
         _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(cam16), ctypes.c_float(camera.depth),
-        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), _p(rgb), _p(wt), _p(rays))
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(cam16),
+        ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), _p(rgb), _p(wt),
+        _p(rays))
     if rc != 0:
         raise ValueError(f"ref_render_direct failed ({rc})")
     return rgb, wt, int(rays[0])
@@ -214,7 +227,7 @@ def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
     depth = scene.max_depth if max_depth is None else max_depth
     rc = lib().ref_render_rlpath(
         _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _p(s6), ctypes.c_int(q_grid),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(s6), ctypes.c_int(q_grid),
         ctypes.c_float(alpha_scale), _p(cam16), ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h),
         ctypes.c_int(spp), ctypes.c_int(depth), ctypes.c_int(spp_per_round), _p(rgb), _p(wt), _p(rays))
     if rc != 0:

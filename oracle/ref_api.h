@@ -39,15 +39,15 @@ float ref_halton(uint64_t index, int dim);
 void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, int nmeshes, float* area);
 int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
                     int nmeshes, const float* box, const int32_t* left, const int32_t* right,
-                    const int32_t* prim, const float* cam16, float cam_depth, int width, int height,
+                    const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals, const float* cam16, float cam_depth, int width, int height,
                     int spp, int max_depth, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
                       int nmeshes, const float* box, const int32_t* left, const int32_t* right,
-                      const int32_t* prim, const float* cam16, float cam_depth, int width, int height,
+                      const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals, const float* cam16, float cam_depth, int width, int height,
                       int spp, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                       const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                      const int32_t* left, const int32_t* right, const int32_t* prim,
+                      const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
                       const float* scene6, int q_grid, float alpha_scale, const float* cam16,
                       float cam_depth, int width, int height, int spp, int max_depth,
                       int spp_per_round, float* film_rgb, float* film_w, int64_t* rays_traced);

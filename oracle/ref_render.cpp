@@ -375,6 +375,8 @@ struct Scene {
     std::vector<int> lights;  // mesh ids of area lights, in mesh order
     std::vector<float> area;  // per mesh
     const float* box; const int32_t* left; const int32_t* right; const int32_t* prim;
+    const float* normals = nullptr;         // float[ntris][9] per-vertex normals, or null
+    const uint8_t* mesh_normals = nullptr;  // per mesh: 1 = use them
 };
 
 f3 tri_v(const Scene& s, int t, int j) {
@@ -423,11 +425,12 @@ bool tri_hit1(const Scene& s, int p, f3 o, f3 d, float& t, float& u, float& v) {
     t = dot(e2, qv) * det;
     return t > 0.f;
 }
-int closest(const Scene& s, f3 o, f3 d, float& tb) {
+int closest(const Scene& s, f3 o, f3 d, float& tb, float* bu = nullptr, float* bv = nullptr) {
     f3 inv = divv(mk(1, 1, 1), d);
     float md;
     tb = FLT_MAX;
     int bp = -1;
+    float best_u = 0, best_v = 0;
     if (!box_hit1(s.box, o, inv, md)) return -1;
     int st[64], top = 0;
     st[0] = 0;
@@ -435,7 +438,12 @@ int closest(const Scene& s, f3 o, f3 d, float& tb) {
         int c = st[top--];
         if (s.prim[c] >= 0) {
             float t, u, v;
-            if (tri_hit1(s, s.prim[c], o, d, t, u, v) && (bp < 0 || t < tb)) { tb = t; bp = s.prim[c]; }
+            if (tri_hit1(s, s.prim[c], o, d, t, u, v) && (bp < 0 || t < tb)) {
+                tb = t;
+                bp = s.prim[c];
+                best_u = u;
+                best_v = v;
+            }
         } else {
             float dl, dr;
             int l = s.left[c], r = s.right[c];
@@ -449,6 +457,8 @@ int closest(const Scene& s, f3 o, f3 d, float& tb) {
             else if (hr) { if (top < 63) st[++top] = r; }
         }
     }
+    if (bu) *bu = best_u;
+    if (bv) *bv = best_v;
     return bp;
 }
 bool visible(const Scene& s, f3 o, f3 d, float lim, int sm, int dm) {
@@ -482,6 +492,20 @@ f3 tri_normal(const Scene& s, int p) {  // Core/Triangle.h:211-222 without verte
     f3 v0 = tri_v(s, p, 0);
     f3 n = normalize(cross(tri_v(s, p, 1) - v0, tri_v(s, p, 2) - v0));
     if (s.meshes[s.tri_mesh[p]].reverse_orientation) n = n * -1.f;
+    return n;
+}
+
+// Triangle::fillIntersectionInformation / TriangleMesh::getNormal
+// (Core/Triangle.h:94-136, Core/TriangleMesh.h:76-104): the interpolated
+// vertex normal normalize(n0 (1 - u - v) + u n1 + v n2) for meshes that have
+// vertex normals, else the geometric normal; flipped for ReverseOrientation.
+f3 shading_normal(const Scene& s, int p, float u, float v) {
+    const int m = s.tri_mesh[p];
+    if (!(s.normals && s.mesh_normals[m])) return tri_normal(s, p);
+    const float* q = s.normals + (int64_t)p * 9;
+    f3 n0 = mk(q[0], q[1], q[2]), n1 = mk(q[3], q[4], q[5]), n2 = mk(q[6], q[7], q[8]);
+    f3 n = normalize((n0 * (1.f - u - v) + u * n1) + v * n2);
+    if (s.meshes[m].reverse_orientation) n = n * -1.f;
     return n;
 }
 
@@ -560,11 +584,11 @@ void gather_film(int width, int height, int spp, const std::vector<float>& res, 
 // pixel order, then sample order.
 extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh,
                                const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                               const int32_t* left, const int32_t* right, const int32_t* prim,
+                               const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
                                const float* cam16, float cam_depth, int width, int height,
                                int spp, int max_depth, float* film_rgb, float* film_w,
                                int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -594,13 +618,13 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
         float mat_pdf = 1;
         bool from_delta = true;
         for (int depth = 0; depth < max_depth; ++depth) {
-            float t;
+            float t, hu, hv;
             ++traced;
-            int p = closest(s, ro, rd, t);
+            int p = closest(s, ro, rd, t, &hu, &hv);
             if (p < 0) break;
             const int mesh = tri_mesh[p];
             const cav_mesh_ref& M = meshes[mesh];
-            f3 n = tri_normal(s, p);
+            f3 n = shading_normal(s, p, hu, hv);
             f3 pos = ro + t * rd;
             Bsdf bsdf = make_bsdf(M);
             f3 t0, t1;
@@ -631,7 +655,7 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
                 float v = tmp * H.rand1(hidx, dim);
                 f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
                 f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-                f3 ln = tri_normal(s, tri);
+                f3 ln = shading_normal(s, tri, u, v);
                 f3 l2s = pos - lp;
                 float c = std::fabs(dot(ln, normalize(l2s)));
                 float prob = dot(l2s, l2s) / (c * s.area[lm]);
@@ -688,10 +712,10 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
 // contribution is Material::eval * lightsCount / pdf (no MIS weight).
 extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh,
                                  const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                                 const int32_t* left, const int32_t* right, const int32_t* prim,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
                                  const float* cam16, float cam_depth, int width, int height, int spp,
                                  float* film_rgb, float* film_w, int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -716,13 +740,13 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
         f3 ro = apply(cam16, mk(0, 0, 0));
         f3 rd = normalize(apply(cam16, pl) - ro);
         f3 result = mk(0, 0, 0);
-        float t;
+        float t, hu, hv;
         ++traced;
-        const int p = closest(s, ro, rd, t);
+        const int p = closest(s, ro, rd, t, &hu, &hv);
         if (p >= 0) {
             const int mesh = tri_mesh[p];
             const cav_mesh_ref& M = meshes[mesh];
-            f3 n = tri_normal(s, p);
+            f3 n = shading_normal(s, p, hu, hv);
             f3 pos = ro + t * rd;
             if (M.is_light) {
                 f3 Le = dot(rd, n) >= 0 ? mk(0, 0, 0) : mk(M.emission[0], M.emission[1], M.emission[2]);
@@ -738,7 +762,7 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
             float v = tmp * H.rand1(hidx, dim);
             f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
             f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-            f3 ln = tri_normal(s, tri);
+            f3 ln = shading_normal(s, tri, u, v);
             f3 l2s = pos - lp;
             float c = std::fabs(dot(ln, normalize(l2s)));
             float prob = dot(l2s, l2s) / (c * s.area[lm]);
@@ -921,12 +945,12 @@ extern "C" void ref_rl_finish_light(float* ldist, float* lres, int32_t* lcount, 
 // the learning rate n/(T+n) (1 = the reference's rule).
 extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                                  const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                                 const int32_t* left, const int32_t* right, const int32_t* prim,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
                                  const float* scene6, int q_grid, float alpha_scale,
                                  const float* cam16, float cam_depth, int width, int height,
                                  int spp, int max_depth, int spp_per_round, float* film_rgb,
                                  float* film_w, int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -975,15 +999,15 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
             f3 rd = normalize(apply(cam16, pl) - ro);
             q.push_back(Task{(uint32_t)si, ro, rd, mk(1, 1, 1), 1.f, true, -1, -1, dim});
         }
-        struct Hit { Task t; int prim; float dist; };
+        struct Hit { Task t; int prim; float dist, hu, hv; };
         for (int depth = 0; depth < max_depth && !q.empty(); ++depth) {
             std::vector<Hit> hits;
             hits.reserve(q.size());
             for (const Task& t : q) {
-                float d;
+                float d, hu, hv;
                 ++traced;
-                int p = closest(s, t.ro, t.rd, d);
-                if (p >= 0) hits.push_back(Hit{t, p, d});
+                int p = closest(s, t.ro, t.rd, d, &hu, &hv);
+                if (p >= 0) hits.push_back(Hit{t, p, d, hu, hv});
             }
             q.clear();
             if (hits.empty()) break;
@@ -1003,7 +1027,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 Task& t = h.t;
                 const int mesh = tri_mesh[h.prim];
                 const cav_mesh_ref& M = meshes[mesh];
-                f3 n = tri_normal(s, h.prim);
+                f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
                 Bsdf bsdf = make_bsdf(M);
                 f3 t0, t1;
@@ -1040,7 +1064,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 float v = tmp * H.rand1(t.sid, t.dim);
                 f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
                 f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-                f3 ln = tri_normal(s, tri);
+                f3 ln = shading_normal(s, tri, u, v);
                 f3 l2s = pos - lp;
                 float c = std::fabs(dot(ln, normalize(l2s)));
                 float lpdf = dot(l2s, l2s) / (c * s.area[lm]);
@@ -1066,7 +1090,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 Hit& h = hits[k];
                 Task& t = h.t;
                 const int mesh = tri_mesh[h.prim];
-                f3 n = tri_normal(s, h.prim);
+                f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
                 Bsdf bsdf = make_bsdf(meshes[mesh]);
                 f3 t0, t1;
@@ -1119,7 +1143,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 Hit& h = hits[k];
                 Task& t = h.t;
                 const int mesh = tri_mesh[h.prim];
-                f3 n = tri_normal(s, h.prim);
+                f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 Bsdf bsdf = make_bsdf(meshes[mesh]);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
@@ -1153,7 +1177,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                     t.mult = t.mult / (1.f - qq);
                 }
                 const int mesh = tri_mesh[h.prim];
-                f3 n = tri_normal(s, h.prim);
+                f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
                 Bsdf bsdf = make_bsdf(meshes[mesh]);
                 f3 t0, t1;

@@ -49,6 +49,31 @@ def test_pbrt_scene_renders_like_arrays(ctx, tmp_path, integrator):
     b.close()
 
 
+@pytest.mark.parametrize("integrator", ["path", "directlighting"])
+def test_pbrt_vertex_normals_render_like_arrays(ctx, tmp_path, integrator):
+    """A smooth-shaded scene from a .pbrt file carries its vertex normals to
+    the device: it renders exactly like the array scene built with the
+    parsed normals, and differently from the same scene without them."""
+    print(f"[pbrt] normals {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=6, width=48, height=40, materials="mixed", smooth=True)
+    p = str(tmp_path / "s.pbrt")
+    pbrt.write(sc, p, spp=4, integrator=integrator)
+    normals, flags = pbrt.parse_normals(p)
+    arr = scenes.SceneArrays(sc.tris, sc.meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height,
+                             sc.max_depth, normals=normals, mesh_normals=flags)
+    flat = scenes.SceneArrays(sc.tris, sc.meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height,
+                              sc.max_depth)
+    b, st = R.Scene.from_pbrt(ctx, p)
+    film_b, _ = R.render(ctx, b, st.spp, integrator=integrator, max_depth=2)
+    for arrays, same in ((arr, True), (flat, False)):
+        a = R.Scene(ctx, arrays, bvh_rounds=st.bvh_opt_rounds, morton_bits=st.morton_bits)
+        film_a, _ = R.render(ctx, a, st.spp, integrator=integrator, camera=for_scene(sc), max_depth=2)
+        np.testing.assert_array_equal(film_b.weight, film_a.weight)
+        assert np.array_equal(film_b.rgb, film_a.rgb) == same
+        a.close()
+    b.close()
+
+
 def test_cli_renders_png(ctx, tmp_path):
     print("[pbrt] CLI", flush=True)
     sc = scenes.cornell_box(seed=4, width=40, height=32)

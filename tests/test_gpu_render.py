@@ -100,3 +100,24 @@ def test_material_sort_keeps_the_film(ctx, integrator, depth):
     np.testing.assert_array_equal(a.weight, b.weight)
     assert sa["closest_rays"] == sb["closest_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
     scene.close()
+
+
+@pytest.mark.parametrize("integrator", ["path", "directlighting", "rlpath"])
+def test_vertex_normals_match_oracle(ctx, integrator):
+    """Smooth-shaded sphere (interpolated per-vertex normals at hits and at
+    light samples): films equal the oracle's."""
+    print(f"[render] vertex normals {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=9, width=48, height=40, materials="mixed", smooth=True)
+    cam = for_scene(sc)
+    depth = 1 if integrator == "rlpath" else sc.max_depth
+    fn = {"path": oracle.render_path, "directlighting": oracle.render_direct}.get(integrator)
+    if fn is None:
+        rgb, w, rays = oracle.render_rlpath(sc, 4, cam, max_depth=1)
+    else:
+        rgb, w, rays = fn(sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()

@@ -108,3 +108,31 @@ def test_interior_generator():
         t = sc.tris[m["first_tri"]].reshape(3, 3).astype(np.float64)
         assert np.dot(np.cross(t[1] - t[0], t[2] - t[0]), centre - t.mean(0)) > 0
     assert set(np.unique(sc.meshes["material"])) == {scenes.MATTE, scenes.PLASTIC, scenes.MIRROR, scenes.GLASS}
+
+
+def test_vertex_normals_equal_to_geometric_change_nothing():
+    """Shading with per-vertex normals that all equal the face normal gives
+    the film of flat shading, for every integrator."""
+    sc = scenes.cornell_box(seed=2, width=24, height=20, materials="mixed")
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    t = sc.tris.reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    flat = scenes.SceneArrays(sc.tris, sc.meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height,
+                              sc.max_depth, normals=np.repeat(n, 3, axis=0).reshape(-1, 9),
+                              mesh_normals=np.ones(len(sc.meshes), np.uint8))
+    for fn in (oracle.render_path, oracle.render_direct):
+        a, wa, _ = fn(sc, 2, cam)
+        b, wb, _ = fn(flat, 2, cam)
+        np.testing.assert_array_equal(wa, wb)
+        np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_smooth_sphere_changes_its_shading_only():
+    sc = scenes.cornell_box(seed=2, width=32, height=32)
+    sm = scenes.cornell_box(seed=2, width=32, height=32, smooth=True)
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    a, _, _ = oracle.render_direct(sc, 2, cam)
+    b, _, _ = oracle.render_direct(sm, 2, cam)
+    changed = np.any(a != b, axis=-1)
+    assert 0 < changed.mean() < 0.2  # the sphere covers a small part of the image

@@ -227,3 +227,81 @@ def test_incomplete_options(tmp_path):
         pbrt.parse(write(tmp_path, 'WorldBegin\nShape "trianglemesh" ' + QUAD + '\nWorldEnd\n', head=head))
     with pytest.raises(pbrt.ParseError, match="cannot open"):
         pbrt.parse(os.path.join(str(tmp_path), "none.pbrt"))
+
+
+def test_vertex_normals(tmp_path):
+    """"normal N" maps by transpose(inverse(upper 3x3)) and is normalised
+    (Core/TriangleMesh.cu:152-166); PLY nx/ny/nz the same way (:81-90);
+    sphere normals are the unit octahedron points mapped likewise
+    (ShapeMesh/Sphere.cu:67-72); meshes without normals get a zero flag."""
+    n = "[ 1 1 0  1 1 0  1 1 0  1 1 0 ]"
+    verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0]], np.float32)
+    hdr = ("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+           "property float nx\nproperty float ny\nproperty float nz\n"
+           "element face 1\nproperty list uchar int vertex_indices\nend_header\n")
+    (tmp_path / "n.ply").write_text(hdr + "0 0 0 0 0 1\n1 0 0 0 1 1\n1 1 0 0 0 2\n3 0 1 2\n")
+    body = f"""WorldBegin
+AttributeBegin
+  Scale 2 1 1
+  Shape "trianglemesh" {QUAD} "normal N" {n}
+AttributeEnd
+AttributeBegin
+  Rotate 90 0 0 1
+  Shape "trianglemesh" {QUAD} "normal N" {n}
+AttributeEnd
+Shape "trianglemesh" {QUAD}
+AttributeBegin
+  Scale 1 1 4
+  Shape "plymesh" "string filename" "n.ply"
+AttributeEnd
+AttributeBegin
+  Translate 3 0 0
+  Scale 1 2 1
+  Shape "sphere" "float radius" [ 0.5 ]
+AttributeEnd
+WorldEnd
+"""
+    p = write(tmp_path, body)
+    tris, m, _, _ = pbrt.parse(p)
+    normals, flags = pbrt.parse_normals(p)
+    assert list(flags) == [1, 1, 0, 1, 1]
+    nv = normals.reshape(-1, 3).astype(np.float64)
+
+    def unit(v):
+        v = np.asarray(v, np.float64)
+        return v / np.linalg.norm(v)
+
+    np.testing.assert_allclose(nv[0:6], np.tile(unit([0.5, 1, 0]), (6, 1)), atol=1e-6)   # diag(1/2, 1, 1)
+    np.testing.assert_allclose(nv[6:12], np.tile(unit([-1, 1, 0]), (6, 1)), atol=1e-6)   # rotated like points
+    np.testing.assert_array_equal(nv[12:18], 0)                                           # no normals
+    np.testing.assert_allclose(nv[18:21], [[0, 0, 1], unit([0, 1, 0.25]), [0, 0, 1]], atol=1e-6)  # diag(1,1,1/4)
+    # sphere: the normal at position p of an ellipsoid x^2 + (y/2)^2 + z^2 = r^2 is (x, y/4, z) normalised
+    s0 = int(m["first_tri"][4])
+    pos = tris[s0:].reshape(-1, 3).astype(np.float64) - [3, 0, 0]
+    expect = pos * [1, 0.25, 1]
+    expect /= np.linalg.norm(expect, axis=1, keepdims=True)
+    np.testing.assert_allclose(nv[3 * s0:], expect, atol=2e-6)
+    # a scene without any normals reports none
+    assert pbrt.parse_normals(write(tmp_path, f'WorldBegin\nShape "trianglemesh" {QUAD}\nWorldEnd\n', name="b.pbrt")) \
+        == (None, None)
+
+
+def test_vertex_normals_errors(tmp_path):
+    with pytest.raises(pbrt.ParseError, match="one normal per vertex"):
+        pbrt.parse(write(tmp_path, f'WorldBegin\nShape "trianglemesh" {QUAD} "normal N" [ 0 0 1 ]\nWorldEnd\n'))
+    with pytest.raises(pbrt.ParseError, match="singular"):
+        pbrt.parse(write(tmp_path, f'WorldBegin\nScale 1 0 1\nShape "trianglemesh" {QUAD} "normal N" '
+                                   '[ 0 0 1  0 0 1  0 0 1  0 0 1 ]\nWorldEnd\n'))
+
+
+def test_writer_round_trip_normals(tmp_path):
+    sc = scenes.cornell_box(seed=5, width=40, height=30, materials="mixed", smooth=True)
+    p = str(tmp_path / "s.pbrt")
+    pbrt.write(sc, p)
+    tris, _, _, _ = pbrt.parse(p)
+    normals, flags = pbrt.parse_normals(p)
+    np.testing.assert_array_equal(tris, sc.tris)
+    np.testing.assert_array_equal(flags, sc.mesh_normals)
+    on = sc.mesh_normals[np.repeat(np.arange(len(sc.meshes)), sc.meshes["num_tris"])].astype(bool)
+    # the parser normalises the written unit normals again, which may move them by an ulp or two
+    np.testing.assert_allclose(normals[on], sc.normals[on], atol=2.5e-7, rtol=0)
