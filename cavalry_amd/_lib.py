@@ -52,11 +52,15 @@ class RenderStats(ctypes.Structure):
                 ("rounds", ctypes.c_int32)]
 
 
+class Texture(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int32), ("meshes", ctypes.c_void_p),
                 ("nmeshes", ctypes.c_int32), ("tri_uv", ctypes.c_void_p), ("tri_normals", ctypes.c_void_p),
                 ("mesh_normals", ctypes.c_void_p), ("bvh_opt_rounds", ctypes.c_int32),
-                ("morton_bits", ctypes.c_int32)]
+                ("morton_bits", ctypes.c_int32), ("textures", ctypes.c_void_p), ("ntextures", ctypes.c_int32)]
 
 
 class ServeConfig(ctypes.Structure):
@@ -114,6 +118,7 @@ _SIGS = {
     "cav_pbrt_parse": ([ctypes.c_char_p, P, ctypes.POINTER(P), ctypes.c_char_p, I], I),
     "cav_pbrt_get": ([P, P, P, P, P], I),
     "cav_pbrt_get_normals": ([P, P, P], I),
+    "cav_pbrt_get_textures": ([P, P, P, P], I),
     "cav_pbrt_destroy": ([P], None),
     "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
     "cav_serve_open": ([P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P)], I),

@@ -29,6 +29,7 @@ SOURCES = [
     "serve.hip",
     "render_capi.cpp",
     "film_io.cpp",
+    "image_read.cpp",
     "pbrt.cpp",
     "server.cpp",
     "capi.cpp",
@@ -63,7 +64,7 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

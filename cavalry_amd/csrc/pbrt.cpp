@@ -16,8 +16,10 @@
 // Differences, each reported through cav_render_setup::warnings and the
 // message list: the device BSDFs cover matte / plastic / mirror / specular
 // glass, so metal, substrate and rough glass are replaced (by matte with the
-// material's Kd, and by specular glass); image textures, non-area lights and
-// the environment map are ignored; the "random" sampler renders with Halton.
+// material's Kd, and by specular glass); texture files are PNG or TGA
+// (8-bit RGBA; the reference also reads EXR and, through its image loader,
+// other formats); non-area lights and the environment map are ignored; the
+// "random" sampler renders with Halton.
 // Unlike the reference, which exits, every error returns CAV_ERR_PARSE with a
 // message.
 #include <cmath>
@@ -35,6 +37,7 @@
 
 #include "../../include/cavalry.h"
 #include "capi_internal.h"
+#include "image_read.h"
 #include "mat4.h"
 
 using namespace cav::m4;
@@ -278,8 +281,9 @@ bool read_transform(Buf& b, M4& t) {
 // ------------------------------------------------------------------ scene
 struct Mesh {
     std::vector<V3> pos;
-    std::vector<V3> nrm;   // per vertex, empty when the mesh has none
-    std::vector<int> idx;  // 3 per triangle
+    std::vector<V3> nrm;     // per vertex, empty when the mesh has none
+    std::vector<float> uv;   // 2 per vertex, empty when the mesh has none
+    std::vector<int> idx;    // 3 per triangle
 };
 
 // normalize(normalMatrix * n) for each given normal
@@ -301,11 +305,14 @@ struct Builder {
     std::vector<float> normals;  // per triangle, 9 floats (zeros for meshes without normals)
     std::vector<uint8_t> mesh_normals;
     bool any_normals = false;
+    std::vector<float> uv;       // per triangle, 6 floats (zeros for meshes without them)
+    bool any_uv = false;
+    std::vector<cav::Image> images;                    // textures, in definition order
+    std::unordered_map<std::string, int> texture_ids;  // Texture name -> index into images
     std::vector<cav_mesh> meshes;
     std::vector<std::string> messages;
     int warnings = 0;
     std::unordered_map<std::string, Material> named;
-    std::unordered_set<std::string> textures;
     // pending AreaLightSource (binds to the next shape, across subsections)
     bool next_is_light = false;
     float light_L[3] = {1, 1, 1};
@@ -342,13 +349,28 @@ void rgb_param(const Params& p, const char* k, float* out, Builder& bd, const st
     }
 }
 
+// Kd of matte and plastic: a colour, or the name of a texture defined
+// before (MatteMaterial.h:41-52, Plastic.h:77-84)
+void kd_param(const Params& p, cav_mesh& m, Builder& bd, const std::string& what) {
+    if (p.has_list("Kd")) {
+        const auto& v = p.list("Kd");
+        if (v.size() < 3) throw ParseError(what + ": Kd needs 3 values");
+        for (int c = 0; c < 3; ++c) m.kd[c] = v[c];
+    } else if (p.strings.count("Kd")) {
+        const std::string& name = p.strings.at("Kd");
+        auto it = bd.texture_ids.find(name);
+        if (it == bd.texture_ids.end()) throw ParseError(what + ": texture not found: " + name);
+        m.texture = it->second;
+    }
+}
+
 // MaterialObject::createFromObjectDefinition (Materials/MaterialObject.h:82-104)
 Material make_material(const Params& p, Builder& bd) {
     const std::string type = p.str("type");
     Material r = default_matte();
     cav_mesh& m = r.m;
     if (type == "matte") {  // MatteMaterial.h:41-52
-        rgb_param(p, "Kd", m.kd, bd, "matte");
+        kd_param(p, m, bd, "matte");
     } else if (type == "plastic") {  // Plastic.h:58-85
         m.material = CAV_MAT_PLASTIC;
         float rough = 0.1f;
@@ -356,7 +378,7 @@ Material make_material(const Params& p, Builder& bd) {
         if (p.has_num("roughness")) rough = p.num("roughness");
         if (p.has_string("remaproughness")) remap = p.str("remaproughness") == "true";
         for (int c = 0; c < 3; ++c) m.kd[c] = m.ks[c] = 0.25f;
-        rgb_param(p, "Kd", m.kd, bd, "plastic");
+        kd_param(p, m, bd, "plastic");
         rgb_param(p, "Ks", m.ks, bd, "plastic");
         m.roughness = remap ? roughness_to_alpha(rough) : rough;
     } else if (type == "mirror") {  // Mirror.h:32-38
@@ -473,7 +495,7 @@ Mesh read_ply(const std::string& path, const M4& t) {
         return v;
     };
     for (const PlyElem& e : elems) {
-        int ix = -1, iy = -1, iz = -1, ivi = -1, inx = -1, iny = -1, inz = -1;
+        int ix = -1, iy = -1, iz = -1, ivi = -1, inx = -1, iny = -1, inz = -1, iu = -1, iv = -1;
         for (size_t k = 0; k < e.props.size(); ++k) {
             if (e.props[k].name == "x") ix = (int)k;
             if (e.props[k].name == "y") iy = (int)k;
@@ -481,6 +503,8 @@ Mesh read_ply(const std::string& path, const M4& t) {
             if (e.props[k].name == "nx") inx = (int)k;
             if (e.props[k].name == "ny") iny = (int)k;
             if (e.props[k].name == "nz") inz = (int)k;
+            if (e.props[k].name == "u") iu = (int)k;
+            if (e.props[k].name == "v") iv = (int)k;
             if (e.props[k].name == "vertex_indices" || e.props[k].name == "vertex_index") ivi = (int)k;
         }
         std::vector<double> vals(e.props.size());
@@ -504,6 +528,7 @@ Mesh read_ply(const std::string& path, const M4& t) {
                 m.pos.push_back(apply(t, {(float)vals[ix], (float)vals[iy], (float)vals[iz]}));
                 if (inx >= 0 && iny >= 0 && inz >= 0)
                     m.nrm.push_back({(float)vals[inx], (float)vals[iny], (float)vals[inz]});
+                if (iu >= 0 && iv >= 0) m.uv.insert(m.uv.end(), {(float)vals[iu], (float)vals[iv]});
             }
         }
     }
@@ -579,6 +604,11 @@ Mesh make_shape(const ObjectDef& d, const M4& t, const std::string& base) {
                 for (size_t k = 0; k + 2 < N.size(); k += 3) n.push_back({N[k], N[k + 1], N[k + 2]});
                 m.nrm = transform_normals(n, t);
             }
+            if (d.params.has_list("uv")) {  // Core/TriangleMesh.cu:171-178
+                m.uv = d.params.list("uv");
+                if (m.uv.size() != 2 * m.pos.size())
+                    throw ParseError("trianglemesh: uv must give two numbers per vertex");
+            }
             return m;
         }
         throw ParseError("No valid inputs for triangle mesh");
@@ -645,6 +675,8 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
     int ntris = 0;
     const bool has_n = !m.nrm.empty();
     if (has_n && (int64_t)m.nrm.size() != nv) throw ParseError("normals and positions differ in count");
+    const bool has_uv = !m.uv.empty();
+    if (has_uv && (int64_t)m.uv.size() != 2 * nv) throw ParseError("texture coordinates and positions differ in count");
     for (size_t k = 0; k + 2 < m.idx.size(); k += 3) {
         for (int j = 0; j < 3; ++j) {
             const int vi = m.idx[k + j];
@@ -652,6 +684,8 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
             bd.soup.insert(bd.soup.end(), {m.pos[vi].x, m.pos[vi].y, m.pos[vi].z});
             if (has_n) bd.normals.insert(bd.normals.end(), {m.nrm[vi].x, m.nrm[vi].y, m.nrm[vi].z});
             else bd.normals.insert(bd.normals.end(), {0.f, 0.f, 0.f});
+            if (has_uv) bd.uv.insert(bd.uv.end(), {m.uv[2 * vi], m.uv[2 * vi + 1]});
+            else bd.uv.insert(bd.uv.end(), {0.f, 0.f});
         }
         ++ntris;
     }
@@ -674,6 +708,7 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
     bd.meshes.push_back(c);
     bd.mesh_normals.push_back(has_n ? 1 : 0);
     bd.any_normals = bd.any_normals || has_n;
+    bd.any_uv = bd.any_uv || has_uv;
 }
 
 // Parsing.cu:272-388: one XBegin ... XEnd section
@@ -724,10 +759,18 @@ void parse_section(Buf& b, State st, const std::string& base, Builder& bd) {
             ObjectDef d = read_object(b);
             d.params.strings["type"] = d.name;
             st.mat = std::make_shared<Material>(make_material(d.params, bd));
-        } else if (kw == "Texture") {
+        } else if (kw == "Texture") {  // Parsing.cu:355-358, Core/Texture.cu:12-27
             const ObjectDef d = read_object(b);
-            bd.textures.insert(d.name);
-            bd.warn("texture '" + d.name + "' is not supported");
+            cav::Image img;
+            if (d.params.has_string("filename")) {
+                std::string err;
+                if (!cav::read_texture_file(join(base, d.params.str("filename")), img, err)) throw ParseError(err);
+            } else {  // a texture without a file is one transparent black texel
+                img.w = img.h = 1;
+                img.rgba.assign(4, 0);
+            }
+            bd.texture_ids[d.name] = (int)bd.images.size();
+            bd.images.push_back(std::move(img));
         } else if (kw == "NamedMaterial") {
             ++b.i;
             const std::string n = b.pop(Tok::String, "material name").text;
@@ -755,6 +798,9 @@ struct cav_pbrt {
     std::vector<float> soup;
     std::vector<float> normals;        // empty when no mesh has vertex normals
     std::vector<uint8_t> mesh_normals;
+    std::vector<float> uv;             // empty when no mesh has texture coordinates
+    std::vector<cav::Image> images;
+    std::vector<cav_texture> textures; // views of images
     std::vector<cav_mesh> meshes;
     cav_render_setup setup{};
     std::string messages;
@@ -831,6 +877,9 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** ou
             r->normals.swap(bd.normals);
             r->mesh_normals.swap(bd.mesh_normals);
         }
+        if (bd.any_uv) r->uv.swap(bd.uv);
+        r->images.swap(bd.images);
+        for (const cav::Image& im : r->images) r->textures.push_back(cav_texture{im.rgba.data(), im.w, im.h});
         for (const std::string& m : bd.messages) r->messages += m + "\n";
         *out = r.release();
         return CAV_OK;
@@ -857,6 +906,15 @@ int cav_pbrt_get_normals(const cav_pbrt* p, const float** tri_normals, const uin
     return CAV_OK;
 }
 
+int cav_pbrt_get_textures(const cav_pbrt* p, const float** tri_uv, const cav_texture** textures,
+                          int32_t* ntextures) {
+    if (!p) return CAV_ERR_INVALID;
+    if (tri_uv) *tri_uv = p->uv.empty() ? nullptr : p->uv.data();
+    if (textures) *textures = p->textures.empty() ? nullptr : p->textures.data();
+    if (ntextures) *ntextures = (int32_t)p->textures.size();
+    return CAV_OK;
+}
+
 void cav_pbrt_destroy(cav_pbrt* p) { delete p; }
 
 int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* ov, cav_scene** out,
@@ -877,6 +935,9 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
     d.mesh_normals = p->normals.empty() ? nullptr : p->mesh_normals.data();
     d.bvh_opt_rounds = s.bvh_opt_rounds;
     d.morton_bits = s.morton_bits;
+    d.tri_uv = p->uv.empty() ? nullptr : p->uv.data();
+    d.textures = p->textures.empty() ? nullptr : p->textures.data();
+    d.ntextures = (int32_t)p->textures.size();
     rc = cav_scene_create_desc(ctx, &d, out);
     if (rc == CAV_OK && setup) *setup = s;
     cav_pbrt_destroy(p);

@@ -15,6 +15,12 @@ struct HaltonDev {
     const uint16_t* perm;    // sum of the primes
 };
 
+// one RGBA8 texture inside the scene's texel buffer
+struct TexDev {
+    int64_t off;  // first texel
+    int32_t w, h;
+};
+
 struct SceneDev {
     const TNode* tn;
     const TTri* tt;
@@ -28,6 +34,11 @@ struct SceneDev {
     int nmeshes, nlights;
     const float* normals;        // float[ntris][9] per-vertex normals, or null
     const uint8_t* mesh_normals; // per mesh: 1 = shade with the interpolated normals
+    const float* uv;             // float[ntris][6] per-vertex texture coordinates, or null (all zero)
+    const uint32_t* texels;      // RGBA8 texels of every texture, row 0 = v 0; null when no mesh has one
+    const TexDev* tex;           // indexed by cav_mesh::texture
+    // hit barycentrics are needed for vertex normals and texture coordinates
+    __host__ __device__ bool needs_bary() const { return normals != nullptr || texels != nullptr; }
 };
 
 struct CameraDev {
@@ -67,7 +78,7 @@ struct RenderArgs {
     uint8_t* vis;
     int32_t* hit_prim;
     float* hit_t;
-    float* hit_u;              // hit barycentrics, written when the scene has vertex normals
+    float* hit_u;              // hit barycentrics, written when scene.needs_bary()
     float* hit_v;
     float* res;                // [nsamp][3]
     float* sxy;                // [nsamp][2]

@@ -232,8 +232,8 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
     render_raygen(a, s);
-    float* hu = a.scene.normals ? a.hit_u : nullptr;
-    float* hv = a.scene.normals ? a.hit_v : nullptr;
+    float* hu = a.scene.needs_bary() ? a.hit_u : nullptr;
+    float* hv = a.scene.needs_bary() ? a.hit_v : nullptr;
     if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[0].ray, a.counts, nq,
                                  a.hit_prim, a.hit_t, hu, hv, a.trace_counters, s)) != hipSuccess)
         return e;
@@ -253,8 +253,8 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
 hipError_t render_path(const RenderArgs& a, hipStream_t s) {
     const int64_t nsamp = a.nq;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
-    float* hu = a.scene.normals ? a.hit_u : nullptr;
-    float* hv = a.scene.normals ? a.hit_v : nullptr;
+    float* hu = a.scene.needs_bary() ? a.hit_u : nullptr;
+    float* hv = a.scene.needs_bary() ? a.hit_v : nullptr;
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
     render_raygen(a, s);

@@ -32,6 +32,8 @@ struct cav_scene {
     float* uv = nullptr;
     float* normals = nullptr;        // float[ntris][9] or null
     uint8_t* mesh_normals = nullptr; // per mesh
+    uint32_t* texels = nullptr;      // every texture's RGBA8 texels
+    TexDev* tex = nullptr;
     std::vector<cav_mesh> host_meshes;
     std::vector<float> host_area;
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // Scene::sceneBounds (Core/Scene.cu:131-136)
@@ -246,9 +248,10 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 
 namespace cav {
 SceneDev scene_dev(const cav_scene* sc) {
-    return SceneDev{sc->bvh->tn, sc->bvh->tt,   sc->bvh->root, sc->ntris,   sc->soup,
-                    sc->tri_mesh, sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
-                    sc->nlights,  sc->normals,  sc->mesh_normals};
+    return SceneDev{sc->bvh->tn,   sc->bvh->tt,  sc->bvh->root, sc->ntris,   sc->soup,
+                    sc->tri_mesh,   sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
+                    sc->nlights,    sc->normals,  sc->mesh_normals, sc->uv,   sc->texels,
+                    sc->tex};
 }
 
 int halton_dev(cav_ctx* ctx, HaltonDev* out) {
@@ -312,6 +315,27 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
     }
     if (next != n) return fail(ctx, CAV_ERR_INVALID, "meshes do not cover every triangle");
     if (nlights == 0) return fail(ctx, CAV_ERR_INVALID, "scene has no area light");
+    // textures: packed into one texel buffer
+    const int ntex = desc->ntextures;
+    if (ntex < 0 || (ntex > 0 && !desc->textures)) return fail(ctx, CAV_ERR_INVALID, "bad texture list");
+    std::vector<TexDev> tex_host((size_t)ntex);
+    int64_t ntexels = 0;
+    for (int k = 0; k < ntex; ++k) {
+        const cav_texture& T = desc->textures[k];
+        // sides up to 2^16 keep u W - 0.5 below W - 0.5 in float for every u < 1
+        if (!T.rgba || T.width < 1 || T.height < 1 || T.width > 65536 || T.height > 65536)
+            return fail(ctx, CAV_ERR_INVALID, "texture " + std::to_string(k) + " must be 1 to 65536 texels on each side");
+        tex_host[(size_t)k] = TexDev{ntexels, T.width, T.height};
+        ntexels += (int64_t)T.width * T.height;
+    }
+    bool any_tex = false;
+    for (int m = 0; m < nmeshes; ++m) {
+        const int32_t ti = meshes[m].texture;
+        if (ti < -1 || ti >= ntex)
+            return fail(ctx, CAV_ERR_INVALID, "mesh " + std::to_string(m) + " names texture " + std::to_string(ti) +
+                                                  " of " + std::to_string(ntex));
+        any_tex = any_tex || ti >= 0;
+    }
     RCHK(ctx, hipSetDevice(ctx->device));
     cav_scene* sc = new cav_scene();
     // union of the triangle boxes, min/max as IEEE minimumNumber/maximumNumber
@@ -359,7 +383,20 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
         if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->normals, 36 * (size_t)n, "vertex normals");
         if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->mesh_normals, (size_t)nmeshes, "mesh normal flags");
     }
+    if (any_tex) {
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->tex, sizeof(TexDev) * (size_t)ntex, "textures");
+        if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->texels, 4 * (size_t)ntexels, "texels");
+    }
     hipError_t e = hipSuccess;
+    if (rc == CAV_OK && any_tex) {
+        e = hipMemcpy(sc->tex, tex_host.data(), sizeof(TexDev) * (size_t)ntex, hipMemcpyHostToDevice);
+        for (int k = 0; k < ntex && e == hipSuccess; ++k) {
+            const cav_texture& T = desc->textures[k];
+            e = hipMemcpy(sc->texels + tex_host[(size_t)k].off, T.rgba, 4 * (size_t)T.width * T.height,
+                          hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "upload textures");
+    }
     if (rc == CAV_OK) {
         e = hipMemcpy(sc->soup, tris, 36 * (size_t)n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(sc->tri_mesh, tri_mesh.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
@@ -386,7 +423,8 @@ int cav_scene_destroy(cav_scene* sc) {
     if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
     if (sc->bvh) cav_bvh_destroy(sc->bvh);
     for (void* p : {(void*)sc->soup, (void*)sc->tri_mesh, (void*)sc->meshes, (void*)sc->area,
-                    (void*)sc->lights, (void*)sc->uv, (void*)sc->normals, (void*)sc->mesh_normals})
+                    (void*)sc->lights, (void*)sc->uv, (void*)sc->normals, (void*)sc->mesh_normals,
+                    (void*)sc->texels, (void*)sc->tex})
         if (p) (void)hipFree(p);
     delete sc;
     return CAV_OK;

@@ -322,8 +322,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
         const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
         const int mesh = sc.tri_mesh[prim];
-        const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
-        const v3 nn = shading_normal(sc, prim, hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f);
+        const float hu = hit_u ? hit_u[i] : 0.f, hv = hit_u ? hit_v[i] : 0.f;
+        const Bsdf bsdf = surface_bsdf(sc, mesh, prim, hu, hv);
+        const v3 nn = shading_normal(sc, prim, hu, hv);
         v3 t0, t1;
         frame(nn, t0, t1);
         const v3 ol = to_local(scale3(rd, -1.f), t0, t1, nn);
@@ -613,8 +614,9 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
         }
         if (alive) {
             const int mesh = sc.tri_mesh[prim];
-            const Bsdf bsdf = make_bsdf(sc.meshes[mesh]);
-            const v3 nn = shading_normal(sc, prim, hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f);
+            const float hu = hit_u ? hit_u[i] : 0.f, hv = hit_u ? hit_v[i] : 0.f;
+            const Bsdf bsdf = surface_bsdf(sc, mesh, prim, hu, hv);
+            const v3 nn = shading_normal(sc, prim, hu, hv);
             v3 t0, t1;
             frame(nn, t0, t1);
             const v3 pos = add3(ro, scale3(rd, hit_t[i]));
@@ -675,8 +677,8 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
 
 hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s) {
     const int64_t nsamp = a.nq;
-    float* hu = a.scene.normals ? a.hit_u : nullptr;
-    float* hv = a.scene.normals ? a.hit_v : nullptr;
+    float* hu = a.scene.needs_bary() ? a.hit_u : nullptr;
+    float* hv = a.scene.needs_bary() ? a.hit_v : nullptr;
     const unsigned nb = (unsigned)((nsamp + kBlock - 1) / kBlock);
     const int L = a.scene.nlights;
     const int64_t nent = (int64_t)rl.grid * rl.grid * rl.grid;

@@ -36,7 +36,7 @@ struct PoolDev {
     uint8_t* vis;
     int32_t* hit_prim;
     float* hit_t;
-    float* hit_u;              // hit barycentrics (scenes with vertex normals)
+    float* hit_u;              // hit barycentrics (when scene.needs_bary())
     float* hit_v;
     uint32_t* counts;          // [0], [1] ray queues, [2] shadow queue
     unsigned long long* rays;  // traced closest / shadow rays

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
         const float* ray = qi.ray + i * 6;
         const v3 ro = mk3(ray[0], ray[1], ray[2]), rd = mk3(ray[3], ray[4], ray[5]);
         float* acc = r.res + (int64_t)sid * 3;
-        const float hu = p.scene.normals ? p.hit_u[i] : 0.f, hv = p.scene.normals ? p.hit_v[i] : 0.f;
+        const float hu = p.scene.needs_bary() ? p.hit_u[i] : 0.f, hv = p.scene.needs_bary() ? p.hit_v[i] : 0.f;
         if (r.integrator == CAV_INTEGRATOR_DIRECT) {
             direct_shade(p.scene, p.halton, sid, ro, rd, prim, p.hit_t[i], hu, hv, acc, o);
         } else {
@@ -142,7 +142,7 @@ hipError_t pool_bounce(const PoolDev& p, int qin, hipStream_t s) {
     const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
     hipError_t e = trace_closest_queue(p.scene.tn, p.scene.tt, p.scene.ntris, p.scene.root, p.q[qin].ray,
                                        p.counts + qin, p.cap, p.hit_prim, p.hit_t,
-                                       p.scene.normals ? p.hit_u : nullptr, p.scene.normals ? p.hit_v : nullptr,
+                                       p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
                                        p.trace_counters, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pool_shade, dim3(nb), dim3(kBlock), 0, s, p, qin);

@@ -301,6 +301,40 @@ __device__ inline Bsdf make_bsdf(const cav_mesh& m) {
     return b;
 }
 
+// Texture2D::readTexture (Core/Texture.h:92-95, texture set up at :101-125):
+// normalized coordinates, wrap addressing on both axes, bilinear filtering of
+// RGBA8 texels read as value / 255. The filter is evaluated here in float
+// with exact fractional weights: texel coordinate x = u W - 0.5, the weights
+// are the fractions of x and y, texels outside the image wrap around.
+__device__ inline float wrap01(float u) {
+    float w = u - floorf(u);
+    if (!(w >= 0.f && w < 1.f)) w = 0.f;  // non-finite input, or a tiny negative u that rounds to 1
+    return w;
+}
+__device__ inline v3 tex_fetch(const SceneDev& sc, int ti, float u, float v) {
+    const TexDev T = sc.tex[ti];
+    const float x = wrap01(u) * (float)T.w - 0.5f, y = wrap01(v) * (float)T.h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    int i1 = i0 + 1, j1 = j0 + 1;
+    if (i0 < 0) i0 += T.w;
+    if (j0 < 0) j0 += T.h;
+    if (i1 >= T.w) i1 -= T.w;
+    if (j1 >= T.h) j1 -= T.h;
+    const uint32_t* px = sc.texels + T.off;
+    const uint32_t c00 = px[(int64_t)j0 * T.w + i0], c10 = px[(int64_t)j0 * T.w + i1];
+    const uint32_t c01 = px[(int64_t)j1 * T.w + i0], c11 = px[(int64_t)j1 * T.w + i1];
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float t00 = (float)((c00 >> (8 * c)) & 255u) / 255.f, t10 = (float)((c10 >> (8 * c)) & 255u) / 255.f;
+        const float t01 = (float)((c01 >> (8 * c)) & 255u) / 255.f, t11 = (float)((c11 >> (8 * c)) & 255u) / 255.f;
+        r[c] = (1.f - b) * ((1.f - a) * t00 + a * t10) + b * ((1.f - a) * t01 + a * t11);
+    }
+    return mk3(r[0], r[1], r[2]);
+}
+
 __device__ inline v3 soup_v(const float* soup, int t, int j) {
     const float* p = soup + (int64_t)t * 9 + j * 3;
     return mk3(p[0], p[1], p[2]);
@@ -340,6 +374,29 @@ __device__ inline v3 apply_m(const float* m, v3 v) {
     return mk3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
 }
 
+// The BSDF at a hit (Material::getBSDF): the mesh's constants, with a Kd
+// texture read at the interpolated texture coordinates
+// tex0 (1 - u - v) + u tex1 + v tex2 (Core/Triangle.h:128-131) multiplying a
+// matte's colour (Materials/MatteMaterial.h:23-33) or replacing a plastic's
+// diffuse colour (Materials/Plastic.h:38-50).
+__device__ inline Bsdf surface_bsdf(const SceneDev& sc, int mesh, int prim, float u, float v) {
+    const cav_mesh& M = sc.meshes[mesh];
+    Bsdf b = make_bsdf(M);
+    const int ti = M.texture;
+    if (sc.texels && ti >= 0 && (M.material == CAV_MAT_MATTE || M.material == CAV_MAT_PLASTIC)) {
+        float tu = 0.f, tv = 0.f;
+        if (sc.uv) {
+            const float* q = sc.uv + (int64_t)prim * 6;
+            const float w = 1.f - u - v;
+            tu = (q[0] * w + u * q[2]) + v * q[4];
+            tv = (q[1] * w + u * q[3]) + v * q[5];
+        }
+        const v3 texel = tex_fetch(sc, ti, tu, tv);
+        b.kd = M.material == CAV_MAT_MATTE ? mul3(b.kd, texel) : texel;
+    }
+    return b;
+}
+
 // What shading one task produces: an optional shadow task and an optional
 // continuation ray.
 struct ShadeOut {
@@ -362,7 +419,7 @@ __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const
     const cav_mesh M = sc.meshes[mesh];
     const v3 nn = shading_normal(sc, prim, hu, hv);
     const v3 pos = add3(ro, scale3(rd, t));
-    const Bsdf bsdf = make_bsdf(M);
+    const Bsdf bsdf = surface_bsdf(sc, mesh, prim, hu, hv);
     v3 t0, t1;
     frame(nn, t0, t1);
     if (M.is_light) {
@@ -467,7 +524,7 @@ __device__ inline void direct_shade(const SceneDev& sc, const HaltonDev& h, uint
     const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
     const v3 dir = nrm(sub3(lp, pos));
     const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
-    const Bsdf bsdf = make_bsdf(M);
+    const Bsdf bsdf = surface_bsdf(sc, mesh, prim, hu, hv);
     v3 t0, t1;
     frame(nn, t0, t1);
     const v3 il = to_local(dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);

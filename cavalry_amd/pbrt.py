@@ -9,6 +9,7 @@ LookAt camera), with float32 values printed so they round-trip bit for bit.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -40,7 +41,8 @@ def overrides(output=None, integrator=None, spp=0, bvh_opt_rounds=-1, morton_bit
 
 
 def _parse(path: str, **kw):
-    """(tris, meshes, setup, messages, normals or None, mesh flags or None)"""
+    """(tris, meshes, setup, messages, normals or None, mesh flags or None,
+    uv or None, textures list)"""
     ov = overrides(**kw)
     h = ctypes.c_void_p()
     err = ctypes.create_string_buffer(1024)
@@ -62,7 +64,18 @@ def _parse(path: str, **kw):
         if nptr.value:
             normals = np.ctypeslib.as_array((ctypes.c_float * (9 * n)).from_address(nptr.value)).reshape(n, 9).copy()
             flags = np.ctypeslib.as_array((ctypes.c_uint8 * m).from_address(fptr.value)).copy()
-        return tris, meshes, st, messages, normals, flags
+        uptr, tptr, ntex = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
+        check(None, lib().cav_pbrt_get_textures(h, ctypes.byref(uptr), ctypes.byref(tptr), ctypes.byref(ntex)))
+        uv = None
+        if uptr.value:
+            uv = np.ctypeslib.as_array((ctypes.c_float * (6 * n)).from_address(uptr.value)).reshape(n, 6).copy()
+        textures = []
+        if ntex.value:
+            views = (_lib.Texture * ntex.value).from_address(tptr.value)
+            for t in views:
+                raw = (ctypes.c_uint8 * (4 * t.width * t.height)).from_address(t.rgba)
+                textures.append(np.frombuffer(bytes(raw), np.uint8).reshape(t.height, t.width, 4).copy())
+        return tris, meshes, st, messages, normals, flags, uv, textures
     finally:
         lib().cav_pbrt_destroy(h)
 
@@ -77,7 +90,14 @@ def parse_normals(path: str, **kw):
     """Per-vertex normals of a parsed scene, gathered per triangle corner:
     (normals [n, 9] float32 or None, one flag per mesh or None). None when
     no shape gave normals ("normal N", PLY nx/ny/nz, spheres)."""
-    return _parse(path, **kw)[4:]
+    return _parse(path, **kw)[4:6]
+
+
+def parse_textures(path: str, **kw):
+    """Texture coordinates and textures of a parsed scene: (uv [n, 6] float32
+    or None, list of RGBA8 [H, W, 4] textures, row 0 at v = 0) that
+    meshes["texture"] indexes."""
+    return _parse(path, **kw)[6:8]
 
 
 def _f(x) -> str:
@@ -89,10 +109,20 @@ def _list(v) -> str:
     return "[ " + " ".join(_f(x) for x in np.asarray(v, np.float32).ravel()) + " ]"
 
 
+def _srgb8(linear8: np.ndarray) -> np.ndarray:
+    """Bytes that the loader's inverse sRGB curve maps back to (about) the
+    given linear bytes."""
+    x = linear8.astype(np.float64) / 255
+    g = np.where(x <= 0.0031308, 12.92 * x, 1.055 * x ** (1 / 2.4) - 0.055)
+    return np.clip(np.round(g * 255), 0, 255).astype(np.uint8)
+
+
 def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", filename: str = "output.png",
           sampler: str = "halton") -> None:
     """Writes a generated scene as a .pbrt file the parser reads back into the
-    same triangles, meshes, camera and vertex normals."""
+    same triangles, meshes, camera, vertex normals and texture coordinates.
+    Textures go to PNG files next to it (<name>_tex<k>.png), sRGB-encoded: the
+    loader's inverse curve returns each byte to within one or two units."""
     out = [
         "# generated scene",
         f"LookAt {' '.join(_f(x) for x in arr.eye)} {' '.join(_f(x) for x in arr.center)} "
@@ -105,13 +135,24 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         'PixelFilter "box"',
         "WorldBegin",
     ]
+    stem = os.path.splitext(os.path.basename(path))[0]
+    for k, tex in enumerate(getattr(arr, "textures", None) or []):
+        name = f"{stem}_tex{k}.png"
+        img = np.ascontiguousarray(_srgb8(tex[::-1, :, :3]))  # files list rows top first
+        check(None, lib().cav_write_png(os.path.join(os.path.dirname(os.path.abspath(path)), name).encode(),
+                                        img.ctypes.data_as(ctypes.c_void_p), tex.shape[1], tex.shape[0]))
+        out.append(f'Texture "tex{k}" "spectrum" "imagemap" "string filename" [ "{name}" ]')
     for mi, m in enumerate(arr.meshes):
         out.append("AttributeBegin")
         mat = int(m["material"])
+        tex = int(m["texture"])
+        kd = f'"texture Kd" "tex{tex}"' if tex >= 0 and mat in (MATTE, PLASTIC) else f'"rgb Kd" {_list(m["kd"])}'
         if mat == MATTE:
-            out.append(f'  Material "matte" "rgb Kd" {_list(m["kd"])}')
+            if tex >= 0 and np.any(m["kd"] != 1):
+                raise ValueError("a textured matte mesh needs kd = 1 to have a .pbrt form")
+            out.append(f'  Material "matte" {kd}')
         elif mat == PLASTIC:
-            out.append(f'  Material "plastic" "rgb Kd" {_list(m["kd"])} "rgb Ks" {_list(m["ks"])} '
+            out.append(f'  Material "plastic" {kd} "rgb Ks" {_list(m["ks"])} '
                        f'"float roughness" [ {_f(m["roughness"])} ] "bool remaproughness" "false"')
         elif mat == MIRROR:
             out.append(f'  Material "mirror" "float Kr" [ {_f(m["kr"][0])} ]')
@@ -129,6 +170,9 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         normals = ""
         if getattr(arr, "normals", None) is not None and (arr.mesh_normals is None or arr.mesh_normals[mi]):
             normals = f' "normal N" {_list(arr.normals[m["first_tri"]:m["first_tri"] + m["num_tris"]])}'
+        uv = getattr(arr, "uv", None)
+        if uv is not None:
+            normals += f' "float uv" {_list(uv[m["first_tri"]:m["first_tri"] + m["num_tris"]])}'
         out.append(f'  Shape "trianglemesh" "integer indices" [ {idx} ] "point P" {_list(t)}{normals}')
         out.append("AttributeEnd")
     out.append("WorldEnd")

@@ -88,10 +88,19 @@ class Scene:
         meshes = np.ascontiguousarray(arrays.meshes)
         normals = getattr(arrays, "normals", None)
         flags = getattr(arrays, "mesh_normals", None)
-        self._keep = (tris, meshes, normals, flags)
-        d = _lib.SceneDesc(tris.ctypes.data, len(tris), meshes.ctypes.data, len(meshes), None,
+        uv = getattr(arrays, "uv", None)
+        textures = getattr(arrays, "textures", None) or []
+        tex = (_lib.Texture * max(1, len(textures)))()
+        for k, t in enumerate(textures):
+            if t.ndim != 3 or t.shape[2] != 4 or t.dtype != np.uint8 or not t.flags.c_contiguous:
+                raise ValueError("textures must be contiguous uint8 [H, W, 4] arrays")
+            tex[k] = _lib.Texture(t.ctypes.data, t.shape[1], t.shape[0])
+        self._keep = (tris, meshes, normals, flags, uv, textures, tex)
+        d = _lib.SceneDesc(tris.ctypes.data, len(tris), meshes.ctypes.data, len(meshes),
+                           None if uv is None else uv.ctypes.data,
                            None if normals is None else normals.ctypes.data,
-                           None if flags is None else flags.ctypes.data, int(bvh_rounds), int(morton_bits))
+                           None if flags is None else flags.ctypes.data, int(bvh_rounds), int(morton_bits),
+                           ctypes.addressof(tex) if textures else None, len(textures))
         check(ctx.handle, lib().cav_scene_create_desc(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
 

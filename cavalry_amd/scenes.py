@@ -79,11 +79,15 @@ class SceneArrays:
     and a LookAt camera."""
 
     def __init__(self, tris, meshes, eye, center, up, fov, width, height, max_depth=8, normals=None,
-                 mesh_normals=None):
+                 mesh_normals=None, uv=None, textures=None):
         self.tris = np.ascontiguousarray(tris, np.float32)
         # optional per-vertex shading normals float32[N, 9] and per-mesh use flags
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32)
         self.mesh_normals = None if mesh_normals is None else np.ascontiguousarray(mesh_normals, np.uint8)
+        # optional per-vertex texture coordinates float32[N, 6] and RGBA8 textures
+        # (uint8 [H, W, 4], row 0 at v = 0) that meshes["texture"] indexes (-1: none)
+        self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32)
+        self.textures = [np.ascontiguousarray(t, np.uint8) for t in (textures or [])]
         self.meshes = meshes
         self.eye = np.asarray(eye, np.float32)
         self.center = np.asarray(center, np.float32)
@@ -178,13 +182,60 @@ def uv_sphere(center, r, nphi=64, ntheta=32):
     return t
 
 
+def planar_uv(tris: np.ndarray, scale: float = 1.0) -> np.ndarray:
+    """Texture coordinates by planar projection along each triangle's
+    dominant normal axis: the other two coordinates times `scale`."""
+    t = np.asarray(tris, np.float64).reshape(-1, 3, 3)
+    n = np.abs(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]))
+    ax = np.argmax(n, axis=1)
+    keep = np.array([[1, 2], [0, 2], [0, 1]])[ax]                 # [N, 2]
+    uv = np.take_along_axis(t, keep[:, None, :].repeat(3, 1), axis=2) * scale
+    return uv.reshape(-1, 6).astype(np.float32)
+
+
+def spherical_uv(tris: np.ndarray, center) -> np.ndarray:
+    """Texture coordinates by spherical projection about `center`:
+    u = atan2(z, x) / 2pi + 1/2, v = asin(y / r) / pi + 1/2."""
+    d = np.asarray(tris, np.float64).reshape(-1, 3, 3) - np.asarray(center, np.float64)
+    r = np.linalg.norm(d, axis=2)
+    u = np.arctan2(d[..., 2], d[..., 0]) / (2 * np.pi) + 0.5
+    v = np.arcsin(np.clip(d[..., 1] / np.maximum(r, 1e-30), -1, 1)) / np.pi + 0.5
+    return np.stack([u, v], 2).reshape(-1, 6).astype(np.float32)
+
+
+def procedural_texture(kind: str, size: int, seed: int) -> np.ndarray:
+    """RGBA8 [size, size, 4] pattern: "checker" (8 x 8 squares of two seeded
+    colours), "stripes" (diagonal bands) or "noise" (value-noise marble)."""
+    rng = _rng(seed, 13)
+    c0 = rng.uniform(0.1, 0.9, 3)
+    c1 = rng.uniform(0.1, 0.9, 3)
+    y, x = np.mgrid[0:size, 0:size].astype(np.float64) / size
+    if kind == "checker":
+        w = ((np.floor(x * 8) + np.floor(y * 8)) % 2)
+    elif kind == "stripes":
+        w = 0.5 + 0.5 * np.sin(2 * np.pi * 6 * (x + y))
+    elif kind == "noise":
+        p = np.stack([x.ravel() * 8, y.ravel() * 8, np.zeros(x.size)], 1)
+        w = (0.5 + 0.5 * np.sin(6 * x + 4 * _value_noise(p, seed).reshape(size, size))) ** 2
+    else:
+        raise ValueError(kind)
+    rgb = c0 * (1 - w[..., None]) + c1 * w[..., None]
+    out = np.empty((size, size, 4), np.uint8)
+    out[..., :3] = np.clip(np.round(rgb * 255), 0, 255)
+    out[..., 3] = 255
+    return out
+
+
 def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: int = 8,
-                materials: str = "matte", smooth: bool = False) -> SceneArrays:
+                materials: str = "matte", smooth: bool = False, textured: bool = False) -> SceneArrays:
     """Cornell-style box: unit box open at z=0 (5 walls x 2 quads), a tall and a
     short box, a 64x32 UV sphere (3968 triangles) and a 0.25x0.25 ceiling light
     of radiance 15 facing down. Albedos U[0.1,0.9] per mesh from the seed.
     materials="mixed" makes the boxes plastic/mirror and the sphere glass.
-    smooth=True gives the sphere radial per-vertex normals (smooth shading)."""
+    smooth=True gives the sphere radial per-vertex normals (smooth shading).
+    textured=True puts a checker on the floor, stripes on the back wall and
+    value-noise marble on the tall box (64 x 64 textures, planar UVs; the
+    matte ones with kd = 1), and spherical UVs on the sphere."""
     rng = _rng(seed, 7)
     b = _Builder()
 
@@ -221,8 +272,19 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
         n /= np.linalg.norm(n, axis=2, keepdims=True)
         normals[m["first_tri"]:m["first_tri"] + m["num_tris"]] = n.reshape(-1, 9)
         flags[7] = 1
+    uv = textures = None
+    if textured:
+        uv = planar_uv(tris, 2.0)
+        m = meshes[7]
+        uv[m["first_tri"]:m["first_tri"] + m["num_tris"]] = spherical_uv(
+            tris[m["first_tri"]:m["first_tri"] + m["num_tris"]], (0.5, 0.72, 0.3))
+        textures = [procedural_texture(k, 64, seed + i) for i, k in enumerate(("checker", "stripes", "noise"))]
+        meshes["texture"][[0, 2, 5]] = [0, 1, 2]
+        # a textured matte takes its colour from the texture alone, as a
+        # .pbrt Kd is either a colour or a texture
+        meshes["kd"][[k for k in (0, 2, 5) if meshes["material"][k] == MATTE]] = 1.0
     return SceneArrays(tris, meshes, (0.5, 0.5, -1.3), (0.5, 0.5, 0.5), (0, 1, 0), 40.0,
-                       width, height, max_depth, normals=normals, mesh_normals=flags)
+                       width, height, max_depth, normals=normals, mesh_normals=flags, uv=uv, textures=textures)
 
 
 @functools.lru_cache(maxsize=4)

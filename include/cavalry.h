@@ -133,7 +133,8 @@ enum { CAV_RENDER_SORT_BY_MATERIAL = 1 };
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),
  * 1 mirror (kr[0]), 4 glass (kr, kt, ior; perfectly specular), 5 plastic
- * (kd, ks, roughness = GGX alpha). texture = -1 for none. is_light marks a
+ * (kd, ks, roughness = GGX alpha). texture: index into the scene's textures
+ * of a Kd texture (matte and plastic), or -1 for none. is_light marks a
  * DiffuseAreaLight of radiance `emission` (Lights/DiffuseAreaLight.h). */
 typedef struct {
     int32_t first_tri, num_tris;
@@ -174,10 +175,23 @@ typedef struct cav_scene cav_scene;
 int cav_scene_create(cav_ctx* ctx, const float* triangles, int n, const cav_mesh* meshes,
                      int nmeshes, const float* tri_uv, int bvh_opt_rounds, int morton_bits,
                      cav_scene** out);
+/* An RGBA8 image (Texture2D, Core/Texture.h:12-125): width * height texels,
+ * row-major, row 0 at texture coordinate v = 0 (the bottom row of an image
+ * file, which the reference flips on load), 1 to 65536 texels per side.
+ * Read with wrap addressing and bilinear filtering as value / 255. */
+typedef struct {
+    const uint8_t* rgba;
+    int32_t width, height;
+} cav_texture;
+
 /* The same with every scene input in one descriptor, including per-vertex
  * shading normals: Triangle::fillIntersectionInformation and
  * TriangleMesh::getNormal (Core/Triangle.h:94-136, Core/TriangleMesh.h:76-104)
- * shade a mesh that has them with normalize(n0 (1-u-v) + u n1 + v n2). */
+ * shade a mesh that has them with normalize(n0 (1-u-v) + u n1 + v n2).
+ * Textures: a mesh whose texture field is k >= 0 reads textures[k] at the
+ * interpolated tri_uv (all zero when tri_uv is NULL); the texel multiplies a
+ * matte colour (Materials/MatteMaterial.h:23-33) or replaces a plastic's
+ * diffuse colour (Materials/Plastic.h:38-50). */
 typedef struct {
     const float* tris;            /* float[ntris][9] */
     int32_t ntris;
@@ -188,6 +202,8 @@ typedef struct {
     const uint8_t* mesh_normals;  /* per mesh 1 = shade with tri_normals (NULL: every mesh) */
     int32_t bvh_opt_rounds;
     int32_t morton_bits;
+    const cav_texture* textures;  /* ntextures images, or NULL */
+    int32_t ntextures;
 } cav_scene_desc;
 int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** out);
 int cav_scene_destroy(cav_scene* scene);
@@ -270,6 +286,12 @@ int cav_pbrt_get(const cav_pbrt* p, const float** tris, const cav_mesh** meshes,
 /* Per-vertex normals of a parsed scene (float[ntris][9] and one flag per
  * mesh), or NULLs when no mesh has any ("normal N", PLY nx/ny/nz, spheres). */
 int cav_pbrt_get_normals(const cav_pbrt* p, const float** tri_normals, const uint8_t** mesh_normals);
+/* Per-vertex texture coordinates (float[ntris][6], NULL when no shape gave
+ * "uv" or PLY u/v) and the textures that cav_mesh::texture indexes
+ * ("Texture" directives, PNG / TGA files decoded as Texture2D receives them:
+ * rows bottom-up, sRGB curve removed). Owned by the parsed scene. */
+int cav_pbrt_get_textures(const cav_pbrt* p, const float** tri_uv, const cav_texture** textures,
+                          int32_t* ntextures);
 void cav_pbrt_destroy(cav_pbrt* p);
 /* Replaces readRenderSetup + Scene::prepareForRender (main.cu:10-15):
  * parse, then cav_scene_create with the setup's BVH rounds and Morton width. */

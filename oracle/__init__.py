@@ -152,16 +152,41 @@ def mesh_areas(tris, meshes) -> np.ndarray:
     return out
 
 
-def _normals(scene):
-    """(tri_normals, mesh_normals) pointers of a scene, or two NULLs."""
-    nrm = getattr(scene, "normals", None)
-    if nrm is None:
-        return None, None
-    flags = getattr(scene, "mesh_normals", None)
-    if flags is None:
-        flags = np.ones(len(scene.meshes), np.uint8)
-    scene._oracle_keep = (np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(flags, np.uint8))
-    return _p(scene._oracle_keep[0]), _p(scene._oracle_keep[1])
+class _Surface(ctypes.Structure):
+    """ref_surface of ref_api.h"""
+    _fields_ = [("tri_normals", ctypes.c_void_p), ("mesh_normals", ctypes.c_void_p), ("tri_uv", ctypes.c_void_p),
+                ("texels", ctypes.c_void_p), ("tex_offset", ctypes.c_void_p), ("tex_size", ctypes.c_void_p)]
+
+
+def _surface(scene):
+    """A ref_surface pointer for the scene's normals, texture coordinates and
+    textures (the arrays are kept alive on the scene object)."""
+    def arr(name, dt):
+        a = getattr(scene, name, None)
+        return None if a is None else np.ascontiguousarray(a, dt)
+
+    nrm, flags, uv = arr("normals", np.float32), arr("mesh_normals", np.uint8), arr("uv", np.float32)
+    textures = getattr(scene, "textures", None) or []
+    if int(np.max(scene.meshes["texture"], initial=-1)) >= len(textures):
+        raise ValueError("a mesh names a missing texture")
+    texels = sizes = offs = None
+    if textures:
+        texels = np.concatenate([np.ascontiguousarray(t, np.uint8).ravel() for t in textures])
+        sizes = np.array([[t.shape[1], t.shape[0]] for t in textures], np.int32)
+        offs = np.concatenate([[0], np.cumsum(sizes[:, 0].astype(np.int64) * sizes[:, 1])[:-1]]).astype(np.int64)
+    keep = (nrm, flags, uv, texels, offs, sizes)
+    s = _Surface(*[None if a is None else a.ctypes.data for a in keep])
+    scene._oracle_keep = (keep, s)
+    return ctypes.byref(s)
+
+
+def tex_fetch(rgba, u: float, v: float) -> np.ndarray:
+    """Bilinear, wrap-addressed read of an RGBA8 texture [H, W, 4] at (u, v)."""
+    t = np.ascontiguousarray(rgba, np.uint8)
+    out = np.zeros(3, np.float32)
+    lib().ref_tex_fetch(_p(t), ctypes.c_int(t.shape[1]), ctypes.c_int(t.shape[0]), ctypes.c_float(u),
+                        ctypes.c_float(v), _p(out))
+    return out
 
 
 def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30):
@@ -179,7 +204,7 @@ def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int =
     tri_mesh = scene.tri_mesh
     rc = lib().ref_render_path(
         _p(tris), ctypes.c_int(len(tris)), _p(tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(cam16),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _surface(scene), _p(cam16),
         ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp),
         ctypes.c_int(scene.max_depth), _p(rgb), _p(wt), _p(rays))
     if rc != 0:
@@ -200,7 +225,7 @@ def render_direct(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
     cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
     rc = lib().ref_render_direct(
         _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(cam16),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _surface(scene), _p(cam16),
         ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp), _p(rgb), _p(wt),
         _p(rays))
     if rc != 0:
@@ -227,7 +252,7 @@ def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
     depth = scene.max_depth if max_depth is None else max_depth
     rc = lib().ref_render_rlpath(
         _p(tris), ctypes.c_int(len(tris)), _p(scene.tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
-        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), *_normals(scene), _p(s6), ctypes.c_int(q_grid),
+        _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _surface(scene), _p(s6), ctypes.c_int(q_grid),
         ctypes.c_float(alpha_scale), _p(cam16), ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h),
         ctypes.c_int(spp), ctypes.c_int(depth), ctypes.c_int(spp_per_round), _p(rgb), _p(wt), _p(rays))
     if rc != 0:

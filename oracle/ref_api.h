@@ -34,20 +34,32 @@ int ref_trace_occluded(const float* box, const int32_t* left, const int32_t* rig
                        const int32_t* dst_mesh, int64_t nrays, uint8_t* visible);
 
 // render oracles (ref_render.cpp)
+// Surface inputs beyond the triangles: per-vertex normals, texture
+// coordinates and the Kd textures that cav_mesh_ref::texture indexes (any
+// pointer may be null).
+typedef struct {
+    const float* tri_normals;     // float[ntris][9]
+    const uint8_t* mesh_normals;  // per mesh 1 = shade with tri_normals (null: every mesh)
+    const float* tri_uv;          // float[ntris][6]
+    const uint8_t* texels;        // RGBA8 texels of every texture, concatenated
+    const int64_t* tex_offset;    // per texture: first texel
+    const int32_t* tex_size;      // per texture: width, height
+} ref_surface;
+void ref_tex_fetch(const uint8_t* rgba, int w, int h, float u, float v, float* out3);
 void ref_halton_table(uint32_t* primes, uint32_t* start, uint16_t* perm, int perm_cap);
 float ref_halton(uint64_t index, int dim);
 void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, int nmeshes, float* area);
 int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
                     int nmeshes, const float* box, const int32_t* left, const int32_t* right,
-                    const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals, const float* cam16, float cam_depth, int width, int height,
+                    const int32_t* prim, const ref_surface* surf, const float* cam16, float cam_depth, int width, int height,
                     int spp, int max_depth, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
                       int nmeshes, const float* box, const int32_t* left, const int32_t* right,
-                      const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals, const float* cam16, float cam_depth, int width, int height,
+                      const int32_t* prim, const ref_surface* surf, const float* cam16, float cam_depth, int width, int height,
                       int spp, float* film_rgb, float* film_w, int64_t* rays_traced);
 int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                       const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                      const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
+                      const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
                       const float* scene6, int q_grid, float alpha_scale, const float* cam16,
                       float cam_depth, int width, int height, int spp, int max_depth,
                       int spp_per_round, float* film_rgb, float* film_w, int64_t* rays_traced);

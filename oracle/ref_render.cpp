@@ -376,7 +376,8 @@ struct Scene {
     std::vector<float> area;  // per mesh
     const float* box; const int32_t* left; const int32_t* right; const int32_t* prim;
     const float* normals = nullptr;         // float[ntris][9] per-vertex normals, or null
-    const uint8_t* mesh_normals = nullptr;  // per mesh: 1 = use them
+    const uint8_t* mesh_normals = nullptr;  // per mesh: 1 = use them (null: every mesh)
+    const ref_surface* surf = nullptr;      // texture coordinates and textures, or null
 };
 
 f3 tri_v(const Scene& s, int t, int j) {
@@ -397,6 +398,54 @@ Bsdf make_bsdf(const cav_mesh_ref& m) {
     float eta = m.material == kGlass ? m.ior : 1.5f;  // Materials/Plastic.h:111, Glass.h:238
     float t = (eta - 1) / (eta + 1);
     b.f0 = mk(t * t, t * t, t * t);
+    return b;
+}
+
+// Texture2D::readTexture (Core/Texture.h:92-95; set up at :101-125):
+// normalized coordinates wrapped to [0, 1) on both axes, bilinear filtering
+// between the four texels around x = u W - 0.5, y = v H - 0.5 (indices wrap
+// around the image), texels read as value / 255. The weights are the exact
+// fractions of x and y.
+float wrap_unit(float u) {
+    float w = u - std::floor(u);
+    if (!(w >= 0.f && w < 1.f)) w = 0.f;
+    return w;
+}
+f3 tex_fetch(const uint8_t* rgba, int W, int H, float u, float v) {
+    const float x = wrap_unit(u) * (float)W - 0.5f, y = wrap_unit(v) * (float)H - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    const int x0 = ((int)fx + W) % W, y0 = ((int)fy + H) % H;
+    const int x1 = (x0 + 1) % W, y1 = (y0 + 1) % H;
+    auto at = [&](int xx, int yy, int c) { return (float)rgba[((int64_t)yy * W + xx) * 4 + c] / 255.f; };
+    float r[3];
+    for (int c = 0; c < 3; ++c) {
+        const float lo = (1.f - a) * at(x0, y0, c) + a * at(x1, y0, c);
+        const float hi = (1.f - a) * at(x0, y1, c) + a * at(x1, y1, c);
+        r[c] = (1.f - b) * lo + b * hi;
+    }
+    return mk(r[0], r[1], r[2]);
+}
+
+// Material::getBSDF at a hit: the mesh's constants; a Kd texture read at
+// tex0 (1 - u - v) + u tex1 + v tex2 (Core/Triangle.h:128-131) multiplies a
+// matte colour (Materials/MatteMaterial.h:23-33) and replaces a plastic's
+// diffuse colour (Materials/Plastic.h:38-50).
+Bsdf surface_bsdf(const Scene& s, int mesh, int prim, float u, float v) {
+    const cav_mesh_ref& M = s.meshes[mesh];
+    Bsdf b = make_bsdf(M);
+    const ref_surface* sf = s.surf;
+    const int ti = M.texture;
+    if (!sf || !sf->texels || ti < 0 || (M.material != kMatte && M.material != kPlastic)) return b;
+    float tu = 0.f, tv = 0.f;
+    if (sf->tri_uv) {
+        const float* q = sf->tri_uv + (int64_t)prim * 6;
+        tu = (q[0] * (1.f - u - v) + u * q[2]) + v * q[4];
+        tv = (q[1] * (1.f - u - v) + u * q[3]) + v * q[5];
+    }
+    const f3 texel = tex_fetch(sf->texels + sf->tex_offset[ti] * 4, sf->tex_size[2 * ti], sf->tex_size[2 * ti + 1],
+                               tu, tv);
+    b.kd = M.material == kMatte ? mulv(b.kd, texel) : texel;
     return b;
 }
 
@@ -501,7 +550,7 @@ f3 tri_normal(const Scene& s, int p) {  // Core/Triangle.h:211-222 without verte
 // vertex normals, else the geometric normal; flipped for ReverseOrientation.
 f3 shading_normal(const Scene& s, int p, float u, float v) {
     const int m = s.tri_mesh[p];
-    if (!(s.normals && s.mesh_normals[m])) return tri_normal(s, p);
+    if (!(s.normals && (!s.mesh_normals || s.mesh_normals[m]))) return tri_normal(s, p);
     const float* q = s.normals + (int64_t)p * 9;
     f3 n0 = mk(q[0], q[1], q[2]), n1 = mk(q[3], q[4], q[5]), n2 = mk(q[6], q[7], q[8]);
     f3 n = normalize((n0 * (1.f - u - v) + u * n1) + v * n2);
@@ -584,11 +633,12 @@ void gather_film(int width, int height, int spp, const std::vector<float>& res, 
 // pixel order, then sample order.
 extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh,
                                const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                               const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
+                               const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
                                const float* cam16, float cam_depth, int width, int height,
                                int spp, int max_depth, float* film_rgb, float* film_w,
                                int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
+             surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -626,7 +676,7 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
             const cav_mesh_ref& M = meshes[mesh];
             f3 n = shading_normal(s, p, hu, hv);
             f3 pos = ro + t * rd;
-            Bsdf bsdf = make_bsdf(M);
+            Bsdf bsdf = surface_bsdf(s, mesh, p, hu, hv);
             f3 t0, t1;
             coordinate_system(n, t0, t1);
             // computeLighting (PathTracingIntegrator.cu:174-225)
@@ -712,10 +762,11 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
 // contribution is Material::eval * lightsCount / pdf (no MIS weight).
 extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh,
                                  const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                                 const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
                                  const float* cam16, float cam_depth, int width, int height, int spp,
                                  float* film_rgb, float* film_w, int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
+             surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -771,7 +822,7 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
             f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
             ++traced;
             if (visible(s, pos, dir, lim, mesh, lm)) {
-                Bsdf bsdf = make_bsdf(M);
+                Bsdf bsdf = surface_bsdf(s, mesh, p, hu, hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
                 f3 il = to_local(dir, t0, t1, n), ol = to_local(rd * -1, t0, t1, n);
@@ -945,12 +996,13 @@ extern "C" void ref_rl_finish_light(float* ldist, float* lres, int32_t* lcount, 
 // the learning rate n/(T+n) (1 = the reference's rule).
 extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tri_mesh,
                                  const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                                 const int32_t* left, const int32_t* right, const int32_t* prim, const float* tri_normals, const uint8_t* mesh_normals,
+                                 const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
                                  const float* scene6, int q_grid, float alpha_scale,
                                  const float* cam16, float cam_depth, int width, int height,
                                  int spp, int max_depth, int spp_per_round, float* film_rgb,
                                  float* film_w, int64_t* rays_traced) {
-    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim, tri_normals, mesh_normals};
+    Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
+             surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
     for (int m = 0; m < nmeshes; ++m)
         if (meshes[m].is_light) s.lights.push_back(m);
     s.area.resize(nmeshes);
@@ -1029,7 +1081,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 const cav_mesh_ref& M = meshes[mesh];
                 f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
-                Bsdf bsdf = make_bsdf(M);
+                Bsdf bsdf = surface_bsdf(s, mesh, h.prim, h.hu, h.hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
                 float* acc = &res[(size_t)t.sid * 3];
@@ -1092,7 +1144,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 const int mesh = tri_mesh[h.prim];
                 f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
-                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                Bsdf bsdf = surface_bsdf(s, mesh, h.prim, h.hu, h.hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
                 const int entry = find_entry(scene6, T.g, pos);
@@ -1144,7 +1196,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 Task& t = h.t;
                 const int mesh = tri_mesh[h.prim];
                 f3 n = shading_normal(s, h.prim, h.hu, h.hv);
-                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                Bsdf bsdf = surface_bsdf(s, mesh, h.prim, h.hu, h.hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
                 f3 il = to_local(lr[k].dir, t0, t1, n), ol = to_local(t.rd * -1.f, t0, t1, n);
@@ -1179,7 +1231,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 const int mesh = tri_mesh[h.prim];
                 f3 n = shading_normal(s, h.prim, h.hu, h.hv);
                 f3 pos = t.ro + h.dist * t.rd;
-                Bsdf bsdf = make_bsdf(meshes[mesh]);
+                Bsdf bsdf = surface_bsdf(s, mesh, h.prim, h.hu, h.hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
                 f3 wo = t.rd * -1.f;
@@ -1223,4 +1275,11 @@ extern "C" int ref_rl_dir_to_cell(const float* dir3) { return dir_to_cell(mk(dir
 extern "C" void ref_rl_cell_dir(float rx, float ry, int cell, float* out3) {
     f3 d = cell_dir(rx, ry, cell);
     out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+}
+
+extern "C" void ref_tex_fetch(const uint8_t* rgba, int w, int h, float u, float v, float* out3) {
+    const f3 r = tex_fetch(rgba, w, h, u, v);
+    out3[0] = r.x;
+    out3[1] = r.y;
+    out3[2] = r.z;
 }

@@ -74,6 +74,29 @@ def test_pbrt_vertex_normals_render_like_arrays(ctx, tmp_path, integrator):
     b.close()
 
 
+@pytest.mark.parametrize("integrator", ["path", "rlpath"])
+def test_pbrt_textures_render_like_arrays(ctx, tmp_path, integrator):
+    """A textured scene from a .pbrt file (PNG textures, "float uv", Kd
+    textures on matte and plastic) renders exactly like the array scene built
+    from the parsed meshes, texture coordinates and textures."""
+    print(f"[pbrt] textures {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=8, width=48, height=40, materials="mixed", textured=True)
+    p = str(tmp_path / "t.pbrt")
+    pbrt.write(sc, p, spp=4, integrator=integrator)
+    tris, meshes, st, _ = pbrt.parse(p)
+    uv, textures = pbrt.parse_textures(p)
+    arr = scenes.SceneArrays(tris, meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height, sc.max_depth,
+                             uv=uv, textures=textures)
+    b, _ = R.Scene.from_pbrt(ctx, p)
+    film_b, _ = R.render(ctx, b, st.spp, integrator=integrator, max_depth=1)
+    a = R.Scene(ctx, arr, bvh_rounds=st.bvh_opt_rounds, morton_bits=st.morton_bits)
+    film_a, _ = R.render(ctx, a, st.spp, integrator=integrator, camera=for_scene(sc), max_depth=1)
+    np.testing.assert_array_equal(film_b.weight, film_a.weight)
+    np.testing.assert_array_equal(film_b.rgb, film_a.rgb)
+    a.close()
+    b.close()
+
+
 def test_cli_renders_png(ctx, tmp_path):
     print("[pbrt] CLI", flush=True)
     sc = scenes.cornell_box(seed=4, width=40, height=32)

@@ -121,3 +121,37 @@ def test_vertex_normals_match_oracle(ctx, integrator):
     np.testing.assert_array_equal(film.rgb, rgb)
     assert st["closest_rays"] + st["shadow_rays"] == rays
     scene.close()
+
+
+@pytest.mark.parametrize("integrator", ["path", "directlighting", "rlpath"])
+def test_textures_match_oracle(ctx, integrator):
+    """Kd textures (checker floor, striped wall, marble plastic box, planar
+    and spherical UVs, bilinear wrap-addressed reads) together with vertex
+    normals: films equal the oracle's."""
+    print(f"[render] textures {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=10, width=48, height=40, materials="mixed", smooth=True, textured=True)
+    cam = for_scene(sc)
+    depth = 1 if integrator == "rlpath" else sc.max_depth
+    if integrator == "rlpath":
+        rgb, w, rays = oracle.render_rlpath(sc, 4, cam, max_depth=1)
+    else:
+        rgb, w, rays = {"path": oracle.render_path, "directlighting": oracle.render_direct}[integrator](sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()
+
+
+def test_texture_errors(ctx):
+    from cavalry_amd._lib import CavError
+    print("[render] texture errors", flush=True)
+    sc = scenes.cornell_box(seed=1, width=16, height=16, textured=True)
+    sc.meshes["texture"][0] = 7  # only 3 textures
+    with pytest.raises(CavError, match="texture"):
+        R.Scene(ctx, sc)
+    sc = scenes.cornell_box(seed=1, width=16, height=16, textured=True)
+    sc.textures[1] = np.zeros((0, 4, 4), np.uint8)
+    with pytest.raises(CavError, match="texture"):
+        R.Scene(ctx, sc)

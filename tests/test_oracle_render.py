@@ -136,3 +136,83 @@ def test_smooth_sphere_changes_its_shading_only():
     b, _, _ = oracle.render_direct(sm, 2, cam)
     changed = np.any(a != b, axis=-1)
     assert 0 < changed.mean() < 0.2  # the sphere covers a small part of the image
+
+
+def _bilinear64(t, u, v):
+    """Independent float64 statement of a wrap-addressed bilinear read."""
+    h, w = t.shape[:2]
+    x, y = (u % 1.0) * w - 0.5, (v % 1.0) * h - 0.5
+    x0, y0 = int(np.floor(x)), int(np.floor(y))
+    a, b = x - x0, y - y0
+    c = t[..., :3].astype(np.float64) / 255
+    at = lambda i, j: c[j % h, i % w]
+    return (1 - b) * ((1 - a) * at(x0, y0) + a * at(x0 + 1, y0)) + b * ((1 - a) * at(x0, y0 + 1) + a * at(x0 + 1, y0 + 1))
+
+
+def test_texture_fetch_known_answers():
+    """Texel centres return the texel exactly; half-way points average two
+    texels exactly; the image wraps around on both axes (u = 0 averages the
+    first and last columns) and in whole periods; random points agree with a
+    float64 statement of the filter."""
+    rng = np.random.default_rng(4)
+    t = rng.integers(0, 256, (4, 8, 4)).astype(np.uint8)
+    c = t[..., :3].astype(np.float32) / np.float32(255)
+    for j in range(4):
+        for i in range(8):
+            u, v = (i + 0.5) / 8, (j + 0.5) / 4
+            np.testing.assert_array_equal(oracle.tex_fetch(t, u, v), c[j, i])
+            np.testing.assert_array_equal(oracle.tex_fetch(t, u + 3, v - 2), c[j, i])
+    half = np.float32(0.5)
+    np.testing.assert_array_equal(oracle.tex_fetch(t, 3 / 8, 0.125), half * c[0, 2] + half * c[0, 3])
+    np.testing.assert_array_equal(oracle.tex_fetch(t, 0.0, 0.125), half * c[0, 7] + half * c[0, 0])
+    np.testing.assert_array_equal(oracle.tex_fetch(t, 1 / 16, 0.0), half * c[3, 0] + half * c[0, 0])
+    for _ in range(200):
+        u, v = rng.uniform(-3, 3, 2).astype(np.float32)
+        np.testing.assert_allclose(oracle.tex_fetch(t, u, v), _bilinear64(t, float(u), float(v)), atol=2e-6)
+    # non-finite coordinates read at 0
+    np.testing.assert_array_equal(oracle.tex_fetch(t, np.nan, np.inf), oracle.tex_fetch(t, 0.0, 0.0))
+
+
+def _with_texture(sc, mesh, tex):
+    meshes = sc.meshes.copy()
+    meshes["texture"][mesh] = 0
+    return scenes.SceneArrays(sc.tris, meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height,
+                              sc.max_depth, uv=scenes.planar_uv(sc.tris), textures=[tex])
+
+
+def test_constant_texture_equals_constant_colour():
+    """A constant Kd texture T multiplies a matte colour (Kd T / 255) and
+    replaces a plastic's diffuse colour (T / 255): the films equal those of
+    the untextured scenes with those colours; mirror and glass ignore it."""
+    sc = scenes.cornell_box(seed=2, width=24, height=20, materials="mixed")
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    T = np.array([200, 60, 120, 255], np.uint8)
+    tex = np.tile(T, (4, 4, 1))
+    col = T[:3].astype(np.float32) / np.float32(255)
+    for mesh in (0, 5, 6):   # matte floor, plastic tall box, mirror short box
+        textured = _with_texture(sc, mesh, tex)
+        meshes = sc.meshes.copy()
+        if meshes["material"][mesh] == scenes.MATTE:
+            meshes["kd"][mesh] = meshes["kd"][mesh] * col
+        elif meshes["material"][mesh] == scenes.PLASTIC:
+            meshes["kd"][mesh] = col
+        plain = scenes.SceneArrays(sc.tris, meshes, sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height,
+                                   sc.max_depth)
+        for fn in (oracle.render_path, oracle.render_direct):
+            a, wa, _ = fn(textured, 2, cam)
+            b, wb, _ = fn(plain, 2, cam)
+            np.testing.assert_array_equal(wa, wb)
+            np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+        a, _, _ = oracle.render_rlpath(textured, 2, cam, max_depth=2)
+        b, _, _ = oracle.render_rlpath(plain, 2, cam, max_depth=2)
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5)
+
+
+def test_textures_change_only_textured_surfaces():
+    sc = scenes.cornell_box(seed=2, width=32, height=32)
+    tx = scenes.cornell_box(seed=2, width=32, height=32, textured=True)
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    a, _, _ = oracle.render_direct(sc, 1, cam)
+    b, _, _ = oracle.render_direct(tx, 1, cam)
+    changed = np.any(a != b, axis=-1).mean()
+    assert 0.2 < changed < 0.9

@@ -305,3 +305,213 @@ def test_writer_round_trip_normals(tmp_path):
     on = sc.mesh_normals[np.repeat(np.arange(len(sc.meshes)), sc.meshes["num_tris"])].astype(bool)
     # the parser normalises the written unit normals again, which may move them by an ulp or two
     np.testing.assert_allclose(normals[on], sc.normals[on], atol=2.5e-7, rtol=0)
+
+
+# ------------------------------------------------------------------ textures
+def _png(path, img, ctype, depth=8, filters=(0,), palette=None, trns=None):
+    """PNG encoder after the specification: rows top first, per-row filter
+    types cycled from `filters`, one zlib stream."""
+    import zlib
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    h, w = img.shape[:2]
+    data = img.reshape(h, w * chans)
+    if depth == 16:
+        data = data.astype(">u2").view(np.uint8).reshape(h, -1)
+    bpp = chans * depth // 8
+    raw = b""
+    prev = np.zeros(data.shape[1], np.int64)
+    for y in range(h):
+        row = data[y].astype(np.int64)
+        ft = filters[y % len(filters)]
+        a = np.concatenate([np.zeros(bpp, np.int64), row[:-bpp]])
+        c = np.concatenate([np.zeros(bpp, np.int64), prev[:-bpp]])
+        if ft == 0:
+            out = row
+        elif ft == 1:
+            out = row - a
+        elif ft == 2:
+            out = row - prev
+        elif ft == 3:
+            out = row - (a + prev) // 2
+        else:
+            p = a + prev - c
+            pa, pb, pc = np.abs(p - a), np.abs(p - prev), np.abs(p - c)
+            pred = np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, prev, c))
+            out = row - pred
+        raw += bytes([ft]) + bytes((out % 256).astype(np.uint8))
+        prev = row
+
+    def chunk(t, b):
+        return struct.pack(">I", len(b)) + t + b + struct.pack(">I", zlib.crc32(t + b) & 0xffffffff)
+
+    body = chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if palette is not None:
+        body += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).ravel()))
+    if trns is not None:
+        body += chunk(b"tRNS", bytes(np.asarray(trns, np.uint8)))
+    body += chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b"")
+    path.write_bytes(b"\x89PNG\r\n\x1a\n" + body)
+
+
+def _inv_gamma8(v):
+    """(uint8)(255 * inverseGammaCorrect(v / 255)) in float64; the library
+    computes it in float, so it may differ by one."""
+    x = np.asarray(v, np.float64) / 255
+    return np.floor(255 * np.where(x <= 0.04045, x / 12.92, ((x + 0.055) / 1.055) ** 2.4)).astype(np.int64)
+
+
+def _tex_scene(tmp_path, tex_lines, kd='"texture Kd" "t0"', material="matte", uv=' "float uv" [ 0 0  1 0  1 1  0 1 ]'):
+    body = tex_lines + f'\nMaterial "{material}" {kd}\nShape "trianglemesh" {QUAD}{uv}\nWorldEnd\n'
+    return write(tmp_path, "WorldBegin\n" + body)
+
+
+def _check_texture(got, rgba_top_first):
+    """got: parsed texture (bottom-up, linearised rgb); expect flipped rows
+    and the inverse sRGB curve on rgb within one unit, alpha as is."""
+    want = rgba_top_first[::-1].astype(np.int64)
+    assert got.shape == want.shape
+    assert np.abs(got[..., :3].astype(np.int64) - _inv_gamma8(want[..., :3])).max() <= 1
+    np.testing.assert_array_equal(got[..., 3], want[..., 3])
+
+
+@pytest.mark.parametrize("ctype,depth", [(0, 8), (2, 8), (4, 8), (6, 8), (6, 16), (0, 16), (3, 8)])
+def test_png_textures(tmp_path, ctype, depth):
+    """Every colour type and all five row filters decode to the pixels
+    written, as RGBA8 with rows bottom-up and the sRGB curve removed."""
+    rng = np.random.default_rng(ctype * 100 + depth)
+    h, w = 7, 5
+    if ctype == 3:
+        pal = rng.integers(0, 256, (6, 3))
+        idx = rng.integers(0, 6, (h, w))
+        trns = [10, 250, 0]
+        _png(tmp_path / "t.png", idx.astype(np.uint8), 3, palette=pal, trns=trns, filters=(0, 1, 2, 3, 4))
+        a = np.where(idx < 3, np.array(trns + [255, 255, 255])[np.minimum(idx, 5)], 255)
+        rgba = np.concatenate([pal[idx], a[..., None]], 2)
+    else:
+        chans = {0: 1, 2: 3, 4: 2, 6: 4}[ctype]
+        img = rng.integers(0, 2 ** depth, (h, w, chans))
+        _png(tmp_path / "t.png", img.astype(np.uint16 if depth == 16 else np.uint8), ctype, depth,
+             filters=(0, 1, 2, 3, 4))
+        hi = img >> 8 if depth == 16 else img
+        g = hi[..., :1]
+        rgba = {0: lambda: np.concatenate([g, g, g, np.full_like(g, 255)], 2),
+                2: lambda: np.concatenate([hi, np.full_like(g, 255)], 2),
+                4: lambda: np.concatenate([g, g, g, hi[..., 1:2]], 2),
+                6: lambda: hi}[ctype]()
+    p = _tex_scene(tmp_path, 'Texture "t0" "spectrum" "imagemap" "string filename" [ "t.png" ]')
+    uv, textures = pbrt.parse_textures(p)
+    assert len(textures) == 1
+    _check_texture(textures[0], rgba)
+    _, m, _, _ = pbrt.parse(p)
+    assert m["texture"][0] == 0
+
+
+def _tga(path, img, rle=False, top_first=False, grey=False):
+    h, w = img.shape[:2]
+    bpp = 1 if grey else img.shape[2]
+    hdr = bytes([0, 0, (11 if rle else 3) if grey else (10 if rle else 2)]) + bytes(9) + \
+        struct.pack("<HHBB", w, h, 8 * bpp, 32 if top_first else 0)
+    rows = img if top_first else img[::-1]
+    px = rows.reshape(h * w, -1)
+    if not grey:
+        px = px[:, [2, 1, 0, 3][:bpp]]   # b, g, r (, a)
+    body = b""
+    if rle:  # pixels in runs: repeat packets for equal neighbours, else literal packets of one
+        k = 0
+        while k < len(px):
+            n = 1
+            while k + n < len(px) and n < 128 and np.array_equal(px[k + n], px[k]):
+                n += 1
+            body += bytes([(n - 1) | (128 if n > 1 else 0)]) + bytes(px[k].astype(np.uint8))
+            k += n
+    else:
+        body = bytes(px.astype(np.uint8).ravel())
+    path.write_bytes(hdr + body)
+
+
+@pytest.mark.parametrize("rle,top_first,chans", [(False, False, 3), (True, True, 4), (True, False, 1),
+                                                 (False, True, 4)])
+def test_tga_textures(tmp_path, rle, top_first, chans):
+    rng = np.random.default_rng(7)
+    img = rng.integers(0, 4, (6, 9, chans)).astype(np.uint8) * 60  # few values: runs for RLE
+    _tga(tmp_path / "t.tga", img, rle=rle, top_first=top_first, grey=chans == 1)
+    p = _tex_scene(tmp_path, 'Texture "t0" "spectrum" "imagemap" "string filename" "t.tga"')
+    _, textures = pbrt.parse_textures(p)
+    if chans == 1:
+        rgba = np.concatenate([img, img, img, np.full_like(img, 255)], 2)
+    elif chans == 3:
+        rgba = np.concatenate([img, np.full_like(img[..., :1], 255)], 2)
+    else:
+        rgba = img
+    _check_texture(textures[0], rgba)
+
+
+def test_texture_binding_and_uv(tmp_path):
+    """Kd textures bind by name to matte and plastic (the texel then decides
+    the colour), the texture list keeps definition order, a Texture without
+    a file is one transparent black texel, "float uv" and PLY u/v give the
+    per-vertex texture coordinates and meshes without any read (0, 0)."""
+    img = np.full((2, 2, 3), 128, np.uint8)
+    _png(tmp_path / "a.png", img, 2)
+    hdr = ("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+           "property float u\nproperty float v\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n")
+    (tmp_path / "uv.ply").write_text(hdr + "0 0 0 0.25 0.5\n1 0 0 0.75 0.5\n1 1 0 0.75 1\n3 0 1 2\n")
+    body = f"""WorldBegin
+Texture "empty" "spectrum" "imagemap"
+Texture "a" "spectrum" "imagemap" "string filename" "a.png"
+Material "plastic" "texture Kd" "a" "float roughness" [ 0.3 ]
+Shape "trianglemesh" {QUAD} "float uv" [ 0 0  2 0  2 2  0 2 ]
+Material "matte" "texture Kd" "empty"
+Shape "plymesh" "string filename" "uv.ply"
+Material "mirror"
+Shape "trianglemesh" {QUAD}
+WorldEnd
+"""
+    p = write(tmp_path, body)
+    _, m, _, _ = pbrt.parse(p)
+    uv, textures = pbrt.parse_textures(p)
+    assert list(m["texture"]) == [1, 0, -1]
+    assert list(m["material"]) == [PLASTIC, MATTE, MIRROR]
+    np.testing.assert_array_equal(textures[0], np.zeros((1, 1, 4), np.uint8))
+    assert textures[1].shape == (2, 2, 4)
+    np.testing.assert_array_equal(uv[0], [0, 0, 2, 0, 2, 2])   # quad triangles 0 1 2 and 0 2 3
+    np.testing.assert_array_equal(uv[1], [0, 0, 2, 2, 0, 2])
+    np.testing.assert_array_equal(uv[2], [0.25, 0.5, 0.75, 0.5, 0.75, 1])
+    np.testing.assert_array_equal(uv[3:], 0)
+    # a scene without textures reports none
+    plain = write(tmp_path, f'WorldBegin\nShape "trianglemesh" {QUAD}\nWorldEnd\n', name="p.pbrt")
+    assert pbrt.parse_textures(plain) == (None, [])
+
+
+@pytest.mark.parametrize("body,needle", [
+    ('Material "matte" "texture Kd" "nope"\n', "texture not found"),
+    ('Texture "t" "spectrum" "imagemap" "string filename" "x.exr"\n', "not supported"),
+    ('Texture "t" "spectrum" "imagemap" "string filename" "missing.png"\n', "cannot open"),
+    (f'Shape "trianglemesh" {QUAD} "float uv" [ 0 0 1 ]\n', "two numbers per vertex"),
+])
+def test_texture_errors(tmp_path, body, needle):
+    with pytest.raises(pbrt.ParseError, match=needle):
+        pbrt.parse(write(tmp_path, "WorldBegin\n" + body + "WorldEnd\n"))
+    (tmp_path / "bad.png").write_bytes(b"\x89PNG\r\n\x1a\nnot really")
+    with pytest.raises(pbrt.ParseError, match="PNG"):
+        pbrt.parse(write(tmp_path, 'WorldBegin\nTexture "t" "spectrum" "imagemap" "string filename" "bad.png"\n'
+                                   'WorldEnd\n', name="b.pbrt"))
+
+
+def test_writer_round_trip_textures(tmp_path):
+    sc = scenes.cornell_box(seed=5, width=40, height=30, materials="mixed", textured=True)
+    p = str(tmp_path / "t.pbrt")
+    pbrt.write(sc, p)
+    tris, m, _, _ = pbrt.parse(p)
+    uv, textures = pbrt.parse_textures(p)
+    np.testing.assert_array_equal(tris, sc.tris)
+    np.testing.assert_array_equal(uv, sc.uv)
+    np.testing.assert_array_equal(m["texture"], sc.meshes["texture"])
+    # a textured plastic's Kd is not written (the texel replaces it): the parser keeps its default
+    want = sc.meshes.copy()
+    want["kd"][(want["texture"] >= 0) & (want["material"] == PLASTIC)] = np.float32(0.25)
+    for a, b in zip(m, want):
+        relevant_equal(a, b)
+    assert len(textures) == len(sc.textures)
+    for a, b in zip(textures, sc.textures):
+        assert np.abs(a[..., :3].astype(int) - b[..., :3]).max() <= 2   # sRGB encode, then decode
