@@ -149,7 +149,8 @@ def render_bench(renders: int, cpu: bool) -> dict:
 
 def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
     """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
-    triangles): one seeded Poisson trace of render requests (resolution from
+    triangles, 1024^2 Kd textures on about a third of the meshes): one seeded
+    Poisson trace of render requests (resolution from
     256^2, 512^2, 1280x720, 1920x1080; spp from 1, 4, 16; path / rlpath
     50/50; max depth 8) replayed through the plain per-request renderer (one
     cav_render at a time, in arrival order) and through the server
@@ -196,7 +197,8 @@ def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
     p, s = plain.summary(), served.summary()
     return {
         "workload": f"{n_requests} requests, Poisson {rate}/s, 4 resident interior scenes "
-                    f"({', '.join(str(len(a.tris)) for a in arrays)} tris), res 256^2..1920x1080, spp 1/4/16, "
+                    f"({', '.join(str(len(a.tris)) for a in arrays)} tris, 4 1024^2 textures each), "
+                    "res 256^2..1920x1080, spp 1/4/16, "
                     "path/rlpath 50/50, max depth 8",
         "plain": p,
         "served": s,

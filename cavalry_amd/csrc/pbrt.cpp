@@ -14,15 +14,15 @@
 // the CLI overrides of main.cu:25-97.
 //
 // Differences, each reported through cav_render_setup::warnings and the
-// message list: the device BSDFs cover matte / plastic / mirror / specular
-// glass, so metal, substrate and rough glass are replaced (by matte with the
-// material's Kd, and by specular glass); texture files are PNG or TGA
+// message list: rough glass is rendered as specular glass; texture files are
+// PNG or TGA
 // (8-bit RGBA; the reference also reads EXR and, through its image loader,
 // other formats); non-area lights and the environment map are ignored; the
 // "random" sampler renders with Halton.
 // Unlike the reference, which exits, every error returns CAV_ERR_PARSE with a
 // message.
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -364,6 +364,20 @@ void kd_param(const Params& p, cav_mesh& m, Builder& bd, const std::string& what
     }
 }
 
+// u/v roughness of metal and substrate with GGX::createFromRoughness
+// (BSDFs/MicrofacetDistribution.h:147-154): 0.1 each unless given,
+// "roughness" sets both, remapped unless "remaproughness" is "false"
+void ggx_roughness(const Params& p, cav_mesh& m) {
+    float u = 0.1f, v = 0.1f;
+    bool remap = true;
+    if (p.has_num("uroughness")) u = p.num("uroughness");
+    if (p.has_num("vroughness")) v = p.num("vroughness");
+    if (p.has_num("roughness")) u = v = p.num("roughness");
+    if (p.has_string("remaproughness")) remap = p.str("remaproughness") == "true";
+    m.roughness = remap ? roughness_to_alpha(u) : u;
+    m.roughness_v = remap ? roughness_to_alpha(v) : v;
+}
+
 // MaterialObject::createFromObjectDefinition (Materials/MaterialObject.h:82-104)
 Material make_material(const Params& p, Builder& bd) {
     const std::string type = p.str("type");
@@ -398,9 +412,27 @@ Material make_material(const Params& p, Builder& bd) {
         if (p.has_num("vroughness")) vr = p.num("vroughness");
         if (p.has_num("roughness")) ur = vr = p.num("roughness");
         if (ur != 0.f || vr != 0.f) bd.warn("glass: rough glass is rendered as specular glass");
-    } else if (type == "metal" || type == "substrate") {
-        rgb_param(p, "Kd", m.kd, bd, type);
-        bd.warn(type + ": not supported by the device BSDFs; rendered as matte");
+    } else if (type == "metal") {  // Metal.h:36-64
+        m.material = CAV_MAT_METAL;
+        float f0[3] = {0.955f, 0.638f, 0.538f};  // copper
+        if (p.has_list("eta") && p.has_list("k")) {
+            const auto& eta = p.list("eta");
+            const auto& k = p.list("k");
+            if (eta.size() < 3 || k.size() < 3) throw ParseError("metal: eta and k need 3 values");
+            for (int c = 0; c < 3; ++c) {  // Fresnel::createFromIOR(eta, k): |((n - 1) / (n + 1))^2|
+                const std::complex<float> n(eta[c], k[c]), one(1.f, 0.f);
+                const std::complex<float> r = (n - one) / (n + one);
+                f0[c] = std::abs(r * r);
+            }
+        }
+        for (int c = 0; c < 3; ++c) m.ks[c] = f0[c];
+        ggx_roughness(p, m);
+    } else if (type == "substrate") {  // Substrate.h:50-83
+        m.material = CAV_MAT_SUBSTRATE;
+        for (int c = 0; c < 3; ++c) m.kd[c] = m.ks[c] = 0.5f;
+        kd_param(p, m, bd, "substrate");
+        rgb_param(p, "Ks", m.ks, bd, "substrate");
+        ggx_roughness(p, m);
     } else {
         bd.warn("unsupported material type: " + type + "; using matte");
         rgb_param(p, "Kd", m.kd, bd, type);

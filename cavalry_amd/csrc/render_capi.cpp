@@ -307,7 +307,7 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
         if (M.first_tri != next || M.num_tris < 1 || M.first_tri + M.num_tris > n)
             return fail(ctx, CAV_ERR_INVALID, "meshes must be contiguous triangle ranges covering the soup in order");
         if (M.material != CAV_MAT_MATTE && M.material != CAV_MAT_MIRROR && M.material != CAV_MAT_GLASS &&
-            M.material != CAV_MAT_PLASTIC)
+            M.material != CAV_MAT_PLASTIC && M.material != CAV_MAT_METAL && M.material != CAV_MAT_SUBSTRATE)
             return fail(ctx, CAV_ERR_INVALID, "unsupported material " + std::to_string(M.material));
         for (int t = M.first_tri; t < M.first_tri + M.num_tris; ++t) tri_mesh[t] = m;
         next += M.num_tris;

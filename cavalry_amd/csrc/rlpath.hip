@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         const float mp = bsdf.pdf(il, ol);
         float spdf = 0;
         if (sumw != 0) spdf = pdfq;
-        if (sumw == 0 || spdf == 0 || bsdf.delta()) spdf = mp;
+        if (sumw == 0 || spdf == 0 || bsdf.delta() || bsdf.almost_delta()) spdf = mp;
         const float w = power_heuristic(lpdf, spdf);
         const v3 add = scale3(mul3(contrib, mult), w);
         res[sid * 3] = res[sid * 3] + add.x;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
             entry = sl.entry[i];
             const int qc = sl.ncell[i];
             v3 f;
-            if (bsdf.delta() || qc < 0) {
+            if (bsdf.delta() || bsdf.almost_delta() || qc < 0) {
                 const float ux = halton(h, sid, dim), uy = halton(h, sid, dim + 1);
                 dim += 2;
                 v3 wl;

@@ -179,6 +179,11 @@ struct Bsdf {
     float kr, ior;
 
     __device__ bool delta() const { return type == CAV_MAT_MIRROR || type == CAV_MAT_GLASS; }
+    // MicrofacetDistribution.h:157-159 through MicrofacetBSDF / FresnelBlendBSDF;
+    // the other BSDFs keep the base class's false (Core/BSDF.h:25)
+    __device__ bool almost_delta() const {
+        return (type == CAV_MAT_METAL || type == CAV_MAT_SUBSTRATE) && ((double)ggx.ax < 0.05 || (double)ggx.ay < 0.05);
+    }
     __device__ v3 lam(v3 i, v3 o) const {
         if (!same_hemi(i, o)) return mk3(0, 0, 0);
         return divs3(kd, (float)kPi);
@@ -196,6 +201,24 @@ struct Bsdf {
     __device__ float mf_pdf(v3 i, v3 o) const {
         const v3 h = nrm(add3(i, o));
         return ggx.pdf(h, o) / (4.f * dot3(o, h));
+    }
+    // FresnelBlendBSDF (BSDFs/Fresnel.h:58-150): Ashikhmin-Shirley diffuse
+    // + GGX specular, sampled half / half
+    __device__ v3 fb_eval(v3 i, v3 o) const {
+        if (!same_hemi(i, o)) return mk3(0, 0, 0);
+        const float c = (float)(28.0 / (23.0 * kPi));
+        v3 d = mul3(scale3(kd, c), sub3(mk3(1, 1, 1), ks));
+        d = scale3(d, 1 - pow5(1 - .5f * fabsf(i.z)));
+        d = scale3(d, 1 - pow5(1 - .5f * fabsf(o.z)));
+        v3 h = add3(i, o);
+        if (h.x == 0 && h.y == 0 && h.z == 0) return mk3(0, 0, 0);
+        h = nrm(h);
+        const v3 s = divs3(scale3(schlick(dot3(i, h), ks), ggx.D(h)),
+                           4 * fabsf(dot3(i, h)) * fmx(fabsf(i.z), fabsf(o.z)));
+        return add3(d, s);
+    }
+    __device__ float fb_pdf(v3 i, v3 o, v3 h) const {
+        return (1.f - 0.5f) * cos_pdf(i.z) + 0.5f * ggx.pdf(h, o) / (4.f * dot3(o, h));
     }
     __device__ v3 srefl(v3 i, v3 o) const {
         if (!same_hemi(i, o)) return mk3(0, 0, 0);
@@ -220,6 +243,8 @@ struct Bsdf {
     __device__ v3 eval(v3 i, v3 o) const {
         if (type == CAV_MAT_MATTE) return lam(i, o);
         if (type == CAV_MAT_PLASTIC) return add3(lam(i, o), mf(i, o));
+        if (type == CAV_MAT_METAL) return mf(i, o);
+        if (type == CAV_MAT_SUBSTRATE) return fb_eval(i, o);
         if (type == CAV_MAT_MIRROR) {
             if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk3(0, 0, 0);
             return scale3(mk3(1, 1, 1), kr / fabsf(i.z));
@@ -235,6 +260,8 @@ struct Bsdf {
     __device__ float pdf(v3 i, v3 o) const {
         if (type == CAV_MAT_MATTE) return cos_pdf(i.z);
         if (type == CAV_MAT_PLASTIC) return 0.5f * cos_pdf(i.z) + (1.f - 0.5f) * mf_pdf(i, o);
+        if (type == CAV_MAT_METAL) return mf_pdf(i, o);
+        if (type == CAV_MAT_SUBSTRATE) return fb_pdf(i, o, nrm(add3(i, o)));
         if (type == CAV_MAT_MIRROR) return (i.x == -o.x && i.y == -o.y && i.z == o.z) ? 1.f : 0.f;
         if (same_hemi(i, o)) {
             if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return 0;
@@ -264,6 +291,29 @@ struct Bsdf {
             p = pdf(wi, o);
             return eval(wi, o);
         }
+        if (type == CAV_MAT_METAL) {  // MicrofacetBSDF::sampleReflection (BSDFs/Microfacet.h:96-105)
+            const v3 h = ggx.sample(ux, uy, o);
+            wi = reflect3(o, h);
+            p = ggx.pdf(h, o) / (4.f * dot3(o, h));
+            return mf(wi, o);
+        }
+        if (type == CAV_MAT_SUBSTRATE) {  // FresnelBlendBSDF::sample (BSDFs/Fresnel.h:101-128)
+            v3 h;
+            if (ux < 0.5f) {
+                h = ggx.sample(ux * (1.f / 0.5f), uy, o);
+                wi = reflect3(o, h);
+                if (!same_hemi(wi, o)) {  // the reference leaves the pdf unset; the value is zero
+                    p = 0;
+                    return mk3(0, 0, 0);
+                }
+            } else {
+                wi = cos_hemi((ux - 0.5f) * (1.f / (1.f - 0.5f)), uy);
+                if (o.z < 0) wi.z *= -1;
+                h = nrm(add3(wi, o));
+            }
+            p = fb_pdf(wi, o, h);
+            return fb_eval(wi, o);
+        }
         if (type == CAV_MAT_MIRROR) {
             wi = scale3(o, -1);
             wi.z = o.z;
@@ -292,12 +342,16 @@ __device__ inline Bsdf make_bsdf(const cav_mesh& m) {
     b.ks = mk3(m.ks[0], m.ks[1], m.ks[2]);
     b.refl = mk3(m.kr[0], m.kr[1], m.kr[2]);
     b.trans = mk3(m.kt[0], m.kt[1], m.kt[2]);
-    b.ggx = Ggx{m.roughness, m.roughness};
+    b.ggx = Ggx{m.roughness, m.roughness_v != 0.f ? m.roughness_v : m.roughness};
     b.kr = m.kr[0];
     b.ior = m.ior;
     const float eta = m.material == CAV_MAT_GLASS ? m.ior : 1.5f;  // Materials/Plastic.h:111, Glass.h:238
     const float t = (eta - 1) / (eta + 1);
     b.f0 = mk3(t * t, t * t, t * t);
+    if (m.material == CAV_MAT_METAL) {  // Materials/Metal.h:24-27: colour 1, the metal's F0
+        b.f0 = b.ks;
+        b.ks = mk3(1, 1, 1);
+    }
     return b;
 }
 
@@ -383,7 +437,8 @@ __device__ inline Bsdf surface_bsdf(const SceneDev& sc, int mesh, int prim, floa
     const cav_mesh& M = sc.meshes[mesh];
     Bsdf b = make_bsdf(M);
     const int ti = M.texture;
-    if (sc.texels && ti >= 0 && (M.material == CAV_MAT_MATTE || M.material == CAV_MAT_PLASTIC)) {
+    if (sc.texels && ti >= 0 &&
+        (M.material == CAV_MAT_MATTE || M.material == CAV_MAT_PLASTIC || M.material == CAV_MAT_SUBSTRATE)) {
         float tu = 0.f, tv = 0.f;
         if (sc.uv) {
             const float* q = sc.uv + (int64_t)prim * 6;

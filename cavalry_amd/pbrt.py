@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import Camera, check, lib
-from .scenes import GLASS, MATTE, MESH_DTYPE, MIRROR, PLASTIC, SceneArrays
+from .scenes import GLASS, MATTE, MESH_DTYPE, METAL, MIRROR, PLASTIC, SUBSTRATE, SceneArrays
 
 
 class Overrides(ctypes.Structure):
@@ -146,7 +146,8 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         out.append("AttributeBegin")
         mat = int(m["material"])
         tex = int(m["texture"])
-        kd = f'"texture Kd" "tex{tex}"' if tex >= 0 and mat in (MATTE, PLASTIC) else f'"rgb Kd" {_list(m["kd"])}'
+        kd = f'"texture Kd" "tex{tex}"' if tex >= 0 and mat in (MATTE, PLASTIC, SUBSTRATE) \
+            else f'"rgb Kd" {_list(m["kd"])}'
         if mat == MATTE:
             if tex >= 0 and np.any(m["kd"] != 1):
                 raise ValueError("a textured matte mesh needs kd = 1 to have a .pbrt form")
@@ -154,6 +155,17 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         elif mat == PLASTIC:
             out.append(f'  Material "plastic" {kd} "rgb Ks" {_list(m["ks"])} '
                        f'"float roughness" [ {_f(m["roughness"])} ] "bool remaproughness" "false"')
+        elif mat == SUBSTRATE:
+            rv = m["roughness_v"] if m["roughness_v"] != 0 else m["roughness"]
+            out.append(f'  Material "substrate" {kd} "rgb Ks" {_list(m["ks"])} '
+                       f'"float uroughness" [ {_f(m["roughness"])} ] "float vroughness" [ {_f(rv)} ] '
+                       '"bool remaproughness" "false"')
+        elif mat == METAL:
+            if not np.array_equal(m["ks"], np.float32([0.955, 0.638, 0.538])):
+                raise ValueError("only copper (the default F0) metal has an exact .pbrt form")
+            rv = m["roughness_v"] if m["roughness_v"] != 0 else m["roughness"]
+            out.append(f'  Material "metal" "float uroughness" [ {_f(m["roughness"])} ] '
+                       f'"float vroughness" [ {_f(rv)} ] "bool remaproughness" "false"')
         elif mat == MIRROR:
             out.append(f'  Material "mirror" "float Kr" [ {_f(m["kr"][0])} ]')
         elif mat == GLASS:

@@ -67,11 +67,11 @@ MESH_DTYPE = np.dtype([
     ("first_tri", "<i4"), ("num_tris", "<i4"), ("material", "<i4"), ("texture", "<i4"),
     ("kd", "<f4", 3), ("ks", "<f4", 3), ("kr", "<f4", 3), ("kt", "<f4", 3),
     ("roughness", "<f4"), ("ior", "<f4"), ("emission", "<f4", 3),
-    ("is_light", "<i4"), ("reverse_orientation", "<i4"),
+    ("is_light", "<i4"), ("reverse_orientation", "<i4"), ("roughness_v", "<f4"),
 ])
-assert MESH_DTYPE.itemsize == 92  # cav_mesh in include/cavalry.h
+assert MESH_DTYPE.itemsize == 96  # cav_mesh in include/cavalry.h
 
-MATTE, MIRROR, GLASS, PLASTIC = 0, 1, 4, 5
+MATTE, MIRROR, SUBSTRATE, METAL, GLASS, PLASTIC = 0, 1, 2, 3, 4, 5
 
 
 class SceneArrays:
@@ -109,17 +109,22 @@ class _Builder:
     def __init__(self):
         self.tris = []
         self.meshes = []
+        self.uv = []
 
     def mesh(self, tris, material=MATTE, kd=(0.5, 0.5, 0.5), ks=(0.25, 0.25, 0.25),
-             kr=(1, 1, 1), kt=(1, 1, 1), roughness=0.1, ior=1.5, emission=None):
+             kr=(1, 1, 1), kt=(1, 1, 1), roughness=0.1, ior=1.5, emission=None, texture=-1, uv=None,
+             roughness_v=0.0):
         tris = np.asarray(tris, np.float32).reshape(-1, 9)
+        if callable(uv):
+            uv = uv()
+        self.uv.append(np.zeros((len(tris), 6), np.float32) if uv is None else np.asarray(uv, np.float32))
         m = np.zeros((), MESH_DTYPE)
         m["first_tri"] = sum(len(t) for t in self.tris)
         m["num_tris"] = len(tris)
         m["material"] = material
-        m["texture"] = -1
+        m["texture"] = texture
         m["kd"], m["ks"], m["kr"], m["kt"] = kd, ks, kr, kt
-        m["roughness"], m["ior"] = roughness, ior
+        m["roughness"], m["ior"], m["roughness_v"] = roughness, ior, roughness_v
         if emission is not None:
             m["emission"] = emission
             m["is_light"] = 1
@@ -203,9 +208,11 @@ def spherical_uv(tris: np.ndarray, center) -> np.ndarray:
     return np.stack([u, v], 2).reshape(-1, 6).astype(np.float32)
 
 
+@functools.lru_cache(maxsize=16)
 def procedural_texture(kind: str, size: int, seed: int) -> np.ndarray:
     """RGBA8 [size, size, 4] pattern: "checker" (8 x 8 squares of two seeded
-    colours), "stripes" (diagonal bands) or "noise" (value-noise marble)."""
+    colours), "stripes" (diagonal bands) or "noise" (value-noise marble).
+    Cached: do not modify the returned array."""
     rng = _rng(seed, 13)
     c0 = rng.uniform(0.1, 0.9, 3)
     c1 = rng.uniform(0.1, 0.9, 3)
@@ -231,7 +238,9 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
     """Cornell-style box: unit box open at z=0 (5 walls x 2 quads), a tall and a
     short box, a 64x32 UV sphere (3968 triangles) and a 0.25x0.25 ceiling light
     of radiance 15 facing down. Albedos U[0.1,0.9] per mesh from the seed.
-    materials="mixed" makes the boxes plastic/mirror and the sphere glass.
+    materials="mixed" makes the boxes plastic/mirror and the sphere glass;
+    materials="metal" makes the tall box anisotropic substrate, the short box
+    copper (GGX alpha 0.2) and the sphere almost-mirror copper (alpha 0.04).
     smooth=True gives the sphere radial per-vertex normals (smooth shading).
     textured=True puts a checker on the floor, stripes on the back wall and
     value-noise marble on the tall box (64 x 64 textures, planar UVs; the
@@ -254,6 +263,11 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
         b.mesh(tall, material=PLASTIC, kd=albedo(), ks=(0.3, 0.3, 0.3), roughness=0.2)
         b.mesh(short, material=MIRROR, kr=(0.9, 0.9, 0.9))
         b.mesh(sph, material=GLASS, kr=(1, 1, 1), kt=(1, 1, 1), ior=1.5)
+    elif materials == "metal":
+        copper = (0.955, 0.638, 0.538)
+        b.mesh(tall, material=SUBSTRATE, kd=albedo(), ks=(0.2, 0.2, 0.2), roughness=0.3, roughness_v=0.1)
+        b.mesh(short, material=METAL, ks=copper, roughness=0.2)
+        b.mesh(sph, material=METAL, ks=copper, roughness=0.04)
     else:
         b.mesh(tall, kd=albedo())
         b.mesh(short, kd=albedo())
@@ -339,21 +353,42 @@ def _value_noise(p: np.ndarray, seed: int) -> np.ndarray:
     return out
 
 
-def interior(n_tris: int, seed: int = 1, width: int = 1920, height: int = 1080, max_depth: int = 16) -> SceneArrays:
+def interior(n_tris: int, seed: int = 1, width: int = 1920, height: int = 1080, max_depth: int = 16,
+             textures: bool = True, tex_size: int = 1024) -> SceneArrays:
     """Procedural interior (the wavefront-at-scale scene family): a 10 x 4 x 10
     room, 64 ceiling quad lights (two equal triangles each, L ~ U[5, 20]),
     displaced icospheres (level 6-8, radial value-noise displacement of
     amplitude U[0.02, 0.1] r) and boxes, about n_tris triangles in total.
     Materials per object: 40% matte, 30% plastic (alpha ~ U[0.05, 0.5]),
-    15% mirror, 15% glass (index 1.5). No textures."""
+    15% mirror, 15% glass (index 1.5). textures=True: four procedural RGBA8
+    textures of tex_size^2 (checker, stripes, two marbles) on the floor, the
+    far wall and half of the matte / plastic objects (spherical UVs on the
+    spheres, planar elsewhere; textured matte has kd = 1)."""
     rng = _rng(seed, 7)
+    trng = _rng(seed, 11)  # texture choices: the geometry and materials do not depend on them
     b = _Builder()
     X, Y, Z = 10.0, 4.0, 10.0
+    textures = [procedural_texture(k, tex_size, seed + i)
+                for i, k in enumerate(("checker", "stripes", "noise", "noise"))] if textures else []
+
+    def textured(tris, mat, uv):
+        if textures and mat["material"] in (MATTE, PLASTIC) and trng.uniform() < 0.5:
+            mat = dict(mat, texture=int(trng.integers(len(textures))), uv=uv)
+            if mat["material"] == MATTE:
+                mat["kd"] = (1.0, 1.0, 1.0)
+        return mat
+
+    def wall(tris, kd, tex):
+        kw = dict(kd=kd)
+        if textures and tex >= 0:
+            kw = dict(kd=(1.0, 1.0, 1.0), texture=tex, uv=planar_uv(np.asarray(tris, np.float32), 0.5))
+        b.mesh(tris, **kw)
+
     # the room's faces, normals pointing inward
-    b.mesh(split_quad((0, 0, 0), (0, 0, Z), (X, 0, 0), 4), kd=(0.6, 0.6, 0.6))        # floor, +y
+    wall(split_quad((0, 0, 0), (0, 0, Z), (X, 0, 0), 4), (0.6, 0.6, 0.6), 0)          # floor, +y
     b.mesh(split_quad((0, Y, 0), (X, 0, 0), (0, 0, Z), 4), kd=(0.7, 0.7, 0.7))        # ceiling, -y
     b.mesh(split_quad((0, 0, 0), (X, 0, 0), (0, Y, 0), 2), kd=(0.5, 0.3, 0.3))        # z = 0, +z
-    b.mesh(split_quad((0, 0, Z), (0, Y, 0), (X, 0, 0), 2), kd=(0.3, 0.5, 0.3))        # z = Z, -z
+    wall(split_quad((0, 0, Z), (0, Y, 0), (X, 0, 0), 2), (0.3, 0.5, 0.3), 1)          # z = Z, -z
     b.mesh(split_quad((0, 0, 0), (0, Y, 0), (0, 0, Z), 2), kd=(0.3, 0.3, 0.5))        # x = 0, +x
     b.mesh(split_quad((X, 0, 0), (0, 0, Z), (0, Y, 0), 2), kd=(0.5, 0.5, 0.3))        # x = X, -x
     for i in range(8):
@@ -385,15 +420,18 @@ def interior(n_tris: int, seed: int = 1, width: int = 1920, height: int = 1080, 
         d = 1.0 + amp / r * _value_noise(v * 3.0 + k * 17.0, seed + k)
         c = np.array([rng.uniform(1.5, X - 1.5), r + 0.05, rng.uniform(2.5, Z - 1.5)])
         pts = c + v * (r * d)[:, None]
-        b.mesh(pts[f].reshape(-1, 9), **material())
+        tris = pts[f].reshape(-1, 9)
+        b.mesh(tris, **textured(tris, material(), lambda: spherical_uv(tris, c)))
         budget -= len(f)
         k += 1
     for _ in range(max(4, budget // 12)):
         lo = np.array([rng.uniform(0.5, X - 1.5), 0.0, rng.uniform(1.5, Z - 1.5)])
         hi = lo + rng.uniform(0.2, 1.0, 3)
-        b.mesh(box_tris(lo, hi), **material())
+        tris = box_tris(lo, hi)
+        b.mesh(tris, **textured(tris, material(), lambda: planar_uv(np.asarray(tris, np.float32), 2.0)))
         if len(b.meshes) > 4000:
             break
     tris, meshes = b.arrays()
+    uv = np.concatenate(b.uv) if textures else None
     return SceneArrays(tris, meshes, (X / 2, 2.0, 0.3), (X / 2, 1.2, Z / 2), (0, 1, 0), 60.0, width, height,
-                       max_depth=max_depth)
+                       max_depth=max_depth, uv=uv, textures=textures)

@@ -119,7 +119,8 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays
                              const float* limit_dev, int64_t nrays, uint8_t* visible_dev);
 
 /* ---- scenes and rendering ---------------------------------------------- */
-enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_GLASS = 4, CAV_MAT_PLASTIC = 5 };
+enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_SUBSTRATE = 2, CAV_MAT_METAL = 3, CAV_MAT_GLASS = 4,
+       CAV_MAT_PLASTIC = 5 };
 /* path (Integrators/PathTracingIntegrator.cu), rlpath
  * (Integrators/ReinforcementLearningPathTracing.cu), directlighting
  * (Integrators/DirectLightingIntegrator.cu: one bounce, NEE without MIS) */
@@ -132,9 +133,12 @@ enum { CAV_RENDER_SORT_BY_MATERIAL = 1 };
 
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),
- * 1 mirror (kr[0]), 4 glass (kr, kt, ior; perfectly specular), 5 plastic
- * (kd, ks, roughness = GGX alpha). texture: index into the scene's textures
- * of a Kd texture (matte and plastic), or -1 for none. is_light marks a
+ * 1 mirror (kr[0]), 2 substrate (kd, ks, GGX alphas roughness and
+ * roughness_v; FresnelBlendBSDF), 3 metal (ks = Schlick F0, GGX alphas;
+ * MicrofacetBSDF of colour 1), 4 glass (kr, kt, ior; perfectly specular),
+ * 5 plastic (kd, ks, roughness = GGX alpha). roughness_v = 0 means the same
+ * alpha on both axes. texture: index into the scene's textures of a Kd
+ * texture (matte, plastic, substrate), or -1 for none. is_light marks a
  * DiffuseAreaLight of radiance `emission` (Lights/DiffuseAreaLight.h). */
 typedef struct {
     int32_t first_tri, num_tris;
@@ -144,6 +148,7 @@ typedef struct {
     float roughness, ior;
     float emission[3];
     int32_t is_light, reverse_orientation;
+    float roughness_v;
 } cav_mesh;
 
 /* Perspective camera: column-major camera-to-world matrix (glm layout), film

@@ -288,3 +288,26 @@ def rl_finish_light(ldist, lres, lcount, lnew, lnewc) -> None:
         assert a.dtype == dt and a.flags.c_contiguous and a.shape == ldist.shape
     lib().ref_rl_finish_light(_p(ldist), _p(lres), _p(lcount), _p(lnew), _p(lnewc),
                               ctypes.c_int(ldist.shape[0]), ctypes.c_int(ldist.shape[1]))
+
+
+def bsdf_eval(mesh, wi, wo):
+    """f(wi, wo) [n, 3] and pdf [n] of one mesh's BSDF in its local frame."""
+    m = np.ascontiguousarray(mesh).copy()
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), wi.shape))
+    f = np.zeros_like(wi)
+    p = np.zeros(len(wi), np.float32)
+    lib().ref_bsdf_eval(_p(m), _p(wi), _p(wo), ctypes.c_int(len(wi)), _p(f), _p(p))
+    return f, p
+
+
+def bsdf_sample(mesh, u, wo):
+    """BSDF::sample for uniforms u [n, 2] and exitant directions wo: (wi, f, pdf)."""
+    m = np.ascontiguousarray(mesh).copy()
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), (len(u), 3)))
+    wi = np.zeros((len(u), 3), np.float32)
+    f = np.zeros((len(u), 3), np.float32)
+    p = np.zeros(len(u), np.float32)
+    lib().ref_bsdf_sample(_p(m), _p(u), _p(wo), ctypes.c_int(len(u)), _p(wi), _p(f), _p(p))
+    return wi, f, p

@@ -8,12 +8,13 @@
 // soup belong to it. Same layout as cav_mesh in include/cavalry.h.
 typedef struct {
     int32_t first_tri, num_tris;
-    int32_t material;  // 0 matte, 1 mirror, 4 glass, 5 plastic (Core/Material.h:10-12)
+    int32_t material;  // 0 matte, 1 mirror, 2 substrate, 3 metal, 4 glass, 5 plastic (Core/Material.h:10-12)
     int32_t texture;   // -1: none
     float kd[3], ks[3], kr[3], kt[3];
     float roughness, ior;
     float emission[3];
     int32_t is_light, reverse_orientation;
+    float roughness_v;  // 0: same as roughness
 } cav_mesh_ref;
 
 extern "C" {
@@ -46,6 +47,9 @@ typedef struct {
     const int32_t* tex_size;      // per texture: width, height
 } ref_surface;
 void ref_tex_fetch(const uint8_t* rgba, int w, int h, float u, float v, float* out3);
+void ref_bsdf_eval(const cav_mesh_ref* m, const float* wi, const float* wo, int n, float* f_out, float* pdf_out);
+void ref_bsdf_sample(const cav_mesh_ref* m, const float* u2, const float* wo, int n, float* wi_out, float* f_out,
+                     float* pdf_out);
 void ref_halton_table(uint32_t* primes, uint32_t* start, uint16_t* perm, int perm_cap);
 float ref_halton(uint64_t index, int dim);
 void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, int nmeshes, float* area);

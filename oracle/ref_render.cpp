@@ -223,19 +223,24 @@ f3 schlick(float c, f3 f0) {  // BSDFs/Fresnel.h:12-17
     return f0 + (mk(1, 1, 1) - f0) * t;
 }
 
-enum { kMatte = 0, kMirror = 1, kGlass = 4, kPlastic = 5 };
+enum { kMatte = 0, kMirror = 1, kSubstrate = 2, kMetal = 3, kGlass = 4, kPlastic = 5 };
 
 struct Bsdf {
     int type;
     f3 kd;            // matte / plastic diffuse
-    f3 ks;            // plastic specular colour
-    GGX ggx;          // plastic
-    f3 f0;            // Fresnel (plastic: from IOR 1.5; glass: from its IOR)
+    f3 ks;            // plastic / substrate specular colour (metal: 1)
+    GGX ggx;          // plastic, metal, substrate
+    f3 f0;            // Fresnel (plastic: from IOR 1.5; glass: from its IOR; metal: its F0)
     float kr;         // mirror reflectivity
     f3 refl, trans;   // glass
     float ior;        // glass below IOR (above = 1)
 
     bool is_delta() const { return type == kMirror || type == kGlass; }
+    // GGX::isAlmostDelta (BSDFs/MicrofacetDistribution.h:157-159) for the
+    // BSDFs that forward it (Microfacet.h:212, Fresnel.h:146); false otherwise
+    bool is_almost_delta() const {
+        return (type == kMetal || type == kSubstrate) && ((double)ggx.ax < 0.05 || (double)ggx.ay < 0.05);
+    }
 
     // LambertianBSDF (BSDFs/Lambertian.h)
     f3 lam_eval(f3 i, f3 o) const {
@@ -255,11 +260,29 @@ struct Bsdf {
     }
     float mf_pdf(f3 i, f3 o, f3 h) const { return ggx.pdf(h, o) / (4.f * dot(o, h)); }
     float mf_pdf(f3 i, f3 o) const { return mf_pdf(i, o, normalize(i + o)); }
+    // FresnelBlendBSDF (BSDFs/Fresnel.h:58-150)
+    f3 fb_eval(f3 i, f3 o) const {
+        if (!same_hemi(i, o)) return mk(0, 0, 0);
+        const float c = (float)(28.0 / (23.0 * kPi));
+        f3 diffuse = mulv(kd * c, mk(1, 1, 1) - ks) * (1 - pow5(1 - .5f * std::fabs(i.z))) *
+                     (1 - pow5(1 - .5f * std::fabs(o.z)));
+        f3 h = i + o;
+        if (h.x == 0 && h.y == 0 && h.z == 0) return mk(0, 0, 0);
+        h = normalize(h);
+        f3 spec = divs(schlick(dot(i, h), ks) * ggx.D(h),
+                       4 * std::fabs(dot(i, h)) * fmaxr(std::fabs(i.z), std::fabs(o.z)));
+        return diffuse + spec;
+    }
+    float fb_pdf(f3 i, f3 o, f3 h) const {
+        return (1.f - 0.5f) * cos_pdf(i.z) + 0.5f * ggx.pdf(h, o) / (4.f * dot(o, h));
+    }
 
     f3 eval(f3 i, f3 o) const {
         switch (type) {
             case kMatte: return lam_eval(i, o);
             case kPlastic: return lam_eval(i, o) + mf_eval(i, o);
+            case kMetal: return mf_eval(i, o);
+            case kSubstrate: return fb_eval(i, o);
             case kMirror:  // BSDFs/MirrorBSDF.h:18-24
                 if (!(i.x == -o.x && i.y == -o.y && i.z == o.z)) return mk(0, 0, 0);
                 return mk(1, 1, 1) * kr / std::fabs(i.z);
@@ -302,6 +325,8 @@ struct Bsdf {
         switch (type) {
             case kMatte: return cos_pdf(i.z);
             case kPlastic: return 0.5f * cos_pdf(i.z) + (1.f - 0.5f) * mf_pdf(i, o);
+            case kMetal: return mf_pdf(i, o);
+            case kSubstrate: return fb_pdf(i, o, normalize(i + o));
             case kMirror:
                 if (i.x == -o.x && i.y == -o.y && i.z == o.z) return 1;
                 return 0;
@@ -339,6 +364,29 @@ struct Bsdf {
                 }
                 pdfv = pdf(wi, o);
                 return eval(wi, o);
+            }
+            case kMetal: {  // MicrofacetBSDF::sampleReflection (BSDFs/Microfacet.h:96-105)
+                f3 h = ggx.sample(ux, uy, o);
+                wi = reflect(o, h);
+                pdfv = mf_pdf(wi, o, h);
+                return mf_eval(wi, o);
+            }
+            case kSubstrate: {  // FresnelBlendBSDF::sample (BSDFs/Fresnel.h:101-128)
+                f3 h;
+                if (ux < 0.5f) {
+                    h = ggx.sample(ux * (1.f / 0.5f), uy, o);
+                    wi = reflect(o, h);
+                    if (!same_hemi(wi, o)) {  // unset pdf in the reference; the value is zero
+                        pdfv = 0;
+                        return mk(0, 0, 0);
+                    }
+                } else {
+                    wi = cosine_hemisphere((ux - 0.5f) * (1.f / (1.f - 0.5f)), uy);
+                    if (o.z < 0) wi.z *= -1;
+                    h = normalize(wi + o);
+                }
+                pdfv = fb_pdf(wi, o, h);
+                return fb_eval(wi, o);
             }
             case kMirror: {
                 wi = o * -1;
@@ -390,7 +438,7 @@ Bsdf make_bsdf(const cav_mesh_ref& m) {
     b.type = m.material;
     b.kd = mk(m.kd[0], m.kd[1], m.kd[2]);
     b.ks = mk(m.ks[0], m.ks[1], m.ks[2]);
-    b.ggx = GGX{m.roughness, m.roughness};
+    b.ggx = GGX{m.roughness, m.roughness_v != 0.f ? m.roughness_v : m.roughness};
     b.kr = m.kr[0];
     b.refl = mk(m.kr[0], m.kr[1], m.kr[2]);
     b.trans = mk(m.kt[0], m.kt[1], m.kt[2]);
@@ -398,6 +446,10 @@ Bsdf make_bsdf(const cav_mesh_ref& m) {
     float eta = m.material == kGlass ? m.ior : 1.5f;  // Materials/Plastic.h:111, Glass.h:238
     float t = (eta - 1) / (eta + 1);
     b.f0 = mk(t * t, t * t, t * t);
+    if (m.material == kMetal) {  // Materials/Metal.h:24-27
+        b.f0 = b.ks;
+        b.ks = mk(1, 1, 1);
+    }
     return b;
 }
 
@@ -436,7 +488,8 @@ Bsdf surface_bsdf(const Scene& s, int mesh, int prim, float u, float v) {
     Bsdf b = make_bsdf(M);
     const ref_surface* sf = s.surf;
     const int ti = M.texture;
-    if (!sf || !sf->texels || ti < 0 || (M.material != kMatte && M.material != kPlastic)) return b;
+    if (!sf || !sf->texels || ti < 0 || (M.material != kMatte && M.material != kPlastic && M.material != kSubstrate))
+        return b;
     float tu = 0.f, tv = 0.f;
     if (sf->tri_uv) {
         const float* q = sf->tri_uv + (int64_t)prim * 6;
@@ -1203,7 +1256,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 float mp = bsdf.pdf(il, ol);
                 float sp = 0;
                 if (qr[k].sum != 0) sp = qr[k].pdfq;
-                if (qr[k].sum == 0 || sp == 0 || bsdf.is_delta()) sp = mp;
+                if (qr[k].sum == 0 || sp == 0 || bsdf.is_delta() || bsdf.is_almost_delta()) sp = mp;
                 float w = power_heuristic(lr[k].lpdf, sp);
                 f3 a = mulv(lr[k].contrib, t.mult) * w;
                 float* acc = &res[(size_t)t.sid * 3];
@@ -1239,7 +1292,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 f3 f, nd;
                 float npdf;
                 int cell;
-                if (bsdf.is_delta() || !ni[k].valid) {
+                if (bsdf.is_delta() || bsdf.is_almost_delta() || !ni[k].valid) {
                     float ux = H.rand1(t.sid, t.dim), uy = H.rand1(t.sid, t.dim);
                     f3 wl;
                     npdf = 0;
@@ -1282,4 +1335,38 @@ extern "C" void ref_tex_fetch(const uint8_t* rgba, int w, int h, float u, float 
     out3[0] = r.x;
     out3[1] = r.y;
     out3[2] = r.z;
+}
+
+// BSDF of one mesh in its local frame (z = shading normal), for unit checks:
+// f(wi, wo) and pdf(wi, wo) of n direction pairs.
+extern "C" void ref_bsdf_eval(const cav_mesh_ref* m, const float* wi, const float* wo, int n, float* f_out,
+                              float* pdf_out) {
+    const Bsdf b = make_bsdf(*m);
+    for (int k = 0; k < n; ++k) {
+        const f3 i = mk(wi[3 * k], wi[3 * k + 1], wi[3 * k + 2]), o = mk(wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]);
+        const f3 f = b.eval(i, o);
+        f_out[3 * k] = f.x;
+        f_out[3 * k + 1] = f.y;
+        f_out[3 * k + 2] = f.z;
+        pdf_out[k] = b.pdf(i, o);
+    }
+}
+
+// BSDF::sample for n (u, wo) pairs: sampled wi, f and pdf.
+extern "C" void ref_bsdf_sample(const cav_mesh_ref* m, const float* u2, const float* wo, int n, float* wi_out,
+                                float* f_out, float* pdf_out) {
+    const Bsdf b = make_bsdf(*m);
+    for (int k = 0; k < n; ++k) {
+        const f3 o = mk(wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]);
+        f3 i = mk(0, 0, 0);
+        float p = 0;
+        const f3 f = b.sample(u2[2 * k], u2[2 * k + 1], i, o, p);
+        wi_out[3 * k] = i.x;
+        wi_out[3 * k + 1] = i.y;
+        wi_out[3 * k + 2] = i.z;
+        f_out[3 * k] = f.x;
+        f_out[3 * k + 1] = f.y;
+        f_out[3 * k + 2] = f.z;
+        pdf_out[k] = p;
+    }
 }

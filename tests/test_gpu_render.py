@@ -144,6 +144,27 @@ def test_textures_match_oracle(ctx, integrator):
     scene.close()
 
 
+@pytest.mark.parametrize("integrator", ["path", "directlighting", "rlpath"])
+def test_metal_substrate_match_oracle(ctx, integrator):
+    """Metal (copper, and almost-delta alpha 0.04 that rlpath samples by the
+    BSDF) and anisotropic substrate with a Kd texture: films equal the
+    oracle's."""
+    print(f"[render] metal/substrate {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=11, width=48, height=40, materials="metal", smooth=True, textured=True)
+    cam = for_scene(sc)
+    depth = 1 if integrator == "rlpath" else sc.max_depth
+    if integrator == "rlpath":
+        rgb, w, rays = oracle.render_rlpath(sc, 4, cam, max_depth=1)
+    else:
+        rgb, w, rays = {"path": oracle.render_path, "directlighting": oracle.render_direct}[integrator](sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()
+
+
 def test_texture_errors(ctx):
     from cavalry_amd._lib import CavError
     print("[render] texture errors", flush=True)

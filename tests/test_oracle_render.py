@@ -108,6 +108,19 @@ def test_interior_generator():
         t = sc.tris[m["first_tri"]].reshape(3, 3).astype(np.float64)
         assert np.dot(np.cross(t[1] - t[0], t[2] - t[0]), centre - t.mean(0)) > 0
     assert set(np.unique(sc.meshes["material"])) == {scenes.MATTE, scenes.PLASTIC, scenes.MIRROR, scenes.GLASS}
+    # textures on the floor, the far wall and some matte / plastic objects only
+    tex = sc.meshes["texture"]
+    assert len(sc.textures) == 4 and all(t.shape == (1024, 1024, 4) for t in sc.textures)
+    assert tex[0] == 0 and tex[3] == 1 and tex.max() < 4
+    assert np.isin(sc.meshes["material"][tex >= 0], [scenes.MATTE, scenes.PLASTIC]).all()
+    assert 0.2 < (tex[70:] >= 0).mean() < 0.5
+    assert (sc.meshes["kd"][(tex >= 0) & (sc.meshes["material"] == scenes.MATTE)] == 1).all()
+    assert sc.uv.shape == (len(sc.tris), 6) and np.isfinite(sc.uv).all()
+    # the texture draws leave the geometry and materials alone
+    plain = scenes.interior(120_000, seed=3, width=32, height=18, textures=False)
+    np.testing.assert_array_equal(plain.tris, sc.tris)
+    np.testing.assert_array_equal(plain.meshes["material"], sc.meshes["material"])
+    assert plain.uv is None and plain.textures == []
 
 
 def test_vertex_normals_equal_to_geometric_change_nothing():

@@ -16,7 +16,7 @@ import pytest
 
 from cavalry_amd import pbrt, scenes
 from cavalry_amd.camera import for_scene
-from cavalry_amd.scenes import GLASS, MATTE, MIRROR, PLASTIC
+from cavalry_amd.scenes import GLASS, MATTE, METAL, MIRROR, PLASTIC, SUBSTRATE
 
 HEAD = """LookAt 0 0 -5  0 0 0  0 1 0
 Camera "perspective" "float fov" [ 45 ]
@@ -46,7 +46,8 @@ def relevant_equal(a, b):
     if a["is_light"]:
         np.testing.assert_array_equal(a["emission"], b["emission"])
     mat = int(a["material"])
-    fields = {MATTE: ["kd"], PLASTIC: ["kd", "ks", "roughness"], MIRROR: [], GLASS: ["kr", "kt", "ior"]}[mat]
+    fields = {MATTE: ["kd"], PLASTIC: ["kd", "ks", "roughness"], MIRROR: [], GLASS: ["kr", "kt", "ior"],
+              SUBSTRATE: ["kd", "ks", "roughness"], METAL: ["ks", "roughness"]}[mat]
     for f in fields:
         np.testing.assert_array_equal(a[f], b[f], err_msg=f)
     if mat == MIRROR:
@@ -168,16 +169,32 @@ Material "glass"
 Shape "trianglemesh" {QUAD}
 Material "metal"
 Shape "trianglemesh" {QUAD}
+Material "substrate"
+Shape "trianglemesh" {QUAD}
+Material "metal" "rgb eta" [ 0.2 0.9 1.1 ] "rgb k" [ 3.9 2.4 2.2 ] "float uroughness" [ 0.2 ] "float vroughness" [ 0.05 ] "bool remaproughness" "false"
+Shape "trianglemesh" {QUAD}
+Material "rubber"
+Shape "trianglemesh" {QUAD}
 WorldEnd
 """
     _, m, st, msgs = pbrt.parse(write(tmp_path, body))
-    assert list(m["material"]) == [MATTE, MIRROR, PLASTIC, GLASS, MATTE]
+    assert list(m["material"]) == [MATTE, MIRROR, PLASTIC, GLASS, METAL, SUBSTRATE, METAL, MATTE]
     assert list(m["kd"][0]) == [1, 1, 1]                 # default matte (Parsing.cu:290-292)
     assert m["kr"][1][0] == np.float32(0.9)               # MirrorMaterial()
     assert list(m["kd"][2]) == [np.float32(0.25)] * 3     # plastic defaults
     assert list(m["ks"][2]) == [np.float32(0.25)] * 3
     assert m["ior"][3] == np.float32(1.5)
-    assert st.warnings == 1 and "metal" in msgs[0]
+    # metal: copper F0 by default, GGX alpha of roughness 0.1 on both axes (Metal.h:36-64)
+    a01 = m["roughness"][2]  # plastic's default roughness is also 0.1, remapped
+    np.testing.assert_array_equal(m["ks"][4], np.float32([0.955, 0.638, 0.538]))
+    assert m["roughness"][4] == a01 and m["roughness_v"][4] == a01
+    # substrate: Kd = Ks = 0.5 (Substrate.h:50-83)
+    assert list(m["kd"][5]) == [0.5] * 3 and list(m["ks"][5]) == [0.5] * 3 and m["roughness_v"][5] == a01
+    # F0 of a complex IOR: |((eta + ik - 1) / (eta + ik + 1))^2|
+    n = np.array([0.2, 0.9, 1.1]) + 1j * np.array([3.9, 2.4, 2.2])
+    np.testing.assert_allclose(m["ks"][6], np.abs(((n - 1) / (n + 1)) ** 2), rtol=1e-6)
+    assert m["roughness"][6] == np.float32(0.2) and m["roughness_v"][6] == np.float32(0.05)
+    assert st.warnings == 1 and "rubber" in msgs[0]
 
 
 def test_include_and_lexer(tmp_path):
