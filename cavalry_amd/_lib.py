@@ -60,7 +60,9 @@ class SceneDesc(ctypes.Structure):
     _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int32), ("meshes", ctypes.c_void_p),
                 ("nmeshes", ctypes.c_int32), ("tri_uv", ctypes.c_void_p), ("tri_normals", ctypes.c_void_p),
                 ("mesh_normals", ctypes.c_void_p), ("bvh_opt_rounds", ctypes.c_int32),
-                ("morton_bits", ctypes.c_int32), ("textures", ctypes.c_void_p), ("ntextures", ctypes.c_int32)]
+                ("morton_bits", ctypes.c_int32), ("textures", ctypes.c_void_p), ("ntextures", ctypes.c_int32),
+                ("env_lights", ctypes.c_void_p), ("nenv", ctypes.c_int32), ("light_order", ctypes.c_void_p),
+                ("nlight_order", ctypes.c_int32), ("env_background", ctypes.c_int32)]
 
 
 class ServeConfig(ctypes.Structure):

@@ -9,6 +9,7 @@ float32[N, 9] soup and rays as float32[R, 6].
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -27,7 +28,9 @@ def _ptr(a):
 
 
 class Context:
-    """One device, one HIP stream (cav_ctx)."""
+    """One device, one HIP stream (cav_ctx). Objects made on it (BVHs,
+    scenes, servers) are closed before the context itself: the library
+    requires them to be destroyed first."""
 
     def __init__(self, device: int = 0):
         h = ctypes.c_void_p()
@@ -35,6 +38,11 @@ class Context:
         if rc != 0:
             raise _lib.CavError(rc, f"cav_ctx_create(device={device}) failed")
         self.handle = h
+        self._children = weakref.WeakSet()
+
+    def own(self, obj) -> None:
+        """Registers an object whose close() must run before this context's."""
+        self._children.add(obj)
 
     @property
     def stream(self) -> int:
@@ -45,6 +53,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             lib().cav_ctx_destroy(self.handle)
             self.handle = None
 
@@ -62,6 +72,7 @@ class BVH:
         self.ctx = ctx
         self.handle = handle
         self.n = n
+        ctx.own(self)
 
     @staticmethod
     def build(ctx: Context, triangles, scene_bounds=None, optimization_rounds: int = 0,

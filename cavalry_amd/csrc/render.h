@@ -21,6 +21,13 @@ struct TexDev {
     int32_t w, h;
 };
 
+// an environment-type light (cav_env_light)
+struct EnvDev {
+    float color[3];
+    int32_t texture;
+    float m[9];  // world -> light, row-major
+};
+
 struct SceneDev {
     const TNode* tn;
     const TTri* tt;
@@ -30,13 +37,21 @@ struct SceneDev {
     const int32_t* tri_mesh;   // ntris
     const cav_mesh* meshes;
     const float* mesh_area;
-    const int32_t* lights;     // mesh ids of area lights
+    const int32_t* lights;     // the light list: area-light mesh id >= 0, or -1 - k for env[k]
     int nmeshes, nlights;
     const float* normals;        // float[ntris][9] per-vertex normals, or null
     const uint8_t* mesh_normals; // per mesh: 1 = shade with the interpolated normals
     const float* uv;             // float[ntris][6] per-vertex texture coordinates, or null (all zero)
     const uint32_t* texels;      // RGBA8 texels of every texture, row 0 = v 0; null when no mesh has one
     const TexDev* tex;           // indexed by cav_mesh::texture
+    const EnvDev* env;           // environment-type lights, or null
+    int env_miss;                // env light that rays which miss see, or -1
+    // Halton dimensions a light sample takes after randInt + rand4: 3 (a
+    // triangle and a point on it) when the scene has area lights, else 2 (a
+    // sphere direction). syncDimension brings every task of a bounce to the
+    // maximum over the queue, which is 3 as soon as any task picked an area
+    // light; this fixes it per scene.
+    int light_dims;
     // hit barycentrics are needed for vertex normals and texture coordinates
     __host__ __device__ bool needs_bary() const { return normals != nullptr || texels != nullptr; }
 };

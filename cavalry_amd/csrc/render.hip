@@ -77,6 +77,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
     bool live = j < (int64_t)n;
     const int64_t i = live ? task_slot(perm, j) : 0;
     int prim = live ? hit_prim[i] : -1;
+    if (live && prim < 0 && sc.env_miss >= 0) {  // intersectScene's miss (PathTracingIntegrator.cu:112-116)
+        float* acc = res + (int64_t)qi.sid[i] * 3;
+        const v3 e = mul3(miss_radiance(sc, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5])),
+                          mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]));
+        acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
+    }
     live = live && prim >= 0;
     ShadeOut o;
     uint32_t sid = 0;
@@ -111,6 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_direct(SceneDev sc, HaltonDev h, Que
     bool live = j < (int64_t)counts[0];
     const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
+    if (live && prim < 0 && sc.env_miss >= 0) {  // DirectLightingIntegrator.cu:31-36: the background
+        float* acc = res + (int64_t)qi.sid[i] * 3;
+        const v3 e = miss_radiance(sc, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5]));
+        acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
+    }
     live = live && prim >= 0;
     ShadeOut o;
     uint32_t sid = 0;
@@ -278,7 +289,7 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
             return e;
         hipLaunchKernelGGL(k_nee, dim3(nb), dim3(kBlock), 0, s, a.sh, a.vis, a.res, a.counts);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, qin);
-        dim0 += 10 + (depth > 3 ? 1 : 0);
+        dim0 += 7 + a.scene.light_dims + (depth > 3 ? 1 : 0);
         qin = qout;
     }
     return hipGetLastError();

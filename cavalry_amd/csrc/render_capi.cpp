@@ -34,6 +34,9 @@ struct cav_scene {
     uint8_t* mesh_normals = nullptr; // per mesh
     uint32_t* texels = nullptr;      // every texture's RGBA8 texels
     TexDev* tex = nullptr;
+    EnvDev* env = nullptr;           // environment-type lights
+    int env_miss = -1;
+    int light_dims = 3;
     std::vector<cav_mesh> host_meshes;
     std::vector<float> host_area;
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // Scene::sceneBounds (Core/Scene.cu:131-136)
@@ -251,7 +254,7 @@ SceneDev scene_dev(const cav_scene* sc) {
     return SceneDev{sc->bvh->tn,   sc->bvh->tt,  sc->bvh->root, sc->ntris,   sc->soup,
                     sc->tri_mesh,   sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
                     sc->nlights,    sc->normals,  sc->mesh_normals, sc->uv,   sc->texels,
-                    sc->tex};
+                    sc->tex,        sc->env,      sc->env_miss,     sc->light_dims};
 }
 
 int halton_dev(cav_ctx* ctx, HaltonDev* out) {
@@ -300,7 +303,6 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
     if (!tris || !meshes || n < 2 || nmeshes < 1) return fail(ctx, CAV_ERR_INVALID, "scene needs >= 2 triangles and >= 1 mesh");
     // meshes must tile the soup in order
     std::vector<int32_t> tri_mesh(n, -1);
-    int nlights = 0;
     int next = 0;
     for (int m = 0; m < nmeshes; ++m) {
         const cav_mesh& M = meshes[m];
@@ -311,10 +313,37 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
             return fail(ctx, CAV_ERR_INVALID, "unsupported material " + std::to_string(M.material));
         for (int t = M.first_tri; t < M.first_tri + M.num_tris; ++t) tri_mesh[t] = m;
         next += M.num_tris;
-        nlights += M.is_light ? 1 : 0;
     }
     if (next != n) return fail(ctx, CAV_ERR_INVALID, "meshes do not cover every triangle");
-    if (nlights == 0) return fail(ctx, CAV_ERR_INVALID, "scene has no area light");
+    // the light list: area-light meshes and environment-type lights in the
+    // scene's order (Scene::lightsHost)
+    const int nenv = desc->nenv;
+    if (nenv < 0 || (nenv > 0 && !desc->env_lights)) return fail(ctx, CAV_ERR_INVALID, "bad environment light list");
+    std::vector<int32_t> lights;
+    if (desc->light_order) {
+        std::vector<uint8_t> seen_mesh((size_t)nmeshes, 0), seen_env((size_t)nenv, 0);
+        for (int k = 0; k < desc->nlight_order; ++k) {
+            const int32_t e = desc->light_order[k];
+            const bool ok = e >= 0 ? (e < nmeshes && meshes[e].is_light && !seen_mesh[(size_t)e]++)
+                                   : (-1 - e < nenv && !seen_env[(size_t)(-1 - e)]++);
+            if (!ok) return fail(ctx, CAV_ERR_INVALID, "light_order entry " + std::to_string(k) + " is not a new light");
+            lights.push_back(e);
+        }
+        for (int m = 0; m < nmeshes; ++m)
+            if (meshes[m].is_light && !seen_mesh[(size_t)m])
+                return fail(ctx, CAV_ERR_INVALID, "light_order misses area light mesh " + std::to_string(m));
+        for (int k = 0; k < nenv; ++k)
+            if (!seen_env[(size_t)k]) return fail(ctx, CAV_ERR_INVALID, "light_order misses environment light " + std::to_string(k));
+    } else {
+        for (int m = 0; m < nmeshes; ++m)
+            if (meshes[m].is_light) lights.push_back(m);
+        for (int k = 0; k < nenv; ++k) lights.push_back(-1 - k);
+    }
+    const int nlights = (int)lights.size();
+    if (nlights == 0) return fail(ctx, CAV_ERR_INVALID, "scene has no light");
+    const int background = nenv > 0 ? desc->env_background : -1;  // ignored without environment lights
+    if (background < -1 || background >= nenv)
+        return fail(ctx, CAV_ERR_INVALID, "env_background must name an environment light or be -1");
     // textures: packed into one texel buffer
     const int ntex = desc->ntextures;
     if (ntex < 0 || (ntex > 0 && !desc->textures)) return fail(ctx, CAV_ERR_INVALID, "bad texture list");
@@ -334,6 +363,13 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
         if (ti < -1 || ti >= ntex)
             return fail(ctx, CAV_ERR_INVALID, "mesh " + std::to_string(m) + " names texture " + std::to_string(ti) +
                                                   " of " + std::to_string(ntex));
+        any_tex = any_tex || ti >= 0;
+    }
+    for (int k = 0; k < nenv; ++k) {
+        const int32_t ti = desc->env_lights[k].texture;
+        if (ti < -1 || ti >= ntex)
+            return fail(ctx, CAV_ERR_INVALID, "environment light " + std::to_string(k) + " names texture " +
+                                                  std::to_string(ti) + " of " + std::to_string(ntex));
         any_tex = any_tex || ti >= 0;
     }
     RCHK(ctx, hipSetDevice(ctx->device));
@@ -357,7 +393,6 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
     sc->host_meshes.assign(meshes, meshes + nmeshes);
     // TriangleMesh::computeArea (Core/TriangleMesh.cu:189-199)
     sc->host_area.resize(nmeshes);
-    std::vector<int32_t> lights;
     for (int m = 0; m < nmeshes; ++m) {
         float a = 0;
         for (int t = meshes[m].first_tri; t < meshes[m].first_tri + meshes[m].num_tris; ++t) {
@@ -368,13 +403,17 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
             a += std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5f;
         }
         sc->host_area[m] = a;
-        if (meshes[m].is_light) lights.push_back(m);
     }
+    sc->env_miss = background;
+    sc->light_dims = 2;
+    for (int32_t e : lights) sc->light_dims = e >= 0 ? 3 : sc->light_dims;
     int rc = dev_alloc(ctx, (void**)&sc->soup, 36 * (size_t)n, "scene soup");
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->tri_mesh, 4 * (size_t)n, "triangle meshes");
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->meshes, sizeof(cav_mesh) * nmeshes, "meshes");
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->area, 4 * (size_t)nmeshes, "mesh areas");
     if (rc == CAV_OK) rc = dev_alloc(ctx, (void**)&sc->lights, 4 * (size_t)nlights, "lights");
+    static_assert(sizeof(EnvDev) == sizeof(cav_env_light), "EnvDev mirrors cav_env_light");
+    if (rc == CAV_OK && nenv > 0) rc = dev_alloc(ctx, (void**)&sc->env, sizeof(EnvDev) * (size_t)nenv, "environment lights");
     if (rc == CAV_OK && tri_uv) rc = dev_alloc(ctx, (void**)&sc->uv, 24 * (size_t)n, "uv");
     std::vector<uint8_t> mflags;
     if (desc->tri_normals) {
@@ -403,6 +442,8 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
         if (e == hipSuccess) e = hipMemcpy(sc->meshes, meshes, sizeof(cav_mesh) * nmeshes, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(sc->area, sc->host_area.data(), 4 * (size_t)nmeshes, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(sc->lights, lights.data(), 4 * (size_t)nlights, hipMemcpyHostToDevice);
+        if (e == hipSuccess && nenv > 0)
+            e = hipMemcpy(sc->env, desc->env_lights, sizeof(EnvDev) * (size_t)nenv, hipMemcpyHostToDevice);
         if (e == hipSuccess && tri_uv) e = hipMemcpy(sc->uv, tri_uv, 24 * (size_t)n, hipMemcpyHostToDevice);
         if (e == hipSuccess && desc->tri_normals)
             e = hipMemcpy(sc->normals, desc->tri_normals, 36 * (size_t)n, hipMemcpyHostToDevice);
@@ -424,7 +465,7 @@ int cav_scene_destroy(cav_scene* sc) {
     if (sc->bvh) cav_bvh_destroy(sc->bvh);
     for (void* p : {(void*)sc->soup, (void*)sc->tri_mesh, (void*)sc->meshes, (void*)sc->area,
                     (void*)sc->lights, (void*)sc->uv, (void*)sc->normals, (void*)sc->mesh_normals,
-                    (void*)sc->texels, (void*)sc->tex})
+                    (void*)sc->texels, (void*)sc->tex, (void*)sc->env})
         if (p) (void)hipFree(p);
     delete sc;
     return CAV_OK;

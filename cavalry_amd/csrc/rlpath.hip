@@ -18,9 +18,10 @@
 //   Q and light proposals, next-bin sample, MIS with the Q pdf) ->
 //   k_finish_q (commit + average per entry) -> k_finish_light (light CDFs) ->
 //   k_next (second syncDimension, Russian roulette, next ray).
-// Halton dimensions follow the reference exactly, including the batch-wide
-// maximum taken by syncDimension, so the device keeps the maximum in a word
-// instead of the host reading it back. Tasks keep their queue slot through a
+// Halton dimensions follow the reference, including the batch-wide maximum
+// taken by syncDimension, so the device keeps the maximum in a word instead
+// of the host reading it back; the light sample's share is the scene's
+// light_dims (see SceneDev). Tasks keep their queue slot through a
 // bounce, so no lighting queue is built: per-slot arrays carry the results
 // from one kernel to the next.
 //
@@ -183,10 +184,16 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
     bool live = j < (int64_t)counts[qin];
     const int64_t i = live ? task_slot(perm, j) : 0;
     const int prim = live ? hit_prim[i] : -1;
+    if (live && prim < 0 && sc.env_miss >= 0) {  // intersectScene's miss (:292-296)
+        float* acc = res + (int64_t)qi.sid[i] * 3;
+        const v3 e = mul3(miss_radiance(sc, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5])),
+                          mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]));
+        acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
+    }
     live = live && prim >= 0;
     v3 pos = mk3(0, 0, 0), dir = pos;
     float lim = 0.f;
-    int mesh = 0, lm = 0;
+    int mesh = 0, dst = 0;
     if (live) {
         const int D = dims[0];
         const uint32_t sid = qi.sid[i];
@@ -219,21 +226,12 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
         const float* cdf = t.ldist + (int64_t)entry * sc.nlights;
         const int li = dist_index(cdf, sc.nlights, halton(h, sid, D));
         const float sel = li == 0 ? cdf[0] : cdf[li] - cdf[li - 1];
-        lm = sc.lights[li];
-        const cav_mesh L = sc.meshes[lm];
-        const int tri = L.first_tri + halton_int(h, sid, D + 5, L.num_tris);
-        const float tmp = sqrtf(halton(h, sid, D + 6));
-        const float u = 1 - tmp;
-        const float v = tmp * halton(h, sid, D + 7);
-        const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
-        const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-        const v3 ln = shading_normal(sc, tri, u, v);
-        const v3 l2s = sub3(pos, lp);
-        const float c = fabsf(dot3(ln, nrm(l2s)));
-        const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
-        dir = nrm(sub3(lp, pos));
-        lim = len3(sub3(lp, pos));
-        const v3 inc = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+        const LightPick L = sample_light(sc, h, sid, D + 5, pos, sc.lights[li]);
+        const float lpdf = L.prob;
+        const v3 inc = L.incident;
+        dir = L.dir;
+        lim = L.lim;
+        dst = L.dst;
         sl.entry[i] = entry;
         sl.light[i] = li;
         sl.sel[i] = sel;
@@ -246,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
     if (live) {
         float* r = sh.ray + (int64_t)ss * 6;
         r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
-        sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = lm;
+        sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = dst;
         sl.sh_idx[i] = (int32_t)ss;
     }
 }
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
     live = live && prim >= 0;
     bool valid = false;
     if (live) {
-        const int D = dims[0] + 8;
+        const int D = dims[0] + 5 + sc.light_dims;
         const uint32_t sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
@@ -596,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
     int cell = 0, entry = 0, dim = 0;
     bool ndelta = false;
     if (live) {
-        dim = dims[0] + 10 + dims[1];
+        dim = dims[0] + 7 + sc.light_dims + dims[1];
         sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);

@@ -19,9 +19,11 @@ namespace {
 using namespace shade;
 
 // first Halton dimension of bounce `depth` of the `path` integrator:
-// 2 for the camera sample, then 10 per bounce plus 1 for Russian roulette
-// after the fourth
-__device__ inline int path_dim0(int depth) { return 2 + 10 * depth + (depth > 4 ? depth - 4 : 0); }
+// 2 for the camera sample, then per bounce 5 + the light's dimensions + 2,
+// plus 1 for Russian roulette after the fourth
+__device__ inline int path_dim0(int depth, int light_dims) {
+    return 2 + (7 + light_dims) * depth + (depth > 4 ? depth - 4 : 0);
+}
 
 __global__ __launch_bounds__(kBlock) void k_pool_raygen(PoolDev p, const Chunk* __restrict__ chunks, int nchunks,
                                                         int64_t total, int qn) {
@@ -72,6 +74,14 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = i < (int64_t)p.counts[qin];
     const int prim = live ? p.hit_prim[i] : -1;
+    if (live && prim < 0 && p.scene.env_miss >= 0) {  // a miss sees the background, weighted as the loops do
+        const Queue& qi = p.q[qin];
+        const ReqDev& r = p.reqs[p.rid[qin][i]];
+        float* acc = r.res + (int64_t)qi.sid[i] * 3;
+        v3 e = miss_radiance(p.scene, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5]));
+        if (r.integrator != CAV_INTEGRATOR_DIRECT) e = mul3(e, mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]));
+        acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
+    }
     live = live && prim >= 0;
     ShadeOut o;
     uint32_t sid = 0;
@@ -89,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
         if (r.integrator == CAV_INTEGRATOR_DIRECT) {
             direct_shade(p.scene, p.halton, sid, ro, rd, prim, p.hit_t[i], hu, hv, acc, o);
         } else {
-            path_shade(depth, path_dim0(depth), p.scene, p.halton, sid, ro, rd,
+            path_shade(depth, path_dim0(depth, p.scene.light_dims), p.scene, p.halton, sid, ro, rd,
                        mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]), qi.pdf[i], qi.delta[i] != 0,
                        prim, p.hit_t[i], hu, hv, acc, o);
             // the plain loop's last bounce pushes rays that are never traced

@@ -127,8 +127,8 @@ struct Ggx {
         if (cosT > .9999) {
             const float r = sqrtf(u1 / (1 - u1));
             const float phi = (float)(6.28318530718 * u2);
-            sx = r * cosf(phi);
-            sy = r * sinf(phi);
+            sx = r * (float)cos((double)phi);  // binary64, rounded once: equal on every host and device
+            sy = r * (float)sin((double)phi);
             return;
         }
         const float sinT = sqrtf(fmx(0.f, 1.f - cosT * cosT));
@@ -452,6 +452,83 @@ __device__ inline Bsdf surface_bsdf(const SceneDev& sc, int mesh, int prim, floa
     return b;
 }
 
+// EnvironmentMap::evaluateRay (Core/Light.h:91-113): the colour, times the
+// texel at (phi / 2pi, theta / pi) of the direction in the light's frame.
+// atan2 and acos are evaluated in binary64 and rounded once (the reference
+// calls the float routines); the M_PI divisions are binary64 as written.
+__device__ inline v3 env_eval(const SceneDev& sc, int e, v3 d) {
+    const EnvDev& E = sc.env[e];
+    v3 r = mk3(E.color[0], E.color[1], E.color[2]);
+    if (E.texture >= 0 && sc.texels) {
+        const float* m = E.m;
+        const v3 l = mk3((m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[3] * d.x + m[4] * d.y) + m[5] * d.z,
+                         (m[6] * d.x + m[7] * d.y) + m[8] * d.z);
+        float phi = (float)atan2((double)l.y, (double)l.x);
+        phi = phi < 0 ? (float)((double)phi + 2 * kPi) : phi;
+        const float theta = (float)acos((double)clampf(l.z, -1.f, 1.f));
+        const float tu = (float)((double)phi / (2.0 * kPi)), tv = (float)((double)theta / kPi);
+        r = mul3(r, tex_fetch(sc, E.texture, tu, tv));
+    }
+    return r;
+}
+
+// Radiance a ray that misses every triangle carries back: the background
+// environment map, or nothing.
+__device__ inline v3 miss_radiance(const SceneDev& sc, v3 d) {
+    return sc.env_miss >= 0 ? env_eval(sc, sc.env_miss, d) : mk3(0, 0, 0);
+}
+
+// uniformSampleSphere (Utils/RandomUtils.h:70-78): angles through binary64
+// M_PI as written, cos / sin in binary64 rounded once.
+__device__ inline v3 uniform_sphere(float x, float y) {
+    const float u = (float)((double)(x * 2.f) * kPi);
+    const float v = (float)(((double)y - 0.5) * kPi);
+    const float cu = (float)cos((double)u), su = (float)sin((double)u);
+    const float cv = (float)cos((double)v), sv = (float)sin((double)v);
+    return mk3(cv * cu, sv, cv * su);
+}
+
+// LightObject::sampleRayToPoint for light-list entry lm seen from pos, with
+// its dimensions from d on (after randInt + rand4): an area light
+// (DiffuseAreaLight::sampleRayToPoint, TriangleMesh::sample: a uniform
+// triangle, then a uniform point; Core/TriangleMesh.h:108-153) or an
+// environment-type light (EnvironmentMap::sampleRayToPoint: a uniform sphere
+// direction, pdf 1 / 4pi, no distance limit; Core/Light.h:75-84).
+struct LightPick {
+    v3 dir, incident;
+    float prob, lim;
+    int dst;  // target mesh the shadow ray ignores (-1: none)
+};
+__device__ inline LightPick sample_light(const SceneDev& sc, const HaltonDev& h, uint32_t sid, int d, v3 pos,
+                                         int lm) {
+    LightPick r;
+    if (lm < 0) {
+        const int e = -1 - lm;
+        r.dir = uniform_sphere(halton(h, sid, d), halton(h, sid, d + 1));
+        r.prob = (float)(1.0 / (4.0 * kPi));
+        r.incident = env_eval(sc, e, r.dir);
+        r.lim = __builtin_inff();
+        r.dst = -1;
+        return r;
+    }
+    const cav_mesh L = sc.meshes[lm];
+    const int tri = L.first_tri + halton_int(h, sid, d, L.num_tris);
+    const float tmp = sqrtf(halton(h, sid, d + 1));
+    const float u = 1 - tmp;
+    const float v = tmp * halton(h, sid, d + 2);
+    const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
+    const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
+    const v3 ln = shading_normal(sc, tri, u, v);
+    const v3 l2s = sub3(pos, lp);
+    const float c = fabsf(dot3(ln, nrm(l2s)));
+    r.prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
+    r.dir = nrm(sub3(lp, pos));
+    r.lim = len3(sub3(lp, pos));
+    r.incident = dot3(r.dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+    r.dst = lm;
+    return r;
+}
+
 // What shading one task produces: an optional shadow task and an optional
 // continuation ray.
 struct ShadeOut {
@@ -492,36 +569,22 @@ __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const
         acc3[0] = acc.x; acc3[1] = acc.y; acc3[2] = acc.z;
     }
     const v3 wo = scale3(rd, -1);
-    // light sample: DiffuseAreaLight::sampleRayToPoint / TriangleMesh::sample
+    // the light sample (randInt, then rand4 that nothing reads, then the light's own)
     const int li = halton_int(h, sid, dim0, sc.nlights);
-    const int lm = sc.lights[li];
-    const cav_mesh L = sc.meshes[lm];
-    const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
-    const float tmp = sqrtf(halton(h, sid, dim0 + 6));
-    const float u = 1 - tmp;
-    const float v = tmp * halton(h, sid, dim0 + 7);
-    const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
-    const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-    const v3 ln = shading_normal(sc, tri, u, v);
-    const v3 l2s = sub3(pos, lp);
-    const float c = fabsf(dot3(ln, nrm(l2s)));
-    const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
-    const v3 dir = nrm(sub3(lp, pos));
-    const float lim = len3(sub3(lp, pos));
-    const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
-    if (isfinite(prob)) {
-        const v3 em = divs3(scale3(mult, (float)sc.nlights), prob);
-        const v3 il = to_local(dir, t0, t1, nn), ol = to_local(wo, t0, t1, nn);
+    const LightPick L = sample_light(sc, h, sid, dim0 + 5, pos, sc.lights[li]);
+    if (isfinite(L.prob)) {
+        const v3 em = divs3(scale3(mult, (float)sc.nlights), L.prob);
+        const v3 il = to_local(L.dir, t0, t1, nn), ol = to_local(wo, t0, t1, nn);
         const float mp = bsdf.pdf(il, ol);
-        const float w = power_heuristic(prob, mp);
-        const float cosv = fabsf(dot3(dir, nn));
-        const v3 e = scale3(mul3(incident, bsdf.eval(il, ol)), cosv);
+        const float w = power_heuristic(L.prob, mp);
+        const float cosv = fabsf(dot3(L.dir, nn));
+        const v3 e = scale3(mul3(L.incident, bsdf.eval(il, ol)), cosv);
         o.contrib = scale3(mul3(e, em), w);
-        o.sro = pos; o.srd = dir; o.slim = lim; o.ssrc = mesh; o.sdst = lm;
+        o.sro = pos; o.srd = L.dir; o.slim = L.lim; o.ssrc = mesh; o.sdst = L.dst;
         o.push_sh = true;
     }
     // genNextRay
-    int dim = dim0 + 8;
+    int dim = dim0 + 5 + sc.light_dims;
     bool alive = true;
     if (depth > 3) {
         float q = 1;
@@ -565,27 +628,14 @@ __device__ inline void direct_shade(const SceneDev& sc, const HaltonDev& h, uint
     }
     const int dim0 = 2;
     const int li = halton_int(h, sid, dim0, sc.nlights);
-    const int lm = sc.lights[li];
-    const cav_mesh L = sc.meshes[lm];
-    const int tri = L.first_tri + halton_int(h, sid, dim0 + 5, L.num_tris);
-    const float tmp = sqrtf(halton(h, sid, dim0 + 6));
-    const float u = 1 - tmp;
-    const float v = tmp * halton(h, sid, dim0 + 7);
-    const v3 p0 = soup_v(sc.soup, tri, 0), p1 = soup_v(sc.soup, tri, 1), p2 = soup_v(sc.soup, tri, 2);
-    const v3 lp = add3(add3(scale3(p0, u), scale3(p1, v)), scale3(p2, 1.f - u - v));
-    const v3 ln = shading_normal(sc, tri, u, v);
-    const v3 l2s = sub3(pos, lp);
-    const float c = fabsf(dot3(ln, nrm(l2s)));
-    const float prob = dot3(l2s, l2s) / (c * sc.mesh_area[lm]);
-    const v3 dir = nrm(sub3(lp, pos));
-    const v3 incident = dot3(dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+    const LightPick L = sample_light(sc, h, sid, dim0 + 5, pos, sc.lights[li]);
     const Bsdf bsdf = surface_bsdf(sc, mesh, prim, hu, hv);
     v3 t0, t1;
     frame(nn, t0, t1);
-    const v3 il = to_local(dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);
-    const float m = (float)sc.nlights / prob;
-    o.contrib = scale3(scale3(mul3(incident, bsdf.eval(il, ol)), fabsf(dot3(dir, nn))), m);
-    o.sro = pos; o.srd = dir; o.slim = len3(sub3(lp, pos)); o.ssrc = mesh; o.sdst = lm;
+    const v3 il = to_local(L.dir, t0, t1, nn), ol = to_local(scale3(rd, -1), t0, t1, nn);
+    const float m = (float)sc.nlights / L.prob;
+    o.contrib = scale3(scale3(mul3(L.incident, bsdf.eval(il, ol)), fabsf(dot3(L.dir, nn))), m);
+    o.sro = pos; o.srd = L.dir; o.slim = L.lim; o.ssrc = mesh; o.sdst = L.dst;
     o.push_sh = true;
 }
 

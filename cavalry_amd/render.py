@@ -83,6 +83,7 @@ class Scene:
     def __init__(self, ctx: Context, arrays, bvh_rounds: int = 0, morton_bits: int = 30):
         self.ctx = ctx
         self.arrays = arrays
+        self.handle = None
         h = ctypes.c_void_p()
         tris = np.ascontiguousarray(arrays.tris, np.float32)
         meshes = np.ascontiguousarray(arrays.meshes)
@@ -95,14 +96,21 @@ class Scene:
             if t.ndim != 3 or t.shape[2] != 4 or t.dtype != np.uint8 or not t.flags.c_contiguous:
                 raise ValueError("textures must be contiguous uint8 [H, W, 4] arrays")
             tex[k] = _lib.Texture(t.ctypes.data, t.shape[1], t.shape[0])
-        self._keep = (tris, meshes, normals, flags, uv, textures, tex)
+        env = getattr(arrays, "env_lights", None)
+        env = None if env is None or len(env) == 0 else np.ascontiguousarray(env)
+        order = getattr(arrays, "light_order", None)
+        self._keep = (tris, meshes, normals, flags, uv, textures, tex, env, order)
         d = _lib.SceneDesc(tris.ctypes.data, len(tris), meshes.ctypes.data, len(meshes),
                            None if uv is None else uv.ctypes.data,
                            None if normals is None else normals.ctypes.data,
                            None if flags is None else flags.ctypes.data, int(bvh_rounds), int(morton_bits),
-                           ctypes.addressof(tex) if textures else None, len(textures))
+                           ctypes.addressof(tex) if textures else None, len(textures),
+                           None if env is None else env.ctypes.data, 0 if env is None else len(env),
+                           None if order is None else order.ctypes.data, 0 if order is None else len(order),
+                           int(getattr(arrays, "env_background", -1)))
         check(ctx.handle, lib().cav_scene_create_desc(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
+        ctx.own(self)
 
     @classmethod
     def from_pbrt(cls, ctx: Context, path: str, **overrides):
@@ -119,6 +127,7 @@ class Scene:
                                                            ctypes.byref(h), ctypes.byref(st)))
         self.handle = h
         self.setup = st
+        ctx.own(self)
         return self, st
 
     def close(self) -> None:

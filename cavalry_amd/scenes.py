@@ -73,13 +73,27 @@ assert MESH_DTYPE.itemsize == 96  # cav_mesh in include/cavalry.h
 
 MATTE, MIRROR, SUBSTRATE, METAL, GLASS, PLASTIC = 0, 1, 2, 3, 4, 5
 
+# an environment-type light (cav_env_light): radiance, texture index (-1:
+# none) and the world -> light rotation, row-major
+ENV_DTYPE = np.dtype([("color", "<f4", 3), ("texture", "<i4"), ("world_to_light", "<f4", 9)])
+assert ENV_DTYPE.itemsize == 52
+
+
+def env_light(color, texture: int = -1, world_to_light=None) -> np.ndarray:
+    e = np.zeros((), ENV_DTYPE)
+    e["color"] = color
+    e["texture"] = texture
+    e["world_to_light"] = np.eye(3).ravel() if world_to_light is None else np.asarray(world_to_light).ravel()
+    return e
+
 
 class SceneArrays:
     """A renderable scene: triangle soup, meshes (contiguous triangle ranges)
     and a LookAt camera."""
 
     def __init__(self, tris, meshes, eye, center, up, fov, width, height, max_depth=8, normals=None,
-                 mesh_normals=None, uv=None, textures=None):
+                 mesh_normals=None, uv=None, textures=None, env_lights=None, light_order=None,
+                 env_background=-1):
         self.tris = np.ascontiguousarray(tris, np.float32)
         # optional per-vertex shading normals float32[N, 9] and per-mesh use flags
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32)
@@ -88,6 +102,13 @@ class SceneArrays:
         # (uint8 [H, W, 4], row 0 at v = 0) that meshes["texture"] indexes (-1: none)
         self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32)
         self.textures = [np.ascontiguousarray(t, np.uint8) for t in (textures or [])]
+        # environment-type lights (ENV_DTYPE), the light list order (mesh id >= 0 or
+        # -1 - k for env_lights[k]; None: area lights in mesh order, then these) and
+        # the one that rays which miss see (-1: none)
+        self.env_lights = (np.zeros(0, ENV_DTYPE) if env_lights is None
+                           else np.ascontiguousarray(np.atleast_1d(np.asarray(env_lights, ENV_DTYPE))))
+        self.light_order = None if light_order is None else np.ascontiguousarray(light_order, np.int32)
+        self.env_background = int(env_background)
         self.meshes = meshes
         self.eye = np.asarray(eye, np.float32)
         self.center = np.asarray(center, np.float32)
@@ -233,8 +254,41 @@ def procedural_texture(kind: str, size: int, seed: int) -> np.ndarray:
     return out
 
 
+def sky_texture(w: int = 32, h: int = 16) -> np.ndarray:
+    """RGBA8 [h, w, 4] sky for environment lights: a vertical blue-to-white
+    gradient with a warm band at one azimuth (row 0 at theta = 0)."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    t = (y + 0.5) / h
+    rgb = np.stack([0.3 + 0.7 * t, 0.5 + 0.5 * t, np.ones_like(t)], -1)
+    rgb[..., 0] += 0.5 * np.exp(-((x + 0.5) / w - 0.25) ** 2 / 0.005)
+    out = np.empty((h, w, 4), np.uint8)
+    out[..., :3] = np.clip(np.round(rgb / rgb.max() * 255), 0, 255)
+    out[..., 3] = 255
+    return out
+
+
+def rotation_y(degrees: float) -> np.ndarray:
+    a = np.radians(degrees)
+    return np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+
+
+def sky_plane(seed: int = 1, width: int = 64, height: int = 48, albedo: float = 0.5, sky: float = 1.0,
+              max_depth: int = 8, black_lights: int = 0) -> SceneArrays:
+    """A 20 x 20 matte floor (albedo `albedo`) under a constant environment
+    light of radiance `sky`, nothing else; the camera looks down at the floor
+    and the horizon. black_lights adds that many environment lights of
+    radiance 0 (how the reference treats a LightSource it does not know)."""
+    b = _Builder()
+    b.mesh(quad((-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)), kd=(albedo, albedo, albedo))
+    env = [env_light((sky, sky, sky))] + [env_light((0, 0, 0)) for _ in range(black_lights)]
+    tris, meshes = b.arrays()
+    return SceneArrays(tris, meshes, (0, 2, -3), (0, 0.3, 2), (0, 1, 0), 60.0, width, height, max_depth,
+                       env_lights=env, env_background=0)
+
+
 def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: int = 8,
-                materials: str = "matte", smooth: bool = False, textured: bool = False) -> SceneArrays:
+                materials: str = "matte", smooth: bool = False, textured: bool = False,
+                env: bool = False) -> SceneArrays:
     """Cornell-style box: unit box open at z=0 (5 walls x 2 quads), a tall and a
     short box, a 64x32 UV sphere (3968 triangles) and a 0.25x0.25 ceiling light
     of radiance 15 facing down. Albedos U[0.1,0.9] per mesh from the seed.
@@ -244,7 +298,9 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
     smooth=True gives the sphere radial per-vertex normals (smooth shading).
     textured=True puts a checker on the floor, stripes on the back wall and
     value-noise marble on the tall box (64 x 64 textures, planar UVs; the
-    matte ones with kd = 1), and spherical UVs on the sphere."""
+    matte ones with kd = 1), and spherical UVs on the sphere. env=True adds a
+    textured sky (rotated 30 degrees about y, seen by rays leaving through the
+    open front) and a black light, ordered sky, ceiling light, black."""
     rng = _rng(seed, 7)
     b = _Builder()
 
@@ -297,8 +353,16 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
         # a textured matte takes its colour from the texture alone, as a
         # .pbrt Kd is either a colour or a texture
         meshes["kd"][[k for k in (0, 2, 5) if meshes["material"][k] == MATTE]] = 1.0
+    envs = order = None
+    background = -1
+    if env:
+        textures = (textures or []) + [sky_texture()]
+        envs = [env_light((0.8, 0.9, 1.0), len(textures) - 1, rotation_y(30).T), env_light((0, 0, 0))]
+        order = [-1, len(meshes) - 1, -2]
+        background = 0
     return SceneArrays(tris, meshes, (0.5, 0.5, -1.3), (0.5, 0.5, 0.5), (0, 1, 0), 40.0,
-                       width, height, max_depth, normals=normals, mesh_normals=flags, uv=uv, textures=textures)
+                       width, height, max_depth, normals=normals, mesh_normals=flags, uv=uv, textures=textures,
+                       env_lights=envs, light_order=order, env_background=background)
 
 
 @functools.lru_cache(maxsize=4)

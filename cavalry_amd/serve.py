@@ -37,6 +37,7 @@ class Server:
         h = ctypes.c_void_p()
         check(ctx.handle, lib().cav_serve_open(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
+        ctx.own(self)
 
     def submit(self, scene: Scene, spp: int, integrator: str = "path", camera=None, max_depth=None,
                q_grid: int = 32, alpha_scale: float = 1.0) -> Ticket:

@@ -189,6 +189,19 @@ typedef struct {
     int32_t width, height;
 } cav_texture;
 
+/* An environment-type light (EnvironmentMap, Core/Light.h:43-122): radiance
+ * `color`, times the texel of `texture` (-1: none) at
+ * (phi / 2pi, theta / pi) of the direction rotated by world_to_light
+ * (row-major 3x3, the upper block of the inverse light-to-world transform).
+ * Sampled as a light by a uniform direction on the sphere (pdf 1 / 4pi) and
+ * an unlimited shadow ray. A LightSource the reference does not know becomes
+ * such a light with color 0 (Lights/LightObject.h:124-126). */
+typedef struct {
+    float color[3];
+    int32_t texture;
+    float world_to_light[9];
+} cav_env_light;
+
 /* The same with every scene input in one descriptor, including per-vertex
  * shading normals: Triangle::fillIntersectionInformation and
  * TriangleMesh::getNormal (Core/Triangle.h:94-136, Core/TriangleMesh.h:76-104)
@@ -209,6 +222,17 @@ typedef struct {
     int32_t morton_bits;
     const cav_texture* textures;  /* ntextures images, or NULL */
     int32_t ntextures;
+    /* Lights beyond the area-light meshes. light_order lists every light in
+     * the order the scene defines them (the order randInt / the learnt
+     * distribution index): a mesh id >= 0 for an area light, -1 - k for
+     * env_lights[k]; NULL means area lights in mesh order, then env_lights.
+     * env_background (-1: none; ignored when nenv is 0) is the light that
+     * rays which miss every triangle see (Scene::environmentMapIndex). */
+    const cav_env_light* env_lights;
+    int32_t nenv;
+    const int32_t* light_order;
+    int32_t nlight_order;
+    int32_t env_background;
 } cav_scene_desc;
 int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** out);
 int cav_scene_destroy(cav_scene* scene);

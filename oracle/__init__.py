@@ -155,7 +155,9 @@ def mesh_areas(tris, meshes) -> np.ndarray:
 class _Surface(ctypes.Structure):
     """ref_surface of ref_api.h"""
     _fields_ = [("tri_normals", ctypes.c_void_p), ("mesh_normals", ctypes.c_void_p), ("tri_uv", ctypes.c_void_p),
-                ("texels", ctypes.c_void_p), ("tex_offset", ctypes.c_void_p), ("tex_size", ctypes.c_void_p)]
+                ("texels", ctypes.c_void_p), ("tex_offset", ctypes.c_void_p), ("tex_size", ctypes.c_void_p),
+                ("env", ctypes.c_void_p), ("nenv", ctypes.c_int32), ("light_order", ctypes.c_void_p),
+                ("nlight_order", ctypes.c_int32), ("env_background", ctypes.c_int32)]
 
 
 def _surface(scene):
@@ -174,8 +176,15 @@ def _surface(scene):
         texels = np.concatenate([np.ascontiguousarray(t, np.uint8).ravel() for t in textures])
         sizes = np.array([[t.shape[1], t.shape[0]] for t in textures], np.int32)
         offs = np.concatenate([[0], np.cumsum(sizes[:, 0].astype(np.int64) * sizes[:, 1])[:-1]]).astype(np.int64)
-    keep = (nrm, flags, uv, texels, offs, sizes)
-    s = _Surface(*[None if a is None else a.ctypes.data for a in keep])
+    env = getattr(scene, "env_lights", None)
+    env = None if env is None or len(env) == 0 else np.ascontiguousarray(env)
+    if env is not None and int(np.max(env["texture"], initial=-1)) >= len(textures):
+        raise ValueError("an environment light names a missing texture")
+    order = arr("light_order", np.int32)
+    keep = (nrm, flags, uv, texels, offs, sizes, env, order)
+    ptr = [None if a is None else a.ctypes.data for a in keep]
+    s = _Surface(*ptr[:6], ptr[6], 0 if env is None else len(env), ptr[7], 0 if order is None else len(order),
+                 int(getattr(scene, "env_background", -1)))
     scene._oracle_keep = (keep, s)
     return ctypes.byref(s)
 
@@ -311,3 +320,11 @@ def bsdf_sample(mesh, u, wo):
     p = np.zeros(len(u), np.float32)
     lib().ref_bsdf_sample(_p(m), _p(u), _p(wo), ctypes.c_int(len(u)), _p(wi), _p(f), _p(p))
     return wi, f, p
+
+
+def env_eval(scene, e: int, dirs) -> np.ndarray:
+    """EnvironmentMap::evaluateRay of scene.env_lights[e] for world directions [n, 3]."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros_like(d)
+    lib().ref_env_eval(_surface(scene), ctypes.c_int(e), _p(d), ctypes.c_int(len(d)), _p(out))
+    return out

@@ -35,9 +35,16 @@ int ref_trace_occluded(const float* box, const int32_t* left, const int32_t* rig
                        const int32_t* dst_mesh, int64_t nrays, uint8_t* visible);
 
 // render oracles (ref_render.cpp)
-// Surface inputs beyond the triangles: per-vertex normals, texture
-// coordinates and the Kd textures that cav_mesh_ref::texture indexes (any
-// pointer may be null).
+// An environment-type light; same layout as cav_env_light in include/cavalry.h.
+typedef struct {
+    float color[3];
+    int32_t texture;          // -1: none
+    float world_to_light[9];  // row-major
+} ref_env_light;
+
+// Inputs beyond the triangles: per-vertex normals, texture coordinates, the
+// Kd textures that cav_mesh_ref::texture indexes, and lights besides the
+// area-light meshes (any pointer may be null).
 typedef struct {
     const float* tri_normals;     // float[ntris][9]
     const uint8_t* mesh_normals;  // per mesh 1 = shade with tri_normals (null: every mesh)
@@ -45,8 +52,14 @@ typedef struct {
     const uint8_t* texels;        // RGBA8 texels of every texture, concatenated
     const int64_t* tex_offset;    // per texture: first texel
     const int32_t* tex_size;      // per texture: width, height
+    const ref_env_light* env;     // nenv environment-type lights
+    int32_t nenv;
+    const int32_t* light_order;   // the light list (mesh id >= 0, or -1 - k for env[k]); null: area lights then env
+    int32_t nlight_order;
+    int32_t env_background;       // env light that missing rays see, -1: none
 } ref_surface;
 void ref_tex_fetch(const uint8_t* rgba, int w, int h, float u, float v, float* out3);
+void ref_env_eval(const ref_surface* surf, int e, const float* dirs, int n, float* out);
 void ref_bsdf_eval(const cav_mesh_ref* m, const float* wi, const float* wo, int n, float* f_out, float* pdf_out);
 void ref_bsdf_sample(const cav_mesh_ref* m, const float* u2, const float* wo, int n, float* wi_out, float* f_out,
                      float* pdf_out);

@@ -176,11 +176,11 @@ struct GGX {
     float G(f3 i, f3 o) const { return 1.f / (1.f + lambda(i) + lambda(o)); }
     float pdf(f3 h, f3 o) const { return D(h) * G1(o) * std::fabs(dot(o, h)) / std::fabs(o.z); }
     static void sample11(float cosT, float u1, float u2, float* sx, float* sy) {
-        if (cosT > .9999) {  // phi is a float here, so cos/sin are the float versions
+        if (cosT > .9999) {  // the reference's float cos/sin, here in binary64 rounded once
             float r = std::sqrt(u1 / (1 - u1));
             float phi = (float)(6.28318530718 * u2);
-            *sx = r * std::cos(phi);
-            *sy = r * std::sin(phi);
+            *sx = r * (float)std::cos((double)phi);
+            *sy = r * (float)std::sin((double)phi);
             return;
         }
         float sinT = std::sqrt(fmaxr(0.f, 1.f - cosT * cosT));
@@ -420,13 +420,38 @@ struct Scene {
     const float* tris; int ntris;
     const cav_mesh_ref* meshes; int nmeshes;
     const int32_t* tri_mesh;
-    std::vector<int> lights;  // mesh ids of area lights, in mesh order
+    std::vector<int> lights;  // the light list: area-light mesh id, or -1 - k for env[k]
     std::vector<float> area;  // per mesh
     const float* box; const int32_t* left; const int32_t* right; const int32_t* prim;
     const float* normals = nullptr;         // float[ntris][9] per-vertex normals, or null
     const uint8_t* mesh_normals = nullptr;  // per mesh: 1 = use them (null: every mesh)
     const ref_surface* surf = nullptr;      // texture coordinates and textures, or null
+    const ref_env_light* env = nullptr;
+    int env_miss = -1;
+    int light_dims = 3;  // Halton dimensions of a light sample after randInt + rand4
 };
+
+// The light list in the scene's order (Scene::lightsHost) and the light
+// sample's share of the Halton dimensions: syncDimension (PathTracingIntegrator
+// .cu:281-315) takes each bounce to the maximum over the queue, 3 (triangle +
+// point) as soon as any task picked an area light, else 2 (a sphere
+// direction); fixed here per scene.
+void init_lights(Scene& s) {
+    const ref_surface* sf = s.surf;
+    s.lights.clear();
+    if (sf && sf->light_order) {
+        s.lights.assign(sf->light_order, sf->light_order + sf->nlight_order);
+    } else {
+        for (int m = 0; m < s.nmeshes; ++m)
+            if (s.meshes[m].is_light) s.lights.push_back(m);
+        for (int k = 0; sf && k < sf->nenv; ++k) s.lights.push_back(-1 - k);
+    }
+    s.env = sf ? sf->env : nullptr;
+    s.env_miss = sf ? sf->env_background : -1;
+    s.light_dims = 2;
+    for (int e : s.lights)
+        if (e >= 0) s.light_dims = 3;
+}
 
 f3 tri_v(const Scene& s, int t, int j) {
     const float* p = s.tris + (int64_t)t * 9 + j * 3;
@@ -500,6 +525,37 @@ Bsdf surface_bsdf(const Scene& s, int mesh, int prim, float u, float v) {
                                tu, tv);
     b.kd = M.material == kMatte ? mulv(b.kd, texel) : texel;
     return b;
+}
+
+// EnvironmentMap::evaluateRay (Core/Light.h:91-113): colour times the texel
+// at (phi / 2pi, theta / pi) of the direction in the light's frame; atan2 and
+// acos in binary64 rounded once, the M_PI divisions in binary64 as written.
+f3 env_eval(const Scene& s, int e, f3 d) {
+    const ref_env_light& E = s.env[e];
+    f3 r = mk(E.color[0], E.color[1], E.color[2]);
+    const ref_surface* sf = s.surf;
+    if (E.texture >= 0 && sf && sf->texels) {
+        const float* m = E.world_to_light;
+        const f3 l = mk((m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[3] * d.x + m[4] * d.y) + m[5] * d.z,
+                        (m[6] * d.x + m[7] * d.y) + m[8] * d.z);
+        float phi = (float)std::atan2((double)l.y, (double)l.x);
+        phi = phi < 0 ? (float)((double)phi + 2 * kPi) : phi;
+        const float theta = (float)std::acos((double)clampf(l.z, -1.f, 1.f));
+        const float tu = (float)((double)phi / (2.0 * kPi)), tv = (float)((double)theta / kPi);
+        const int ti = E.texture;
+        r = mulv(r, tex_fetch(sf->texels + sf->tex_offset[ti] * 4, sf->tex_size[2 * ti], sf->tex_size[2 * ti + 1],
+                              tu, tv));
+    }
+    return r;
+}
+
+// uniformSampleSphere (Utils/RandomUtils.h:70-78), cos / sin in binary64
+f3 uniform_sphere(float x, float y) {
+    const float u = (float)((double)(x * 2.f) * kPi);
+    const float v = (float)(((double)y - 0.5) * kPi);
+    const float cu = (float)std::cos((double)u), su = (float)std::sin((double)u);
+    const float cv = (float)std::cos((double)v), sv = (float)std::sin((double)v);
+    return mk(cv * cu, sv, cv * su);
 }
 
 // single-ray traversal on the reference node arrays (BVH/BVH.h)
@@ -613,6 +669,44 @@ f3 shading_normal(const Scene& s, int p, float u, float v) {
 
 float power_heuristic(float a, float b) { a *= a; b *= b; return a / (a + b); }
 
+// LightObject::sampleRayToPoint of light-list entry lm from pos, reading
+// dimensions from dim on and leaving dim light_dims further: an area light
+// (DiffuseAreaLight / TriangleMesh::sample, Core/TriangleMesh.h:108-153) or
+// an environment-type light (uniform sphere direction, pdf 1 / 4pi, no
+// distance limit, Core/Light.h:75-84).
+struct LightPick { f3 dir, incident; float prob, lim; int dst; };
+LightPick sample_light(const Scene& s, const Halton& H, uint64_t hidx, int& dim, f3 pos, int lm) {
+    LightPick r;
+    const int d0 = dim;
+    if (lm < 0) {
+        const float x = H.rand1(hidx, dim);
+        const float y = H.rand1(hidx, dim);
+        r.dir = uniform_sphere(x, y);
+        r.prob = (float)(1.0 / (4.0 * kPi));
+        r.incident = env_eval(s, -1 - lm, r.dir);
+        r.lim = INFINITY;
+        r.dst = -1;
+    } else {
+        const cav_mesh_ref& L = s.meshes[lm];
+        int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
+        float tmp = std::sqrt(H.rand1(hidx, dim));
+        float u = 1 - tmp;
+        float v = tmp * H.rand1(hidx, dim);
+        f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+        f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
+        f3 ln = shading_normal(s, tri, u, v);
+        f3 l2s = pos - lp;
+        float c = std::fabs(dot(ln, normalize(l2s)));
+        r.prob = dot(l2s, l2s) / (c * s.area[lm]);
+        r.dir = normalize(lp - pos);
+        r.lim = length(lp - pos);
+        r.incident = dot(r.dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+        r.dst = lm;
+    }
+    dim = d0 + s.light_dims;
+    return r;
+}
+
 // glm mat4 * vec4 followed by the divide (Utils/MathsCommons.h:57-60)
 f3 apply(const float* m, f3 v) {
     float r[4];
@@ -692,8 +786,7 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
                                int64_t* rays_traced) {
     Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
              surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
-    for (int m = 0; m < nmeshes; ++m)
-        if (meshes[m].is_light) s.lights.push_back(m);
+    init_lights(s);
     s.area.resize(nmeshes);
     ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
     const Halton& H = halton();
@@ -724,7 +817,10 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
             float t, hu, hv;
             ++traced;
             int p = closest(s, ro, rd, t, &hu, &hv);
-            if (p < 0) break;
+            if (p < 0) {  // intersectScene's miss (PathTracingIntegrator.cu:112-116)
+                if (s.env_miss >= 0) result = result + mulv(env_eval(s, s.env_miss, rd), mult);
+                break;
+            }
             const int mesh = tri_mesh[p];
             const cav_mesh_ref& M = meshes[mesh];
             f3 n = shading_normal(s, p, hu, hv);
@@ -749,24 +845,11 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
             int li = H.rand_int(nl, hidx, dim);
             dim += 4;  // rand4, unused by the reference
             {
-                // DiffuseAreaLight::sampleRayToPoint + TriangleMesh::sample
-                const int lm = s.lights[li];
-                const cav_mesh_ref& L = meshes[lm];
-                int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
-                float tmp = std::sqrt(H.rand1(hidx, dim));
-                float u = 1 - tmp;
-                float v = tmp * H.rand1(hidx, dim);
-                f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
-                f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-                f3 ln = shading_normal(s, tri, u, v);
-                f3 l2s = pos - lp;
-                float c = std::fabs(dot(ln, normalize(l2s)));
-                float prob = dot(l2s, l2s) / (c * s.area[lm]);
-                f3 dir = normalize(lp - pos);
-                float lim = length(lp - pos);
-                f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+                const LightPick L = sample_light(s, H, hidx, dim, pos, s.lights[li]);
+                const float prob = L.prob;
+                const f3 dir = L.dir, incident = L.incident;
                 // a shadow ray only matters (and is only counted) when prob is finite
-                if (std::isfinite(prob) && (++traced, visible(s, pos, dir, lim, mesh, lm))) {
+                if (std::isfinite(prob) && (++traced, visible(s, pos, dir, L.lim, mesh, L.dst))) {
                     f3 em = mult * (float)nl / prob;
                     f3 il = to_local(dir, t0, t1, n), ol = to_local(wo, t0, t1, n);
                     float mp = bsdf.pdf(il, ol);
@@ -820,8 +903,7 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
                                  float* film_rgb, float* film_w, int64_t* rays_traced) {
     Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
              surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
-    for (int m = 0; m < nmeshes; ++m)
-        if (meshes[m].is_light) s.lights.push_back(m);
+    init_lights(s);
     s.area.resize(nmeshes);
     ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
     const Halton& H = halton();
@@ -858,23 +940,11 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
             }
             const int li = H.rand_int(nl, hidx, dim);
             dim += 4;  // rand4, unused by the reference
-            const int lm = s.lights[li];
-            const cav_mesh_ref& L = meshes[lm];
-            int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
-            float tmp = std::sqrt(H.rand1(hidx, dim));
-            float u = 1 - tmp;
-            float v = tmp * H.rand1(hidx, dim);
-            f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
-            f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-            f3 ln = shading_normal(s, tri, u, v);
-            f3 l2s = pos - lp;
-            float c = std::fabs(dot(ln, normalize(l2s)));
-            float prob = dot(l2s, l2s) / (c * s.area[lm]);
-            f3 dir = normalize(lp - pos);
-            float lim = length(lp - pos);
-            f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+            const LightPick L = sample_light(s, H, hidx, dim, pos, s.lights[li]);
+            const float prob = L.prob;
+            const f3 dir = L.dir, incident = L.incident;
             ++traced;
-            if (visible(s, pos, dir, lim, mesh, lm)) {
+            if (visible(s, pos, dir, L.lim, mesh, L.dst)) {
                 Bsdf bsdf = surface_bsdf(s, mesh, p, hu, hv);
                 f3 t0, t1;
                 coordinate_system(n, t0, t1);
@@ -883,6 +953,8 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
                 f3 contrib = (mulv(incident, bsdf.eval(il, ol)) * std::fabs(dot(dir, n))) * mult;
                 result = result + contrib;
             }
+        } else if (s.env_miss >= 0) {  // DirectLightingIntegrator.cu:31-36
+            result = result + env_eval(s, s.env_miss, rd);
         }
         res[si * 3 + 0] = result.x;
         res[si * 3 + 1] = result.y;
@@ -1056,8 +1128,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                                  float* film_w, int64_t* rays_traced) {
     Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
              surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
-    for (int m = 0; m < nmeshes; ++m)
-        if (meshes[m].is_light) s.lights.push_back(m);
+    init_lights(s);
     s.area.resize(nmeshes);
     ref_mesh_areas(tris, meshes, nmeshes, s.area.data());
     const Halton& H = halton();
@@ -1112,7 +1183,13 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 float d, hu, hv;
                 ++traced;
                 int p = closest(s, t.ro, t.rd, d, &hu, &hv);
-                if (p >= 0) hits.push_back(Hit{t, p, d, hu, hv});
+                if (p >= 0) {
+                    hits.push_back(Hit{t, p, d, hu, hv});
+                } else if (s.env_miss >= 0) {  // intersectScene's miss (RL .cu:292-296)
+                    f3 a = mulv(env_eval(s, s.env_miss, t.rd), t.mult);
+                    float* acc = &res[(size_t)t.sid * 3];
+                    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
+                }
             }
             q.clear();
             if (hits.empty()) break;
@@ -1161,23 +1238,12 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                 const int li = dist_index(cdf, L, H.rand1(t.sid, t.dim));
                 const float sel = dist_pdf(cdf, li);
                 t.dim += 4;  // rand4
-                const int lm = s.lights[li];
-                const cav_mesh_ref& Lm = meshes[lm];
-                int tri = Lm.first_tri + H.rand_int(Lm.num_tris, t.sid, t.dim);
-                float tmp = std::sqrt(H.rand1(t.sid, t.dim));
-                float u = 1 - tmp;
-                float v = tmp * H.rand1(t.sid, t.dim);
-                f3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
-                f3 lp = (p0 * u + p1 * v) + p2 * (1.f - u - v);
-                f3 ln = shading_normal(s, tri, u, v);
-                f3 l2s = pos - lp;
-                float c = std::fabs(dot(ln, normalize(l2s)));
-                float lpdf = dot(l2s, l2s) / (c * s.area[lm]);
-                f3 dir = normalize(lp - pos);
-                float lim = length(lp - pos);
-                f3 incident = dot(dir, ln) >= 0 ? mk(0, 0, 0) : mk(Lm.emission[0], Lm.emission[1], Lm.emission[2]);
+                const LightPick Lp = sample_light(s, H, t.sid, t.dim, pos, s.lights[li]);
+                float lpdf = Lp.prob;
+                const f3 dir = Lp.dir;
+                f3 incident = Lp.incident;
                 ++traced;
-                if (!(visible(s, pos, dir, lim, mesh, lm) && std::isfinite(lpdf))) {
+                if (!(visible(s, pos, dir, Lp.lim, mesh, Lp.dst) && std::isfinite(lpdf))) {
                     incident = mk(0, 0, 0);
                     lpdf = 1;
                 }
@@ -1368,5 +1434,18 @@ extern "C" void ref_bsdf_sample(const cav_mesh_ref* m, const float* u2, const fl
         f_out[3 * k + 1] = f.y;
         f_out[3 * k + 2] = f.z;
         pdf_out[k] = p;
+    }
+}
+
+// EnvironmentMap::evaluateRay of environment light e of surf for n world directions.
+extern "C" void ref_env_eval(const ref_surface* surf, int e, const float* dirs, int n, float* out) {
+    Scene s{};
+    s.surf = surf;
+    s.env = surf->env;
+    for (int k = 0; k < n; ++k) {
+        const f3 r = env_eval(s, e, mk(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        out[3 * k] = r.x;
+        out[3 * k + 1] = r.y;
+        out[3 * k + 2] = r.z;
     }
 }

@@ -165,6 +165,51 @@ def test_metal_substrate_match_oracle(ctx, integrator):
     scene.close()
 
 
+@pytest.mark.parametrize("integrator", ["path", "directlighting", "rlpath"])
+@pytest.mark.parametrize("kind", ["cornell", "sky"])
+def test_environment_lights_match_oracle(ctx, integrator, kind):
+    """Environment lights: a rotated textured sky seen through the open front
+    of the Cornell box plus a black light, in a custom light order; and a
+    floor under a constant sky with no area light at all (the light sample
+    then takes 2 Halton dimensions instead of 3). Films equal the oracle's."""
+    print(f"[render] environment {kind} {integrator}", flush=True)
+    if kind == "cornell":
+        sc = scenes.cornell_box(seed=12, width=48, height=40, materials="mixed", smooth=True, textured=True,
+                                env=True)
+    else:
+        sc = scenes.sky_plane(width=48, height=40, albedo=0.6, sky=1.3, max_depth=4, black_lights=1)
+    cam = for_scene(sc)
+    depth = 1 if integrator == "rlpath" else sc.max_depth
+    if integrator == "rlpath":
+        rgb, w, rays = oracle.render_rlpath(sc, 4, cam, max_depth=1)
+    else:
+        rgb, w, rays = {"path": oracle.render_path, "directlighting": oracle.render_direct}[integrator](sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()
+
+
+def test_environment_errors(ctx):
+    from cavalry_amd._lib import CavError
+    print("[render] environment errors", flush=True)
+    for order in ([-1, 8], [-1, 8, -2, -2], [-1, 3, -2], [-3, 8, -2]):
+        sc = scenes.cornell_box(seed=1, width=16, height=16, env=True)
+        sc.light_order = np.int32(order)
+        with pytest.raises(CavError, match="light"):
+            R.Scene(ctx, sc)
+    sc = scenes.cornell_box(seed=1, width=16, height=16, env=True)
+    sc.env_background = 2
+    with pytest.raises(CavError, match="env_background"):
+        R.Scene(ctx, sc)
+    sc = scenes.cornell_box(seed=1, width=16, height=16, env=True)
+    sc.env_lights["texture"][0] = 5
+    with pytest.raises(CavError, match="texture"):
+        R.Scene(ctx, sc)
+
+
 def test_texture_errors(ctx):
     from cavalry_amd._lib import CavError
     print("[render] texture errors", flush=True)
