@@ -121,6 +121,7 @@ _SIGS = {
     "cav_pbrt_get": ([P, P, P, P, P], I),
     "cav_pbrt_get_normals": ([P, P, P], I),
     "cav_pbrt_get_textures": ([P, P, P, P], I),
+    "cav_pbrt_get_lights": ([P, P, P, P, P, P], I),
     "cav_pbrt_destroy": ([P], None),
     "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
     "cav_serve_open": ([P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P)], I),

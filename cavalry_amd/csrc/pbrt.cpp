@@ -15,10 +15,12 @@
 //
 // Differences, each reported through cav_render_setup::warnings and the
 // message list: rough glass is rendered as specular glass; texture files are
-// PNG or TGA
-// (8-bit RGBA; the reference also reads EXR and, through its image loader,
-// other formats); non-area lights and the environment map are ignored; the
-// "random" sampler renders with Halton.
+// PNG or TGA (8-bit RGBA; the reference also reads EXR and, through its image
+// loader, other formats); the "random" sampler renders with Halton. Lights
+// follow the reference's list (Scene::lightsHost): area lights and
+// "infinite" environment maps in directive order, and any other LightSource
+// / AreaLightSource type as the black environment light the reference's
+// factory returns for it (Lights/LightObject.h:124-126).
 // Unlike the reference, which exits, every error returns CAV_ERR_PARSE with a
 // message.
 #include <cmath>
@@ -316,6 +318,10 @@ struct Builder {
     // pending AreaLightSource (binds to the next shape, across subsections)
     bool next_is_light = false;
     float light_L[3] = {1, 1, 1};
+    int pending_slot = -1;               // its place in `order`, filled with the mesh id
+    std::vector<cav_env_light> envs;     // environment-type lights
+    std::vector<int32_t> order;          // the light list: mesh id, or -1 - k for envs[k]
+    int env_background = -1;             // the last "infinite" light (Parsing.cu:339-341)
 
     void warn(const std::string& s) {
         ++warnings;
@@ -726,7 +732,8 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
     c.num_tris = ntris;
     c.reverse_orientation = reverse ? 1 : 0;
     c.is_light = 0;
-    if (bd.next_is_light) {
+    const bool light = bd.next_is_light;
+    if (light) {
         c.is_light = 1;
         for (int q = 0; q < 3; ++q) c.emission[q] = bd.light_L[q];
         bd.next_is_light = false;
@@ -737,10 +744,19 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
         bd.warn("shape without triangles skipped");
         return;
     }
+    if (light) {
+        bd.order[(size_t)bd.pending_slot] = (int32_t)bd.meshes.size();
+        bd.pending_slot = -1;
+    }
     bd.meshes.push_back(c);
     bd.mesh_normals.push_back(has_n ? 1 : 0);
     bd.any_normals = bd.any_normals || has_n;
     bd.any_uv = bd.any_uv || has_uv;
+}
+
+void add_env_light(Builder& bd, const cav_env_light& e) {
+    bd.order.push_back(-1 - (int32_t)bd.envs.size());
+    bd.envs.push_back(e);
 }
 
 // Parsing.cu:272-388: one XBegin ... XEnd section
@@ -774,15 +790,47 @@ void parse_section(Buf& b, State st, const std::string& base, Builder& bd) {
                 const auto& v = d.params.list(key);
                 for (int q = 0; q < 3; ++q) L[q] = v.size() >= 3 ? v[q] : (v.empty() ? 1.f : v[0]);
             }
-            if (d.name == "diffuse") {
+            if (d.name == "diffuse") {  // Parsing.cu:320-333: listed now, bound to the next shape
+                if (bd.pending_slot >= 0) {
+                    bd.warn("AreaLightSource without a shape dropped");
+                    bd.order.erase(bd.order.begin() + bd.pending_slot);
+                }
                 bd.next_is_light = true;
                 for (int q = 0; q < 3; ++q) bd.light_L[q] = L[q];
+                bd.pending_slot = (int)bd.order.size();
+                bd.order.push_back(INT32_MIN);
             } else {
-                bd.warn("area light '" + d.name + "' is not supported");
+                bd.warn("area light '" + d.name + "' counts as a black light, as in the reference");
+                add_env_light(bd, cav_env_light{{0.f, 0.f, 0.f}, -1, {1, 0, 0, 0, 1, 0, 0, 0, 1}});
             }
-        } else if (kw == "LightSource") {
+        } else if (kw == "LightSource") {  // Parsing.cu:335-342, Lights/LightObject.h:89-126
             const ObjectDef d = read_object(b);
-            bd.warn("light source '" + d.name + "' is not supported (area lights only)");
+            cav_env_light e{{1.f, 1.f, 1.f}, -1, {1, 0, 0, 0, 1, 0, 0, 0, 1}};
+            const char* key = d.params.has_list("L") ? "L" : (d.params.has_list("l") ? "l" : nullptr);
+            if (d.name == "infinite" && d.params.has_list("scale")) key = "scale";
+            if (key) {
+                const auto& v = d.params.list(key);
+                for (int q = 0; q < 3; ++q) e.color[q] = v.size() >= 3 ? v[q] : (v.empty() ? 1.f : v[0]);
+            }
+            if (d.name == "infinite") {
+                if (d.params.has_string("mapname")) {
+                    cav::Image img;
+                    std::string err;
+                    if (!cav::read_texture_file(join(base, d.params.str("mapname")), img, err)) throw ParseError(err);
+                    e.texture = (int32_t)bd.images.size();
+                    bd.images.push_back(std::move(img));
+                }
+                // glm::mat3(inverse(lightToWorld)), as a row-major 3x3
+                float inv[16];
+                if (!invert(st.t.m, inv)) throw ParseError("infinite light: singular transform");
+                for (int r = 0; r < 3; ++r)
+                    for (int cc = 0; cc < 3; ++cc) e.world_to_light[r * 3 + cc] = inv[cc * 4 + r];
+                bd.env_background = (int)bd.envs.size();
+            } else {
+                bd.warn("light source '" + d.name + "' counts as a black light, as in the reference");
+                e.color[0] = e.color[1] = e.color[2] = 0.f;
+            }
+            add_env_light(bd, e);
         } else if (read_transform(b, st.t)) {
         } else if (kw == "MakeNamedMaterial") {
             const ObjectDef d = read_object(b);
@@ -833,6 +881,9 @@ struct cav_pbrt {
     std::vector<float> uv;             // empty when no mesh has texture coordinates
     std::vector<cav::Image> images;
     std::vector<cav_texture> textures; // views of images
+    std::vector<cav_env_light> envs;
+    std::vector<int32_t> order;        // the light list (cav_scene_desc::light_order)
+    int env_background = -1;
     std::vector<cav_mesh> meshes;
     cav_render_setup setup{};
     std::string messages;
@@ -877,6 +928,10 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** ou
         State st{identity(), nullptr};
         parse_section(b, st, dir_of(path), bd);
         if (bd.meshes.empty()) throw ParseError("scene has no shapes");
+        if (bd.pending_slot >= 0) {
+            bd.warn("AreaLightSource without a shape dropped");
+            bd.order.erase(bd.order.begin() + bd.pending_slot);
+        }
         std::unique_ptr<cav_pbrt> r(new cav_pbrt());
         cav_render_setup& s = r->setup;
         s.width = (int)su.film.params.num("xresolution");
@@ -902,7 +957,7 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** ou
         s.warnings = bd.warnings;
         s.ntris = (int64_t)bd.soup.size() / 9;
         s.nmeshes = (int)bd.meshes.size();
-        for (const cav_mesh& m : bd.meshes) s.nlights += m.is_light ? 1 : 0;
+        s.nlights = (int32_t)bd.order.size();
         r->soup.swap(bd.soup);
         r->meshes.swap(bd.meshes);
         if (bd.any_normals) {
@@ -910,6 +965,9 @@ int cav_pbrt_parse(const char* path, const cav_pbrt_overrides* ov, cav_pbrt** ou
             r->mesh_normals.swap(bd.mesh_normals);
         }
         if (bd.any_uv) r->uv.swap(bd.uv);
+        r->envs.swap(bd.envs);
+        r->order.swap(bd.order);
+        r->env_background = bd.env_background;
         r->images.swap(bd.images);
         for (const cav::Image& im : r->images) r->textures.push_back(cav_texture{im.rgba.data(), im.w, im.h});
         for (const std::string& m : bd.messages) r->messages += m + "\n";
@@ -947,6 +1005,17 @@ int cav_pbrt_get_textures(const cav_pbrt* p, const float** tri_uv, const cav_tex
     return CAV_OK;
 }
 
+int cav_pbrt_get_lights(const cav_pbrt* p, const cav_env_light** env_lights, int32_t* nenv,
+                        const int32_t** light_order, int32_t* nlight_order, int32_t* env_background) {
+    if (!p) return CAV_ERR_INVALID;
+    if (env_lights) *env_lights = p->envs.empty() ? nullptr : p->envs.data();
+    if (nenv) *nenv = (int32_t)p->envs.size();
+    if (light_order) *light_order = p->order.data();
+    if (nlight_order) *nlight_order = (int32_t)p->order.size();
+    if (env_background) *env_background = p->env_background;
+    return CAV_OK;
+}
+
 void cav_pbrt_destroy(cav_pbrt* p) { delete p; }
 
 int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_overrides* ov, cav_scene** out,
@@ -970,6 +1039,11 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
     d.tri_uv = p->uv.empty() ? nullptr : p->uv.data();
     d.textures = p->textures.empty() ? nullptr : p->textures.data();
     d.ntextures = (int32_t)p->textures.size();
+    d.env_lights = p->envs.empty() ? nullptr : p->envs.data();
+    d.nenv = (int32_t)p->envs.size();
+    d.light_order = p->order.data();
+    d.nlight_order = (int32_t)p->order.size();
+    d.env_background = p->env_background;
     rc = cav_scene_create_desc(ctx, &d, out);
     if (rc == CAV_OK && setup) *setup = s;
     cav_pbrt_destroy(p);

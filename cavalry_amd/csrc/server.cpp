@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -157,6 +158,7 @@ struct cav_server {
     std::chrono::steady_clock::time_point t0;
     HaltonDev halton{};
     std::mutex mu;
+    std::atomic<int> api_waiting{0};  // API calls waiting for mu
     std::condition_variable cv, done_cv;
     bool stop = false;
     int64_t next_id = 1;
@@ -173,6 +175,16 @@ struct cav_server {
     int64_t rl_rays[2] = {0, 0};
     std::string error;
 };
+
+namespace {
+// The lock an API call takes. It announces itself first: the worker re-takes
+// mu right after each step, and std::mutex is not fair, so without this a
+// submitter could wait until the pools drain.
+struct ApiLock {
+    std::unique_lock<std::mutex> lk;
+    explicit ApiLock(cav_server* sv) : lk((sv->api_waiting.fetch_add(1), sv->mu)) { sv->api_waiting.fetch_sub(1); }
+};
+}  // namespace
 
 namespace {
 
@@ -324,22 +336,31 @@ int pool_step(cav_server* sv, Pool* p) {
         if (hipEventSynchronize(p->ring[s % ring]) != hipSuccess) return CAV_ERR_HIP;
         p->done_upto = s + 1;
     }
-    // admission: round-robin quantum over the scene's pending requests
+    // admission: round-robin, one chunk (at most the quantum) per request in
+    // turn; a request that got one goes to the back, so room freed at the
+    // next step goes to the requests after it
     Chunk* hc = p->h_chunks + (size_t)(p->step % ring) * kSlots;
     int nch = 0;
     int64_t total = 0;
     const int64_t cap = p->dev.cap;
-    for (size_t k = 0; k < p->pending.size() && nch < kSlots;) {
-        Request* r = p->pending[k];
+    const size_t turns = p->pending.size();
+    for (size_t t = 0; t < turns && nch < kSlots; ++t) {
+        Request* r = p->pending.front();
+        p->pending.pop_front();
         if (r->cancel && !r->active) {
             r->status = CAV_SERVE_CANCELLED;
-            p->pending.erase(p->pending.begin() + (std::ptrdiff_t)k);
             sv->done_cv.notify_all();
             continue;
         }
         const int64_t room = cap - p->reserved;
-        if (room <= 0) break;
-        if (!r->active && !activate(sv, p, r)) { ++k; continue; }
+        if (room <= 0) {
+            p->pending.push_front(r);
+            break;
+        }
+        if (!r->active && !activate(sv, p, r)) {
+            p->pending.push_back(r);
+            continue;
+        }
         const int64_t n = std::min({room, sv->cfg.chunk_samples, r->nsamp - r->next_sample});
         if (r->t_start < 0) {
             r->t_start = now_ms(sv->t0);
@@ -353,8 +374,7 @@ int pool_step(cav_server* sv, Pool* p) {
         p->reserved += n;
         const int64_t last = r->prm.integrator == CAV_INTEGRATOR_DIRECT ? 0 : r->prm.max_depth - 1;
         p->live.push_back({LiveChunk{r, p->step + last}, n});
-        if (r->next_sample == r->nsamp) p->pending.erase(p->pending.begin() + (std::ptrdiff_t)k);
-        else ++k;
+        if (r->next_sample != r->nsamp) p->pending.push_back(r);
     }
     if (nch) {
         Chunk* dc = p->d_chunks + (size_t)(p->step % ring) * kSlots;
@@ -459,10 +479,12 @@ void worker_main(cav_server* sv) {
         for (auto& kv : sv->reqs) finishing = finishing || kv.second->finishing;
         if (sv->stop && !busy && !finishing && sv->incoming.empty()) break;
         if (busy) {
-            // let submitters and pollers in between steps
-            lk.unlock();
-            std::this_thread::yield();
-            lk.lock();
+            // let waiting submitters and pollers in between steps
+            if (sv->api_waiting.load() > 0) {
+                lk.unlock();
+                while (sv->api_waiting.load() > 0) std::this_thread::yield();
+                lk.lock();
+            }
         } else if (finishing) {
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(100));
@@ -582,7 +604,7 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
     r->out_w = film_weight;
     r->npix = npix;
     r->nsamp = npix * prm->spp;
-    std::lock_guard<std::mutex> lk(sv->mu);
+    ApiLock lock(sv);
     if (sv->stop || !sv->error.empty()) return fail(sv->ctx, CAV_ERR_INVALID, "server is closing or failed");
     if (prm->integrator != CAV_INTEGRATOR_RLPATH) {
         const size_t need = 20 * (size_t)r->nsamp + 160 * (size_t)npix + 5 * 256;
@@ -602,7 +624,8 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
 
 int cav_serve_poll(cav_server* sv, int64_t id, int32_t wait, cav_serve_result* out) {
     if (!sv || !out) return CAV_ERR_INVALID;
-    std::unique_lock<std::mutex> lk(sv->mu);
+    ApiLock lock(sv);
+    std::unique_lock<std::mutex>& lk = lock.lk;
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return CAV_ERR_NOT_FOUND;
     Request* r = it->second.get();
@@ -623,7 +646,7 @@ int cav_serve_poll(cav_server* sv, int64_t id, int32_t wait, cav_serve_result* o
 
 int cav_serve_cancel(cav_server* sv, int64_t id) {
     if (!sv) return CAV_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(sv->mu);
+    ApiLock lock(sv);
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return CAV_ERR_NOT_FOUND;
     it->second->cancel = true;
@@ -634,7 +657,7 @@ int cav_serve_cancel(cav_server* sv, int64_t id) {
 int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
     if (!sv) return CAV_ERR_INVALID;
     {
-        std::lock_guard<std::mutex> lk(sv->mu);
+        ApiLock lock(sv);
         sv->stop = true;
         sv->cv.notify_all();
     }

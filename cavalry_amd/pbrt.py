@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import Camera, check, lib
-from .scenes import GLASS, MATTE, MESH_DTYPE, METAL, MIRROR, PLASTIC, SUBSTRATE, SceneArrays
+from .scenes import ENV_DTYPE, GLASS, MATTE, MESH_DTYPE, METAL, MIRROR, PLASTIC, SUBSTRATE, SceneArrays
 
 
 class Overrides(ctypes.Structure):
@@ -75,7 +75,17 @@ def _parse(path: str, **kw):
             for t in views:
                 raw = (ctypes.c_uint8 * (4 * t.width * t.height)).from_address(t.rgba)
                 textures.append(np.frombuffer(bytes(raw), np.uint8).reshape(t.height, t.width, 4).copy())
-        return tris, meshes, st, messages, normals, flags, uv, textures
+        eptr, nenv, optr, nord, bg = (ctypes.c_void_p(), ctypes.c_int32(), ctypes.c_void_p(), ctypes.c_int32(),
+                                      ctypes.c_int32())
+        check(None, lib().cav_pbrt_get_lights(h, ctypes.byref(eptr), ctypes.byref(nenv), ctypes.byref(optr),
+                                              ctypes.byref(nord), ctypes.byref(bg)))
+        envs = np.zeros(0, ENV_DTYPE)
+        if nenv.value:
+            raw = (ctypes.c_char * (ENV_DTYPE.itemsize * nenv.value)).from_address(eptr.value)
+            envs = np.frombuffer(bytes(raw), ENV_DTYPE).copy()
+        order = np.ctypeslib.as_array((ctypes.c_int32 * nord.value).from_address(optr.value)).copy() \
+            if nord.value else np.zeros(0, np.int32)
+        return tris, meshes, st, messages, normals, flags, uv, textures, envs, order, int(bg.value)
     finally:
         lib().cav_pbrt_destroy(h)
 
@@ -100,6 +110,13 @@ def parse_textures(path: str, **kw):
     return _parse(path, **kw)[6:8]
 
 
+def parse_lights(path: str, **kw):
+    """The light list of a parsed scene: (environment lights (ENV_DTYPE),
+    light order (mesh id >= 0 or -1 - k for environment light k),
+    background environment light or -1)."""
+    return _parse(path, **kw)[8:11]
+
+
 def _f(x) -> str:
     """float32 value printed so that strtof reads it back exactly."""
     return np.format_float_positional(np.float32(x), unique=True, trim="-")
@@ -122,7 +139,11 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
     """Writes a generated scene as a .pbrt file the parser reads back into the
     same triangles, meshes, camera, vertex normals and texture coordinates.
     Textures go to PNG files next to it (<name>_tex<k>.png), sRGB-encoded: the
-    loader's inverse curve returns each byte to within one or two units."""
+    loader's inverse curve returns each byte to within one or two units.
+    Environment lights become "infinite" LightSources (a Transform for their
+    rotation; black ones a "point" light, which the reader lists as black)
+    placed so the file order is the light order; the reader numbers them in
+    that order."""
     out = [
         "# generated scene",
         f"LookAt {' '.join(_f(x) for x in arr.eye)} {' '.join(_f(x) for x in arr.center)} "
@@ -136,13 +157,51 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         "WorldBegin",
     ]
     stem = os.path.splitext(os.path.basename(path))[0]
+    envs = getattr(arr, "env_lights", None)
+    envs = [] if envs is None else list(envs)
+    order = getattr(arr, "light_order", None)
+    if order is None:
+        order = [i for i, m in enumerate(arr.meshes) if m["is_light"]] + [-1 - k for k in range(len(envs))]
+    order = [int(e) for e in order]
+    area_order = [e for e in order if e >= 0]
+    if area_order != sorted(area_order):
+        raise ValueError("area lights must appear in the light order in mesh order to have a .pbrt form")
+    # the parser numbers environment lights in file order, which is light order,
+    # and makes the last "infinite" one (a lit one; black ones are written as
+    # another type) the background
+    lit = [-1 - e for e in order if e < 0 and (envs[-1 - e]["color"].any() or envs[-1 - e]["texture"] >= 0)]
+    if int(getattr(arr, "env_background", -1)) != (lit[-1] if lit else -1):
+        raise ValueError("the background must be the last lit environment light in light order")
     for k, tex in enumerate(getattr(arr, "textures", None) or []):
         name = f"{stem}_tex{k}.png"
         img = np.ascontiguousarray(_srgb8(tex[::-1, :, :3]))  # files list rows top first
         check(None, lib().cav_write_png(os.path.join(os.path.dirname(os.path.abspath(path)), name).encode(),
                                         img.ctypes.data_as(ctypes.c_void_p), tex.shape[1], tex.shape[0]))
-        out.append(f'Texture "tex{k}" "spectrum" "imagemap" "string filename" [ "{name}" ]')
+        if (arr.meshes["texture"] == k).any():  # a map of a light is read through its own "mapname"
+            out.append(f'Texture "tex{k}" "spectrum" "imagemap" "string filename" [ "{name}" ]')
+    pos = 0  # next entry of the light order to write
+
+    def env_lines(k):
+        e = envs[k]
+        if not (e["color"].any() or e["texture"] >= 0):
+            return ['LightSource "point" "rgb L" [ 0 0 0 ]']  # any unknown type is a black light
+        lines = ["AttributeBegin"]
+        R = np.asarray(e["world_to_light"], np.float64).reshape(3, 3)
+        if not np.allclose(R, np.eye(3)):
+            L = np.eye(4)
+            L[:3, :3] = np.linalg.inv(R)  # light to world; Transform lists it column by column
+            lines.append(f"  Transform {_list(L.T)}")
+        tex = f' "string mapname" [ "{stem}_tex{int(e["texture"])}.png" ]' if e["texture"] >= 0 else ""
+        lines.append(f'  LightSource "infinite" "rgb L" {_list(e["color"])}{tex}')
+        lines.append("AttributeEnd")
+        return lines
+
     for mi, m in enumerate(arr.meshes):
+        if m["is_light"]:
+            while order[pos] < 0:
+                out += env_lines(-1 - order[pos])
+                pos += 1
+            pos += 1
         out.append("AttributeBegin")
         mat = int(m["material"])
         tex = int(m["texture"])
@@ -187,6 +246,8 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
             normals += f' "float uv" {_list(uv[m["first_tri"]:m["first_tri"] + m["num_tris"]])}'
         out.append(f'  Shape "trianglemesh" "integer indices" [ {idx} ] "point P" {_list(t)}{normals}')
         out.append("AttributeEnd")
+    for e in order[pos:]:
+        out += env_lines(-1 - e)
     out.append("WorldEnd")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")

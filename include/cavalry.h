@@ -321,6 +321,12 @@ int cav_pbrt_get_normals(const cav_pbrt* p, const float** tri_normals, const uin
  * rows bottom-up, sRGB curve removed). Owned by the parsed scene. */
 int cav_pbrt_get_textures(const cav_pbrt* p, const float** tri_uv, const cav_texture** textures,
                           int32_t* ntextures);
+/* The light list of a parsed scene (cav_scene_desc's env_lights,
+ * light_order, env_background): "infinite" LightSources with their colour,
+ * map and rotation; other LightSource / AreaLightSource types as black
+ * lights; all in directive order. Owned by the parsed scene. */
+int cav_pbrt_get_lights(const cav_pbrt* p, const cav_env_light** env_lights, int32_t* nenv,
+                        const int32_t** light_order, int32_t* nlight_order, int32_t* env_background);
 void cav_pbrt_destroy(cav_pbrt* p);
 /* Replaces readRenderSetup + Scene::prepareForRender (main.cu:10-15):
  * parse, then cav_scene_create with the setup's BVH rounds and Morton width. */

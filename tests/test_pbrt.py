@@ -532,3 +532,52 @@ def test_writer_round_trip_textures(tmp_path):
     assert len(textures) == len(sc.textures)
     for a, b in zip(textures, sc.textures):
         assert np.abs(a[..., :3].astype(int) - b[..., :3]).max() <= 2   # sRGB encode, then decode
+
+
+def test_light_sources_and_order(tmp_path):
+    """LightSource "infinite" (L, scale, mapname, the inverse of its
+    transform), other types as black lights, AreaLightSource listed where it
+    appears (Parsing.cu:320-342, Lights/LightObject.h:89-126)."""
+    _png(tmp_path / "sky.png", np.full((2, 4, 3), 200, np.uint8), 2)
+    body = f"""WorldBegin
+LightSource "point" "rgb I" [ 5 5 5 ]
+AttributeBegin
+  AreaLightSource "diffuse" "rgb L" [ 3 3 3 ]
+  Shape "trianglemesh" {QUAD}
+AttributeEnd
+Shape "trianglemesh" {QUAD}
+AttributeBegin
+  Rotate 90 0 1 0
+  LightSource "infinite" "rgb L" [ 0.5 0.6 0.7 ] "string mapname" "sky.png"
+AttributeEnd
+LightSource "infinite" "rgb scale" [ 2 2 2 ]
+WorldEnd
+"""
+    p = write(tmp_path, body)
+    envs, order, background = pbrt.parse_lights(p)
+    _, m, st, msgs = pbrt.parse(p)
+    assert list(order) == [-1, 0, -2, -3] and background == 2 and st.nlights == 4
+    np.testing.assert_array_equal(envs["color"], np.float32([[0, 0, 0], [0.5, 0.6, 0.7], [2, 2, 2]]))
+    assert list(envs["texture"]) == [-1, 0, -1]
+    # world -> light = inverse of a 90-degree turn about y, row-major
+    np.testing.assert_allclose(envs["world_to_light"][1].reshape(3, 3), scenes.rotation_y(90).T, atol=1e-6)
+    np.testing.assert_array_equal(envs["world_to_light"][2], np.eye(3, dtype=np.float32).ravel())
+    assert list(m["is_light"]) == [1, 0] and any("point" in s for s in msgs)
+    _, textures = pbrt.parse_textures(p)
+    assert len(textures) == 1 and textures[0].shape == (2, 4, 4)
+
+
+def test_writer_round_trip_lights(tmp_path):
+    sc = scenes.cornell_box(seed=6, width=40, height=30, materials="mixed", env=True)
+    p = str(tmp_path / "e.pbrt")
+    pbrt.write(sc, p)
+    envs, order, background = pbrt.parse_lights(p)
+    np.testing.assert_array_equal(order, sc.light_order)
+    assert background == sc.env_background
+    np.testing.assert_array_equal(envs["color"], sc.env_lights["color"])
+    np.testing.assert_allclose(envs["world_to_light"], sc.env_lights["world_to_light"], atol=1e-6)
+    # the sky map is read again through "mapname": same image, its own index
+    _, textures = pbrt.parse_textures(p)
+    assert envs["texture"][1] == -1 and len(textures) == 1
+    got, want = textures[envs["texture"][0]], sc.textures[sc.env_lights["texture"][0]]
+    assert np.abs(got[..., :3].astype(int) - want[..., :3]).max() <= 2
