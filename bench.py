@@ -187,7 +187,11 @@ def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
     worst_rl = 0.0
     for r, a, b in zip(trace, plain.films, served.films):
         if r.integrator == "rlpath":
-            ia, ib = a.image.astype(np.float64), b.image.astype(np.float64)
+            # the learnt tables take proposals through float atomics, so an rlpath
+            # film is not bit-reproducible; compared in the display range so a
+            # single firefly does not dominate
+            ia = np.clip(a.image.astype(np.float64), 0, 1)
+            ib = np.clip(b.image.astype(np.float64), 0, 1)
             worst_rl = max(worst_rl, float(np.mean((ib - ia) ** 2 / (ia * ia + 1e-2))))
         else:
             same = same and np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
@@ -204,7 +208,7 @@ def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
         "served": s,
         "speedup_mrays": round(s["mrays_s"] / p["mrays_s"], 3),
         "path_films_identical": bool(same),
-        "rlpath_max_rel_mse_vs_plain": worst_rl,
+        "rlpath_max_rel_mse_vs_plain_ldr": worst_rl,
         "server_steps": st["steps"],
     }
 
@@ -264,12 +268,14 @@ def main() -> int:
     log("[bench] first build (allocates)")
     bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
     if args.probe:
-        # the copy calibrates HBM byte counters: it moves exactly 2^28 B each way
-        src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+        # an elementwise kernel (a copy may go to a DMA engine, which the
+        # counters do not see) calibrates the HBM byte counters: it reads and
+        # writes exactly 2^28 B
+        src = torch.zeros(1 << 28, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.add(src, 1, out=dst)
         torch.cuda.synchronize()
-        log("[bench] probe done: one build, one 2^28-byte device copy")
+        log("[bench] probe done: one build, one 2^28-byte elementwise kernel")
         return 0
 
     def step(record: bool):

@@ -8,6 +8,7 @@ float32[N, 9] soup and rays as float32[R, 6].
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import weakref
 
@@ -27,6 +28,17 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all() -> None:
+    # while the HIP runtime is still up: interpreter teardown may otherwise
+    # free device objects after it
+    for c in list(_LIVE):
+        c.close()
+
+
 class Context:
     """One device, one HIP stream (cav_ctx). Objects made on it (BVHs,
     scenes, servers) are closed before the context itself: the library
@@ -39,6 +51,7 @@ class Context:
             raise _lib.CavError(rc, f"cav_ctx_create(device={device}) failed")
         self.handle = h
         self._children = weakref.WeakSet()
+        _LIVE.add(self)
 
     def own(self, obj) -> None:
         """Registers an object whose close() must run before this context's."""

@@ -329,7 +329,16 @@ struct Bsdf {
             return r;
         }
         v3 r = mk3(0, 0, 0);
-        if (srefract(o, wi)) { p = 1; r = strans(wi, o); }
+        if (srefract(o, wi)) {
+            p = 1;
+            r = strans(wi, o);
+        } else {
+            // total internal reflection: the reference leaves the direction and
+            // pdf unset (BSDFs/Specular.h:70-82); defined here as the mirror
+            // direction with pdf 0 (the value is 0 either way)
+            wi = mk3(-o.x, -o.y, o.z);
+            p = 0;
+        }
         p *= (1.f - pr);
         return r;
     }

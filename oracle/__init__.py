@@ -198,26 +198,41 @@ def tex_fetch(rgba, u: float, v: float) -> np.ndarray:
     return out
 
 
-def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30):
+def render_path(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30, crop=None,
+                max_depth: int | None = None, samples: bool = False, bvh=None):
     """`path` integrator on the CPU. Returns (film_rgb [H,W,3], film_w [H,W],
-    rays_traced). camera is a cavalry_amd Camera (cav_camera)."""
+    rays_traced). camera is a cavalry_amd Camera (cav_camera); the image size
+    is the camera's. crop = (x0, y0, x1, y1) renders only those pixels of the
+    full image (same sample indices, so the crop equals that part of the full
+    film) and returns films of the crop's size; samples=True adds, per crop
+    pixel and sample, radiance and film position [h, w, spp, 5]. bvh: node
+    arrays from an earlier lbvh_build + treelet_optimize of the same triangles."""
     tris = np.ascontiguousarray(scene.tris, np.float32)
-    _, _, nd = lbvh_build(tris, morton_bits)
-    if bvh_rounds:
-        treelet_optimize(nd, bvh_rounds)
-    w, h = scene.width, scene.height
-    rgb = np.zeros((h, w, 3), np.float32)
-    wt = np.zeros((h, w), np.float32)
+    if bvh is None:
+        _, _, nd = lbvh_build(tris, morton_bits)
+        if bvh_rounds:
+            treelet_optimize(nd, bvh_rounds)
+    else:
+        nd = bvh
+    w, h = int(camera.width), int(camera.height)
+    x0, y0, x1, y1 = crop if crop is not None else (0, 0, w, h)
+    rgb = np.zeros((y1 - y0, x1 - x0, 3), np.float32)
+    wt = np.zeros((y1 - y0, x1 - x0), np.float32)
+    smp = np.zeros((y1 - y0, x1 - x0, spp, 5), np.float32) if samples else None
     rays = np.zeros(1, np.int64)
     cam16 = np.ctypeslib.as_array(camera.cam_to_world).astype(np.float32)
     tri_mesh = scene.tri_mesh
-    rc = lib().ref_render_path(
+    rect = np.int32([x0, y0, x1, y1])
+    depth = scene.max_depth if max_depth is None else max_depth
+    rc = lib().ref_render_path_crop(
         _p(tris), ctypes.c_int(len(tris)), _p(tri_mesh), _p(scene.meshes), ctypes.c_int(len(scene.meshes)),
         _p(nd.box), _p(nd.left), _p(nd.right), _p(nd.prim), _surface(scene), _p(cam16),
         ctypes.c_float(camera.depth), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(spp),
-        ctypes.c_int(scene.max_depth), _p(rgb), _p(wt), _p(rays))
+        ctypes.c_int(depth), _p(rect), _p(rgb), _p(wt), _p(smp) if samples else None, _p(rays))
     if rc != 0:
         raise ValueError(f"ref_render_path failed ({rc})")
+    if samples:
+        return rgb, wt, int(rays[0]), smp
     return rgb, wt, int(rays[0])
 
 

@@ -70,6 +70,11 @@ int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh, const
                     int nmeshes, const float* box, const int32_t* left, const int32_t* right,
                     const int32_t* prim, const ref_surface* surf, const float* cam16, float cam_depth, int width, int height,
                     int spp, int max_depth, float* film_rgb, float* film_w, int64_t* rays_traced);
+int ref_render_path_crop(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
+                         int nmeshes, const float* box, const int32_t* left, const int32_t* right,
+                         const int32_t* prim, const ref_surface* surf, const float* cam16, float cam_depth,
+                         int width, int height, int spp, int max_depth, const int32_t* crop, float* film_rgb,
+                         float* film_w, float* samples_out, int64_t* rays_traced);
 int ref_render_direct(const float* tris, int ntris, const int32_t* tri_mesh, const cav_mesh_ref* meshes,
                       int nmeshes, const float* box, const int32_t* left, const int32_t* right,
                       const int32_t* prim, const ref_surface* surf, const float* cam16, float cam_depth, int width, int height,

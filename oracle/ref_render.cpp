@@ -407,6 +407,11 @@ struct Bsdf {
                 if (spec_refract(o, wi)) {
                     pdfv = 1;
                     r = spec_trans(wi, o);
+                } else {
+                    // total internal reflection: direction and pdf are unset in the
+                    // reference (BSDFs/Specular.h:70-82); the mirror direction, pdf 0
+                    wi = mk(-o.x, -o.y, o.z);
+                    pdfv = 0;
                 }
                 pdfv *= (1.f - pr);
                 return r;
@@ -745,10 +750,13 @@ extern "C" void ref_mesh_areas(const float* tris, const cav_mesh_ref* meshes, in
 // contribute the sum (in sample order) of their samples whose filter support
 // covers the pixel; non-finite samples are skipped
 void gather_film(int width, int height, int spp, const std::vector<float>& res, const std::vector<float>& sx,
-                 const std::vector<float>& sy, float* film_rgb, float* film_w) {
+                 const std::vector<float>& sy, float* film_rgb, float* film_w, int x0, int y0, int x1, int y1) {
+    // pixels of [x0, x1) x [y0, y1), stored row by row from (x0, y0)
+    const int cw = x1 - x0;
 #pragma omp parallel for schedule(static)
-    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
-        const int x = (int)(pix % width), y = (int)(pix / width);
+    for (int64_t cpix = 0; cpix < (int64_t)cw * (y1 - y0); ++cpix) {
+        const int x = x0 + (int)(cpix % cw), y = y0 + (int)(cpix / cw);
+        const int64_t pix = cpix;
         float r = 0, g = 0, b = 0, w = 0;
         for (int py = y - 1; py <= y + 1; ++py)
             for (int px = x - 1; px <= x + 1; ++px) {
@@ -778,12 +786,17 @@ void gather_film(int width, int height, int spp, const std::vector<float>& res, 
 // weight of the box-filtered film (Core/Film.h:25-49), accumulated in a fixed
 // order: for each pixel, samples of pixels (y-1..y+1, x-1..x+1) in row-major
 // pixel order, then sample order.
-extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh,
-                               const cav_mesh_ref* meshes, int nmeshes, const float* box,
-                               const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
-                               const float* cam16, float cam_depth, int width, int height,
-                               int spp, int max_depth, float* film_rgb, float* film_w,
-                               int64_t* rays_traced) {
+// The same restricted to the pixels of crop = {x0, y0, x1, y1} of the full
+// image: only the samples that reach them (their pixels and the ring around)
+// are traced, with the full image's sample indices, so the crop equals that
+// part of the full film. samples_out (optional): per crop pixel and sample,
+// radiance r, g, b and film position x, y.
+extern "C" int ref_render_path_crop(const float* tris, int ntris, const int32_t* tri_mesh,
+                                    const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                                    const int32_t* left, const int32_t* right, const int32_t* prim,
+                                    const ref_surface* surf, const float* cam16, float cam_depth, int width,
+                                    int height, int spp, int max_depth, const int32_t* crop, float* film_rgb,
+                                    float* film_w, float* samples_out, int64_t* rays_traced) {
     Scene s{tris, ntris, meshes, nmeshes, tri_mesh, {}, {}, box, left, right, prim,
              surf ? surf->tri_normals : nullptr, surf ? surf->mesh_normals : nullptr, surf};
     init_lights(s);
@@ -794,9 +807,16 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
     std::vector<float> res((size_t)nsamp * 3), sx((size_t)nsamp), sy((size_t)nsamp);
     int64_t traced = 0;
     const int nl = (int)s.lights.size();
+    const int cx0 = crop[0], cy0 = crop[1], cx1 = crop[2], cy1 = crop[3];
+    if (cx0 < 0 || cy0 < 0 || cx1 > width || cy1 > height || cx0 >= cx1 || cy0 >= cy1) return -1;
+    const int ex0 = std::max(0, cx0 - 1), ey0 = std::max(0, cy0 - 1);
+    const int ex1 = std::min(width, cx1 + 1), ey1 = std::min(height, cy1 + 1);
+    const int64_t ew = ex1 - ex0, nwork = ew * (ey1 - ey0) * spp;
 
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : traced)
-    for (int64_t si = 0; si < nsamp; ++si) {
+    for (int64_t j = 0; j < nwork; ++j) {
+        const int64_t wp = j / spp;
+        const int64_t si = ((int64_t)(ey0 + wp / ew) * width + (ex0 + wp % ew)) * spp + j % spp;
         const int64_t pix = si / spp;
         const int x = (int)(pix % width), y = (int)(pix / width);
         const uint64_t hidx = (uint64_t)pix * spp + (uint64_t)(si % spp);  // Core/Sampler.cu:29-51
@@ -886,9 +906,32 @@ extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_
         res[si * 3 + 2] = result.z;
     }
 
-    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w, cx0, cy0, cx1, cy1);
+    if (samples_out) {
+        const int cw = cx1 - cx0;
+        for (int y = cy0; y < cy1; ++y)
+            for (int x = cx0; x < cx1; ++x)
+                for (int k = 0; k < spp; ++k) {
+                    const int64_t si = ((int64_t)y * width + x) * spp + k;
+                    float* o = samples_out + (((int64_t)(y - cy0) * cw + (x - cx0)) * spp + k) * 5;
+                    o[0] = res[si * 3]; o[1] = res[si * 3 + 1]; o[2] = res[si * 3 + 2];
+                    o[3] = sx[si]; o[4] = sy[si];
+                }
+    }
     if (rays_traced) *rays_traced = traced;
     return 0;
+}
+
+extern "C" int ref_render_path(const float* tris, int ntris, const int32_t* tri_mesh,
+                               const cav_mesh_ref* meshes, int nmeshes, const float* box,
+                               const int32_t* left, const int32_t* right, const int32_t* prim, const ref_surface* surf,
+                               const float* cam16, float cam_depth, int width, int height,
+                               int spp, int max_depth, float* film_rgb, float* film_w,
+                               int64_t* rays_traced) {
+    const int32_t full[4] = {0, 0, width, height};
+    return ref_render_path_crop(tris, ntris, tri_mesh, meshes, nmeshes, box, left, right, prim, surf, cam16,
+                                cam_depth, width, height, spp, max_depth, full, film_rgb, film_w, nullptr,
+                                rays_traced);
 }
 
 
@@ -960,7 +1003,7 @@ extern "C" int ref_render_direct(const float* tris, int ntris, const int32_t* tr
         res[si * 3 + 1] = result.y;
         res[si * 3 + 2] = result.z;
     }
-    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w, 0, 0, width, height);
     if (rays_traced) *rays_traced = traced;
     return 0;
 }
@@ -1385,7 +1428,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
         }
     }  // rounds
 
-    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w);
+    gather_film(width, height, spp, res, sx, sy, film_rgb, film_w, 0, 0, width, height);
     if (rays_traced) *rays_traced = traced;
     return 0;
 }

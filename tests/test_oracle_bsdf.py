@@ -134,3 +134,18 @@ def test_albedo_estimators_agree_and_conserve(name):
         uni = (fu[:, 0].astype(np.float64) * np.abs(u[:, 2])).mean() * 4 * np.pi
         assert abs(w.mean() - uni) < 0.02 + 0.02 * uni, (name, w.mean(), uni)
         assert uni <= 1.02, (name, uni)
+
+
+def test_glass_total_internal_reflection_sample_is_defined():
+    """Choosing transmission where refraction is impossible returns zero;
+    the reference leaves the direction and pdf unset there
+    (BSDFs/Specular.h:70-82), the oracle (and the device code) define them
+    as the mirror direction and 0, so the zero-throughput continuation ray is
+    the same on every run."""
+    m = mesh(scenes.GLASS)
+    m["kr"], m["kt"] = (1, 1, 1), (1, 1, 1)
+    wo = np.float32([0.95, 0.0, -np.sqrt(1 - 0.95 ** 2)])   # inside, grazing: sin^2 * 1.5^2 > 1
+    wi, f, p = oracle.bsdf_sample(m, [[0.999, 0.3]] * 3, wo)  # u above the reflection probability
+    np.testing.assert_array_equal(f, 0)
+    np.testing.assert_array_equal(p, 0)
+    np.testing.assert_array_equal(wi, np.tile(np.float32([-wo[0], -wo[1], wo[2]]), (3, 1)))

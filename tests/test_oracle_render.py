@@ -229,3 +229,19 @@ def test_textures_change_only_textured_surfaces():
     b, _, _ = oracle.render_direct(tx, 1, cam)
     changed = np.any(a != b, axis=-1).mean()
     assert 0.2 < changed < 0.9
+
+
+def test_crop_equals_the_full_film():
+    """A crop traces only the samples that reach its pixels, with the full
+    image's sample indices: its films equal that part of the full films,
+    including the border, where pixels outside the image get no samples."""
+    sc = scenes.cornell_box(seed=3, width=40, height=30, materials="mixed", textured=True)
+    cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+    rgb, w, _ = oracle.render_path(sc, 4, cam)
+    for x0, y0, x1, y1 in ((10, 7, 17, 12), (0, 0, 3, 2), (35, 26, 40, 30)):
+        crgb, cw, _, smp = oracle.render_path(sc, 4, cam, crop=(x0, y0, x1, y1), samples=True)
+        np.testing.assert_array_equal(crgb, rgb[y0:y1, x0:x1])
+        np.testing.assert_array_equal(cw, w[y0:y1, x0:x1])
+        # film positions lie within one pixel of their pixel's corner
+        xs = np.arange(x0, x1)[None, :, None]
+        assert ((smp[..., 3] >= xs) & (smp[..., 3] < xs + 1)).all()
