@@ -231,7 +231,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--probe", action="store_true",
-                    help="one LBVH build and one 256 MiB device copy, then exit (for rocprofv3 --pmc runs)")
+                    help="one LBVH build, one closest-hit and one shadow trace, and one 2^28-byte "
+                         "elementwise add, then exit (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -268,6 +269,11 @@ def main() -> int:
     log("[bench] first build (allocates)")
     bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
     if args.probe:
+        check(ctx.handle, lib().cav_trace_closest_async(
+            ctx.handle, bvh.handle, rays.data_ptr(), r, hit_prim.data_ptr(), hit_t.data_ptr()))
+        check(ctx.handle, lib().cav_trace_occluded_async(
+            ctx.handle, bvh.handle, srays.data_ptr(), lim.data_ptr(), r, vis.data_ptr()))
+        ctx.synchronize()
         # an elementwise kernel (a copy may go to a DMA engine, which the
         # counters do not see) calibrates the HBM byte counters: it reads and
         # writes exactly 2^28 B
@@ -275,7 +281,7 @@ def main() -> int:
         dst = torch.empty_like(src)
         torch.add(src, 1, out=dst)
         torch.cuda.synchronize()
-        log("[bench] probe done: one build, one 2^28-byte elementwise kernel")
+        log("[bench] probe done: one build, two traces, one 2^28-byte elementwise kernel")
         return 0
 
     def step(record: bool):

@@ -267,13 +267,31 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_
     return off + inc - v;
 }
 
+// exclusive count of set flags before this thread's in the block, and the
+// block's total
+__device__ __forceinline__ uint32_t block_flag_excl(bool f, uint32_t* ws, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t b = __ballot(f);
+    if (lane == 0) ws[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) { if (w < wave) off += ws[w]; total += ws[w]; }
+    __syncthreads();
+    return off + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+}
+
+// Tile t covers ids [t * kScanTile, (t + 1) * kScanTile); pass j of a tile
+// reads ids tile + j * kBlock + thread, so every load is coalesced.
 __global__ __launch_bounds__(kBlock) void k_leafcount(const int4* __restrict__ topo, int64_t m,
                                                       uint32_t* __restrict__ partial) {
     __shared__ uint32_t ws[kBlock / 64];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    const int64_t tile = (int64_t)blockIdx.x * kScanTile;
     uint32_t c = 0;
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < m && topo[base + j].w >= 0) ++c;
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t id = tile + (int64_t)j * kBlock + threadIdx.x;
+        if (id < m && topo[id].w >= 0) ++c;
+    }
     uint32_t tot;
     block_excl(c, ws, tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -293,19 +311,15 @@ __global__ __launch_bounds__(kBlock) void k_refs(const int4* __restrict__ topo, 
                                                  const uint32_t* __restrict__ partial,
                                                  int32_t* __restrict__ ref) {
     __shared__ uint32_t ws[kBlock / 64];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    bool leaf[kScanItems];
-    uint32_t c = 0;
+    const int64_t tile = (int64_t)blockIdx.x * kScanTile;
+    uint32_t run = partial[blockIdx.x];
     for (int j = 0; j < kScanItems; ++j) {
-        leaf[j] = base + j < m && topo[base + j].w >= 0;
-        c += leaf[j];
-    }
-    uint32_t tot;
-    uint32_t run = block_excl(c, ws, tot) + partial[blockIdx.x];
-    for (int j = 0; j < kScanItems; ++j) {
-        if (base + j >= m) break;
-        ref[base + j] = leaf[j] ? ~(int32_t)run : (int32_t)(base + j - run);
-        run += leaf[j];
+        const int64_t id = tile + (int64_t)j * kBlock + threadIdx.x;
+        const bool leaf = id < m && topo[id].w >= 0;
+        uint32_t tot;
+        const uint32_t ex = run + block_flag_excl(leaf, ws, tot);
+        if (id < m) ref[id] = leaf ? ~(int32_t)ex : (int32_t)(id - ex);
+        run += tot;
     }
 }
 

@@ -286,6 +286,7 @@ struct Mesh {
     std::vector<V3> nrm;     // per vertex, empty when the mesh has none
     std::vector<float> uv;   // 2 per vertex, empty when the mesh has none
     std::vector<int> idx;    // 3 per triangle
+    float sphere[4] = {0, 0, 0, 0};  // centre, radius of a sphere shape (radius 0: none)
 };
 
 // normalize(normalMatrix * n) for each given normal
@@ -597,6 +598,12 @@ Mesh sphere_mesh(float radius, const M4& t) {
     for (const V3& v : p) m.pos.push_back(apply(t, {v.x * radius, v.y * radius, v.z * radius}));
     m.nrm = transform_normals(p, t);  // the unit-sphere points are their own normals
     m.idx = cur;
+    // shapeParams (ShapeMesh/Sphere.cu:86-88): the transformed centre, and the
+    // distance from it to the transformed point (radius, 0, 0)
+    const V3 c = apply(t, {0, 0, 0}), e = apply(t, {radius, 0, 0});
+    const V3 dc{c.x - e.x, c.y - e.y, c.z - e.z};
+    m.sphere[0] = c.x; m.sphere[1] = c.y; m.sphere[2] = c.z;
+    m.sphere[3] = std::sqrt(dc.x * dc.x + dc.y * dc.y + dc.z * dc.z);
     return m;
 }
 
@@ -730,6 +737,7 @@ void add_shape(Builder& bd, const Mesh& m, const Material& mat, bool reverse) {
     if (first + ntris > INT32_MAX) throw ParseError("more than 2^31 triangles");
     c.first_tri = (int32_t)first;
     c.num_tris = ntris;
+    for (int q = 0; q < 4; ++q) c.sphere[q] = m.sphere[q];
     c.reverse_orientation = reverse ? 1 : 0;
     c.is_light = 0;
     const bool light = bd.next_is_light;

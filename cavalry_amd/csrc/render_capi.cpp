@@ -405,6 +405,14 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
         sc->host_area[m] = a;
     }
     sc->env_miss = background;
+    // Halton dimensions of one light sample after randInt + rand4: the
+    // reference's syncDimension takes each bounce to the queue's maximum, 3
+    // (triangle, then a point in it) once any task samples a triangle of an
+    // area light, else 2 (a direction: environment lights, a sphere light
+    // seen from outside). Fixed per scene: 3 with any area light. A hit on a
+    // sphere light's own (tessellated, so inside) surface samples its
+    // triangles, so a bounce whose tasks all draw directions is the one case
+    // where the reference uses 2 and this uses 3.
     sc->light_dims = 2;
     for (int32_t e : lights) sc->light_dims = e >= 0 ? 3 : sc->light_dims;
     int rc = dev_alloc(ctx, (void**)&sc->soup, 36 * (size_t)n, "scene soup");

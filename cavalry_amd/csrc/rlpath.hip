@@ -211,9 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
             if (qi.delta[i]) {
                 acc = add3(acc, mul3(Le, mult));
             } else if (depth > 0) {
-                const v3 l2s = sub3(ro, pos);
-                const float c = fabsf(dot3(nn, nrm(l2s)));
-                const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[mesh]);
+                const float lpdf = light_pdf(sc, mesh, ro, pos, nn);
                 const float w = power_heuristic(qi.pdf[i], lpdf);
                 if (isfinite(w)) {
                     emis = luminance(Le);
@@ -271,6 +269,8 @@ struct BinEval {
     const float* Qe;
     const float* cs;
     float px;
+    bool matte;   // Lambertian: luminance(f) is one constant on the exitant side
+    float lum_m;  // luminance(kd / pi)
 
     __device__ inline void row(int ti, float& u, float& xy) const {
         u = ((float)ti + px) * (1.f / (float)kNY);
@@ -279,6 +279,10 @@ struct BinEval {
     }
     __device__ inline float bin(float u, float xy, int p, int c) const {
         const v3 d = mk3(xy * cs[p * kBlock], xy * cs[(kNX + p) * kBlock], u);
+        if (matte) {  // the same bits as the general case: the local z is dot3(nn, d)
+            const float z = dot3(nn, d);
+            return (fabsf(z) * (z * ol.z > 0 ? lum_m : 0.f)) * Qe[c];
+        }
         const float scat = luminance(bsdf.eval(to_local(d, t0, t1, nn), ol));
         return (fabsf(dot3(d, nn)) * scat) * Qe[c];
     }
@@ -357,7 +361,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
             cs[p * kBlock] = cos_f(phi);
             cs[(kNX + p) * kBlock] = sin_f(phi);
         }
-        const BinEval be{bsdf, t0, t1, nn, ol, Qe, cs, px};
+        const bool matte = bsdf.type == CAV_MAT_MATTE;
+        const BinEval be{bsdf, t0, t1, nn, ol, Qe, cs, px, matte,
+                         matte ? luminance(divs3(bsdf.kd, (float)kPi)) : 0.f};
         float rows[kNY];
         float sum = 0.f, lc0 = 0.f, lc1 = 0.f;
         bool mono = true;

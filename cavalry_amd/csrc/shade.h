@@ -497,6 +497,55 @@ __device__ inline v3 uniform_sphere(float x, float y) {
     return mk3(cv * cu, sv, cv * su);
 }
 
+// TriangleMesh::sampleSphere (Core/TriangleMesh.h:155-211): a direction
+// uniform within the cone the sphere (centre c, radius rad) subtends from pos,
+// the point where it meets the sphere, its outward normal and the cone's
+// solid-angle pdf. Small cones (sin^2 < sin^2 1.5 deg) draw sin^2 theta
+// directly, as the reference does.
+__device__ inline void sample_sphere(v3 pos, v3 c, float rad, float ux, float uy, v3& p, v3& n, float& pdf) {
+    const float dc = len3(sub3(pos, c));
+    const float inv_dc = 1 / dc;
+    const v3 wc = scale3(sub3(c, pos), inv_dc);
+    v3 wx, wy;
+    frame(wc, wx, wy);
+    const float st_max = rad * inv_dc;
+    const float st_max2 = st_max * st_max;
+    const float inv_st_max = 1 / st_max;
+    const float ct_max = sqrtf(fmx(0.f, 1 - st_max2));
+    float ct = (ct_max - 1) * ux + 1;
+    float st2 = 1 - ct * ct;
+    if (st_max2 < 0.00068523f) {
+        st2 = st_max2 * ux;
+        ct = sqrtf(1 - st2);
+    }
+    const float ca = st2 * inv_st_max + ct * sqrtf(fmx(0.f, 1.f - st2 * inv_st_max * inv_st_max));
+    const float sa = sqrtf(fmx(0.f, 1.f - ca * ca));
+    const float phi = (float)((double)(uy * 2.f) * kPi);
+    const float cp = (float)cos((double)phi), sp = (float)sin((double)phi);
+    n = scale3(add3(add3(scale3(wx, sa * cp), scale3(wy, sa * sp)), scale3(wc, ct)), -1.f);
+    p = add3(c, scale3(n, rad));
+    pdf = (float)(1.0 / (2.0 * kPi * (double)(1 - ct_max)));
+}
+
+// TriangleMesh::pdf (Core/TriangleMesh.h:214-232) of a ray from o that hit
+// light mesh m at pos with normal nn: the cone pdf of a sphere seen from
+// outside, else the area pdf converted to solid angle.
+__device__ inline float light_pdf(const SceneDev& sc, int m, v3 o, v3 pos, v3 nn) {
+    const cav_mesh& L = sc.meshes[m];
+    if (L.sphere[3] > 0) {
+        const v3 oc = sub3(o, mk3(L.sphere[0], L.sphere[1], L.sphere[2]));
+        const float d2 = dot3(oc, oc);
+        if (d2 > L.sphere[3] * L.sphere[3]) {
+            const float st2 = L.sphere[3] * L.sphere[3] / d2;
+            const float ct = sqrtf(fmx(0.f, 1.f - st2));
+            return (float)(1.0 / (2.0 * kPi * (double)(1 - ct)));
+        }
+    }
+    const v3 l2s = sub3(o, pos);
+    const float c = fabsf(dot3(nn, nrm(l2s)));
+    return dot3(l2s, l2s) / (c * sc.mesh_area[m]);
+}
+
 // LightObject::sampleRayToPoint for light-list entry lm seen from pos, with
 // its dimensions from d on (after randInt + rand4): an area light
 // (DiffuseAreaLight::sampleRayToPoint, TriangleMesh::sample: a uniform
@@ -521,6 +570,20 @@ __device__ inline LightPick sample_light(const SceneDev& sc, const HaltonDev& h,
         return r;
     }
     const cav_mesh L = sc.meshes[lm];
+    r.dst = lm;
+    if (L.sphere[3] > 0) {  // TriangleMesh::sample (Core/TriangleMesh.h:112-118)
+        const v3 c = mk3(L.sphere[0], L.sphere[1], L.sphere[2]);
+        const v3 pc = sub3(pos, c);
+        if (dot3(pc, pc) > L.sphere[3] * L.sphere[3]) {
+            v3 lp, ln;
+            sample_sphere(pos, c, L.sphere[3], halton(h, sid, d), halton(h, sid, d + 1), lp, ln, r.prob);
+            if (L.reverse_orientation) ln = scale3(ln, -1.f);
+            r.dir = nrm(sub3(lp, pos));
+            r.lim = len3(sub3(lp, pos));
+            r.incident = dot3(r.dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
+            return r;
+        }
+    }
     const int tri = L.first_tri + halton_int(h, sid, d, L.num_tris);
     const float tmp = sqrtf(halton(h, sid, d + 1));
     const float u = 1 - tmp;
@@ -534,7 +597,6 @@ __device__ inline LightPick sample_light(const SceneDev& sc, const HaltonDev& h,
     r.dir = nrm(sub3(lp, pos));
     r.lim = len3(sub3(lp, pos));
     r.incident = dot3(r.dir, ln) >= 0 ? mk3(0, 0, 0) : mk3(L.emission[0], L.emission[1], L.emission[2]);
-    r.dst = lm;
     return r;
 }
 
@@ -569,9 +631,7 @@ __device__ inline void path_shade(int depth, int dim0, const SceneDev& sc, const
         if (from_delta) {
             acc = add3(acc, mul3(Le, mult));
         } else if (depth > 0) {
-            const v3 l2s = sub3(ro, pos);
-            const float c = fabsf(dot3(nn, nrm(l2s)));
-            const float lpdf = dot3(l2s, l2s) / (c * sc.mesh_area[mesh]);
+            const float lpdf = light_pdf(sc, mesh, ro, pos, nn);
             const float w = power_heuristic(mat_pdf, lpdf);
             if (isfinite(w)) acc = add3(acc, scale3(mul3(Le, mult), w));
         }

@@ -37,7 +37,10 @@ struct RayStack {
         return sp < D ? lds[sp][threadIdx.x] : spill[sp - D];
     }
 };
-constexpr int kLdsDepth = 32;
+#ifndef CAV_TRACE_LDS_DEPTH
+#define CAV_TRACE_LDS_DEPTH 32
+#endif
+constexpr int kLdsDepth = CAV_TRACE_LDS_DEPTH;
 constexpr int kChunk = 256;  // rays taken from the global counter at a time
 
 // Core/BoundingBox.h:55-74; returns the entry distance or -1.

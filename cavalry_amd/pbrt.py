@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import Camera, check, lib
-from .scenes import ENV_DTYPE, GLASS, MATTE, MESH_DTYPE, METAL, MIRROR, PLASTIC, SUBSTRATE, SceneArrays
+from .scenes import (ENV_DTYPE, GLASS, MATTE, MESH_DTYPE, METAL, MIRROR, PLASTIC, SUBSTRATE, SceneArrays,
+                     octa_sphere)
 
 
 class Overrides(ctypes.Structure):
@@ -237,6 +238,29 @@ def write(arr: SceneArrays, path: str, spp: int = 16, integrator: str = "path", 
         if m["reverse_orientation"]:
             out.append("  ReverseOrientation")
         t = arr.tris[m["first_tri"]:m["first_tri"] + m["num_tris"]]
+        if m["sphere"][3] > 0:
+            # a sphere shape: written as one when the triangles are those the
+            # parser will make again from the centre and radius
+            # (the stored radius went through the transform's rounding, so
+            # the radius to write is searched among its float neighbours)
+            c = m["sphere"][:3]
+            r = None
+            for k in range(-8, 9):
+                cand = np.float32(m["sphere"][3])
+                step = np.float32(np.inf if k > 0 else 0)
+                for _ in range(abs(k)):
+                    cand = np.nextafter(cand, step)
+                st, _, ss = octa_sphere(c, cand)
+                if np.array_equal(st, t) and np.array_equal(ss, m["sphere"]):
+                    r = cand
+                    break
+            if r is None:
+                raise ValueError("a sphere mesh that is not the sphere shape of its centre and radius "
+                                 "has no .pbrt form")
+            out.append(f"  Translate {_f(c[0])} {_f(c[1])} {_f(c[2])}")
+            out.append(f'  Shape "sphere" "float radius" [ {_f(r)} ]')
+            out.append("AttributeEnd")
+            continue
         idx = " ".join(str(i) for i in range(3 * len(t)))
         normals = ""
         if getattr(arr, "normals", None) is not None and (arr.mesh_normals is None or arr.mesh_normals[mi]):

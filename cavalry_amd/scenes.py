@@ -68,8 +68,9 @@ MESH_DTYPE = np.dtype([
     ("kd", "<f4", 3), ("ks", "<f4", 3), ("kr", "<f4", 3), ("kt", "<f4", 3),
     ("roughness", "<f4"), ("ior", "<f4"), ("emission", "<f4", 3),
     ("is_light", "<i4"), ("reverse_orientation", "<i4"), ("roughness_v", "<f4"),
+    ("sphere", "<f4", 4),
 ])
-assert MESH_DTYPE.itemsize == 96  # cav_mesh in include/cavalry.h
+assert MESH_DTYPE.itemsize == 112  # cav_mesh in include/cavalry.h
 
 MATTE, MIRROR, SUBSTRATE, METAL, GLASS, PLASTIC = 0, 1, 2, 3, 4, 5
 
@@ -134,7 +135,7 @@ class _Builder:
 
     def mesh(self, tris, material=MATTE, kd=(0.5, 0.5, 0.5), ks=(0.25, 0.25, 0.25),
              kr=(1, 1, 1), kt=(1, 1, 1), roughness=0.1, ior=1.5, emission=None, texture=-1, uv=None,
-             roughness_v=0.0):
+             roughness_v=0.0, sphere=None):
         tris = np.asarray(tris, np.float32).reshape(-1, 9)
         if callable(uv):
             uv = uv()
@@ -149,6 +150,8 @@ class _Builder:
         if emission is not None:
             m["emission"] = emission
             m["is_light"] = 1
+        if sphere is not None:
+            m["sphere"] = sphere
         self.tris.append(tris)
         self.meshes.append(m)
 
@@ -206,6 +209,43 @@ def uv_sphere(center, r, nphi=64, ntheta=32):
     for j in range(nphi):
         t.append(list(p(ntheta, 0)) + list(p(ntheta - 1, j)) + list(p(ntheta - 1, j + 1)))
     return t
+
+
+def octa_sphere(center, radius) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The sphere shape of the scene format (ShapeMesh/Sphere.cu:3-90): an
+    octahedron subdivided four times (2048 triangles), midpoints normalised,
+    scaled by the radius and translated to the centre, with the unit points as
+    vertex normals, in float32 as the .pbrt parser computes it for a
+    `Translate` + `Shape "sphere"`. Returns (tris [2048, 9], normals [2048, 9],
+    sphere = centre and the radius the reference derives from the transform)."""
+    f = np.float32
+
+    def normalize(v):
+        d = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]
+        inv = f(1.0) / np.sqrt(d)
+        return np.array([v[0] * inv, v[1] * inv, v[2] * inv], f)
+
+    pts = [np.array(v, f) for v in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    cur = [(0, 2, 4), (0, 2, 5), (0, 3, 4), (0, 3, 5), (1, 2, 4), (1, 2, 5), (1, 3, 4), (1, 3, 5)]
+    for _ in range(4):
+        nxt = []
+        for i0, i1, i2 in cur:
+            ids = []
+            for a, b in ((i0, i1), (i0, i2), (i1, i2)):
+                pts.append(normalize((pts[a] + pts[b]) * f(0.5)))
+                ids.append(len(pts) - 1)
+            i01, i02, i12 = ids
+            nxt += [(i01, i02, i12), (i0, i01, i02), (i1, i01, i12), (i2, i02, i12)]
+        cur = nxt
+    unit = np.array(pts, f)
+    c, r = np.asarray(center, f), f(radius)
+    pos = unit * r + c
+    nrm = np.array([normalize(v) for v in unit], f)
+    idx = np.array(cur, np.int64)
+    # shapeParams: |centre - T(radius, 0, 0)|, rounded as the transform does
+    d = c - np.array([r + c[0], c[1], c[2]], f)
+    rad = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+    return pos[idx].reshape(-1, 9), nrm[idx].reshape(-1, 9), np.array([c[0], c[1], c[2], rad], f)
 
 
 def planar_uv(tris: np.ndarray, scale: float = 1.0) -> np.ndarray:
@@ -286,9 +326,26 @@ def sky_plane(seed: int = 1, width: int = 64, height: int = 48, albedo: float = 
                        env_lights=env, env_background=0)
 
 
+def sphere_light_plane(width: int = 64, height: int = 48, albedo: float = 0.5, radiance: float = 2.0,
+                       radius: float = 0.7, max_depth: int = 2, light_kd: float = 0.0) -> SceneArrays:
+    """A 20 x 20 matte floor (albedo `albedo`, normal +y) under one sphere
+    light of radius `radius` centred at height 1 (the octahedral sphere
+    shape, radiance `radiance`, diffuse albedo `light_kd`); the camera looks
+    at the floor under the sphere from the side."""
+    b = _Builder()
+    b.mesh(quad((-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)), kd=(albedo, albedo, albedo))
+    st, sn, ss = octa_sphere((0, 1, 0), radius)
+    b.mesh(st, kd=(light_kd,) * 3, emission=(radiance,) * 3, sphere=ss)
+    tris, meshes = b.arrays()
+    normals = np.zeros_like(tris)
+    normals[len(tris) - len(st):] = sn
+    return SceneArrays(tris, meshes, (0, 0.6, -4), (0, 0, 0.5), (0, 1, 0), 50.0, width, height, max_depth,
+                       normals=normals, mesh_normals=np.array([0, 1], np.uint8))
+
+
 def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: int = 8,
                 materials: str = "matte", smooth: bool = False, textured: bool = False,
-                env: bool = False) -> SceneArrays:
+                env: bool = False, lights: str = "quad") -> SceneArrays:
     """Cornell-style box: unit box open at z=0 (5 walls x 2 quads), a tall and a
     short box, a 64x32 UV sphere (3968 triangles) and a 0.25x0.25 ceiling light
     of radiance 15 facing down. Albedos U[0.1,0.9] per mesh from the seed.
@@ -300,7 +357,10 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
     value-noise marble on the tall box (64 x 64 textures, planar UVs; the
     matte ones with kd = 1), and spherical UVs on the sphere. env=True adds a
     textured sky (rotated 30 degrees about y, seen by rays leaving through the
-    open front) and a black light, ordered sky, ceiling light, black."""
+    open front) and a black light, ordered sky, ceiling light, black.
+    lights="sphere" replaces the ceiling quad with a sphere light (the
+    octahedral sphere shape, radius 0.08 at (0.5, 0.85, 0.5), radiance 20,
+    sampled within the cone it subtends); lights="both" keeps the quad too."""
     rng = _rng(seed, 7)
     b = _Builder()
 
@@ -330,12 +390,25 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
         b.mesh(sph, kd=albedo())
     light = quad((0.375, 0.999, 0.375), (0.625, 0.999, 0.375), (0.625, 0.999, 0.625),
                  (0.375, 0.999, 0.625))
-    b.mesh(light, kd=albedo(), emission=(15.0, 15.0, 15.0))
+    if lights not in ("quad", "sphere", "both"):
+        raise ValueError(f"lights must be quad, sphere or both, not {lights!r}")
+    ka = albedo()
+    if lights != "sphere":
+        b.mesh(light, kd=ka, emission=(15.0, 15.0, 15.0))
+    sl_n = None
+    if lights != "quad":
+        sl, sl_n, sl_s = octa_sphere((0.5, 0.85, 0.5), 0.08)
+        b.mesh(sl, kd=ka, emission=(20.0, 20.0, 20.0), sphere=sl_s)
     tris, meshes = b.arrays()
     normals = flags = None
-    if smooth:
+    if smooth or sl_n is not None:
         normals = np.zeros_like(tris)
         flags = np.zeros(len(meshes), np.uint8)
+    if sl_n is not None:
+        m = meshes[-1]
+        normals[m["first_tri"]:m["first_tri"] + m["num_tris"]] = sl_n
+        flags[-1] = 1
+    if smooth:
         m = meshes[7]  # the sphere
         t = tris[m["first_tri"]:m["first_tri"] + m["num_tris"]].reshape(-1, 3, 3).astype(np.float64)
         n = t - np.array([0.5, 0.72, 0.3])
@@ -358,7 +431,7 @@ def cornell_box(seed: int = 1, width: int = 512, height: int = 512, max_depth: i
     if env:
         textures = (textures or []) + [sky_texture()]
         envs = [env_light((0.8, 0.9, 1.0), len(textures) - 1, rotation_y(30).T), env_light((0, 0, 0))]
-        order = [-1, len(meshes) - 1, -2]
+        order = [-1] + [i for i in range(len(meshes)) if meshes[i]["is_light"]] + [-2]
         background = 0
     return SceneArrays(tris, meshes, (0.5, 0.5, -1.3), (0.5, 0.5, 0.5), (0, 1, 0), 40.0,
                        width, height, max_depth, normals=normals, mesh_normals=flags, uv=uv, textures=textures,

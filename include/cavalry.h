@@ -139,7 +139,11 @@ enum { CAV_RENDER_SORT_BY_MATERIAL = 1 };
  * 5 plastic (kd, ks, roughness = GGX alpha). roughness_v = 0 means the same
  * alpha on both axes. texture: index into the scene's textures of a Kd
  * texture (matte, plastic, substrate), or -1 for none. is_light marks a
- * DiffuseAreaLight of radiance `emission` (Lights/DiffuseAreaLight.h). */
+ * DiffuseAreaLight of radiance `emission` (Lights/DiffuseAreaLight.h).
+ * sphere = centre and radius when the triangles tessellate a sphere shape
+ * (MeshShapeType::Sphere, ShapeMesh/Sphere.cu:84-88): as a light it is
+ * sampled within the cone it subtends (TriangleMesh::sampleSphere,
+ * Core/TriangleMesh.h:155-211); radius 0 = a plain triangle mesh. */
 typedef struct {
     int32_t first_tri, num_tris;
     int32_t material;
@@ -149,6 +153,7 @@ typedef struct {
     float emission[3];
     int32_t is_light, reverse_orientation;
     float roughness_v;
+    float sphere[4];
 } cav_mesh;
 
 /* Perspective camera: column-major camera-to-world matrix (glm layout), film

@@ -15,6 +15,7 @@ typedef struct {
     float emission[3];
     int32_t is_light, reverse_orientation;
     float roughness_v;  // 0: same as roughness
+    float sphere[4];    // centre, radius of an analytic sphere shape; radius 0: none
 } cav_mesh_ref;
 
 extern "C" {

@@ -439,8 +439,10 @@ struct Scene {
 // The light list in the scene's order (Scene::lightsHost) and the light
 // sample's share of the Halton dimensions: syncDimension (PathTracingIntegrator
 // .cu:281-315) takes each bounce to the maximum over the queue, 3 (triangle +
-// point) as soon as any task picked an area light, else 2 (a sphere
-// direction); fixed here per scene.
+// point) as soon as any task sampled an area light's triangles, else 2 (a
+// direction: environment lights, sphere lights seen from outside); fixed here
+// per scene, 3 with any area light (hits on a sphere light's tessellated
+// surface lie inside the sphere and sample its triangles).
 void init_lights(Scene& s) {
     const ref_surface* sf = s.surf;
     s.lights.clear();
@@ -674,6 +676,53 @@ f3 shading_normal(const Scene& s, int p, float u, float v) {
 
 float power_heuristic(float a, float b) { a *= a; b *= b; return a / (a + b); }
 
+// TriangleMesh::sampleSphere (Core/TriangleMesh.h:155-211): a direction
+// uniform in the cone the sphere subtends from pos; the point it meets, the
+// outward normal there, and the cone pdf. Trig in binary64 rounded once.
+void sample_sphere(f3 pos, f3 c, float rad, float ux, float uy, f3& p, f3& n, float& pdf) {
+    float dc = length(pos - c);
+    float inv_dc = 1 / dc;
+    f3 wc = (c - pos) * inv_dc;
+    f3 wx, wy;
+    coordinate_system(wc, wx, wy);
+    float st_max = rad * inv_dc;
+    float st_max2 = st_max * st_max;
+    float inv_st_max = 1 / st_max;
+    float ct_max = std::sqrt(std::max(0.f, 1 - st_max2));
+    float ct = (ct_max - 1) * ux + 1;
+    float st2 = 1 - ct * ct;
+    if (st_max2 < 0.00068523f) {  // sin^2(1.5 deg): small-angle form
+        st2 = st_max2 * ux;
+        ct = std::sqrt(1 - st2);
+    }
+    float ca = st2 * inv_st_max + ct * std::sqrt(std::max(0.f, 1.f - st2 * inv_st_max * inv_st_max));
+    float sa = std::sqrt(std::max(0.f, 1.f - ca * ca));
+    float phi = (float)((double)(uy * 2.f) * kPi);
+    float cp = (float)std::cos((double)phi), sp = (float)std::sin((double)phi);
+    n = ((wx * (sa * cp) + wy * (sa * sp)) + wc * ct) * -1.f;
+    p = c + n * rad;
+    pdf = (float)(1.0 / (2.0 * kPi * (double)(1 - ct_max)));
+}
+
+// TriangleMesh::pdf (Core/TriangleMesh.h:214-232) for a ray from o that hit
+// light mesh m at pos, normal n: the cone pdf for a sphere seen from outside,
+// else the area pdf in solid angle.
+float light_pdf(const Scene& s, int m, f3 o, f3 pos, f3 n) {
+    const cav_mesh_ref& L = s.meshes[m];
+    if (L.sphere[3] > 0) {
+        f3 oc = o - mk(L.sphere[0], L.sphere[1], L.sphere[2]);
+        float d2 = dot(oc, oc);
+        if (d2 > L.sphere[3] * L.sphere[3]) {
+            float st2 = L.sphere[3] * L.sphere[3] / d2;
+            float ct = std::sqrt(std::max(0.f, 1.f - st2));
+            return (float)(1.0 / (2.0 * kPi * (double)(1 - ct)));
+        }
+    }
+    f3 l2s = o - pos;
+    float c = std::fabs(dot(n, normalize(l2s)));
+    return dot(l2s, l2s) / (c * s.area[m]);
+}
+
 // LightObject::sampleRayToPoint of light-list entry lm from pos, reading
 // dimensions from dim on and leaving dim light_dims further: an area light
 // (DiffuseAreaLight / TriangleMesh::sample, Core/TriangleMesh.h:108-153) or
@@ -693,6 +742,21 @@ LightPick sample_light(const Scene& s, const Halton& H, uint64_t hidx, int& dim,
         r.dst = -1;
     } else {
         const cav_mesh_ref& L = s.meshes[lm];
+        r.dst = lm;
+        const f3 sc = mk(L.sphere[0], L.sphere[1], L.sphere[2]);
+        const f3 pc = pos - sc;
+        if (L.sphere[3] > 0 && dot(pc, pc) > L.sphere[3] * L.sphere[3]) {  // Core/TriangleMesh.h:112-118
+            const float ux = H.rand1(hidx, dim);
+            const float uy = H.rand1(hidx, dim);
+            f3 lp, ln;
+            sample_sphere(pos, sc, L.sphere[3], ux, uy, lp, ln, r.prob);
+            if (L.reverse_orientation) ln = ln * -1.f;
+            r.dir = normalize(lp - pos);
+            r.lim = length(lp - pos);
+            r.incident = dot(r.dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
+            dim = d0 + s.light_dims;
+            return r;
+        }
         int tri = L.first_tri + H.rand_int(L.num_tris, hidx, dim);
         float tmp = std::sqrt(H.rand1(hidx, dim));
         float u = 1 - tmp;
@@ -706,7 +770,6 @@ LightPick sample_light(const Scene& s, const Halton& H, uint64_t hidx, int& dim,
         r.dir = normalize(lp - pos);
         r.lim = length(lp - pos);
         r.incident = dot(r.dir, ln) >= 0 ? mk(0, 0, 0) : mk(L.emission[0], L.emission[1], L.emission[2]);
-        r.dst = lm;
     }
     dim = d0 + s.light_dims;
     return r;
@@ -854,9 +917,7 @@ extern "C" int ref_render_path_crop(const float* tris, int ntris, const int32_t*
                 if (from_delta) {
                     result = result + mulv(Le, mult);
                 } else if (depth > 0) {
-                    f3 l2s = ro - pos;  // TriangleMesh::pdf (Core/TriangleMesh.h:213-225)
-                    float c = std::fabs(dot(n, normalize(l2s)));
-                    float lpdf = dot(l2s, l2s) / (c * s.area[mesh]);
+                    float lpdf = light_pdf(s, mesh, ro, pos, n);
                     float w = power_heuristic(mat_pdf, lpdf);
                     if (std::isfinite(w)) result = result + mulv(Le, mult) * w;
                 }
@@ -1265,9 +1326,7 @@ extern "C" int ref_render_rlpath(const float* tris, int ntris, const int32_t* tr
                         f3 a = mulv(Le, t.mult);
                         acc[0] += a.x; acc[1] += a.y; acc[2] += a.z;
                     } else if (depth > 0) {
-                        f3 l2s = t.ro - pos;
-                        float c = std::fabs(dot(n, normalize(l2s)));
-                        float lp = dot(l2s, l2s) / (c * s.area[mesh]);
+                        float lp = light_pdf(s, mesh, t.ro, pos, n);
                         float w = power_heuristic(t.pdf, lp);
                         if (std::isfinite(w)) {
                             emis = Le;

@@ -192,6 +192,34 @@ def test_environment_lights_match_oracle(ctx, integrator, kind):
     scene.close()
 
 
+@pytest.mark.parametrize("integrator", ["path", "directlighting", "rlpath"])
+@pytest.mark.parametrize("kind", ["cornell_sphere", "cornell_both", "plane"])
+def test_sphere_lights_match_oracle(ctx, integrator, kind):
+    """Sphere lights sampled within the cone they subtend, and the cone pdf
+    in the MIS weight of BSDF rays that hit them: the Cornell box lit by a
+    sphere (mixed materials, the sphere's own diffuse surface hit and shaded
+    too), by the sphere and the ceiling quad, and a floor under a large
+    sphere. Films equal the oracle's."""
+    print(f"[render] sphere lights {kind} {integrator}", flush=True)
+    if kind == "plane":
+        sc = scenes.sphere_light_plane(width=48, height=40, radius=0.7, max_depth=4, light_kd=0.5)
+    else:
+        sc = scenes.cornell_box(seed=13, width=48, height=40, materials="mixed",
+                                lights="sphere" if kind == "cornell_sphere" else "both")
+    cam = for_scene(sc)
+    depth = 1 if integrator == "rlpath" else sc.max_depth
+    if integrator == "rlpath":
+        rgb, w, rays = oracle.render_rlpath(sc, 4, cam, max_depth=1)
+    else:
+        rgb, w, rays = {"path": oracle.render_path, "directlighting": oracle.render_direct}[integrator](sc, 4, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, 4, integrator=integrator, camera=cam, max_depth=depth)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()
+
+
 def test_environment_errors(ctx):
     from cavalry_amd._lib import CavError
     print("[render] environment errors", flush=True)

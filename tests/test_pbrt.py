@@ -156,6 +156,45 @@ WorldEnd
     assert m["num_tris"][5] == 8 * 4 ** 4
     np.testing.assert_allclose(np.linalg.norm(sph.astype(np.float64), axis=1), 2.0, rtol=1e-6)
     assert st.nlights == 1
+    # its shape parameters (ShapeMesh/Sphere.cu:86-88); other meshes have none
+    assert list(m["sphere"][5]) == [0, 0, 0, 2] and not m["sphere"][:5].any()
+
+
+def test_sphere_shape_parameters(tmp_path):
+    """centre = T(0), radius = |centre - T(r, 0, 0)| through the current
+    transform; the triangles are the octahedral sphere shape."""
+    body = """WorldBegin
+AttributeBegin
+  AreaLightSource "diffuse" "rgb L" [ 3 3 3 ]
+  Translate 1 2 3
+  Scale 2 2 2
+  Shape "sphere" "float radius" [ 0.5 ]
+AttributeEnd
+AttributeBegin
+  Translate 0.5 0.85 0.5
+  Shape "sphere" "float radius" [ 0.08 ]
+AttributeEnd
+WorldEnd
+"""
+    tris, m, st, _ = pbrt.parse(write(tmp_path, body))
+    assert list(m["sphere"][0]) == [1, 2, 3, 1] and m["is_light"][0] == 1
+    t1, n1, s1 = scenes.octa_sphere((0.5, 0.85, 0.5), 0.08)
+    np.testing.assert_array_equal(m["sphere"][1], s1)
+    np.testing.assert_array_equal(tris[m["first_tri"][1]:], t1)
+    normals, _ = pbrt.parse_normals(write(tmp_path, body))
+    np.testing.assert_allclose(normals[m["first_tri"][1]:], n1, atol=2.5e-7)
+
+
+def test_writer_round_trip_sphere_lights(tmp_path):
+    sc = scenes.cornell_box(seed=3, width=40, height=30, lights="both")
+    p = str(tmp_path / "s.pbrt")
+    pbrt.write(sc, p)
+    tris, meshes, st, msgs = pbrt.parse(p)
+    np.testing.assert_array_equal(tris, sc.tris)
+    np.testing.assert_array_equal(meshes["sphere"], sc.meshes["sphere"])
+    for a, b in zip(meshes, sc.meshes):
+        relevant_equal(a, b)
+    assert st.nlights == 2
 
 
 def test_material_defaults(tmp_path):
