@@ -83,7 +83,8 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {(void**)&h->b.leaf_ids, sizeof(uint32_t) * n, "leaf_ids"},
         {(void**)&h->b.counters, 256, "counters"},
         {(void**)&h->tn, sizeof(TNode) * (n - 1), "tnodes"},
-        {(void**)&h->tt, sizeof(TTri) * n, "ttris"},
+        // + 16 B: traversal reads 64 bytes at every record, nodes and triangles alike
+        {(void**)&h->tt, sizeof(TTri) * n + 16, "ttris"},
         {(void**)&h->root, 64, "root"},
         {(void**)&h->b.lvl_list, sizeof(uint32_t) * m, "level list"},
         {(void**)&h->b.lvl_cnt, sizeof(uint32_t) * (kMaxLevels + 2), "level counts"},

@@ -21,22 +21,26 @@ constexpr int kStack = 64;  // BVH/BVH.h:41,122
 
 // Traversal stack of kStack entries per ray: the first D in LDS (one column
 // per thread, so a wave's accesses to one depth hit 64 distinct banks), the
-// rest in private memory, which only unusually deep traversals touch.
+// rest in private memory, which only unusually deep traversals touch. The
+// depth sp is a separate local so that it stays in a register: a depth held
+// beside the dynamically indexed spill array would live in scratch memory
+// with it, a scratch load and store on every push and pop.
 template <int D>
-struct RayStack {
-    int (*lds)[kBlock];
-    int spill[kStack - D];
-    int sp = 0;
-    __device__ __forceinline__ void push(int v) {
-        if (sp < D) lds[sp][threadIdx.x] = v;
-        else spill[sp - D] = v;
-        ++sp;
-    }
-    __device__ __forceinline__ int pop() {
-        --sp;
-        return sp < D ? lds[sp][threadIdx.x] : spill[sp - D];
-    }
-};
+__device__ __forceinline__ void stack_push(int (*lds)[kBlock], volatile int* spill, int& sp, int v) {
+    if (sp < D) lds[sp][threadIdx.x] = v;
+    else spill[sp - D] = v;
+    ++sp;
+}
+// (the LDS read goes through an LDS-qualified pointer so that the compiler
+// cannot merge it with the spill read into one flat load through a selected
+// pointer)
+typedef __attribute__((address_space(3))) int lds_int;
+template <int D>
+__device__ __forceinline__ int stack_pop(int (*lds)[kBlock], const volatile int* spill, int& sp) {
+    --sp;
+    if (sp < D) return *(const lds_int*)(&lds[sp][threadIdx.x]);
+    return spill[sp - D];
+}
 #ifndef CAV_TRACE_LDS_DEPTH
 #define CAV_TRACE_LDS_DEPTH 32
 #endif
@@ -113,8 +117,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
     int bp = -1, sm = -1, dm = -1, cur = 0, steps = 0;
     bool hit = false, use_lim = false;
     uint32_t ovf = 0;
-    RayStack<kLdsDepth> st;
-    st.lds = lstack;
+    volatile int spill[kStack - kLdsDepth];  // in scratch memory, not registers
+    int sp = 0;
     int64_t chunk_base = 0;
     int chunk_avail = 0;
 
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
                     load_ray(rays, ri, o, d);
                     inv = div3(mk3(1.f, 1.f, 1.f), d);
                     best = FLT_MAX; hit = false; bp = -1; bu = 0.f; bv = 0.f;
-                    st.sp = 0; cur = 0; steps = 0;
+                    sp = 0; cur = 0; steps = 0;
                     if (kClosest) {
                         // BVH/BVH.h:34-38: a ray missing the root box has no hit
                         if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
@@ -175,41 +179,53 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
         if (++steps > max_steps) {
             ++ovf;
             done = true;
-        } else if (cur >= 0) {
-            const TNode nd = tn[cur];
-            const float dl = slab(nd.l0.x, nd.l0.y, nd.l0.z, nd.l0.w, nd.l1.x, nd.l1.y, o, inv);
-            const float dr = slab(nd.l1.z, nd.l1.w, nd.r1.x, nd.r1.y, nd.r1.z, nd.r1.w, o, inv);
-            const bool hl = kClosest ? (dl >= 0 && dl < best) : dl >= 0;
-            const bool hr = kClosest ? (dr >= 0 && dr < best) : dr >= 0;
-            bool next = false;
-            if (hl && hr) {
-                const bool lfar = dl > dr;
-                const int far = lfar ? nd.refs.x : nd.refs.y;
-                const int near = lfar ? nd.refs.y : nd.refs.x;
-                if (st.sp < kStack) st.push(far); else ++ovf;
-                if (st.sp < kStack) { cur = near; next = true; } else ++ovf;
-            } else if (hl || hr) {
-                if (st.sp < kStack) { cur = hl ? nd.refs.x : nd.refs.y; next = true; } else ++ovf;
-            }
-            if (!next) {
-                if (st.sp == 0) done = true;
-                else cur = st.pop();
-            }
         } else {
-            const TTri tr = tt[~cur];
-            float t, u, v;
-            if (kClosest) {
-                if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
-                    hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+            // One 64-byte fetch for either kind of step (a node, or a triangle
+            // and 16 bytes beyond it), so that a wave with lanes at nodes and
+            // lanes at triangles waits for memory once per step.
+            const bool at_node = cur >= 0;
+            const float4* rec = at_node ? reinterpret_cast<const float4*>(tn + cur)
+                                        : reinterpret_cast<const float4*>(tt + ~cur);
+            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+            int4 refs = reinterpret_cast<const int4*>(rec)[3];
+            // (an empty asm that takes the node-only words keeps the compiler
+            // from moving their loads into the node branch, a second wait)
+            asm volatile("" : "+v"(q1.w), "+v"(q2.w), "+v"(refs.x), "+v"(refs.y));
+            if (at_node) {
+                const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
+                const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
+                const bool hl = kClosest ? (dl >= 0 && dl < best) : dl >= 0;
+                const bool hr = kClosest ? (dr >= 0 && dr < best) : dr >= 0;
+                bool next = false;
+                if (hl && hr) {
+                    const bool lfar = dl > dr;
+                    const int far = lfar ? refs.x : refs.y;
+                    const int near = lfar ? refs.y : refs.x;
+                    if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
+                    if (sp < kStack) { cur = near; next = true; } else ++ovf;
+                } else if (hl || hr) {
+                    if (sp < kStack) { cur = hl ? refs.x : refs.y; next = true; } else ++ovf;
+                }
+                if (!next) {
+                    if (sp == 0) done = true;
+                    else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
                 }
             } else {
-                const int mesh = __float_as_int(tr.b.w);
-                if (mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim))
-                    hit = true;  // occluded
+                const TTri tr{q0, q1, q2};
+                float t, u, v;
+                if (kClosest) {
+                    if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
+                        hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+                    }
+                } else {
+                    const int mesh = __float_as_int(tr.b.w);
+                    if (mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim))
+                        hit = true;  // occluded
+                }
+                if (!kClosest && hit) done = true;
+                else if (sp == 0) done = true;
+                else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
             }
-            if (!kClosest && hit) done = true;
-            else if (st.sp == 0) done = true;
-            else cur = st.pop();
         }
         if (done) {
             if (kClosest) {
