@@ -51,7 +51,7 @@ int dev_alloc(cav_ctx* ctx, void** p, size_t bytes, const char* what) {
 static void bvh_free(cav_bvh* h) {
     void* ptrs[] = {h->b.keys[0], h->b.keys[1], h->b.vals[0], h->b.vals[1], h->b.hist, h->b.counts,
                     h->b.offsets, h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,
-                    h->b.partial, h->b.leaf_ids, h->b.counters, h->tn, h->tt, h->root, h->mesh,
+                    h->b.partial, h->b.leaf_ids, h->b.counters, h->rec, h->root, h->mesh,
                     h->b.lvl_list, h->b.lvl_cnt, h->b.lvl_cursor, h->b.lvl_off,
                     h->b.bounds_part};
     for (void* p : ptrs)
@@ -82,9 +82,7 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {(void**)&h->b.partial, sizeof(uint32_t) * (lbvh_partials(m) + 1), "partial"},
         {(void**)&h->b.leaf_ids, sizeof(uint32_t) * n, "leaf_ids"},
         {(void**)&h->b.counters, 256, "counters"},
-        {(void**)&h->tn, sizeof(TNode) * (n - 1), "tnodes"},
-        // + 16 B: traversal reads 64 bytes at every record, nodes and triangles alike
-        {(void**)&h->tt, sizeof(TTri) * n + 16, "ttris"},
+        {(void**)&h->rec, sizeof(TRec) * 2 * (size_t)n, "traversal records"},
         {(void**)&h->root, 64, "root"},
         {(void**)&h->b.lvl_list, sizeof(uint32_t) * m, "level list"},
         {(void**)&h->b.lvl_cnt, sizeof(uint32_t) * (kMaxLevels + 2), "level counts"},
@@ -140,7 +138,7 @@ static int bvh_run(cav_ctx* ctx, cav_bvh* h, const float* tris_dev, const float*
     mark(mk, "readback");
     for (int r = 0; r < opt_rounds; ++r) HIPCHK(ctx, treelet_round_impl(h->b, n, r == 0, s, mk));
     if (opt_rounds > 0) HIPCHK(ctx, lbvh_refs(h->b, n, s, mk, "flatten_refs"));
-    HIPCHK(ctx, lbvh_flatten(h->b, tris_dev, h->mesh, n, opt_rounds == 0, h->tn, h->tt, h->root, s, mk));
+    HIPCHK(ctx, lbvh_flatten(h->b, tris_dev, h->mesh, n, opt_rounds == 0, h->rec, h->root, s, mk));
     float root_final[8];
     HIPCHK(ctx, hipMemcpyAsync(root_final, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
@@ -216,6 +214,7 @@ int cav_bvh_build_device(cav_ctx* ctx, const float* tris_dev, int n, const float
     *out = nullptr;
     if (!tris_dev) return fail(ctx, CAV_ERR_INVALID, "triangles is null");
     if (n < 2) return fail(ctx, CAV_ERR_INVALID, "BVH needs at least 2 triangles");
+    if (n >= (1 << 30)) return fail(ctx, CAV_ERR_INVALID, "BVH takes fewer than 2^30 triangles");
     if (morton_bits != 30 && morton_bits != 63)
         return fail(ctx, CAV_ERR_INVALID, "morton_bits must be 30 or 63");
     if (opt_rounds < 0) return fail(ctx, CAV_ERR_INVALID, "opt_rounds must be >= 0");
@@ -245,6 +244,7 @@ int cav_bvh_build(cav_ctx* ctx, const float* tris, int n, const float* scene_bou
     *out = nullptr;
     if (!tris) return fail(ctx, CAV_ERR_INVALID, "triangles is null");
     if (n < 2) return fail(ctx, CAV_ERR_INVALID, "BVH needs at least 2 triangles");
+    if (n >= (1 << 30)) return fail(ctx, CAV_ERR_INVALID, "BVH takes fewer than 2^30 triangles");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     float* dt = nullptr;
     int32_t* dm = nullptr;
@@ -337,7 +337,7 @@ int cav_bvh_export(cav_ctx* ctx, const cav_bvh* h, float* box6, float* sa, float
 int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
                             int32_t* prim, float* t) {
     if (!ctx || !h || (nrays > 0 && (!rays || !prim || !t))) return CAV_ERR_INVALID;
-    HIPCHK(ctx, trace_closest(h->tn, h->tt, h->n, h->root, rays, nrays, prim, t, nullptr, nullptr,
+    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, nullptr, nullptr,
                               h->b.counters, ctx->stream));
     return CAV_OK;
 }
@@ -345,7 +345,7 @@ int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, i
 int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
                              const float* limit, int64_t nrays, uint8_t* visible) {
     if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
-    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, nullptr, nullptr, nrays, visible,
+    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, nullptr, nullptr, nrays, visible,
                                h->b.counters, ctx->stream));
     return CAV_OK;
 }
@@ -357,7 +357,7 @@ int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* h, const float* rays, 
         return fail(ctx, CAV_ERR_INVALID, "null ray or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_closest(h->tn, h->tt, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->stream));
+    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -373,7 +373,7 @@ int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
         return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_occluded(h->tn, h->tt, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->stream));
+    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -20,8 +20,7 @@ struct cav_bvh {
     int morton_bits = 30;
     int opt_rounds = 0;
     cav::LbvhBuffers b;
-    cav::TNode* tn = nullptr;
-    cav::TTri* tt = nullptr;
+    cav::TRec* rec = nullptr;   // 2n traversal records (sibling-pair layout)
     float* root = nullptr;      // 8 floats
     int32_t* mesh = nullptr;    // n, original order (may be null)
     cav_bvh_stats stats{};

@@ -69,16 +69,23 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
 //          leaves, -1 for internal nodes; left/right = -1 for leaves)
 struct NodeBox { float4 a, b; };
 
-// Traversal-layout internal node (64 bytes): both child boxes and refs.
-// ref >= 0: internal node index; ref < 0: triangle slot ~ref.
+// Traversal-layout internal node (64 bytes): both child boxes and the
+// children's refs, ref >= 0: an internal node's slot, ref < 0: ~(a leaf's slot).
 struct TNode {
     float4 l0;  // left lo.xyz, left hi.x
     float4 l1;  // left hi.yz, right lo.xy
     float4 r1;  // right lo.z, right hi.xyz
-    int4 refs;  // left ref, right ref, pad, pad
+    int4 refs;  // left ref, right ref, 0, 0 (ref = slot, or ~slot for a leaf)
 };
 
-// Traversal-layout triangle (48 bytes): v0 | prim, e1 | mesh, e2 | pad.
+// A leaf's triangle: v0 | prim, e1 | mesh, e2 | 0 (the first 48 bytes of a
+// leaf's record; the last 16 are zero).
 struct TTri { float4 a, b, c; };
+
+// Traversal records, 64 bytes each (a TNode or a padded TTri), in the
+// sibling-pair layout: the root at slot 0, the children of the internal node
+// of rank p at slots 2p + 2 (left) and 2p + 3 (right). A pair of siblings
+// fills one 128-byte L2 line, and every record sits in one line.
+struct TRec { float4 q0, q1, q2, q3; };
 
 }  // namespace cav

@@ -68,7 +68,7 @@ hipError_t lbvh_build(LbvhBuffers& b, const float* tris, int n, int morton_bits,
                       Marker* mk);
 hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s, Marker* mk, const char* label);
 hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, int n,
-                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s,
+                        bool identity_refs, TRec* rec, float* root, hipStream_t s,
                         Marker* mk);
 int64_t lbvh_partials(int64_t m);
 
@@ -85,18 +85,18 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // Traversal over a flattened BVH of ntris triangles. counters[0] (closest) and
 // counters[1] (occluded) count stack overflows and step-cap exits (either
 // means the results are partial); counters[2..5] hold the ray dispensers.
-hipError_t trace_closest(const TNode* tn, const TTri* tt, int ntris, const float* root,
+hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
                          float* v, uint32_t* counters, hipStream_t s);
-hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s);
 
-hipError_t trace_closest_queue(const TNode* tn, const TTri* tt, int ntris, const float* root,
+hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
                                hipStream_t s);
-hipError_t trace_occluded_queue(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
                                 uint32_t* counters, hipStream_t s);

@@ -328,44 +328,49 @@ __device__ __forceinline__ int32_t ref_of(const int32_t* ref, int n, int id) {
     return id < n - 1 ? id : ~(id - (n - 1));
 }
 
-// copyToBVH (BVH/BVH.cu:304-315) into the traversal layout: one 64-byte node
-// per internal node holding both child boxes, one 48-byte triangle per leaf.
+// copyToBVH (BVH/BVH.cu:304-315) into the traversal layout: one 64-byte
+// record per node in the sibling-pair layout (TRec), an internal node holding
+// both child boxes, a leaf its triangle. A node's slot follows from its
+// parent's rank (ref_of) and which child it is; the root is at slot 0.
 __global__ __launch_bounds__(kBlock) void k_flatten(const NodeBox* __restrict__ nbox,
                                                     const int4* __restrict__ topo, int n,
                                                     const int32_t* __restrict__ ref,
                                                     const float* __restrict__ tris,
                                                     const int32_t* __restrict__ mesh,
-                                                    TNode* __restrict__ tn,
-                                                    TTri* __restrict__ tt,
+                                                    TRec* __restrict__ rec,
                                                     float* __restrict__ root) {
     const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t m = 2 * (int64_t)n - 1;
     if (id >= m) return;
     const int4 tp = topo[id];
-    const int32_t me = ref_of(ref, n, (int)id);
+    int64_t slot = 0;
     if (id == 0) {
         const NodeBox b = nbox[0];
         root[0] = b.a.x; root[1] = b.a.y; root[2] = b.a.z;
         root[3] = b.b.x; root[4] = b.b.y; root[5] = b.b.z;
+    } else {
+        const int par = tp.z;
+        slot = 2 * (int64_t)ref_of(ref, n, par) + 2 + (topo[par].x == (int)id ? 0 : 1);
     }
+    TRec o;
     if (tp.w < 0) {
         const NodeBox bl = nbox[tp.x], br = nbox[tp.y];
-        TNode o;
-        o.l0 = make_float4(bl.a.x, bl.a.y, bl.a.z, bl.b.x);
-        o.l1 = make_float4(bl.b.y, bl.b.z, br.a.x, br.a.y);
-        o.r1 = make_float4(br.a.z, br.b.x, br.b.y, br.b.z);
-        o.refs = make_int4(ref_of(ref, n, tp.x), ref_of(ref, n, tp.y), 0, 0);
-        tn[me] = o;
+        const int32_t base = 2 * ref_of(ref, n, (int)id) + 2;
+        const bool ll = topo[tp.x].w >= 0, rl = topo[tp.y].w >= 0;
+        o.q0 = make_float4(bl.a.x, bl.a.y, bl.a.z, bl.b.x);
+        o.q1 = make_float4(bl.b.y, bl.b.z, br.a.x, br.a.y);
+        o.q2 = make_float4(br.a.z, br.b.x, br.b.y, br.b.z);
+        o.q3 = make_float4(__int_as_float(ll ? ~base : base), __int_as_float(rl ? ~(base + 1) : base + 1), 0.f, 0.f);
     } else {
         const int p = tp.w;
         const v3 v0 = load_v(tris, p, 0), v1 = load_v(tris, p, 1), v2 = load_v(tris, p, 2);
         const v3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
-        TTri o;
-        o.a = make_float4(v0.x, v0.y, v0.z, __int_as_float(p));
-        o.b = make_float4(e1.x, e1.y, e1.z, __int_as_float(mesh ? mesh[p] : 0));
-        o.c = make_float4(e2.x, e2.y, e2.z, 0.f);
-        tt[~me] = o;
+        o.q0 = make_float4(v0.x, v0.y, v0.z, __int_as_float(p));
+        o.q1 = make_float4(e1.x, e1.y, e1.z, __int_as_float(mesh ? mesh[p] : 0));
+        o.q2 = make_float4(e2.x, e2.y, e2.z, 0.f);
+        o.q3 = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    rec[slot] = o;
 }
 
 template <typename K>
@@ -429,12 +434,12 @@ hipError_t lbvh_refs(LbvhBuffers& b, int n, hipStream_t s, Marker* mk, const cha
 }
 
 hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, int n,
-                        bool identity_refs, TNode* tn, TTri* tt, float* root, hipStream_t s,
+                        bool identity_refs, TRec* rec, float* root, hipStream_t s,
                         Marker* mk) {
     const int64_t m = 2 * (int64_t)n - 1;
     const int nb = (int)((m + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_flatten, dim3(nb), dim3(kBlock), 0, s, b.nbox, b.topo, n,
-                       identity_refs ? nullptr : b.ref, tris, mesh, tn, tt, root);
+                       identity_refs ? nullptr : b.ref, tris, mesh, rec, root);
     mark(mk, "flatten");
     return hipGetLastError();
 }

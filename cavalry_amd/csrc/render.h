@@ -29,8 +29,7 @@ struct EnvDev {
 };
 
 struct SceneDev {
-    const TNode* tn;
-    const TTri* tt;
+    const TRec* rec;
     const float* root;
     int ntris;
     const float* soup;         // float[ntris][9], original order

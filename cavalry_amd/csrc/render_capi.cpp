@@ -251,7 +251,7 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 
 namespace cav {
 SceneDev scene_dev(const cav_scene* sc) {
-    return SceneDev{sc->bvh->tn,   sc->bvh->tt,  sc->bvh->root, sc->ntris,   sc->soup,
+    return SceneDev{sc->bvh->rec,  sc->bvh->root, sc->ntris,   sc->soup,
                     sc->tri_mesh,   sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
                     sc->nlights,    sc->normals,  sc->mesh_normals, sc->uv,   sc->texels,
                     sc->tex,        sc->env,      sc->env_miss,     sc->light_dims};

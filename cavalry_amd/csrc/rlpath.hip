@@ -706,7 +706,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
     int qin = 0;
     for (int depth = 0; depth < a.max_depth; ++depth) {
         const int qout = qin ^ 1;
-        if ((e = trace_closest_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.scene.root, a.q[qin].ray,
+        if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[qin].ray,
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
                                      a.trace_counters, s)) != hipSuccess)
             return e;
@@ -719,7 +719,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
         hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
                            rl.slots, a.sh, t, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin,
                            a.sort_perm);
-        if ((e = trace_occluded_queue(a.scene.tn, a.scene.tt, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
+        if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,

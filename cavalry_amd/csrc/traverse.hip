@@ -6,8 +6,9 @@
 // children tested at their parent, the far child stacked, the near one taken
 // next, left first on ties), so hit ids and distances match bit for bit.
 //
-// Layout read: TNode[N-1] (64 B, both child boxes + refs), TTri[N] (48 B,
-// v0/e1/e2 + original index + mesh id), rays float[R][6].
+// Layout read: TRec[2N] (64-byte records in the sibling-pair layout: an
+// internal node's two child boxes and child refs, or a leaf's v0/e1/e2 +
+// original index + mesh id), rays float[R][6].
 #include <cfloat>
 
 #include "dev_common.h"
@@ -91,8 +92,7 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
 // with one atomic. Each ray's node order is the reference's, so results do not
 // depend on which lane or wave traced it.
 template <bool kClosest>
-__global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
-                                                  const TTri* __restrict__ tt,
+__global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
                                                   const float* __restrict__ rays, int64_t nrays_arg,
                                                   const uint32_t* __restrict__ nrays_dev,
@@ -180,12 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TNode* __restrict__ tn,
             ++ovf;
             done = true;
         } else {
-            // One 64-byte fetch for either kind of step (a node, or a triangle
-            // and 16 bytes beyond it), so that a wave with lanes at nodes and
-            // lanes at triangles waits for memory once per step.
+            // One 64-byte fetch for either kind of step, so that a wave with
+            // lanes at nodes and lanes at triangles waits for memory once.
             const bool at_node = cur >= 0;
-            const float4* rec = at_node ? reinterpret_cast<const float4*>(tn + cur)
-                                        : reinterpret_cast<const float4*>(tt + ~cur);
+            const float4* rec = reinterpret_cast<const float4*>(recs + (at_node ? cur : ~cur));
             float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
             int4 refs = reinterpret_cast<const int4*>(rec)[3];
             // (an empty asm that takes the node-only words keeps the compiler
@@ -248,7 +246,8 @@ __global__ void k_clear_words(uint32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-int persistent_blocks(const void* kernel) {
+// Workgroups of a persistent launch: max_per_cu per CU (0: as many as fit).
+int persistent_blocks(const void* kernel, int max_per_cu) {
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -257,38 +256,47 @@ int persistent_blocks(const void* kernel) {
     }
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) per = 4;
+    if (max_per_cu > 0 && per > max_per_cu) per = max_per_cu;
     return cus * per;
 }
 
+// Workgroups per CU for the standalone entry points. Their batches are
+// typically large and incoherent over trees far larger than L2: 3 of the 5
+// workgroups that fit per CU trace 2^24 random rays through 2^24 triangles
+// about 4% faster (fewer rays in flight thrash L2 less). The wavefront loops
+// keep every workgroup that fits: their scenes are smaller and the extra
+// waves hide latency (the Cornell-box render is 5% slower at 3).
+constexpr int kStandalonePerCu = 3;
+
 }  // namespace
 
-hipError_t trace_closest(const TNode* tn, const TTri* tt, int ntris, const float* root,
+hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
                          float* v, uint32_t* counters, hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
     if (e != hipSuccess) return e;
     const void* k = reinterpret_cast<const void*>(&k_trace<true>);
-    int64_t blocks = persistent_blocks(k);
+    int64_t blocks = persistent_blocks(k, kStandalonePerCu);
     const int64_t need = (nrays + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, root, rays,
+    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, root, rays,
                        nrays, nullptr, nullptr, nullptr, nullptr, prim, t, u, v, nullptr, counters,
                        reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
 
-hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
     if (e != hipSuccess) return e;
     const void* k = reinterpret_cast<const void*>(&k_trace<false>);
-    int64_t blocks = persistent_blocks(k);
+    int64_t blocks = persistent_blocks(k, kStandalonePerCu);
     const int64_t need = (nrays + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, nullptr,
+    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, nullptr,
                        rays, nrays, nullptr, limit, src, dst, nullptr, nullptr, nullptr, nullptr, visible,
                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
                        2 * ntris - 1);
@@ -297,33 +305,33 @@ hipError_t trace_occluded(const TNode* tn, const TTri* tt, int ntris, const floa
 
 // Queue variants: the ray count is read on the device (no host round trip),
 // the grid is the full persistent size. counters as above.
-hipError_t trace_closest_queue(const TNode* tn, const TTri* tt, int ntris, const float* root,
+hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
                                hipStream_t s) {
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 2, 2);
     const void* k = reinterpret_cast<const void*>(&k_trace<true>);
-    int64_t blocks = persistent_blocks(k);
+    int64_t blocks = persistent_blocks(k, 0);
     const int64_t need = (capacity + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, root, rays,
+    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, root, rays,
                        capacity, nrays_dev, nullptr, nullptr, nullptr, prim, t, u, v, nullptr,
                        counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
 
-hipError_t trace_occluded_queue(const TNode* tn, const TTri* tt, int ntris, const float* rays,
+hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
                                 uint32_t* counters, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 4, 2);
     const void* k = reinterpret_cast<const void*>(&k_trace<false>);
-    int64_t blocks = persistent_blocks(k);
+    int64_t blocks = persistent_blocks(k, 0);
     const int64_t need = (capacity + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, tn, tt, nullptr,
+    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, nullptr,
                        rays, capacity, nrays_dev, limit, src, dst, nullptr, nullptr, nullptr, nullptr,
                        visible, counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
                        2 * ntris - 1);
