@@ -32,15 +32,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #   morton        reads the soup, writes key + index (+ the 24 B leaf box of
 #                 the canonical model);
 #   karras        reads a key, writes 8 B of children and 8 B of parents;
-#   refit_leaves  (k_leaves) reads the sorted index and the triangle, writes
-#                 the 32 B leaf box/area/cost record, then climbs: each of the
-#                 2(N-1) arrivals at an internal node reads the child's parent
-#                 id (4 B) and does one 4 B atomic read-modify-write (8 B).
+#   refit_leaves  (k_leaves) reads the sorted index, the triangle and the
+#                 leaf's parent id, writes the 32 B leaf box/area/cost record
+#                 and the 16 B leaf topology record, then climbs: both
+#                 arrivals at an internal node read its leaf range (8 B each;
+#                 the gate is an LDS word unless the range leaves the
+#                 workgroup), the second writes its height word (4 B) and
+#                 reads its parent id (4 B).
 ALG_BYTES_LAUNCH_30 = {
     "bounds": lambda n: 36 * n,
     "morton": lambda n: (36 + 4 + 4 + 24) * n,
     "karras": lambda n: (4 + 8 + 8) * n,
-    "refit_leaves": lambda n: (4 + 36 + 32) * n + 2 * (n - 1) * (4 + 8),
+    "refit_leaves": lambda n: (4 + 36 + 4 + 32 + 16) * n + (n - 1) * (2 * 8 + 4 + 4),
 }
 ALG_BYTES_PER_TRI_30 = 248  # whole build, canonical model: S1 68 + S2 68 + S3 20 + S4 92
 

@@ -193,7 +193,8 @@ __device__ __forceinline__ int prefix_len(const K* __restrict__ codes, int keybi
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_karras(const K* __restrict__ codes, int n,
                                                    int keybits, int4* __restrict__ topo,
-                                                   uint32_t* __restrict__ visited) {
+                                                   uint32_t* __restrict__ visited,
+                                                   int2* __restrict__ rng) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
     int d = 1;
@@ -240,6 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const K* __restrict__ codes, 
     reinterpret_cast<int*>(&topo[rid])[2] = i;
     if (i == 0) me[2] = 0;
     visited[i] = 0;
+    rng[i] = make_int2(min(i, j), max(i, j));  // the leaves under node i
 }
 
 // ---- device-wide exclusive scan of leaf flags (restructured trees) ----
@@ -394,8 +396,9 @@ hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hip
     const K* codes = sb.keys[sb.result];
     b.order = sb.vals[sb.result];
     const int nk = (n - 1 + kBlock - 1) / kBlock;
+    // node ranges for the refit's climb, in ref (free until a treelet round)
     hipLaunchKernelGGL(k_karras<K>, dim3(nk), dim3(kBlock), 0, s, codes, n, keybits, b.topo,
-                       b.visited);
+                       b.visited, reinterpret_cast<int2*>(b.ref));
     mark(mk, "karras");
     // refit (BVH/BVH.cu:217-259): leaf boxes + heights, then level by level
     if ((e = levels_from_leaves(b, tris, n, s, mk)) != hipSuccess) return e;

@@ -52,12 +52,21 @@ __device__ __forceinline__ void climb_heights(const int4* __restrict__ topo, uin
 }
 
 // fillLeafBoundingBoxes (BVH/BVH.cu:46-56) for sorted position k, written to
-// leaf node N-1+k, followed by the height climb.
+// leaf node N-1+k, followed by the height climb. In a Karras tree the leaves
+// under internal node p are a contiguous range rng[p] with p at one end, so a
+// node whose range lies within this workgroup's leaves has both climbers here
+// and is gated by an LDS word; only nodes whose range leaves the workgroup use
+// the global word, as in climb_heights. Either way the second visitor stores
+// the node's final (2 << 16 | height).
 __global__ __launch_bounds__(kBlock) void k_leaves(const float* __restrict__ tris, int n,
                                                    const uint32_t* __restrict__ order,
                                                    NodeBox* __restrict__ nbox, int4* topo,
-                                                   uint32_t* hw) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
+                                                   uint32_t* hw, const int2* __restrict__ rng) {
+    __shared__ uint32_t gate[kBlock];
+    const int k0 = blockIdx.x * kBlock;
+    const int k = k0 + threadIdx.x;
+    gate[threadIdx.x] = 0u;
+    __syncthreads();
     if (k >= n) return;
     const int t = (int)order[k];
     v3 a, b, c;
@@ -69,7 +78,19 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const float* __restrict__ tri
     const int parent = topo[node].z;
     nbox[node] = NodeBox{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa))};
     topo[node] = make_int4(-1, -1, parent, t);
-    climb_heights(topo, hw, node, n);
+    int cur = node, p = parent;
+    uint32_t h = 0;
+    for (int step = 0; cur != 0 && step < n; ++step) {
+        const int2 r = rng[p];
+        const uint32_t add = (1u << 16) | (h + 1);
+        const uint32_t old = (r.x >= k0 && r.y < k0 + kBlock) ? atomicAdd(&gate[p - k0], add)
+                                                              : atomicAdd(&hw[p], add);
+        if ((old >> 16) != 1u) return;  // first visitor stops here
+        h = max(old & 0xffffu, h + 1);
+        hw[p] = (2u << 16) | h;
+        cur = p;
+        if (cur != 0) p = topo[cur].z;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_heights(const int4* __restrict__ topo,
@@ -171,7 +192,8 @@ __global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict_
 
 hipError_t levels_from_leaves(LbvhBuffers& b, const float* tris, int n, hipStream_t s, Marker* mk) {
     const int nb = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBlock), 0, s, tris, n, b.order, b.nbox, b.topo, b.visited);
+    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBlock), 0, s, tris, n, b.order, b.nbox, b.topo, b.visited,
+                       reinterpret_cast<const int2*>(b.ref));
     mark(mk, "refit_leaves");
     return hipGetLastError();
 }
