@@ -137,6 +137,15 @@ struct RlArgs {
     float *Q, *newQ, *ldist, *lres, *lnew;
     int32_t *cnt, *total, *lcount, *lnewc;
     int32_t* dims;  // [0] synced dimension at the bounce start, [1] any Q sample
+    // Deterministic table updates: each bounce's updates as records
+    // (key, index) + value, sorted by key (table, bin or (entry, light),
+    // sample, kind) and summed per key in that order. Null: atomics.
+    uint64_t* rec_keys[2];  // rec_cap each
+    uint32_t* rec_idx[2];   // rec_cap each
+    float* rec_val;         // rec_cap
+    uint32_t* rec_count;    // 1
+    uint32_t *rec_hist, *rec_counts, *rec_offsets;  // radix sort work space
+    int64_t rec_cap;
 };
 
 // one spp round (a.spp_done .. a.spp_done + a.spp_round) of each integrator;

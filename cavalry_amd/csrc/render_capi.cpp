@@ -63,8 +63,8 @@ struct RenderState {
     RenderArgs a{};
     hipEvent_t ev[2] = {nullptr, nullptr};
     // rlpath: per-slot state (sized like the sample buffers) and the tables
-    std::vector<void*> rl_bufs, rl_tab;
-    int64_t rl_cap = 0, rl_cells = 0, rl_lcells = 0;
+    std::vector<void*> rl_bufs, rl_tab, rl_rec;
+    int64_t rl_cap = 0, rl_cells = 0, rl_lcells = 0, rl_rec_cap = 0;
     RlArgs rl{};
 };
 
@@ -80,6 +80,7 @@ void free_bufs(RenderState* rs) {
     rs->pix_cap = 0;
     free_list(rs->rl_bufs);
     free_list(rs->rl_tab);
+    free_list(rs->rl_rec);
     rs->rl_cap = rs->rl_cells = rs->rl_lcells = 0;
 }
 
@@ -147,6 +148,32 @@ int grab_into(cav_ctx* ctx, std::vector<void*>& list, T*& p, size_t bytes, const
 template <typename T>
 int grab(cav_ctx* ctx, RenderState* rs, T*& p, size_t bytes, const char* what) {
     return grab_into(ctx, rs->bufs, p, bytes, what);
+}
+
+// Deterministic rlpath: records of up to `cap` table updates per bounce and
+// the work space of their radix sort.
+int ensure_rl_rec(cav_ctx* ctx, RenderState* rs, int64_t cap) {
+    RlArgs& r = rs->rl;
+    if (cap <= rs->rl_rec_cap) return CAV_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    free_list(rs->rl_rec);
+    rs->rl_rec_cap = 0;
+    auto& L = rs->rl_rec;
+    const size_t C = (size_t)cap, T = (size_t)sort_tiles(cap);
+    int rc = CAV_OK;
+    for (int k = 0; k < 2 && rc == CAV_OK; ++k) {
+        rc = grab_into(ctx, L, r.rec_keys[k], 8 * C, "rl update keys");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_idx[k], 4 * C, "rl update order");
+    }
+    if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_val, 4 * C, "rl update values");
+    if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_count, 256, "rl update count");
+    if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_hist, 4 * 8 * 256, "rl update sort histogram");
+    if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_counts, 4 * 256 * T, "rl update sort counts");
+    if (rc == CAV_OK) rc = grab_into(ctx, L, r.rec_offsets, 4 * 256 * T, "rl update sort offsets");
+    if (rc != CAV_OK) { free_list(L); return rc; }
+    rs->rl_rec_cap = cap;
+    r.rec_cap = cap;
+    return CAV_OK;
 }
 
 // rlpath buffers: per-slot state for nsamp tasks, a Q table of `cells` bins
@@ -529,6 +556,14 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     if ((rc = ensure_buffers(ctx, rs, nq, nsamp, npix)) != CAV_OK) return rc;
     const int64_t nent = (int64_t)grid * grid * grid;
     if (rl && (rc = ensure_rl(ctx, rs, nq, nent * 128, nent * sc->nlights)) != CAV_OK) return rc;
+    const bool det = rl && (prm->flags & CAV_RENDER_DETERMINISTIC);
+    if (det) {
+        // update keys hold a Q bin or an (entry, light) index in 25 bits
+        if (nent * 128 > ((int64_t)1 << 25) || nent * sc->nlights > ((int64_t)1 << 25))
+            return fail(ctx, CAV_ERR_INVALID, "deterministic rlpath: more than 2^25 Q bins or light statistics");
+        // at most 3 updates per task: light statistics, previous bin, direct proposal
+        if ((rc = ensure_rl_rec(ctx, rs, 3 * nq)) != CAV_OK) return rc;
+    }
     RenderArgs a = rs->a;
     a.scene = scene_dev(sc);
     a.halton = HaltonDev{rs->primes, rs->start, rs->perm};
@@ -547,6 +582,12 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));
     RCHK(ctx, hipEventRecord(rs->ev[0], s));
     RlArgs r = rs->rl;
+    if (!det) {
+        r.rec_keys[0] = r.rec_keys[1] = nullptr;
+        r.rec_cap = 0;
+    } else {
+        r.rec_cap = 3 * nq;
+    }
     r.grid = grid;
     r.alpha_scale = alpha_scale;
     std::memcpy(r.bounds, sc->bounds, sizeof r.bounds);

@@ -32,6 +32,7 @@
 
 #include "../../include/cavalry.h"
 #include "dev_common.h"
+#include "radix_sort.h"
 #include "render.h"
 #include "shade.h"
 
@@ -277,14 +278,19 @@ struct BinEval {
         u = u * 2 - 1.f;
         xy = sqrtf(1.0f - u * u);
     }
-    __device__ inline float bin(float u, float xy, int p, int c) const {
+    // bin p of a row: |cos| * luminance(f) * q, q = the bin's Q value
+    __device__ inline float bin(float u, float xy, int p, float q) const {
         const v3 d = mk3(xy * cs[p * kBlock], xy * cs[(kNX + p) * kBlock], u);
         if (matte) {  // the same bits as the general case: the local z is dot3(nn, d)
             const float z = dot3(nn, d);
-            return (fabsf(z) * (z * ol.z > 0 ? lum_m : 0.f)) * Qe[c];
+            return (fabsf(z) * (z * ol.z > 0 ? lum_m : 0.f)) * q;
         }
         const float scat = luminance(bsdf.eval(to_local(d, t0, t1, nn), ol));
-        return (fabsf(dot3(d, nn)) * scat) * Qe[c];
+        return (fabsf(dot3(d, nn)) * scat) * q;
+    }
+    // the Q values of bins 4 g .. 4 g + 3 of row ti (one 16-byte load)
+    __device__ inline float4 q4(int ti, int g) const {
+        return reinterpret_cast<const float4*>(Qe)[ti * (kNX / 4) + g];
     }
     // prefix[m] and prefix[m - 1] (0 for m = 0), summed bin by bin
     __device__ inline void prefix(int m, float& pm, float& pm1) const {
@@ -294,11 +300,67 @@ struct BinEval {
             pm1 = s;
             float u, xy;
             row(c / kNX, u, xy);
-            s += bin(u, xy, c % kNX, c);
+            s += bin(u, xy, c % kNX, Qe[c]);
         }
         pm = s;
     }
 };
+
+// For every lane with `on`: fsum[k] += v and cnt[k] += 1, as atomics. Lanes of
+// the wave that share k are summed first, so one atomic pair per distinct k
+// reaches L2 (neighbouring paths share Q entries, cells and lights, and
+// same-address atomics serialise there). Float sums are order-dependent
+// either way, as in the reference. Call with the whole wave converged.
+__device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool on) {
+    const int lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(on);
+    while (todo) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const int64_t lk = __shfl(k, lead, 64);
+        const bool mine = ((todo >> lane) & 1ull) && k == lk;
+        const uint64_t grp = __ballot(mine);
+        float sv = mine ? v : 0.f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o, 64);
+        if (lane == lead) {
+            atomicAdd(fsum + lk, sv);
+            atomicAdd(cnt + lk, (int)__popcll(grp));
+        }
+        todo &= ~grp;
+    }
+}
+
+// Deterministic mode: the table updates of a bounce as records. Key bits:
+// 58 = light statistics (else Q), 33..57 = the table index, 1..32 = the
+// sample, 0 = the kind (0: the previous bin's proposal, 1: the direct-light
+// proposal), the order in which the oracle, which runs the tasks one by one
+// in sample order, adds them.
+struct RlRecords {
+    uint64_t* keys;
+    float* val;
+    uint32_t* count;
+};
+constexpr int kRecKeyShift = 33;
+constexpr int kRecKeyBits = 25;
+
+__device__ inline uint64_t rec_key(bool light, int64_t k, uint32_t sid, int kind) {
+    return (light ? (1ull << 58) : 0ull) | ((uint64_t)k << kRecKeyShift) | ((uint64_t)sid << 1) | (uint64_t)kind;
+}
+// append (key, v) for every lane with `on`; call with the whole wave converged
+__device__ inline void rec_push(const RlRecords& r, uint64_t key, float v, bool on) {
+    const uint64_t m = __ballot(on);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(r.count, (uint32_t)__popcll(m));
+    base = __shfl(base, first, 64);
+    if (on) {
+        const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        r.keys[slot] = key;
+        r.val[slot] = v;
+    }
+}
 
 // The end of computeLighting (after the visibility test), then
 // computeQDistributions (:517-606) and accumulateLightingContribs (:609-647).
@@ -309,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
                                                   const float* __restrict__ hit_u,
                                                   const float* __restrict__ hit_v, float* __restrict__ res,
                                                   const uint32_t* counts, int32_t* dims, int qin,
-                                                  const uint32_t* __restrict__ perm) {
+                                                  const uint32_t* __restrict__ perm, RlRecords rec) {
     __shared__ float trig[2 * kNX][kBlock];
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = j < (int64_t)counts[qin];
@@ -317,9 +379,15 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
     bool valid = false;
+    // the table updates of this task, made after the divergent part
+    bool up_l = false, up_p = false, up_d = false;
+    int64_t k_l = 0, k_p = 0, k_d = 0;
+    float v_l = 0.f, v_p = 0.f, v_d = 0.f;
+    uint32_t sid_up = 0;
     if (live) {
         const int D = dims[0] + 5 + sc.light_dims;
         const uint32_t sid = qi.sid[i];
+        sid_up = sid;
         const float* r = qi.ray + i * 6;
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
         const v3 mult = mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]);
@@ -344,8 +412,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         v3 contrib = scale3(mul3(inc, bsdf.eval(il, ol)), fabsf(dot3(dir, nn)));
         contrib = scale3(contrib, 1.f / (lpdf * sl.sel[i]));
         const float sampled = luminance(inc);
-        atomicAdd(t.lnew + (int64_t)entry * sc.nlights + li, sampled);
-        atomicAdd(t.lnewc + (int64_t)entry * sc.nlights + li, 1);
+        up_l = true;
+        k_l = (int64_t)entry * sc.nlights + li;
+        v_l = sampled;
 
         // 128-bin distribution of |cos| * luminance(f) * Q at one shared offset.
         // Bin c = 16 * theta_row + phi_column. The running (unnormalised) sum
@@ -370,14 +439,19 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         for (int ti = 0; ti < kNY; ++ti) {
             float u, xy;
             be.row(ti, u, xy);
-#pragma unroll 2
-            for (int p = 0; p < kNX; ++p) {
-                const int c = ti * kNX + p;
-                const float q = be.bin(u, xy, p, c);
-                mono = mono && q >= 0.f;
-                sum += q;
-                lc0 = c == lc ? sum : lc0;
-                lc1 = c == lc - 1 ? sum : lc1;
+            for (int g = 0; g < kNX / 4; ++g) {
+                const float4 qg = be.q4(ti, g);
+                const float qs[4] = {qg.x, qg.y, qg.z, qg.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int p = 4 * g + j;
+                    const int c = ti * kNX + p;
+                    const float q = be.bin(u, xy, p, qs[j]);
+                    mono = mono && q >= 0.f;
+                    sum += q;
+                    lc0 = c == lc ? sum : lc0;
+                    lc1 = c == lc - 1 ? sum : lc1;
+                }
             }
 #pragma unroll
             for (int k = 0; k < kNY; ++k) rows[k] = k == ti ? sum : rows[k];
@@ -387,9 +461,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         const float sumw = sum * cell_sa;
         const int prev = rq.prev_entry[i];
         if (prev != -1) {
-            const int64_t k = (int64_t)prev * kXY + rq.prev_cell[i];
-            atomicAdd(t.newQ + k, sumw + sl.emis[i]);
-            atomicAdd(t.cnt + k, 1);
+            up_p = true;
+            k_p = (int64_t)prev * kXY + rq.prev_cell[i];
+            v_p = sumw + sl.emis[i];
         }
         // sample a bin: Distribution1D::sample on cdf[c] = prefix[c] / sum
         int cell = 0;
@@ -409,16 +483,25 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
                 be.row(r, u, xy);
                 float prevc = r == 0 ? 0.f : pre / sum;
                 int found = kNX - 1;
-                for (int p = 0; p < kNX; ++p) {
-                    pre += be.bin(u, xy, p, r * kNX + p);
-                    const float cm = pre / sum;
-                    if (cm >= f || p == kNX - 1) {
-                        found = p;
-                        b0 = cm;
-                        b1 = prevc;
-                        break;
+                bool hit = false;
+                for (int g = 0; g < kNX / 4 && !hit; ++g) {
+                    const float4 qg = be.q4(r, g);
+                    const float qs[4] = {qg.x, qg.y, qg.z, qg.w};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int p = 4 * g + j;
+                        if (hit) break;
+                        pre += be.bin(u, xy, p, qs[j]);
+                        const float cm = pre / sum;
+                        if (cm >= f || p == kNX - 1) {
+                            found = p;
+                            b0 = cm;
+                            b1 = prevc;
+                            hit = true;
+                        } else {
+                            prevc = cm;
+                        }
                     }
-                    prevc = cm;
                 }
                 cell = r * kNX + found;
             } else {
@@ -452,9 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
             const float ratio = fmn(1.f, 0.5f * sa / cell_sa);
             const float prop = sampled * ratio;
             if (prop > Qe[lc]) {
-                const int64_t k = (int64_t)entry * kXY + lc;
-                atomicAdd(t.newQ + k, prop);
-                atomicAdd(t.cnt + k, 1);
+                up_d = true;
+                k_d = (int64_t)entry * kXY + lc;
+                v_d = prop;
             }
         }
         if (sumw != 0) {
@@ -478,6 +561,15 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         res[sid * 3] = res[sid * 3] + add.x;
         res[sid * 3 + 1] = res[sid * 3 + 1] + add.y;
         res[sid * 3 + 2] = res[sid * 3 + 2] + add.z;
+    }
+    if (rec.keys) {
+        rec_push(rec, rec_key(true, k_l, sid_up, 0), v_l, up_l);
+        rec_push(rec, rec_key(false, k_p, sid_up, 0), v_p, up_p);
+        rec_push(rec, rec_key(false, k_d, sid_up, 1), v_d, up_d);
+    } else {
+        wave_add(t.lnew, t.lnewc, k_l, v_l, up_l);
+        wave_add(t.newQ, t.cnt, k_p, v_p, up_p);
+        wave_add(t.newQ, t.cnt, k_d, v_d, up_d);
     }
     if (__ballot(valid) != 0 && (threadIdx.x & 63) == 0 &&
         __hip_atomic_load(&dims[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
@@ -677,6 +769,46 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
     }
 }
 
+// Deterministic mode, before k_qdist: every key slot empty (all ones, sorted
+// last), values numbered in slot order, no records.
+__global__ __launch_bounds__(kBlock) void k_rec_reset(uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                      uint32_t* count, int64_t cap) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) *count = 0u;
+    if (i < cap) {
+        keys[i] = ~0ull;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// After the sort: the first record of each key sums its run in order, from 0
+// as the oracle's += does, and sets the table word and its count (the tables
+// hold zeros there: k_finish_q / k_finish_light cleared them).
+__global__ __launch_bounds__(kBlock) void k_rec_sum(const uint64_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ idx,
+                                                    const float* __restrict__ val, int64_t cap, RlTables t) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= cap) return;
+    const uint64_t k = keys[i];
+    if (k == ~0ull) return;
+    const uint64_t seg = k >> kRecKeyShift;
+    if (i > 0 && (keys[i - 1] >> kRecKeyShift) == seg) return;
+    float sum = 0.f;
+    int32_t n = 0;
+    for (int64_t j = i; j < cap && (keys[j] >> kRecKeyShift) == seg; ++j) {
+        sum += val[idx[j]];
+        ++n;
+    }
+    const int64_t key = (int64_t)(seg & ((1ull << kRecKeyBits) - 1ull));
+    if (seg >> kRecKeyBits) {
+        t.lnew[key] = sum;
+        t.lnewc[key] = n;
+    } else {
+        t.newQ[key] = sum;
+        t.cnt[key] = n;
+    }
+}
+
 }  // namespace
 
 hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables, hipStream_t s) {
@@ -722,8 +854,23 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
             return e;
+        const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count};
+        const unsigned rb = (unsigned)((rl.rec_cap + kBlock - 1) / kBlock);
+        if (rec.keys)
+            hipLaunchKernelGGL(k_rec_reset, dim3(rb), dim3(kBlock), 0, s, rl.rec_keys[0], rl.rec_idx[0],
+                               rl.rec_count, rl.rec_cap);
         hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
-                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, a.sort_perm);
+                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, a.sort_perm,
+                           rec);
+        if (rec.keys) {
+            SortBuffers<uint64_t> sb;
+            sb.keys[0] = rl.rec_keys[0]; sb.keys[1] = rl.rec_keys[1];
+            sb.vals[0] = rl.rec_idx[0]; sb.vals[1] = rl.rec_idx[1];
+            sb.hist = rl.rec_hist; sb.counts = rl.rec_counts; sb.offsets = rl.rec_offsets;
+            if ((e = radix_sort_pairs(sb, rl.rec_cap, 64, s, nullptr)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_rec_sum, dim3(rb), dim3(kBlock), 0, s, sb.keys[sb.result], sb.vals[sb.result],
+                               rl.rec_val, rl.rec_cap, t);
+        }
         hipLaunchKernelGGL(k_finish_q, dim3((unsigned)((nent + kEntryWaves - 1) / kEntryWaves)),
                            dim3(64 * kEntryWaves), 0, s, t, nent, rl.alpha_scale);
         hipLaunchKernelGGL(k_finish_light, dim3((unsigned)((nent + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,

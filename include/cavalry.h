@@ -128,8 +128,12 @@ enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1, CAV_INTEGRATOR_DIRECT
 /* cav_render_params.flags. SORT_BY_MATERIAL shades each bounce's hits in
  * material order (sortLightingQueue, Integrators/PathTracingIntegrator.cu:50-92
  * and ReinforcementLearningPathTracing.cu:226-272, by material only): the
- * film does not change, only the speed. */
-enum { CAV_RENDER_SORT_BY_MATERIAL = 1 };
+ * film does not change, only the speed. DETERMINISTIC makes rlpath's table
+ * updates (the Q proposals and light statistics that the reference adds with
+ * float atomics, ReinforcementLearningPathTracing.cu:453-456,576-585) sum in
+ * sample order, so the film is the same on every run; slower (one sort of
+ * the updates per bounce). Other integrators ignore it. */
+enum { CAV_RENDER_SORT_BY_MATERIAL = 1, CAV_RENDER_DETERMINISTIC = 2 };
 
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),

@@ -6,7 +6,9 @@ bit. Deeper, the Q proposals land through float atomics in a run-dependent
 order, as in the reference, so the comparison is statistical: the GPU and the
 oracle films must agree in mean and be equally close to a high-spp `path`
 image of the same scene (rendered on the GPU, whose `path` films equal the
-oracle's)."""
+oracle's). In deterministic mode the table updates are summed per key in
+sample order, the order in which the oracle adds them, so the films are equal
+bit for bit at every depth and with rounds."""
 import numpy as np
 import pytest
 
@@ -111,4 +113,31 @@ def test_rlpath_rounds_match_oracle_rounds(ctx):
           f"oracle={float(ora.mean()):.5f}", flush=True)
     assert abs(float(film.image.mean()) - float(ora.mean())) < 0.01 * float(ora.mean())
     assert err < 1e-3
+    scene.close()
+
+
+@pytest.mark.parametrize("materials,kind", [("matte", "plain"), ("mixed", "plain"), ("metal", "sphere_light"),
+                                            ("mixed", "rounds")])
+def test_rlpath_deterministic_matches_oracle_exactly(ctx, materials, kind):
+    """Deterministic table updates: the film equals the oracle's bit for bit
+    at max depth 8 (learnt Q tables and light CDFs in use from the second
+    bounce on), also across spp rounds, and two runs give the same film."""
+    print(f"[rlpath] deterministic {materials} {kind}", flush=True)
+    sc = scenes.cornell_box(seed=9, width=40, height=32, materials=materials,
+                            lights="both" if kind == "sphere_light" else "quad")
+    cam = for_scene(sc)
+    spp, rnd = (8, 2) if kind == "rounds" else (4, 0)
+    rgb, w, rays = oracle.render_rlpath(sc, spp, cam, max_depth=8, spp_per_round=rnd)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, spp, integrator="rlpath", camera=cam, max_depth=8, spp_per_round=rnd,
+                        deterministic=True)
+    again, _ = R.render(ctx, scene, spp, integrator="rlpath", camera=cam, max_depth=8, spp_per_round=rnd,
+                        deterministic=True)
+    same = float(np.mean(np.all(film.rgb.view(np.uint32) == rgb.view(np.uint32), axis=-1)))
+    print(f"[rlpath] identical pixels={same:.5f} rays={st['closest_rays'] + st['shadow_rays']} oracle={rays}",
+          flush=True)
+    np.testing.assert_array_equal(film.weight, w)
+    np.testing.assert_array_equal(film.rgb, rgb)
+    np.testing.assert_array_equal(again.rgb, film.rgb)
+    assert st["closest_rays"] + st["shadow_rays"] == rays
     scene.close()
