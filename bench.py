@@ -36,14 +36,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #                 leaf's parent id, writes the 32 B leaf box/area/cost record
 #                 and the 16 B leaf topology record, then climbs: both
 #                 arrivals at an internal node read its leaf range (8 B each;
-#                 the gate is an LDS word unless the range leaves the
-#                 workgroup), the second writes its height word (4 B) and
-#                 reads its parent id (4 B).
+#                 the gate and the children's boxes pass through LDS unless
+#                 the range leaves the workgroup), the second writes the
+#                 node's height word (4 B) and 32 B box record and reads its
+#                 parent id (4 B).
 ALG_BYTES_LAUNCH_30 = {
     "bounds": lambda n: 36 * n,
     "morton": lambda n: (36 + 4 + 4 + 24) * n,
     "karras": lambda n: (4 + 8 + 8) * n,
-    "refit_leaves": lambda n: (4 + 36 + 4 + 32 + 16) * n + (n - 1) * (2 * 8 + 4 + 4),
+    "refit_leaves": lambda n: (4 + 36 + 4 + 32 + 16) * n + (n - 1) * (2 * 8 + 4 + 32 + 4),
 }
 ALG_BYTES_PER_TRI_30 = 248  # whole build, canonical model: S1 68 + S2 68 + S3 20 + S4 92
 

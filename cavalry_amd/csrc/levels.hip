@@ -51,18 +51,32 @@ __device__ __forceinline__ void climb_heights(const int4* __restrict__ topo, uin
     }
 }
 
+// computeBounds (BVH/BVH.cu:217-259): the box, area and cost of a node from
+// its left and right children's.
+__device__ __forceinline__ NodeBox combine(const NodeBox& bl, const NodeBox& br) {
+    const v3 lo = mk3(fmn(bl.a.x, br.a.x), fmn(bl.a.y, br.a.y), fmn(bl.a.z, br.a.z));
+    const v3 hi = mk3(fmx(bl.b.x, br.b.x), fmx(bl.b.y, br.b.y), fmx(bl.b.z, br.b.z));
+    const float sa = surface_area(lo, hi);
+    return NodeBox{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, internal_cost3(sa, bl.b.w, br.b.w))};
+}
+
 // fillLeafBoundingBoxes (BVH/BVH.cu:46-56) for sorted position k, written to
 // leaf node N-1+k, followed by the height climb. In a Karras tree the leaves
 // under internal node p are a contiguous range rng[p] with p at one end, so a
 // node whose range lies within this workgroup's leaves has both climbers here
 // and is gated by an LDS word; only nodes whose range leaves the workgroup use
 // the global word, as in climb_heights. Either way the second visitor stores
-// the node's final (2 << 16 | height).
+// the node's final (2 << 16 | height). At a workgroup-local node each climber
+// also leaves its child's box in LDS before the gate, and the second one
+// refits the node from the two (the reference's computeBounds as a climb,
+// BVH/BVH.cu:217-259, with LDS in place of a device-wide fence). Nodes above
+// a boundary-crossing node all cross too; levels_refit does those.
 __global__ __launch_bounds__(kBlock) void k_leaves(const float* __restrict__ tris, int n,
                                                    const uint32_t* __restrict__ order,
                                                    NodeBox* __restrict__ nbox, int4* topo,
                                                    uint32_t* hw, const int2* __restrict__ rng) {
     __shared__ uint32_t gate[kBlock];
+    __shared__ NodeBox sbox[2 * kBlock];  // per local node: its left, right child's box
     const int k0 = blockIdx.x * kBlock;
     const int k = k0 + threadIdx.x;
     gate[threadIdx.x] = 0u;
@@ -76,19 +90,34 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const float* __restrict__ tri
     const float sa = surface_area(lo, hi);
     const int node = n - 1 + k;
     const int parent = topo[node].z;
-    nbox[node] = NodeBox{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa))};
+    NodeBox cb{make_float4(lo.x, lo.y, lo.z, sa), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa))};
+    nbox[node] = cb;
     topo[node] = make_int4(-1, -1, parent, t);
-    int cur = node, p = parent;
+    int cur = node, p = parent, first = k;  // first = the first leaf under cur
     uint32_t h = 0;
     for (int step = 0; cur != 0 && step < n; ++step) {
         const int2 r = rng[p];
         const uint32_t add = (1u << 16) | (h + 1);
-        const uint32_t old = (r.x >= k0 && r.y < k0 + kBlock) ? atomicAdd(&gate[p - k0], add)
-                                                              : atomicAdd(&hw[p], add);
+        const bool local = r.x >= k0 && r.y < k0 + kBlock;
+        const int side = first == r.x ? 0 : 1;  // cur is p's left child iff their ranges start together
+        uint32_t old;
+        if (local) {
+            sbox[2 * (p - k0) + side] = cb;
+            __threadfence_block();
+            old = atomicAdd(&gate[p - k0], add);
+        } else {
+            old = atomicAdd(&hw[p], add);
+        }
         if ((old >> 16) != 1u) return;  // first visitor stops here
         h = max(old & 0xffffu, h + 1);
         hw[p] = (2u << 16) | h;
+        if (local) {
+            __threadfence_block();
+            cb = side == 0 ? combine(cb, sbox[2 * (p - k0) + 1]) : combine(sbox[2 * (p - k0)], cb);
+            nbox[p] = cb;
+        }
         cur = p;
+        first = r.x;
         if (cur != 0) p = topo[cur].z;
     }
 }
@@ -163,27 +192,36 @@ __global__ __launch_bounds__(kBlock) void k_level_scatter(const int4* __restrict
 // computeBounds (BVH/BVH.cu:217-259) for one internal node whose children are final.
 __device__ __forceinline__ void refit_node(int id, NodeBox* nbox, const int4* __restrict__ topo) {
     const int4 tp = topo[id];
-    const NodeBox bl = nbox[tp.x], br = nbox[tp.y];
-    const v3 lo = mk3(fmn(bl.a.x, br.a.x), fmn(bl.a.y, br.a.y), fmn(bl.a.z, br.a.z));
-    const v3 hi = mk3(fmx(bl.b.x, br.b.x), fmx(bl.b.y, br.b.y), fmx(bl.b.z, br.b.z));
-    const float sa = surface_area(lo, hi);
-    nbox[id] = NodeBox{make_float4(lo.x, lo.y, lo.z, sa),
-                       make_float4(hi.x, hi.y, hi.z, internal_cost3(sa, bl.b.w, br.b.w))};
+    nbox[id] = combine(nbox[tp.x], nbox[tp.y]);
+}
+
+// Nodes whose leaves lie within one k_leaves workgroup were refitted there.
+constexpr int kClimbLeaves = kBlock;
+__device__ __forceinline__ bool climb_refitted(const int2* rng, int id) {
+    if (!rng) return false;
+    const int2 r = rng[id];
+    return r.x / kClimbLeaves == r.y / kClimbLeaves;
 }
 
 __global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restrict__ list, int count,
-                                                        NodeBox* nbox, const int4* __restrict__ topo) {
+                                                        NodeBox* nbox, const int4* __restrict__ topo,
+                                                        const int2* __restrict__ rng) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < count) refit_node((int)list[i], nbox, topo);
+    if (i < count) {
+        const int id = (int)list[i];
+        if (!climb_refitted(rng, id)) refit_node(id, nbox, topo);
+    }
 }
 
 // Levels h0..h1 in one workgroup; a barrier separates the levels.
 __global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ off, int h0, int h1,
-                                                     NodeBox* nbox, const int4* __restrict__ topo) {
+                                                     NodeBox* nbox, const int4* __restrict__ topo,
+                                                     const int2* __restrict__ rng) {
     for (int h = h0; h <= h1; ++h) {
         const uint32_t b = off[h], e = off[h + 1];
-        for (uint32_t i = b + threadIdx.x; i < e; i += 1024) refit_node((int)list[i], nbox, topo);
+        for (uint32_t i = b + threadIdx.x; i < e; i += 1024)
+            if (!climb_refitted(rng, (int)list[i])) refit_node((int)list[i], nbox, topo);
         __syncthreads();
     }
 }
@@ -244,13 +282,13 @@ hipError_t levels_refit(LbvhBuffers& b, hipStream_t s, Marker* mk) {
         const uint32_t rest = b.h_off[top + 1] - b.h_off[h];
         if (rest <= (uint32_t)kTailNodes) {
             hipLaunchKernelGGL(k_refit_tail, dim3(1), dim3(1024), 0, s, b.lvl_list, b.lvl_off, h, top,
-                               b.nbox, b.topo);
+                               b.nbox, b.topo, reinterpret_cast<const int2*>(b.ref));
             break;
         }
         const int cnt = (int)b.h_lvl[h];
         if (cnt == 0) continue;
         hipLaunchKernelGGL(k_refit_level, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                           b.lvl_list + b.h_off[h], cnt, b.nbox, b.topo);
+                           b.lvl_list + b.h_off[h], cnt, b.nbox, b.topo, reinterpret_cast<const int2*>(b.ref));
     }
     mark(mk, "refit");
     return hipGetLastError();

@@ -81,7 +81,7 @@ void free_bufs(RenderState* rs) {
     free_list(rs->rl_bufs);
     free_list(rs->rl_tab);
     free_list(rs->rl_rec);
-    rs->rl_cap = rs->rl_cells = rs->rl_lcells = 0;
+    rs->rl_cap = rs->rl_cells = rs->rl_lcells = rs->rl_rec_cap = 0;
 }
 
 // Halton digit permutations (Samplers/HaltonSampler.cu:7-57): the first 1024

@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import RenderParams, ServeConfig, ServeResult, ServeStats, check, lib
-from .render import INTEGRATORS, Film, Scene, render
+from .render import DETERMINISTIC, INTEGRATORS, Film, Scene, render
 
 QUEUED, RUNNING, DONE, CANCELLED, FAILED = range(5)  # CAV_SERVE_*
 
@@ -40,11 +40,14 @@ class Server:
         ctx.own(self)
 
     def submit(self, scene: Scene, spp: int, integrator: str = "path", camera=None, max_depth=None,
-               q_grid: int = 32, alpha_scale: float = 1.0) -> Ticket:
+               q_grid: int = 32, alpha_scale: float = 1.0, deterministic: bool = False) -> Ticket:
+        """Queues a render (the parameters of render.render); deterministic
+        applies to rlpath as there."""
         cam = camera if camera is not None else scene.setup.camera
         depth = max_depth if max_depth is not None else (scene.arrays.max_depth if scene.arrays is not None
                                                          else scene.setup.max_depth)
-        prm = RenderParams(INTEGRATORS[integrator], int(spp), int(depth), 0, int(q_grid), float(alpha_scale), 0)
+        prm = RenderParams(INTEGRATORS[integrator], int(spp), int(depth), DETERMINISTIC if deterministic else 0,
+                           int(q_grid), float(alpha_scale), 0)
         film = Film(np.zeros((cam.height, cam.width, 3), np.float32), np.zeros((cam.height, cam.width), np.float32))
         rid = ctypes.c_int64()
         check(self.ctx.handle, lib().cav_serve_submit(

@@ -141,3 +141,19 @@ def test_rlpath_deterministic_matches_oracle_exactly(ctx, materials, kind):
     np.testing.assert_array_equal(again.rgb, film.rgb)
     assert st["closest_rays"] + st["shadow_rays"] == rays
     scene.close()
+
+
+def test_rlpath_deterministic_after_buffers_grow(ctx):
+    """A larger render in between reallocates the render buffers; the next
+    deterministic render allocates its update records again and gives the
+    same film as before (regression: its capacity was kept while the
+    records were freed)."""
+    print("[rlpath] deterministic, buffers grow in between", flush=True)
+    sc = scenes.cornell_box(seed=5, width=32, height=24)
+    big = scenes.cornell_box(seed=5, width=96, height=80)
+    scene = R.Scene(ctx, sc)
+    first, _ = R.render(ctx, scene, 2, integrator="rlpath", max_depth=4, deterministic=True)
+    R.render(ctx, scene, 8, integrator="path", camera=for_scene(big), max_depth=2)
+    again, _ = R.render(ctx, scene, 2, integrator="rlpath", max_depth=4, deterministic=True)
+    np.testing.assert_array_equal(again.rgb, first.rgb)
+    scene.close()

@@ -5,7 +5,7 @@ bit for bit, the same request rendered alone by cav_render, although the
 requests are admitted in chunks and share every launch with other requests
 at other depths; rlpath films (rendered on the server's second stream) match
 the alone render within rel-MSE 1e-4 (their Q proposals land through float
-atomics). Also: requests submitted while others run start before those
+atomics), and bit for bit in deterministic mode. Also: requests submitted while others run start before those
 finish, cancellation, hipGraph replay, and argument errors."""
 import time
 
@@ -42,6 +42,7 @@ REQUESTS = [  # scene, width, height, spp, integrator, depth, camera shift
     (1, 40, 40, 9, "path", 5, 0.05),
     (0, 96, 64, 2, "directlighting", 1, -0.05),
     (1, 64, 64, 3, "rlpath", 6, 0.0),
+    (0, 40, 32, 2, "rlpath_det", 5, 0.03),
     (0, 33, 17, 16, "path", 12, 0.1),
     (1, 80, 60, 1, "directlighting", 1, 0.0),
     (0, 48, 48, 6, "path", 3, -0.1),
@@ -57,14 +58,18 @@ def test_served_films_equal_alone(ctx, two_scenes, graphs):
     tickets = []
     for s, w, h, spp, integ, depth, shift in REQUESTS:
         arr, sc = two_scenes[s]
-        tickets.append(srv.submit(sc, spp, integ, camera=cam_for(arr, w, h, shift), max_depth=depth))
+        det = integ == "rlpath_det"
+        tickets.append(srv.submit(sc, spp, "rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
+                                  max_depth=depth, deterministic=det))
     outs = [srv.poll(t, wait=True) for t in tickets]
     stats = srv.close()
     assert stats["pools"] == 2 and stats["steps"] > 0
     for (s, w, h, spp, integ, depth, shift), t, out in zip(REQUESTS, tickets, outs):
         assert out["status"] == DONE, out
         arr, sc = two_scenes[s]
-        alone, _ = R.render(ctx, sc, spp, integrator=integ, camera=cam_for(arr, w, h, shift), max_depth=depth)
+        det = integ == "rlpath_det"
+        alone, _ = R.render(ctx, sc, spp, integrator="rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
+                            max_depth=depth, deterministic=det)
         np.testing.assert_array_equal(t.film.weight, alone.weight)
         if integ == "rlpath":
             err = rel_mse(t.film.image, alone.image)
