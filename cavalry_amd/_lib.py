@@ -68,7 +68,8 @@ class SceneDesc(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("arena_bytes", ctypes.c_int64), ("host_arena_bytes", ctypes.c_int64),
                 ("pool_tasks", ctypes.c_int64), ("chunk_samples", ctypes.c_int64),
-                ("max_steps_ahead", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
+                ("max_steps_ahead", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
+                ("rl_workers", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):

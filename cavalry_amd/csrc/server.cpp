@@ -15,9 +15,11 @@
 // do not fit the arena wait (admission by memory).
 //
 // rlpath requests learn from every sample of a bounce of their own render,
-// so they are not mixed: a second worker renders them one at a time through
-// cav_render on a context of its own; their kernels overlap the pools' on the
-// device (separate streams, default priority).
+// so they are not mixed: rl_workers threads render them whole through
+// cav_render, each on a context (stream) of its own, so they overlap each
+// other and the pools on the device (separate streams, default priority). A
+// small rlpath request leaves most of the device idle while its slowest rays
+// finish each bounce; several at once fill it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -169,8 +171,9 @@ struct cav_server {
     Arena dev_arena, host_arena;
     char* dev_base = nullptr;
     char* host_base = nullptr;
-    std::thread worker, rl_worker;
-    cav_ctx* rl_ctx = nullptr;
+    std::thread worker;
+    std::vector<std::thread> rl_workers;
+    std::vector<cav_ctx*> rl_ctxs;
     unsigned long long pool_rays[2] = {0, 0};
     int64_t rl_rays[2] = {0, 0};
     std::string error;
@@ -503,7 +506,7 @@ void worker_main(cav_server* sv) {
     }
 }
 
-void rl_main(cav_server* sv) {
+void rl_main(cav_server* sv, cav_ctx* rl_ctx) {
     std::unique_lock<std::mutex> lk(sv->mu);
     while (true) {
         sv->cv.wait(lk, [&] { return sv->stop || !sv->rl_queue.empty(); });
@@ -522,7 +525,7 @@ void rl_main(cav_server* sv) {
         r->t_start = now_ms(sv->t0);
         lk.unlock();
         cav_render_stats st{};
-        const int rc = cav_render(sv->rl_ctx, r->scene, &r->cam, &r->prm, r->out_rgb, r->out_w, &st);
+        const int rc = cav_render(rl_ctx, r->scene, &r->cam, &r->prm, r->out_rgb, r->out_w, &st);
         lk.lock();
         r->t_done = now_ms(sv->t0);
         r->closest = st.closest_rays;
@@ -555,6 +558,8 @@ int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) 
     if (c.max_steps_ahead <= 0) c.max_steps_ahead = 4;
     if (c.chunk_samples <= 0) c.chunk_samples = c.pool_tasks / 4;
     if (c.host_arena_bytes <= 0) c.host_arena_bytes = (int64_t)1 << 30;
+    if (c.rl_workers <= 0) c.rl_workers = 4;
+    if (c.rl_workers > 16) return fail(ctx, CAV_ERR_INVALID, "rl_workers above 16");
     if (c.pool_tasks >= ((int64_t)1 << 31)) return fail(ctx, CAV_ERR_INVALID, "pool_tasks must be below 2^31");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CAV_ERR_HIP, "hipSetDevice");
     int rc = halton_dev(ctx, &sv->halton);
@@ -567,15 +572,20 @@ int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) 
     }
     sv->dev_arena.init(sv->dev_base, (size_t)c.arena_bytes);
     sv->host_arena.init(sv->host_base, (size_t)c.host_arena_bytes);
-    if ((rc = cav_ctx_create(ctx->device, &sv->rl_ctx)) != CAV_OK) {
-        (void)hipFree(sv->dev_base);
-        (void)hipHostFree(sv->host_base);
-        return fail(ctx, rc, "server rlpath context");
+    for (int k = 0; k < c.rl_workers; ++k) {
+        cav_ctx* rc_ctx = nullptr;
+        if ((rc = cav_ctx_create(ctx->device, &rc_ctx)) != CAV_OK) {
+            for (cav_ctx* x : sv->rl_ctxs) cav_ctx_destroy(x);
+            (void)hipFree(sv->dev_base);
+            (void)hipHostFree(sv->host_base);
+            return fail(ctx, rc, "server rlpath context");
+        }
+        sv->rl_ctxs.push_back(rc_ctx);
     }
     sv->t0 = std::chrono::steady_clock::now();
     cav_server* raw = sv.get();
     raw->worker = std::thread(worker_main, raw);
-    raw->rl_worker = std::thread(rl_main, raw);
+    for (cav_ctx* x : raw->rl_ctxs) raw->rl_workers.emplace_back(rl_main, raw, x);
     *out = sv.release();
     return CAV_OK;
 }
@@ -662,7 +672,8 @@ int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
         sv->cv.notify_all();
     }
     if (sv->worker.joinable()) sv->worker.join();
-    if (sv->rl_worker.joinable()) sv->rl_worker.join();
+    for (auto& t : sv->rl_workers)
+        if (t.joinable()) t.join();
     if (stats) {
         stats->pool_closest_rays = (int64_t)sv->pool_rays[0];
         stats->pool_shadow_rays = (int64_t)sv->pool_rays[1];
@@ -676,7 +687,7 @@ int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
     for (auto& p : sv->pools) destroy_pool(p.get());
     for (auto& kv : sv->reqs)
         if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
-    if (sv->rl_ctx) cav_ctx_destroy(sv->rl_ctx);
+    for (cav_ctx* x : sv->rl_ctxs) cav_ctx_destroy(x);
     if (sv->dev_base) (void)hipFree(sv->dev_base);
     if (sv->host_base) (void)hipHostFree(sv->host_base);
     delete sv;

@@ -248,12 +248,13 @@ __global__ void k_clear_words(uint32_t* p, int n) {
 
 // Workgroups of a persistent launch: max_per_cu per CU (0: as many as fit).
 int persistent_blocks(const void* kernel, int max_per_cu) {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+    // (initialised once, thread-safely: contexts on several threads launch here)
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return n;
+    }();
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) per = 4;
     if (max_per_cu > 0 && per > max_per_cu) per = max_per_cu;

@@ -30,10 +30,10 @@ class Ticket:
 
 class Server:
     def __init__(self, ctx, arena_bytes=0, host_arena_bytes=0, pool_tasks=0, chunk_samples=0,
-                 max_steps_ahead=0, use_graphs=False):
+                 max_steps_ahead=0, use_graphs=False, rl_workers=0):
         self.ctx = ctx
         cfg = ServeConfig(int(arena_bytes), int(host_arena_bytes), int(pool_tasks), int(chunk_samples),
-                          int(max_steps_ahead), 1 if use_graphs else 0)
+                          int(max_steps_ahead), 1 if use_graphs else 0, int(rl_workers))
         h = ctypes.c_void_p()
         check(ctx.handle, lib().cav_serve_open(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

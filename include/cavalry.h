@@ -350,7 +350,8 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
  * rays of new requests join at every bounce step, tasks of many requests and
  * depths share each launch, and each film is bit-identical to the same
  * request rendered alone by cav_render. rlpath requests (batch-coupled
- * learning) run one at a time on a second stream, overlapping the pools.
+ * learning) are rendered whole, `rl_workers` at a time, each on a context and
+ * stream of its own, overlapping each other and the pools.
  * Admission is bounded by the pool's task slots and by a device arena for
  * per-request sample and film buffers; the server allocates nothing while
  * serving. All functions are thread-safe. */
@@ -362,6 +363,7 @@ typedef struct {
     int64_t chunk_samples;     /* admission quantum per request per step (0: pool_tasks / 4) */
     int32_t max_steps_ahead;   /* bounce steps the host may enqueue ahead of the device (0: 4) */
     int32_t use_graphs;        /* replay each pool's bounce sequence from a captured hipGraph */
+    int32_t rl_workers;        /* rlpath requests rendered at once (0: 4) */
 } cav_serve_config;
 enum { CAV_SERVE_QUEUED = 0, CAV_SERVE_RUNNING = 1, CAV_SERVE_DONE = 2, CAV_SERVE_CANCELLED = 3,
        CAV_SERVE_FAILED = 4 };
