@@ -110,7 +110,7 @@ def _time_renders(ctx, scene, cam, integrator: str, renders: int) -> dict:
     }
 
 
-def render_bench(renders: int, cpu: bool) -> dict:
+def render_bench(renders: int, cpu: bool, tune_launch: bool = True) -> dict:
     """Path tracing on the Cornell-box workload: 512x512, 16 spp, max depth 8,
     no treelet rounds, with the `path` and `rlpath` integrators.
     Mrays/s = (closest + shadow rays) / device time."""
@@ -121,6 +121,10 @@ def render_bench(renders: int, cpu: bool) -> dict:
     scene = R.Scene(ctx, sc, bvh_rounds=0)
     cam = for_scene(sc)
     out = {"workload": "Cornell box (4014 triangles), 512x512, 16 spp, path, max depth 8, Halton"}
+    if tune_launch:  # the render loops' traversal launch shape, autotuned on this render
+        from cavalry_amd import tune
+        res = tune.tune_render(ctx, scene, 16, integrator="path", camera=cam, reps=1, use_cache=False)
+        out["trace_launch"] = {"shape": res["shape"], "candidates": len(res["timings"])}
     out.update(_time_renders(ctx, scene, cam, "path", renders))
     rl = {"workload": "same scene and sampling, rlpath (32^3 Q entries x 128 bins, reference learning rate)"}
     rl.update(_time_renders(ctx, scene, cam, "rlpath", renders))
@@ -234,6 +238,10 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default traversal launch shapes instead of autotuning them")
+    ap.add_argument("--tune-rays", type=int, default=1 << 22,
+                    help="rays (a prefix of the workload's) the autotuner times each candidate on")
     ap.add_argument("--probe", action="store_true",
                     help="one LBVH build, one closest-hit and one shadow trace, and one 2^28-byte "
                          "elementwise add, then exit (for rocprofv3 --pmc runs)")
@@ -272,6 +280,17 @@ def main() -> int:
     stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
     log("[bench] first build (allocates)")
     bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
+    launch = None
+    if not args.probe and not args.no_tune:
+        # the traversal launch shape, autotuned on a prefix of the same rays
+        # (before the warmup; not part of the timed steps)
+        from cavalry_amd import tune
+        tr = min(args.tune_rays, r)
+        t0 = time.perf_counter()
+        res = tune.tune_trace(ctx, bvh, rays, tr, shadow_rays=srays, limit=lim, reps=1, use_cache=False)
+        launch = {"shape": res["shape"], "tuned_on_rays": tr, "candidates": len(res["timings"]),
+                  "tune_s": round(time.perf_counter() - t0, 2)}
+        log(f"[bench] autotuned launch shape {res['shape']} in {launch['tune_s']} s")
     if args.probe:
         check(ctx.handle, lib().cav_trace_closest_async(
             ctx.handle, bvh.handle, rays.data_ptr(), r, hit_prim.data_ptr(), hit_t.data_ptr()))
@@ -389,6 +408,7 @@ def main() -> int:
         "hit_fraction": round(float((hit_prim >= 0).float().mean().item()), 4),
         "visible_fraction": round(float(vis.float().mean().item()), 4),
         "roofline": roofline,
+        "trace_launch": launch,
         "cpu_baseline": None,
     }
     bvh.close()
@@ -396,7 +416,7 @@ def main() -> int:
     del tris, rays, srays, lim, hit_prim, hit_t, vis
     if not args.no_render:
         log("[bench] path tracing (Cornell box 512x512x16)")
-        out["path_tracing"] = render_bench(args.renders, not args.no_cpu)
+        out["path_tracing"] = render_bench(args.renders, not args.no_cpu, not args.no_tune)
     if not args.no_serve:
         log("[bench] serving")
         out["serving"] = serving_bench(args.serve_requests, args.serve_rate, args.seed)

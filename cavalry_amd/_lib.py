@@ -72,6 +72,11 @@ class ServeConfig(ctypes.Structure):
                 ("rl_workers", ctypes.c_int32)]
 
 
+class TraceLaunch(ctypes.Structure):
+    _fields_ = [("workgroups_per_cu", ctypes.c_int32), ("ray_chunk", ctypes.c_int32),
+                ("lds_stack", ctypes.c_int32)]
+
+
 class ServeResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("error", ctypes.c_int32), ("submit_ms", ctypes.c_double),
                 ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
@@ -125,6 +130,8 @@ _SIGS = {
     "cav_pbrt_get_lights": ([P, P, P, P, P, P], I),
     "cav_pbrt_destroy": ([P], None),
     "cav_scene_create_from_pbrt": ([P, ctypes.c_char_p, P, ctypes.POINTER(P), P], I),
+    "cav_trace_set_launch": ([P, ctypes.c_int32, ctypes.POINTER(TraceLaunch)], I),
+    "cav_trace_get_launch": ([P, ctypes.c_int32, ctypes.POINTER(TraceLaunch)], I),
     "cav_serve_open": ([P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P)], I),
     "cav_serve_submit": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), P, P,
                           ctypes.POINTER(I64)], I),

@@ -334,11 +334,33 @@ int cav_bvh_export(cav_ctx* ctx, const cav_bvh* h, float* box6, float* sa, float
     return CAV_OK;
 }
 
+int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* l) {
+    if (!ctx || !l) return CAV_ERR_INVALID;
+    if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
+        return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
+    if (l->workgroups_per_cu < 0 || l->workgroups_per_cu > 64)
+        return fail(ctx, CAV_ERR_INVALID, "workgroups_per_cu must be in 0..64");
+    if (l->ray_chunk < 64 || l->ray_chunk > 65536)
+        return fail(ctx, CAV_ERR_INVALID, "ray_chunk must be in 64..65536");
+    if (l->lds_stack != 32 && l->lds_stack != 48) return fail(ctx, CAV_ERR_INVALID, "lds_stack must be 32 or 48");
+    ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack};
+    return CAV_OK;
+}
+
+int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
+    if (!ctx || !out) return CAV_ERR_INVALID;
+    if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
+        return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
+    const TraceLaunch& l = ctx->launch[which];
+    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth};
+    return CAV_OK;
+}
+
 int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
                             int32_t* prim, float* t) {
     if (!ctx || !h || (nrays > 0 && (!rays || !prim || !t))) return CAV_ERR_INVALID;
     HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, nullptr, nullptr,
-                              h->b.counters, ctx->stream));
+                              h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
     return CAV_OK;
 }
 
@@ -346,7 +368,7 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
                              const float* limit, int64_t nrays, uint8_t* visible) {
     if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
     HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, nullptr, nullptr, nrays, visible,
-                               h->b.counters, ctx->stream));
+                               h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
     return CAV_OK;
 }
 
@@ -357,7 +379,7 @@ int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* h, const float* rays, 
         return fail(ctx, CAV_ERR_INVALID, "null ray or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->stream));
+    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -373,7 +395,7 @@ int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
         return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->stream));
+    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -85,20 +85,29 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // Traversal over a flattened BVH of ntris triangles. counters[0] (closest) and
 // counters[1] (occluded) count stack overflows and step-cap exits (either
 // means the results are partial); counters[2..5] hold the ray dispensers.
+// Launch shape of the persistent traversal kernels. Each ray's node order is
+// fixed, so results do not depend on it; the autotuner picks it.
+struct TraceLaunch {
+    int per_cu = 0;      // workgroups per CU (0: as many as fit)
+    int chunk = 256;     // rays a wave takes from the dispenser at a time (64..4096)
+    int lds_depth = 32;  // stack entries in LDS (32 or 48); deeper ones in scratch
+};
+
 hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* counters, hipStream_t s);
+                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s);
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
-                          int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s);
+                          int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
+                          hipStream_t s);
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
-                               hipStream_t s);
+                               const TraceLaunch& L, hipStream_t s);
 hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
-                                uint32_t* counters, hipStream_t s);
+                                uint32_t* counters, const TraceLaunch& L, hipStream_t s);
 
 }  // namespace cav

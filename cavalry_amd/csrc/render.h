@@ -81,6 +81,7 @@ struct Shadow {
 
 struct RenderArgs {
     SceneDev scene;
+    TraceLaunch launch;      // traversal launch shape
     HaltonDev halton;
     CameraDev cam;
     int width, height, spp, max_depth;

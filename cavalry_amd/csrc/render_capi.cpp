@@ -566,6 +566,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     }
     RenderArgs a = rs->a;
     a.scene = scene_dev(sc);
+    a.launch = ctx->launch[CAV_TRACE_WAVEFRONT];
     a.halton = HaltonDev{rs->primes, rs->start, rs->perm};
     std::memcpy(a.cam.m, cam->cam_to_world, sizeof a.cam.m);
     a.cam.depth = cam->depth;

@@ -840,7 +840,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
         const int qout = qin ^ 1;
         if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[qin].ray,
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
-                                     a.trace_counters, s)) != hipSuccess)
+                                     a.trace_counters, a.launch, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_dim_max, dim3((unsigned)((nsamp + kMaxBlock - 1) / kMaxBlock)), dim3(kMaxBlock), 0, s,
                            a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
@@ -852,7 +852,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
                            rl.slots, a.sh, t, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin,
                            a.sort_perm);
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
-                                      a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, s)) != hipSuccess)
+                                      a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s)) != hipSuccess)
             return e;
         const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count};
         const unsigned rb = (unsigned)((rl.rec_cap + kBlock - 1) / kBlock);

@@ -27,6 +27,7 @@ struct Chunk {
 
 struct PoolDev {
     SceneDev scene;
+    TraceLaunch launch;  // traversal launch shape
     HaltonDev halton;
     Queue q[2];
     int32_t* rid[2];   // request slot per task

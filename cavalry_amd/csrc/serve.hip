@@ -153,11 +153,11 @@ hipError_t pool_bounce(const PoolDev& p, int qin, hipStream_t s) {
     hipError_t e = trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray,
                                        p.counts + qin, p.cap, p.hit_prim, p.hit_t,
                                        p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
-                                       p.trace_counters, s);
+                                       p.trace_counters, p.launch, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pool_shade, dim3(nb), dim3(kBlock), 0, s, p, qin);
     e = trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit,
-                             p.sh.src, p.sh.dst, p.vis, p.trace_counters, s);
+                             p.sh.src, p.sh.dst, p.vis, p.trace_counters, p.launch, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pool_nee, dim3(nb), dim3(kBlock), 0, s, p);
     hipLaunchKernelGGL(k_pool_step, dim3(1), dim3(64), 0, s, p, qin);

@@ -203,6 +203,7 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     const size_t C = (size_t)sv->cfg.pool_tasks;
     PoolDev& d = p->dev;
     d.scene = scene_dev(sc);
+    d.launch = sv->ctx->launch[CAV_TRACE_WAVEFRONT];
     d.halton = sv->halton;
     d.cap = (int64_t)C;
     auto& L = p->bufs;
@@ -580,6 +581,9 @@ int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) 
             (void)hipHostFree(sv->host_base);
             return fail(ctx, rc, "server rlpath context");
         }
+        // the caller's launch shapes (the autotuner sets them on its context)
+        rc_ctx->launch[CAV_TRACE_STANDALONE] = ctx->launch[CAV_TRACE_STANDALONE];
+        rc_ctx->launch[CAV_TRACE_WAVEFRONT] = ctx->launch[CAV_TRACE_WAVEFRONT];
         sv->rl_ctxs.push_back(rc_ctx);
     }
     sv->t0 = std::chrono::steady_clock::now();

@@ -42,11 +42,6 @@ __device__ __forceinline__ int stack_pop(int (*lds)[kBlock], const volatile int*
     if (sp < D) return *(const lds_int*)(&lds[sp][threadIdx.x]);
     return spill[sp - D];
 }
-#ifndef CAV_TRACE_LDS_DEPTH
-#define CAV_TRACE_LDS_DEPTH 32
-#endif
-constexpr int kLdsDepth = CAV_TRACE_LDS_DEPTH;
-constexpr int kChunk = 256;  // rays taken from the global counter at a time
 
 // Core/BoundingBox.h:55-74; returns the entry distance or -1.
 __device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz,
@@ -91,7 +86,7 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
 // coincident Morton codes). Rays are handed out per wave from chunks taken
 // with one atomic. Each ray's node order is the reference's, so results do not
 // depend on which lane or wave traced it.
-template <bool kClosest>
+template <bool kClosest, int kLdsDepth>
 __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
                                                   const float* __restrict__ rays, int64_t nrays_arg,
@@ -105,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   uint8_t* __restrict__ ovis,
                                                   uint32_t* __restrict__ overflow,
                                                   unsigned long long* __restrict__ next_ray,
-                                                  int max_steps) {
+                                                  int max_steps, int chunk) {
     __shared__ int lstack[kLdsDepth][kBlock];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -132,11 +127,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
             if (chunk_avail < cnt) {
                 unsigned long long nb = 0;
                 const int first = __ffsll((long long)need) - 1;
-                if (lane == first) nb = atomicAdd(next_ray, (unsigned long long)kChunk);
+                if (lane == first) nb = atomicAdd(next_ray, (unsigned long long)chunk);
                 nb = __shfl(nb, first, 64);
                 mine = rank < chunk_avail ? chunk_base + rank : (int64_t)nb + (rank - chunk_avail);
                 chunk_base = (int64_t)nb + (cnt - chunk_avail);
-                chunk_avail = kChunk - (cnt - chunk_avail);
+                chunk_avail = chunk - (cnt - chunk_avail);
             } else {
                 mine = chunk_base + rank;
                 chunk_base += cnt;
@@ -261,46 +256,47 @@ int persistent_blocks(const void* kernel, int max_per_cu) {
     return cus * per;
 }
 
-// Workgroups per CU for the standalone entry points. Their batches are
-// typically large and incoherent over trees far larger than L2: 3 of the 5
-// workgroups that fit per CU trace 2^24 random rays through 2^24 triangles
-// about 4% faster (fewer rays in flight thrash L2 less). The wavefront loops
-// keep every workgroup that fits: their scenes are smaller and the extra
-// waves hide latency (the Cornell-box render is 5% slower at 3).
-constexpr int kStandalonePerCu = 3;
+// The kernel of the launch's stack depth on a persistent grid of its
+// workgroups per CU, for `work` rays at most.
+template <bool C, typename... A>
+void launch_trace(const TraceLaunch& L, int64_t work, hipStream_t s, A... args) {
+    const bool deep = L.lds_depth >= 48;
+    const void* k = deep ? reinterpret_cast<const void*>(&k_trace<C, 48>)
+                         : reinterpret_cast<const void*>(&k_trace<C, 32>);
+    int64_t blocks = persistent_blocks(k, L.per_cu);
+    const int64_t need = (work + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    if (deep)
+        hipLaunchKernelGGL((k_trace<C, 48>), dim3((unsigned)blocks), dim3(kBlock), 0, s, args..., L.chunk);
+    else
+        hipLaunchKernelGGL((k_trace<C, 32>), dim3((unsigned)blocks), dim3(kBlock), 0, s, args..., L.chunk);
+}
 
 }  // namespace
 
 hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* counters, hipStream_t s) {
+                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
     if (e != hipSuccess) return e;
-    const void* k = reinterpret_cast<const void*>(&k_trace<true>);
-    int64_t blocks = persistent_blocks(k, kStandalonePerCu);
-    const int64_t need = (nrays + kBlock - 1) / kBlock;
-    if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, root, rays,
-                       nrays, nullptr, nullptr, nullptr, nullptr, prim, t, u, v, nullptr, counters,
-                       reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
+    launch_trace<true>(L, nrays, s, rec, root, rays, nrays, (const uint32_t*)nullptr, (const float*)nullptr,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
+                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
 
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
-                          int64_t nrays, uint8_t* visible, uint32_t* counters, hipStream_t s) {
+                          int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
+                          hipStream_t s) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
     if (e != hipSuccess) return e;
-    const void* k = reinterpret_cast<const void*>(&k_trace<false>);
-    int64_t blocks = persistent_blocks(k, kStandalonePerCu);
-    const int64_t need = (nrays + kBlock - 1) / kBlock;
-    if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, nullptr,
-                       rays, nrays, nullptr, limit, src, dst, nullptr, nullptr, nullptr, nullptr, visible,
-                       counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
-                       2 * ntris - 1);
+    launch_trace<false>(L, nrays, s, rec, (const float*)nullptr, rays, nrays, (const uint32_t*)nullptr, limit, src,
+                        dst, (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
+                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
     return hipGetLastError();
 }
 
@@ -309,15 +305,10 @@ hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
-                               hipStream_t s) {
+                               const TraceLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 2, 2);
-    const void* k = reinterpret_cast<const void*>(&k_trace<true>);
-    int64_t blocks = persistent_blocks(k, 0);
-    const int64_t need = (capacity + kBlock - 1) / kBlock;
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_trace<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, root, rays,
-                       capacity, nrays_dev, nullptr, nullptr, nullptr, prim, t, u, v, nullptr,
+    launch_trace<true>(L, capacity, s, rec, root, rays, capacity, nrays_dev, (const float*)nullptr,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
                        counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
     return hipGetLastError();
 }
@@ -325,17 +316,11 @@ hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
 hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
-                                uint32_t* counters, hipStream_t s) {
+                                uint32_t* counters, const TraceLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 4, 2);
-    const void* k = reinterpret_cast<const void*>(&k_trace<false>);
-    int64_t blocks = persistent_blocks(k, 0);
-    const int64_t need = (capacity + kBlock - 1) / kBlock;
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_trace<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, rec, nullptr,
-                       rays, capacity, nrays_dev, limit, src, dst, nullptr, nullptr, nullptr, nullptr,
-                       visible, counters + 1, reinterpret_cast<unsigned long long*>(counters + 4),
-                       2 * ntris - 1);
+    launch_trace<false>(L, capacity, s, rec, (const float*)nullptr, rays, capacity, nrays_dev, limit, src, dst,
+                        (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
+                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
     return hipGetLastError();
 }
 

@@ -1,0 +1,35 @@
+"""The traversal autotuner's search and cache, with a synthetic timer (no
+GPU): coordinate descent finds the fastest shape of a separable cost, every
+shape it tries is a valid one, and the cache round-trips."""
+import json
+
+from cavalry_amd import tune
+
+
+def test_descent_finds_separable_minimum(monkeypatch):
+    tried, cur = [], {}
+
+    def setter(ctx, which, shape):
+        tried.append(dict(shape))
+        cur.update(shape)
+
+    monkeypatch.setattr(tune, "set_launch", setter)
+    cost = lambda s: ((s["workgroups_per_cu"] - 3) ** 2 + abs(s["ray_chunk"] - 512) / 128
+                      + (0 if s["lds_stack"] == 48 else 0.5) + 10)
+    best, timings = tune._descend(None, tune.STANDALONE, lambda: cost(cur),
+                                  {"workgroups_per_cu": 5, "ray_chunk": 256, "lds_stack": 32}, reps=2)
+    assert best == {"workgroups_per_cu": 3, "ray_chunk": 512, "lds_stack": 48}
+    assert tried[-1] == best  # the winner is left set
+    for s in tried:
+        assert s["workgroups_per_cu"] in tune.PER_CU and s["ray_chunk"] in tune.CHUNKS and s["lds_stack"] in tune.LDS
+    assert min(timings.values()) == cost(best)
+
+
+def test_cache_round_trip(tmp_path):
+    p = str(tmp_path / "t.json")
+    tune._store(p, "k1", {"shape": {"workgroups_per_cu": 2, "ray_chunk": 128, "lds_stack": 48}, "ms": 1.5})
+    tune._store(p, "k2", {"shape": {"workgroups_per_cu": 4, "ray_chunk": 256, "lds_stack": 32}, "ms": 2.5})
+    data = json.load(open(p))
+    assert set(data["entries"]) == {"k1", "k2"}
+    assert tune._load(p)["entries"]["k1"]["shape"]["lds_stack"] == 48
+    assert tune._load(str(tmp_path / "missing.json")) == {}
