@@ -24,6 +24,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Hardware queues this process may use (read when HIP initialises, so before
+# torch or the library touches the GPU). The server runs one stream per
+# resident scene's pool plus one per rlpath worker (8 here); with HIP's
+# default of 4 queues, streams share queues and their launches serialise
+# (served/plain 1.33x at 4 queues, 1.44x at 8, 1.51x at 16 on the serving
+# trace). The single-stream paths do not depend on it.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 # Algorithmic bytes of one launch of each single-launch LBVH build kernel

@@ -58,6 +58,13 @@ __device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > 
 __device__ inline float acos_f(float x) { return (float)acos((double)x); }
 __device__ inline float cos_f(float x) { return (float)cos((double)x); }
 __device__ inline float sin_f(float x) { return (float)sin((double)x); }
+// both at once (one argument reduction); each rounded once to float
+__device__ inline void sincos_f(float x, float& s, float& c) {
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    s = (float)sd;
+    c = (float)cd;
+}
 
 __device__ inline float luminance(v3 c) {
     return (c.x * 0.212671f + c.y * 0.715160f) + c.z * 0.072169f;
@@ -427,8 +434,10 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         for (int p = 0; p < kNX; ++p) {
             const float v = ((float)p + py) * (1.f / (float)kNX);
             const float phi = (float)(2 * kPi * (double)v);
-            cs[p * kBlock] = cos_f(phi);
-            cs[(kNX + p) * kBlock] = sin_f(phi);
+            float sp, cp;
+            sincos_f(phi, sp, cp);
+            cs[p * kBlock] = cp;
+            cs[(kNX + p) * kBlock] = sp;
         }
         const bool matte = bsdf.type == CAV_MAT_MATTE;
         const BinEval be{bsdf, t0, t1, nn, ol, Qe, cs, px, matte,
