@@ -29,8 +29,9 @@ sys.path.insert(0, ROOT)
 # resident scene's pool plus one per rlpath worker (8 here); with HIP's
 # default of 4 queues, streams share queues and their launches serialise
 # (served/plain 1.33x at 4 queues, 1.44x at 8, 1.51x at 16 on the serving
-# trace). The single-stream paths do not depend on it.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# trace). The single-stream paths do not depend on it. Set, not defaulted: an
+# environment that already holds HIP's default 4 would otherwise keep it.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
@@ -226,6 +227,80 @@ def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
         "path_films_identical": bool(same),
         "rlpath_max_rel_mse_vs_plain_ldr": worst_rl,
         "server_steps": st["steps"],
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+    }
+
+
+def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: int) -> dict:
+    """rlpath against path at equal render time on an interior scene (max
+    depth 16, 3 treelet rounds). The reference image is a path render of
+    ref_spp samples per pixel whose film sums are binary64 (FILM_F64). Each
+    time budget is the device time of path at 64, 256 and 1024 spp; rlpath
+    (64^3 Q entries x 128 bins as the workload asks, and the reference's 32^3;
+    the reference's learning rate) renders at the spp that a linear fit of
+    its own time against spp (measured at 64 and 1024 spp) puts within that
+    budget. Error = rel-MSE against the reference
+    image. Each render pays a fixed device time (about 90 ms on 2M
+    triangles at max depth 16: 32 traversal launches, each as long as its
+    slowest ray), so budgets below ~64 spp measure that, not the
+    integrators."""
+    import numpy as np
+    from cavalry_amd import Context, render as R, scenes
+    from cavalry_amd.camera import for_scene
+
+    sc = scenes.interior(n_tris, seed=seed, width=width, height=height, max_depth=16)
+    ctx = Context(0)
+    scene = R.Scene(ctx, sc, bvh_rounds=3)
+    cam = for_scene(sc)
+    log(f"[bench] equal time: reference image, {ref_spp} spp, binary64 film")
+    ref_film, ref_st = R.render(ctx, scene, ref_spp, integrator="path", camera=cam, film_f64=True,
+                               spp_per_round=min(ref_spp, 1024))
+    ref = ref_film.rgb.astype(np.float64) / np.maximum(ref_film.weight.astype(np.float64), 1e-30)[..., None]
+
+    def run(integrator, spp, q_grid=64):
+        # each integrator with its faster hit order: rlpath's 128-bin BSDF
+        # evaluation diverges less over material-sorted hits (-10%), path's
+        # shading is too cheap to pay for the sort (+6%)
+        film, st = R.render(ctx, scene, spp, integrator=integrator, camera=cam, q_grid=q_grid,
+                            sort_by_material=integrator == "rlpath")
+        return {"spp": int(spp), "ms": round(float(st["ms_total"]), 3),
+                "rel_mse": R.rel_mse(film.image, ref),
+                "rel_mse_ldr": R.rel_mse(np.clip(film.image, 0, 1), np.clip(ref, 0, 1)),
+                "mean": float(film.image.mean()),
+                "mrays_s": round(float(st["mrays_per_s"]), 1)}
+
+    run("path", 4)
+    run("rlpath", 4)  # warm-up: allocations
+    fits = {}
+    for q in (64, 32):  # the workload's 64^3 Q entries, and the reference's 32^3
+        f = {s: run("rlpath", s, q) for s in (64, 1024)}
+        slope = (f[1024]["ms"] - f[64]["ms"]) / 960.0
+        fits[q] = (slope, f[64]["ms"] - 64 * slope)
+    budgets = []
+    for spp in (64, 256, 1024):
+        p = run("path", spp)
+        b = {"budget_ms": p["ms"], "path": p}
+        for q, (slope, icpt) in fits.items():
+            rl_spp = max(1, int((p["ms"] - icpt) / slope)) if slope > 0 else spp
+            rl = run("rlpath", rl_spp, q)
+            b[f"rlpath_q{q}"] = rl
+            b[f"rlpath_q{q}_over_path_error"] = round(rl["rel_mse"] / p["rel_mse"], 4)
+            log(f"[bench] equal time {p['ms']:.1f} ms: path {spp} spp rel-MSE {p['rel_mse']:.3e}, "
+                f"rlpath {q}^3 {rl_spp} spp ({rl['ms']:.1f} ms) rel-MSE {rl['rel_mse']:.3e}")
+        budgets.append(b)
+    scene.close()
+    ctx.close()
+    return {
+        "workload": (f"interior scene ({len(sc.tris)} tris, 4 1024^2 textures, 64 quad lights), "
+                     f"{width}x{height}, max depth 16, 3 treelet rounds; reference: path at {ref_spp} "
+                     "spp with binary64 film sums; rlpath with 64^3 and 32^3 Q entries, material-sorted hits"),
+        "reference": {"spp": ref_spp, "ms": round(float(ref_st["ms_total"]), 1),
+                      "mrays_s": round(float(ref_st["mrays_per_s"]), 1),
+                      # rel-MSE falls as 1 / spp: the reference's own error, from path at 1024 spp
+                      "own_rel_mse_est": budgets[-1]["path"]["rel_mse"] * 1024 / ref_spp},
+        "reference_mean": float(ref.mean()),
+        "rlpath_time_fit_ms": {f"q{q}": {"per_spp": round(a, 4), "fixed": round(c, 3)} for q, (a, c) in fits.items()},
+        "budgets": budgets,
     }
 
 
@@ -246,6 +321,10 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-equal-time", action="store_true")
+    ap.add_argument("--et-tris", type=int, default=2_000_000)
+    ap.add_argument("--et-res", default="320x180", help="WxH of the equal-time comparison")
+    ap.add_argument("--et-ref-spp", type=int, default=16384)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default traversal launch shapes instead of autotuning them")
     ap.add_argument("--tune-rays", type=int, default=1 << 22,
@@ -425,6 +504,10 @@ def main() -> int:
     if not args.no_render:
         log("[bench] path tracing (Cornell box 512x512x16)")
         out["path_tracing"] = render_bench(args.renders, not args.no_cpu, not args.no_tune)
+    if not args.no_equal_time:
+        w, h = (int(x) for x in args.et_res.lower().split("x"))
+        log(f"[bench] rlpath vs path at equal time ({w}x{h}, {args.et_tris} tris)")
+        out["equal_time"] = equal_time_bench(args.seed, args.et_tris, w, h, args.et_ref_spp)
     if not args.no_serve:
         log("[bench] serving")
         out["serving"] = serving_bench(args.serve_requests, args.serve_rate, args.seed)

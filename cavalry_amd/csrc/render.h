@@ -102,7 +102,8 @@ struct RenderArgs {
     uint32_t* trace_counters;  // traversal counters (8 words)
     float* film_rgb;           // [w*h][3]
     float* film_w;             // [w*h]
-    float4* film_partial;      // [9][w*h] per-source-pixel partial sums
+    float4* film_partial;      // [9][w*h] per-source-pixel partial sums (room for double ones)
+    bool film_f64;             // film sums in binary64 (CAV_RENDER_FILM_F64)
     uint32_t* sort_perm;       // [cap] hit queue order by material (null: queue order)
     uint32_t* sort_hist;       // sort_hist_words(cap)
 };

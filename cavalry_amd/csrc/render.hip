@@ -166,17 +166,26 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int qin) {
 // (in order) of those whose filter support covers pixel p + (dx, dy).
 // k_film_final: each target pixel adds the partials of its 3x3 neighbours in
 // row-major neighbour order. Samples with a non-finite value are skipped.
+// Partial and final sums in T: float (the reference's FP32 film) or double
+// (CAV_RENDER_FILM_F64: the same sums in the same order in binary64, rounded
+// to binary32 once when written out).
+template <typename T>
+struct alignas(4 * sizeof(T)) Sum4 {
+    T x, y, z, w;
+};
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_film_partial(int width, int height, int spp,
                                                          const float* __restrict__ res,
                                                          const float* __restrict__ sxy,
-                                                         float4* __restrict__ partial) {
+                                                         Sum4<T>* __restrict__ partial) {
     const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t npix = (int64_t)width * height;
     if (pix >= npix) return;
     const int sx0 = (int)(pix % width), sy0 = (int)(pix / width);
-    float4 acc[9];
+    Sum4<T> acc[9];
 #pragma unroll
-    for (int d = 0; d < 9; ++d) acc[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int d = 0; d < 9; ++d) acc[d] = Sum4<T>{0, 0, 0, 0};
     for (int k = 0; k < spp; ++k) {
         const int64_t si = pix * spp + k;
         const float c0 = res[si * 3], c1 = res[si * 3 + 1], c2 = res[si * 3 + 2];
@@ -189,32 +198,42 @@ __global__ __launch_bounds__(kBlock) void k_film_partial(int width, int height, 
             const int x = sx0 + d % 3 - 1, y = sy0 + d / 3 - 1;
             if (x < x_lo || x > x_hi || y < y_lo || y > y_hi) continue;
             if (fabsf((float)x - fx) > 1.f || fabsf((float)y - fy) > 1.f) continue;
-            acc[d].x += c0; acc[d].y += c1; acc[d].z += c2; acc[d].w += 1.f;
+            acc[d].x += (T)c0; acc[d].y += (T)c1; acc[d].z += (T)c2; acc[d].w += (T)1;
         }
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) partial[d * npix + pix] = acc[d];
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_film_final(int width, int height,
-                                                       const float4* __restrict__ partial,
+                                                       const Sum4<T>* __restrict__ partial,
                                                        float* __restrict__ rgb, float* __restrict__ wt) {
     const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t npix = (int64_t)width * height;
     if (pix >= npix) return;
     const int x = (int)(pix % width), y = (int)(pix / width);
-    float r = 0, g = 0, b = 0, w = 0;
+    T r = 0, g = 0, b = 0, w = 0;
     for (int py = y - 1; py <= y + 1; ++py)
         for (int px = x - 1; px <= x + 1; ++px) {
             if (px < 0 || py < 0 || px >= width || py >= height) continue;
             const int d = (y - py + 1) * 3 + (x - px + 1);  // offset from the source to this pixel
-            const float4 v = partial[d * npix + (int64_t)py * width + px];
+            const Sum4<T> v = partial[d * npix + (int64_t)py * width + px];
             r += v.x; g += v.y; b += v.z; w += v.w;
         }
-    rgb[pix * 3] = r;
-    rgb[pix * 3 + 1] = g;
-    rgb[pix * 3 + 2] = b;
-    wt[pix] = w;
+    rgb[pix * 3] = (float)r;
+    rgb[pix * 3 + 1] = (float)g;
+    rgb[pix * 3 + 2] = (float)b;
+    wt[pix] = (float)w;
+}
+
+template <typename T>
+void film_sums(int width, int height, int spp, const float* res, const float* sxy, Sum4<T>* partial, float* rgb,
+               float* wt, hipStream_t s) {
+    const int64_t npix = (int64_t)width * height;
+    const dim3 grid((unsigned)((npix + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_film_partial<T>, grid, dim3(kBlock), 0, s, width, height, spp, res, sxy, partial);
+    hipLaunchKernelGGL(k_film_final<T>, grid, dim3(kBlock), 0, s, width, height, partial, rgb, wt);
 }
 
 }  // namespace
@@ -227,14 +246,15 @@ void render_raygen(const RenderArgs& a, hipStream_t s) {
 
 void film_launch(int width, int height, int spp, const float* res, const float* sxy, float4* partial, float* rgb,
                  float* wt, hipStream_t s) {
-    const int64_t npix = (int64_t)width * height;
-    const dim3 grid((unsigned)((npix + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_film_partial, grid, dim3(kBlock), 0, s, width, height, spp, res, sxy, partial);
-    hipLaunchKernelGGL(k_film_final, grid, dim3(kBlock), 0, s, width, height, partial, rgb, wt);
+    film_sums(width, height, spp, res, sxy, reinterpret_cast<Sum4<float>*>(partial), rgb, wt, s);
 }
 
 void render_film(const RenderArgs& a, hipStream_t s) {
-    film_launch(a.width, a.height, a.spp, a.res, a.sxy, a.film_partial, a.film_rgb, a.film_w, s);
+    if (a.film_f64)  // film_partial holds 9 x w*h double sums (twice the float ones)
+        film_sums(a.width, a.height, a.spp, a.res, a.sxy, reinterpret_cast<Sum4<double>*>(a.film_partial),
+                  a.film_rgb, a.film_w, s);
+    else
+        film_launch(a.width, a.height, a.spp, a.res, a.sxy, a.film_partial, a.film_rgb, a.film_w, s);
 }
 
 hipError_t render_direct(const RenderArgs& a, hipStream_t s) {

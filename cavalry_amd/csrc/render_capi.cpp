@@ -264,7 +264,7 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
     if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 64, "trace counters");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
-    if (rc == CAV_OK) rc = grab(ctx, rs, a.film_partial, 9 * 16 * P, "film partial sums");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.film_partial, 9 * 32 * P, "film partial sums");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.sort_perm, 4 * S, "material order");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.sort_hist, 4 * (size_t)sort_hist_words(nq), "material histogram");
     if (rc != CAV_OK) { free_bufs(rs); return rc; }
@@ -578,6 +578,7 @@ int cav_render(cav_ctx* ctx, cav_scene* sc, const cav_camera* cam, const cav_ren
     a.max_depth = prm->max_depth;
     a.nsamp = nsamp;
     if (!(prm->flags & CAV_RENDER_SORT_BY_MATERIAL)) a.sort_perm = nullptr;
+    a.film_f64 = (prm->flags & CAV_RENDER_FILM_F64) != 0;
     hipStream_t s = ctx->stream;
     RCHK(ctx, hipMemsetAsync(a.rays, 0, 16, s));
     RCHK(ctx, hipMemsetAsync(a.trace_counters, 0, 32, s));

@@ -316,8 +316,12 @@ struct BinEval {
 // For every lane with `on`: fsum[k] += v and cnt[k] += 1, as atomics. Lanes of
 // the wave that share k are summed first, so one atomic pair per distinct k
 // reaches L2 (neighbouring paths share Q entries, cells and lights, and
-// same-address atomics serialise there). Float sums are order-dependent
-// either way, as in the reference. Call with the whole wave converged.
+// same-address atomics serialise there). Each group costs a dependent chain
+// of cross-lane reductions, so once the leader's k is a singleton (a wave of
+// incoherent paths) the remaining lanes add one by one instead (rlpath at
+// 320x180x256 spp on 2M triangles: 265 -> 256 ms). Float sums are
+// order-dependent either way, as in the reference. Call with the whole wave
+// converged.
 __device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool on) {
     const int lane = threadIdx.x & 63;
     uint64_t todo = __ballot(on);
@@ -326,6 +330,7 @@ __device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool 
         const int64_t lk = __shfl(k, lead, 64);
         const bool mine = ((todo >> lane) & 1ull) && k == lk;
         const uint64_t grp = __ballot(mine);
+        if (grp == (1ull << lead)) break;
         float sv = mine ? v : 0.f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o, 64);
@@ -334,6 +339,10 @@ __device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool 
             atomicAdd(cnt + lk, (int)__popcll(grp));
         }
         todo &= ~grp;
+    }
+    if ((todo >> lane) & 1ull) {
+        atomicAdd(fsum + k, v);
+        atomicAdd(cnt + k, 1);
     }
 }
 

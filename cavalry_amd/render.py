@@ -20,6 +20,7 @@ from .camera import for_scene
 INTEGRATORS = {"path": 0, "rlpath": 1, "directlighting": 2}
 SORT_BY_MATERIAL = 1  # CAV_RENDER_SORT_BY_MATERIAL
 DETERMINISTIC = 2     # CAV_RENDER_DETERMINISTIC
+FILM_F64 = 4          # CAV_RENDER_FILM_F64
 
 
 def default_bvh_rounds(spp: int) -> int:
@@ -145,14 +146,16 @@ class Scene:
 
 def render(ctx: Context, scene: Scene, spp: int, integrator: str = "path", camera=None,
            max_depth: int | None = None, q_grid: int = 32, alpha_scale: float = 1.0,
-           spp_per_round: int = 0, sort_by_material: bool = False, deterministic: bool = False):
+           spp_per_round: int = 0, sort_by_material: bool = False, deterministic: bool = False,
+           film_f64: bool = False):
     """Renders the scene; returns (Film, stats dict). q_grid and alpha_scale
     apply to rlpath (Q entries per axis; scale of the n/(T+n) learning rate);
     spp_per_round fixes the samples per pixel of each round (0: as many as
     device memory holds); sort_by_material shades each bounce's hits in
     material order (same film, different speed); deterministic makes rlpath
     sum its table updates in sample order instead of with float atomics (the
-    same film on every run, and the oracle's)."""
+    same film on every run, and the oracle's); film_f64 adds the film sums in
+    binary64 (high-spp reference images)."""
     a = scene.arrays
     if a is None:  # a scene parsed from a file: its setup gives the defaults
         st = scene.setup
@@ -164,7 +167,8 @@ def render(ctx: Context, scene: Scene, spp: int, integrator: str = "path", camer
         raise ValueError(f"unknown integrator {integrator!r}")
     prm = _lib.RenderParams(INTEGRATORS[integrator], int(spp),
                             int(max_depth if max_depth is not None else a.max_depth),
-                            (SORT_BY_MATERIAL if sort_by_material else 0) | (DETERMINISTIC if deterministic else 0),
+                            (SORT_BY_MATERIAL if sort_by_material else 0) | (DETERMINISTIC if deterministic else 0)
+                            | (FILM_F64 if film_f64 else 0),
                             int(q_grid), float(alpha_scale), int(spp_per_round))
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
     w = np.zeros((cam.height, cam.width), np.float32)

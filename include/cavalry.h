@@ -132,8 +132,11 @@ enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1, CAV_INTEGRATOR_DIRECT
  * updates (the Q proposals and light statistics that the reference adds with
  * float atomics, ReinforcementLearningPathTracing.cu:453-456,576-585) sum in
  * sample order, so the film is the same on every run; slower (one sort of
- * the updates per bounce). Other integrators ignore it. */
-enum { CAV_RENDER_SORT_BY_MATERIAL = 1, CAV_RENDER_DETERMINISTIC = 2 };
+ * the updates per bounce). Other integrators ignore it. FILM_F64 (cav_render
+ * only) adds each pixel's film sums in binary64 instead of the reference's
+ * binary32 (Core/Film.h), in the same order, and rounds them to binary32
+ * once: for high-spp reference images. */
+enum { CAV_RENDER_SORT_BY_MATERIAL = 1, CAV_RENDER_DETERMINISTIC = 2, CAV_RENDER_FILM_F64 = 4 };
 
 /* One mesh: triangles [first_tri, first_tri + num_tris) of the scene's soup.
  * material follows MaterialType (Core/Material.h:10-12): 0 matte (kd),

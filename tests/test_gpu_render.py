@@ -73,6 +73,27 @@ def test_path_rounds_equal_single_round_oracle(ctx):
     scene.close()
 
 
+def test_film_f64_sums(ctx):
+    """FILM_F64 adds the same samples in the same order in binary64: the
+    weights (small integers) are identical, the sums agree with the binary32
+    film to the rounding of at most 9 x spp binary32 additions, and the float
+    film after a later plain render is the oracle's again (the flag does not
+    stick to the context's buffers)."""
+    print("[render] cornell matte spp=64 film_f64", flush=True)
+    sc = scenes.cornell_box(seed=6, width=40, height=30)
+    cam = for_scene(sc)
+    scene = R.Scene(ctx, sc)
+    f32, _ = R.render(ctx, scene, 64, camera=cam)
+    f64, _ = R.render(ctx, scene, 64, camera=cam, film_f64=True)
+    np.testing.assert_array_equal(f64.weight, f32.weight)
+    np.testing.assert_allclose(f64.rgb, f32.rgb, rtol=9 * 64 * 2 ** -24, atol=1e-6)
+    rgb, w, _ = oracle.render_path(sc, 4, cam)
+    again, _ = R.render(ctx, scene, 4, camera=cam)
+    np.testing.assert_array_equal(again.weight, w)
+    assert rel_mse(again.image, rgb / np.maximum(w, 1e-30)[..., None]) <= REL_MSE_TOL
+    scene.close()
+
+
 @pytest.mark.parametrize("materials", ["matte", "mixed"])
 def test_directlighting_matches_oracle(ctx, materials):
     print(f"[render] directlighting {materials}", flush=True)
