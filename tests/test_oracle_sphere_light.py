@@ -18,6 +18,11 @@ rho L r^2 cos(beta) / D^2:
   tessellated sphere gives weights that do not sum to 1: +10% here). The
   BSDF rays meet the 2048-triangle tessellation, not the sphere, which is
   within the tolerance;
+* rlpath at max depth 2 combines the cone sample with the direction drawn
+  from the learnt 128-bin Q distribution (power heuristic against the
+  bin's pdf, ReinforcementLearningPathTracing.cu:416,644); the
+  weights must again sum to 1 whatever the table has learnt, so the same
+  closed form holds;
 * a small far sphere (sin^2 of the cone's half-angle below sin^2 1.5 deg)
   goes through the reference's small-angle form and keeps the same answer.
 """
@@ -67,13 +72,15 @@ def _floor_pixels(sc, r, margin=1.0):
 
 
 @pytest.mark.parametrize("r", [0.7, 0.02])
-@pytest.mark.parametrize("integrator", ["direct", "path"])
+@pytest.mark.parametrize("integrator", ["direct", "path", "rlpath"])
 def test_sphere_light_floor_closed_form(integrator, r):
     rho, L = 0.5, 2.0 * (0.7 / r) ** 2
     sc = scenes.sphere_light_plane(width=64, height=48, albedo=rho, radiance=L, radius=r, max_depth=2)
     cam = cam_of(sc)
     if integrator == "direct":
         rgb, w, _ = oracle.render_direct(sc, 64, cam)
+    elif integrator == "rlpath":
+        rgb, w, _ = oracle.render_rlpath(sc, 64, cam, max_depth=2)
     else:
         rgb, w, _ = oracle.render_path(sc, 64, cam)
     img = (rgb / w[..., None])[..., 0]
