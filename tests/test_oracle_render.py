@@ -6,6 +6,7 @@
 * furnace: inside a closed box whose every face is a diffuse area light of
   radiance Le and albedo rho, the radiance is Le / (1 - rho) everywhere; the
   estimator (NEE + MIS + Russian roulette) must agree to within its noise;
+* the same furnace holds for rlpath (learnt light choice and directions);
 * film: weights count exactly the samples whose box-filter support covers
   each pixel (4 per sample away from the edges).
 """
@@ -245,3 +246,17 @@ def test_crop_equals_the_full_film():
         # film positions lie within one pixel of their pixel's corner
         xs = np.arange(x0, x1)[None, :, None]
         assert ((smp[..., 3] >= xs) & (smp[..., 3] < xs + 1)).all()
+
+
+def test_rlpath_furnace_is_le_over_one_minus_rho():
+    """rlpath learns its light choice and its 128-bin direction distribution
+    as it goes (Integrators/ReinforcementLearningPathTracing.cu); whatever it
+    has learnt, the MIS weights of the light sample and of the Q-sampled hit
+    sum to 1, so the furnace's Le / (1 - rho) still holds over many bounces."""
+    for rho in (0.3, 0.6):
+        sc = furnace_scene(rho=rho)
+        cam = look_at(sc.eye, sc.center, sc.up, sc.fov, sc.width, sc.height)
+        rgb, w, _ = oracle.render_rlpath(sc, 32, cam)
+        expect = 1.0 / (1.0 - rho)
+        mean = float((rgb / w[..., None]).mean())
+        assert abs(mean - expect) < 0.01 * expect, (rho, mean, expect)
