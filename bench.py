@@ -259,8 +259,8 @@ def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: i
 
     def run(integrator, spp, q_grid=64):
         # each integrator with its faster hit order: rlpath's 128-bin BSDF
-        # evaluation diverges less over material-sorted hits (-10%), path's
-        # shading is too cheap to pay for the sort (+6%)
+        # evaluation diverges less over material-sorted hits (-12%), path's
+        # shading is too cheap to pay for the sort (+5%)
         film, st = R.render(ctx, scene, spp, integrator=integrator, camera=cam, q_grid=q_grid,
                             sort_by_material=integrator == "rlpath")
         return {"spp": int(spp), "ms": round(float(st["ms_total"]), 3),

@@ -15,6 +15,7 @@ namespace {
 constexpr int kSortBlock = 1024;
 constexpr int kKeys = 8;  // material ids 0..5 (MaterialType), 7 = miss
 constexpr int kWaves = kSortBlock / 64;
+constexpr int kScanTiles = kSortBlock;  // tiles of the histogram scan (one block scans their sums)
 
 __device__ inline int hit_key(const SceneDev& sc, const int32_t* hit_prim, int64_t i) {
     const int prim = hit_prim[i];
@@ -46,29 +47,67 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(SceneDev sc, const int
     if (threadIdx.x < kKeys) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// exclusive scan of hist in key-major order, one block
-__global__ __launch_bounds__(kSortBlock) void k_sort_scan(uint32_t* __restrict__ hist, int total) {
-    __shared__ uint32_t part[kSortBlock];
-    const int chunk = (total + kSortBlock - 1) / kSortBlock;
-    const int lo = threadIdx.x * chunk, hi = min(lo + chunk, total);
-    uint32_t s = 0;
-    for (int k = lo; k < hi; ++k) s += hist[k];
-    part[threadIdx.x] = s;
+// Exclusive scan of v over the block (kSortBlock threads); `total` gets the
+// block's sum. Every thread of the block must call it.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t wsum[kWaves];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int t = 0; t < kSortBlock; ++t) {
-            const uint32_t v = part[t];
-            part[t] = run;
-            run += v;
+    if (wv == 0) {
+        uint32_t w = lane < kWaves ? wsum[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < kWaves; o <<= 1) {
+            const uint32_t y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
         }
+        if (lane < kWaves) wsum[lane] = w;  // inclusive over the waves
     }
     __syncthreads();
-    uint32_t run = part[threadIdx.x];
-    for (int k = lo; k < hi; ++k) {
-        const uint32_t v = hist[k];
-        hist[k] = run;
-        run += v;
+    const uint32_t before = wv == 0 ? 0u : wsum[wv - 1];
+    total = wsum[kWaves - 1];
+    __syncthreads();  // wsum is reused by the next call
+    return before + x - v;
+}
+
+// The key-major exclusive scan of hist in three launches (reduce, scan of
+// the tile sums, rescan with the tile's offset) over at most kScanTiles tiles
+// of `tile` consecutive words: one block walking the whole histogram waited
+// on its dependent loads for ~300 us per bounce at 2^24 queue slots.
+__global__ __launch_bounds__(kSortBlock) void k_scan_reduce(const uint32_t* __restrict__ hist, int total, int tile,
+                                                            uint32_t* __restrict__ tsum) {
+    const int lo = blockIdx.x * tile, hi = min(lo + tile, total);
+    uint32_t v = 0;
+    for (int k = lo + (int)threadIdx.x; k < hi; k += kSortBlock) v += hist[k];
+    uint32_t t;
+    (void)block_excl_scan(v, t);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_scan_tiles(uint32_t* __restrict__ tsum, int ntiles) {
+    const uint32_t v = (int)threadIdx.x < ntiles ? tsum[threadIdx.x] : 0u;
+    uint32_t t;
+    const uint32_t e = block_excl_scan(v, t);
+    if ((int)threadIdx.x < ntiles) tsum[threadIdx.x] = e;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_scan_apply(uint32_t* __restrict__ hist, int total, int tile,
+                                                           const uint32_t* __restrict__ tsum) {
+    const int lo = blockIdx.x * tile, hi = min(lo + tile, total);
+    uint32_t carry = tsum[blockIdx.x];
+    for (int base = lo; base < hi; base += kSortBlock) {
+        const int k = base + (int)threadIdx.x;
+        const uint32_t v = k < hi ? hist[k] : 0u;
+        uint32_t t;
+        const uint32_t e = block_excl_scan(v, t);
+        if (k < hi) hist[k] = carry + e;
+        carry += t;
     }
 }
 
@@ -105,12 +144,21 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(SceneDev sc, const 
 hipError_t sort_hits_by_material(const SceneDev& sc, const int32_t* hit_prim, const uint32_t* count, int64_t cap,
                                  uint32_t* hist, uint32_t* perm, hipStream_t s) {
     const int nblk = (int)((cap + kSortBlock - 1) / kSortBlock);
+    if (nblk == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortBlock), 0, s, sc, hit_prim, count, hist, nblk);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortBlock), 0, s, hist, nblk * kKeys);
+    // histogram words, then kScanTiles words of tile sums (sort_hist_words)
+    const int total = nblk * kKeys;
+    int tile = (total + kScanTiles - 1) / kScanTiles;
+    tile = (tile + kSortBlock - 1) / kSortBlock * kSortBlock;
+    const int ntiles = (total + tile - 1) / tile;
+    uint32_t* tsum = hist + total;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kSortBlock), 0, s, hist, total, tile, tsum);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kSortBlock), 0, s, tsum, ntiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kSortBlock), 0, s, hist, total, tile, tsum);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortBlock), 0, s, sc, hit_prim, count, hist, nblk, perm);
     return hipGetLastError();
 }
 
-int64_t sort_hist_words(int64_t cap) { return ((cap + kSortBlock - 1) / kSortBlock) * kKeys; }
+int64_t sort_hist_words(int64_t cap) { return ((cap + kSortBlock - 1) / kSortBlock) * kKeys + kScanTiles; }
 
 }  // namespace cav
