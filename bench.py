@@ -420,6 +420,7 @@ def main() -> int:
 
     torch.cuda.synchronize()
     ctx.synchronize()
+    bvh.trace_counters(reset=True)  # count stack overflows of the timed traces only
     recs = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -428,6 +429,12 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     log(f"[bench] {args.steps} timed steps in {wall:.3f} s")
+    # every timed ray finished inside the reference's 64-entry stack, or the
+    # results are partial and the run fails
+    trav = bvh.trace_counters()
+    trav["max_leaf_depth"] = bvh.max_depth()
+    trav["stack_entries"] = 64
+    log(f"[bench] traversal counters {trav}")
 
     ms_step = wall * 1000.0 / args.steps
     stages = {}
@@ -500,6 +507,7 @@ def main() -> int:
         "hit_fraction": round(float((hit_prim >= 0).float().mean().item()), 4),
         "visible_fraction": round(float(vis.float().mean().item()), 4),
         "roofline": roofline,
+        "traversal": trav,
         "trace_launch": launch,
         "cpu_baseline": None,
     }
@@ -520,6 +528,9 @@ def main() -> int:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)
     print(json.dumps(out), flush=True)
+    if trav["overflow_closest"] or trav["overflow_occluded"]:
+        print("bench: traversal stack overflow in the timed steps (results partial)", file=sys.stderr)
+        return 3
     return 0
 
 

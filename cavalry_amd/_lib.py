@@ -86,7 +86,7 @@ class ServeResult(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("pool_closest_rays", ctypes.c_int64), ("pool_shadow_rays", ctypes.c_int64),
                 ("rl_closest_rays", ctypes.c_int64), ("rl_shadow_rays", ctypes.c_int64),
-                ("steps", ctypes.c_int64), ("pools", ctypes.c_int32)]
+                ("steps", ctypes.c_int64), ("pools", ctypes.c_int32), ("graph_pools", ctypes.c_int32)]
 
 
 _lib = None
@@ -115,6 +115,8 @@ _SIGS = {
     "cav_trace_occluded_device": ([P, P, P, P, P, P, I64, P], I),
     "cav_trace_closest_async": ([P, P, P, I64, P, P], I),
     "cav_trace_occluded_async": ([P, P, P, P, I64, P], I),
+    "cav_trace_counters": ([P, P, P, ctypes.c_int32], I),
+    "cav_bvh_max_depth": ([P, P, ctypes.POINTER(ctypes.c_int32)], I),
     "cav_camera_look_at": ([P, P, P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
     "cav_scene_create": ([P, P, I, P, I, P, I, I, ctypes.POINTER(P)], I),
     "cav_scene_create_desc": ([P, ctypes.POINTER(SceneDesc), ctypes.POINTER(P)], I),

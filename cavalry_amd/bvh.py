@@ -66,7 +66,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
-            for child in list(self._children):
+            # servers first (they hold scenes), then scenes (they own BVHs), then the rest
+            for child in sorted(self._children, key=lambda c: getattr(c, "close_rank", 2)):
                 child.close()
             lib().cav_ctx_destroy(self.handle)
             self.handle = None
@@ -166,6 +167,19 @@ class BVH:
             self.ctx.handle, self.handle, _ptr(rays), _ptr(limit), _ptr(src), _ptr(dst), r,
             _ptr(vis)))
         return vis
+
+    def trace_counters(self, reset: bool = False) -> dict:
+        """Stack overflows and deepest stacks of this BVH's standalone traces."""
+        out = np.zeros(4, np.uint32)
+        check(self.ctx.handle, lib().cav_trace_counters(self.ctx.handle, self.handle, _ptr(out), 1 if reset else 0))
+        return {"overflow_closest": int(out[0]), "overflow_occluded": int(out[1]),
+                "max_stack_closest": int(out[2]), "max_stack_occluded": int(out[3])}
+
+    def max_depth(self) -> int:
+        """Deepest leaf, in edges from the root."""
+        d = ctypes.c_int32()
+        check(self.ctx.handle, lib().cav_bvh_max_depth(self.ctx.handle, self.handle, ctypes.byref(d)))
+        return int(d.value)
 
     def close(self) -> None:
         if self.handle:

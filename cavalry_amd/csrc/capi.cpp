@@ -372,6 +372,37 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
     return CAV_OK;
 }
 
+int cav_trace_counters(cav_ctx* ctx, const cav_bvh* h, uint32_t* out4, int32_t reset) {
+    if (!ctx || !h || !out4) return CAV_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    uint32_t w[8];
+    HIPCHK(ctx, hipMemcpyAsync(w, h->b.counters, sizeof w, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    out4[0] = w[0];
+    out4[1] = w[1];
+    out4[2] = w[6];
+    out4[3] = w[7];
+    if (reset) {
+        HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 8, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(h->b.counters + 6, 0, 8, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return CAV_OK;
+}
+
+int cav_bvh_max_depth(cav_ctx* ctx, const cav_bvh* h, int32_t* depth) {
+    if (!ctx || !h || !depth) return CAV_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // counters[8] is free scratch (the traversal uses words 0..7)
+    uint32_t* word = h->b.counters + 8;
+    HIPCHK(ctx, levels_max_depth(h->b, h->n, word, ctx->stream));
+    uint32_t d = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&d, word, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *depth = (int32_t)d;
+    return CAV_OK;
+}
+
 int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
                              int32_t* prim, float* t, float* u, float* v) {
     if (!ctx || !h) return CAV_ERR_INVALID;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "dev_common.h"
+#include "launch.h"
 #include "radix_sort.h"
 
 namespace cav {
@@ -77,6 +78,8 @@ hipError_t levels_from_leaves(LbvhBuffers& b, const float* tris, int n, hipStrea
 hipError_t levels_from_list(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
 hipError_t levels_bucket(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
 hipError_t levels_refit(LbvhBuffers& b, hipStream_t s, Marker* mk);
+// deepest leaf (edges from the root) of the current tree into *out (device word)
+hipError_t levels_max_depth(const LbvhBuffers& b, int n, uint32_t* out, hipStream_t s);
 
 // treelet.hip: one restructuring round; `first` = leaves still at ids N-1..2N-2
 hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, Marker* mk);
@@ -84,7 +87,9 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // traverse.hip
 // Traversal over a flattened BVH of ntris triangles. counters[0] (closest) and
 // counters[1] (occluded) count stack overflows and step-cap exits (either
-// means the results are partial); counters[2..5] hold the ray dispensers.
+// means the results are partial); counters[2..5] hold the ray dispensers;
+// counters[6] (closest) and counters[7] (occluded) keep the deepest stack any
+// ray reached (atomic max, never reset by a launch).
 // Launch shape of the persistent traversal kernels. Each ray's node order is
 // fixed, so results do not depend on it; the autotuner picks it.
 struct TraceLaunch {
@@ -109,5 +114,12 @@ hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
                                 uint32_t* counters, const TraceLaunch& L, hipStream_t s);
+// the same, launched through a sink (a stream or a graph under construction)
+void trace_closest_queue(const TRec* rec, int ntris, const float* root, const float* rays,
+                         const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
+                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls);
+void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
+                          int64_t capacity, const float* limit, const int32_t* src, const int32_t* dst,
+                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls);
 
 }  // namespace cav

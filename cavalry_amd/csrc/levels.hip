@@ -226,7 +226,32 @@ __global__ __launch_bounds__(1024) void k_refit_tail(const uint32_t* __restrict_
     }
 }
 
+// Depth of every leaf (parent hops to the root, at most n) and their maximum.
+__global__ __launch_bounds__(kBlock) void k_max_depth(const int4* __restrict__ topo, int64_t m, int n,
+                                                      uint32_t* __restrict__ out) {
+    const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t d = 0;
+    if (id < m && topo[id].w >= 0) {
+        int cur = (int)id;
+        for (int step = 0; cur != 0 && step < n; ++step) {
+            cur = topo[cur].z;
+            ++d;
+        }
+    }
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, dd, 64));
+    if ((threadIdx.x & 63) == 0 && d > 0) atomicMax(out, d);
+}
+
 }  // namespace
+
+hipError_t levels_max_depth(const LbvhBuffers& b, int n, uint32_t* out, hipStream_t s) {
+    const int64_t m = 2 * (int64_t)n - 1;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, out, (int64_t)1, 0u);
+    hipLaunchKernelGGL(k_max_depth, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, b.topo, m, n,
+                       out);
+    return hipGetLastError();
+}
 
 hipError_t levels_from_leaves(LbvhBuffers& b, const float* tris, int n, hipStream_t s, Marker* mk) {
     const int nb = (n + kBlock - 1) / kBlock;

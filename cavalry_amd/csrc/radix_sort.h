@@ -12,6 +12,7 @@ struct SortBuffers {
     uint32_t* hist = nullptr;                // passes * 256
     uint32_t* counts = nullptr;              // 256 * tiles
     uint32_t* offsets = nullptr;             // 256 * tiles
+    const uint32_t* n_dev = nullptr;         // optional: sort only the first min(n, *n_dev) pairs
     int result = 0;                          // index of the buffer holding the output
 };
 

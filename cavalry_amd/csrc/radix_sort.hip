@@ -29,11 +29,20 @@ __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
     return (uint32_t)(k >> shift) & (kRadix - 1);
 }
 
+// element count: n, or the smaller count held on the device
+__device__ __forceinline__ int64_t live_n(int64_t n, const uint32_t* n_dev) {
+    if (!n_dev) return n;
+    const int64_t d = (int64_t)*n_dev;
+    return d < n ? d : n;
+}
+
 // One read of the keys: digit histograms for every pass at once.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_hist(const K* __restrict__ keys, int64_t n,
-                                                 int passes, uint32_t* __restrict__ hist) {
+                                                 const uint32_t* __restrict__ n_dev, int passes,
+                                                 uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[8 * kRadix];
+    n = live_n(n, n_dev);
     for (int i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0;
     __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -49,9 +58,10 @@ __global__ __launch_bounds__(kBlock) void k_hist(const K* __restrict__ keys, int
 // counts[d * tiles + t] = number of keys of tile t with digit d.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_upsweep(const K* __restrict__ keys, int64_t n,
-                                                    int shift, int tiles,
+                                                    const uint32_t* __restrict__ n_dev, int shift, int tiles,
                                                     uint32_t* __restrict__ counts) {
     __shared__ uint32_t h[kWaves][kRadix];
+    n = live_n(n, n_dev);
     const int wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&h[0][0])[i] = 0;
     __syncthreads();
@@ -126,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
                                                       const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout,
                                                       uint32_t* __restrict__ vout, int64_t n,
-                                                      int shift, int tiles,
+                                                      const uint32_t* __restrict__ n_dev, int shift, int tiles,
                                                       const uint32_t* __restrict__ offsets) {
     __shared__ uint32_t wh[kWaves][kRadix];
     __shared__ uint32_t tstart[kRadix];
@@ -137,6 +147,8 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t tbase = (int64_t)blockIdx.x * kTile;
+    n = live_n(n, n_dev);
+    if (tbase >= n) return;  // a tile past the live count (the block is uniform)
     const int tile_n = (int)((n - tbase) < kTile ? (n - tbase) : kTile);
 
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wh[0][0])[i] = 0;
@@ -226,19 +238,19 @@ hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s, 
     if (hblocks > 2048) hblocks = 2048;
     if (hblocks < 1) hblocks = 1;
     mark(mk, "sort_clear");
-    hipLaunchKernelGGL(k_hist<K>, dim3(hblocks), dim3(kBlock), 0, s, b.keys[0], n, passes, b.hist);
+    hipLaunchKernelGGL(k_hist<K>, dim3(hblocks), dim3(kBlock), 0, s, b.keys[0], n, b.n_dev, passes, b.hist);
     mark(mk, "sort_hist");
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_upsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur], n, shift,
+        hipLaunchKernelGGL(k_upsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur], n, b.n_dev, shift,
                            tiles, b.counts);
         mark(mk, "sort_upsweep");
         hipLaunchKernelGGL(k_scan, dim3(kRadix), dim3(kBlock), 0, s, b.counts, tiles,
                            b.hist + p * kRadix, b.offsets);
         mark(mk, "sort_scan");
         hipLaunchKernelGGL(k_downsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur],
-                           b.vals[cur], b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, tiles,
+                           b.vals[cur], b.keys[cur ^ 1], b.vals[cur ^ 1], n, b.n_dev, shift, tiles,
                            b.offsets);
         mark(mk, "sort_downsweep");
         cur ^= 1;

@@ -148,6 +148,10 @@ struct RlArgs {
     uint32_t* rec_count;    // 1
     uint32_t *rec_hist, *rec_counts, *rec_offsets;  // radix sort work space
     int64_t rec_cap;
+    // shading order (CAV_RENDER_SORT_BY_MATERIAL): (material, Q entry) keys
+    // and queue slots of each bounce's hits, and their radix sort's work space
+    uint32_t *skeys[2], *svals[2];
+    uint32_t *shist, *scounts, *soffsets;
 };
 
 // one spp round (a.spp_done .. a.spp_done + a.spp_round) of each integrator;

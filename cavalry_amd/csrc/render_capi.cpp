@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -40,6 +41,7 @@ struct cav_scene {
     std::vector<cav_mesh> host_meshes;
     std::vector<float> host_area;
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // Scene::sceneBounds (Core/Scene.cu:131-136)
+    std::atomic<int> users{0};  // server pools and unfinished server requests on this scene
 };
 
 namespace {
@@ -47,7 +49,7 @@ namespace {
 // device bytes per sample of a round (queues, shadow tasks, hits) and per
 // sample of the whole render (radiance + film position)
 constexpr double kQueueBytes = 2 * (24 + 4 + 12 + 4 + 1) + (24 + 4 + 4 + 4 + 4 + 12) + 1 + 16 + 4;
-constexpr double kRlQueueBytes = 2 * 12 + (4 * 5 + 12 * 3 + 4 + 4 + 4);
+constexpr double kRlQueueBytes = 2 * 12 + (4 * 5 + 12 * 3 + 4 + 4 + 4) + 16 + 2 * 4 * 256.0 / 4096;
 constexpr double kResultBytes = 12 + 8;
 
 // Per-context render state: Halton tables and the sample/queue buffers,
@@ -205,6 +207,14 @@ int ensure_rl(cav_ctx* ctx, RenderState* rs, int64_t nsamp, int64_t cells, int64
         if (rc == CAV_OK) rc = grab_into(ctx, L, sl.npdf, 4 * S, "rl next pdf");
         if (rc == CAV_OK) rc = grab_into(ctx, L, sl.ncell, 4 * S, "rl next bin");
         if (rc == CAV_OK) rc = grab_into(ctx, L, r.dims, 64, "rl dimensions");
+        const size_t T = (size_t)sort_tiles(nsamp);
+        for (int k = 0; k < 2 && rc == CAV_OK; ++k) {
+            rc = grab_into(ctx, L, r.skeys[k], 4 * S, "rl shading-order keys");
+            if (rc == CAV_OK) rc = grab_into(ctx, L, r.svals[k], 4 * S, "rl shading order");
+        }
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.shist, 4 * 8 * 256, "rl shading-order histogram");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.scounts, 4 * 256 * T, "rl shading-order counts");
+        if (rc == CAV_OK) rc = grab_into(ctx, L, r.soffsets, 4 * 256 * T, "rl shading-order offsets");
         if (rc != CAV_OK) { free_list(L); return rc; }
         rs->rl_cap = nsamp;
     }
@@ -277,6 +287,8 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 }  // namespace
 
 namespace cav {
+void scene_hold(cav_scene* sc, int delta) { sc->users.fetch_add(delta); }
+
 SceneDev scene_dev(const cav_scene* sc) {
     return SceneDev{sc->bvh->rec,  sc->bvh->root, sc->ntris,   sc->soup,
                     sc->tri_mesh,   sc->meshes,   sc->area,      sc->lights,  sc->nmeshes,
@@ -496,6 +508,8 @@ int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** 
 
 int cav_scene_destroy(cav_scene* sc) {
     if (!sc) return CAV_ERR_INVALID;
+    if (sc->users.load() > 0)
+        return fail(sc->ctx, CAV_ERR_BUSY, "the scene is in use by a server (close the server first)");
     if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
     if (sc->bvh) cav_bvh_destroy(sc->bvh);
     for (void* p : {(void*)sc->soup, (void*)sc->tri_mesh, (void*)sc->meshes, (void*)sc->area,

@@ -787,6 +787,28 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
     }
 }
 
+// sortLightingQueue's key (:226-272): material * entries + Q entry of the hit
+// (misses after every material), value = queue slot, for the first *count
+// slots; the stable radix sort of these pairs is the shading order.
+__global__ __launch_bounds__(kBlock) void k_entry_keys(SceneDev sc, Grid gr, Queue qi, const int32_t* __restrict__ hit_prim,
+                                                       const float* __restrict__ hit_t, const uint32_t* count,
+                                                       int64_t cap, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= cap || i >= (int64_t)*count) return;
+    const uint32_t nent = (uint32_t)gr.g * (uint32_t)gr.g * (uint32_t)gr.g;
+    const int prim = hit_prim[i];
+    uint32_t key = 7u * nent;
+    if (prim >= 0) {
+        const int m = sc.meshes[sc.tri_mesh[prim]].material;
+        const float* r = qi.ray + i * 6;
+        const v3 pos = add3(mk3(r[0], r[1], r[2]), scale3(mk3(r[3], r[4], r[5]), hit_t[i]));
+        key = (uint32_t)(m >= 0 && m < 6 ? m : 6) * nent + (uint32_t)q_entry(gr, pos);
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
 // Deterministic mode, before k_qdist: every key slot empty (all ones, sorted
 // last), values numbered in slot order, no records.
 __global__ __launch_bounds__(kBlock) void k_rec_reset(uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
@@ -862,13 +884,24 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
             return e;
         hipLaunchKernelGGL(k_dim_max, dim3((unsigned)((nsamp + kMaxBlock - 1) / kMaxBlock)), dim3(kMaxBlock), 0, s,
                            a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
-        if (a.sort_perm &&
-            (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts + qin, nsamp, a.sort_hist, a.sort_perm, s)) !=
-                hipSuccess)
-            return e;
+        const uint32_t* perm = nullptr;
+        if (a.sort_perm) {  // shading order: stable sort by (material, Q entry)
+            const unsigned kb = (unsigned)((nsamp + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(k_entry_keys, dim3(kb), dim3(kBlock), 0, s, a.scene, gr, a.q[qin], a.hit_prim,
+                               a.hit_t, a.counts + qin, nsamp, rl.skeys[0], rl.svals[0]);
+            SortBuffers<uint32_t> sb;
+            sb.keys[0] = rl.skeys[0]; sb.keys[1] = rl.skeys[1];
+            sb.vals[0] = rl.svals[0]; sb.vals[1] = rl.svals[1];
+            sb.hist = rl.shist; sb.counts = rl.scounts; sb.offsets = rl.soffsets;
+            sb.n_dev = a.counts + qin;
+            int bits = 1;
+            while (((int64_t)1 << bits) < 8 * nent) ++bits;
+            if ((e = radix_sort_pairs(sb, nsamp, bits, s, nullptr)) != hipSuccess) return e;
+            perm = sb.vals[sb.result];
+        }
         hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
                            rl.slots, a.sh, t, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin,
-                           a.sort_perm);
+                           perm);
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s)) != hipSuccess)
             return e;
@@ -878,7 +911,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
             hipLaunchKernelGGL(k_rec_reset, dim3(rb), dim3(kBlock), 0, s, rl.rec_keys[0], rl.rec_idx[0],
                                rl.rec_count, rl.rec_cap);
         hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
-                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, a.sort_perm,
+                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, perm,
                            rec);
         if (rec.keys) {
             SortBuffers<uint64_t> sb;
@@ -895,7 +928,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
                            nent, L);
         hipLaunchKernelGGL(k_next, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, a.q[qin], a.q[qout],
                            rl.q[qout], rl.slots, a.hit_prim, a.hit_t, hu, hv, a.counts, rl.dims, qin, qout,
-                           a.sort_perm);
+                           perm);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
         qin = qout;
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "launch.h"
 #include "render.h"
 
 namespace cav {
@@ -47,10 +48,12 @@ struct PoolDev {
 };
 
 // camera rays of the chunks, appended to queue qn
-void pool_raygen(const PoolDev& p, const Chunk* chunks_dev, int nchunks, int64_t total, int qn, hipStream_t s);
+hipError_t pool_raygen(const PoolDev& p, const Chunk* chunks_dev, int nchunks, int64_t total, int qn,
+                       hipStream_t s);
 // one bounce of every task in queue qin: closest hits, shading (path or
 // directlighting per task), shadow rays, their contributions; continuation
-// rays go to queue qin ^ 1
-hipError_t pool_bounce(const PoolDev& p, int qin, hipStream_t s);
+// rays go to queue qin ^ 1. Launched on a stream or recorded into a graph;
+// ls.err / ls.failed name the first launch that failed.
+void pool_bounce(const PoolDev& p, int qin, LaunchSink& ls);
 
 }  // namespace cav

@@ -10,6 +10,7 @@
 // others, and whether its samples are admitted at once or in chunks.
 #include "../../include/cavalry.h"
 #include "dev_common.h"
+#include "launch.h"
 #include "serve.h"
 #include "shade.h"
 
@@ -142,26 +143,25 @@ __global__ void k_pool_step(PoolDev p, int qin) {
 
 }  // namespace
 
-void pool_raygen(const PoolDev& p, const Chunk* chunks_dev, int nchunks, int64_t total, int qn, hipStream_t s) {
-    if (total <= 0) return;
-    hipLaunchKernelGGL(k_pool_raygen, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p,
-                       chunks_dev, nchunks, total, qn);
+hipError_t pool_raygen(const PoolDev& p, const Chunk* chunks_dev, int nchunks, int64_t total, int qn,
+                       hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    LaunchSink ls(s);
+    ls.launch("k_pool_raygen", &k_pool_raygen, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), p,
+              chunks_dev, nchunks, total, qn);
+    return ls.err;
 }
 
-hipError_t pool_bounce(const PoolDev& p, int qin, hipStream_t s) {
+void pool_bounce(const PoolDev& p, int qin, LaunchSink& ls) {
     const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
-    hipError_t e = trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray,
-                                       p.counts + qin, p.cap, p.hit_prim, p.hit_t,
-                                       p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
-                                       p.trace_counters, p.launch, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pool_shade, dim3(nb), dim3(kBlock), 0, s, p, qin);
-    e = trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit,
-                             p.sh.src, p.sh.dst, p.vis, p.trace_counters, p.launch, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pool_nee, dim3(nb), dim3(kBlock), 0, s, p);
-    hipLaunchKernelGGL(k_pool_step, dim3(1), dim3(64), 0, s, p, qin);
-    return hipGetLastError();
+    trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray, p.counts + qin, p.cap, p.hit_prim,
+                        p.hit_t, p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
+                        p.trace_counters, p.launch, ls);
+    ls.launch("k_pool_shade", &k_pool_shade, dim3(nb), dim3(kBlock), p, qin);
+    trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit, p.sh.src, p.sh.dst,
+                         p.vis, p.trace_counters, p.launch, ls);
+    ls.launch("k_pool_nee", &k_pool_nee, dim3(nb), dim3(kBlock), p);
+    ls.launch("k_pool_step", &k_pool_step, dim3(1), dim3(64), p, qin);
 }
 
 }  // namespace cav

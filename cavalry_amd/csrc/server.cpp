@@ -44,6 +44,7 @@
 namespace cav {
 SceneDev scene_dev(const cav_scene* sc);
 int halton_dev(cav_ctx* ctx, HaltonDev* out);
+void scene_hold(cav_scene* sc, int delta);
 }  // namespace cav
 
 using namespace cav;
@@ -55,6 +56,9 @@ constexpr int kSlots = 256;  // requests in flight per pool
 double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// "what: HIP error string", for failure messages
+std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
 
 // first-fit sub-allocator over one pre-allocated block (no allocation calls
 // while serving)
@@ -122,9 +126,12 @@ struct Request {
     float *res = nullptr, *sxy = nullptr, *rgb = nullptr, *w = nullptr;
     float4* partial = nullptr;
     float *h_rgb = nullptr, *h_w = nullptr;
+    uint32_t* h_ovf = nullptr;  // pool overflow counters at admission [0..1] and at the end [2..3]
     hipEvent_t ready = nullptr;
     bool finishing = false;
     bool cancel = false;
+    bool final = false;  // a final status was set (the scene's hold released)
+    std::string msg;     // why it failed
 };
 
 struct LiveChunk {
@@ -149,7 +156,7 @@ struct Pool {
     Chunk* h_chunks = nullptr;  // pinned
     std::vector<hipEvent_t> ring;
     int64_t done_upto = 0;      // steps known complete on the device
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // the bounce sequence per queue parity
 };
 
 }  // namespace
@@ -176,7 +183,9 @@ struct cav_server {
     std::vector<cav_ctx*> rl_ctxs;
     unsigned long long pool_rays[2] = {0, 0};
     int64_t rl_rays[2] = {0, 0};
-    std::string error;
+    std::string error;  // fatal: the failing call and its HIP error
+    std::string graph_note;  // why a pool runs without graphs
+    int graph_failures = 0;
 };
 
 namespace {
@@ -190,6 +199,18 @@ struct ApiLock {
 }  // namespace
 
 namespace {
+
+// Sets a request's final status once, releasing its hold on the scene.
+void finalize(cav_server* sv, Request* r, int32_t status, int32_t error = CAV_OK, const std::string& msg = "") {
+    if (r->final) return;
+    r->final = true;
+    r->status = status;
+    r->error = error;
+    if (!msg.empty()) r->msg = msg;
+    r->t_done = now_ms(sv->t0);
+    scene_hold(r->scene, -1);
+    sv->done_cv.notify_all();
+}
 
 int alloc_dev(std::vector<void*>& list, void** p, size_t bytes) {
     if (hipMalloc(p, bytes) != hipSuccess) return CAV_ERR_OOM;
@@ -236,7 +257,11 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
          hipMemsetAsync(d.rays, 0, 64, p->stream) == hipSuccess &&
          hipMemsetAsync(d.trace_counters, 0, 64, p->stream) == hipSuccess &&
          hipStreamSynchronize(p->stream) == hipSuccess;
-    if (!ok) return CAV_ERR_HIP;
+    if (!ok) {
+        sv->error = "creating the ray pool's stream, events or pinned staging failed";
+        return CAV_ERR_HIP;
+    }
+    scene_hold(sc, 1);
     out = p.get();
     sv->pools.push_back(std::move(p));
     return CAV_OK;
@@ -252,6 +277,7 @@ void destroy_pool(Pool* p) {
     if (p->h_reqs) (void)hipHostFree(p->h_reqs);
     if (p->h_chunks) (void)hipHostFree(p->h_chunks);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    scene_hold(p->scene, -1);
 }
 
 Pool* pool_for(cav_server* sv, cav_scene* sc) {
@@ -269,7 +295,7 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     if (slot < 0) return false;
     const size_t S = (size_t)r->nsamp, P = (size_t)r->npix;
     const size_t dlen = 12 * S + 8 * S + 144 * P + 12 * P + 4 * P + 5 * 256;
-    const size_t hlen = 16 * P + 2 * 256;
+    const size_t hlen = 16 * P + 3 * 256;
     size_t doff = 0, hoff = 0;
     if (!sv->dev_arena.alloc(dlen, doff)) return false;
     if (!sv->host_arena.alloc(hlen, hoff)) {
@@ -290,6 +316,7 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     char* h = sv->host_arena.at(hoff);
     r->h_rgb = reinterpret_cast<float*>(carve(h, 12 * P));
     r->h_w = reinterpret_cast<float*>(carve(h, 4 * P));
+    r->h_ovf = reinterpret_cast<uint32_t*>(carve(h, 16));
     r->dev_off = doff; r->dev_len = dlen; r->host_off = hoff; r->host_len = hlen;
     r->slot = slot;
     p->slots[slot] = r;
@@ -306,7 +333,11 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     e.sxy = r->sxy;
     // the pool's earlier steps no longer read this slot (its previous owner
     // finished), so the stream-ordered copy is safe
-    (void)hipMemcpyAsync(p->d_reqs + slot, &e, sizeof(ReqDev), hipMemcpyHostToDevice, p->stream);
+    hipError_t he = hipMemcpyAsync(p->d_reqs + slot, &e, sizeof(ReqDev), hipMemcpyHostToDevice, p->stream);
+    // the pool's overflow counters before the request's first bounce
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(r->h_ovf, p->dev.trace_counters, 8, hipMemcpyDeviceToHost, p->stream);
+    if (he != hipSuccess) sv->error = hip_msg("request admission (hipMemcpyAsync)", he);
     r->active = true;
     return true;
 }
@@ -317,17 +348,29 @@ void release_request_memory(cav_server* sv, Request* r) {
     r->dev_len = r->host_len = 0;
 }
 
-int capture_graphs(Pool* p) {
+// Records the pool's bounce sequence for each queue parity into a graph,
+// node by node (no stream capture). On failure the pool runs without graphs
+// and the reason is kept in `why`.
+bool build_graphs(Pool* p, std::string& why) {
     for (int k = 0; k < 2; ++k) {
         hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return CAV_ERR_HIP;
-        const hipError_t e = pool_bounce(p->dev, k, p->stream);
-        if (hipStreamEndCapture(p->stream, &g) != hipSuccess || e != hipSuccess) return CAV_ERR_HIP;
-        const hipError_t ie = hipGraphInstantiate(&p->graph[k], g, nullptr, nullptr, 0);
+        hipError_t e = hipGraphCreate(&g, 0);
+        if (e != hipSuccess) { why = hip_msg("hipGraphCreate", e); break; }
+        LaunchSink ls(g);
+        pool_bounce(p->dev, k, ls);
+        if (ls.err != hipSuccess) {
+            why = hip_msg((std::string("hipGraphAddKernelNode ") + ls.failed).c_str(), ls.err);
+        } else if ((e = hipGraphInstantiate(&p->graph[k], g, nullptr, nullptr, 0)) != hipSuccess) {
+            p->graph[k] = nullptr;
+            why = hip_msg("hipGraphInstantiate", e);
+        }
         (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return CAV_ERR_HIP;
+        if (!why.empty()) break;
     }
-    return CAV_OK;
+    if (why.empty()) return true;
+    for (auto& x : p->graph)
+        if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+    return false;
 }
 
 // one step of one pool; called with sv->mu held (it touches request state),
@@ -337,7 +380,8 @@ int pool_step(cav_server* sv, Pool* p) {
     // wait until the chunk/event ring slot of step - ring + 1 is free
     while (p->step - p->done_upto >= sv->cfg.max_steps_ahead) {
         const int64_t s = p->done_upto;
-        if (hipEventSynchronize(p->ring[s % ring]) != hipSuccess) return CAV_ERR_HIP;
+        const hipError_t e = hipEventSynchronize(p->ring[s % ring]);
+        if (e != hipSuccess) { sv->error = hip_msg("hipEventSynchronize (bounce step)", e); return CAV_ERR_HIP; }
         p->done_upto = s + 1;
     }
     // admission: round-robin, one chunk (at most the quantum) per request in
@@ -352,8 +396,7 @@ int pool_step(cav_server* sv, Pool* p) {
         Request* r = p->pending.front();
         p->pending.pop_front();
         if (r->cancel && !r->active) {
-            r->status = CAV_SERVE_CANCELLED;
-            sv->done_cv.notify_all();
+            finalize(sv, r, CAV_SERVE_CANCELLED);
             continue;
         }
         const int64_t room = cap - p->reserved;
@@ -365,6 +408,7 @@ int pool_step(cav_server* sv, Pool* p) {
             p->pending.push_back(r);
             continue;
         }
+        if (!sv->error.empty()) return CAV_ERR_HIP;
         const int64_t n = std::min({room, sv->cfg.chunk_samples, r->nsamp - r->next_sample});
         if (r->t_start < 0) {
             r->t_start = now_ms(sv->t0);
@@ -382,14 +426,23 @@ int pool_step(cav_server* sv, Pool* p) {
     }
     if (nch) {
         Chunk* dc = p->d_chunks + (size_t)(p->step % ring) * kSlots;
-        if (hipMemcpyAsync(dc, hc, sizeof(Chunk) * nch, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+        hipError_t e = hipMemcpyAsync(dc, hc, sizeof(Chunk) * nch, hipMemcpyHostToDevice, p->stream);
+        if (e != hipSuccess) { sv->error = hip_msg("hipMemcpyAsync (chunk list)", e); return CAV_ERR_HIP; }
+        if ((e = pool_raygen(p->dev, dc, nch, total, p->qin, p->stream)) != hipSuccess) {
+            sv->error = hip_msg("k_pool_raygen launch", e);
             return CAV_ERR_HIP;
-        pool_raygen(p->dev, dc, nch, total, p->qin, p->stream);
+        }
     }
     if (p->graph[p->qin]) {
-        if (hipGraphLaunch(p->graph[p->qin], p->stream) != hipSuccess) return CAV_ERR_HIP;
-    } else if (pool_bounce(p->dev, p->qin, p->stream) != hipSuccess) {
-        return CAV_ERR_HIP;
+        const hipError_t e = hipGraphLaunch(p->graph[p->qin], p->stream);
+        if (e != hipSuccess) { sv->error = hip_msg("hipGraphLaunch (bounce step)", e); return CAV_ERR_HIP; }
+    } else {
+        LaunchSink ls(p->stream);
+        pool_bounce(p->dev, p->qin, ls);
+        if (ls.err != hipSuccess) {
+            sv->error = hip_msg((std::string(ls.failed) + " launch").c_str(), ls.err);
+            return CAV_ERR_HIP;
+        }
     }
     // chunks whose last bounce was this step
     for (size_t k = 0; k < p->live.size();) {
@@ -400,16 +453,26 @@ int pool_step(cav_server* sv, Pool* p) {
         p->live.pop_back();
         if (--r->outstanding == 0 && r->next_sample == r->nsamp) {
             film_launch(r->cam.width, r->cam.height, r->prm.spp, r->res, r->sxy, r->partial, r->rgb, r->w, p->stream);
-            (void)hipMemcpyAsync(r->h_rgb, r->rgb, 12 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
-            (void)hipMemcpyAsync(r->h_w, r->w, 4 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
-            if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) != hipSuccess ||
-                hipEventRecord(r->ready, p->stream) != hipSuccess)
-                return CAV_ERR_HIP;
+            hipError_t e = hipGetLastError();
+            const char* what = "film kernels";
+            if (e == hipSuccess) {
+                what = "hipMemcpyAsync (film to staging)";
+                e = hipMemcpyAsync(r->h_rgb, r->rgb, 12 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(r->h_w, r->w, 4 * (size_t)r->npix, hipMemcpyDeviceToHost, p->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(r->h_ovf + 2, p->dev.trace_counters, 8, hipMemcpyDeviceToHost, p->stream);
+            if (e == hipSuccess) {
+                what = "request event";
+                e = hipEventCreateWithFlags(&r->ready, hipEventDisableTiming);
+            }
+            if (e == hipSuccess) e = hipEventRecord(r->ready, p->stream);
+            if (e != hipSuccess) { sv->error = hip_msg(what, e); return CAV_ERR_HIP; }
             r->finishing = true;
             p->slots[r->slot] = nullptr;
         }
     }
-    if (hipEventRecord(p->ring[p->step % ring], p->stream) != hipSuccess) return CAV_ERR_HIP;
+    const hipError_t e = hipEventRecord(p->ring[p->step % ring], p->stream);
+    if (e != hipSuccess) { sv->error = hip_msg("hipEventRecord (bounce step)", e); return CAV_ERR_HIP; }
     ++p->step;
     p->qin ^= 1;
     return CAV_OK;
@@ -424,19 +487,27 @@ void finish_ready(cav_server* sv) {
         r->finishing = false;
         (void)hipEventDestroy(r->ready);
         r->ready = nullptr;
+        // any traversal of the pool cut short while this request was in it
+        // (the counter is pool-wide, so a neighbour's overflow fails it too)
+        const bool ovf = r->h_ovf[2] != r->h_ovf[0] || r->h_ovf[3] != r->h_ovf[1];
+        int32_t st = CAV_SERVE_DONE, err = CAV_OK;
+        std::string msg;
         if (q != hipSuccess) {
-            r->status = CAV_SERVE_FAILED;
-            r->error = CAV_ERR_HIP;
+            st = CAV_SERVE_FAILED;
+            err = CAV_ERR_HIP;
+            msg = hip_msg("request completion event", q);
         } else if (r->cancel) {
-            r->status = CAV_SERVE_CANCELLED;
+            st = CAV_SERVE_CANCELLED;
+        } else if (ovf) {
+            st = CAV_SERVE_FAILED;
+            err = CAV_ERR_OVERFLOW;
+            msg = "traversal stack overflow in the request's ray pool";
         } else {
             std::memcpy(r->out_rgb, r->h_rgb, 12 * (size_t)r->npix);
             std::memcpy(r->out_w, r->h_w, 4 * (size_t)r->npix);
-            r->status = CAV_SERVE_DONE;
         }
-        r->t_done = now_ms(sv->t0);
         release_request_memory(sv, r);
-        sv->done_cv.notify_all();
+        finalize(sv, r, st, err, msg);
     }
 }
 
@@ -449,15 +520,17 @@ void worker_main(cav_server* sv) {
             sv->incoming.pop_front();
             Pool* p = pool_for(sv, r->scene);
             int rc = CAV_OK;
+            std::string why;
             if (!p) {
                 rc = make_pool(sv, r->scene, p);
-                if (rc == CAV_OK && sv->cfg.use_graphs) rc = capture_graphs(p);
+                if (rc == CAV_OK && sv->cfg.use_graphs && !build_graphs(p, why)) {
+                    // the pool runs its launches directly; the reason is kept
+                    sv->graph_note = why;
+                    ++sv->graph_failures;
+                }
             }
             if (rc != CAV_OK) {
-                r->status = CAV_SERVE_FAILED;
-                r->error = rc;
-                r->t_done = now_ms(sv->t0);
-                sv->done_cv.notify_all();
+                finalize(sv, r, CAV_SERVE_FAILED, rc, sv->error.empty() ? "ray pool allocation failed" : sv->error);
                 continue;
             }
             p->pending.push_back(r);
@@ -468,13 +541,12 @@ void worker_main(cav_server* sv) {
             if (p->live.empty() && p->pending.empty()) continue;
             busy = true;
             if (pool_step(sv, p) != CAV_OK) {
-                sv->error = "a HIP call failed while serving";
-                for (auto& kv : sv->reqs)
-                    if (kv.second->status == CAV_SERVE_QUEUED || kv.second->status == CAV_SERVE_RUNNING) {
-                        kv.second->status = CAV_SERVE_FAILED;
-                        kv.second->error = CAV_ERR_HIP;
-                    }
-                sv->done_cv.notify_all();
+                if (sv->error.empty()) sv->error = "a HIP call failed while serving";
+                for (auto& kv : sv->reqs) {
+                    Request* q = kv.second.get();
+                    if (q->prm.integrator != CAV_INTEGRATOR_RLPATH && !q->final)
+                        finalize(sv, q, CAV_SERVE_FAILED, CAV_ERR_HIP, sv->error);
+                }
                 return;
             }
         }
@@ -518,8 +590,7 @@ void rl_main(cav_server* sv, cav_ctx* rl_ctx) {
         Request* r = sv->rl_queue.front();
         sv->rl_queue.pop_front();
         if (r->cancel) {
-            r->status = CAV_SERVE_CANCELLED;
-            sv->done_cv.notify_all();
+            finalize(sv, r, CAV_SERVE_CANCELLED);
             continue;
         }
         r->status = CAV_SERVE_RUNNING;
@@ -527,19 +598,14 @@ void rl_main(cav_server* sv, cav_ctx* rl_ctx) {
         lk.unlock();
         cav_render_stats st{};
         const int rc = cav_render(rl_ctx, r->scene, &r->cam, &r->prm, r->out_rgb, r->out_w, &st);
+        const std::string why = rc != CAV_OK ? std::string(cav_last_error(rl_ctx)) : std::string();
         lk.lock();
-        r->t_done = now_ms(sv->t0);
         r->closest = st.closest_rays;
         r->shadow = st.shadow_rays;
         sv->rl_rays[0] += st.closest_rays;
         sv->rl_rays[1] += st.shadow_rays;
-        if (rc != CAV_OK) {
-            r->status = CAV_SERVE_FAILED;
-            r->error = rc;
-        } else {
-            r->status = r->cancel ? CAV_SERVE_CANCELLED : CAV_SERVE_DONE;
-        }
-        sv->done_cv.notify_all();
+        if (rc != CAV_OK) finalize(sv, r, CAV_SERVE_FAILED, rc, why);
+        else finalize(sv, r, r->cancel ? CAV_SERVE_CANCELLED : CAV_SERVE_DONE);
     }
 }
 
@@ -619,7 +685,8 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
     r->npix = npix;
     r->nsamp = npix * prm->spp;
     ApiLock lock(sv);
-    if (sv->stop || !sv->error.empty()) return fail(sv->ctx, CAV_ERR_INVALID, "server is closing or failed");
+    if (sv->stop) return fail(sv->ctx, CAV_ERR_INVALID, "the server is closing");
+    if (!sv->error.empty()) return fail(sv->ctx, CAV_ERR_HIP, "the server failed: " + sv->error);
     if (prm->integrator != CAV_INTEGRATOR_RLPATH) {
         const size_t need = 20 * (size_t)r->nsamp + 160 * (size_t)npix + 5 * 256;
         if (need > (size_t)sv->cfg.arena_bytes || 16 * (size_t)npix + 512 > (size_t)sv->cfg.host_arena_bytes)
@@ -629,6 +696,7 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
     r->t_submit = now_ms(sv->t0);
     *id = r->id;
     Request* raw = r.get();
+    scene_hold(scene, 1);  // until the request's final status
     sv->reqs.emplace(raw->id, std::move(r));
     if (prm->integrator == CAV_INTEGRATOR_RLPATH) sv->rl_queue.push_back(raw);
     else sv->incoming.push_back(raw);
@@ -654,6 +722,7 @@ int cav_serve_poll(cav_server* sv, int64_t id, int32_t wait, cav_serve_result* o
     out->done_ms = r->t_done;
     out->closest_rays = r->closest;
     out->shadow_rays = r->shadow;
+    if (r->status == CAV_SERVE_FAILED) fail(sv->ctx, r->error, "request " + std::to_string(id) + ": " + r->msg);
     if (final_state()) sv->reqs.erase(it);  // the id is released with its final status
     return CAV_OK;
 }
@@ -687,10 +756,17 @@ int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
         for (auto& p : sv->pools) steps += p->step;
         stats->steps = steps;
         stats->pools = (int32_t)sv->pools.size();
+        int32_t g = 0;
+        for (auto& p : sv->pools) g += p->graph[0] != nullptr;
+        stats->graph_pools = g;
     }
     for (auto& p : sv->pools) destroy_pool(p.get());
-    for (auto& kv : sv->reqs)
-        if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
+    for (auto& kv : sv->reqs) {
+        Request* r = kv.second.get();
+        if (r->ready) (void)hipEventDestroy(r->ready);
+        // requests a failed server never finished still hold their scenes
+        if (!r->final) finalize(sv, r, CAV_SERVE_FAILED, CAV_ERR_HIP, sv->error);
+    }
     for (cav_ctx* x : sv->rl_ctxs) cav_ctx_destroy(x);
     if (sv->dev_base) (void)hipFree(sv->dev_base);
     if (sv->host_base) (void)hipHostFree(sv->host_base);

@@ -12,6 +12,7 @@
 #include <cfloat>
 
 #include "dev_common.h"
+#include "launch.h"
 #include "lbvh.h"
 
 namespace cav {
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
     int bp = -1, sm = -1, dm = -1, cur = 0, steps = 0;
     bool hit = false, use_lim = false;
     uint32_t ovf = 0;
+    int maxsp = 0;  // deepest stack this lane reached
     volatile int spill[kStack - kLdsDepth];  // in scratch memory, not registers
     int sp = 0;
     int64_t chunk_base = 0;
@@ -195,6 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                     const int far = lfar ? refs.x : refs.y;
                     const int near = lfar ? refs.y : refs.x;
                     if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
+                    maxsp = max(maxsp, sp);
                     if (sp < kStack) { cur = near; next = true; } else ++ovf;
                 } else if (hl || hr) {
                     if (sp < kStack) { cur = hl ? refs.x : refs.y; next = true; } else ++ovf;
@@ -233,6 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
         }
     }
     if (ovf) atomicAdd(overflow, ovf);
+    // deepest stack over the launch: overflow[6] (6 words past the overflow count)
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) maxsp = max(maxsp, __shfl_xor(maxsp, dd, 64));
+    if (lane == 0 && maxsp > 0) atomicMax(overflow + 6, (uint32_t)maxsp);
 }
 
 // zeroes n words; a kernel rather than hipMemsetAsync so the launch sequence
@@ -259,7 +266,7 @@ int persistent_blocks(const void* kernel, int max_per_cu) {
 // The kernel of the launch's stack depth on a persistent grid of its
 // workgroups per CU, for `work` rays at most.
 template <bool C, typename... A>
-void launch_trace(const TraceLaunch& L, int64_t work, hipStream_t s, A... args) {
+void launch_trace(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
     const bool deep = L.lds_depth >= 48;
     const void* k = deep ? reinterpret_cast<const void*>(&k_trace<C, 48>)
                          : reinterpret_cast<const void*>(&k_trace<C, 32>);
@@ -267,10 +274,9 @@ void launch_trace(const TraceLaunch& L, int64_t work, hipStream_t s, A... args) 
     const int64_t need = (work + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    if (deep)
-        hipLaunchKernelGGL((k_trace<C, 48>), dim3((unsigned)blocks), dim3(kBlock), 0, s, args..., L.chunk);
-    else
-        hipLaunchKernelGGL((k_trace<C, 32>), dim3((unsigned)blocks), dim3(kBlock), 0, s, args..., L.chunk);
+    const char* name = C ? "k_trace<closest>" : "k_trace<any>";
+    if (deep) ls.launch(name, &k_trace<C, 48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    else ls.launch(name, &k_trace<C, 32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
 }
 
 }  // namespace
@@ -281,10 +287,11 @@ hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
     if (e != hipSuccess) return e;
-    launch_trace<true>(L, nrays, s, rec, root, rays, nrays, (const uint32_t*)nullptr, (const float*)nullptr,
+    LaunchSink ls(s);
+    launch_trace<true>(L, nrays, ls, rec, root, rays, nrays, (const uint32_t*)nullptr, (const float*)nullptr,
                        (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
                        counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
-    return hipGetLastError();
+    return ls.err;
 }
 
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
@@ -294,34 +301,50 @@ hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
     if (e != hipSuccess) return e;
-    launch_trace<false>(L, nrays, s, rec, (const float*)nullptr, rays, nrays, (const uint32_t*)nullptr, limit, src,
+    LaunchSink ls(s);
+    launch_trace<false>(L, nrays, ls, rec, (const float*)nullptr, rays, nrays, (const uint32_t*)nullptr, limit, src,
                         dst, (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
                         counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
-    return hipGetLastError();
+    return ls.err;
 }
 
 // Queue variants: the ray count is read on the device (no host round trip),
-// the grid is the full persistent size. counters as above.
+// the grid is the full persistent size. counters as above. The dispenser is
+// cleared by a kernel, so the sequence can be recorded into a graph.
+void trace_closest_queue(const TRec* rec, int ntris, const float* root, const float* rays,
+                         const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
+                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
+    ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 2, 2);
+    launch_trace<true>(L, capacity, ls, rec, root, rays, capacity, nrays_dev, (const float*)nullptr,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
+                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
+}
+
+void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
+                          int64_t capacity, const float* limit, const int32_t* src, const int32_t* dst,
+                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
+    ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 4, 2);
+    launch_trace<false>(L, capacity, ls, rec, (const float*)nullptr, rays, capacity, nrays_dev, limit, src, dst,
+                        (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
+                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
+}
+
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
                                const TraceLaunch& L, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 2, 2);
-    launch_trace<true>(L, capacity, s, rec, root, rays, capacity, nrays_dev, (const float*)nullptr,
-                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
-                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
-    return hipGetLastError();
+    LaunchSink ls(s);
+    trace_closest_queue(rec, ntris, root, rays, nrays_dev, capacity, prim, t, u, v, counters, L, ls);
+    return ls.err;
 }
 
 hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
                                 uint32_t* counters, const TraceLaunch& L, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters + 4, 2);
-    launch_trace<false>(L, capacity, s, rec, (const float*)nullptr, rays, capacity, nrays_dev, limit, src, dst,
-                        (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
-                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
-    return hipGetLastError();
+    LaunchSink ls(s);
+    trace_occluded_queue(rec, ntris, rays, nrays_dev, capacity, limit, src, dst, visible, counters, L, ls);
+    return ls.err;
 }
 
 }  // namespace cav

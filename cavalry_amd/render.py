@@ -132,9 +132,22 @@ class Scene:
         ctx.own(self)
         return self, st
 
+    def bvh(self):
+        """The scene's BVH (owned by the scene) as a BVH object that does not
+        destroy it: export(), trace counters, max_depth()."""
+        from .bvh import BVH
+        view = BVH.__new__(BVH)
+        view.ctx = self.ctx
+        view.handle = ctypes.c_void_p(lib().cav_scene_bvh(self.handle))
+        view.n = int(len(self.arrays.tris)) if self.arrays is not None else int(self.setup.ntris)
+        view.close = lambda: None  # the scene frees it
+        return view
+
+    close_rank = 1  # after the servers that render it (Context.close)
+
     def close(self) -> None:
         if self.handle:
-            lib().cav_scene_destroy(self.handle)
+            check(self.ctx.handle, lib().cav_scene_destroy(self.handle))
             self.handle = None
 
     def __del__(self):

@@ -37,6 +37,10 @@ class Server:
         h = ctypes.c_void_p()
         check(ctx.handle, lib().cav_serve_open(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
+        # Films the server may still write (asynchronously, from its own
+        # threads): each stays referenced until its request's final status is
+        # polled or the server is closed, whatever the caller keeps.
+        self._films = {}
         ctx.own(self)
 
     def submit(self, scene: Scene, spp: int, integrator: str = "path", camera=None, max_depth=None,
@@ -54,21 +58,30 @@ class Server:
             self.handle, scene.handle, ctypes.byref(cam), ctypes.byref(prm),
             film.rgb.ctypes.data_as(ctypes.c_void_p), film.weight.ctypes.data_as(ctypes.c_void_p),
             ctypes.byref(rid)))
+        self._films[int(rid.value)] = film
         return Ticket(int(rid.value), film, integrator)
 
     def poll(self, t: Ticket, wait: bool = True) -> dict:
         r = ServeResult()
         check(self.ctx.handle, lib().cav_serve_poll(self.handle, t.id, 1 if wait else 0, ctypes.byref(r)))
-        return {k: getattr(r, k) for k, _ in r._fields_}
+        out = {k: getattr(r, k) for k, _ in r._fields_}
+        if out["status"] in (DONE, CANCELLED, FAILED):
+            self._films.pop(t.id, None)  # the server no longer writes it
+            if out["status"] == FAILED:
+                out["message"] = (lib().cav_last_error(self.ctx.handle) or b"").decode(errors="replace")
+        return out
 
     def cancel(self, t: Ticket) -> None:
         check(self.ctx.handle, lib().cav_serve_cancel(self.handle, t.id))
+
+    close_rank = 0  # closed before the scenes it renders (Context.close)
 
     def close(self) -> dict:
         st = ServeStats()
         if self.handle:
             check(None, lib().cav_serve_close(self.handle, ctypes.byref(st)))
             self.handle = None
+            self._films.clear()
         return {k: getattr(st, k) for k, _ in st._fields_}
 
     def __del__(self):
@@ -145,7 +158,8 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
     for t in tickets:
         out = server.poll(t, wait=True)
         if out["status"] != DONE:
-            raise RuntimeError(f"request {t.id} ended with status {out['status']} (error {out['error']})")
+            raise RuntimeError(f"request {t.id} ended with status {out['status']} (error {out['error']}): "
+                               f"{out.get('message', '')}")
         res.latencies_ms.append(out["done_ms"] - out["submit_ms"])
         first_submit = out["submit_ms"] if first_submit is None else min(first_submit, out["submit_ms"])
         last_done = max(last_done, out["done_ms"])

@@ -118,6 +118,19 @@ int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_
 int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
                              const float* limit_dev, int64_t nrays, uint8_t* visible_dev);
 
+/* Traversal health of a BVH's standalone launches (cav_trace_*): out[0] and
+ * out[1] = closest-hit / any-hit rays cut short by the reference's 64-entry
+ * stack (BVH/BVH.h:41,122; results partial), out[2] and out[3] = the deepest
+ * stack any closest-hit / any-hit ray reached. The _async variants only add
+ * to these; the synchronous variants clear out[0] / out[1] before their
+ * launch. reset != 0 clears all four after reading. No counterpart in the
+ * reference, which writes past its stack. */
+int cav_trace_counters(cav_ctx* ctx, const cav_bvh* bvh, uint32_t* out, int32_t reset);
+/* The deepest leaf of the BVH, in edges from the root: no traversal can push
+ * more stack entries than this, so a depth of at most 64 rules out overflow
+ * (BVH/BVH.h:41,122). Computed on the device on each call. */
+int cav_bvh_max_depth(cav_ctx* ctx, const cav_bvh* bvh, int32_t* depth);
+
 /* ---- scenes and rendering ---------------------------------------------- */
 enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_SUBSTRATE = 2, CAV_MAT_METAL = 3, CAV_MAT_GLASS = 4,
        CAV_MAT_PLASTIC = 5 };
@@ -126,9 +139,10 @@ enum { CAV_MAT_MATTE = 0, CAV_MAT_MIRROR = 1, CAV_MAT_SUBSTRATE = 2, CAV_MAT_MET
  * (Integrators/DirectLightingIntegrator.cu: one bounce, NEE without MIS) */
 enum { CAV_INTEGRATOR_PATH = 0, CAV_INTEGRATOR_RLPATH = 1, CAV_INTEGRATOR_DIRECT = 2 };
 /* cav_render_params.flags. SORT_BY_MATERIAL shades each bounce's hits in
- * material order (sortLightingQueue, Integrators/PathTracingIntegrator.cu:50-92
- * and ReinforcementLearningPathTracing.cu:226-272, by material only): the
- * film does not change, only the speed. DETERMINISTIC makes rlpath's table
+ * the order of the reference's sortLightingQueue: by material for path and
+ * directlighting (Integrators/PathTracingIntegrator.cu:50-92), by material
+ * and then Q entry for rlpath (ReinforcementLearningPathTracing.cu:226-272,
+ * stable): the film does not change, only the speed. DETERMINISTIC makes rlpath's table
  * updates (the Q proposals and light statistics that the reference adds with
  * float atomics, ReinforcementLearningPathTracing.cu:453-456,576-585) sum in
  * sample order, so the film is the same on every run; slower (one sort of
@@ -247,6 +261,10 @@ typedef struct {
     int32_t env_background;
 } cav_scene_desc;
 int cav_scene_create_desc(cav_ctx* ctx, const cav_scene_desc* desc, cav_scene** out);
+/* Frees the scene. Refused with CAV_ERR_BUSY while a server still holds it:
+ * a server holds a scene from the submit of a request on it until that
+ * request's final status, and for as long as it keeps a ray pool for it
+ * (until cav_serve_close). Close servers before the scenes they render. */
 int cav_scene_destroy(cav_scene* scene);
 /* the scene's BVH (owned by the scene) */
 const cav_bvh* cav_scene_bvh(const cav_scene* scene);
@@ -400,13 +418,15 @@ typedef struct {
     int64_t pool_closest_rays, pool_shadow_rays, rl_closest_rays, rl_shadow_rays;
     int64_t steps;                    /* bounce steps over all pools */
     int32_t pools;
+    int32_t graph_pools;              /* pools that replayed their bounce steps from graphs */
 } cav_serve_stats;
 
 int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out);
 /* Queues a render of `scene` (created by cav_scene_create on the same
  * device) with the camera and parameters of cav_render. The film buffers
  * (float[h][w][3] sums and float[h][w] weights) must stay valid until poll
- * returns a final status. */
+ * returns a final status for the request or cav_serve_close returns (close
+ * finishes every admitted request and writes its film). */
 int cav_serve_submit(cav_server* s, cav_scene* scene, const cav_camera* camera, const cav_render_params* params,
                      float* film_rgb, float* film_weight, int64_t* id);
 /* Status of a request; wait != 0 blocks until it is final. Returning a final

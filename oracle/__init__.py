@@ -259,15 +259,20 @@ def render_direct(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int
 
 def render_rlpath(scene, spp: int, camera, bvh_rounds: int = 0, morton_bits: int = 30,
                   q_grid: int = 32, alpha_scale: float = 1.0, max_depth: int | None = None,
-                  spp_per_round: int = 0):
+                  spp_per_round: int = 0, bvh=None):
     """`rlpath` integrator on the CPU, bounce by bounce over the samples in
     sample order, in rounds of spp_per_round samples per pixel (0: one round).
+    bvh: node arrays from an earlier lbvh_build + treelet_optimize of the same
+    triangles. The image size is the camera's.
     Returns (film_rgb [H,W,3], film_w [H,W], rays_traced)."""
     tris = np.ascontiguousarray(scene.tris, np.float32)
-    _, _, nd = lbvh_build(tris, morton_bits)
-    if bvh_rounds:
-        treelet_optimize(nd, bvh_rounds)
-    w, h = scene.width, scene.height
+    if bvh is None:
+        _, _, nd = lbvh_build(tris, morton_bits)
+        if bvh_rounds:
+            treelet_optimize(nd, bvh_rounds)
+    else:
+        nd = bvh
+    w, h = int(camera.width), int(camera.height)
     rgb = np.zeros((h, w, 3), np.float32)
     wt = np.zeros((h, w), np.float32)
     rays = np.zeros(1, np.int64)

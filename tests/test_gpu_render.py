@@ -110,7 +110,7 @@ def test_directlighting_matches_oracle(ctx, materials):
 
 @pytest.mark.parametrize("integrator,depth", [("path", 8), ("directlighting", 1), ("rlpath", 1)])
 def test_material_sort_keeps_the_film(ctx, integrator, depth):
-    """Shading hits in material order changes only the speed."""
+    """Shading hits in sortLightingQueue's order changes only the speed."""
     print(f"[render] material sort {integrator}", flush=True)
     sc = scenes.cornell_box(seed=8, width=48, height=40, materials="mixed")
     cam = for_scene(sc)
@@ -120,6 +120,48 @@ def test_material_sort_keeps_the_film(ctx, integrator, depth):
     np.testing.assert_array_equal(a.rgb, b.rgb)
     np.testing.assert_array_equal(a.weight, b.weight)
     assert sa["closest_rays"] == sb["closest_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
+    scene.close()
+
+
+# (width, height, spp): one histogram tile of the material sort (at most
+# 131072 hits), two tiles, and four (the carry across tiles of its scan)
+SORT_SIZES = [(48, 40, 4), (256, 256, 4), (384, 384, 3)]
+
+
+@pytest.mark.parametrize("w,h,spp", SORT_SIZES)
+def test_material_sorted_path_matches_oracle(ctx, w, h, spp):
+    """path with the material-sorted shading order (sortLightingQueue,
+    PathTracingIntegrator.cu:50-92) against the oracle, bit for bit."""
+    print(f"[render] material-sorted path {w}x{h}x{spp}", flush=True)
+    sc = scenes.cornell_box(seed=9, width=w, height=h, materials="mixed")
+    cam = for_scene(sc)
+    rgb, wt, rays = oracle.render_path(sc, spp, cam)
+    scene = R.Scene(ctx, sc)
+    film, st = R.render(ctx, scene, spp, camera=cam, sort_by_material=True)
+    np.testing.assert_array_equal(film.weight, wt)
+    np.testing.assert_array_equal(film.rgb.view(np.uint32), rgb.view(np.uint32))
+    assert st["closest_rays"] + st["shadow_rays"] == rays
+    scene.close()
+
+
+@pytest.mark.parametrize("w,h,spp", SORT_SIZES)
+def test_entry_sorted_rlpath_deterministic(ctx, w, h, spp):
+    """Deterministic rlpath with the (material, Q entry) shading order of
+    the reference's rlpath sortLightingQueue (RL :226-272) equals the
+    unsorted render bit for bit, and the oracle's."""
+    print(f"[rlpath] entry-sorted deterministic {w}x{h}x{spp}", flush=True)
+    sc = scenes.cornell_box(seed=10, width=w, height=h, materials="mixed")
+    cam = for_scene(sc)
+    scene = R.Scene(ctx, sc)
+    a, sa = R.render(ctx, scene, spp, integrator="rlpath", camera=cam, deterministic=True)
+    b, sb = R.render(ctx, scene, spp, integrator="rlpath", camera=cam, deterministic=True, sort_by_material=True)
+    np.testing.assert_array_equal(b.weight, a.weight)
+    np.testing.assert_array_equal(b.rgb.view(np.uint32), a.rgb.view(np.uint32))
+    assert sa["closest_rays"] == sb["closest_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
+    rgb, wt, rays = oracle.render_rlpath(sc, spp, cam)
+    np.testing.assert_array_equal(b.weight, wt)
+    np.testing.assert_array_equal(b.rgb.view(np.uint32), rgb.view(np.uint32))
+    assert sb["closest_rays"] + sb["shadow_rays"] == rays
     scene.close()
 
 

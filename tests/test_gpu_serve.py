@@ -64,6 +64,8 @@ def test_served_films_equal_alone(ctx, two_scenes, graphs):
     outs = [srv.poll(t, wait=True) for t in tickets]
     stats = srv.close()
     assert stats["pools"] == 2 and stats["steps"] > 0
+    if graphs:  # both pools replayed their bounce steps from graphs
+        assert stats["graph_pools"] == 2
     for (s, w, h, spp, integ, depth, shift), t, out in zip(REQUESTS, tickets, outs):
         assert out["status"] == DONE, out
         arr, sc = two_scenes[s]
@@ -115,3 +117,75 @@ def test_cancel_and_errors(ctx, two_scenes):
     with pytest.raises(CavError):  # larger than the arena
         srv.submit(sc, 64, "path", camera=cam_for(arr, 1024, 1024))
     srv.close()
+
+
+def test_scene_destroy_refused_while_served(ctx):
+    """A scene a server still holds (pending request or ray pool) cannot be
+    destroyed; after the server closes it can."""
+    from cavalry_amd._lib import CavError
+    print("[serve] scene lifetime", flush=True)
+    arr = scenes.cornell_box(seed=13, width=32, height=32)
+    sc = R.Scene(ctx, arr)
+    srv = Server(ctx, arena_bytes=1 << 26, host_arena_bytes=1 << 22, pool_tasks=1 << 14)
+    t = srv.submit(sc, 2, "path", camera=cam_for(arr, 32, 32), max_depth=4)
+    with pytest.raises(CavError) as e:
+        sc.close()
+    assert e.value.code == 6  # CAV_ERR_BUSY
+    assert srv.poll(t, wait=True)["status"] == DONE
+    with pytest.raises(CavError):  # the server keeps its ray pool for the scene
+        sc.close()
+    srv.close()
+    sc.close()
+    assert not sc.handle
+
+
+@pytest.fixture(scope="module")
+def interior_pair(ctx):
+    """Two resident procedural interiors (the serving workload's scene
+    family), smaller than the benchmark's and with 256^2 textures."""
+    a = scenes.interior(100_000, seed=3, width=64, height=64, max_depth=16, tex_size=256)
+    b = scenes.interior(250_000, seed=4, width=64, height=64, max_depth=16, tex_size=256)
+    sa, sb = R.Scene(ctx, a, bvh_rounds=3), R.Scene(ctx, b, bvh_rounds=3)
+    yield (a, sa), (b, sb)
+    sa.close()
+    sb.close()
+
+
+INTERIOR_REQUESTS = [  # scene, width, height, spp, integrator, depth, camera shift
+    (0, 96, 64, 4, "path", 16, 0.0),
+    (1, 64, 48, 8, "path", 16, 0.3),
+    (1, 80, 45, 2, "rlpath_det", 16, 0.0),
+    (0, 120, 68, 1, "path", 16, -0.4),
+    (0, 48, 48, 4, "rlpath_det", 12, 0.2),
+    (1, 128, 72, 2, "path", 16, -0.2),
+]
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_interior_served_films_equal_alone(ctx, interior_pair, graphs):
+    """Serving on the procedural interiors: chunked admission into a small
+    pool mixes requests of both scenes and several depths in every launch;
+    every film equals the request rendered alone bit for bit."""
+    print(f"[serve] interior mix invariance, graphs={graphs}", flush=True)
+    srv = Server(ctx, arena_bytes=1 << 29, host_arena_bytes=1 << 24, pool_tasks=40000, chunk_samples=6000,
+                 use_graphs=graphs)
+    tickets = []
+    for s, w, h, spp, integ, depth, shift in INTERIOR_REQUESTS:
+        arr, sc = interior_pair[s]
+        det = integ == "rlpath_det"
+        tickets.append(srv.submit(sc, spp, "rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
+                                  max_depth=depth, deterministic=det))
+    outs = [srv.poll(t, wait=True) for t in tickets]
+    stats = srv.close()
+    print(f"[serve] stats {stats}", flush=True)
+    assert stats["pools"] == 2
+    if graphs:
+        assert stats["graph_pools"] == 2
+    for (s, w, h, spp, integ, depth, shift), t, out in zip(INTERIOR_REQUESTS, tickets, outs):
+        assert out["status"] == DONE, out
+        arr, sc = interior_pair[s]
+        det = integ == "rlpath_det"
+        alone, _ = R.render(ctx, sc, spp, integrator="rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
+                            max_depth=depth, deterministic=det)
+        np.testing.assert_array_equal(t.film.weight, alone.weight)
+        np.testing.assert_array_equal(t.film.rgb, alone.rgb)
