@@ -26,11 +26,11 @@ sys.path.insert(0, ROOT)
 
 # Hardware queues this process may use (read when HIP initialises, so before
 # torch or the library touches the GPU). The server runs one stream per
-# resident scene's pool plus one per rlpath worker (8 here); with HIP's
-# default of 4 queues, streams share queues and their launches serialise
-# (served/plain 1.33x at 4 queues, 1.44x at 8, 1.51x at 16 on the serving
-# trace). The single-stream paths do not depend on it. Set, not defaulted: an
-# environment that already holds HIP's default 4 would otherwise keep it.
+# resident scene's pool (4 here) beside the library's context stream and
+# torch's; with HIP's default of 4 queues, streams share queues and their
+# launches serialise. The single-stream paths do not depend on it. Set, not
+# defaulted: an environment that already holds HIP's default 4 would
+# otherwise keep it.
 os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
@@ -164,15 +164,18 @@ def render_bench(renders: int, cpu: bool, tune_launch: bool = True) -> dict:
     return out
 
 
-def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
+def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = True) -> dict:
     """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
-    triangles, 1024^2 Kd textures on about a third of the meshes): one seeded
-    Poisson trace of render requests (resolution from
-    256^2, 512^2, 1280x720, 1920x1080; spp from 1, 4, 16; path / rlpath
-    50/50; max depth 8) replayed through the plain per-request renderer (one
-    cav_render at a time, in arrival order) and through the server
-    (continuous batching). Latency = arrival -> film ready. Throughput =
-    traced rays / (first arrival -> last film)."""
+    triangles, 1024^2 Kd textures on about a third of the meshes): for each
+    arrival rate, one seeded Poisson trace of render requests (resolution
+    from 256^2, 512^2, 1280x720, 1920x1080; spp from 1, 4, 16, 64; path /
+    rlpath 50/50; max depth 8; the same requests at every rate) replayed
+    through the plain per-request renderer (one cav_render at a time, in
+    arrival order) and through the server (continuous batching; rlpath
+    requests pooled one round at a time; bounce steps replayed from graphs).
+    The plain renderer renders each rlpath request with the server's round
+    size, so both make the same film. Latency = arrival -> film ready.
+    Throughput = traced rays / (first arrival -> last film)."""
     import numpy as np
     from cavalry_amd import Context, render as R, scenes
     from cavalry_amd.camera import look_at
@@ -189,50 +192,54 @@ def serving_bench(n_requests: int, rate: float, seed: int) -> dict:
         at = np.float32([rng.uniform(2, 8), rng.uniform(0.3, 1.5), rng.uniform(4, 9)])
         return look_at(eye, at, (0, 1, 0), 60.0, w, h)
 
-    rng = np.random.default_rng(seed)
-    trace = poisson_trace(rng, n_requests, rate, res_scenes,
-                          [(256, 256), (512, 512), (1280, 720), (1920, 1080)], [1, 4, 16],
-                          ["path", "rlpath"], 8, camera)
-    log(f"[bench] serving: plain per-request renderer, {n_requests} requests at {rate}/s")
-    plain = run_plain(ctx, res_scenes, trace)
-    log("[bench] serving: server (continuous batching)")
-    # 8 rlpath workers (the library's default is 4): rlpath requests render
-    # whole, and with 16 hardware queues 8 of them overlap the 4 pools
-    # (served/plain 1.28x and 1.39x at 4 workers, 1.58x and 1.81x at 8, two
-    # runs each)
-    srv = Server(ctx, arena_bytes=48 << 30, host_arena_bytes=4 << 30, pool_tasks=8 << 20, rl_workers=8)
-    served = run_served(srv, res_scenes, trace)
-    st = srv.close()
-    served.rays += st["pool_closest_rays"] + st["pool_shadow_rays"]
-    same = True
-    worst_rl = 0.0
-    for r, a, b in zip(trace, plain.films, served.films):
-        if r.integrator == "rlpath":
-            # the learnt tables take proposals through float atomics, so an rlpath
-            # film is not bit-reproducible; compared in the display range so a
-            # single firefly does not dominate
-            ia = np.clip(a.image.astype(np.float64), 0, 1)
-            ib = np.clip(b.image.astype(np.float64), 0, 1)
-            worst_rl = max(worst_rl, float(np.mean((ib - ia) ** 2 / (ia * ia + 1e-2))))
-        else:
-            same = same and np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
+    cfg = dict(arena_bytes=64 << 30, host_arena_bytes=4 << 30, pool_tasks=8 << 20, use_graphs=use_graphs)
+    per_rate = []
+    for rate in rates:
+        rng = np.random.default_rng(seed)
+        trace = poisson_trace(rng, n_requests, rate, res_scenes,
+                              [(256, 256), (512, 512), (1280, 720), (1920, 1080)], [1, 4, 16, 64],
+                              ["path", "rlpath"], 8, camera)
+        srv = Server(ctx, **cfg)
+        rl_round = (lambda r, srv=srv: srv.rl_round_spp(r.camera.width * r.camera.height, r.spp))
+        log(f"[bench] serving at {rate}/s: plain per-request renderer, {n_requests} requests")
+        plain = run_plain(ctx, res_scenes, trace, rl_round=rl_round)
+        log(f"[bench] serving at {rate}/s: server (continuous batching)")
+        served = run_served(srv, res_scenes, trace)
+        st = srv.close()
+        served.rays += st["pool_closest_rays"] + st["pool_shadow_rays"]
+        same = True
+        worst_rl = 0.0
+        for r, a, b in zip(trace, plain.films, served.films):
+            if r.integrator == "rlpath":
+                # Q proposals land through float atomics, so an rlpath film is
+                # not bit-reproducible; compared in the display range so a
+                # single firefly does not dominate
+                ia = np.clip(a.image.astype(np.float64), 0, 1)
+                ib = np.clip(b.image.astype(np.float64), 0, 1)
+                worst_rl = max(worst_rl, float(np.mean((ib - ia) ** 2 / (ia * ia + 1e-2))))
+            else:
+                same = same and np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
+        p, s_ = plain.summary(), served.summary()
+        per_rate.append({
+            "rate_per_s": rate, "plain": p, "served": s_,
+            "speedup_mrays": round(s_["mrays_s"] / p["mrays_s"], 3),
+            "path_films_identical": bool(same),
+            "rlpath_max_rel_mse_vs_plain_ldr": worst_rl,
+            "server_steps": st["steps"], "graph_pools": st["graph_pools"],
+        })
+        log(f"[bench] serving at {rate}/s: served/plain {per_rate[-1]['speedup_mrays']}x, "
+            f"p50 {s_['p50_ms']} vs {p['p50_ms']} ms, p99 {s_['p99_ms']} vs {p['p99_ms']} ms")
     for sc in res_scenes:
         sc.close()
     ctx.close()
-    p, s = plain.summary(), served.summary()
     return {
-        "workload": f"{n_requests} requests, Poisson {rate}/s, 4 resident interior scenes "
+        "workload": f"{n_requests} requests per rate, Poisson arrivals, 4 resident interior scenes "
                     f"({', '.join(str(len(a.tris)) for a in arrays)} tris, 4 1024^2 textures each), "
-                    "res 256^2..1920x1080, spp 1/4/16, "
-                    "path/rlpath 50/50, max depth 8",
-        "plain": p,
-        "served": s,
-        "speedup_mrays": round(s["mrays_s"] / p["mrays_s"], 3),
-        "path_films_identical": bool(same),
-        "rlpath_max_rel_mse_vs_plain_ldr": worst_rl,
-        "server_steps": st["steps"],
+                    "res 256^2..1920x1080, spp 1/4/16/64, path/rlpath 50/50, max depth 8",
+        "rates": per_rate,
+        "speedup_mrays_min": min(r["speedup_mrays"] for r in per_rate),
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
-        "rl_workers": 8,
+        "use_graphs": use_graphs,
     }
 
 
@@ -324,8 +331,9 @@ def main() -> int:
     ap.add_argument("--renders", type=int, default=3)
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests per arrival rate")
+    ap.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates (req/s), comma-separated")
+    ap.add_argument("--serve-no-graphs", action="store_true")
     ap.add_argument("--no-equal-time", action="store_true")
     ap.add_argument("--et-tris", type=int, default=2_000_000)
     ap.add_argument("--et-res", default="320x180", help="WxH of the equal-time comparison")
@@ -523,7 +531,8 @@ def main() -> int:
         out["equal_time"] = equal_time_bench(args.seed, args.et_tris, w, h, args.et_ref_spp)
     if not args.no_serve:
         log("[bench] serving")
-        out["serving"] = serving_bench(args.serve_requests, args.serve_rate, args.seed)
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = serving_bench(args.serve_requests, rates, args.seed, not args.serve_no_graphs)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)

@@ -115,56 +115,83 @@ inline M4 look_at_lh(V3 eye, V3 center, V3 up) {
     return r;
 }
 
-// General 4x4 inverse by cofactors; false when singular.
+// 4x4 inverse in glm::inverse's operation order (glm 0.9.8,
+// detail/func_matrix.inl:297-356, which the reference calls at
+// SceneLoading/Parsing.cu:257 and Core/Light.h:58): the 2x2 minors of the
+// lower rows, the cofactors as (a f0 - b f1) + c f2, and the determinant as
+// the first column times the first cofactor row summed pairwise, its
+// reciprocal multiplied into every element. m(c, r) = column c, row r.
+// False when singular.
 inline bool invert(const float* m, float* out) {
-    float inv[16];
-    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
-    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
-    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
-    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
-    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
-    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
-    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
-    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
-    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
-    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
-    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
-    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
-    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
-    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
-    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
-    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
-    const float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    auto at = [&](int c, int r) { return m[c * 4 + r]; };
+    const float c00 = at(2, 2) * at(3, 3) - at(3, 2) * at(2, 3);
+    const float c02 = at(1, 2) * at(3, 3) - at(3, 2) * at(1, 3);
+    const float c03 = at(1, 2) * at(2, 3) - at(2, 2) * at(1, 3);
+    const float c04 = at(2, 1) * at(3, 3) - at(3, 1) * at(2, 3);
+    const float c06 = at(1, 1) * at(3, 3) - at(3, 1) * at(1, 3);
+    const float c07 = at(1, 1) * at(2, 3) - at(2, 1) * at(1, 3);
+    const float c08 = at(2, 1) * at(3, 2) - at(3, 1) * at(2, 2);
+    const float c10 = at(1, 1) * at(3, 2) - at(3, 1) * at(1, 2);
+    const float c11 = at(1, 1) * at(2, 2) - at(2, 1) * at(1, 2);
+    const float c12 = at(2, 0) * at(3, 3) - at(3, 0) * at(2, 3);
+    const float c14 = at(1, 0) * at(3, 3) - at(3, 0) * at(1, 3);
+    const float c15 = at(1, 0) * at(2, 3) - at(2, 0) * at(1, 3);
+    const float c16 = at(2, 0) * at(3, 2) - at(3, 0) * at(2, 2);
+    const float c18 = at(1, 0) * at(3, 2) - at(3, 0) * at(1, 2);
+    const float c19 = at(1, 0) * at(2, 2) - at(2, 0) * at(1, 2);
+    const float c20 = at(2, 0) * at(3, 1) - at(3, 0) * at(2, 1);
+    const float c22 = at(1, 0) * at(3, 1) - at(3, 0) * at(1, 1);
+    const float c23 = at(1, 0) * at(2, 1) - at(2, 0) * at(1, 1);
+    const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
+    const float f3[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+    const float v0[4] = {at(1, 0), at(0, 0), at(0, 0), at(0, 0)};
+    const float v1[4] = {at(1, 1), at(0, 1), at(0, 1), at(0, 1)};
+    const float v2[4] = {at(1, 2), at(0, 2), at(0, 2), at(0, 2)};
+    const float v3[4] = {at(1, 3), at(0, 3), at(0, 3), at(0, 3)};
+    float inv[4][4];  // inv[column][row]
+    for (int k = 0; k < 4; ++k) {
+        const float sa = (k & 1) ? -1.f : 1.f;  // (+1, -1, +1, -1) on columns 0 and 2
+        const float sb = -sa;                   // (-1, +1, -1, +1) on columns 1 and 3
+        inv[0][k] = ((v1[k] * f0[k] - v2[k] * f1[k]) + v3[k] * f2[k]) * sa;
+        inv[1][k] = ((v0[k] * f0[k] - v2[k] * f3[k]) + v3[k] * f4[k]) * sb;
+        inv[2][k] = ((v0[k] * f1[k] - v1[k] * f3[k]) + v3[k] * f5[k]) * sa;
+        inv[3][k] = ((v0[k] * f2[k] - v1[k] * f4[k]) + v2[k] * f5[k]) * sb;
+    }
+    const float d0 = at(0, 0) * inv[0][0], d1 = at(0, 1) * inv[1][0];
+    const float d2 = at(0, 2) * inv[2][0], d3 = at(0, 3) * inv[3][0];
+    const float det = (d0 + d1) + (d2 + d3);
     if (det == 0 || !std::isfinite(det)) return false;
     const float r = 1.0f / det;
-    for (int i = 0; i < 16; ++i) out[i] = inv[i] * r;
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 4; ++k) out[c * 4 + k] = inv[c][k] * r;
     return true;
 }
 
 // transpose(inverse(upper 3x3 of m)) (getTransformForNormal,
-// Utils/MathsCommons.h:80-82), as a row-major 3x3 r[row][col]; false when
-// singular
+// Utils/MathsCommons.h:80-82), as a row-major 3x3 r[row][col], with glm's
+// 3x3 inverse (detail/func_matrix.inl:272-292): the determinant by the first
+// column's cofactors, its reciprocal multiplied into each cofactor. False
+// when singular.
 inline bool normal_matrix(const M4& m, float r[3][3]) {
-    // a[row][col] of the upper 3x3
-    float a[3][3];
-    for (int row = 0; row < 3; ++row)
-        for (int col = 0; col < 3; ++col) a[row][col] = m.at(col, row);
-    const float c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
-    const float c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
-    const float c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
-    const float det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+    auto g = [&](int c, int w) { return m.at(c, w); };  // glm m[c][w] of the upper 3x3
+    const float det = (+g(0, 0) * (g(1, 1) * g(2, 2) - g(2, 1) * g(1, 2)) -
+                       g(1, 0) * (g(0, 1) * g(2, 2) - g(2, 1) * g(0, 2))) +
+                      g(2, 0) * (g(0, 1) * g(1, 2) - g(1, 1) * g(0, 2));
     if (det == 0 || !std::isfinite(det)) return false;
-    const float inv = 1.0f / det;
-    // inverse(a)[i][j] = cofactor[j][i] / det, so its transpose is cofactor[i][j] / det
-    r[0][0] = c00 * inv;
-    r[0][1] = c01 * inv;
-    r[0][2] = c02 * inv;
-    r[1][0] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * inv;
-    r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * inv;
-    r[1][2] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * inv;
-    r[2][0] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * inv;
-    r[2][1] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * inv;
-    r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * inv;
+    const float o = 1.0f / det;
+    float inv[3][3];  // inv[column][row], as glm's Inverse[c][w]
+    inv[0][0] = +(g(1, 1) * g(2, 2) - g(2, 1) * g(1, 2)) * o;
+    inv[1][0] = -(g(1, 0) * g(2, 2) - g(2, 0) * g(1, 2)) * o;
+    inv[2][0] = +(g(1, 0) * g(2, 1) - g(2, 0) * g(1, 1)) * o;
+    inv[0][1] = -(g(0, 1) * g(2, 2) - g(2, 1) * g(0, 2)) * o;
+    inv[1][1] = +(g(0, 0) * g(2, 2) - g(2, 0) * g(0, 2)) * o;
+    inv[2][1] = -(g(0, 0) * g(2, 1) - g(2, 0) * g(0, 1)) * o;
+    inv[0][2] = +(g(0, 1) * g(1, 2) - g(1, 1) * g(0, 2)) * o;
+    inv[1][2] = -(g(0, 0) * g(1, 2) - g(1, 0) * g(0, 2)) * o;
+    inv[2][2] = +(g(0, 0) * g(1, 1) - g(1, 0) * g(0, 1)) * o;
+    // the transpose, row-major: r[row][col] = transpose[col][row] = inv[row][col]
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) r[row][col] = inv[row][col];
     return true;
 }
 

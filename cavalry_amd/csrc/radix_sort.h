@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "launch.h"
+
 namespace cav {
 
 template <typename K>
@@ -23,5 +25,7 @@ hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, h
                             Marker* mk);
 hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
                             Marker* mk);
+// launched through a sink (a stream or a graph under construction)
+hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, LaunchSink& ls);
 
 }  // namespace cav

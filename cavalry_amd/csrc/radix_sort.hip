@@ -228,35 +228,38 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
     }
 }
 
+__global__ void k_zero(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// Launches go through ls (a stream, or a graph under construction); mk marks
+// stream launches for the build's per-kernel timings.
 template <typename K>
-hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, hipStream_t s, Marker* mk) {
+hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, LaunchSink& ls, Marker* mk) {
     const int passes = (key_bits + 7) / 8;
     const int tiles = (int)((n + kTile - 1) / kTile);
-    hipError_t e;
-    if ((e = hipMemsetAsync(b.hist, 0, sizeof(uint32_t) * passes * kRadix, s)) != hipSuccess) return e;
     int hblocks = (int)((n + kBlock * 16 - 1) / (kBlock * 16));
     if (hblocks > 2048) hblocks = 2048;
     if (hblocks < 1) hblocks = 1;
+    ls.launch("k_zero", &k_zero, dim3(1), dim3(kBlock), b.hist, passes * kRadix);
     mark(mk, "sort_clear");
-    hipLaunchKernelGGL(k_hist<K>, dim3(hblocks), dim3(kBlock), 0, s, b.keys[0], n, b.n_dev, passes, b.hist);
+    ls.launch("k_hist", &k_hist<K>, dim3(hblocks), dim3(kBlock), b.keys[0], n, b.n_dev, passes, b.hist);
     mark(mk, "sort_hist");
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_upsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur], n, b.n_dev, shift,
-                           tiles, b.counts);
+        ls.launch("k_upsweep", &k_upsweep<K>, dim3(tiles), dim3(kBlock), b.keys[cur], n, b.n_dev, shift, tiles,
+                  b.counts);
         mark(mk, "sort_upsweep");
-        hipLaunchKernelGGL(k_scan, dim3(kRadix), dim3(kBlock), 0, s, b.counts, tiles,
-                           b.hist + p * kRadix, b.offsets);
+        ls.launch("k_scan", &k_scan, dim3(kRadix), dim3(kBlock), b.counts, tiles, b.hist + p * kRadix, b.offsets);
         mark(mk, "sort_scan");
-        hipLaunchKernelGGL(k_downsweep<K>, dim3(tiles), dim3(kBlock), 0, s, b.keys[cur],
-                           b.vals[cur], b.keys[cur ^ 1], b.vals[cur ^ 1], n, b.n_dev, shift, tiles,
-                           b.offsets);
+        ls.launch("k_downsweep", &k_downsweep<K>, dim3(tiles), dim3(kBlock), b.keys[cur], b.vals[cur],
+                  b.keys[cur ^ 1], b.vals[cur ^ 1], n, b.n_dev, shift, tiles, b.offsets);
         mark(mk, "sort_downsweep");
         cur ^= 1;
     }
     b.result = cur;
-    return hipGetLastError();
+    return ls.err;
 }
 
 }  // namespace
@@ -266,11 +269,16 @@ int sort_radix() { return kRadix; }
 
 hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
                             Marker* mk) {
-    return sort_impl<uint32_t>(b, n, key_bits, s, mk);
+    LaunchSink ls(s);
+    return sort_impl<uint32_t>(b, n, key_bits, ls, mk);
 }
 hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
                             Marker* mk) {
-    return sort_impl<uint64_t>(b, n, key_bits, s, mk);
+    LaunchSink ls(s);
+    return sort_impl<uint64_t>(b, n, key_bits, ls, mk);
+}
+hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, LaunchSink& ls) {
+    return sort_impl<uint64_t>(b, n, key_bits, ls, nullptr);
 }
 
 }  // namespace cav

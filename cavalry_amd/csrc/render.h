@@ -130,6 +130,51 @@ struct RlSlots {
     int32_t* ncell;   // Q-sampled bin, -1 when the distribution was empty
 };
 
+// Q table and light statistics of one rlpath render (E = entries, L = lights)
+struct RlTables {
+    float* Q;        // [E][128]
+    float* newQ;     // [E][128]
+    int32_t* cnt;    // [E][128] proposals this bounce
+    int32_t* total;  // [E][128]
+    float* ldist;    // [E][L] light-selection CDF
+    float* lres;     // [E][L]
+    float* lnew;     // [E][L]
+    int32_t* lcount; // [E][L]
+    int32_t* lnewc;  // [E][L]
+};
+
+// findQEntry's grid (ReinforcementLearningPathTracing.cu:155-165): g entries
+// per axis over the scene box
+struct Grid {
+    float lo[3], size[3];
+    int g;
+};
+
+// Deterministic table updates of a bounce as (key, value) records
+struct RlRecords {
+    uint64_t* keys;
+    float* val;
+    uint32_t* count;
+    uint32_t* idx;  // record number, the sort's value
+};
+
+// Everything one rlpath request's tasks read and update besides their own
+// queue slots: its tables and grid, its per-sample results, its
+// syncDimension words ([0] the bounce's synced dimension, [1] whether any
+// task sampled a Q bin), its learning-rate scale, its index in the top bits
+// of deterministic record keys and whether it uses them. A render has one; a server pool one
+// per request slot (t.Q == null for slots that are not rlpath requests).
+struct RlReqState {
+    RlTables t;
+    Grid gr;
+    float* res;
+    int32_t* dims;
+    float alpha_scale;
+    uint32_t rlx;
+    int max_depth;  // continuation rays of the last bounce are not queued
+    int det;        // updates go to the sorted records (CAV_RENDER_DETERMINISTIC), else float atomics
+};
+
 struct RlArgs {
     int grid;           // Q entries per axis
     float alpha_scale;  // 1: the reference's learning rate n/(T+n)

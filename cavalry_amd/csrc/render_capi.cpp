@@ -288,6 +288,7 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
 
 namespace cav {
 void scene_hold(cav_scene* sc, int delta) { sc->users.fetch_add(delta); }
+void scene_bounds(const cav_scene* sc, float out[6]) { std::memcpy(out, sc->bounds, sizeof sc->bounds); }
 
 SceneDev scene_dev(const cav_scene* sc) {
     return SceneDev{sc->bvh->rec,  sc->bvh->root, sc->ntris,   sc->soup,

@@ -34,6 +34,7 @@
 #include "dev_common.h"
 #include "radix_sort.h"
 #include "render.h"
+#include "serve.h"
 #include "shade.h"
 
 namespace cav {
@@ -85,10 +86,6 @@ __device__ inline int dir_to_cell(v3 d) {
 }
 
 // findQEntry (:155-165)
-struct Grid {
-    float lo[3], size[3];
-    int g;
-};
 __device__ inline int q_entry(const Grid& gr, v3 p) {
     const float gf = (float)gr.g;
     const float ex = gr.size[0] / gf, ey = gr.size[1] / gf, ez = gr.size[2] / gf;
@@ -115,18 +112,30 @@ __device__ inline int dist_index(const float* cdf, int n, float f) {
 }
 
 // -------------------------------------------------------------- state
-struct RlTables {
-    float* Q;        // [E][128]
-    float* newQ;     // [E][128]
-    int32_t* cnt;    // [E][128] proposals this bounce
-    int32_t* total;  // [E][128]
-    float* ldist;    // [E][L] light-selection CDF
-    float* lres;     // [E][L]
-    float* lnew;     // [E][L]
-    int32_t* lcount; // [E][L]
-    int32_t* lnewc;  // [E][L]
+// Where a task's request state comes from. RenderSrc: a render, one request
+// for every task. PoolSrc: a server pool, the task's own request slot; tasks
+// of other integrators share the queues and are skipped.
+struct RenderSrc {
+    RlReqState s;
+    int depth_;
+    __device__ bool take(int64_t) const { return true; }
+    __device__ const RlReqState& get(int64_t) const { return s; }
+    __device__ int depth(int64_t) const { return depth_; }
+    __device__ int slot(int64_t) const { return 0; }
+    __device__ void tag_shadow(uint32_t, int64_t) const {}
 };
-
+struct PoolSrc {
+    const RlReqState* rl;  // per request slot
+    const int32_t* rid;    // request slot of each task of the input queue
+    const uint8_t* dep;    // bounce index of each task
+    int32_t* sh_rid;       // request slot of each shadow task
+    uint8_t* live;         // per request slot: it had a hit this bounce
+    __device__ bool take(int64_t i) const { return rl[rid[i]].t.Q != nullptr; }
+    __device__ const RlReqState& get(int64_t i) const { return rl[rid[i]]; }
+    __device__ int depth(int64_t i) const { return dep[i]; }
+    __device__ int slot(int64_t i) const { return rid[i]; }
+    __device__ void tag_shadow(uint32_t ss, int64_t i) const { sh_rid[ss] = rid[i]; }
+};
 __global__ __launch_bounds__(kBlock) void k_init_tables(RlTables t, int64_t ncells, int64_t nent, int L) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < ncells) {  // QEntry() (:36-41) and prepareForUpdateQ
@@ -178,22 +187,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_dim_max(const uint32_t* counts, i
     }
 }
 
+// The same per pool task into its own request's word; marks the request as
+// having hits this bounce.
+__global__ __launch_bounds__(kBlock) void k_pool_dim_max(PoolSrc src, const uint32_t* counts, int qin, RlQueue q,
+                                                         const int32_t* __restrict__ hit_prim) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)counts[qin] || hit_prim[i] < 0 || !src.take(i)) return;
+    int32_t* d = src.get(i).dims;
+    const int v = q.dim[i];
+    if (v > __hip_atomic_load(&d[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&d[0], v);
+    const int r = src.rid[i];
+    if (!src.live[r]) src.live[r] = 1;
+}
+
 // computeLighting (:388-460) up to the visibility test: emission with MIS,
 // learnt light choice, point on the light. The shadow task carries the slot.
-__global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, HaltonDev h, Grid gr,
-                                                     Queue qi, RlSlots sl, Shadow sh, RlTables t,
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_lighting(Src src, SceneDev sc, HaltonDev h,
+                                                     Queue qi, RlSlots sl, Shadow sh,
                                                      const int32_t* __restrict__ hit_prim,
                                                      const float* __restrict__ hit_t,
                                                      const float* __restrict__ hit_u,
-                                                     const float* __restrict__ hit_v, float* __restrict__ res,
-                                                     uint32_t* counts, const int32_t* dims, int qin,
+                                                     const float* __restrict__ hit_v,
+                                                     uint32_t* counts, int qin,
                                                      const uint32_t* __restrict__ perm) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = j < (int64_t)counts[qin];
     const int64_t i = live ? task_slot(perm, j) : 0;
+    live = live && src.take(i);
     const int prim = live ? hit_prim[i] : -1;
+    const RlReqState* R = live ? &src.get(i) : nullptr;
     if (live && prim < 0 && sc.env_miss >= 0) {  // intersectScene's miss (:292-296)
-        float* acc = res + (int64_t)qi.sid[i] * 3;
+        float* acc = R->res + (int64_t)qi.sid[i] * 3;
         const v3 e = mul3(miss_radiance(sc, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5])),
                           mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]));
         acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
@@ -203,7 +228,9 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
     float lim = 0.f;
     int mesh = 0, dst = 0;
     if (live) {
-        const int D = dims[0];
+        const int D = R->dims[0];
+        const int depth = src.depth(i);
+        float* res = R->res;
         const uint32_t sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
@@ -228,8 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
             }
             res[sid * 3] = acc.x; res[sid * 3 + 1] = acc.y; res[sid * 3 + 2] = acc.z;
         }
-        const int entry = q_entry(gr, pos);
-        const float* cdf = t.ldist + (int64_t)entry * sc.nlights;
+        const int entry = q_entry(R->gr, pos);
+        const float* cdf = R->t.ldist + (int64_t)entry * sc.nlights;
         const int li = dist_index(cdf, sc.nlights, halton(h, sid, D));
         const float sel = li == 0 ? cdf[0] : cdf[li] - cdf[li - 1];
         const LightPick L = sample_light(sc, h, sid, D + 5, pos, sc.lights[li]);
@@ -252,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_lighting(int depth, SceneDev sc, Hal
         r[0] = pos.x; r[1] = pos.y; r[2] = pos.z; r[3] = dir.x; r[4] = dir.y; r[5] = dir.z;
         sh.limit[ss] = lim; sh.src[ss] = mesh; sh.dst[ss] = dst;
         sl.sh_idx[i] = (int32_t)ss;
+        src.tag_shadow(ss, i);
     }
 }
 
@@ -322,27 +350,30 @@ struct BinEval {
 // 320x180x256 spp on 2M triangles: 265 -> 256 ms). Float sums are
 // order-dependent either way, as in the reference. Call with the whole wave
 // converged.
-__device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool on) {
+// fsum / cnt point at this lane's words; lanes with the same words (the same
+// table and index) are summed first.
+__device__ inline void wave_add(float* fsum, int* cnt, float v, bool on) {
     const int lane = threadIdx.x & 63;
     uint64_t todo = __ballot(on);
+    const unsigned long long key = reinterpret_cast<unsigned long long>(fsum);
     while (todo) {
         const int lead = __ffsll((unsigned long long)todo) - 1;
-        const int64_t lk = __shfl(k, lead, 64);
-        const bool mine = ((todo >> lane) & 1ull) && k == lk;
+        const unsigned long long lk = __shfl(key, lead, 64);
+        const bool mine = ((todo >> lane) & 1ull) && key == lk;
         const uint64_t grp = __ballot(mine);
         if (grp == (1ull << lead)) break;
         float sv = mine ? v : 0.f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o, 64);
         if (lane == lead) {
-            atomicAdd(fsum + lk, sv);
-            atomicAdd(cnt + lk, (int)__popcll(grp));
+            atomicAdd(fsum, sv);
+            atomicAdd(cnt, (int)__popcll(grp));
         }
         todo &= ~grp;
     }
     if ((todo >> lane) & 1ull) {
-        atomicAdd(fsum + k, v);
-        atomicAdd(cnt + k, 1);
+        atomicAdd(fsum, v);
+        atomicAdd(cnt, 1);
     }
 }
 
@@ -351,16 +382,15 @@ __device__ inline void wave_add(float* fsum, int* cnt, int64_t k, float v, bool 
 // sample, 0 = the kind (0: the previous bin's proposal, 1: the direct-light
 // proposal), the order in which the oracle, which runs the tasks one by one
 // in sample order, adds them.
-struct RlRecords {
-    uint64_t* keys;
-    float* val;
-    uint32_t* count;
-};
 constexpr int kRecKeyShift = 33;
 constexpr int kRecKeyBits = 25;
 
-__device__ inline uint64_t rec_key(bool light, int64_t k, uint32_t sid, int kind) {
-    return (light ? (1ull << 58) : 0ull) | ((uint64_t)k << kRecKeyShift) | ((uint64_t)sid << 1) | (uint64_t)kind;
+// bits 59..63: the request's index (rlx; 0 in a render, its slot in a pool)
+constexpr int kRecReqShift = 59;
+constexpr int kRecMaxReqs = 32;
+__device__ inline uint64_t rec_key(uint32_t rlx, bool light, int64_t k, uint32_t sid, int kind) {
+    return ((uint64_t)rlx << kRecReqShift) | (light ? (1ull << 58) : 0ull) | ((uint64_t)k << kRecKeyShift) |
+           ((uint64_t)sid << 1) | (uint64_t)kind;
 }
 // append (key, v) for every lane with `on`; call with the whole wave converged
 __device__ inline void rec_push(const RlRecords& r, uint64_t key, float v, bool on) {
@@ -375,25 +405,34 @@ __device__ inline void rec_push(const RlRecords& r, uint64_t key, float v, bool 
         const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         r.keys[slot] = key;
         r.val[slot] = v;
+        r.idx[slot] = slot;
     }
 }
 
 // The end of computeLighting (after the visibility test), then
 // computeQDistributions (:517-606) and accumulateLightingContribs (:609-647).
-__global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
-                                                  RlSlots sl, RlTables t, const uint8_t* __restrict__ vis,
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_qdist(Src src, SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
+                                                  RlSlots sl, const uint8_t* __restrict__ vis,
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t,
                                                   const float* __restrict__ hit_u,
-                                                  const float* __restrict__ hit_v, float* __restrict__ res,
-                                                  const uint32_t* counts, int32_t* dims, int qin,
+                                                  const float* __restrict__ hit_v,
+                                                  const uint32_t* counts, int qin,
                                                   const uint32_t* __restrict__ perm, RlRecords rec) {
     __shared__ float trig[2 * kNX][kBlock];
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = j < (int64_t)counts[qin];
     const int64_t i = live ? task_slot(perm, j) : 0;
+    live = live && src.take(i);
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
+    const RlReqState* R = live ? &src.get(i) : nullptr;
+    float *up_fl = nullptr, *up_fp = nullptr, *up_fd = nullptr;  // the words the updates go to (atomics)
+    int32_t *up_cl = nullptr, *up_cp = nullptr, *up_cd = nullptr;
+    uint32_t rlx = 0;
+    bool use_rec = false;  // this task's request sums its updates from records
+    int32_t* dims = nullptr;
     bool valid = false;
     // the table updates of this task, made after the divergent part
     bool up_l = false, up_p = false, up_d = false;
@@ -401,6 +440,11 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
     float v_l = 0.f, v_p = 0.f, v_d = 0.f;
     uint32_t sid_up = 0;
     if (live) {
+        const RlTables& t = R->t;
+        float* res = R->res;
+        dims = R->dims;
+        rlx = R->rlx;
+        use_rec = R->det != 0;
         const int D = dims[0] + 5 + sc.light_dims;
         const uint32_t sid = qi.sid[i];
         sid_up = sid;
@@ -431,6 +475,8 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         up_l = true;
         k_l = (int64_t)entry * sc.nlights + li;
         v_l = sampled;
+        up_fl = t.lnew + k_l;
+        up_cl = t.lnewc + k_l;
 
         // 128-bin distribution of |cos| * luminance(f) * Q at one shared offset.
         // Bin c = 16 * theta_row + phi_column. The running (unnormalised) sum
@@ -482,6 +528,8 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
             up_p = true;
             k_p = (int64_t)prev * kXY + rq.prev_cell[i];
             v_p = sumw + sl.emis[i];
+            up_fp = t.newQ + k_p;
+            up_cp = t.cnt + k_p;
         }
         // sample a bin: Distribution1D::sample on cdf[c] = prefix[c] / sum
         int cell = 0;
@@ -556,6 +604,8 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
                 up_d = true;
                 k_d = (int64_t)entry * kXY + lc;
                 v_d = prop;
+                up_fd = t.newQ + k_d;
+                up_cd = t.cnt + k_d;
             }
         }
         if (sumw != 0) {
@@ -580,17 +630,17 @@ __global__ __launch_bounds__(kBlock) void k_qdist(SceneDev sc, HaltonDev h, Queu
         res[sid * 3 + 1] = res[sid * 3 + 1] + add.y;
         res[sid * 3 + 2] = res[sid * 3 + 2] + add.z;
     }
+    // (both branches run for the whole wave: a pool wave may hold tasks of
+    // deterministic and of atomic requests)
     if (rec.keys) {
-        rec_push(rec, rec_key(true, k_l, sid_up, 0), v_l, up_l);
-        rec_push(rec, rec_key(false, k_p, sid_up, 0), v_p, up_p);
-        rec_push(rec, rec_key(false, k_d, sid_up, 1), v_d, up_d);
-    } else {
-        wave_add(t.lnew, t.lnewc, k_l, v_l, up_l);
-        wave_add(t.newQ, t.cnt, k_p, v_p, up_p);
-        wave_add(t.newQ, t.cnt, k_d, v_d, up_d);
+        rec_push(rec, rec_key(rlx, true, k_l, sid_up, 0), v_l, up_l && use_rec);
+        rec_push(rec, rec_key(rlx, false, k_p, sid_up, 0), v_p, up_p && use_rec);
+        rec_push(rec, rec_key(rlx, false, k_d, sid_up, 1), v_d, up_d && use_rec);
     }
-    if (__ballot(valid) != 0 && (threadIdx.x & 63) == 0 &&
-        __hip_atomic_load(&dims[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    wave_add(up_fl, up_cl, v_l, up_l && !use_rec);
+    wave_add(up_fp, up_cp, v_p, up_p && !use_rec);
+    wave_add(up_fd, up_cd, v_d, up_d && !use_rec);
+    if (valid && __hip_atomic_load(&dims[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
         atomicMax(&dims[1], 1);
 }
 
@@ -602,10 +652,8 @@ __device__ inline float lane_f(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-__global__ __launch_bounds__(64 * kEntryWaves) void k_finish_q(RlTables t, int64_t nent, float alpha_scale) {
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t e = (int64_t)blockIdx.x * kEntryWaves + wv;
-    if (e >= nent) return;
+__device__ inline void finish_q_entry(const RlTables& t, int64_t e, float alpha_scale) {
+    const int lane = threadIdx.x & 63;
     const int64_t base = e * kXY;
     const int c0 = lane, c1 = lane + 64;
     const int n0 = t.cnt[base + c0], n1 = t.cnt[base + c1];
@@ -650,11 +698,28 @@ __global__ __launch_bounds__(64 * kEntryWaves) void k_finish_q(RlTables t, int64
     t.Q[base + c1] = q1;
 }
 
+__global__ __launch_bounds__(64 * kEntryWaves) void k_finish_q(RlTables t, int64_t nent, float alpha_scale) {
+    const int64_t e = (int64_t)blockIdx.x * kEntryWaves + (threadIdx.x >> 6);
+    if (e < nent) finish_q_entry(t, e, alpha_scale);
+}
+
+// The same over the tables of every pool request with hits this bounce
+// (live[slot]), one entry per wave on a fixed grid.
+__global__ __launch_bounds__(64 * kEntryWaves) void k_pool_finish_q(const RlReqState* __restrict__ rl,
+                                                                    const uint8_t* __restrict__ live, int nslots) {
+    const int64_t w0 = (int64_t)blockIdx.x * kEntryWaves + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * kEntryWaves;
+    for (int sl = 0; sl < nslots; ++sl) {
+        if (!live[sl]) continue;
+        const RlReqState& R = rl[sl];
+        const int64_t nent = (int64_t)R.gr.g * R.gr.g * R.gr.g;
+        for (int64_t e = w0; e < nent; e += nw) finish_q_entry(R.t, e, R.alpha_scale);
+    }
+}
+
 // finishLightingTraining (:463-514) per entry, clearing the new statistics
 // (prepareLightingTraining).
-__global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nent, int L) {
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= nent) return;
+__device__ inline void finish_light_entry(const RlTables& t, int64_t e, int L) {
     float* res = t.lres + e * L;
     float* nres = t.lnew + e * L;
     int32_t* cnt = t.lcount + e * L;
@@ -690,19 +755,39 @@ __global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nen
     for (int k = 0; k < L; ++k) cdf[k] = cdf[k] / dens;
 }
 
+__global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nent, int L) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < nent) finish_light_entry(t, e, L);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pool_finish_light(const RlReqState* __restrict__ rl,
+                                                              const uint8_t* __restrict__ live, int nslots, int L) {
+    const int64_t t0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nt = (int64_t)gridDim.x * kBlock;
+    for (int sl = 0; sl < nslots; ++sl) {
+        if (!live[sl]) continue;
+        const RlReqState& R = rl[sl];
+        const int64_t nent = (int64_t)R.gr.g * R.gr.g * R.gr.g;
+        for (int64_t e = t0; e < nent; e += nt) finish_light_entry(R.t, e, L);
+    }
+}
+
 // genNextRay (:305-375) after the second syncDimension.
-__global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonDev h, Queue qi, Queue qo,
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_next(Src src, SceneDev sc, HaltonDev h, Queue qi, Queue qo,
                                                  RlQueue rqo, RlSlots sl, const int32_t* __restrict__ hit_prim,
                                                  const float* __restrict__ hit_t,
                                                  const float* __restrict__ hit_u,
                                                  const float* __restrict__ hit_v, uint32_t* counts,
-                                                 const int32_t* dims, int qin, int qout,
-                                                 const uint32_t* __restrict__ perm) {
+                                                 int qin, int qout, const uint32_t* __restrict__ perm,
+                                                 int32_t* __restrict__ rid_out, uint8_t* __restrict__ dep_out) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = j < (int64_t)counts[qin];
     const int64_t i = live ? task_slot(perm, j) : 0;
+    live = live && src.take(i);
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
+    int depth = 0, rid = 0;
     bool push = false;
     uint32_t sid = 0;
     v3 nro = mk3(0, 0, 0), nrd = nro, mult = nro;
@@ -710,6 +795,10 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
     int cell = 0, entry = 0, dim = 0;
     bool ndelta = false;
     if (live) {
+        const RlReqState& R = src.get(i);
+        const int32_t* dims = R.dims;
+        depth = src.depth(i);
+        rid = src.slot(i);
         dim = dims[0] + 7 + sc.light_dims + dims[1];
         sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
@@ -757,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
             if (zero3(f)) mult = mk3(0, 0, 0);
             else mult = divs3(scale3(mul3(mult, f), fabsf(dot3(nrd, nn))), npdf);
             ndelta = bsdf.delta();
-            push = true;
+            push = depth + 1 < R.max_depth;  // the last bounce's rays are never traced
         }
     }
     const uint32_t ns = block_push<kBlock>(push, &counts[qout]);
@@ -773,6 +862,10 @@ __global__ __launch_bounds__(kBlock) void k_next(int depth, SceneDev sc, HaltonD
         rqo.prev_entry[ns] = entry;
         rqo.prev_cell[ns] = cell;
         rqo.dim[ns] = dim;
+        if (rid_out) {
+            rid_out[ns] = rid;
+            dep_out[ns] = (uint8_t)(depth + 1);
+        }
     }
 }
 
@@ -785,6 +878,17 @@ __global__ void k_step(uint32_t* counts, unsigned long long* rays, int32_t* dims
         dims[0] = 0;
         dims[1] = 0;
     }
+}
+
+// A pool bounce's end for its rlpath requests: clear the syncDimension words
+// of the requests that had hits, and their marks.
+__global__ void k_pool_rl_step(const RlReqState* __restrict__ rl, uint8_t* live, int nslots) {
+    for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x)
+        if (live[sl]) {
+            rl[sl].dims[0] = 0;
+            rl[sl].dims[1] = 0;
+            live[sl] = 0;
+        }
 }
 
 // sortLightingQueue's key (:226-272): material * entries + Q entry of the hit
@@ -809,44 +913,68 @@ __global__ __launch_bounds__(kBlock) void k_entry_keys(SceneDev sc, Grid gr, Que
     vals[i] = (uint32_t)i;
 }
 
-// Deterministic mode, before k_qdist: every key slot empty (all ones, sorted
-// last), values numbered in slot order, no records.
-__global__ __launch_bounds__(kBlock) void k_rec_reset(uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
-                                                      uint32_t* count, int64_t cap) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) *count = 0u;
-    if (i < cap) {
-        keys[i] = ~0ull;
-        idx[i] = (uint32_t)i;
-    }
+// Deterministic mode, before k_qdist: no records yet (k_qdist appends them;
+// the sort takes only the appended ones).
+__global__ void k_rec_reset(uint32_t* count) {
+    if (threadIdx.x == 0) *count = 0u;
 }
+
+// The tables a record's request index selects.
+struct RenderTables {
+    RlTables t;
+    __device__ const RlTables& of(uint32_t) const { return t; }
+};
+struct PoolTables {
+    const RlReqState* rl;
+    __device__ const RlTables& of(uint32_t x) const { return rl[x].t; }
+};
 
 // After the sort: the first record of each key sums its run in order, from 0
 // as the oracle's += does, and sets the table word and its count (the tables
 // hold zeros there: k_finish_q / k_finish_light cleared them).
+template <class Tab>
 __global__ __launch_bounds__(kBlock) void k_rec_sum(const uint64_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ idx,
-                                                    const float* __restrict__ val, int64_t cap, RlTables t) {
+                                                    const float* __restrict__ val, const uint32_t* count,
+                                                    int64_t cap, Tab tab) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= cap) return;
+    const int64_t n = (int64_t)*count < cap ? (int64_t)*count : cap;
+    if (i >= n) return;
     const uint64_t k = keys[i];
-    if (k == ~0ull) return;
     const uint64_t seg = k >> kRecKeyShift;
     if (i > 0 && (keys[i - 1] >> kRecKeyShift) == seg) return;
     float sum = 0.f;
-    int32_t n = 0;
-    for (int64_t j = i; j < cap && (keys[j] >> kRecKeyShift) == seg; ++j) {
+    int32_t c = 0;
+    for (int64_t j = i; j < n && (keys[j] >> kRecKeyShift) == seg; ++j) {
         sum += val[idx[j]];
-        ++n;
+        ++c;
     }
+    const RlTables& t = tab.of((uint32_t)(k >> kRecReqShift));
     const int64_t key = (int64_t)(seg & ((1ull << kRecKeyBits) - 1ull));
-    if (seg >> kRecKeyBits) {
+    if ((seg >> kRecKeyBits) & 1ull) {
         t.lnew[key] = sum;
-        t.lnewc[key] = n;
+        t.lnewc[key] = c;
     } else {
         t.newQ[key] = sum;
-        t.cnt[key] = n;
+        t.cnt[key] = c;
     }
+}
+
+// Sorts a bounce's records by key (only the appended ones) and sums them.
+template <class Tab>
+hipError_t rec_sort_sum(uint64_t* const keys[2], uint32_t* const idx[2], float* val, uint32_t* count,
+                        uint32_t* hist, uint32_t* counts, uint32_t* offsets, int64_t cap, Tab tab,
+                        LaunchSink& ls) {
+    SortBuffers<uint64_t> sb;
+    sb.keys[0] = keys[0]; sb.keys[1] = keys[1];
+    sb.vals[0] = idx[0]; sb.vals[1] = idx[1];
+    sb.hist = hist; sb.counts = counts; sb.offsets = offsets;
+    sb.n_dev = count;
+    hipError_t e = radix_sort_pairs(sb, cap, 64, ls);
+    if (e != hipSuccess) return e;
+    ls.launch("k_rec_sum", &k_rec_sum<Tab>, dim3((unsigned)((cap + kBlock - 1) / kBlock)), dim3(kBlock),
+              sb.keys[sb.result], sb.vals[sb.result], val, count, cap, tab);
+    return ls.err;
 }
 
 }  // namespace
@@ -866,6 +994,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
     }
     gr.g = rl.grid;
     RlTables t{rl.Q, rl.newQ, rl.cnt, rl.total, rl.ldist, rl.lres, rl.lnew, rl.lcount, rl.lnewc};
+    const RlReqState st{t, gr, a.res, rl.dims, rl.alpha_scale, 0u, a.max_depth, rl.rec_keys[0] ? 1 : 0};
     hipError_t e = hipMemsetAsync(a.counts, 0, 64, s);
     if (e != hipSuccess) return e;
     if ((e = hipMemsetAsync(rl.dims, 0, 16, s)) != hipSuccess) return e;
@@ -875,9 +1004,11 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
                        ncells, nent, L);
     render_raygen(a, s);
     hipLaunchKernelGGL(k_init_tasks, dim3(nb), dim3(kBlock), 0, s, nsamp, rl.q[0]);
+    LaunchSink ls(s);
     int qin = 0;
     for (int depth = 0; depth < a.max_depth; ++depth) {
         const int qout = qin ^ 1;
+        const RenderSrc src{st, depth};
         if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[qin].ray,
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
                                      a.trace_counters, a.launch, s)) != hipSuccess)
@@ -899,40 +1030,96 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
             if ((e = radix_sort_pairs(sb, nsamp, bits, s, nullptr)) != hipSuccess) return e;
             perm = sb.vals[sb.result];
         }
-        hipLaunchKernelGGL(k_lighting, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, gr, a.q[qin],
-                           rl.slots, a.sh, t, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin,
-                           perm);
+        ls.launch("k_lighting", &k_lighting<RenderSrc>, dim3(nb), dim3(kBlock), src, a.scene, a.halton, a.q[qin],
+                  rl.slots, a.sh, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, perm);
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
                                       a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s)) != hipSuccess)
             return e;
-        const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count};
-        const unsigned rb = (unsigned)((rl.rec_cap + kBlock - 1) / kBlock);
-        if (rec.keys)
-            hipLaunchKernelGGL(k_rec_reset, dim3(rb), dim3(kBlock), 0, s, rl.rec_keys[0], rl.rec_idx[0],
-                               rl.rec_count, rl.rec_cap);
-        hipLaunchKernelGGL(k_qdist, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[qin], rl.q[qin], rl.slots,
-                           t, a.vis, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, rl.dims, qin, perm,
-                           rec);
-        if (rec.keys) {
-            SortBuffers<uint64_t> sb;
-            sb.keys[0] = rl.rec_keys[0]; sb.keys[1] = rl.rec_keys[1];
-            sb.vals[0] = rl.rec_idx[0]; sb.vals[1] = rl.rec_idx[1];
-            sb.hist = rl.rec_hist; sb.counts = rl.rec_counts; sb.offsets = rl.rec_offsets;
-            if ((e = radix_sort_pairs(sb, rl.rec_cap, 64, s, nullptr)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_rec_sum, dim3(rb), dim3(kBlock), 0, s, sb.keys[sb.result], sb.vals[sb.result],
-                               rl.rec_val, rl.rec_cap, t);
-        }
+        const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count, rl.rec_idx[0]};
+        if (rec.keys) ls.launch("k_rec_reset", &k_rec_reset, dim3(1), dim3(64), rl.rec_count);
+        ls.launch("k_qdist", &k_qdist<RenderSrc>, dim3(nb), dim3(kBlock), src, a.scene, a.halton, a.q[qin], rl.q[qin],
+                  rl.slots, a.vis, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, perm, rec);
+        if (rec.keys &&
+            (e = rec_sort_sum(rl.rec_keys, rl.rec_idx, rl.rec_val, rl.rec_count, rl.rec_hist, rl.rec_counts,
+                              rl.rec_offsets, rl.rec_cap, RenderTables{t}, ls)) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(k_finish_q, dim3((unsigned)((nent + kEntryWaves - 1) / kEntryWaves)),
                            dim3(64 * kEntryWaves), 0, s, t, nent, rl.alpha_scale);
         hipLaunchKernelGGL(k_finish_light, dim3((unsigned)((nent + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t,
                            nent, L);
-        hipLaunchKernelGGL(k_next, dim3(nb), dim3(kBlock), 0, s, depth, a.scene, a.halton, a.q[qin], a.q[qout],
-                           rl.q[qout], rl.slots, a.hit_prim, a.hit_t, hu, hv, a.counts, rl.dims, qin, qout,
-                           perm);
+        ls.launch("k_next", &k_next<RenderSrc>, dim3(nb), dim3(kBlock), src, a.scene, a.halton, a.q[qin], a.q[qout],
+                  rl.q[qout], rl.slots, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, qout, perm, (int32_t*)nullptr,
+                  (uint8_t*)nullptr);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, rl.dims, qin);
         qin = qout;
+        if (ls.err != hipSuccess) return ls.err;
     }
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ server pools
+namespace {
+
+PoolSrc pool_src(const PoolDev& p, int qin) {
+    return PoolSrc{p.rl.req, p.rid[qin], p.depth[qin], p.sh_rid, p.rl.live};
+}
+
+__global__ void k_clear_dims(int32_t* dims) {
+    if (threadIdx.x < 2) dims[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+void pool_rl_lighting(const PoolDev& p, int qin, LaunchSink& ls) {
+    const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
+    const bool bary = p.scene.needs_bary();
+    const PoolSrc src = pool_src(p, qin);
+    ls.launch("k_pool_dim_max", &k_pool_dim_max, dim3(nb), dim3(kBlock), src, p.counts, qin, p.rl.q[qin], p.hit_prim);
+    ls.launch("k_lighting<pool>", &k_lighting<PoolSrc>, dim3(nb), dim3(kBlock), src, p.scene, p.halton, p.q[qin],
+              p.rl.sl, p.sh, p.hit_prim, p.hit_t, bary ? p.hit_u : nullptr, bary ? p.hit_v : nullptr, p.counts, qin,
+              (const uint32_t*)nullptr);
+}
+
+void pool_rl_learn(const PoolDev& p, int qin, bool det, LaunchSink& ls) {
+    const int qout = qin ^ 1;
+    const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
+    const bool bary = p.scene.needs_bary();
+    float* hu = bary ? p.hit_u : nullptr;
+    float* hv = bary ? p.hit_v : nullptr;
+    const PoolSrc src = pool_src(p, qin);
+    RlRecords rec{nullptr, nullptr, nullptr, nullptr};
+    if (det) {
+        rec = RlRecords{p.rl.rec_keys[0], p.rl.rec_val, p.rl.rec_count, p.rl.rec_idx[0]};
+        ls.launch("k_rec_reset", &k_rec_reset, dim3(1), dim3(64), p.rl.rec_count);
+    }
+    ls.launch("k_qdist<pool>", &k_qdist<PoolSrc>, dim3(nb), dim3(kBlock), src, p.scene, p.halton, p.q[qin],
+              p.rl.q[qin], p.rl.sl, p.vis, p.hit_prim, p.hit_t, hu, hv, p.counts, qin, (const uint32_t*)nullptr, rec);
+    if (det)
+        (void)rec_sort_sum(p.rl.rec_keys, p.rl.rec_idx, p.rl.rec_val, p.rl.rec_count, p.rl.rec_hist, p.rl.rec_counts,
+                           p.rl.rec_offsets, p.rl.rec_cap, PoolTables{p.rl.req}, ls);
+    // the updates of every request with hits; a fixed grid, so that the
+    // launch does not depend on which requests are in the pool
+    ls.launch("k_pool_finish_q", &k_pool_finish_q, dim3(2048), dim3(64 * kEntryWaves), (const RlReqState*)p.rl.req,
+              (const uint8_t*)p.rl.live, kPoolSlots);
+    ls.launch("k_pool_finish_light", &k_pool_finish_light, dim3(512), dim3(kBlock), (const RlReqState*)p.rl.req,
+              (const uint8_t*)p.rl.live, kPoolSlots, p.scene.nlights);
+    ls.launch("k_next<pool>", &k_next<PoolSrc>, dim3(nb), dim3(kBlock), src, p.scene, p.halton, p.q[qin], p.q[qout],
+              p.rl.q[qout], p.rl.sl, p.hit_prim, p.hit_t, hu, hv, p.counts, qin, qout, (const uint32_t*)nullptr,
+              p.rid[qout], p.depth[qout]);
+    ls.launch("k_pool_rl_step", &k_pool_rl_step, dim3(1), dim3(kPoolSlots), (const RlReqState*)p.rl.req, p.rl.live,
+              kPoolSlots);
+}
+
+hipError_t pool_rl_init(const PoolDev& p, const RlReqState& st, int nlights, hipStream_t s) {
+    (void)p;
+    const int64_t nent = (int64_t)st.gr.g * st.gr.g * st.gr.g;
+    const int64_t ncells = nent * kXY;
+    const int64_t ninit = ncells > nent * nlights ? ncells : nent * nlights;
+    LaunchSink ls(s);
+    ls.launch("k_init_tables", &k_init_tables, dim3((unsigned)((ninit + kBlock - 1) / kBlock)), dim3(kBlock), st.t,
+              ncells, nent, nlights);
+    ls.launch("k_clear_dims", &k_clear_dims, dim3(1), dim3(64), st.dims);
+    return ls.err;
 }
 
 }  // namespace cav

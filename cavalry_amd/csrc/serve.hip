@@ -42,8 +42,13 @@ __global__ __launch_bounds__(kBlock) void k_pool_raygen(PoolDev p, const Chunk* 
         }
         const Chunk c = chunks[lo];
         rid = c.rid;
-        si = c.first + (g - c.off);
         const ReqDev& r = p.reqs[rid];
+        if (c.spp_round > 0) {  // an rlpath round: c.spp_round samples of every pixel from sample c.first
+            const int64_t k = g - c.off;
+            si = (k / c.spp_round) * r.spp + c.first + k % c.spp_round;
+        } else {
+            si = c.first + (g - c.off);
+        }
         const int64_t pix = si / r.spp;
         const int x = (int)(pix % r.cam.width), y = (int)(pix / r.cam.width);
         float fx = (float)x, fy = (float)y;
@@ -67,6 +72,12 @@ __global__ __launch_bounds__(kBlock) void k_pool_raygen(PoolDev p, const Chunk* 
         q.delta[slot] = 1;
         p.rid[qn][slot] = rid;
         p.depth[qn][slot] = 0;
+        if (p.reqs[rid].integrator == CAV_INTEGRATOR_RLPATH) {
+            // genInitialRays' QEntryInfo {-1, -1}; the camera sample used 2 dimensions
+            p.rl.q[qn].prev_entry[slot] = -1;
+            p.rl.q[qn].prev_cell[slot] = -1;
+            p.rl.q[qn].dim[slot] = 2;
+        }
     }
 }
 
@@ -74,6 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
     const int qout = qin ^ 1;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool live = i < (int64_t)p.counts[qin];
+    // rlpath tasks are shaded by the rlpath kernels
+    live = live && p.reqs[p.rid[qin][i]].integrator != CAV_INTEGRATOR_RLPATH;
     const int prim = live ? p.hit_prim[i] : -1;
     if (live && prim < 0 && p.scene.env_miss >= 0) {  // a miss sees the background, weighted as the loops do
         const Queue& qi = p.q[qin];
@@ -125,8 +138,10 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
 __global__ __launch_bounds__(kBlock) void k_pool_nee(PoolDev p) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= (int64_t)p.counts[2] || !p.vis[i]) return;
+    const ReqDev& r = p.reqs[p.sh_rid[i]];
+    if (r.integrator == CAV_INTEGRATOR_RLPATH) return;  // read by k_qdist
     const uint32_t sid = p.sh.sid[i];
-    float* res = p.reqs[p.sh_rid[i]].res + (int64_t)sid * 3;
+    float* res = r.res + (int64_t)sid * 3;
     res[0] = res[0] + p.sh.contrib[i * 3];
     res[1] = res[1] + p.sh.contrib[i * 3 + 1];
     res[2] = res[2] + p.sh.contrib[i * 3 + 2];
@@ -152,15 +167,17 @@ hipError_t pool_raygen(const PoolDev& p, const Chunk* chunks_dev, int nchunks, i
     return ls.err;
 }
 
-void pool_bounce(const PoolDev& p, int qin, LaunchSink& ls) {
+void pool_bounce(const PoolDev& p, int qin, int kind, LaunchSink& ls) {
     const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
     trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray, p.counts + qin, p.cap, p.hit_prim,
                         p.hit_t, p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
                         p.trace_counters, p.launch, ls);
     ls.launch("k_pool_shade", &k_pool_shade, dim3(nb), dim3(kBlock), p, qin);
+    if (kind != kBouncePlain) pool_rl_lighting(p, qin, ls);
     trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit, p.sh.src, p.sh.dst,
                          p.vis, p.trace_counters, p.launch, ls);
     ls.launch("k_pool_nee", &k_pool_nee, dim3(nb), dim3(kBlock), p);
+    if (kind != kBouncePlain) pool_rl_learn(p, qin, kind == kBounceRlDet, ls);
     ls.launch("k_pool_step", &k_pool_step, dim3(1), dim3(64), p, qin);
 }
 

@@ -3,23 +3,25 @@
 // per process; this is the serving layer around its integrators.
 //
 // One worker thread owns a persistent ray pool per scene. Every step it
-// admits camera-ray chunks of the scene's pending `path` / `directlighting`
-// requests into the pool (round-robin, a fixed quantum per request, within
-// the pool's free task slots and the device arena), then enqueues one bounce
-// of every task in the pool. Tasks of different requests and different
-// depths share every launch. A chunk admitted at step s has finished after
-// step s + max_depth - 1 (step s for directlighting), so completion is known
-// on the host by counting steps; the request's film is then enqueued and
-// copied to pinned staging, and an event says when it is ready. The host runs
-// at most `max_steps_ahead` steps ahead of the device. Requests whose buffers
-// do not fit the arena wait (admission by memory).
+// admits camera rays of the scene's pending requests into the pool
+// (round-robin, within the pool's free task slots and the device arena),
+// then enqueues one bounce of every task in the pool. Tasks of different
+// requests, integrators and depths share every launch. A batch of samples
+// admitted at step s has finished after step s + max_depth - 1 (step s for
+// directlighting), so completion is known on the host by counting steps; the
+// request's film is then enqueued and copied to pinned staging, and an event
+// says when it is ready. The host runs at most `max_steps_ahead` steps ahead
+// of the device. Requests whose buffers do not fit the arena wait (admission
+// by memory).
 //
-// rlpath requests learn from every sample of a bounce of their own render,
-// so they are not mixed: rl_workers threads render them whole through
-// cav_render, each on a context (stream) of its own, so they overlap each
-// other and the pools on the device (separate streams, default priority). A
-// small rlpath request leaves most of the device idle while its slowest rays
-// finish each bounce; several at once fill it.
+// path / directlighting requests are admitted in chunks of contiguous sample
+// ids (at most chunk_samples per request per step). rlpath learns from every
+// sample of a bounce of its own round (the reference's render loop,
+// ReinforcementLearningPathTracing.cu:802-950), so an rlpath request is
+// admitted one whole round at a time (round_spp samples of every pixel) once
+// its previous round has finished, with its own Q table, light statistics
+// and syncDimension words in its arena block: its film is the one cav_render
+// makes with spp_per_round = round_spp, whatever else shares the pool.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,13 +47,15 @@ namespace cav {
 SceneDev scene_dev(const cav_scene* sc);
 int halton_dev(cav_ctx* ctx, HaltonDev* out);
 void scene_hold(cav_scene* sc, int delta);
+void scene_bounds(const cav_scene* sc, float out[6]);
 }  // namespace cav
 
 using namespace cav;
 
 namespace {
 
-constexpr int kSlots = 256;  // requests in flight per pool
+constexpr int kSlots = kPoolSlots;
+constexpr int kQBins = 128;  // direction bins per Q entry (QEntry::NUM_XY)
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -116,12 +120,13 @@ struct Request {
     int32_t status = CAV_SERVE_QUEUED;
     int32_t error = CAV_OK;
     double t_submit = 0, t_start = -1, t_done = -1;
-    int64_t closest = 0, shadow = 0;
-    // pooled requests
     bool active = false;
     int slot = -1;
-    int64_t next_sample = 0;
-    int64_t outstanding = 0;  // chunks not yet finished
+    int64_t next_sample = 0;  // path / directlighting: next sample id to admit
+    int spp_admitted = 0;     // rlpath: samples per pixel admitted so far
+    int round_spp = 0;        // rlpath: samples per pixel per round
+    int grid = 32;            // rlpath: Q entries per axis
+    int64_t outstanding = 0;  // admitted batches not yet finished
     size_t dev_off = 0, dev_len = 0, host_off = 0, host_len = 0;
     float *res = nullptr, *sxy = nullptr, *rgb = nullptr, *w = nullptr;
     float4* partial = nullptr;
@@ -132,6 +137,9 @@ struct Request {
     bool cancel = false;
     bool final = false;  // a final status was set (the scene's hold released)
     std::string msg;     // why it failed
+    bool rl() const { return prm.integrator == CAV_INTEGRATOR_RLPATH; }
+    bool det() const { return rl() && (prm.flags & CAV_RENDER_DETERMINISTIC); }
+    bool all_admitted() const { return rl() ? spp_admitted == prm.spp : next_sample == nsamp; }
 };
 
 struct LiveChunk {
@@ -147,16 +155,19 @@ struct Pool {
     int qin = 0;
     int64_t step = 0;
     int64_t reserved = 0;
-    std::vector<std::pair<LiveChunk, int64_t>> live;  // chunk, samples
+    std::vector<std::pair<LiveChunk, int64_t>> live;  // batch, samples
     std::deque<Request*> pending;
     Request* slots[kSlots] = {};
     ReqDev* d_reqs = nullptr;
-    ReqDev* h_reqs = nullptr;  // pinned staging of request entries, [ring][kSlots]
-    Chunk* d_chunks = nullptr;  // [ring][kSlots]
-    Chunk* h_chunks = nullptr;  // pinned
+    ReqDev* h_reqs = nullptr;        // pinned staging of request entries, [ring][kSlots]
+    RlReqState* h_rl = nullptr;      // pinned staging of rlpath states, [ring][kSlots]
+    Chunk* d_chunks = nullptr;       // [ring][kSlots]
+    Chunk* h_chunks = nullptr;       // pinned
     std::vector<hipEvent_t> ring;
-    int64_t done_upto = 0;      // steps known complete on the device
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // the bounce sequence per queue parity
+    int64_t done_upto = 0;           // steps known complete on the device
+    // the bounce sequence per queue parity and bounce kind (kBounce*)
+    hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    bool graphs = false;
 };
 
 }  // namespace
@@ -173,19 +184,14 @@ struct cav_server {
     int64_t next_id = 1;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request*> incoming;
-    std::deque<Request*> rl_queue;
     std::vector<std::unique_ptr<Pool>> pools;
     Arena dev_arena, host_arena;
     char* dev_base = nullptr;
     char* host_base = nullptr;
     std::thread worker;
-    std::vector<std::thread> rl_workers;
-    std::vector<cav_ctx*> rl_ctxs;
     unsigned long long pool_rays[2] = {0, 0};
-    int64_t rl_rays[2] = {0, 0};
-    std::string error;  // fatal: the failing call and its HIP error
+    std::string error;       // fatal: the failing call and its HIP error
     std::string graph_note;  // why a pool runs without graphs
-    int graph_failures = 0;
 };
 
 namespace {
@@ -196,9 +202,6 @@ struct ApiLock {
     std::unique_lock<std::mutex> lk;
     explicit ApiLock(cav_server* sv) : lk((sv->api_waiting.fetch_add(1), sv->mu)) { sv->api_waiting.fetch_sub(1); }
 };
-}  // namespace
-
-namespace {
 
 // Sets a request's final status once, releasing its hold on the scene.
 void finalize(cav_server* sv, Request* r, int32_t status, int32_t error = CAV_OK, const std::string& msg = "") {
@@ -210,6 +213,27 @@ void finalize(cav_server* sv, Request* r, int32_t status, int32_t error = CAV_OK
     r->t_done = now_ms(sv->t0);
     scene_hold(r->scene, -1);
     sv->done_cv.notify_all();
+}
+
+// rlpath samples per pixel per round: the caller's, or as many as the
+// admission quantum holds (at least one)
+int rl_round_spp(const cav_serve_config& c, const cav_render_params& prm, int64_t npix) {
+    if (prm.spp_per_round > 0) return std::min(prm.spp_per_round, prm.spp);
+    const int64_t k = c.chunk_samples / npix;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(prm.spp, k));
+}
+
+int rl_grid(const cav_render_params& prm) { return prm.q_grid > 0 ? prm.q_grid : 32; }
+
+// device bytes of a request's arena block
+size_t request_bytes(const Request& r, int nlights) {
+    const size_t S = (size_t)r.nsamp, P = (size_t)r.npix;
+    size_t b = 12 * S + 8 * S + 144 * P + 12 * P + 4 * P + 5 * 256;
+    if (r.rl()) {
+        const size_t E = (size_t)r.grid * r.grid * r.grid;
+        b += 4 * (4 * E * kQBins + 256) + 5 * (4 * E * nlights + 256) + 256;
+    }
+    return b;
 }
 
 int alloc_dev(std::vector<void*>& list, void** p, size_t bytes) {
@@ -240,15 +264,29 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     A(d.sh.contrib, 12 * C); A(d.sh_rid, 4 * C); A(d.vis, C); A(d.hit_prim, 4 * C); A(d.hit_t, 4 * C);
     A(d.hit_u, 4 * C); A(d.hit_v, 4 * C);
     A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 64);
+    // rlpath tasks: per-slot learning state, queue and bounce state, records
+    PoolRl& R = d.rl;
+    A(R.req, sizeof(RlReqState) * kSlots); A(R.live, kSlots);
+    for (int k = 0; k < 2; ++k) { A(R.q[k].prev_entry, 4 * C); A(R.q[k].prev_cell, 4 * C); A(R.q[k].dim, 4 * C); }
+    A(R.sl.entry, 4 * C); A(R.sl.light, 4 * C); A(R.sl.sel, 4 * C); A(R.sl.lpdf, 4 * C); A(R.sl.emis, 4 * C);
+    A(R.sl.ldir, 12 * C); A(R.sl.inc, 12 * C); A(R.sl.sh_idx, 4 * C); A(R.sl.ndir, 12 * C); A(R.sl.npdf, 4 * C);
+    A(R.sl.ncell, 4 * C);
+    R.rec_cap = 3 * (int64_t)C;  // at most 3 updates per task
+    const size_t RC = (size_t)R.rec_cap, RT = (size_t)sort_tiles(R.rec_cap);
+    for (int k = 0; k < 2; ++k) { A(R.rec_keys[k], 8 * RC); A(R.rec_idx[k], 4 * RC); }
+    A(R.rec_val, 4 * RC); A(R.rec_count, 256); A(R.rec_hist, 4 * 8 * 256); A(R.rec_counts, 4 * 256 * RT);
+    A(R.rec_offsets, 4 * 256 * RT);
     A(p->d_reqs, sizeof(ReqDev) * kSlots);
     const int ring = sv->cfg.max_steps_ahead + 2;
     A(p->d_chunks, sizeof(Chunk) * kSlots * ring);
     if (rc != CAV_OK) {
         for (void* q : L) (void)hipFree(q);
+        sv->error = "ray pool allocation (" + std::to_string(C) + " tasks)";
         return rc;
     }
     d.reqs = p->d_reqs;
     bool ok = hipHostMalloc(reinterpret_cast<void**>(&p->h_reqs), sizeof(ReqDev) * kSlots * ring, 0) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void**>(&p->h_rl), sizeof(RlReqState) * kSlots * ring, 0) == hipSuccess &&
               hipHostMalloc(reinterpret_cast<void**>(&p->h_chunks), sizeof(Chunk) * kSlots * ring, 0) == hipSuccess &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
     p->ring.assign(ring, nullptr);
@@ -256,6 +294,8 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     ok = ok && hipMemsetAsync(d.counts, 0, 64, p->stream) == hipSuccess &&
          hipMemsetAsync(d.rays, 0, 64, p->stream) == hipSuccess &&
          hipMemsetAsync(d.trace_counters, 0, 64, p->stream) == hipSuccess &&
+         hipMemsetAsync(R.req, 0, sizeof(RlReqState) * kSlots, p->stream) == hipSuccess &&
+         hipMemsetAsync(R.live, 0, kSlots, p->stream) == hipSuccess &&
          hipStreamSynchronize(p->stream) == hipSuccess;
     if (!ok) {
         sv->error = "creating the ray pool's stream, events or pinned staging failed";
@@ -269,12 +309,14 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
 
 void destroy_pool(Pool* p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    for (int k = 0; k < 2; ++k)
-        if (p->graph[k]) (void)hipGraphExecDestroy(p->graph[k]);
+    for (auto& row : p->graph)
+        for (auto& g : row)
+            if (g) (void)hipGraphExecDestroy(g);
     for (auto e : p->ring)
         if (e) (void)hipEventDestroy(e);
     for (void* q : p->bufs) (void)hipFree(q);
     if (p->h_reqs) (void)hipHostFree(p->h_reqs);
+    if (p->h_rl) (void)hipHostFree(p->h_rl);
     if (p->h_chunks) (void)hipHostFree(p->h_chunks);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     scene_hold(p->scene, -1);
@@ -286,15 +328,25 @@ Pool* pool_for(cav_server* sv, cav_scene* sc) {
     return nullptr;
 }
 
-// buffers and a slot for a request's first admission; false if they do not
-// fit now
+// A request slot, buffers and (rlpath) fresh tables for a request's first
+// admission; false if they do not fit now. Deterministic rlpath requests take
+// slots below kPoolDetSlots (their index in the update records), the others
+// prefer the slots above.
 bool activate(cav_server* sv, Pool* p, Request* r) {
     int slot = -1;
-    for (int k = 0; k < kSlots; ++k)
-        if (!p->slots[k]) { slot = k; break; }
+    if (r->det()) {
+        for (int k = 0; k < kPoolDetSlots && slot < 0; ++k)
+            if (!p->slots[k]) slot = k;
+    } else {
+        for (int k = kPoolDetSlots; k < kSlots && slot < 0; ++k)
+            if (!p->slots[k]) slot = k;
+        for (int k = 0; k < kPoolDetSlots && slot < 0; ++k)
+            if (!p->slots[k]) slot = k;
+    }
     if (slot < 0) return false;
+    const SceneDev& sd = p->dev.scene;
     const size_t S = (size_t)r->nsamp, P = (size_t)r->npix;
-    const size_t dlen = 12 * S + 8 * S + 144 * P + 12 * P + 4 * P + 5 * 256;
+    const size_t dlen = request_bytes(*r, sd.nlights);
     const size_t hlen = 16 * P + 3 * 256;
     size_t doff = 0, hoff = 0;
     if (!sv->dev_arena.alloc(dlen, doff)) return false;
@@ -313,6 +365,37 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     r->partial = reinterpret_cast<float4*>(carve(c, 144 * P));
     r->rgb = reinterpret_cast<float*>(carve(c, 12 * P));
     r->w = reinterpret_cast<float*>(carve(c, 4 * P));
+    const int ring = (int)p->ring.size();
+    const size_t stage = (size_t)(p->step % ring) * kSlots + slot;
+    // staged per step: an entry copied at an earlier step may still be queued
+    RlReqState& st = p->h_rl[stage];
+    std::memset(&st, 0, sizeof st);
+    if (r->rl()) {
+        const size_t E = (size_t)r->grid * r->grid * r->grid, NL = (size_t)sd.nlights;
+        RlTables& t = st.t;
+        t.Q = reinterpret_cast<float*>(carve(c, 4 * E * kQBins));
+        t.newQ = reinterpret_cast<float*>(carve(c, 4 * E * kQBins));
+        t.cnt = reinterpret_cast<int32_t*>(carve(c, 4 * E * kQBins));
+        t.total = reinterpret_cast<int32_t*>(carve(c, 4 * E * kQBins));
+        t.ldist = reinterpret_cast<float*>(carve(c, 4 * E * NL));
+        t.lres = reinterpret_cast<float*>(carve(c, 4 * E * NL));
+        t.lnew = reinterpret_cast<float*>(carve(c, 4 * E * NL));
+        t.lcount = reinterpret_cast<int32_t*>(carve(c, 4 * E * NL));
+        t.lnewc = reinterpret_cast<int32_t*>(carve(c, 4 * E * NL));
+        st.dims = reinterpret_cast<int32_t*>(carve(c, 64));
+        float b6[6];
+        scene_bounds(p->scene, b6);
+        for (int k = 0; k < 3; ++k) {
+            st.gr.lo[k] = b6[k];
+            st.gr.size[k] = b6[3 + k] - b6[k];
+        }
+        st.gr.g = r->grid;
+        st.res = r->res;
+        st.alpha_scale = r->prm.alpha_scale > 0.f ? r->prm.alpha_scale : 1.f;
+        st.rlx = (uint32_t)slot;
+        st.max_depth = r->prm.max_depth;
+        st.det = r->det() ? 1 : 0;
+    }
     char* h = sv->host_arena.at(hoff);
     r->h_rgb = reinterpret_cast<float*>(carve(h, 12 * P));
     r->h_w = reinterpret_cast<float*>(carve(h, 4 * P));
@@ -320,8 +403,7 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     r->dev_off = doff; r->dev_len = dlen; r->host_off = hoff; r->host_len = hlen;
     r->slot = slot;
     p->slots[slot] = r;
-    // staged per step: an entry copied at an earlier step may still be queued
-    ReqDev& e = p->h_reqs[(size_t)(p->step % (int64_t)p->ring.size()) * kSlots + slot];
+    ReqDev& e = p->h_reqs[stage];
     std::memcpy(e.cam.m, r->cam.cam_to_world, sizeof e.cam.m);
     e.cam.depth = r->cam.depth;
     e.cam.width = r->cam.width;
@@ -332,12 +414,15 @@ bool activate(cav_server* sv, Pool* p, Request* r) {
     e.res = r->res;
     e.sxy = r->sxy;
     // the pool's earlier steps no longer read this slot (its previous owner
-    // finished), so the stream-ordered copy is safe
+    // finished), so the stream-ordered copies are safe
     hipError_t he = hipMemcpyAsync(p->d_reqs + slot, &e, sizeof(ReqDev), hipMemcpyHostToDevice, p->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(p->dev.rl.req + slot, &st, sizeof(RlReqState), hipMemcpyHostToDevice, p->stream);
+    if (he == hipSuccess && r->rl()) he = pool_rl_init(p->dev, st, sd.nlights, p->stream);
     // the pool's overflow counters before the request's first bounce
     if (he == hipSuccess)
         he = hipMemcpyAsync(r->h_ovf, p->dev.trace_counters, 8, hipMemcpyDeviceToHost, p->stream);
-    if (he != hipSuccess) sv->error = hip_msg("request admission (hipMemcpyAsync)", he);
+    if (he != hipSuccess) sv->error = hip_msg("request admission", he);
     r->active = true;
     return true;
 }
@@ -348,29 +433,93 @@ void release_request_memory(cav_server* sv, Request* r) {
     r->dev_len = r->host_len = 0;
 }
 
-// Records the pool's bounce sequence for each queue parity into a graph,
-// node by node (no stream capture). On failure the pool runs without graphs
-// and the reason is kept in `why`.
+// Records the pool's bounce sequences (each queue parity and bounce kind)
+// into graphs, node by node (no stream capture). On failure the pool runs its
+// launches directly and the reason is kept in `why`.
 bool build_graphs(Pool* p, std::string& why) {
-    for (int k = 0; k < 2; ++k) {
-        hipGraph_t g = nullptr;
-        hipError_t e = hipGraphCreate(&g, 0);
-        if (e != hipSuccess) { why = hip_msg("hipGraphCreate", e); break; }
-        LaunchSink ls(g);
-        pool_bounce(p->dev, k, ls);
-        if (ls.err != hipSuccess) {
-            why = hip_msg((std::string("hipGraphAddKernelNode ") + ls.failed).c_str(), ls.err);
-        } else if ((e = hipGraphInstantiate(&p->graph[k], g, nullptr, nullptr, 0)) != hipSuccess) {
-            p->graph[k] = nullptr;
-            why = hip_msg("hipGraphInstantiate", e);
+    for (int k = 0; k < 2 && why.empty(); ++k)
+        for (int kind = kBouncePlain; kind <= kBounceRlDet && why.empty(); ++kind) {
+            hipGraph_t g = nullptr;
+            hipError_t e = hipGraphCreate(&g, 0);
+            if (e != hipSuccess) { why = hip_msg("hipGraphCreate", e); break; }
+            LaunchSink ls(g);
+            pool_bounce(p->dev, k, kind, ls);
+            if (ls.err != hipSuccess) {
+                why = hip_msg((std::string("hipGraphAddKernelNode ") + ls.failed).c_str(), ls.err);
+            } else if ((e = hipGraphInstantiate(&p->graph[k][kind], g, nullptr, nullptr, 0)) != hipSuccess) {
+                p->graph[k][kind] = nullptr;
+                why = hip_msg("hipGraphInstantiate", e);
+            }
+            (void)hipGraphDestroy(g);
         }
-        (void)hipGraphDestroy(g);
-        if (!why.empty()) break;
-    }
-    if (why.empty()) return true;
-    for (auto& x : p->graph)
-        if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+    p->graphs = why.empty();
+    if (p->graphs) return true;
+    for (auto& row : p->graph)
+        for (auto& x : row)
+            if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
     return false;
+}
+
+// Admission of one step: round-robin over the pending requests. A path or
+// directlighting request gets one chunk (at most the quantum) and goes to the
+// back, so room freed at the next step goes to the requests after it. An
+// rlpath request gets its next round once the previous one has finished; a
+// round that does not fit yet holds the queue, so the room it needs builds up.
+int admit(cav_server* sv, Pool* p, Chunk* hc, int& nch, int64_t& total) {
+    const int64_t cap = p->dev.cap;
+    const size_t turns = p->pending.size();
+    for (size_t t = 0; t < turns && nch < kSlots; ++t) {
+        Request* r = p->pending.front();
+        p->pending.pop_front();
+        if (r->cancel && !r->active) {
+            finalize(sv, r, CAV_SERVE_CANCELLED);
+            continue;
+        }
+        if (r->rl() && r->outstanding > 0) {  // its current round is still in the pool
+            p->pending.push_back(r);
+            continue;
+        }
+        const int64_t room = cap - p->reserved;
+        if (room <= 0) {
+            p->pending.push_front(r);
+            break;
+        }
+        int spp_now = 0;
+        int64_t n = 0;
+        if (r->rl()) {
+            spp_now = std::min(r->round_spp, r->prm.spp - r->spp_admitted);
+            n = r->npix * spp_now;
+            if (n > room) {
+                p->pending.push_front(r);
+                break;
+            }
+        }
+        if (!r->active && !activate(sv, p, r)) {
+            p->pending.push_back(r);
+            continue;
+        }
+        if (!sv->error.empty()) return CAV_ERR_HIP;
+        if (r->t_start < 0) {
+            r->t_start = now_ms(sv->t0);
+            r->status = CAV_SERVE_RUNNING;
+        }
+        if (r->rl()) {
+            hc[nch] = Chunk{r->slot, spp_now, r->spp_admitted, n, total};
+            r->spp_admitted += spp_now;
+        } else {
+            n = std::min({room, sv->cfg.chunk_samples, r->nsamp - r->next_sample});
+            hc[nch] = Chunk{r->slot, 0, r->next_sample, n, total};
+            r->next_sample += n;
+        }
+        ++nch;
+        total += n;
+        r->outstanding += 1;
+        p->reserved += n;
+        const int64_t last = r->prm.integrator == CAV_INTEGRATOR_DIRECT ? 0 : r->prm.max_depth - 1;
+        p->live.push_back({LiveChunk{r, p->step + last}, n});
+        if (!r->all_admitted()) p->pending.push_back(r);
+    }
+    return CAV_OK;
 }
 
 // one step of one pool; called with sv->mu held (it touches request state),
@@ -384,46 +533,10 @@ int pool_step(cav_server* sv, Pool* p) {
         if (e != hipSuccess) { sv->error = hip_msg("hipEventSynchronize (bounce step)", e); return CAV_ERR_HIP; }
         p->done_upto = s + 1;
     }
-    // admission: round-robin, one chunk (at most the quantum) per request in
-    // turn; a request that got one goes to the back, so room freed at the
-    // next step goes to the requests after it
     Chunk* hc = p->h_chunks + (size_t)(p->step % ring) * kSlots;
     int nch = 0;
     int64_t total = 0;
-    const int64_t cap = p->dev.cap;
-    const size_t turns = p->pending.size();
-    for (size_t t = 0; t < turns && nch < kSlots; ++t) {
-        Request* r = p->pending.front();
-        p->pending.pop_front();
-        if (r->cancel && !r->active) {
-            finalize(sv, r, CAV_SERVE_CANCELLED);
-            continue;
-        }
-        const int64_t room = cap - p->reserved;
-        if (room <= 0) {
-            p->pending.push_front(r);
-            break;
-        }
-        if (!r->active && !activate(sv, p, r)) {
-            p->pending.push_back(r);
-            continue;
-        }
-        if (!sv->error.empty()) return CAV_ERR_HIP;
-        const int64_t n = std::min({room, sv->cfg.chunk_samples, r->nsamp - r->next_sample});
-        if (r->t_start < 0) {
-            r->t_start = now_ms(sv->t0);
-            r->status = CAV_SERVE_RUNNING;
-        }
-        hc[nch] = Chunk{r->slot, 0, r->next_sample, n, total};
-        ++nch;
-        total += n;
-        r->next_sample += n;
-        r->outstanding += 1;
-        p->reserved += n;
-        const int64_t last = r->prm.integrator == CAV_INTEGRATOR_DIRECT ? 0 : r->prm.max_depth - 1;
-        p->live.push_back({LiveChunk{r, p->step + last}, n});
-        if (r->next_sample != r->nsamp) p->pending.push_back(r);
-    }
+    if (admit(sv, p, hc, nch, total) != CAV_OK) return CAV_ERR_HIP;
     if (nch) {
         Chunk* dc = p->d_chunks + (size_t)(p->step % ring) * kSlots;
         hipError_t e = hipMemcpyAsync(dc, hc, sizeof(Chunk) * nch, hipMemcpyHostToDevice, p->stream);
@@ -433,25 +546,30 @@ int pool_step(cav_server* sv, Pool* p) {
             return CAV_ERR_HIP;
         }
     }
-    if (p->graph[p->qin]) {
-        const hipError_t e = hipGraphLaunch(p->graph[p->qin], p->stream);
+    // the bounce kind: rlpath kernels only while rlpath tasks are in the
+    // pool, record sorting only for deterministic ones
+    int kind = kBouncePlain;
+    for (const auto& lc : p->live)
+        if (lc.first.req->rl()) kind = std::max(kind, lc.first.req->det() ? (int)kBounceRlDet : (int)kBounceRl);
+    if (p->graphs) {
+        const hipError_t e = hipGraphLaunch(p->graph[p->qin][kind], p->stream);
         if (e != hipSuccess) { sv->error = hip_msg("hipGraphLaunch (bounce step)", e); return CAV_ERR_HIP; }
     } else {
         LaunchSink ls(p->stream);
-        pool_bounce(p->dev, p->qin, ls);
+        pool_bounce(p->dev, p->qin, kind, ls);
         if (ls.err != hipSuccess) {
             sv->error = hip_msg((std::string(ls.failed) + " launch").c_str(), ls.err);
             return CAV_ERR_HIP;
         }
     }
-    // chunks whose last bounce was this step
+    // batches whose last bounce was this step
     for (size_t k = 0; k < p->live.size();) {
         if (p->live[k].first.done_step != p->step) { ++k; continue; }
         Request* r = p->live[k].first.req;
         p->reserved -= p->live[k].second;
         p->live[k] = p->live.back();
         p->live.pop_back();
-        if (--r->outstanding == 0 && r->next_sample == r->nsamp) {
+        if (--r->outstanding == 0 && r->all_admitted()) {
             film_launch(r->cam.width, r->cam.height, r->prm.spp, r->res, r->sxy, r->partial, r->rgb, r->w, p->stream);
             hipError_t e = hipGetLastError();
             const char* what = "film kernels";
@@ -523,11 +641,8 @@ void worker_main(cav_server* sv) {
             std::string why;
             if (!p) {
                 rc = make_pool(sv, r->scene, p);
-                if (rc == CAV_OK && sv->cfg.use_graphs && !build_graphs(p, why)) {
-                    // the pool runs its launches directly; the reason is kept
-                    sv->graph_note = why;
-                    ++sv->graph_failures;
-                }
+                // without graphs the pool launches its kernels directly; the reason is kept
+                if (rc == CAV_OK && sv->cfg.use_graphs && !build_graphs(p, why)) sv->graph_note = why;
             }
             if (rc != CAV_OK) {
                 finalize(sv, r, CAV_SERVE_FAILED, rc, sv->error.empty() ? "ray pool allocation failed" : sv->error);
@@ -542,11 +657,8 @@ void worker_main(cav_server* sv) {
             busy = true;
             if (pool_step(sv, p) != CAV_OK) {
                 if (sv->error.empty()) sv->error = "a HIP call failed while serving";
-                for (auto& kv : sv->reqs) {
-                    Request* q = kv.second.get();
-                    if (q->prm.integrator != CAV_INTEGRATOR_RLPATH && !q->final)
-                        finalize(sv, q, CAV_SERVE_FAILED, CAV_ERR_HIP, sv->error);
-                }
+                for (auto& kv : sv->reqs)
+                    if (!kv.second->final) finalize(sv, kv.second.get(), CAV_SERVE_FAILED, CAV_ERR_HIP, sv->error);
                 return;
             }
         }
@@ -579,36 +691,6 @@ void worker_main(cav_server* sv) {
     }
 }
 
-void rl_main(cav_server* sv, cav_ctx* rl_ctx) {
-    std::unique_lock<std::mutex> lk(sv->mu);
-    while (true) {
-        sv->cv.wait(lk, [&] { return sv->stop || !sv->rl_queue.empty(); });
-        if (sv->rl_queue.empty()) {
-            if (sv->stop) break;
-            continue;
-        }
-        Request* r = sv->rl_queue.front();
-        sv->rl_queue.pop_front();
-        if (r->cancel) {
-            finalize(sv, r, CAV_SERVE_CANCELLED);
-            continue;
-        }
-        r->status = CAV_SERVE_RUNNING;
-        r->t_start = now_ms(sv->t0);
-        lk.unlock();
-        cav_render_stats st{};
-        const int rc = cav_render(rl_ctx, r->scene, &r->cam, &r->prm, r->out_rgb, r->out_w, &st);
-        const std::string why = rc != CAV_OK ? std::string(cav_last_error(rl_ctx)) : std::string();
-        lk.lock();
-        r->closest = st.closest_rays;
-        r->shadow = st.shadow_rays;
-        sv->rl_rays[0] += st.closest_rays;
-        sv->rl_rays[1] += st.shadow_rays;
-        if (rc != CAV_OK) finalize(sv, r, CAV_SERVE_FAILED, rc, why);
-        else finalize(sv, r, r->cancel ? CAV_SERVE_CANCELLED : CAV_SERVE_DONE);
-    }
-}
-
 }  // namespace
 
 extern "C" {
@@ -625,9 +707,7 @@ int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) 
     if (c.max_steps_ahead <= 0) c.max_steps_ahead = 4;
     if (c.chunk_samples <= 0) c.chunk_samples = c.pool_tasks / 4;
     if (c.host_arena_bytes <= 0) c.host_arena_bytes = (int64_t)1 << 30;
-    if (c.rl_workers <= 0) c.rl_workers = 4;
-    if (c.rl_workers > 16) return fail(ctx, CAV_ERR_INVALID, "rl_workers above 16");
-    if (c.pool_tasks >= ((int64_t)1 << 31)) return fail(ctx, CAV_ERR_INVALID, "pool_tasks must be below 2^31");
+    if (c.pool_tasks >= ((int64_t)1 << 31) / 3) return fail(ctx, CAV_ERR_INVALID, "pool_tasks must be below 2^31 / 3");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, CAV_ERR_HIP, "hipSetDevice");
     int rc = halton_dev(ctx, &sv->halton);
     if (rc != CAV_OK) return rc;
@@ -639,23 +719,9 @@ int cav_serve_open(cav_ctx* ctx, const cav_serve_config* cfg, cav_server** out) 
     }
     sv->dev_arena.init(sv->dev_base, (size_t)c.arena_bytes);
     sv->host_arena.init(sv->host_base, (size_t)c.host_arena_bytes);
-    for (int k = 0; k < c.rl_workers; ++k) {
-        cav_ctx* rc_ctx = nullptr;
-        if ((rc = cav_ctx_create(ctx->device, &rc_ctx)) != CAV_OK) {
-            for (cav_ctx* x : sv->rl_ctxs) cav_ctx_destroy(x);
-            (void)hipFree(sv->dev_base);
-            (void)hipHostFree(sv->host_base);
-            return fail(ctx, rc, "server rlpath context");
-        }
-        // the caller's launch shapes (the autotuner sets them on its context)
-        rc_ctx->launch[CAV_TRACE_STANDALONE] = ctx->launch[CAV_TRACE_STANDALONE];
-        rc_ctx->launch[CAV_TRACE_WAVEFRONT] = ctx->launch[CAV_TRACE_WAVEFRONT];
-        sv->rl_ctxs.push_back(rc_ctx);
-    }
     sv->t0 = std::chrono::steady_clock::now();
     cav_server* raw = sv.get();
     raw->worker = std::thread(worker_main, raw);
-    for (cav_ctx* x : raw->rl_ctxs) raw->rl_workers.emplace_back(rl_main, raw, x);
     *out = sv.release();
     return CAV_OK;
 }
@@ -674,8 +740,11 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
     if (prm->integrator == CAV_INTEGRATOR_PATH &&
         2 + 10LL * prm->max_depth + (prm->max_depth > 4 ? prm->max_depth - 4 : 0) > 1024)
         return fail(sv->ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
-    if (prm->integrator == CAV_INTEGRATOR_PATH && prm->max_depth > 255)
+    if (prm->integrator == CAV_INTEGRATOR_RLPATH && 2 + 14LL * prm->max_depth > 1024)
+        return fail(sv->ctx, CAV_ERR_INVALID, "max_depth exceeds the 1024 Halton dimensions");
+    if (prm->integrator != CAV_INTEGRATOR_DIRECT && prm->max_depth > 255)
         return fail(sv->ctx, CAV_ERR_INVALID, "max_depth above 255");
+    if (prm->spp_per_round < 0) return fail(sv->ctx, CAV_ERR_INVALID, "spp_per_round must be >= 0");
     std::unique_ptr<Request> r(new Request());
     r->scene = scene;
     r->cam = *cam;
@@ -687,19 +756,26 @@ int cav_serve_submit(cav_server* sv, cav_scene* scene, const cav_camera* cam, co
     ApiLock lock(sv);
     if (sv->stop) return fail(sv->ctx, CAV_ERR_INVALID, "the server is closing");
     if (!sv->error.empty()) return fail(sv->ctx, CAV_ERR_HIP, "the server failed: " + sv->error);
-    if (prm->integrator != CAV_INTEGRATOR_RLPATH) {
-        const size_t need = 20 * (size_t)r->nsamp + 160 * (size_t)npix + 5 * 256;
-        if (need > (size_t)sv->cfg.arena_bytes || 16 * (size_t)npix + 512 > (size_t)sv->cfg.host_arena_bytes)
-            return fail(sv->ctx, CAV_ERR_OOM, "request larger than the server's arenas");
+    if (r->rl()) {
+        const int g = rl_grid(*prm);
+        if (g < 1 || g > 64) return fail(sv->ctx, CAV_ERR_INVALID, "q_grid must be in [1, 64]");
+        if (!(prm->alpha_scale >= 0.f && prm->alpha_scale <= 1.f))
+            return fail(sv->ctx, CAV_ERR_INVALID, "alpha_scale must be in [0, 1]");
+        r->grid = g;
+        r->round_spp = rl_round_spp(sv->cfg, *prm, npix);
+        if (npix * r->round_spp > sv->cfg.pool_tasks)
+            return fail(sv->ctx, CAV_ERR_INVALID, "one rlpath round (every pixel) exceeds the pool's task slots");
     }
+    const size_t need = request_bytes(*r, scene_dev(scene).nlights);
+    if (need > (size_t)sv->cfg.arena_bytes || 16 * (size_t)npix + 3 * 256 > (size_t)sv->cfg.host_arena_bytes)
+        return fail(sv->ctx, CAV_ERR_OOM, "request larger than the server's arenas");
     r->id = sv->next_id++;
     r->t_submit = now_ms(sv->t0);
     *id = r->id;
     Request* raw = r.get();
     scene_hold(scene, 1);  // until the request's final status
     sv->reqs.emplace(raw->id, std::move(r));
-    if (prm->integrator == CAV_INTEGRATOR_RLPATH) sv->rl_queue.push_back(raw);
-    else sv->incoming.push_back(raw);
+    sv->incoming.push_back(raw);
     sv->cv.notify_all();
     return CAV_OK;
 }
@@ -720,8 +796,8 @@ int cav_serve_poll(cav_server* sv, int64_t id, int32_t wait, cav_serve_result* o
     out->submit_ms = r->t_submit;
     out->start_ms = r->t_start;
     out->done_ms = r->t_done;
-    out->closest_rays = r->closest;
-    out->shadow_rays = r->shadow;
+    out->closest_rays = 0;
+    out->shadow_rays = 0;
     if (r->status == CAV_SERVE_FAILED) fail(sv->ctx, r->error, "request " + std::to_string(id) + ": " + r->msg);
     if (final_state()) sv->reqs.erase(it);  // the id is released with its final status
     return CAV_OK;
@@ -745,19 +821,17 @@ int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
         sv->cv.notify_all();
     }
     if (sv->worker.joinable()) sv->worker.join();
-    for (auto& t : sv->rl_workers)
-        if (t.joinable()) t.join();
     if (stats) {
         stats->pool_closest_rays = (int64_t)sv->pool_rays[0];
         stats->pool_shadow_rays = (int64_t)sv->pool_rays[1];
-        stats->rl_closest_rays = sv->rl_rays[0];
-        stats->rl_shadow_rays = sv->rl_rays[1];
+        stats->rl_closest_rays = 0;
+        stats->rl_shadow_rays = 0;
         int64_t steps = 0;
         for (auto& p : sv->pools) steps += p->step;
         stats->steps = steps;
         stats->pools = (int32_t)sv->pools.size();
         int32_t g = 0;
-        for (auto& p : sv->pools) g += p->graph[0] != nullptr;
+        for (auto& p : sv->pools) g += p->graphs ? 1 : 0;
         stats->graph_pools = g;
     }
     for (auto& p : sv->pools) destroy_pool(p.get());
@@ -767,7 +841,6 @@ int cav_serve_close(cav_server* sv, cav_serve_stats* stats) {
         // requests a failed server never finished still hold their scenes
         if (!r->final) finalize(sv, r, CAV_SERVE_FAILED, CAV_ERR_HIP, sv->error);
     }
-    for (cav_ctx* x : sv->rl_ctxs) cav_ctx_destroy(x);
     if (sv->dev_base) (void)hipFree(sv->dev_base);
     if (sv->host_base) (void)hipHostFree(sv->host_base);
     delete sv;

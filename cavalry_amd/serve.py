@@ -32,6 +32,8 @@ class Server:
     def __init__(self, ctx, arena_bytes=0, host_arena_bytes=0, pool_tasks=0, chunk_samples=0,
                  max_steps_ahead=0, use_graphs=False, rl_workers=0):
         self.ctx = ctx
+        self.pool_tasks = int(pool_tasks) if pool_tasks > 0 else 4 << 20
+        self.chunk_samples = int(chunk_samples) if chunk_samples > 0 else self.pool_tasks // 4
         cfg = ServeConfig(int(arena_bytes), int(host_arena_bytes), int(pool_tasks), int(chunk_samples),
                           int(max_steps_ahead), 1 if use_graphs else 0, int(rl_workers))
         h = ctypes.c_void_p()
@@ -43,15 +45,25 @@ class Server:
         self._films = {}
         ctx.own(self)
 
+    def rl_round_spp(self, pixels: int, spp: int, spp_per_round: int = 0) -> int:
+        """Samples per pixel of each round the server gives an rlpath request
+        (as the library chooses them): the film equals render.render's with
+        spp_per_round set to this."""
+        if spp_per_round > 0:
+            return min(spp_per_round, spp)
+        return max(1, min(spp, self.chunk_samples // pixels))
+
     def submit(self, scene: Scene, spp: int, integrator: str = "path", camera=None, max_depth=None,
-               q_grid: int = 32, alpha_scale: float = 1.0, deterministic: bool = False) -> Ticket:
+               q_grid: int = 32, alpha_scale: float = 1.0, deterministic: bool = False,
+               spp_per_round: int = 0) -> Ticket:
         """Queues a render (the parameters of render.render); deterministic
-        applies to rlpath as there."""
+        applies to rlpath as there, spp_per_round to rlpath's rounds (0: as
+        rl_round_spp chooses)."""
         cam = camera if camera is not None else scene.setup.camera
         depth = max_depth if max_depth is not None else (scene.arrays.max_depth if scene.arrays is not None
                                                          else scene.setup.max_depth)
         prm = RenderParams(INTEGRATORS[integrator], int(spp), int(depth), DETERMINISTIC if deterministic else 0,
-                           int(q_grid), float(alpha_scale), 0)
+                           int(q_grid), float(alpha_scale), int(spp_per_round))
         film = Film(np.zeros((cam.height, cam.width, 3), np.float32), np.zeros((cam.height, cam.width), np.float32))
         rid = ctypes.c_int64()
         check(self.ctx.handle, lib().cav_serve_submit(
@@ -136,8 +148,8 @@ def poisson_trace(rng: np.random.Generator, n: int, rate: float, scenes: list, r
 def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
     """Submits each request at its arrival time (from a thread), waits for
     all, and measures arrival -> film-ready latency on the server's clock.
-    `rays` counts the rlpath requests' rays only: pooled requests share the
-    pools' tallies, which cav_serve_close reports."""
+    `rays` stays 0: the requests share the pools' ray tallies, which
+    cav_serve_close reports."""
     tickets = [None] * len(trace)
     start = time.perf_counter()
 
@@ -170,9 +182,11 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
     return res
 
 
-def run_plain(ctx, scenes: list, trace: list) -> TraceResult:
+def run_plain(ctx, scenes: list, trace: list, rl_round=None) -> TraceResult:
     """The plain per-request renderer: one cav_render at a time, in arrival
-    order, each starting when it has arrived and the previous one is done."""
+    order, each starting when it has arrived and the previous one is done.
+    rl_round(req) gives an rlpath request's spp_per_round (None: one round,
+    as device memory allows)."""
     res = TraceResult()
     start = time.perf_counter()
     first = None
@@ -181,8 +195,9 @@ def run_plain(ctx, scenes: list, trace: list) -> TraceResult:
         if now < r.arrival:
             time.sleep(r.arrival - now)
         arrived = start + r.arrival
+        spr = rl_round(r) if (rl_round is not None and r.integrator == "rlpath") else 0
         film, st = render(ctx, scenes[r.scene], r.spp, integrator=r.integrator, camera=r.camera,
-                          max_depth=r.max_depth)
+                          max_depth=r.max_depth, spp_per_round=spr)
         done = time.perf_counter()
         first = arrived if first is None else first
         res.latencies_ms.append((done - arrived) * 1e3)

@@ -386,16 +386,21 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out);
 /* ---------------------------------------------------------------- serving */
 
 /* No counterpart in the reference (it renders one scene per process): a
- * render server over resident scenes with continuous batching. path and
- * directlighting requests of one scene share a persistent ray pool: camera
- * rays of new requests join at every bounce step, tasks of many requests and
- * depths share each launch, and each film is bit-identical to the same
- * request rendered alone by cav_render. rlpath requests (batch-coupled
- * learning) are rendered whole, `rl_workers` at a time, each on a context and
- * stream of its own, overlapping each other and the pools.
+ * render server over resident scenes with continuous batching. All requests
+ * on one scene share a persistent ray pool: camera rays of new requests join
+ * at every bounce step, and tasks of many requests, integrators and depths
+ * share each launch. path and directlighting requests are admitted in chunks
+ * of samples; each film is bit-identical to the same request rendered alone
+ * by cav_render. An rlpath request (whose learning couples the samples of a
+ * bounce) is admitted one round at a time, spp_per_round samples of every
+ * pixel (0: as many as chunk_samples holds, at least 1), with its own Q table
+ * and light statistics; its film is cav_render's with the same
+ * spp_per_round (bit for bit with CAV_RENDER_DETERMINISTIC; at most 32
+ * deterministic rlpath requests per pool at once). The bounce sequence of a
+ * pool can be replayed from graphs built once per pool (use_graphs).
  * Admission is bounded by the pool's task slots and by a device arena for
- * per-request sample and film buffers; the server allocates nothing while
- * serving. All functions are thread-safe. */
+ * per-request sample, film and table buffers; the server allocates nothing
+ * while serving. All functions are thread-safe. */
 typedef struct cav_server cav_server;
 typedef struct {
     int64_t arena_bytes;       /* device arena for admitted requests (0: 16 GiB); 20 B/sample + 160 B/pixel */
@@ -403,8 +408,8 @@ typedef struct {
     int64_t pool_tasks;        /* ray-pool capacity per scene (0: 4M tasks, ~170 B each) */
     int64_t chunk_samples;     /* admission quantum per request per step (0: pool_tasks / 4) */
     int32_t max_steps_ahead;   /* bounce steps the host may enqueue ahead of the device (0: 4) */
-    int32_t use_graphs;        /* replay each pool's bounce sequence from a captured hipGraph */
-    int32_t rl_workers;        /* rlpath requests rendered at once (0: 4) */
+    int32_t use_graphs;        /* replay each pool's bounce sequences from hipGraphs built once per pool */
+    int32_t rl_workers;        /* unused (rlpath requests share the pools) */
 } cav_serve_config;
 enum { CAV_SERVE_QUEUED = 0, CAV_SERVE_RUNNING = 1, CAV_SERVE_DONE = 2, CAV_SERVE_CANCELLED = 3,
        CAV_SERVE_FAILED = 4 };
@@ -412,7 +417,7 @@ typedef struct {
     int32_t status;                   /* CAV_SERVE_* */
     int32_t error;                    /* cav_status when FAILED */
     double submit_ms, start_ms, done_ms;  /* server clock (ms since cav_serve_open); -1 if not reached */
-    int64_t closest_rays, shadow_rays;    /* rlpath requests only (pooled requests share the pool's tallies) */
+    int64_t closest_rays, shadow_rays;    /* 0: requests share their pool's ray tallies (cav_serve_stats) */
 } cav_serve_result;
 typedef struct {
     int64_t pool_closest_rays, pool_shadow_rays, rl_closest_rays, rl_shadow_rays;

@@ -3,9 +3,10 @@
 Mix invariance: every path / directlighting film the server returns equals,
 bit for bit, the same request rendered alone by cav_render, although the
 requests are admitted in chunks and share every launch with other requests
-at other depths; rlpath films (rendered on the server's second stream) match
-the alone render within rel-MSE 1e-4 (their Q proposals land through float
-atomics), and bit for bit in deterministic mode. Also: requests submitted while others run start before those
+at other depths. rlpath requests share the pools too, one whole round (the
+server's round size) at a time, each with its own tables: their films match
+cav_render with the same spp_per_round within rel-MSE 1e-4 (Q proposals
+land through float atomics), and bit for bit in deterministic mode. Also: requests submitted while others run start before those
 finish, cancellation, hipGraph replay, and argument errors."""
 import time
 
@@ -70,8 +71,9 @@ def test_served_films_equal_alone(ctx, two_scenes, graphs):
         assert out["status"] == DONE, out
         arr, sc = two_scenes[s]
         det = integ == "rlpath_det"
+        spr = srv.rl_round_spp(w * h, spp) if integ.startswith("rlpath") else 0
         alone, _ = R.render(ctx, sc, spp, integrator="rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
-                            max_depth=depth, deterministic=det)
+                            max_depth=depth, deterministic=det, spp_per_round=spr)
         np.testing.assert_array_equal(t.film.weight, alone.weight)
         if integ == "rlpath":
             err = rel_mse(t.film.image, alone.image)
@@ -185,7 +187,8 @@ def test_interior_served_films_equal_alone(ctx, interior_pair, graphs):
         assert out["status"] == DONE, out
         arr, sc = interior_pair[s]
         det = integ == "rlpath_det"
+        spr = srv.rl_round_spp(w * h, spp) if det else 0
         alone, _ = R.render(ctx, sc, spp, integrator="rlpath" if det else integ, camera=cam_for(arr, w, h, shift),
-                            max_depth=depth, deterministic=det)
+                            max_depth=depth, deterministic=det, spp_per_round=spr)
         np.testing.assert_array_equal(t.film.weight, alone.weight)
         np.testing.assert_array_equal(t.film.rgb, alone.rgb)
