@@ -35,27 +35,28 @@ os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
-# Algorithmic bytes of one launch of each single-launch LBVH build kernel
-# (30-bit keys, K = 4 B), as functions of N:
-#   bounds        reads the 36 B soup;
-#   morton        reads the soup, writes key + index (+ the 24 B leaf box of
-#                 the canonical model);
-#   karras        reads a key, writes 8 B of children and 8 B of parents;
-#   refit_leaves  (k_leaves) reads the sorted index, the triangle and the
-#                 leaf's parent id, writes the 32 B leaf box/area/cost record
-#                 and the 16 B leaf topology record, then climbs: both
-#                 arrivals at an internal node read its leaf range (8 B each;
-#                 the gate and the children's boxes pass through LDS unless
-#                 the range leaves the workgroup), the second writes the
-#                 node's height word (4 B) and 32 B box record and reads its
-#                 parent id (4 B).
+# Algorithmic bytes of one launch of each LBVH build kernel (30-bit keys,
+# K = 4 B), as functions of N:
+#   bounds           reads the 36 B soup;
+#   morton           reads the soup, writes key + index, clears the sort's
+#                    look-back words (4 passes x 256 x N/4096 words = 1 B/tri);
+#   sort_onesweep    one 8-bit digit pass: reads and writes key + index;
+#   refit_bottom_up  per leaf: reads the sorted index, the triangle and its
+#                    code, writes the 32 B box/area/cost record, the 12 B of
+#                    its topology record it owns and its parent's id (4 B);
+#                    per internal node: the 32 B record, 12 B of topology, the
+#                    parent id and the 4 B height word. The codes of the
+#                    boundaries and the boxes passed between arrivals stay
+#                    in LDS except at the few splits whose leaves span two
+#                    workgroups.
 ALG_BYTES_LAUNCH_30 = {
     "bounds": lambda n: 36 * n,
-    "morton": lambda n: (36 + 4 + 4 + 24) * n,
-    "karras": lambda n: (4 + 8 + 8) * n,
-    "refit_leaves": lambda n: (4 + 36 + 4 + 32 + 16) * n + (n - 1) * (2 * 8 + 4 + 32 + 4),
+    "morton": lambda n: (36 + 4 + 4 + 1) * n,
+    "sort_onesweep": lambda n: 2 * (4 + 4) * n,
+    "refit_bottom_up": lambda n: (4 + 36 + 4 + 32 + 12 + 4) * n + (n - 1) * (32 + 12 + 4 + 4),
 }
 ALG_BYTES_PER_TRI_30 = 248  # whole build, canonical model: S1 68 + S2 68 + S3 20 + S4 92
+TRACE_RECORD_BYTES = 64  # one traversal record (node or triangle) per step
 
 
 def log(msg: str) -> None:
@@ -429,6 +430,7 @@ def main() -> int:
     torch.cuda.synchronize()
     ctx.synchronize()
     bvh.trace_counters(reset=True)  # count stack overflows of the timed traces only
+    bvh.trace_fetches(reset=True)
     recs = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -440,6 +442,7 @@ def main() -> int:
     # every timed ray finished inside the reference's 64-entry stack, or the
     # results are partial and the run fails
     trav = bvh.trace_counters()
+    fetches = bvh.trace_fetches()
     trav["max_leaf_depth"] = bvh.max_depth()
     trav["stack_entries"] = 64
     log(f"[bench] traversal counters {trav}")
@@ -450,6 +453,7 @@ def main() -> int:
         stages[k[3:]] = round(float(np.mean([rec[0][k] for rec in recs])), 4)
     stages["trace_closest"] = round(float(np.mean([rec[2] for rec in recs])), 4)
     stages["trace_shadow"] = round(float(np.mean([rec[3] for rec in recs])), 4)
+    # (the radix tree is emitted by the refit's climb: karras is 0)
     lbvh_ms = sum(stages[k] for k in ("bounds", "morton", "sort", "karras", "refit"))
 
     # per-kernel averages over the timed steps (same launch sequence every step)
@@ -461,24 +465,52 @@ def main() -> int:
     launches = {k: len(v) // args.steps for k, v in kernels.items()}
     totals = {k: per_launch[k] * launches[k] for k in per_launch}
 
-    # roofline of the dominant single-launch LBVH build kernel (HBM-bound model)
-    lbvh_kernels = [k for k in ALG_BYTES_LAUNCH_30 if k in totals and launches.get(k) == 1]
+    # roofline of the LBVH build's dominant kernel (HBM-bound model)
+    lbvh_kernels = [k for k in ALG_BYTES_LAUNCH_30 if k in totals]
     dom = max(lbvh_kernels, key=lambda k: totals[k])
     alg = ALG_BYTES_LAUNCH_30[dom](n)
     achieved = alg / (per_launch[dom] * 1e-3) / 1e9
-    roofline = {
+    lbvh_roofline = {
         "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "avg_launch_ms": round(per_launch[dom], 4),
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "avg_launch_ms": round(per_launch[dom], 4), "launches_per_step": launches[dom],
         "alg_bytes_per_launch": alg,
-        "lbvh_build_frac": round(ALG_BYTES_PER_TRI_30 * n / (lbvh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "build_frac": round(ALG_BYTES_PER_TRI_30 * n / (lbvh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "build_alg_bytes_per_tri": ALG_BYTES_PER_TRI_30,
+    }
+    # roofline of the step's dominant kernel, the traversal: algorithmic bytes
+    # = the 64-byte records the reference's visiting order fetches (counted by
+    # the kernel over the timed steps) + the rays read and the hits written
+    fetch = {k: v / args.steps for k, v in fetches.items()}
+    trav_bytes = {
+        "closest": fetch["closest"] * TRACE_RECORD_BYTES + r * (24 + 8),
+        "occluded": fetch["occluded"] * TRACE_RECORD_BYTES + r * (24 + 4 + 1),
+    }
+    trav_ms = {"closest": stages["trace_closest"], "occluded": stages["trace_shadow"]}
+    tdom = max(trav_ms, key=lambda k: trav_ms[k])
+    t_ach = trav_bytes[tdom] / (trav_ms[tdom] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": f"k_trace<{'closest' if tdom == 'closest' else 'any'}>",
+        "achieved": round(t_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(t_ach / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_ms": round(trav_ms[tdom], 4), "alg_bytes_per_launch": int(trav_bytes[tdom]),
+        "records_per_ray": round(fetch[tdom] / r, 1),
+        "note": "algorithmic bytes count every record the reference's visiting order fetches, "
+                "whether L2 or HBM serves it",
+        "lbvh_build_frac": lbvh_roofline["build_frac"],
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            tr = json.load(f).get(dom)
-        if tr:
-            roofline["traffic"] = tr
+            tr = json.load(f)
+        if tr.get(roofline["kernel"]):
+            roofline["traffic"] = tr[roofline["kernel"]]
+            roofline["traffic_source"] = tr.get("_source", "profiles/traffic.json")
+        if tr.get(dom):
+            lbvh_roofline["traffic"] = tr[dom]
+            lbvh_roofline["traffic_source"] = tr.get("_source", "profiles/traffic.json")
+    trav["records_per_ray_closest"] = round(fetch["closest"] / r, 1)
+    trav["records_per_ray_occluded"] = round(fetch["occluded"] / r, 1)
 
     value = 2 * r / (ms_step * 1e-3) / 1e6
     out = {
@@ -515,6 +547,7 @@ def main() -> int:
         "hit_fraction": round(float((hit_prim >= 0).float().mean().item()), 4),
         "visible_fraction": round(float(vis.float().mean().item()), 4),
         "roofline": roofline,
+        "lbvh_roofline": lbvh_roofline,
         "traversal": trav,
         "trace_launch": launch,
         "cpu_baseline": None,

@@ -21,6 +21,7 @@ SOURCES = [
     "radix_sort.hip",
     "lbvh.hip",
     "levels.hip",
+    "bottomup.hip",
     "treelet.hip",
     "traverse.hip",
     "render.hip",

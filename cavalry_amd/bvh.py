@@ -175,6 +175,13 @@ class BVH:
         return {"overflow_closest": int(out[0]), "overflow_occluded": int(out[1]),
                 "max_stack_closest": int(out[2]), "max_stack_occluded": int(out[3])}
 
+    def trace_fetches(self, reset: bool = False) -> dict:
+        """64-byte records (node visits + triangle tests) fetched by the
+        closest-hit and any-hit launches since the last reset."""
+        out = np.zeros(2, np.uint64)
+        check(self.ctx.handle, lib().cav_trace_fetches(self.ctx.handle, self.handle, _ptr(out), 1 if reset else 0))
+        return {"closest": int(out[0]), "occluded": int(out[1])}
+
     def max_depth(self) -> int:
         """Deepest leaf, in edges from the root."""
         d = ctypes.c_int32()

@@ -1,0 +1,283 @@
+// LBVH topology and refit in one bottom-up pass over the sorted leaves.
+//
+// Reference: buildRadixTree + computeBounds (BVH/BVH.cu:122-259), with the
+// leaf records of fillLeafBoundingBoxes (BVH/BVH.cu:46-56).
+//
+// The reference emits the Karras radix tree top-down (one thread per internal
+// node, binary searches over the codes) and then refits it bottom-up. Both
+// describe one tree: the binary radix tree over the sorted (code, index)
+// pairs. With delta(j) the common-prefix length of sorted positions j and j+1
+// (the reference's commonPrefixLength, index bits breaking equal codes),
+//   * the node whose leaves are [l, r] has its split at the boundary of
+//     smallest delta inside the range, and its parent's split at whichever of
+//     l-1 and r has the larger delta (they are never equal; l = 0 or r = N-1
+//     leaves only the other side): it is the parent's left child when that is
+//     r, the right child when it is l-1;
+//   * the reference numbers internal nodes by one end of their range: a left
+//     child is node r, a right child node l, the root node 0; a child whose
+//     range is one leaf k is leaf N-1+k.
+// So one climb from every leaf, the second arrival at each split continuing
+// (the reference's computeBounds gate), produces the reference's node ids,
+// children, parents and boxes without the top-down searches.
+//
+// Each workgroup takes 256 consecutive sorted leaves. A split whose node's
+// leaves all lie in the workgroup (nearer boundaries of smaller delta on both
+// sides inside the workgroup, found by a min-scan over the workgroup's deltas)
+// is gated in LDS, and the first arrival leaves its child's box there. The
+// few other splits use a 64-bit word in HBM: the first arrival writes its
+// child's box and topology record with agent-scope stores, waits for them to
+// complete, then adds
+// its far range end and height to the word; the second arrival reads the
+// word, clears it for the next build and loads the box with agent-scope
+// loads. Every access that crosses workgroups is an agent-scope atomic, so
+// no cache write-back or invalidate is needed. Results do not depend on
+// which arrival is second: both children are final when their parent is
+// computed, as in the reference.
+#include "dev_common.h"
+#include "lbvh.h"
+
+namespace cav {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ v3 load_tri_v(const float* __restrict__ t, int64_t tri, int j) {
+    const float* p = t + tri * 9 + j * 3;
+    return mk3(p[0], p[1], p[2]);
+}
+
+// BVH/BVH.cu:101-120 for adjacent positions j, j+1 (codes a, b).
+template <typename K>
+__device__ __forceinline__ int delta_of(K a, K b, int keybits, uint32_t j) {
+    if (a != b) {
+        const int w = (int)sizeof(K) * 8;
+        const int clz = sizeof(K) == 8 ? __clzll((long long)(a ^ b)) : __clz((int)(a ^ b));
+        return keybits - w + clz;
+    }
+    return keybits + __clz((int)(j ^ (j + 1u)));
+}
+
+// computeBounds (BVH/BVH.cu:217-259)
+__device__ __forceinline__ NodeBox combine(const NodeBox& bl, const NodeBox& br) {
+    const v3 lo = mk3(fmn(bl.a.x, br.a.x), fmn(bl.a.y, br.a.y), fmn(bl.a.z, br.a.z));
+    const v3 hi = mk3(fmx(bl.b.x, br.b.x), fmx(bl.b.y, br.b.y), fmx(bl.b.z, br.b.z));
+    const float sa = surface_area(lo, hi);
+    return NodeBox{make_float4(lo.x, lo.y, lo.z, sa),
+                   make_float4(hi.x, hi.y, hi.z, internal_cost3(sa, bl.b.w, br.b.w))};
+}
+
+// A node box as four 64-bit agent-scope atomic stores / loads: coherent for
+// a reader on another XCD without flushing any cache.
+__device__ __forceinline__ void store_box_agent(NodeBox* p, const NodeBox& b) {
+    uint64_t w[4];
+    __builtin_memcpy(w, &b, 32);
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) __hip_atomic_store(q + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ NodeBox load_box_agent(const NodeBox* p) {
+    uint64_t w[4];
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    NodeBox b;
+    __builtin_memcpy(&b, w, 32);
+    return b;
+}
+
+__device__ __forceinline__ void store_rec_agent(int4* p, int4 v) {
+    uint64_t w[2];
+    __builtin_memcpy(w, &v, 16);
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    __hip_atomic_store(q, w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, w[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int4 load_rec_agent(const int4* p) {
+    uint64_t w[2];
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+    w[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w[1] = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int4 v;
+    __builtin_memcpy(&v, w, 16);
+    return v;
+}
+
+// Inclusive min-scan of one value per thread over the workgroup.
+__device__ __forceinline__ int block_min_scan(int v, int* wmin) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, d, 64);
+        if (lane >= d) v = min(v, o);
+    }
+    if (lane == 63) wmin[wave] = v;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) v = min(v, wmin[w]);
+    __syncthreads();
+    return v;
+}
+
+constexpr uint64_t kArrive = 1ull << 48;
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ codes, int n, int keybits,
+                                                      const uint32_t* __restrict__ order,
+                                                      const float* __restrict__ tris,
+                                                      NodeBox* nbox, int4* topo, uint32_t* hw,
+                                                      unsigned long long* gate64) {
+    __shared__ K skey[kBlock + 2];        // codes of positions k0-1 .. k0+256
+    __shared__ int sdel[kBlock + 1];      // delta of boundaries k0-1 .. k0+255 (-1: none)
+    __shared__ int wmin[kBlock / 64];
+    __shared__ uint8_t slocal[kBlock];    // split k0+i has all its leaves in this workgroup
+    __shared__ uint32_t gate[kBlock];     // arrivals << 16 | height, per local split
+    __shared__ NodeBox sbox[2 * kBlock];  // per local split: left, right child's box
+    __shared__ int sfar[2 * kBlock];      // per local split: left child's first, right child's last leaf
+    __shared__ int4 srec[2 * kBlock];     // per local split: left, right child's {left, right, -, prim}
+
+    const int k0 = blockIdx.x * kBlock;
+    const int t = threadIdx.x;
+    for (int i = t; i < kBlock + 2; i += kBlock) {
+        const int64_t j = (int64_t)k0 - 1 + i;
+        skey[i] = (j >= 0 && j < n) ? codes[j] : (K)0;
+    }
+    gate[t] = 0u;
+    __syncthreads();
+    for (int i = t; i <= kBlock; i += kBlock) {
+        const int64_t j = (int64_t)k0 - 1 + i;
+        sdel[i] = (j < 0 || j >= n - 1) ? -1 : delta_of<K>(skey[i], skey[i + 1], keybits, (uint32_t)j);
+    }
+    __syncthreads();
+    // split g = k0 + t: a smaller delta among boundaries k0-1 .. g-1 (prefix
+    // min of sdel[0..t]) and among g+1 .. k0+255 (suffix min of sdel[t+2..256])
+    const int pre = block_min_scan(sdel[t], wmin);
+    const int suf_rev = block_min_scan(sdel[kBlock - t], wmin);  // min of sdel[256-t .. 256]
+    __shared__ int ssuf[kBlock];
+    ssuf[kBlock - 1 - t] = suf_rev;  // ssuf[i] = min of sdel[i+1 .. 256]
+    __syncthreads();
+    {
+        const int g = k0 + t;
+        const int dg = sdel[t + 1];
+        const int suf = t + 1 < kBlock ? ssuf[t + 1] : 0x7fffffff;  // min of sdel[t+2 .. 256]
+        slocal[t] = (t < kBlock - 1 && g <= n - 2 && pre < dg && suf < dg) ? 1 : 0;
+    }
+    __syncthreads();
+
+    const int k = k0 + t;
+    if (k >= n) return;
+    auto delta = [&](int j) -> int {  // delta of boundary j, -1 outside [0, n-2]
+        if (j < 0 || j >= n - 1) return -1;
+        const int i = j - k0 + 1;
+        if (i >= 0 && i <= kBlock) return sdel[i];
+        return delta_of<K>(codes[j], codes[j + 1], keybits, (uint32_t)j);
+    };
+
+    // the leaf (BVH/BVH.cu:46-56)
+    const int tri = (int)order[k];
+    const v3 a = load_tri_v(tris, tri, 0), b = load_tri_v(tris, tri, 1), c = load_tri_v(tris, tri, 2);
+    const v3 lo = mk3(fmn(a.x, fmn(b.x, c.x)), fmn(a.y, fmn(b.y, c.y)), fmn(a.z, fmn(b.z, c.z)));
+    const v3 hi = mk3(fmx(a.x, fmx(b.x, c.x)), fmx(a.y, fmx(b.y, c.y)), fmx(a.z, fmx(b.z, c.z)));
+    const float sa0 = surface_area(lo, hi);
+    NodeBox cb{make_float4(lo.x, lo.y, lo.z, sa0), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa0))};
+    int cur = n - 1 + k;
+    // cur's topology record but its parent: {left, right, -, prim}. Each
+    // node's whole record is written once, by the arrival that completes its
+    // parent (the only one that knows the parent's id).
+    int4 crec = make_int4(-1, -1, 0, tri);
+    int l = k, r = k;
+    uint32_t h = 0;
+    // side of cur in its parent: cur is the left child iff delta(r) > delta(l-1)
+    bool left = delta(r) > delta(l - 1);
+    for (int step = 0; step < n; ++step) {
+        const int g = left ? r : l - 1;  // the parent's split
+        const int far = left ? l : r;
+        const uint32_t hp = min(h + 1u, 0xffffu);
+        uint32_t hs;
+        int ofar;
+        NodeBox sib;
+        int4 srec_o;
+        const int li = g - k0;
+        const bool local = li >= 0 && li < kBlock && slocal[li];
+        if (local) {
+            nbox[cur] = cb;
+            sbox[2 * li + (left ? 0 : 1)] = cb;
+            sfar[2 * li + (left ? 0 : 1)] = far;
+            srec[2 * li + (left ? 0 : 1)] = crec;
+            __threadfence_block();
+            const uint32_t old = atomicAdd(&gate[li], (1u << 16) | hp);
+            if ((old >> 16) == 0u) return;  // first arrival: the sibling continues
+            __threadfence_block();
+            hs = old & 0xffffu;
+            sib = sbox[2 * li + (left ? 1 : 0)];
+            ofar = sfar[2 * li + (left ? 1 : 0)];
+            srec_o = srec[2 * li + (left ? 1 : 0)];
+        } else {
+            // the box, and the topology record without its parent, for the
+            // sibling's arrival (the record is rewritten whole by whichever
+            // arrival completes the parent)
+            store_box_agent(nbox + cur, cb);
+            store_rec_agent(topo + cur, crec);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);  // the stores have completed
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint64_t add = kArrive | ((uint64_t)hp << 32) | (uint32_t)far;
+            const uint64_t old = __hip_atomic_fetch_add(gate64 + g, (unsigned long long)add, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if ((old >> 48) == 0u) return;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __hip_atomic_store(gate64 + g, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            hs = (uint32_t)(old >> 32) & 0xffffu;
+            ofar = (int)(uint32_t)old;
+        }
+        // the parent: leaves [lp, rp], split g
+        const int lp = left ? l : ofar, rp = left ? ofar : r;
+        const int lid = lp == g ? n - 1 + g : g;
+        const int rid = g + 1 == rp ? n - 1 + rp : g + 1;
+        if (!local) {  // the sibling's, written before its arrival
+            sib = load_box_agent(nbox + (left ? rid : lid));
+            srec_o = load_rec_agent(topo + (left ? rid : lid));
+        }
+        const NodeBox pb = left ? combine(cb, sib) : combine(sib, cb);
+        h = max(h + 1u, hs);
+        const bool root = lp == 0 && rp == n - 1;
+        bool pleft = false;
+        int pid = 0;
+        if (!root) {
+            pleft = delta(rp) > delta(lp - 1);
+            pid = pleft ? rp : lp;
+        }
+        const int4 lrec = left ? crec : srec_o, rrec = left ? srec_o : crec;
+        topo[lid] = make_int4(lrec.x, lrec.y, pid, lrec.w);
+        topo[rid] = make_int4(rrec.x, rrec.y, pid, rrec.w);
+        hw[pid] = (2u << 16) | min(h, 0xffffu);
+        crec = make_int4(lid, rid, 0, -1);
+        if (root) {
+            topo[0] = make_int4(lid, rid, 0, -1);
+            nbox[0] = pb;
+            return;
+        }
+        cur = pid;
+        l = lp;
+        r = rp;
+        cb = pb;
+        left = pleft;
+    }
+}
+
+}  // namespace
+
+hipError_t lbvh_bottom_up(LbvhBuffers& b, const void* codes, int keybits, const float* tris, int n,
+                          hipStream_t s, Marker* mk) {
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (keybits > 32)
+        hipLaunchKernelGGL(k_bottom_up<uint64_t>, dim3(nb), dim3(kBlock), 0, s,
+                           static_cast<const uint64_t*>(codes), n, keybits, b.order, tris, b.nbox, b.topo,
+                           b.visited, b.gate64);
+    else
+        hipLaunchKernelGGL(k_bottom_up<uint32_t>, dim3(nb), dim3(kBlock), 0, s,
+                           static_cast<const uint32_t*>(codes), n, keybits, b.order, tris, b.nbox, b.topo,
+                           b.visited, b.gate64);
+    mark(mk, "refit_bottom_up");
+    return hipGetLastError();
+}
+
+}  // namespace cav

@@ -50,10 +50,10 @@ int dev_alloc(cav_ctx* ctx, void** p, size_t bytes, const char* what) {
 
 static void bvh_free(cav_bvh* h) {
     void* ptrs[] = {h->b.keys[0], h->b.keys[1], h->b.vals[0], h->b.vals[1], h->b.hist, h->b.counts,
-                    h->b.offsets, h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,
+                    h->b.scene_ord, h->b.nbox, h->b.topo, h->b.visited, h->b.ref,
                     h->b.partial, h->b.leaf_ids, h->b.counters, h->rec, h->root, h->mesh,
                     h->b.lvl_list, h->b.lvl_cnt, h->b.lvl_cursor, h->b.lvl_off,
-                    h->b.bounds_part};
+                    h->b.bounds_part, h->b.gate64};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->b.h_lvl) (void)hipHostFree(h->b.h_lvl);
@@ -70,14 +70,14 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {&h->b.keys[1], sizeof(uint64_t) * n, "keys1"},
         {(void**)&h->b.vals[0], sizeof(uint32_t) * n, "vals0"},
         {(void**)&h->b.vals[1], sizeof(uint32_t) * n, "vals1"},
-        {(void**)&h->b.hist, sizeof(uint32_t) * 8 * 256, "hist"},
-        {(void**)&h->b.counts, sizeof(uint32_t) * 256 * tiles, "counts"},
-        {(void**)&h->b.offsets, sizeof(uint32_t) * 256 * tiles, "offsets"},
+        {(void**)&h->b.hist, sizeof(uint32_t) * (kSortHistWords + 16), "sort histograms"},
+        {(void**)&h->b.counts, sizeof(uint32_t) * 8 * 256 * tiles, "sort look-back"},
         {(void**)&h->b.scene_ord, 64, "scene"},
         {(void**)&h->b.bounds_part, sizeof(uint32_t) * 6 * 1024, "bounds partials"},
         {(void**)&h->b.nbox, sizeof(NodeBox) * m, "nbox"},
         {(void**)&h->b.topo, sizeof(int4) * m, "topo"},
         {(void**)&h->b.visited, sizeof(uint32_t) * m, "visited"},
+        {(void**)&h->b.gate64, sizeof(uint64_t) * n, "split gates"},
         {(void**)&h->b.ref, sizeof(int32_t) * m, "ref"},
         {(void**)&h->b.partial, sizeof(uint32_t) * (lbvh_partials(m) + 1), "partial"},
         {(void**)&h->b.leaf_ids, sizeof(uint32_t) * n, "leaf_ids"},
@@ -93,6 +93,8 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         int rc = dev_alloc(ctx, r.p, r.bytes, r.what);
         if (rc != CAV_OK) return rc;
     }
+    // the bottom-up build leaves every split gate at zero for the next build
+    HIPCHK(ctx, hipMemsetAsync(h->b.gate64, 0, sizeof(uint64_t) * n, ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&h->b.h_lvl, sizeof(uint32_t) * (kMaxLevels + 2), hipHostMallocDefault));
     h->b.h_off = new uint32_t[kMaxLevels + 2]();
     for (auto& e : h->mk.ev) HIPCHK(ctx, hipEventCreate(&e));
@@ -134,7 +136,9 @@ static int bvh_run(cav_ctx* ctx, cav_bvh* h, const float* tris_dev, const float*
     }
     HIPCHK(ctx, lbvh_build(h->b, tris_dev, n, morton_bits, s, mk));
     float root_lbvh[8];
+    uint32_t sort_err = 0;
     HIPCHK(ctx, hipMemcpyAsync(root_lbvh, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(&sort_err, h->b.hist + kSortHistWords + 8, 4, hipMemcpyDeviceToHost, s));
     mark(mk, "readback");
     for (int r = 0; r < opt_rounds; ++r) HIPCHK(ctx, treelet_round_impl(h->b, n, r == 0, s, mk));
     if (opt_rounds > 0) HIPCHK(ctx, lbvh_refs(h->b, n, s, mk, "flatten_refs"));
@@ -142,6 +146,7 @@ static int bvh_run(cav_ctx* ctx, cav_bvh* h, const float* tris_dev, const float*
     float root_final[8];
     HIPCHK(ctx, hipMemcpyAsync(root_final, h->b.nbox, sizeof(NodeBox), hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    if (sort_err) return fail(ctx, CAV_ERR_HIP, "radix sort: a look-back wait exceeded its bound");
     cav_bvh_stats st{};
     st.n = n;
     st.morton_bits = morton_bits;
@@ -390,10 +395,19 @@ int cav_trace_counters(cav_ctx* ctx, const cav_bvh* h, uint32_t* out4, int32_t r
     return CAV_OK;
 }
 
+int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out2, int32_t reset) {
+    if (!ctx || !h || !out2) return CAV_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(out2, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (reset) HIPCHK(ctx, hipMemsetAsync(h->b.counters + 16, 0, 16, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CAV_OK;
+}
+
 int cav_bvh_max_depth(cav_ctx* ctx, const cav_bvh* h, int32_t* depth) {
     if (!ctx || !h || !depth) return CAV_ERR_INVALID;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    // counters[8] is free scratch (the traversal uses words 0..7)
+    // counters[8] is free scratch (the traversal uses words 0..7 and 16..19)
     uint32_t* word = h->b.counters + 8;
     HIPCHK(ctx, levels_max_depth(h->b, h->n, word, ctx->stream));
     uint32_t d = 0;

@@ -30,23 +30,23 @@ struct Marker {
 };
 inline void mark(Marker* m, const char* nm) { if (m) m->mark(nm); }
 
+constexpr int kSortHistWords = 8 * 256;  // digit counts of up to 8 radix passes
 constexpr int kMaxLevels = 4096;  // node heights supported by the level scheduler
 // top levels holding at most this many nodes run in one workgroup
-constexpr int kTailNodes = 16384;        // refit
 constexpr int kTreeletTailNodes = 256;   // treelet rounds
 
 // Device buffers of one BVH build (library-owned; sized for n triangles).
 struct LbvhBuffers {
     void* keys[2] = {nullptr, nullptr};       // u32 or u64 Morton codes
     uint32_t* vals[2] = {nullptr, nullptr};   // triangle indices
-    uint32_t* hist = nullptr;                 // 8 * 256
-    uint32_t* counts = nullptr;               // 256 * tiles
-    uint32_t* offsets = nullptr;              // 256 * tiles
+    uint32_t* hist = nullptr;                 // kSortHistWords digit counts, 8 tile counters, error word
+    uint32_t* counts = nullptr;               // 8 * 256 * tiles sort look-back words
     uint32_t* scene_ord = nullptr;            // 6 order-preserving ints
     uint32_t* bounds_part = nullptr;          // per-workgroup partial bounds (6 each)
     NodeBox* nbox = nullptr;                  // 2n-1
     int4* topo = nullptr;                     // 2n-1
     uint32_t* visited = nullptr;              // 2n-1 (arrivals << 16 | height)
+    unsigned long long* gate64 = nullptr;     // n-1 split gates of the bottom-up build (zero between builds)
     int32_t* ref = nullptr;                   // 2n-1 (restructured trees)
     uint32_t* partial = nullptr;              // scan partials
     uint32_t* leaf_ids = nullptr;             // n (treelet round start list)
@@ -73,11 +73,14 @@ hipError_t lbvh_flatten(LbvhBuffers& b, const float* tris, const int32_t* mesh, 
                         Marker* mk);
 int64_t lbvh_partials(int64_t m);
 
-// levels.hip: height climb + bucketing + level-ordered refit
-hipError_t levels_from_leaves(LbvhBuffers& b, const float* tris, int n, hipStream_t s, Marker* mk);
+// bottomup.hip: node ids, children, parents, boxes and heights of the
+// radix tree over the sorted codes, in one climb from the leaves
+hipError_t lbvh_bottom_up(LbvhBuffers& b, const void* codes, int keybits, const float* tris, int n,
+                          hipStream_t s, Marker* mk);
+
+// levels.hip: height climb + bucketing by height (treelet rounds)
 hipError_t levels_from_list(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
 hipError_t levels_bucket(LbvhBuffers& b, int n, hipStream_t s, Marker* mk);
-hipError_t levels_refit(LbvhBuffers& b, hipStream_t s, Marker* mk);
 // deepest leaf (edges from the root) of the current tree into *out (device word)
 hipError_t levels_max_depth(const LbvhBuffers& b, int n, uint32_t* out, hipStream_t s);
 
@@ -89,7 +92,9 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // counters[1] (occluded) count stack overflows and step-cap exits (either
 // means the results are partial); counters[2..5] hold the ray dispensers;
 // counters[6] (closest) and counters[7] (occluded) keep the deepest stack any
-// ray reached (atomic max, never reset by a launch).
+// ray reached (atomic max, never reset by a launch); the 64-bit words at
+// counters[16] (closest) and counters[18] (occluded) add up the 64-byte
+// records every launch fetched (never reset by a launch).
 // Launch shape of the persistent traversal kernels. Each ray's node order is
 // fixed, so results do not depend on it; the autotuner picks it.
 struct TraceLaunch {

@@ -155,93 +155,62 @@ __device__ __forceinline__ uint32_t morton_code<uint32_t>(v3 c) {
 }
 
 // BVH/BVH.cu:79-92: code of the triangle-box centroid normalised by the scene
-// box; values[i] = i so the stable sort yields the original index.
+// box; values[i] = i so the stable sort yields the original index. Also
+// counts the codes' 8-bit digits for every radix-sort pass (equal digits of a
+// wave are found by eight ballots and added once by their lowest lane; one
+// global add per digit and workgroup, over a bounded grid) and clears the
+// sort's look-back words.
+constexpr int kMortonBlocks = 1024;
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tris, int64_t n,
                                                    const uint32_t* __restrict__ ob,
                                                    K* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+                                                   uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ hist, int passes,
+                                                   uint32_t* __restrict__ zero, int64_t nzero) {
+    __shared__ uint32_t h[8 * 256];
+    for (int j = threadIdx.x; j < passes * 256; j += kBlock) h[j] = 0u;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nzero; j += stride) zero[j] = 0u;
+    __syncthreads();
     const v3 slo = mk3(ord2f(ob[0]), ord2f(ob[1]), ord2f(ob[2]));
     const v3 shi = mk3(ord2f(ob[3]), ord2f(ob[4]), ord2f(ob[5]));
     const v3 sc = scale3(add3(slo, shi), 0.5f);
     const v3 se = sub3(shi, slo);
-    v3 lo, hi;
-    tri_box(load_v(tris, i, 0), load_v(tris, i, 1), load_v(tris, i, 2), lo, hi);
-    const v3 c = div3(sub3(scale3(add3(lo, hi), 0.5f), sc), se);
-    keys[i] = morton_code<K>(c);
-    vals[i] = (uint32_t)i;
+    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    // every wave runs the same trip count (the ballots need all lanes)
+    const int64_t trips = (n + stride - 1) / stride;
+    for (int64_t it = 0; it < trips; ++it) {
+        const int64_t i = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        const bool valid = i < n;
+        K key = 0;
+        if (valid) {
+            v3 lo, hi;
+            tri_box(load_v(tris, i, 0), load_v(tris, i, 1), load_v(tris, i, 2), lo, hi);
+            const v3 c = div3(sub3(scale3(add3(lo, hi), 0.5f), sc), se);
+            key = morton_code<K>(c);
+            keys[i] = key;
+            vals[i] = (uint32_t)i;
+        }
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t d = (uint32_t)(key >> (8 * p)) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            if (valid && (peers & lt) == 0) atomicAdd(&h[p * 256 + d], (uint32_t)__popcll(peers));
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < passes * 256; j += kBlock)
+        if (h[j]) atomicAdd(&hist[j], h[j]);
 }
 
-// BVH/BVH.cu:101-120 with W-bit key words holding keybits-bit codes.
-template <typename K>
-__device__ __forceinline__ int prefix_len(const K* __restrict__ codes, int keybits, uint32_t i,
-                                          uint32_t j) {
-    const K a = codes[i], b = codes[j];
-    if (a != b) {
-        const int w = (int)sizeof(K) * 8;
-        const int clz = sizeof(K) == 8 ? __clzll((long long)(a ^ b)) : __clz((int)(a ^ b));
-        return keybits - w + clz;
-    }
-    const uint32_t y = i ^ j;
-    if (y == 0) return keybits + 32;
-    return keybits + __clz((int)y);
-}
-
-// BVH/BVH.cu:122-215: internal node i of the Karras radix tree.
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_karras(const K* __restrict__ codes, int n,
-                                                   int keybits, int4* __restrict__ topo,
-                                                   uint32_t* __restrict__ visited,
-                                                   int2* __restrict__ rng) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = 1;
-    if (i > 0) {
-        int a = prefix_len(codes, keybits, i, i + 1) - prefix_len(codes, keybits, i, i - 1);
-        d = a >= 0 ? 1 : -1;
-    }
-    const int dmin = i > 0 ? prefix_len(codes, keybits, i, i - d) : 0;
-    uint32_t maxlen = 2;
-    while (true) {
-        const uint32_t j = (uint32_t)i + maxlen * (uint32_t)d;
-        if (!(j < (uint32_t)n)) break;
-        if (!(prefix_len(codes, keybits, i, j) > dmin)) break;
-        maxlen *= 2;
-    }
-    uint32_t len = 0;
-    for (int p = 1;; ++p) {
-        const uint32_t t = maxlen >> p;
-        if (t == 0) break;
-        const int j = (int)((uint32_t)i + (len + t) * (uint32_t)d);
-        if (j >= 0 && j < n && prefix_len(codes, keybits, i, j) > dmin) len += t;
-    }
-    int j = (int)((uint32_t)i + len * (uint32_t)d);
-    if (i == 0) j = n - 1;
-    const int plen = prefix_len(codes, keybits, i, j);
-    uint32_t split = 0;
-    for (int p = 1;; ++p) {
-        const uint32_t t = (uint32_t)ceilf((float)len / (float)(1 << p));
-        const int k = (int)((uint32_t)i + (split + t) * (uint32_t)d);
-        if (k >= 0 && k < n && prefix_len(codes, keybits, i, k) > plen) split += t;
-        if (t <= 1) break;
-    }
-    const int spos = (int)((uint32_t)i + split * (uint32_t)d) + min(0, d);
-    const int lc = spos, rc = spos + 1;
-    const bool lleaf = min(i, j) == lc;
-    const bool rleaf = (max(i, j) == rc) || (rc == n - 1);
-    const int lid = lleaf ? (n - 1 + lc) : lc;
-    const int rid = rleaf ? (n - 1 + rc) : rc;
-    int* me = reinterpret_cast<int*>(&topo[i]);
-    me[0] = lid;
-    me[1] = rid;
-    me[3] = -1;
-    reinterpret_cast<int*>(&topo[lid])[2] = i;
-    reinterpret_cast<int*>(&topo[rid])[2] = i;
-    if (i == 0) me[2] = 0;
-    visited[i] = 0;
-    rng[i] = make_int2(min(i, j), max(i, j));  // the leaves under node i
+__global__ void k_zero_words(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 // ---- device-wide exclusive scan of leaf flags (restructured trees) ----
@@ -378,32 +347,30 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const NodeBox* __restrict__ 
 template <typename K>
 hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hipStream_t s,
                       Marker* mk) {
-    SortBuffers<K> sb;
-    sb.keys[0] = reinterpret_cast<K*>(b.keys[0]);
-    sb.keys[1] = reinterpret_cast<K*>(b.keys[1]);
-    sb.vals[0] = b.vals[0];
-    sb.vals[1] = b.vals[1];
-    sb.hist = b.hist;
-    sb.counts = b.counts;
-    sb.offsets = b.offsets;
-    const int nb = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_morton<K>, dim3(nb), dim3(kBlock), 0, s, tris, (int64_t)n,
-                       b.scene_ord, sb.keys[0], sb.vals[0]);
+    const int passes = (keybits + 7) / 8;
+    const int64_t tiles = sort_tiles(n);
+    OnesweepBuffers<K> ob;
+    ob.keys[0] = reinterpret_cast<K*>(b.keys[0]);
+    ob.keys[1] = reinterpret_cast<K*>(b.keys[1]);
+    ob.vals[0] = b.vals[0];
+    ob.vals[1] = b.vals[1];
+    ob.hist = b.hist;
+    ob.status = b.counts;
+    ob.tile_ctr = b.hist + kSortHistWords;
+    ob.err = b.hist + kSortHistWords + 8;
+    int nb = (n + kBlock - 1) / kBlock;
+    if (nb > kMortonBlocks) nb = kMortonBlocks;
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBlock), 0, s, b.hist, kSortHistWords + 16);
+    hipLaunchKernelGGL(k_morton<K>, dim3(nb), dim3(kBlock), 0, s, tris, (int64_t)n, b.scene_ord, ob.keys[0],
+                       ob.vals[0], b.hist, passes, b.counts, (int64_t)passes * 256 * tiles);
     mark(mk, "morton");
-    hipError_t e = radix_sort_pairs(sb, n, keybits, s, mk);
+    hipError_t e = onesweep_sort_pairs(ob, n, keybits, s, mk);
     if (e != hipSuccess) return e;
-    b.sorted = sb.result;
-    const K* codes = sb.keys[sb.result];
-    b.order = sb.vals[sb.result];
-    const int nk = (n - 1 + kBlock - 1) / kBlock;
-    // node ranges for the refit's climb, in ref (free until a treelet round)
-    hipLaunchKernelGGL(k_karras<K>, dim3(nk), dim3(kBlock), 0, s, codes, n, keybits, b.topo,
-                       b.visited, reinterpret_cast<int2*>(b.ref));
-    mark(mk, "karras");
-    // refit (BVH/BVH.cu:217-259): leaf boxes + heights, then level by level
-    if ((e = levels_from_leaves(b, tris, n, s, mk)) != hipSuccess) return e;
-    if ((e = levels_bucket(b, n, s, mk)) != hipSuccess) return e;
-    return levels_refit(b, s, mk);
+    b.sorted = ob.result;
+    const K* codes = ob.keys[ob.result];
+    b.order = ob.vals[ob.result];
+    // radix tree + refit (BVH/BVH.cu:122-259) in one climb from the leaves
+    return lbvh_bottom_up(b, codes, keybits, tris, n, s, mk);
 }
 
 }  // namespace

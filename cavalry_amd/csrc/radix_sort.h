@@ -18,6 +18,18 @@ struct SortBuffers {
     int result = 0;                          // index of the buffer holding the output
 };
 
+// Onesweep sort (one kernel per 8-bit digit pass, decoupled look-back).
+template <typename K>
+struct OnesweepBuffers {
+    K* keys[2] = {nullptr, nullptr};         // keys[0] holds the input
+    uint32_t* vals[2] = {nullptr, nullptr};  // vals[0] holds the input
+    const uint32_t* hist = nullptr;          // passes * 256 digit counts of the keys
+    uint32_t* status = nullptr;              // passes * 256 * tiles look-back words, zero before the sort
+    uint32_t* tile_ctr = nullptr;            // passes tile counters, zero before the sort
+    uint32_t* err = nullptr;                 // nonzero: a look-back wait exceeded its bound
+    int result = 0;                          // index of the buffer holding the output
+};
+
 struct Marker;
 int64_t sort_tiles(int64_t n);
 int sort_radix();
@@ -25,6 +37,10 @@ hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, h
                             Marker* mk);
 hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
                             Marker* mk);
+hipError_t onesweep_sort_pairs(OnesweepBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
+                               Marker* mk);
+hipError_t onesweep_sort_pairs(OnesweepBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
+                               Marker* mk);
 // launched through a sink (a stream or a graph under construction)
 hipError_t radix_sort_pairs(SortBuffers<uint64_t>& b, int64_t n, int key_bits, LaunchSink& ls);
 

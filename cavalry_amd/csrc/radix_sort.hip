@@ -11,6 +11,8 @@
 //   downsweep : wave64 ballot multisplit ranks each key inside its tile (stable
 //               in element order), stages the tile in LDS in digit order and
 //               writes each digit run contiguously to its global offset.
+// The LBVH build uses the single-kernel-per-pass form further down
+// (onesweep), whose digit histograms the Morton kernel accumulates.
 #include "dev_common.h"
 #include "lbvh.h"
 #include "radix_sort.h"
@@ -228,6 +230,172 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
     }
 }
 
+// ---------------------------------------------------------------- onesweep
+// One kernel per digit pass (Merrill & Garland's single-pass radix sort with
+// decoupled look-back). Workgroups take tiles in launch order from a counter,
+// so every tile a workgroup waits on belongs to a workgroup already running.
+// Per tile: rank the keys by digit (the ballot multisplit above), publish the
+// tile's per-digit counts, add up predecessors' counts back to the first
+// published inclusive prefix, publish this tile's inclusive prefix, and
+// scatter the tile through LDS so each digit run is written contiguously.
+// Look-back words: flag << 30 | count (flag 1: this tile's count, 2: count
+// of this and all earlier tiles), read and written as agent-scope atomics.
+// The global start of every digit comes from the pass's digit histogram,
+// which the key producer accumulated beforehand.
+constexpr uint32_t kAgg = 1u << 30, kIncl = 2u << 30, kCountMask = (1u << 30) - 1u;
+constexpr int kSpinLimit = 1 << 24;  // bounded wait; exceeding it flags *err (never expected)
+constexpr int kLookback = 8;          // predecessors read per look-back step
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin,
+                                                     const uint32_t* __restrict__ vin,
+                                                     K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                     int64_t n, int shift,
+                                                     const uint32_t* __restrict__ phist,
+                                                     uint32_t* status, uint32_t* tile_ctr,
+                                                     uint32_t* err) {
+    __shared__ uint32_t wh[kWaves][kRadix];
+    __shared__ uint32_t tstart[kRadix];
+    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t stile;
+    __shared__ K skey[kTile];
+    __shared__ uint32_t sval[kTile];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) stile = atomicAdd(tile_ctr, 1u);
+    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wh[0][0])[i] = 0;
+    __syncthreads();
+    const int tile = (int)stile;
+    const int64_t tbase = (int64_t)tile * kTile;
+    const int tile_n = (int)((n - tbase) < kTile ? (n - tbase) : kTile);
+
+    K key[kItems];
+    uint32_t val[kItems];
+    uint32_t rank[kItems];
+    const int wbase = wave * 64 * kItems;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        int e = wbase + j * 64 + lane;
+        if (e < tile_n) { key[j] = kin[tbase + e]; val[j] = vin[tbase + e]; }
+    }
+
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int e = wbase + j * 64 + lane;
+        const bool valid = e < tile_n;
+        const uint32_t d = valid ? digit_of(key[j], shift) : 0u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            const uint32_t before = (uint32_t)__popcll(peers & lt);
+            const uint32_t base = wh[wave][d];
+            if (before == 0) wh[wave][d] = base + (uint32_t)__popcll(peers);
+            r = base + before;
+        }
+        rank[j] = r;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            uint32_t c = wh[w][d];
+            wh[w][d] = run;
+            run += c;
+        }
+        // look-back for digit d: publish, then sum predecessors' counts
+        uint32_t* my = status + (int64_t)tile * kRadix + d;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(my, kIncl | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, kAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // predecessors are read kLookback at a time (independent loads in
+            // flight), then added newest first up to the first inclusive
+            // count or the first one not yet published, which is read again
+            int t = tile - 1;
+            int spins = 0;
+            bool done = false;
+            while (!done) {
+                uint32_t w[kLookback];
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q)
+                    w[q] = t - q >= 0 ? __hip_atomic_load(status + (int64_t)(t - q) * kRadix + d, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kIncl;
+                int used = 0;
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q) {
+                    const uint32_t f = w[q] & ~kCountMask;
+                    if (f == 0u) break;
+                    excl += w[q] & kCountMask;
+                    used = q + 1;
+                    if (f == kIncl) { done = true; break; }
+                }
+                t -= used;
+                if (!done && used == 0) {
+                    if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __hip_atomic_store(my, kIncl | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // start of digit d in the output: digits below d, then earlier tiles
+        uint32_t tot;
+        const uint32_t below = block_excl_scan256(phist[d], wsum, tot);
+        gbase[d] = below + excl;
+        tstart[d] = block_excl_scan256(run, wsum, tot);
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int e = wbase + j * 64 + lane;
+        if (e < tile_n) {
+            const uint32_t d = digit_of(key[j], shift);
+            const uint32_t pos = tstart[d] + wh[wave][d] + rank[j];
+            skey[pos] = key[j];
+            sval[pos] = val[j];
+        }
+    }
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < tile_n; i += kBlock) {
+        const K k = skey[i];
+        const uint32_t d = digit_of(k, shift);
+        const int64_t o = (int64_t)gbase[d] + (i - (int)tstart[d]);
+        kout[o] = k;
+        vout[o] = sval[i];
+    }
+}
+
+template <typename K>
+hipError_t onesweep_impl(OnesweepBuffers<K>& b, int64_t n, int key_bits, hipStream_t s, Marker* mk) {
+    const int passes = (key_bits + 7) / 8;
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_onesweep<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, b.keys[cur], b.vals[cur],
+                           b.keys[cur ^ 1], b.vals[cur ^ 1], n, 8 * p, b.hist + p * kRadix,
+                           b.status + (int64_t)p * kRadix * tiles, b.tile_ctr + p, b.err);
+        mark(mk, "sort_onesweep");
+        cur ^= 1;
+    }
+    b.result = cur;
+    return hipGetLastError();
+}
+
 __global__ void k_zero(uint32_t* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
@@ -265,6 +433,15 @@ hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, LaunchSink& ls,
 }  // namespace
 
 int64_t sort_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
+
+hipError_t onesweep_sort_pairs(OnesweepBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
+                               Marker* mk) {
+    return onesweep_impl<uint32_t>(b, n, key_bits, s, mk);
+}
+hipError_t onesweep_sort_pairs(OnesweepBuffers<uint64_t>& b, int64_t n, int key_bits, hipStream_t s,
+                               Marker* mk) {
+    return onesweep_impl<uint64_t>(b, n, key_bits, s, mk);
+}
 int sort_radix() { return kRadix; }
 
 hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,

@@ -99,7 +99,7 @@ struct RenderArgs {
     float* sxy;                // [nsamp][2]
     uint32_t* counts;          // queue counts (16 words)
     unsigned long long* rays;  // traced closest / shadow rays
-    uint32_t* trace_counters;  // traversal counters (8 words)
+    uint32_t* trace_counters;  // traversal counters (32 words, see lbvh.h)
     float* film_rgb;           // [w*h][3]
     float* film_w;             // [w*h]
     float4* film_partial;      // [9][w*h] per-source-pixel partial sums (room for double ones)

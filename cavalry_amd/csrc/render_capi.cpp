@@ -271,7 +271,7 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
     if (rc == CAV_OK) rc = grab(ctx, rs, a.sxy, 8 * T, "sample positions");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.counts, 64, "queue counts");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.rays, 64, "ray tallies");
-    if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 64, "trace counters");
+    if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 128, "trace counters");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_partial, 9 * 32 * P, "film partial sums");

@@ -263,7 +263,7 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     A(d.sh.ray, 24 * C); A(d.sh.limit, 4 * C); A(d.sh.src, 4 * C); A(d.sh.dst, 4 * C); A(d.sh.sid, 4 * C);
     A(d.sh.contrib, 12 * C); A(d.sh_rid, 4 * C); A(d.vis, C); A(d.hit_prim, 4 * C); A(d.hit_t, 4 * C);
     A(d.hit_u, 4 * C); A(d.hit_v, 4 * C);
-    A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 64);
+    A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 128);
     // rlpath tasks: per-slot learning state, queue and bounce state, records
     PoolRl& R = d.rl;
     A(R.req, sizeof(RlReqState) * kSlots); A(R.live, kSlots);
@@ -293,7 +293,7 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     for (auto& e : p->ring) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMemsetAsync(d.counts, 0, 64, p->stream) == hipSuccess &&
          hipMemsetAsync(d.rays, 0, 64, p->stream) == hipSuccess &&
-         hipMemsetAsync(d.trace_counters, 0, 64, p->stream) == hipSuccess &&
+         hipMemsetAsync(d.trace_counters, 0, 128, p->stream) == hipSuccess &&
          hipMemsetAsync(R.req, 0, sizeof(RlReqState) * kSlots, p->stream) == hipSuccess &&
          hipMemsetAsync(R.live, 0, kSlots, p->stream) == hipSuccess &&
          hipStreamSynchronize(p->stream) == hipSuccess;

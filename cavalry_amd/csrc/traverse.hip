@@ -101,6 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   uint8_t* __restrict__ ovis,
                                                   uint32_t* __restrict__ overflow,
                                                   unsigned long long* __restrict__ next_ray,
+                                                  unsigned long long* __restrict__ fetched,
                                                   int max_steps, int chunk) {
     __shared__ int lstack[kLdsDepth][kBlock];
     const int lane = threadIdx.x & 63;
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
     bool hit = false, use_lim = false;
     uint32_t ovf = 0;
     int maxsp = 0;  // deepest stack this lane reached
+    uint64_t nfetch = 0;  // 64-byte records this lane fetched (one per step)
     volatile int spill[kStack - kLdsDepth];  // in scratch memory, not registers
     int sp = 0;
     int64_t chunk_base = 0;
@@ -180,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
             // One 64-byte fetch for either kind of step, so that a wave with
             // lanes at nodes and lanes at triangles waits for memory once.
             const bool at_node = cur >= 0;
+            ++nfetch;
             const float4* rec = reinterpret_cast<const float4*>(recs + (at_node ? cur : ~cur));
             float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
             int4 refs = reinterpret_cast<const int4*>(rec)[3];
@@ -240,6 +243,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
 #pragma unroll
     for (int dd = 32; dd >= 1; dd >>= 1) maxsp = max(maxsp, __shfl_xor(maxsp, dd, 64));
     if (lane == 0 && maxsp > 0) atomicMax(overflow + 6, (uint32_t)maxsp);
+    // records fetched over the launch (summed over every launch until reset)
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) nfetch += __shfl_xor(nfetch, dd, 64);
+    if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
+}
+
+// 64-bit count of records fetched by closest-hit (words 16-17) or any-hit
+// (words 18-19) launches
+__host__ __device__ inline unsigned long long* fetch_word(uint32_t* counters, bool closest) {
+    return reinterpret_cast<unsigned long long*>(counters + (closest ? 16 : 18));
 }
 
 // zeroes n words; a kernel rather than hipMemsetAsync so the launch sequence
@@ -290,7 +303,8 @@ hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
     LaunchSink ls(s);
     launch_trace<true>(L, nrays, ls, rec, root, rays, nrays, (const uint32_t*)nullptr, (const float*)nullptr,
                        (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
-                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
+                       counters, reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true),
+                       2 * ntris - 1);
     return ls.err;
 }
 
@@ -304,7 +318,8 @@ hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
     LaunchSink ls(s);
     launch_trace<false>(L, nrays, ls, rec, (const float*)nullptr, rays, nrays, (const uint32_t*)nullptr, limit, src,
                         dst, (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
-                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
+                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), fetch_word(counters, false),
+                        2 * ntris - 1);
     return ls.err;
 }
 
@@ -317,7 +332,8 @@ void trace_closest_queue(const TRec* rec, int ntris, const float* root, const fl
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 2, 2);
     launch_trace<true>(L, capacity, ls, rec, root, rays, capacity, nrays_dev, (const float*)nullptr,
                        (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
-                       counters, reinterpret_cast<unsigned long long*>(counters + 2), 2 * ntris - 1);
+                       counters, reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true),
+                       2 * ntris - 1);
 }
 
 void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
@@ -326,7 +342,8 @@ void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const u
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 4, 2);
     launch_trace<false>(L, capacity, ls, rec, (const float*)nullptr, rays, capacity, nrays_dev, limit, src, dst,
                         (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
-                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), 2 * ntris - 1);
+                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), fetch_word(counters, false),
+                        2 * ntris - 1);
 }
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,

@@ -314,8 +314,9 @@ __global__ void k_leaf_ids(const int32_t* __restrict__ ref, int64_t m, uint32_t*
 
 }  // namespace
 
-// One restructuring round. The first round reuses the height levels of the
-// refit (same tree); later rounds list the leaves by a scan and climb again.
+// One restructuring round. The first round buckets the nodes by the heights
+// the build's climb left (same tree); later rounds list the leaves by a scan
+// and climb again.
 hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, Marker* mk) {
     const int64_t m = 2 * (int64_t)n - 1;
     hipError_t e;
@@ -323,9 +324,9 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
         if ((e = lbvh_refs(b, n, s, nullptr, nullptr)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_leaf_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, b.ref, m, b.leaf_ids);
         if ((e = levels_from_list(b, n, s, nullptr)) != hipSuccess) return e;
-        if ((e = levels_bucket(b, n, s, nullptr)) != hipSuccess) return e;
-        mark(mk, "treelet_setup");
     }
+    if ((e = levels_bucket(b, n, s, nullptr)) != hipSuccess) return e;
+    mark(mk, "treelet_setup");
     const int top = b.height;
     for (int h = 3; h <= top; ++h) {
         const uint32_t rest = b.h_off[top + 1] - b.h_off[h];

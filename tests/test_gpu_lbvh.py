@@ -72,3 +72,37 @@ def test_build_errors(ctx):
         BVH.build(ctx, np.zeros((1, 9), np.float32))
     with pytest.raises(CavError):
         BVH.build(ctx, np.zeros((4, 9), np.float32), morton_bits=31)
+
+
+def test_treelet_round_hand_derived(ctx):
+    # the 7-leaf case worked by hand in test_treelet_golden.py
+    from test_treelet_golden import G, triangles
+    bvh = BVH.build(ctx, triangles(), optimization_rounds=1, morton_bits=30)
+    ex = bvh.export()
+    want = G["after_one_round"]
+    for k in ("left", "right", "parent", "prim"):
+        np.testing.assert_array_equal(ex[k], np.asarray(want[k]), err_msg=k)
+    np.testing.assert_array_equal(ex["box"][:, [0, 3]], np.asarray(want["x_extent"], np.float32))
+    bvh.close()
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_rebuild_in_place_bit_exact(ctx, bits):
+    # one BVH object rebuilt from other triangles of the same count: the
+    # split gates the first build crossed must be clear again for the second
+    n = 300000
+    print(f"[lbvh] rebuild n={n} bits={bits}", flush=True)
+    a = scenes.gmm_triangles(n, seed=31)
+    b = scenes.gmm_triangles(n, seed=32)
+    import torch
+    bvh = BVH.build(ctx, a, optimization_rounds=0, morton_bits=bits)
+    for tris in (b, a):
+        dev = torch.from_numpy(tris).to("cuda:0")  # rebuild reads device triangles
+        torch.cuda.synchronize()
+        bvh.rebuild(dev, optimization_rounds=0, morton_bits=bits)
+        codes, order, nd = oracle.lbvh_build(tris, bits)
+        ex = bvh.export()
+        np.testing.assert_array_equal(ex["codes"], codes)
+        np.testing.assert_array_equal(ex["order"], order)
+        assert_nodes_equal(ex, nd, f"rebuild n={n} bits={bits}: ")
+    bvh.close()
