@@ -56,6 +56,7 @@ static void bvh_free(cav_bvh* h) {
                     h->b.bounds_part, h->b.gate64};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->spill) (void)hipFree(h->spill);
     if (h->b.h_lvl) (void)hipHostFree(h->b.h_lvl);
     delete[] h->b.h_off;
     if (h->events)
@@ -93,8 +94,10 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         int rc = dev_alloc(ctx, r.p, r.bytes, r.what);
         if (rc != CAV_OK) return rc;
     }
-    // the bottom-up build leaves every split gate at zero for the next build
+    // the bottom-up build leaves every split gate at zero for the next build;
+    // the traversal counters only ever add to (or max into) their words
     HIPCHK(ctx, hipMemsetAsync(h->b.gate64, 0, sizeof(uint64_t) * n, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 256, ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&h->b.h_lvl, sizeof(uint32_t) * (kMaxLevels + 2), hipHostMallocDefault));
     h->b.h_off = new uint32_t[kMaxLevels + 2]();
     for (auto& e : h->mk.ev) HIPCHK(ctx, hipEventCreate(&e));
@@ -361,6 +364,23 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
     return CAV_OK;
 }
 
+// Grows the BVH's any-hit spill pool to hold a batch of nrays (the stream
+// is drained before an old pool is freed).
+static int ensure_spill(cav_ctx* ctx, const cav_bvh* h, int64_t nrays) {
+    const uint32_t need = any_spill_capacity(nrays);
+    if (h->spill_cap >= need) return CAV_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h->spill) (void)hipFree(h->spill);
+    h->spill = nullptr;
+    h->spill_cap = 0;
+    void* p = nullptr;
+    const int rc = dev_alloc(ctx, &p, (sizeof(AnySpill) + sizeof(uint32_t)) * (size_t)need, "any-hit spill pool");
+    if (rc != CAV_OK) return rc;
+    h->spill = static_cast<AnySpill*>(p);
+    h->spill_cap = need;
+    return CAV_OK;
+}
+
 int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
                             int32_t* prim, float* t) {
     if (!ctx || !h || (nrays > 0 && (!rays || !prim || !t))) return CAV_ERR_INVALID;
@@ -372,8 +392,11 @@ int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, i
 int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
                              const float* limit, int64_t nrays, uint8_t* visible) {
     if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
+    const int rc = ensure_spill(ctx, h, nrays);
+    if (rc != CAV_OK) return rc;
     HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, nullptr, nullptr, nrays, visible,
-                               h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
+                               h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, h->spill,
+                               h->spill_cap));
     return CAV_OK;
 }
 
@@ -395,12 +418,15 @@ int cav_trace_counters(cav_ctx* ctx, const cav_bvh* h, uint32_t* out4, int32_t r
     return CAV_OK;
 }
 
-int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out2, int32_t reset) {
-    if (!ctx || !h || !out2) return CAV_ERR_INVALID;
+int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out3, int32_t reset) {
+    if (!ctx || !h || !out3) return CAV_ERR_INVALID;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(out2, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t spilled = 0;
+    HIPCHK(ctx, hipMemcpyAsync(out3, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&spilled, h->b.counters + 20, 4, hipMemcpyDeviceToHost, ctx->stream));
     if (reset) HIPCHK(ctx, hipMemsetAsync(h->b.counters + 16, 0, 16, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    out3[2] = spilled;
     return CAV_OK;
 }
 
@@ -440,7 +466,10 @@ int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
         return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
+    const int rc = ensure_spill(ctx, h, nrays);
+    if (rc != CAV_OK) return rc;
+    HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters,
+                               ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, h->spill, h->spill_cap));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -94,7 +94,9 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // counters[6] (closest) and counters[7] (occluded) keep the deepest stack any
 // ray reached (atomic max, never reset by a launch); the 64-bit words at
 // counters[16] (closest) and counters[18] (occluded) add up the 64-byte
-// records every launch fetched (never reset by a launch).
+// records every launch fetched (never reset by a launch); counters[20..25]
+// are the cooperative any-hit pass's spill count, spill cursor, failure
+// count and ray dispenser (cleared by each launch).
 // Launch shape of the persistent traversal kernels. Each ray's node order is
 // fixed, so results do not depend on it; the autotuner picks it.
 struct TraceLaunch {
@@ -106,10 +108,27 @@ struct TraceLaunch {
 hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
                          float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s);
+// Any-hit rays that outgrow a step budget hand their pending sibling pairs
+// to a cooperative pass (one wave per ray): one record each.
+constexpr int kSpillItems = 65;  // the current pair + a full stack
+struct AnySpill {
+    int64_t ray;
+    int32_t n;                     // pending pairs
+    int32_t pad;
+    uint32_t item[kSpillItems];    // (base slot >> 1) << 2 | active mask
+    uint8_t byte[kSpillItems];     // s of the far or only member | 128: near is the right one
+};
+// spill capacity worth reserving for a batch of nrays any-hit rays
+inline uint32_t any_spill_capacity(int64_t nrays) {
+    const int64_t c = nrays / 32 + 4096;
+    return (uint32_t)(c < (int64_t)(1u << 30) ? c : (1u << 30));
+}
+
+// spill: nullptr (every ray traced by one lane) or spill_cap records
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
-                          hipStream_t s);
+                          hipStream_t s, AnySpill* spill = nullptr, uint32_t spill_cap = 0);
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,

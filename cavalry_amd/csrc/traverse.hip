@@ -37,6 +37,7 @@ __device__ __forceinline__ void stack_push(int (*lds)[kBlock], volatile int* spi
 // cannot merge it with the spill read into one flat load through a selected
 // pointer)
 typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) uint32_t lds_uint;
 template <int D>
 __device__ __forceinline__ int stack_pop(int (*lds)[kBlock], const volatile int* spill, int& sp) {
     --sp;
@@ -81,24 +82,25 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
     d = mk3(b.y, c.x, c.y);
 }
 
-// Persistent traversal: a fixed grid whose lanes each keep one ray in flight
-// and pull a new one as soon as theirs finishes, so a wave is not held by its
-// slowest ray (nodes visited per ray vary by 1000x on scenes with many
-// coincident Morton codes). Rays are handed out per wave from chunks taken
-// with one atomic. Each ray's node order is the reference's, so results do not
-// depend on which lane or wave traced it.
-template <bool kClosest, int kLdsDepth>
+// Persistent closest-hit traversal: a fixed grid whose lanes each keep one
+// ray in flight and pull a new one as soon as theirs finishes, so a wave is
+// not held by its slowest ray (nodes visited per ray vary by 1000x on scenes
+// with many coincident Morton codes). Rays are handed out per wave from
+// chunks taken with one atomic. Each ray's node order is the reference's, so
+// results do not depend on which lane or wave traced it.
+//
+// When both children of a node are hit and the nearer one is a leaf, the
+// reference tests that leaf and then pops its far sibling straight away
+// (BVH/BVH.h:72-95): here one step fetches both records, which share one
+// 128-byte line in the sibling-pair layout, and does both in that order.
+template <int kLdsDepth>
 __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
                                                   const float* __restrict__ rays, int64_t nrays_arg,
                                                   const uint32_t* __restrict__ nrays_dev,
-                                                  const float* __restrict__ limit,
-                                                  const int32_t* __restrict__ src,
-                                                  const int32_t* __restrict__ dst,
                                                   int32_t* __restrict__ oprim,
                                                   float* __restrict__ ot, float* __restrict__ ou,
                                                   float* __restrict__ ov,
-                                                  uint8_t* __restrict__ ovis,
                                                   uint32_t* __restrict__ overflow,
                                                   unsigned long long* __restrict__ next_ray,
                                                   unsigned long long* __restrict__ fetched,
@@ -110,12 +112,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
 
     int64_t ri = -1;       // ray held by this lane; -1 = needs one, -2 = pool drained
     v3 o = mk3(0, 0, 0), d = o, inv = o;
-    float best = FLT_MAX, bu = 0.f, bv = 0.f, lim = 0.f;
-    int bp = -1, sm = -1, dm = -1, cur = 0, steps = 0;
-    bool hit = false, use_lim = false;
+    float best = FLT_MAX, bu = 0.f, bv = 0.f;
+    int bp = -1, cur = 0, steps = 0;
+    bool hit = false;
+    bool fused = false;  // cur is a near leaf whose far sibling pfar comes in the same step
+    int pfar = 0;
     uint32_t ovf = 0;
     int maxsp = 0;  // deepest stack this lane reached
-    uint64_t nfetch = 0;  // 64-byte records this lane fetched (one per step)
+    uint64_t nfetch = 0;  // 64-byte records this lane fetched
     volatile int spill[kStack - kLdsDepth];  // in scratch memory, not registers
     int sp = 0;
     int64_t chunk_base = 0;
@@ -149,20 +153,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                     load_ray(rays, ri, o, d);
                     inv = div3(mk3(1.f, 1.f, 1.f), d);
                     best = FLT_MAX; hit = false; bp = -1; bu = 0.f; bv = 0.f;
-                    sp = 0; cur = 0; steps = 0;
-                    if (kClosest) {
-                        // BVH/BVH.h:34-38: a ray missing the root box has no hit
-                        if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
-                            oprim[ri] = -1; ot[ri] = 0.f;
-                            if (ou) ou[ri] = 0.f;
-                            if (ov) ov[ri] = 0.f;
-                            ri = -1;
-                        }
-                    } else {
-                        lim = limit[ri];
-                        use_lim = !(lim == INFINITY);
-                        sm = src ? src[ri] : -1;
-                        dm = dst ? dst[ri] : -1;
+                    sp = 0; cur = 0; steps = 0; fused = false;
+                    // BVH/BVH.h:34-38: a ray missing the root box has no hit
+                    if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
+                        oprim[ri] = -1; ot[ri] = 0.f;
+                        if (ou) ou[ri] = 0.f;
+                        if (ov) ov[ri] = 0.f;
+                        ri = -1;
                     }
                 }
             }
@@ -179,62 +176,74 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
             ++ovf;
             done = true;
         } else {
-            // One 64-byte fetch for either kind of step, so that a wave with
-            // lanes at nodes and lanes at triangles waits for memory once.
+            // One fetch for either kind of step, so that a wave with lanes at
+            // nodes and lanes at triangles waits for memory once.
             const bool at_node = cur >= 0;
-            ++nfetch;
             const float4* rec = reinterpret_cast<const float4*>(recs + (at_node ? cur : ~cur));
             float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
             int4 refs = reinterpret_cast<const int4*>(rec)[3];
             // (an empty asm that takes the node-only words keeps the compiler
             // from moving their loads into the node branch, a second wait)
             asm volatile("" : "+v"(q1.w), "+v"(q2.w), "+v"(refs.x), "+v"(refs.y));
-            if (at_node) {
-                const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
-                const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
-                const bool hl = kClosest ? (dl >= 0 && dl < best) : dl >= 0;
-                const bool hr = kClosest ? (dr >= 0 && dr < best) : dr >= 0;
-                bool next = false;
+            float4 f0 = q0, f1 = q1, f2 = q2;
+            int4 frefs = refs;
+            const bool pair = fused;
+            if (pair) {  // the far sibling, in the same 128-byte line
+                const float4* fr = reinterpret_cast<const float4*>(recs + (pfar >= 0 ? pfar : ~pfar));
+                f0 = fr[0]; f1 = fr[1]; f2 = fr[2];
+                frefs = reinterpret_cast<const int4*>(fr)[3];
+            }
+            nfetch += pair ? 2 : 1;
+            fused = false;
+            bool next = false;  // cur set for the next step
+            // BVH/BVH.h:72-95 at a node
+            auto visit_node = [&](const float4& a, const float4& b, const float4& c, const int4& rf) {
+                const float dl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv);
+                const float dr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv);
+                const bool hl = dl >= 0 && dl < best;
+                const bool hr = dr >= 0 && dr < best;
                 if (hl && hr) {
                     const bool lfar = dl > dr;
-                    const int far = lfar ? refs.x : refs.y;
-                    const int near = lfar ? refs.y : refs.x;
-                    if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
-                    maxsp = max(maxsp, sp);
-                    if (sp < kStack) { cur = near; next = true; } else ++ovf;
-                } else if (hl || hr) {
-                    if (sp < kStack) { cur = hl ? refs.x : refs.y; next = true; } else ++ovf;
-                }
-                if (!next) {
-                    if (sp == 0) done = true;
-                    else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
-                }
-            } else {
-                const TTri tr{q0, q1, q2};
-                float t, u, v;
-                if (kClosest) {
-                    if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
-                        hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+                    const int far = lfar ? rf.x : rf.y;
+                    const int near = lfar ? rf.y : rf.x;
+                    if (near < 0 && sp + 1 < kStack) {
+                        // the reference pushes far (sp + 1 entries), tests the
+                        // near leaf and pops far again: both in the next fetch
+                        maxsp = max(maxsp, sp + 1);
+                        cur = near; pfar = far; fused = true; next = true;
+                    } else {
+                        if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
+                        maxsp = max(maxsp, sp);
+                        if (sp < kStack) { cur = near; next = true; } else ++ovf;
                     }
-                } else {
-                    const int mesh = __float_as_int(tr.b.w);
-                    if (mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim))
-                        hit = true;  // occluded
+                } else if (hl || hr) {
+                    if (sp < kStack) { cur = hl ? rf.x : rf.y; next = true; } else ++ovf;
                 }
-                if (!kClosest && hit) done = true;
-                else if (sp == 0) done = true;
+            };
+            // BVH/BVH.h:58-70 at a leaf
+            auto visit_leaf = [&](const float4& a, const float4& b, const float4& c) {
+                const TTri tr{a, b, c};
+                float t, u, v;
+                if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
+                    hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
+                }
+            };
+            if (at_node) visit_node(q0, q1, q2, refs);
+            else visit_leaf(q0, q1, q2);
+            if (pair) {  // the far sibling, as the reference's pop after the near leaf
+                if (pfar >= 0) visit_node(f0, f1, f2, frefs);
+                else visit_leaf(f0, f1, f2);
+            }
+            if (!next) {
+                if (sp == 0) done = true;
                 else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
             }
         }
         if (done) {
-            if (kClosest) {
-                oprim[ri] = hit ? bp : -1;
-                ot[ri] = hit ? best : 0.f;
-                if (ou) ou[ri] = hit ? bu : 0.f;
-                if (ov) ov[ri] = hit ? bv : 0.f;
-            } else {
-                ovis[ri] = hit ? 0 : 1;
-            }
+            oprim[ri] = hit ? bp : -1;
+            ot[ri] = hit ? best : 0.f;
+            if (ou) ou[ri] = hit ? bu : 0.f;
+            if (ov) ov[ri] = hit ? bv : 0.f;
             ri = -1;
         }
     }
@@ -246,6 +255,349 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
     // records fetched over the launch (summed over every launch until reset)
 #pragma unroll
     for (int dd = 32; dd >= 1; dd >>= 1) nfetch += __shfl_xor(nfetch, dd, 64);
+    if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
+}
+
+// Any-hit traversal over sibling pairs.
+//
+// BVH::testVisibility (BVH/BVH.h:109-189) answers whether any triangle of a
+// leaf reached through hit boxes blocks the ray before its limit; which one
+// it meets first does not change the answer. So each step here takes a pair
+// of siblings -- the two 64-byte records sharing one 128-byte line in the
+// sibling-pair layout -- tests both nodes, and goes on with the pairs of
+// their children: half the dependent steps of one node per step, and no
+// half-used lines. The reference's 64-entry stack is kept exactly: each
+// node carries the stack depth the reference would have at it (s = far
+// siblings pending below it; a far child has its parent's s, a near child
+// s + 1, and a near child that would need s = 64 is dropped and counted, as
+// the reference's failed push is), so the same nodes are visited and the
+// same overflows counted (all of them when no occluder ends the ray first).
+//
+// A pending pair is (base slot >> 1) << 2 | active mask (bit 0: the left
+// sibling at the base slot, bit 1: the right one) and one byte: the s of its
+// far or only member, | 128 when the near member is the right one. The root
+// is the pair at slot 0 with only bit 0 active.
+struct PairOut {
+    uint32_t item[2];
+    uint32_t byte[2];
+    int n;
+};
+
+__device__ __forceinline__ void pair_step(const float4 (&r)[8], uint32_t item, uint32_t byte, v3 o, v3 d,
+                                          v3 inv, int sm, int dm, bool use_lim, float lim, bool& hit,
+                                          uint32_t& ovf, int& maxsp, PairOut& out) {
+    const uint32_t mask = item & 3u;
+    const int sb = (int)(byte & 63u);
+    const bool near_right = (byte & 128u) != 0;
+    out.n = 0;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        if (!((mask >> m) & 1u) || hit) continue;
+        const float4 q0 = r[4 * m], q1 = r[4 * m + 1], q2 = r[4 * m + 2], q3 = r[4 * m + 3];
+        const int s = (mask == 3u && (m == 1) == near_right) ? sb + 1 : sb;
+        const int refl = __float_as_int(q3.x);
+        if (refl == 0) {  // a leaf's record: its triangle
+            const TTri tr{q0, q1, q2};
+            const int mesh = __float_as_int(tr.b.w);
+            float t, u, v;
+            if (mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim)) hit = true;
+        } else {
+            const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
+            const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
+            const bool hl = dl >= 0, hr = dr >= 0;
+            if (hl || hr) {
+                const int cb = refl >= 0 ? refl : ~refl;  // the children's slots: cb, cb + 1
+                uint32_t cm = (hl ? 1u : 0u) | (hr ? 2u : 0u);
+                bool nr = false;
+                if (cm == 3u) {
+                    maxsp = max(maxsp, s + 1);  // the far child's push
+                    nr = dl > dr;               // BVH/BVH.h:171-178: the nearer child is taken first
+                    if (s + 1 >= kStack) {      // the near child would need a 65th entry
+                        ++ovf;
+                        cm = nr ? 1u : 2u;      // the far child alone
+                    }
+                }
+                out.item[out.n] = ((uint32_t)(cb >> 1) << 2) | cm;
+                out.byte[out.n] = (uint32_t)s | (nr && cm == 3u ? 128u : 0u);
+                ++out.n;
+            }
+        }
+    }
+}
+
+// Phase 1: one lane per ray. Once every ray has been handed out, a ray that
+// has taken `spill_after` steps (0: never) is written to the spill pool,
+// while it has room, with its pending pairs, and finished by phase 2: the
+// long rays still in flight at the end are spread over whole waves instead
+// of keeping one lane each busy while the rest of the device idles. ids
+// (optional) maps the dispensed index to the ray index.
+template <int kLdsDepth>
+__global__ __launch_bounds__(kBlock) void k_trace_any_pairs(const TRec* __restrict__ recs,
+                                                            const float* __restrict__ rays, int64_t nrays_arg,
+                                                            const uint32_t* __restrict__ nrays_dev,
+                                                            const uint32_t* __restrict__ ids,
+                                                            const float* __restrict__ limit,
+                                                            const int32_t* __restrict__ src,
+                                                            const int32_t* __restrict__ dst,
+                                                            uint8_t* __restrict__ ovis,
+                                                            uint32_t* __restrict__ overflow,
+                                                            unsigned long long* __restrict__ next_ray,
+                                                            unsigned long long* __restrict__ fetched,
+                                                            AnySpill* __restrict__ spill, uint32_t* spill_n,
+                                                            uint32_t spill_cap, int spill_after,
+                                                            int max_steps, int chunk) {
+    __shared__ uint32_t litem[kLdsDepth][kBlock];
+    __shared__ uint8_t lsb[kLdsDepth][kBlock];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t nrays = nrays_dev ? (int64_t)*nrays_dev : nrays_arg;
+
+    int64_t ri = -1;  // ray held by this lane; -1 = needs one, -2 = pool drained
+    v3 o = mk3(0, 0, 0), d = o, inv = o;
+    float lim = 0.f;
+    int sm = -1, dm = -1, steps = 0;
+    bool hit = false, use_lim = false, may_spill = false;
+    bool drained = false;  // every ray has been handed out
+    uint32_t ovf = 0;
+    int maxsp = 0;  // deepest stack the reference's traversal reaches on this lane's rays
+    uint64_t nfetch = 0;
+    volatile uint32_t spill_i[kStack - kLdsDepth];
+    volatile uint8_t spill_b[kStack - kLdsDepth];
+    int sp = 0;
+    uint32_t citem = 0;  // current pair
+    uint32_t cbyte = 0;
+    int64_t chunk_base = 0;
+    int chunk_avail = 0;
+
+    while (true) {
+        // ---- hand new rays to idle lanes (wave-uniform control flow)
+        const uint64_t need = __ballot(ri == -1);
+        if (need) {
+            const int cnt = __popcll(need);
+            const int rank = __popcll(need & lt);
+            int64_t mine;
+            if (chunk_avail < cnt) {
+                unsigned long long nb = 0;
+                const int first = __ffsll((long long)need) - 1;
+                if (lane == first) nb = atomicAdd(next_ray, (unsigned long long)chunk);
+                nb = __shfl(nb, first, 64);
+                mine = rank < chunk_avail ? chunk_base + rank : (int64_t)nb + (rank - chunk_avail);
+                chunk_base = (int64_t)nb + (cnt - chunk_avail);
+                chunk_avail = chunk - (cnt - chunk_avail);
+            } else {
+                mine = chunk_base + rank;
+                chunk_base += cnt;
+                chunk_avail -= cnt;
+            }
+            if (ri == -1) {
+                if (mine >= nrays) {
+                    ri = -2;
+                } else {
+                    ri = ids ? (int64_t)ids[mine] : mine;
+                    load_ray(rays, ri, o, d);
+                    inv = div3(mk3(1.f, 1.f, 1.f), d);
+                    hit = false;
+                    sp = 0;
+                    steps = 0;
+                    citem = 1u;  // the root alone: slot 0, s = 0
+                    cbyte = 0u;
+                    may_spill = spill_after > 0;
+                    lim = limit[ri];
+                    use_lim = !(lim == INFINITY);
+                    sm = src ? src[ri] : -1;
+                    dm = dst ? dst[ri] : -1;
+                }
+            }
+        }
+        if (!__ballot(ri >= 0)) break;  // every lane drained
+        if (ri < 0) continue;
+
+        if (may_spill && steps >= spill_after && !drained && (steps & 255) == 0)
+            drained = (int64_t)__hip_atomic_load(next_ray, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nrays;
+        if (may_spill && drained && steps >= spill_after) {
+            // a long ray at the end: hand its pending pairs to the cooperative pass
+            const uint32_t slot = atomicAdd(spill_n, 1u);
+            if (slot < spill_cap) {
+                AnySpill* rec = spill + slot;
+                rec->ray = ri;
+                rec->n = sp + 1;
+                rec->item[0] = citem;
+                rec->byte[0] = (uint8_t)cbyte;
+                for (int i = 0; i < sp; ++i) {
+                    rec->item[i + 1] = i < kLdsDepth ? *(const volatile lds_uint*)(&litem[i][threadIdx.x])
+                                                     : spill_i[i - kLdsDepth];
+                    rec->byte[i + 1] = i < kLdsDepth ? lsb[i][threadIdx.x] : spill_b[i - kLdsDepth];
+                }
+                ri = -1;
+                continue;
+            }
+            may_spill = false;  // the pool is full: finish it here
+        }
+
+        // ---- one step: both members of the current pair
+        bool done = false;
+        if (++steps > max_steps) {
+            ++ovf;
+            done = true;
+        } else {
+            const float4* rec = reinterpret_cast<const float4*>(recs + ((int64_t)(citem >> 2) << 1));
+            float4 r[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) r[k] = rec[k];
+            nfetch += (uint64_t)__popc(citem & 3u);
+            PairOut po;
+            pair_step(r, citem, cbyte, o, d, inv, sm, dm, use_lim, lim, hit, ovf, maxsp, po);
+            if (hit) {
+                done = true;
+            } else if (po.n > 0) {
+                citem = po.item[0];
+                cbyte = po.byte[0];
+                if (po.n == 2) {
+                    if (sp < kStack) {
+                        if (sp < kLdsDepth) {
+                            litem[sp][threadIdx.x] = po.item[1];
+                            lsb[sp][threadIdx.x] = (uint8_t)po.byte[1];
+                        } else {
+                            spill_i[sp - kLdsDepth] = po.item[1];
+                            spill_b[sp - kLdsDepth] = (uint8_t)po.byte[1];
+                        }
+                        ++sp;
+                    } else {
+                        ++ovf;  // only under trees deeper than the stack
+                    }
+                }
+            } else if (sp == 0) {
+                done = true;
+            } else {
+                --sp;
+                if (sp < kLdsDepth) {
+                    citem = *(const volatile lds_uint*)(&litem[sp][threadIdx.x]);
+                    cbyte = lsb[sp][threadIdx.x];
+                } else {
+                    citem = spill_i[sp - kLdsDepth];
+                    cbyte = spill_b[sp - kLdsDepth];
+                }
+            }
+        }
+        if (done) {
+            ovis[ri] = hit ? 0 : 1;
+            ri = -1;
+        }
+    }
+    if (ovf) atomicAdd(overflow, ovf);
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        maxsp = max(maxsp, __shfl_xor(maxsp, dd, 64));
+        nfetch += __shfl_xor(nfetch, dd, 64);
+    }
+    if (lane == 0 && maxsp > 0) atomicMax(overflow + 6, (uint32_t)maxsp);
+    if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
+}
+
+// Phase 2: one wave per spilled ray. The wave keeps the ray's pending pairs
+// in LDS and takes up to 64 of them at a time, newest first, one per lane;
+// their child pairs go back on top. It stops at the first occluder or when
+// nothing is pending. The visited set and overflow count are the reference's
+// (the answer does not depend on the order). Should the list outgrow LDS --
+// it holds at most 64 pairs per tree level, 3200 at the 50 levels of a
+// 2^24-triangle tree -- the ray goes to the fail list and phase 3 traces it
+// again from its root on one lane.
+constexpr int kCoopCap = 4096;
+__global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, const float* __restrict__ rays,
+                                                 const float* __restrict__ limit, const int32_t* __restrict__ src,
+                                                 const int32_t* __restrict__ dst, uint8_t* __restrict__ ovis,
+                                                 uint32_t* __restrict__ overflow,
+                                                 unsigned long long* __restrict__ fetched,
+                                                 const AnySpill* __restrict__ spill, const uint32_t* spill_n,
+                                                 uint32_t spill_cap, uint32_t* next_spill,
+                                                 uint32_t* __restrict__ fail_ids, uint32_t* fail_n) {
+    __shared__ uint32_t witem[kCoopCap];
+    __shared__ uint8_t wbyte[kCoopCap];
+    const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t total = min(*spill_n, spill_cap);
+    uint32_t ovf = 0;
+    int maxsp = 0;
+    uint64_t nfetch = 0;
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(next_spill, 1u);
+        idx = __shfl(idx, 0, 64);
+        if (idx >= total) break;
+        const AnySpill* sr = spill + idx;
+        const int64_t ri = sr->ray;
+        const int n0 = sr->n;
+        v3 o, d;
+        load_ray(rays, ri, o, d);
+        const v3 inv = div3(mk3(1.f, 1.f, 1.f), d);
+        const float lim = limit[ri];
+        const bool use_lim = !(lim == INFINITY);
+        const int sm = src ? src[ri] : -1, dm = dst ? dst[ri] : -1;
+        for (int i = lane; i < n0; i += 64) {
+            witem[i] = sr->item[i];
+            wbyte[i] = sr->byte[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        int count = n0;
+        bool occluded = false, failed = false;
+        while (count > 0) {
+            const int k = count < 64 ? count : 64;
+            const bool act = lane < k;
+            uint32_t item = 0, byte = 0;
+            if (act) {
+                item = witem[count - 1 - lane];
+                byte = wbyte[count - 1 - lane];
+            }
+            count -= k;
+            __builtin_amdgcn_wave_barrier();
+            bool hit = false;
+            PairOut po;
+            po.n = 0;
+            if (act) {
+                const float4* rec = reinterpret_cast<const float4*>(recs + ((int64_t)(item >> 2) << 1));
+                float4 r[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) r[q] = rec[q];
+                nfetch += (uint64_t)__popc(item & 3u);
+                pair_step(r, item, byte, o, d, inv, sm, dm, use_lim, lim, hit, ovf, maxsp, po);
+            }
+            if (__ballot(hit)) {
+                occluded = true;
+                break;
+            }
+            // append the child pairs: a wave prefix over the lanes' counts
+            int pos = po.n;
+#pragma unroll
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                const int v = __shfl_up(pos, dd, 64);
+                if (lane >= dd) pos += v;
+            }
+            const int added = __shfl(pos, 63, 64);
+            pos -= po.n;  // exclusive
+            if (count + added > kCoopCap) {
+                failed = true;
+                break;
+            }
+            for (int j = 0; j < po.n; ++j) {
+                witem[count + pos + j] = po.item[j];
+                wbyte[count + pos + j] = (uint8_t)po.byte[j];
+            }
+            count += added;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            if (failed) fail_ids[atomicAdd(fail_n, 1u)] = (uint32_t)ri;
+            else ovis[ri] = occluded ? 0 : 1;
+        }
+        (void)lt;
+    }
+    if (ovf) atomicAdd(overflow, ovf);
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        maxsp = max(maxsp, __shfl_xor(maxsp, dd, 64));
+        nfetch += __shfl_xor(nfetch, dd, 64);
+    }
+    if (lane == 0 && maxsp > 0) atomicMax(overflow + 6, (uint32_t)maxsp);
     if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
 }
 
@@ -278,18 +630,68 @@ int persistent_blocks(const void* kernel, int max_per_cu) {
 
 // The kernel of the launch's stack depth on a persistent grid of its
 // workgroups per CU, for `work` rays at most.
-template <bool C, typename... A>
+template <typename... A>
 void launch_trace(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
     const bool deep = L.lds_depth >= 48;
-    const void* k = deep ? reinterpret_cast<const void*>(&k_trace<C, 48>)
-                         : reinterpret_cast<const void*>(&k_trace<C, 32>);
+    const void* k = deep ? reinterpret_cast<const void*>(&k_trace<48>) : reinterpret_cast<const void*>(&k_trace<32>);
     int64_t blocks = persistent_blocks(k, L.per_cu);
     const int64_t need = (work + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    const char* name = C ? "k_trace<closest>" : "k_trace<any>";
-    if (deep) ls.launch(name, &k_trace<C, 48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
-    else ls.launch(name, &k_trace<C, 32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    if (deep) ls.launch("k_trace<closest>", &k_trace<48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    else ls.launch("k_trace<closest>", &k_trace<32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+}
+
+// The any-hit pair kernel on the same persistent grid.
+template <typename... A>
+void launch_any_pairs(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
+    const bool deep = L.lds_depth >= 48;
+    const void* k = deep ? reinterpret_cast<const void*>(&k_trace_any_pairs<48>)
+                         : reinterpret_cast<const void*>(&k_trace_any_pairs<32>);
+    int64_t blocks = persistent_blocks(k, L.per_cu);
+    const int64_t need = (work + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    if (deep) ls.launch("k_trace_any_pairs", &k_trace_any_pairs<48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    else ls.launch("k_trace_any_pairs", &k_trace_any_pairs<32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+}
+
+// Steps (sibling pairs) a ray must have taken, once the rays have all been
+// handed out, to be handed to the cooperative pass: the short rays still in
+// flight then finish where they are.
+constexpr int kAnySpillAfter = 1024;
+
+// Any-hit in three launches: phase 1 (one lane per ray, long rays spilled),
+// phase 2 (a wave per spilled ray), phase 3 (rays phase 2 could not hold,
+// from their roots, one lane each; normally none). Words 20..25 of the
+// counters: spill count, spill cursor, failure count, phase-3 dispenser.
+void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, const uint32_t* nrays_dev,
+               const float* limit, const int32_t* src, const int32_t* dst, uint8_t* visible, uint32_t* counters,
+               const TraceLaunch& L, AnySpill* spill, uint32_t spill_cap, LaunchSink& ls) {
+    const bool coop = spill && spill_cap > 0 && nrays < (int64_t(1) << 32);
+    auto* disp = reinterpret_cast<unsigned long long*>(counters + 4);
+    launch_any_pairs(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
+                     counters + 1, disp, fetch_word(counters, false), coop ? spill : (AnySpill*)nullptr,
+                     counters + 20, coop ? spill_cap : 0u, coop ? kAnySpillAfter : 0, 2 * ntris - 1);
+    if (!coop) return;
+    uint32_t* fail_ids = reinterpret_cast<uint32_t*>(spill + spill_cap);
+    static const int coop_blocks = [] {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_any_coop), 64, 0) !=
+                hipSuccess || per <= 0)
+            per = 4;
+        return cus * per;
+    }();
+    ls.launch("k_any_coop", &k_any_coop, dim3((unsigned)coop_blocks), dim3(64), rec, rays, limit, src, dst, visible,
+              counters + 1, fetch_word(counters, false), (const AnySpill*)spill, (const uint32_t*)(counters + 20),
+              spill_cap, counters + 21, fail_ids, counters + 22);
+    launch_any_pairs(L, (int64_t)spill_cap, ls, rec, rays, (int64_t)spill_cap, (const uint32_t*)(counters + 22),
+                     (const uint32_t*)fail_ids, limit, src, dst, visible, counters + 1,
+                     reinterpret_cast<unsigned long long*>(counters + 24), fetch_word(counters, false),
+                     (AnySpill*)nullptr, counters + 20, 0u, 0, 2 * ntris - 1);
 }
 
 }  // namespace
@@ -301,25 +703,21 @@ hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
     hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
     if (e != hipSuccess) return e;
     LaunchSink ls(s);
-    launch_trace<true>(L, nrays, ls, rec, root, rays, nrays, (const uint32_t*)nullptr, (const float*)nullptr,
-                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
-                       counters, reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true),
-                       2 * ntris - 1);
+    launch_trace(L, nrays, ls, rec, root, rays, nrays, (const uint32_t*)nullptr, prim, t, u, v, counters,
+                 reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true), 2 * ntris - 1);
     return ls.err;
 }
 
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
-                          hipStream_t s) {
+                          hipStream_t s, AnySpill* spill, uint32_t spill_cap) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(counters + 20, 0, 24, s);
     if (e != hipSuccess) return e;
     LaunchSink ls(s);
-    launch_trace<false>(L, nrays, ls, rec, (const float*)nullptr, rays, nrays, (const uint32_t*)nullptr, limit, src,
-                        dst, (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
-                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), fetch_word(counters, false),
-                        2 * ntris - 1);
+    trace_any(rec, ntris, rays, nrays, nullptr, limit, src, dst, visible, counters, L, spill, spill_cap, ls);
     return ls.err;
 }
 
@@ -330,20 +728,15 @@ void trace_closest_queue(const TRec* rec, int ntris, const float* root, const fl
                          const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
                          uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 2, 2);
-    launch_trace<true>(L, capacity, ls, rec, root, rays, capacity, nrays_dev, (const float*)nullptr,
-                       (const int32_t*)nullptr, (const int32_t*)nullptr, prim, t, u, v, (uint8_t*)nullptr,
-                       counters, reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true),
-                       2 * ntris - 1);
+    launch_trace(L, capacity, ls, rec, root, rays, capacity, nrays_dev, prim, t, u, v, counters,
+                 reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true), 2 * ntris - 1);
 }
 
 void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
                           int64_t capacity, const float* limit, const int32_t* src, const int32_t* dst,
                           uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 4, 2);
-    launch_trace<false>(L, capacity, ls, rec, (const float*)nullptr, rays, capacity, nrays_dev, limit, src, dst,
-                        (int32_t*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr, visible,
-                        counters + 1, reinterpret_cast<unsigned long long*>(counters + 4), fetch_word(counters, false),
-                        2 * ntris - 1);
+    trace_any(rec, ntris, rays, capacity, nrays_dev, limit, src, dst, visible, counters, L, nullptr, 0, ls);
 }
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,

@@ -128,9 +128,11 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays
 int cav_trace_counters(cav_ctx* ctx, const cav_bvh* bvh, uint32_t* out, int32_t reset);
 /* Work of a BVH's standalone launches: out[0] / out[1] = 64-byte traversal
  * records (node visits and triangle tests, one record each) fetched by every
- * closest-hit / any-hit launch since the last reset. reset != 0 clears both
- * after reading. No counterpart in the reference; it measures the
- * algorithmic traffic of BVH::intersect / testVisibility (BVH/BVH.h:23-189). */
+ * closest-hit / any-hit launch since the last reset; out[2] = rays of the
+ * last any-hit launch that outgrew one lane's step budget and were finished
+ * by a whole wave. reset != 0 clears out[0..1] after reading. No counterpart
+ * in the reference; it measures the algorithmic traffic of BVH::intersect /
+ * testVisibility (BVH/BVH.h:23-189). */
 int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* bvh, uint64_t* out, int32_t reset);
 /* The deepest leaf of the BVH, in edges from the root: no traversal can push
  * more stack entries than this, so a depth of at most 64 rules out overflow

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# torch first: it loads the HIP runtime it was built with, which the library
+# then shares; loaded after the library, torch finds no GPU
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
