@@ -317,6 +317,72 @@ def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: i
     }
 
 
+def config2_bench(seed: int, sizes: list, width: int, height: int, spp: int, cpu: bool) -> dict:
+    """Wavefront path tracing at scale: the procedural interior (displaced
+    icospheres and boxes, 64 quad lights, matte / plastic / mirror / glass,
+    1024^2 textures) at each triangle count, 3 treelet rounds, max depth 16
+    (Russian roulette from bounce 3), width x height at spp samples per
+    pixel, with `path` and with `rlpath` (64^3 Q entries x 128 bins,
+    material-sorted hits). Mrays/s = (closest + shadow rays) / device time
+    of the render (scene upload and BVH build excluded). The CPU oracle
+    renders a 256x256 crop at 4 spp of the first scene with `path`, over the
+    GPU's BVH (bit-identical to the oracle's own; its 3 treelet rounds would
+    take the oracle ~30 s single-threaded), timed on this host's cores."""
+    import numpy as np
+    import oracle
+    from cavalry_amd import Context, render as R, scenes
+    from cavalry_amd.camera import look_at
+    ctx = Context(0)
+    out = {"workload": (f"interior scenes, {width}x{height}, {spp} spp, max depth 16, 3 treelet rounds; "
+                        "path, and rlpath with 64^3 Q entries"), "scenes": []}
+    for k, n in enumerate(sizes):
+        log(f"[bench] config 2: interior, {n} triangles")
+        arr = scenes.interior(n, seed=seed + k, width=width, height=height, max_depth=16)
+        sc = R.Scene(ctx, arr, bvh_rounds=3)
+        cam = look_at(arr.eye, arr.center, arr.up, arr.fov, width, height)
+        rec = {"n_tris": int(len(arr.tris))}
+        R.render(ctx, sc, 1, camera=cam, max_depth=16)  # warm-up (allocates)
+        for integ, kw in (("path", {}), ("rlpath", {"q_grid": 64, "sort_by_material": True})):
+            film, st = R.render(ctx, sc, spp, integrator=integ, camera=cam, max_depth=16, **kw)
+            rays = int(st["closest_rays"] + st["shadow_rays"])
+            rec[integ] = {"mrays_s": round(rays / (st["ms_total"] * 1e-3) / 1e6, 1),
+                          "ms": round(float(st["ms_total"]), 1), "rays": rays,
+                          "mean": round(float(film.image.mean()), 5)}
+            log(f"[bench] config 2: {n} tris, {integ}: {rec[integ]['mrays_s']} Mrays/s ({rec[integ]['ms']} ms)")
+        if cpu and k == 0:
+            ex = sc.bvh().export()
+            nd = oracle.Nodes(len(arr.tris))
+            for key in ("box", "sa", "cost", "left", "right", "parent", "prim"):
+                setattr(nd, key, np.ascontiguousarray(ex[key]))
+            x0, y0 = width // 2 - 128, height // 2 - 128
+            t0 = time.perf_counter()
+            _, _, rays_c = oracle.render_path(arr, 4, cam, crop=(x0, y0, x0 + 256, y0 + 256), max_depth=16, bvh=nd)
+            dt = time.perf_counter() - t0
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            out["cpu_baseline"] = {
+                "value": round(rays_c / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                "sample": (f"path, the 256x256 crop at the image centre at 4 spp, max depth 16, on the "
+                           f"{len(arr.tris)}-triangle scene; {dt:.1f} s on {threads} threads")}
+            log(f"[bench] config 2: CPU oracle crop {out['cpu_baseline']['value']} Mrays/s")
+        out["scenes"].append(rec)
+        sc.close()
+    ctx.close()
+    return out
+
+
+def cpu_one_thread(n: int, nrays: int, rounds: int, seed: int) -> dict:
+    """The CPU oracle's LBVH build and traversal on one thread (a child
+    process with OMP_NUM_THREADS=1), on a bounded sample."""
+    import subprocess
+    code = (f"import json, bench; print(json.dumps(bench.cpu_baseline({n}, {nrays}, {rounds}, {seed})))")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +402,10 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--serve-no-graphs", action="store_true")
     ap.add_argument("--no-equal-time", action="store_true")
+    ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--c2-tris", default="2000000,8000000", help="triangle counts of the at-scale interiors")
+    ap.add_argument("--c2-res", default="1920x1080")
+    ap.add_argument("--c2-spp", type=int, default=64)
     ap.add_argument("--et-tris", type=int, default=2_000_000)
     ap.add_argument("--et-res", default="320x180", help="WxH of the equal-time comparison")
     ap.add_argument("--et-ref-spp", type=int, default=16384)
@@ -558,6 +628,11 @@ def main() -> int:
     if not args.no_render:
         log("[bench] path tracing (Cornell box 512x512x16)")
         out["path_tracing"] = render_bench(args.renders, not args.no_cpu, not args.no_tune)
+    if not args.no_config2:
+        w, h = (int(x) for x in args.c2_res.lower().split("x"))
+        sizes = [int(x) for x in args.c2_tris.split(",") if x]
+        log(f"[bench] path tracing at scale ({w}x{h}, {args.c2_spp} spp, {sizes} tris)")
+        out["path_tracing_at_scale"] = config2_bench(args.seed, sizes, w, h, args.c2_spp, not args.no_cpu)
     if not args.no_equal_time:
         w, h = (int(x) for x in args.et_res.lower().split("x"))
         log(f"[bench] rlpath vs path at equal time ({w}x{h}, {args.et_tris} tris)")
@@ -569,6 +644,8 @@ def main() -> int:
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)
+        log("[bench] CPU oracle baseline on one thread at N=2^18, rays=2^18")
+        out["cpu_baseline"]["one_thread"] = cpu_one_thread(1 << 18, 1 << 18, args.rounds, args.seed)
     print(json.dumps(out), flush=True)
     if trav["overflow_closest"] or trav["overflow_occluded"]:
         print("bench: traversal stack overflow in the timed steps (results partial)", file=sys.stderr)

@@ -179,9 +179,10 @@ class BVH:
         """64-byte records (node visits + triangle tests) fetched by the
         closest-hit and any-hit launches since the last reset, and the rays
         of the last any-hit launch finished by the cooperative pass."""
-        out = np.zeros(3, np.uint64)
+        out = np.zeros(4, np.uint64)
         check(self.ctx.handle, lib().cav_trace_fetches(self.ctx.handle, self.handle, _ptr(out), 1 if reset else 0))
-        return {"closest": int(out[0]), "occluded": int(out[1]), "spilled_rays": int(out[2])}
+        return {"closest": int(out[0]), "occluded": int(out[1]), "spilled_rays": int(out[2]),
+                "respilled_rays": int(out[3])}
 
     def max_depth(self) -> int:
         """Deepest leaf, in edges from the root."""

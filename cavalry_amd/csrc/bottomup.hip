@@ -102,6 +102,22 @@ __device__ __forceinline__ int4 load_rec_agent(const int4* p) {
     return v;
 }
 
+// The topology record but its parent of the node with leaves [l, r] and
+// split g (internal; children as buildRadixTree numbers them,
+// BVH/BVH.cu:186-190), or of a leaf (g = ~prim).
+__device__ __forceinline__ int4 node_rec(int g, int l, int r, int n) {
+    if (g < 0) return make_int4(-1, -1, 0, ~g);
+    return make_int4(l == g ? n - 1 + g : g, g + 1 == r ? n - 1 + r : g + 1, 0, -1);
+}
+
+// A workgroup barrier for LDS data only: this wave's LDS operations are
+// done (lgkmcnt(0)), while its global loads (the leaf's gather) stay in
+// flight across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt(63) expcnt(7) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+}
+
 // Inclusive min-scan of one value per thread over the workgroup.
 __device__ __forceinline__ int block_min_scan(int v, int* wmin) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -111,9 +127,9 @@ __device__ __forceinline__ int block_min_scan(int v, int* wmin) {
         if (lane >= d) v = min(v, o);
     }
     if (lane == 63) wmin[wave] = v;
-    __syncthreads();
+    lds_barrier();
     for (int w = 0; w < wave; ++w) v = min(v, wmin[w]);
-    __syncthreads();
+    lds_barrier();
     return v;
 }
 
@@ -132,37 +148,49 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
     __shared__ uint32_t gate[kBlock];     // arrivals << 16 | height, per local split
     __shared__ NodeBox sbox[2 * kBlock];  // per local split: left, right child's box
     __shared__ int sfar[2 * kBlock];      // per local split: left child's first, right child's last leaf
-    __shared__ int4 srec[2 * kBlock];     // per local split: left, right child's {left, right, -, prim}
+    __shared__ int ssplit[2 * kBlock];    // per local split: left, right child's split, or ~prim for a leaf
 
     const int k0 = blockIdx.x * kBlock;
     const int t = threadIdx.x;
-    for (int i = t; i < kBlock + 2; i += kBlock) {
-        const int64_t j = (int64_t)k0 - 1 + i;
-        skey[i] = (j >= 0 && j < n) ? codes[j] : (K)0;
+    const int k = k0 + t;
+    // codes of positions k0-1+t and k0+255+t (the latter for t < 2)
+    const int64_t j0 = (int64_t)k0 - 1 + t, j1 = j0 + kBlock;
+    const K key0 = (j0 >= 0 && j0 < n) ? codes[j0] : (K)0;
+    const K key1 = (t < 2 && j1 < n) ? codes[j1] : (K)0;
+    // the leaf's triangle (BVH/BVH.cu:46-56), a random gather: issued before
+    // the setup below, which syncs on LDS only, so that it stays in flight
+    int tri = 0;
+    v3 a = mk3(0.f, 0.f, 0.f), b = a, c = a;
+    if (k < n) {
+        tri = (int)order[k];
+        a = load_tri_v(tris, tri, 0);
+        b = load_tri_v(tris, tri, 1);
+        c = load_tri_v(tris, tri, 2);
     }
+    skey[t] = key0;
+    if (t < 2) skey[kBlock + t] = key1;
     gate[t] = 0u;
-    __syncthreads();
+    lds_barrier();
     for (int i = t; i <= kBlock; i += kBlock) {
         const int64_t j = (int64_t)k0 - 1 + i;
         sdel[i] = (j < 0 || j >= n - 1) ? -1 : delta_of<K>(skey[i], skey[i + 1], keybits, (uint32_t)j);
     }
-    __syncthreads();
+    lds_barrier();
     // split g = k0 + t: a smaller delta among boundaries k0-1 .. g-1 (prefix
     // min of sdel[0..t]) and among g+1 .. k0+255 (suffix min of sdel[t+2..256])
     const int pre = block_min_scan(sdel[t], wmin);
     const int suf_rev = block_min_scan(sdel[kBlock - t], wmin);  // min of sdel[256-t .. 256]
     __shared__ int ssuf[kBlock];
     ssuf[kBlock - 1 - t] = suf_rev;  // ssuf[i] = min of sdel[i+1 .. 256]
-    __syncthreads();
+    lds_barrier();
     {
         const int g = k0 + t;
         const int dg = sdel[t + 1];
         const int suf = t + 1 < kBlock ? ssuf[t + 1] : 0x7fffffff;  // min of sdel[t+2 .. 256]
         slocal[t] = (t < kBlock - 1 && g <= n - 2 && pre < dg && suf < dg) ? 1 : 0;
     }
-    __syncthreads();
+    lds_barrier();
 
-    const int k = k0 + t;
     if (k >= n) return;
     auto delta = [&](int j) -> int {  // delta of boundary j, -1 outside [0, n-2]
         if (j < 0 || j >= n - 1) return -1;
@@ -171,18 +199,16 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
         return delta_of<K>(codes[j], codes[j + 1], keybits, (uint32_t)j);
     };
 
-    // the leaf (BVH/BVH.cu:46-56)
-    const int tri = (int)order[k];
-    const v3 a = load_tri_v(tris, tri, 0), b = load_tri_v(tris, tri, 1), c = load_tri_v(tris, tri, 2);
     const v3 lo = mk3(fmn(a.x, fmn(b.x, c.x)), fmn(a.y, fmn(b.y, c.y)), fmn(a.z, fmn(b.z, c.z)));
     const v3 hi = mk3(fmx(a.x, fmx(b.x, c.x)), fmx(a.y, fmx(b.y, c.y)), fmx(a.z, fmx(b.z, c.z)));
     const float sa0 = surface_area(lo, hi);
     NodeBox cb{make_float4(lo.x, lo.y, lo.z, sa0), make_float4(hi.x, hi.y, hi.z, leaf_cost(sa0))};
     int cur = n - 1 + k;
-    // cur's topology record but its parent: {left, right, -, prim}. Each
-    // node's whole record is written once, by the arrival that completes its
-    // parent (the only one that knows the parent's id).
-    int4 crec = make_int4(-1, -1, 0, tri);
+    // cur's split, or ~prim for a leaf: with its range it gives cur's whole
+    // topology record but the parent. Each node's record is written once, by
+    // the arrival that completes its parent (the only one that knows the
+    // parent's id).
+    int csplit = ~tri;
     int l = k, r = k;
     uint32_t h = 0;
     // side of cur in its parent: cur is the left child iff delta(r) > delta(l-1)
@@ -194,14 +220,14 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
         uint32_t hs;
         int ofar;
         NodeBox sib;
-        int4 srec_o;
+        int4 srec_o;  // the sibling's record but its parent
         const int li = g - k0;
         const bool local = li >= 0 && li < kBlock && slocal[li];
         if (local) {
             nbox[cur] = cb;
             sbox[2 * li + (left ? 0 : 1)] = cb;
             sfar[2 * li + (left ? 0 : 1)] = far;
-            srec[2 * li + (left ? 0 : 1)] = crec;
+            ssplit[2 * li + (left ? 0 : 1)] = csplit;
             __threadfence_block();
             const uint32_t old = atomicAdd(&gate[li], (1u << 16) | hp);
             if ((old >> 16) == 0u) return;  // first arrival: the sibling continues
@@ -209,13 +235,15 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
             hs = old & 0xffffu;
             sib = sbox[2 * li + (left ? 1 : 0)];
             ofar = sfar[2 * li + (left ? 1 : 0)];
-            srec_o = srec[2 * li + (left ? 1 : 0)];
+            const int os = ssplit[2 * li + (left ? 1 : 0)];
+            // the sibling's leaves: [g + 1, ofar] (right) or [ofar, g] (left)
+            srec_o = left ? node_rec(os, g + 1, ofar, n) : node_rec(os, ofar, g, n);
         } else {
             // the box, and the topology record without its parent, for the
             // sibling's arrival (the record is rewritten whole by whichever
             // arrival completes the parent)
             store_box_agent(nbox + cur, cb);
-            store_rec_agent(topo + cur, crec);
+            store_rec_agent(topo + cur, node_rec(csplit, l, r, n));
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             __builtin_amdgcn_s_waitcnt(0);  // the stores have completed
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -245,11 +273,12 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
             pleft = delta(rp) > delta(lp - 1);
             pid = pleft ? rp : lp;
         }
+        const int4 crec = node_rec(csplit, l, r, n);
         const int4 lrec = left ? crec : srec_o, rrec = left ? srec_o : crec;
         topo[lid] = make_int4(lrec.x, lrec.y, pid, lrec.w);
         topo[rid] = make_int4(rrec.x, rrec.y, pid, rrec.w);
         hw[pid] = (2u << 16) | min(h, 0xffffu);
-        crec = make_int4(lid, rid, 0, -1);
+        csplit = g;
         if (root) {
             topo[0] = make_int4(lid, rid, 0, -1);
             nbox[0] = pb;

@@ -418,15 +418,16 @@ int cav_trace_counters(cav_ctx* ctx, const cav_bvh* h, uint32_t* out4, int32_t r
     return CAV_OK;
 }
 
-int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out3, int32_t reset) {
-    if (!ctx || !h || !out3) return CAV_ERR_INVALID;
+int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out4, int32_t reset) {
+    if (!ctx || !h || !out4) return CAV_ERR_INVALID;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    uint32_t spilled = 0;
-    HIPCHK(ctx, hipMemcpyAsync(out3, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&spilled, h->b.counters + 20, 4, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t w[3] = {0, 0, 0};  // spill count, spill cursor, failures
+    HIPCHK(ctx, hipMemcpyAsync(out4, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(w, h->b.counters + 20, 12, hipMemcpyDeviceToHost, ctx->stream));
     if (reset) HIPCHK(ctx, hipMemsetAsync(h->b.counters + 16, 0, 16, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    out3[2] = spilled;
+    out4[2] = w[0] < h->spill_cap ? w[0] : h->spill_cap;
+    out4[3] = w[2];
     return CAV_OK;
 }
 

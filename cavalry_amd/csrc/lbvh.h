@@ -118,10 +118,14 @@ struct AnySpill {
     uint32_t item[kSpillItems];    // (base slot >> 1) << 2 | active mask
     uint8_t byte[kSpillItems];     // s of the far or only member | 128: near is the right one
 };
-// spill capacity worth reserving for a batch of nrays any-hit rays
+// spill capacity for a batch of nrays any-hit rays. Rays spill only once
+// every ray has been handed out, so about one per lane of the persistent grid
+// (at most 8 workgroups x 256 lanes on 256 CUs) plus the few rays a wave
+// still holds in its chunk; a full pool only means more rays finish on
+// their lane.
 inline uint32_t any_spill_capacity(int64_t nrays) {
-    const int64_t c = nrays / 32 + 4096;
-    return (uint32_t)(c < (int64_t)(1u << 30) ? c : (1u << 30));
+    const int64_t cap = int64_t(1) << 20;
+    return (uint32_t)(nrays < cap ? nrays : cap);
 }
 
 // spill: nullptr (every ray traced by one lane) or spill_cap records

@@ -88,11 +88,6 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
 // with many coincident Morton codes). Rays are handed out per wave from
 // chunks taken with one atomic. Each ray's node order is the reference's, so
 // results do not depend on which lane or wave traced it.
-//
-// When both children of a node are hit and the nearer one is a leaf, the
-// reference tests that leaf and then pops its far sibling straight away
-// (BVH/BVH.h:72-95): here one step fetches both records, which share one
-// 128-byte line in the sibling-pair layout, and does both in that order.
 template <int kLdsDepth>
 __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
@@ -115,11 +110,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
     float best = FLT_MAX, bu = 0.f, bv = 0.f;
     int bp = -1, cur = 0, steps = 0;
     bool hit = false;
-    bool fused = false;  // cur is a near leaf whose far sibling pfar comes in the same step
-    int pfar = 0;
     uint32_t ovf = 0;
     int maxsp = 0;  // deepest stack this lane reached
-    uint64_t nfetch = 0;  // 64-byte records this lane fetched
+    uint64_t nfetch = 0;  // 64-byte records this lane fetched (one per step)
     volatile int spill[kStack - kLdsDepth];  // in scratch memory, not registers
     int sp = 0;
     int64_t chunk_base = 0;
@@ -153,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                     load_ray(rays, ri, o, d);
                     inv = div3(mk3(1.f, 1.f, 1.f), d);
                     best = FLT_MAX; hit = false; bp = -1; bu = 0.f; bv = 0.f;
-                    sp = 0; cur = 0; steps = 0; fused = false;
+                    sp = 0; cur = 0; steps = 0;
                     // BVH/BVH.h:34-38: a ray missing the root box has no hit
                     if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
                         oprim[ri] = -1; ot[ri] = 0.f;
@@ -176,65 +169,42 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
             ++ovf;
             done = true;
         } else {
-            // One fetch for either kind of step, so that a wave with lanes at
-            // nodes and lanes at triangles waits for memory once.
+            // One 64-byte fetch for either kind of step, so that a wave with
+            // lanes at nodes and lanes at triangles waits for memory once.
             const bool at_node = cur >= 0;
+            ++nfetch;
             const float4* rec = reinterpret_cast<const float4*>(recs + (at_node ? cur : ~cur));
             float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
             int4 refs = reinterpret_cast<const int4*>(rec)[3];
             // (an empty asm that takes the node-only words keeps the compiler
             // from moving their loads into the node branch, a second wait)
             asm volatile("" : "+v"(q1.w), "+v"(q2.w), "+v"(refs.x), "+v"(refs.y));
-            float4 f0 = q0, f1 = q1, f2 = q2;
-            int4 frefs = refs;
-            const bool pair = fused;
-            if (pair) {  // the far sibling, in the same 128-byte line
-                const float4* fr = reinterpret_cast<const float4*>(recs + (pfar >= 0 ? pfar : ~pfar));
-                f0 = fr[0]; f1 = fr[1]; f2 = fr[2];
-                frefs = reinterpret_cast<const int4*>(fr)[3];
-            }
-            nfetch += pair ? 2 : 1;
-            fused = false;
-            bool next = false;  // cur set for the next step
-            // BVH/BVH.h:72-95 at a node
-            auto visit_node = [&](const float4& a, const float4& b, const float4& c, const int4& rf) {
-                const float dl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv);
-                const float dr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv);
+            if (at_node) {  // BVH/BVH.h:72-95
+                const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
+                const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
                 const bool hl = dl >= 0 && dl < best;
                 const bool hr = dr >= 0 && dr < best;
+                bool next = false;
                 if (hl && hr) {
                     const bool lfar = dl > dr;
-                    const int far = lfar ? rf.x : rf.y;
-                    const int near = lfar ? rf.y : rf.x;
-                    if (near < 0 && sp + 1 < kStack) {
-                        // the reference pushes far (sp + 1 entries), tests the
-                        // near leaf and pops far again: both in the next fetch
-                        maxsp = max(maxsp, sp + 1);
-                        cur = near; pfar = far; fused = true; next = true;
-                    } else {
-                        if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
-                        maxsp = max(maxsp, sp);
-                        if (sp < kStack) { cur = near; next = true; } else ++ovf;
-                    }
+                    const int far = lfar ? refs.x : refs.y;
+                    const int near = lfar ? refs.y : refs.x;
+                    if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
+                    maxsp = max(maxsp, sp);
+                    if (sp < kStack) { cur = near; next = true; } else ++ovf;
                 } else if (hl || hr) {
-                    if (sp < kStack) { cur = hl ? rf.x : rf.y; next = true; } else ++ovf;
+                    if (sp < kStack) { cur = hl ? refs.x : refs.y; next = true; } else ++ovf;
                 }
-            };
-            // BVH/BVH.h:58-70 at a leaf
-            auto visit_leaf = [&](const float4& a, const float4& b, const float4& c) {
-                const TTri tr{a, b, c};
+                if (!next) {
+                    if (sp == 0) done = true;
+                    else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
+                }
+            } else {  // BVH/BVH.h:58-70
+                const TTri tr{q0, q1, q2};
                 float t, u, v;
                 if (tri_hit(tr, o, d, t, u, v) && (!hit || t < best)) {
                     hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
                 }
-            };
-            if (at_node) visit_node(q0, q1, q2, refs);
-            else visit_leaf(q0, q1, q2);
-            if (pair) {  // the far sibling, as the reference's pop after the near leaf
-                if (pfar >= 0) visit_node(f0, f1, f2, frefs);
-                else visit_leaf(f0, f1, f2);
-            }
-            if (!next) {
                 if (sp == 0) done = true;
                 else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
             }

@@ -129,8 +129,10 @@ int cav_trace_counters(cav_ctx* ctx, const cav_bvh* bvh, uint32_t* out, int32_t 
 /* Work of a BVH's standalone launches: out[0] / out[1] = 64-byte traversal
  * records (node visits and triangle tests, one record each) fetched by every
  * closest-hit / any-hit launch since the last reset; out[2] = rays of the
- * last any-hit launch that outgrew one lane's step budget and were finished
- * by a whole wave. reset != 0 clears out[0..1] after reading. No counterpart
+ * last any-hit launch finished by a whole wave (long rays still in flight
+ * once every ray was handed out), out[3] = those of them the wave could not
+ * hold and one lane traced again from the root. reset != 0 clears out[0..1]
+ * after reading. No counterpart
  * in the reference; it measures the algorithmic traffic of BVH::intersect /
  * testVisibility (BVH/BVH.h:23-189). */
 int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* bvh, uint64_t* out, int32_t reset);
