@@ -117,6 +117,7 @@ _SIGS = {
     "cav_trace_occluded_async": ([P, P, P, P, I64, P], I),
     "cav_trace_counters": ([P, P, P, ctypes.c_int32], I),
     "cav_trace_fetches": ([P, P, P, ctypes.c_int32], I),
+    "cav_trace_closest_noalloc": ([P, P, P, ctypes.c_int64, P, P, P, P], I),
     "cav_bvh_max_depth": ([P, P, ctypes.POINTER(ctypes.c_int32)], I),
     "cav_camera_look_at": ([P, P, P, ctypes.c_float, I, I, ctypes.POINTER(Camera)], I),
     "cav_scene_create": ([P, P, I, P, I, P, I, I, ctypes.POINTER(P)], I),

@@ -34,6 +34,7 @@ SOURCES = [
     "pbrt.cpp",
     "server.cpp",
     "capi.cpp",
+    "noalloc.hip",
     "camera.cpp",
     "ctx_release.cpp",
 ]

@@ -187,6 +187,9 @@ int cav_ctx_create(int device, cav_ctx** out) {
     if (device < 0 || device >= count) return CAV_ERR_INVALID;
     e = hipSetDevice(device);
     if (e != hipSuccess) return CAV_ERR_HIP;
+    // the no-alloc entry point's staging array on this device (its failure
+    // only disables that entry point, which then reports it)
+    (void)noalloc_init(device);
     cav_ctx* c = new cav_ctx();
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -45,4 +45,6 @@ int hip_fail(cav_ctx* ctx, hipError_t e, const char* what);
 int dev_alloc(cav_ctx* ctx, void** p, size_t bytes, const char* what);
 // frees context-owned state of other subsystems before the stream goes away
 void ctx_release(cav_ctx* ctx);
+// noalloc.hip: finds the no-alloc staging array on `device` (current), once per device
+hipError_t noalloc_init(int device);
 }  // namespace cav

@@ -111,6 +111,17 @@ int cav_trace_occluded(cav_ctx* ctx, const cav_bvh* bvh, const float* rays, cons
 int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
                               const float* limit_dev, const int32_t* src_mesh_dev,
                               const int32_t* dst_mesh_dev, int64_t nrays, uint8_t* visible_dev);
+/* cav_trace_closest without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as cav_trace_closest (host buffers), staged
+ * through a static device buffer of CAV_NOALLOC_MAX_RAYS rays that each GPU
+ * holds until the process ends; larger batches are refused with
+ * CAV_ERR_INVALID. Calls on one GPU share that buffer, so calls that overlap
+ * on one GPU (several contexts or threads) race. */
+#define CAV_NOALLOC_MAX_RAYS 262144
+int cav_trace_closest_noalloc(cav_ctx* ctx, const cav_bvh* bvh, const float* rays, int64_t nrays,
+                              int32_t* hit_prim, float* hit_t, float* hit_u, float* hit_v);
 /* Launch without synchronising (device pointers); errors surface at the next
  * synchronising call. Used by timing harnesses. */
 int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays_dev,
