@@ -20,19 +20,25 @@
 // (the reference's computeBounds gate), produces the reference's node ids,
 // children, parents and boxes without the top-down searches.
 //
-// Each workgroup takes 256 consecutive sorted leaves. A split whose node's
-// leaves all lie in the workgroup (nearer boundaries of smaller delta on both
-// sides inside the workgroup, found by a min-scan over the workgroup's deltas)
-// is gated in LDS, and the first arrival leaves its child's box there. The
-// few other splits use a 64-bit word in HBM: the first arrival writes its
-// child's box and topology record with agent-scope stores, waits for them to
-// complete, then adds
-// its far range end and height to the word; the second arrival reads the
-// word, clears it for the next build and loads the box with agent-scope
-// loads. Every access that crosses workgroups is an agent-scope atomic, so
-// no cache write-back or invalidate is needed. Results do not depend on
-// which arrival is second: both children are final when their parent is
-// computed, as in the reference.
+// Two passes. Pass 1 gives each workgroup 256 consecutive sorted leaves. A
+// split whose node's leaves all lie in the workgroup (nearer boundaries of
+// smaller delta on both sides inside the workgroup, found by a min-scan over
+// the workgroup's deltas) is gated in LDS: the first arrival leaves its
+// child's box there, the second completes the node and climbs on. A climb
+// ends at the first split that crosses workgroups, where the child's box,
+// topology record and an arrival word (far leaf, height, the delta of its
+// outer range boundary) are left in HBM; no wave waits on another
+// workgroup. About 2% of the splits cross workgroups. Pass 2 climbs them:
+// a split both of whose children pass 1 completed sits at a workgroup edge,
+// and one thread per edge starts there. Its arrival at a split higher up
+// stores the child's box and record with agent-scope stores, waits for them
+// to complete and adds its arrival word to the split's 64-bit gate; the
+// second arrival reads the first's word back from the add, clears the gate
+// for the next build and loads the box and record with agent-scope loads.
+// Every access that crosses workgroups within pass 2 is an agent-scope
+// atomic, so no cache write-back or invalidate is needed. Results do not
+// depend on which arrival is second: both children are final when their
+// parent is computed, as in the reference.
 #include "dev_common.h"
 #include "lbvh.h"
 
@@ -133,8 +139,29 @@ __device__ __forceinline__ int block_min_scan(int v, int* wmin) {
     return v;
 }
 
-constexpr uint64_t kArrive = 1ull << 48;
+// A child's arrival at a split that crosses workgroups, in one word: an
+// arrival bit, the delta of the child's outer range boundary + 1 (7 bits:
+// deltas are at most 94, -1 for none), the child's height + 1 (16 bits) and
+// the child's far leaf (its first leaf for a left child, last for a right).
+constexpr uint64_t kArrive = 1ull << 63;
+__device__ __forceinline__ uint64_t pack_arrival(int far, uint32_t hp, int dout) {
+    return kArrive | ((uint64_t)(uint32_t)(dout + 1) << 56) | ((uint64_t)hp << 32) | (uint32_t)far;
+}
+__device__ __forceinline__ int arrival_far(uint64_t w) { return (int)(uint32_t)w; }
+__device__ __forceinline__ uint32_t arrival_hp(uint64_t w) { return (uint32_t)(w >> 32) & 0xffffu; }
+__device__ __forceinline__ int arrival_dout(uint64_t w) { return (int)((w >> 56) & 0x7fu) - 1; }
 
+// Split g is the last boundary of a workgroup: both its children can lie
+// inside workgroups (one each), so both can arrive in the local pass. Such a
+// split's arrivals go to two words of their own (left, right child) behind
+// the gates, and its gate counts arrivals.
+__device__ __forceinline__ bool edge_split(int g) { return g % kBlock == kBlock - 1; }
+
+// Pass 1: one workgroup per 256 consecutive sorted leaves. Every leaf climbs
+// through the splits whose node lies inside the workgroup (gated in LDS, the
+// second arrival continuing); a climb ends at the first split that crosses
+// workgroups, where it leaves its node's box, topology record and arrival
+// word in HBM for pass 2. No wave waits on another workgroup.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ codes, int n, int keybits,
                                                       const uint32_t* __restrict__ order,
@@ -157,16 +184,11 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
     const int64_t j0 = (int64_t)k0 - 1 + t, j1 = j0 + kBlock;
     const K key0 = (j0 >= 0 && j0 < n) ? codes[j0] : (K)0;
     const K key1 = (t < 2 && j1 < n) ? codes[j1] : (K)0;
-    // the leaf's triangle (BVH/BVH.cu:46-56), a random gather: issued before
-    // the setup below, which syncs on LDS only, so that it stays in flight
-    int tri = 0;
-    v3 a = mk3(0.f, 0.f, 0.f), b = a, c = a;
-    if (k < n) {
-        tri = (int)order[k];
-        a = load_tri_v(tris, tri, 0);
-        b = load_tri_v(tris, tri, 1);
-        c = load_tri_v(tris, tri, 2);
-    }
+    // the leaf's triangle (BVH/BVH.cu:46-56), a random gather, issued by every
+    // thread (past the end: the last leaf's) so that no branch forces a wait
+    // for it before the setup below, which syncs on LDS only
+    const int tri = (int)order[k < n ? k : n - 1];
+    const v3 a = load_tri_v(tris, tri, 0), b = load_tri_v(tris, tri, 1), c = load_tri_v(tris, tri, 2);
     skey[t] = key0;
     if (t < 2) skey[kBlock + t] = key1;
     gate[t] = 0u;
@@ -192,12 +214,9 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
     lds_barrier();
 
     if (k >= n) return;
-    auto delta = [&](int j) -> int {  // delta of boundary j, -1 outside [0, n-2]
-        if (j < 0 || j >= n - 1) return -1;
-        const int i = j - k0 + 1;
-        if (i >= 0 && i <= kBlock) return sdel[i];
-        return delta_of<K>(codes[j], codes[j + 1], keybits, (uint32_t)j);
-    };
+    // delta of boundary j for j in [k0-1, k0+255]: every node this pass
+    // completes lies inside the workgroup, so its range boundaries are there
+    auto delta = [&](int j) -> int { return sdel[j - k0 + 1]; };
 
     const v3 lo = mk3(fmn(a.x, fmn(b.x, c.x)), fmn(a.y, fmn(b.y, c.y)), fmn(a.z, fmn(b.z, c.z)));
     const v3 hi = mk3(fmx(a.x, fmx(b.x, c.x)), fmx(a.y, fmx(b.y, c.y)), fmx(a.z, fmx(b.z, c.z)));
@@ -211,88 +230,169 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(const K* __restrict__ code
     int csplit = ~tri;
     int l = k, r = k;
     uint32_t h = 0;
-    // side of cur in its parent: cur is the left child iff delta(r) > delta(l-1)
-    bool left = delta(r) > delta(l - 1);
-    for (int step = 0; step < n; ++step) {
+    int dl = delta(k - 1), dr = delta(k);  // deltas of cur's range boundaries l-1 and r
+    for (int step = 0; step < kBlock; ++step) {
+        // side of cur in its parent: the left child iff delta(r) > delta(l-1)
+        const bool left = dr > dl;
         const int g = left ? r : l - 1;  // the parent's split
         const int far = left ? l : r;
         const uint32_t hp = min(h + 1u, 0xffffu);
-        uint32_t hs;
-        int ofar;
-        NodeBox sib;
-        int4 srec_o;  // the sibling's record but its parent
         const int li = g - k0;
-        const bool local = li >= 0 && li < kBlock && slocal[li];
-        if (local) {
+        if (!(li >= 0 && li < kBlock && slocal[li])) {
+            // a split that crosses workgroups: pass 2 completes it
             nbox[cur] = cb;
-            sbox[2 * li + (left ? 0 : 1)] = cb;
-            sfar[2 * li + (left ? 0 : 1)] = far;
-            ssplit[2 * li + (left ? 0 : 1)] = csplit;
-            __threadfence_block();
-            const uint32_t old = atomicAdd(&gate[li], (1u << 16) | hp);
-            if ((old >> 16) == 0u) return;  // first arrival: the sibling continues
-            __threadfence_block();
-            hs = old & 0xffffu;
-            sib = sbox[2 * li + (left ? 1 : 0)];
-            ofar = sfar[2 * li + (left ? 1 : 0)];
-            const int os = ssplit[2 * li + (left ? 1 : 0)];
-            // the sibling's leaves: [g + 1, ofar] (right) or [ofar, g] (left)
-            srec_o = left ? node_rec(os, g + 1, ofar, n) : node_rec(os, ofar, g, n);
-        } else {
-            // the box, and the topology record without its parent, for the
-            // sibling's arrival (the record is rewritten whole by whichever
-            // arrival completes the parent)
-            store_box_agent(nbox + cur, cb);
-            store_rec_agent(topo + cur, node_rec(csplit, l, r, n));
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            __builtin_amdgcn_s_waitcnt(0);  // the stores have completed
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const uint64_t add = kArrive | ((uint64_t)hp << 32) | (uint32_t)far;
-            const uint64_t old = __hip_atomic_fetch_add(gate64 + g, (unsigned long long)add, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-            if ((old >> 48) == 0u) return;
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            __hip_atomic_store(gate64 + g, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            hs = (uint32_t)(old >> 32) & 0xffffu;
-            ofar = (int)(uint32_t)old;
+            topo[cur] = node_rec(csplit, l, r, n);
+            const uint64_t w = pack_arrival(far, hp, left ? dl : dr);
+            if (edge_split(g)) {
+                gate64[n + 2 * ((g + 1) / kBlock) + (left ? 0 : 1)] = w;
+                (void)__hip_atomic_fetch_add(gate64 + g, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                gate64[g] = w;  // the only arrival in this pass: the sibling crosses workgroups
+            }
+            return;
         }
-        // the parent: leaves [lp, rp], split g
+        nbox[cur] = cb;
+        sbox[2 * li + (left ? 0 : 1)] = cb;
+        sfar[2 * li + (left ? 0 : 1)] = far;
+        ssplit[2 * li + (left ? 0 : 1)] = csplit;
+        __threadfence_block();
+        const uint32_t old = atomicAdd(&gate[li], (1u << 16) | hp);
+        if ((old >> 16) == 0u) return;  // first arrival: the sibling continues
+        __threadfence_block();
+        const uint32_t hs = old & 0xffffu;
+        const NodeBox sib = sbox[2 * li + (left ? 1 : 0)];
+        const int ofar = sfar[2 * li + (left ? 1 : 0)];
+        const int os = ssplit[2 * li + (left ? 1 : 0)];
+        // the parent: leaves [lp, rp], split g; the sibling's leaves are
+        // [g + 1, ofar] (right) or [ofar, g] (left)
         const int lp = left ? l : ofar, rp = left ? ofar : r;
         const int lid = lp == g ? n - 1 + g : g;
         const int rid = g + 1 == rp ? n - 1 + rp : g + 1;
-        if (!local) {  // the sibling's, written before its arrival
-            sib = load_box_agent(nbox + (left ? rid : lid));
-            srec_o = load_rec_agent(topo + (left ? rid : lid));
-        }
+        const int4 srec_o = left ? node_rec(os, g + 1, ofar, n) : node_rec(os, ofar, g, n);
         const NodeBox pb = left ? combine(cb, sib) : combine(sib, cb);
         h = max(h + 1u, hs);
         const bool root = lp == 0 && rp == n - 1;
-        bool pleft = false;
-        int pid = 0;
-        if (!root) {
-            pleft = delta(rp) > delta(lp - 1);
-            pid = pleft ? rp : lp;
-        }
+        const int dlp = delta(lp - 1), drp = delta(rp);
+        const int pid = root ? 0 : (drp > dlp ? rp : lp);
         const int4 crec = node_rec(csplit, l, r, n);
         const int4 lrec = left ? crec : srec_o, rrec = left ? srec_o : crec;
         topo[lid] = make_int4(lrec.x, lrec.y, pid, lrec.w);
         topo[rid] = make_int4(rrec.x, rrec.y, pid, rrec.w);
         hw[pid] = (2u << 16) | min(h, 0xffffu);
+        if (root) {  // all n leaves in this workgroup
+            topo[0] = make_int4(lid, rid, 0, -1);
+            nbox[0] = pb;
+            return;
+        }
         csplit = g;
+        cur = pid;
+        l = lp;
+        r = rp;
+        dl = dlp;
+        dr = drp;
+        cb = pb;
+    }
+}
+
+// Pass 2: the splits that cross workgroups, climbed from the workgroup
+// edges whose split had both children completed by pass 1 (one thread per
+// edge). A climb stops at a split whose other child has not arrived yet;
+// its arrival (box and record with agent-scope stores, waited for, then the
+// arrival word added to the gate) lets that child's climb continue. The
+// node ids, parents and boxes are those of pass 1's rule.
+__global__ __launch_bounds__(kBlock) void k_climb_crossing(int n, int nedges, NodeBox* nbox, int4* topo,
+                                                           uint32_t* hw, unsigned long long* gate64) {
+    const int e = blockIdx.x * kBlock + threadIdx.x + 1;  // the edge between workgroups e-1 and e
+    if (e > nedges) return;
+    unsigned long long* edge = gate64 + n;  // two arrival words per edge split
+    int g = e * kBlock - 1;
+    // (pass 1's words are read with agent-scope loads like pass 2's own)
+    if (__hip_atomic_load(gate64 + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 2ull)
+        return;  // a child crosses workgroups: its climb continues here
+    __hip_atomic_store(gate64 + g, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wl = __hip_atomic_load(edge + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wr = __hip_atomic_load(edge + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int lp = arrival_far(wl), rp = arrival_far(wr);
+    int dlp = arrival_dout(wl), drp = arrival_dout(wr);  // deltas of boundaries lp-1 and rp
+    uint32_t h = max(arrival_hp(wl), arrival_hp(wr));
+    int lid = lp == g ? n - 1 + g : g;
+    int rid = g + 1 == rp ? n - 1 + rp : g + 1;
+    NodeBox bl = load_box_agent(nbox + lid), br = load_box_agent(nbox + rid);
+    int4 lrec = load_rec_agent(topo + lid), rrec = load_rec_agent(topo + rid);
+    for (int step = 0; step < n; ++step) {
+        const NodeBox pb = combine(bl, br);
+        const bool root = lp == 0 && rp == n - 1;
+        const bool pleft = !root && drp > dlp;
+        const int pid = root ? 0 : (pleft ? rp : lp);
+        topo[lid] = make_int4(lrec.x, lrec.y, pid, lrec.w);
+        topo[rid] = make_int4(rrec.x, rrec.y, pid, rrec.w);
+        hw[pid] = (2u << 16) | min(h, 0xffffu);
         if (root) {
             topo[0] = make_int4(lid, rid, 0, -1);
             nbox[0] = pb;
             return;
         }
-        cur = pid;
-        l = lp;
-        r = rp;
-        cb = pb;
-        left = pleft;
+        // arrive at the split above node pid (leaves [lp, rp])
+        const int g2 = pleft ? rp : lp - 1;
+        const uint32_t hp = min(h + 1u, 0xffffu);
+        const uint64_t w = pack_arrival(pleft ? lp : rp, hp, pleft ? dlp : drp);
+        store_box_agent(nbox + pid, pb);
+        store_rec_agent(topo + pid, make_int4(lid, rid, 0, -1));
+        uint64_t other;
+        if (edge_split(g2)) {
+            const int e2 = (g2 + 1) / kBlock;
+            __hip_atomic_store(edge + 2 * e2 + (pleft ? 0 : 1), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);  // the stores have completed
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint64_t old = __hip_atomic_fetch_add(gate64 + g2, 1ull, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) return;  // first arrival
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __hip_atomic_store(gate64 + g2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            other = __hip_atomic_load(edge + 2 * e2 + (pleft ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);  // the stores have completed
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint64_t old = __hip_atomic_fetch_add(gate64 + g2, (unsigned long long)w, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) return;  // first arrival
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __hip_atomic_store(gate64 + g2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            other = old;
+        }
+        // the parent of pid and its sibling: leaves [lp', rp'], split g2
+        const int ofar = arrival_far(other);
+        const uint32_t hs = arrival_hp(other);
+        const int ds = arrival_dout(other);
+        if (pleft) {
+            rp = ofar;
+            drp = ds;
+        } else {
+            lp = ofar;
+            dlp = ds;
+        }
+        const int4 prec = make_int4(lid, rid, 0, -1);  // pid's record but its parent
+        g = g2;
+        lid = lp == g ? n - 1 + g : g;
+        rid = g + 1 == rp ? n - 1 + rp : g + 1;
+        const int sid = pleft ? rid : lid;
+        const NodeBox sb = load_box_agent(nbox + sid);
+        const int4 sr = load_rec_agent(topo + sid);
+        bl = pleft ? pb : sb;
+        br = pleft ? sb : pb;
+        lrec = pleft ? prec : sr;
+        rrec = pleft ? sr : prec;
+        h = max(hp, hs);
     }
 }
 
 }  // namespace
+
+// Words of the split gates for n leaves: n-1 gates, then two arrival words
+// per workgroup edge.
+int64_t bottom_up_gate_words(int n) { return (int64_t)n + 2 * ((int64_t)(n + kBlock - 1) / kBlock) + 2; }
 
 hipError_t lbvh_bottom_up(LbvhBuffers& b, const void* codes, int keybits, const float* tris, int n,
                           hipStream_t s, Marker* mk) {
@@ -306,6 +406,12 @@ hipError_t lbvh_bottom_up(LbvhBuffers& b, const void* codes, int keybits, const 
                            static_cast<const uint32_t*>(codes), n, keybits, b.order, tris, b.nbox, b.topo,
                            b.visited, b.gate64);
     mark(mk, "refit_bottom_up");
+    const int nedges = nb - 1;
+    if (nedges > 0) {
+        hipLaunchKernelGGL(k_climb_crossing, dim3((unsigned)((nedges + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           n, nedges, b.nbox, b.topo, b.visited, b.gate64);
+        mark(mk, "refit_crossing");
+    }
     return hipGetLastError();
 }
 

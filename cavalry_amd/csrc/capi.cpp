@@ -78,7 +78,7 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
         {(void**)&h->b.nbox, sizeof(NodeBox) * m, "nbox"},
         {(void**)&h->b.topo, sizeof(int4) * m, "topo"},
         {(void**)&h->b.visited, sizeof(uint32_t) * m, "visited"},
-        {(void**)&h->b.gate64, sizeof(uint64_t) * n, "split gates"},
+        {(void**)&h->b.gate64, sizeof(uint64_t) * bottom_up_gate_words(n), "split gates"},
         {(void**)&h->b.ref, sizeof(int32_t) * m, "ref"},
         {(void**)&h->b.partial, sizeof(uint32_t) * (lbvh_partials(m) + 1), "partial"},
         {(void**)&h->b.leaf_ids, sizeof(uint32_t) * n, "leaf_ids"},
@@ -96,7 +96,7 @@ static int bvh_alloc(cav_ctx* ctx, cav_bvh* h, int n) {
     }
     // the bottom-up build leaves every split gate at zero for the next build;
     // the traversal counters only ever add to (or max into) their words
-    HIPCHK(ctx, hipMemsetAsync(h->b.gate64, 0, sizeof(uint64_t) * n, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(h->b.gate64, 0, sizeof(uint64_t) * bottom_up_gate_words(n), ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 256, ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&h->b.h_lvl, sizeof(uint32_t) * (kMaxLevels + 2), hipHostMallocDefault));
     h->b.h_off = new uint32_t[kMaxLevels + 2]();
@@ -354,7 +354,13 @@ int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* l)
     if (l->ray_chunk < 64 || l->ray_chunk > 65536)
         return fail(ctx, CAV_ERR_INVALID, "ray_chunk must be in 64..65536");
     if (l->lds_stack != 32 && l->lds_stack != 48) return fail(ctx, CAV_ERR_INVALID, "lds_stack must be 32 or 48");
-    ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack};
+    const auto size_ok = [](int32_t v) { return v == 0 || v == 64 || v == 128 || v == 256; };
+    if (!size_ok(l->workgroup_size)) return fail(ctx, CAV_ERR_INVALID, "workgroup_size must be 0, 64, 128 or 256");
+    if (!size_ok(l->shade_workgroup_size))
+        return fail(ctx, CAV_ERR_INVALID, "shade_workgroup_size must be 0, 64, 128 or 256");
+    ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack,
+                                     l->workgroup_size ? l->workgroup_size : 256,
+                                     l->shade_workgroup_size ? l->shade_workgroup_size : 256};
     return CAV_OK;
 }
 
@@ -363,7 +369,7 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
     if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
         return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
     const TraceLaunch& l = ctx->launch[which];
-    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth};
+    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block};
     return CAV_OK;
 }
 

@@ -46,7 +46,7 @@ struct LbvhBuffers {
     NodeBox* nbox = nullptr;                  // 2n-1
     int4* topo = nullptr;                     // 2n-1
     uint32_t* visited = nullptr;              // 2n-1 (arrivals << 16 | height)
-    unsigned long long* gate64 = nullptr;     // n-1 split gates of the bottom-up build (zero between builds)
+    unsigned long long* gate64 = nullptr;     // bottom_up_gate_words(n): split gates and edge arrivals (gates zero between builds)
     int32_t* ref = nullptr;                   // 2n-1 (restructured trees)
     uint32_t* partial = nullptr;              // scan partials
     uint32_t* leaf_ids = nullptr;             // n (treelet round start list)
@@ -75,6 +75,7 @@ int64_t lbvh_partials(int64_t m);
 
 // bottomup.hip: node ids, children, parents, boxes and heights of the
 // radix tree over the sorted codes, in one climb from the leaves
+int64_t bottom_up_gate_words(int n);
 hipError_t lbvh_bottom_up(LbvhBuffers& b, const void* codes, int keybits, const float* tris, int n,
                           hipStream_t s, Marker* mk);
 
@@ -97,12 +98,16 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // records every launch fetched (never reset by a launch); counters[20..25]
 // are the cooperative any-hit pass's spill count, spill cursor, failure
 // count and ray dispenser (cleared by each launch).
-// Launch shape of the persistent traversal kernels. Each ray's node order is
-// fixed, so results do not depend on it; the autotuner picks it.
+// Launch shape of the persistent traversal kernels (and, for the render
+// loops, of the per-sample shading kernels). Each ray's node order is fixed
+// and each sample's shading is independent of where it runs, so results do
+// not depend on it; the autotuner picks it.
 struct TraceLaunch {
-    int per_cu = 0;      // workgroups per CU (0: as many as fit)
-    int chunk = 256;     // rays a wave takes from the dispenser at a time (64..4096)
-    int lds_depth = 32;  // stack entries in LDS (32 or 48); deeper ones in scratch
+    int per_cu = 0;         // workgroups per CU (0: as many as fit)
+    int chunk = 256;        // rays a wave takes from the dispenser at a time (64..4096)
+    int lds_depth = 32;     // stack entries in LDS (32 or 48); deeper ones in scratch
+    int block = 256;        // threads per traversal workgroup (64, 128 or 256)
+    int shade_block = 256;  // threads per shading workgroup (64, 128 or 256)
 };
 
 hipError_t trace_closest(const TRec* rec, int ntris, const float* root,

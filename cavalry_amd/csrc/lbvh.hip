@@ -161,6 +161,7 @@ __device__ __forceinline__ uint32_t morton_code<uint32_t>(v3 c) {
 // global add per digit and workgroup, over a bounded grid) and clears the
 // sort's look-back words.
 constexpr int kMortonBlocks = 1024;
+constexpr int kMortonUnroll = 4;
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tris, int64_t n,
                                                    const uint32_t* __restrict__ ob,
@@ -178,30 +179,45 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tri
     const v3 sc = scale3(add3(slo, shi), 0.5f);
     const v3 se = sub3(shi, slo);
     const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
-    // every wave runs the same trip count (the ballots need all lanes)
+    // every wave runs the same trip count (the ballots need all lanes); the
+    // triangles of kMortonUnroll trips are loaded before any is used (past
+    // the end: the last triangle again), so their loads overlap
     const int64_t trips = (n + stride - 1) / stride;
-    for (int64_t it = 0; it < trips; ++it) {
-        const int64_t i = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        const bool valid = i < n;
-        K key = 0;
-        if (valid) {
-            v3 lo, hi;
-            tri_box(load_v(tris, i, 0), load_v(tris, i, 1), load_v(tris, i, 2), lo, hi);
-            const v3 c = div3(sub3(scale3(add3(lo, hi), 0.5f), sc), se);
-            key = morton_code<K>(c);
-            keys[i] = key;
-            vals[i] = (uint32_t)i;
-        }
-        for (int p = 0; p < passes; ++p) {
-            const uint32_t d = (uint32_t)(key >> (8 * p)) & 255u;
-            uint64_t peers = __ballot(valid);
+    const int64_t first = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t it = 0; it < trips; it += kMortonUnroll) {
+        v3 va[kMortonUnroll], vb[kMortonUnroll], vc[kMortonUnroll];
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
+        for (int u = 0; u < kMortonUnroll; ++u) {
+            const int64_t i = (it + u) * stride + first;
+            const int64_t ic = i < n ? i : n - 1;
+            va[u] = load_v(tris, ic, 0);
+            vb[u] = load_v(tris, ic, 1);
+            vc[u] = load_v(tris, ic, 2);
+        }
+#pragma unroll
+        for (int u = 0; u < kMortonUnroll; ++u) {
+            const int64_t i = (it + u) * stride + first;
+            const bool valid = i < n;
+            K key = 0;
+            if (valid) {
+                v3 lo, hi;
+                tri_box(va[u], vb[u], vc[u], lo, hi);
+                const v3 c = div3(sub3(scale3(add3(lo, hi), 0.5f), sc), se);
+                key = morton_code<K>(c);
+                keys[i] = key;
+                vals[i] = (uint32_t)i;
             }
-            if (valid && (peers & lt) == 0) atomicAdd(&h[p * 256 + d], (uint32_t)__popcll(peers));
+            for (int p = 0; p < passes; ++p) {
+                const uint32_t d = (uint32_t)(key >> (8 * p)) & 255u;
+                uint64_t peers = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t m = __ballot(bit);
+                    peers &= bit ? m : ~m;
+                }
+                if (valid && (peers & lt) == 0) atomicAdd(&h[p * 256 + d], (uint32_t)__popcll(peers));
+            }
         }
     }
     __syncthreads();

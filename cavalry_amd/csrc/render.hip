@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(int64_t nq, int width, int sp
 // computeLighting + the shading half of evalMaterial + genNextRay for one
 // bounce (PathTracingIntegrator.cu:124-249); the per-task work is
 // shade::path_shade.
-__global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev sc, HaltonDev h,
+template <int B>
+__global__ __launch_bounds__(B) void k_shade(int depth, int dim0, SceneDev sc, HaltonDev h,
                                                   Queue qi, Queue qo, Shadow sh,
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t,
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
                                                   const float* __restrict__ hit_v,
                                                   float* __restrict__ res, uint32_t* counts,
                                                   int qin, int qout, const uint32_t* __restrict__ perm) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * B + threadIdx.x;
     const uint32_t n = counts[qin];
     bool live = j < (int64_t)n;
     const int64_t i = live ? task_slot(perm, j) : 0;
@@ -93,12 +94,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(int depth, int dim0, SceneDev 
                    mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]), qi.pdf[i], qi.delta[i] != 0, prim,
                    hit_t[i], hit_u ? hit_u[i] : 0.f, hit_u ? hit_v[i] : 0.f, res + (int64_t)sid * 3, o);
     }
-    const uint32_t ss = block_push<kBlock>(o.push_sh, &counts[2]);
+    const uint32_t ss = block_push<B>(o.push_sh, &counts[2]);
     if (o.push_sh) {
         write_shadow(sh, ss, o);
         sh.sid[ss] = sid;
     }
-    const uint32_t ns = block_push<kBlock>(o.push_next, &counts[qout]);
+    const uint32_t ns = block_push<B>(o.push_next, &counts[qout]);
     if (o.push_next) {
         write_next(qo, ns, o);
         qo.sid[ns] = sid;
@@ -300,9 +301,22 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
             (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts + qin, nsamp, a.sort_hist, a.sort_perm, s)) !=
                 hipSuccess)
             return e;
-        hipLaunchKernelGGL(k_shade, dim3(nb), dim3(kBlock), 0, s, depth, dim0, a.scene, a.halton,
-                           a.q[qin], a.q[qout], a.sh, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, qin, qout,
-                           a.sort_perm);
+        {
+            // each sample's shading is its own: the workgroup size changes only the speed
+            LaunchSink ls(s);
+            const auto grid = [nsamp](int b) { return dim3((unsigned)((nsamp + b - 1) / b)); };
+            const int sb = a.launch.shade_block;
+            if (sb == 64)
+                ls.launch("k_shade", &k_shade<64>, grid(64), dim3(64), depth, dim0, a.scene, a.halton, a.q[qin],
+                          a.q[qout], a.sh, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, qin, qout, a.sort_perm);
+            else if (sb == 128)
+                ls.launch("k_shade", &k_shade<128>, grid(128), dim3(128), depth, dim0, a.scene, a.halton, a.q[qin],
+                          a.q[qout], a.sh, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, qin, qout, a.sort_perm);
+            else
+                ls.launch("k_shade", &k_shade<256>, grid(256), dim3(256), depth, dim0, a.scene, a.halton, a.q[qin],
+                          a.q[qout], a.sh, a.hit_prim, a.hit_t, hu, hv, a.res, a.counts, qin, qout, a.sort_perm);
+            if (ls.err != hipSuccess) return ls.err;
+        }
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2,
                                       nsamp, a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch,
                                       s)) != hipSuccess)

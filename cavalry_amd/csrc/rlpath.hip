@@ -297,8 +297,9 @@ __device__ inline v3 cell_dir(float rx, float ry, int cell) {
 
 // Bins of the Q-weighted distribution: qv = |cos| * luminance(f) * Q for
 // the bin directions at offset (px, py). Each thread keeps the cos / sin of
-// its 16 column angles in LDS (`cs`, stride kBlock) so the bin loops need no
-// register arrays.
+// its 16 column angles in LDS (`cs`, stride B: the workgroup size) so the
+// bin loops need no register arrays.
+template <int B>
 struct BinEval {
     Bsdf bsdf;
     v3 t0, t1, nn, ol;
@@ -315,7 +316,7 @@ struct BinEval {
     }
     // bin p of a row: |cos| * luminance(f) * q, q = the bin's Q value
     __device__ inline float bin(float u, float xy, int p, float q) const {
-        const v3 d = mk3(xy * cs[p * kBlock], xy * cs[(kNX + p) * kBlock], u);
+        const v3 d = mk3(xy * cs[p * B], xy * cs[(kNX + p) * B], u);
         if (matte) {  // the same bits as the general case: the local z is dot3(nn, d)
             const float z = dot3(nn, d);
             return (fabsf(z) * (z * ol.z > 0 ? lum_m : 0.f)) * q;
@@ -411,8 +412,8 @@ __device__ inline void rec_push(const RlRecords& r, uint64_t key, float v, bool 
 
 // The end of computeLighting (after the visibility test), then
 // computeQDistributions (:517-606) and accumulateLightingContribs (:609-647).
-template <class Src>
-__global__ __launch_bounds__(kBlock) void k_qdist(Src src, SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
+template <class Src, int B>
+__global__ __launch_bounds__(B) void k_qdist(Src src, SceneDev sc, HaltonDev h, Queue qi, RlQueue rq,
                                                   RlSlots sl, const uint8_t* __restrict__ vis,
                                                   const int32_t* __restrict__ hit_prim,
                                                   const float* __restrict__ hit_t,
@@ -420,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_qdist(Src src, SceneDev sc, HaltonDe
                                                   const float* __restrict__ hit_v,
                                                   const uint32_t* counts, int qin,
                                                   const uint32_t* __restrict__ perm, RlRecords rec) {
-    __shared__ float trig[2 * kNX][kBlock];
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    __shared__ float trig[2 * kNX][B];
+    const int64_t j = (int64_t)blockIdx.x * B + threadIdx.x;
     bool live = j < (int64_t)counts[qin];
     const int64_t i = live ? task_slot(perm, j) : 0;
     live = live && src.take(i);
@@ -491,11 +492,11 @@ __global__ __launch_bounds__(kBlock) void k_qdist(Src src, SceneDev sc, HaltonDe
             const float phi = (float)(2 * kPi * (double)v);
             float sp, cp;
             sincos_f(phi, sp, cp);
-            cs[p * kBlock] = cp;
-            cs[(kNX + p) * kBlock] = sp;
+            cs[p * B] = cp;
+            cs[(kNX + p) * B] = sp;
         }
         const bool matte = bsdf.type == CAV_MAT_MATTE;
-        const BinEval be{bsdf, t0, t1, nn, ol, Qe, cs, px, matte,
+        const BinEval<B> be{bsdf, t0, t1, nn, ol, Qe, cs, px, matte,
                          matte ? luminance(divs3(bsdf.kd, (float)kPi)) : 0.f};
         float rows[kNY];
         float sum = 0.f, lc0 = 0.f, lc1 = 0.f;
@@ -753,6 +754,17 @@ __device__ inline void finish_light_entry(const RlTables& t, int64_t e, int L) {
         cdf[k] = dens;
     }
     for (int k = 0; k < L; ++k) cdf[k] = cdf[k] / dens;
+}
+
+// k_qdist over `count` task slots in workgroups of `block` threads (64, 128
+// or 256): each task's distribution and updates are its own, so the
+// workgroup size changes only the speed.
+template <class Src, typename... A>
+void launch_qdist(const char* name, int block, int64_t count, LaunchSink& ls, A... args) {
+    const auto grid = [count](int b) { return dim3((unsigned)((count + b - 1) / b)); };
+    if (block == 64) ls.launch(name, &k_qdist<Src, 64>, grid(64), dim3(64), args...);
+    else if (block == 128) ls.launch(name, &k_qdist<Src, 128>, grid(128), dim3(128), args...);
+    else ls.launch(name, &k_qdist<Src, 256>, grid(256), dim3(256), args...);
 }
 
 __global__ __launch_bounds__(kBlock) void k_finish_light(RlTables t, int64_t nent, int L) {
@@ -1037,8 +1049,8 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
             return e;
         const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count, rl.rec_idx[0]};
         if (rec.keys) ls.launch("k_rec_reset", &k_rec_reset, dim3(1), dim3(64), rl.rec_count);
-        ls.launch("k_qdist", &k_qdist<RenderSrc>, dim3(nb), dim3(kBlock), src, a.scene, a.halton, a.q[qin], rl.q[qin],
-                  rl.slots, a.vis, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, perm, rec);
+        launch_qdist<RenderSrc>("k_qdist", a.launch.shade_block, nsamp, ls, src, a.scene, a.halton, a.q[qin],
+                                rl.q[qin], rl.slots, a.vis, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, perm, rec);
         if (rec.keys &&
             (e = rec_sort_sum(rl.rec_keys, rl.rec_idx, rl.rec_val, rl.rec_count, rl.rec_hist, rl.rec_counts,
                               rl.rec_offsets, rl.rec_cap, RenderTables{t}, ls)) != hipSuccess)
@@ -1092,8 +1104,9 @@ void pool_rl_learn(const PoolDev& p, int qin, bool det, LaunchSink& ls) {
         rec = RlRecords{p.rl.rec_keys[0], p.rl.rec_val, p.rl.rec_count, p.rl.rec_idx[0]};
         ls.launch("k_rec_reset", &k_rec_reset, dim3(1), dim3(64), p.rl.rec_count);
     }
-    ls.launch("k_qdist<pool>", &k_qdist<PoolSrc>, dim3(nb), dim3(kBlock), src, p.scene, p.halton, p.q[qin],
-              p.rl.q[qin], p.rl.sl, p.vis, p.hit_prim, p.hit_t, hu, hv, p.counts, qin, (const uint32_t*)nullptr, rec);
+    launch_qdist<PoolSrc>("k_qdist<pool>", p.launch.shade_block, p.cap, ls, src, p.scene, p.halton, p.q[qin],
+                          p.rl.q[qin], p.rl.sl, p.vis, p.hit_prim, p.hit_t, hu, hv, p.counts, qin,
+                          (const uint32_t*)nullptr, rec);
     if (det)
         (void)rec_sort_sum(p.rl.rec_keys, p.rl.rec_idx, p.rl.rec_val, p.rl.rec_count, p.rl.rec_hist, p.rl.rec_counts,
                            p.rl.rec_offsets, p.rl.rec_cap, PoolTables{p.rl.req}, ls);

@@ -81,9 +81,10 @@ __global__ __launch_bounds__(kBlock) void k_pool_raygen(PoolDev p, const Chunk* 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
+template <int B>
+__global__ __launch_bounds__(B) void k_pool_shade(PoolDev p, int qin) {
     const int qout = qin ^ 1;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
     bool live = i < (int64_t)p.counts[qin];
     // rlpath tasks are shaded by the rlpath kernels
     live = live && p.reqs[p.rid[qin][i]].integrator != CAV_INTEGRATOR_RLPATH;
@@ -120,13 +121,13 @@ __global__ __launch_bounds__(kBlock) void k_pool_shade(PoolDev p, int qin) {
             if (depth + 1 >= r.max_depth) o.push_next = false;
         }
     }
-    const uint32_t ss = block_push<kBlock>(o.push_sh, &p.counts[2]);
+    const uint32_t ss = block_push<B>(o.push_sh, &p.counts[2]);
     if (o.push_sh) {
         write_shadow(p.sh, ss, o);
         p.sh.sid[ss] = sid;
         p.sh_rid[ss] = rid;
     }
-    const uint32_t ns = block_push<kBlock>(o.push_next, &p.counts[qout]);
+    const uint32_t ns = block_push<B>(o.push_next, &p.counts[qout]);
     if (o.push_next) {
         write_next(p.q[qout], ns, o);
         p.q[qout].sid[ns] = sid;
@@ -172,7 +173,10 @@ void pool_bounce(const PoolDev& p, int qin, int kind, LaunchSink& ls) {
     trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray, p.counts + qin, p.cap, p.hit_prim,
                         p.hit_t, p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
                         p.trace_counters, p.launch, ls);
-    ls.launch("k_pool_shade", &k_pool_shade, dim3(nb), dim3(kBlock), p, qin);
+    const auto grid = [&p](int b) { return dim3((unsigned)((p.cap + b - 1) / b)); };
+    if (p.launch.shade_block == 64) ls.launch("k_pool_shade", &k_pool_shade<64>, grid(64), dim3(64), p, qin);
+    else if (p.launch.shade_block == 128) ls.launch("k_pool_shade", &k_pool_shade<128>, grid(128), dim3(128), p, qin);
+    else ls.launch("k_pool_shade", &k_pool_shade<256>, grid(256), dim3(256), p, qin);
     if (kind != kBouncePlain) pool_rl_lighting(p, qin, ls);
     trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit, p.sh.src, p.sh.dst,
                          p.vis, p.trace_counters, p.launch, ls);

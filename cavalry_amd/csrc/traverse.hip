@@ -18,7 +18,6 @@
 namespace cav {
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kStack = 64;  // BVH/BVH.h:41,122
 
 // Traversal stack of kStack entries per ray: the first D in LDS (one column
@@ -27,8 +26,8 @@ constexpr int kStack = 64;  // BVH/BVH.h:41,122
 // depth sp is a separate local so that it stays in a register: a depth held
 // beside the dynamically indexed spill array would live in scratch memory
 // with it, a scratch load and store on every push and pop.
-template <int D>
-__device__ __forceinline__ void stack_push(int (*lds)[kBlock], volatile int* spill, int& sp, int v) {
+template <int D, int B>
+__device__ __forceinline__ void stack_push(int (*lds)[B], volatile int* spill, int& sp, int v) {
     if (sp < D) lds[sp][threadIdx.x] = v;
     else spill[sp - D] = v;
     ++sp;
@@ -38,8 +37,8 @@ __device__ __forceinline__ void stack_push(int (*lds)[kBlock], volatile int* spi
 // pointer)
 typedef __attribute__((address_space(3))) int lds_int;
 typedef __attribute__((address_space(3))) uint32_t lds_uint;
-template <int D>
-__device__ __forceinline__ int stack_pop(int (*lds)[kBlock], const volatile int* spill, int& sp) {
+template <int D, int B>
+__device__ __forceinline__ int stack_pop(int (*lds)[B], const volatile int* spill, int& sp) {
     --sp;
     if (sp < D) return *(const lds_int*)(&lds[sp][threadIdx.x]);
     return spill[sp - D];
@@ -88,8 +87,8 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
 // with many coincident Morton codes). Rays are handed out per wave from
 // chunks taken with one atomic. Each ray's node order is the reference's, so
 // results do not depend on which lane or wave traced it.
-template <int kLdsDepth>
-__global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
+template <int kLdsDepth, int kB>
+__global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
                                                   const float* __restrict__ rays, int64_t nrays_arg,
                                                   const uint32_t* __restrict__ nrays_dev,
@@ -100,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                                                   unsigned long long* __restrict__ next_ray,
                                                   unsigned long long* __restrict__ fetched,
                                                   int max_steps, int chunk) {
-    __shared__ int lstack[kLdsDepth][kBlock];
+    __shared__ int lstack[kLdsDepth][kB];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     const int64_t nrays = nrays_dev ? (int64_t)*nrays_dev : nrays_arg;
@@ -189,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                     const bool lfar = dl > dr;
                     const int far = lfar ? refs.x : refs.y;
                     const int near = lfar ? refs.y : refs.x;
-                    if (sp < kStack) stack_push<kLdsDepth>(lstack, spill, sp, far); else ++ovf;
+                    if (sp < kStack) stack_push<kLdsDepth, kB>(lstack, spill, sp, far); else ++ovf;
                     maxsp = max(maxsp, sp);
                     if (sp < kStack) { cur = near; next = true; } else ++ovf;
                 } else if (hl || hr) {
@@ -197,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                 }
                 if (!next) {
                     if (sp == 0) done = true;
-                    else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
+                    else cur = stack_pop<kLdsDepth, kB>(lstack, spill, sp);
                 }
             } else {  // BVH/BVH.h:58-70
                 const TTri tr{q0, q1, q2};
@@ -206,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TRec* __restrict__ recs,
                     hit = true; best = t; bu = u; bv = v; bp = __float_as_int(tr.a.w);
                 }
                 if (sp == 0) done = true;
-                else cur = stack_pop<kLdsDepth>(lstack, spill, sp);
+                else cur = stack_pop<kLdsDepth, kB>(lstack, spill, sp);
             }
         }
         if (done) {
@@ -301,8 +300,8 @@ __device__ __forceinline__ void pair_step(const float4 (&r)[8], uint32_t item, u
 // long rays still in flight at the end are spread over whole waves instead
 // of keeping one lane each busy while the rest of the device idles. ids
 // (optional) maps the dispensed index to the ray index.
-template <int kLdsDepth>
-__global__ __launch_bounds__(kBlock) void k_trace_any_pairs(const TRec* __restrict__ recs,
+template <int kLdsDepth, int kB>
+__global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__ recs,
                                                             const float* __restrict__ rays, int64_t nrays_arg,
                                                             const uint32_t* __restrict__ nrays_dev,
                                                             const uint32_t* __restrict__ ids,
@@ -316,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_any_pairs(const TRec* __restri
                                                             AnySpill* __restrict__ spill, uint32_t* spill_n,
                                                             uint32_t spill_cap, int spill_after,
                                                             int max_steps, int chunk) {
-    __shared__ uint32_t litem[kLdsDepth][kBlock];
-    __shared__ uint8_t lsb[kLdsDepth][kBlock];
+    __shared__ uint32_t litem[kLdsDepth][kB];
+    __shared__ uint8_t lsb[kLdsDepth][kB];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     const int64_t nrays = nrays_dev ? (int64_t)*nrays_dev : nrays_arg;
@@ -583,8 +582,9 @@ __global__ void k_clear_words(uint32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-// Workgroups of a persistent launch: max_per_cu per CU (0: as many as fit).
-int persistent_blocks(const void* kernel, int max_per_cu) {
+// Workgroups of a persistent launch of `threads`-thread workgroups:
+// max_per_cu per CU (0: as many as fit).
+int persistent_blocks(const void* kernel, int threads, int max_per_cu) {
     // (initialised once, thread-safely: contexts on several threads launch here)
     static const int cus = [] {
         int dev = 0, n = 0;
@@ -593,37 +593,56 @@ int persistent_blocks(const void* kernel, int max_per_cu) {
         return n;
     }();
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) per = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0)
+        per = 4 * 256 / threads;
     if (max_per_cu > 0 && per > max_per_cu) per = max_per_cu;
     return cus * per;
 }
 
-// The kernel of the launch's stack depth on a persistent grid of its
-// workgroups per CU, for `work` rays at most.
-template <typename... A>
-void launch_trace(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
-    const bool deep = L.lds_depth >= 48;
-    const void* k = deep ? reinterpret_cast<const void*>(&k_trace<48>) : reinterpret_cast<const void*>(&k_trace<32>);
-    int64_t blocks = persistent_blocks(k, L.per_cu);
-    const int64_t need = (work + kBlock - 1) / kBlock;
+// One persistent kernel instance (stack depth D, B-thread workgroups) on
+// its grid, for `work` rays at most.
+template <int D, int B, typename... P, typename... A>
+void launch_persistent(const char* name, void (*kernel)(P...), const TraceLaunch& L, int64_t work, LaunchSink& ls,
+                       A... args) {
+    int64_t blocks = persistent_blocks(reinterpret_cast<const void*>(kernel), B, L.per_cu);
+    const int64_t need = (work + B - 1) / B;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    if (deep) ls.launch("k_trace<closest>", &k_trace<48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
-    else ls.launch("k_trace<closest>", &k_trace<32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    ls.launch(name, kernel, dim3((unsigned)blocks), dim3(B), args..., L.chunk);
 }
 
-// The any-hit pair kernel on the same persistent grid.
+// The closest-hit kernel of the launch's stack depth and workgroup size.
+template <typename... A>
+void launch_trace(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
+    const char* nm = "k_trace<closest>";
+    const bool deep = L.lds_depth >= 48;
+    if (L.block == 64) {
+        if (deep) launch_persistent<48, 64>(nm, &k_trace<48, 64>, L, work, ls, args...);
+        else launch_persistent<32, 64>(nm, &k_trace<32, 64>, L, work, ls, args...);
+    } else if (L.block == 128) {
+        if (deep) launch_persistent<48, 128>(nm, &k_trace<48, 128>, L, work, ls, args...);
+        else launch_persistent<32, 128>(nm, &k_trace<32, 128>, L, work, ls, args...);
+    } else {
+        if (deep) launch_persistent<48, 256>(nm, &k_trace<48, 256>, L, work, ls, args...);
+        else launch_persistent<32, 256>(nm, &k_trace<32, 256>, L, work, ls, args...);
+    }
+}
+
+// The any-hit pair kernel, likewise.
 template <typename... A>
 void launch_any_pairs(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
+    const char* nm = "k_trace_any_pairs";
     const bool deep = L.lds_depth >= 48;
-    const void* k = deep ? reinterpret_cast<const void*>(&k_trace_any_pairs<48>)
-                         : reinterpret_cast<const void*>(&k_trace_any_pairs<32>);
-    int64_t blocks = persistent_blocks(k, L.per_cu);
-    const int64_t need = (work + kBlock - 1) / kBlock;
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
-    if (deep) ls.launch("k_trace_any_pairs", &k_trace_any_pairs<48>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
-    else ls.launch("k_trace_any_pairs", &k_trace_any_pairs<32>, dim3((unsigned)blocks), dim3(kBlock), args..., L.chunk);
+    if (L.block == 64) {
+        if (deep) launch_persistent<48, 64>(nm, &k_trace_any_pairs<48, 64>, L, work, ls, args...);
+        else launch_persistent<32, 64>(nm, &k_trace_any_pairs<32, 64>, L, work, ls, args...);
+    } else if (L.block == 128) {
+        if (deep) launch_persistent<48, 128>(nm, &k_trace_any_pairs<48, 128>, L, work, ls, args...);
+        else launch_persistent<32, 128>(nm, &k_trace_any_pairs<32, 128>, L, work, ls, args...);
+    } else {
+        if (deep) launch_persistent<48, 256>(nm, &k_trace_any_pairs<48, 256>, L, work, ls, args...);
+        else launch_persistent<32, 256>(nm, &k_trace_any_pairs<32, 256>, L, work, ls, args...);
+    }
 }
 
 // Steps (sibling pairs) a ray must have taken, once the rays have all been

@@ -388,17 +388,23 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
 
 /* Launch shape of the persistent traversal kernels, per context: workgroups
  * per CU (0: as many as fit), rays a wave takes from the dispenser at a time
- * (64..65536) and traversal-stack entries kept in LDS (32 or 48; the rest of
- * the reference's 64 in scratch). Each ray's node order is the reference's
- * either way, so results do not depend on it. STANDALONE applies to
- * cav_trace_*, WAVEFRONT to cav_render and to servers opened on the context
- * afterwards. No counterpart in the reference (its launch shapes are
- * compile-time); cavalry_amd/tune.py times the candidates and keeps the
- * fastest in a JSON cache. */
+ * (64..65536), traversal-stack entries kept in LDS (32 or 48; the rest of
+ * the reference's 64 in scratch) and threads per traversal workgroup (64,
+ * 128 or 256; 0: 256); for WAVEFRONT also the threads per workgroup of the
+ * per-sample shading kernels (path shading, the RL integrator's Q-table
+ * sampling; 64, 128 or 256; 0: 256). Each ray's node order is the
+ * reference's and each sample's shading is its own either way, so results do
+ * not depend on it. STANDALONE applies to cav_trace_*, WAVEFRONT to
+ * cav_render and to servers opened on the context afterwards. No
+ * counterpart in the reference (its launch shapes are compile-time, 256
+ * threads: Utils/GpuCommons.h:63); cavalry_amd/tune.py times the candidates
+ * and keeps the fastest in a JSON cache. */
 typedef struct {
     int32_t workgroups_per_cu;
     int32_t ray_chunk;
     int32_t lds_stack;
+    int32_t workgroup_size;
+    int32_t shade_workgroup_size;
 } cav_trace_launch;
 enum { CAV_TRACE_STANDALONE = 0, CAV_TRACE_WAVEFRONT = 1 };
 int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* launch);

@@ -27,11 +27,11 @@ def test_every_shape_gives_the_same_hits(ctx, traced):
     print("[tune] shapes", flush=True)
     saved = tune.get_launch(ctx, tune.STANDALONE)
     ref = None
-    for per_cu in (1, 3, 0):
+    for size, per_cu in ((256, 1), (256, 3), (256, 0), (128, 5), (64, 12), (64, 0)):
         for chunk in (64, 1024):
             for lds in (32, 48):
                 tune.set_launch(ctx, tune.STANDALONE, {"workgroups_per_cu": per_cu, "ray_chunk": chunk,
-                                                       "lds_stack": lds})
+                                                       "lds_stack": lds, "workgroup_size": size})
                 h = bvh.intersect(rays)
                 v = bvh.test_visibility(srays, lim)
                 got = (h["prim"].copy(), h["t"].copy(), v.copy())
@@ -49,6 +49,42 @@ def test_launch_arguments_checked(ctx):
         tune.set_launch(ctx, tune.WAVEFRONT, {"workgroups_per_cu": 2, "ray_chunk": 8, "lds_stack": 32})
     with pytest.raises(CavError, match="which"):
         tune.set_launch(ctx, 7, {"workgroups_per_cu": 2, "ray_chunk": 256, "lds_stack": 32})
+    with pytest.raises(CavError, match="workgroup_size"):
+        tune.set_launch(ctx, tune.STANDALONE, {"workgroups_per_cu": 2, "ray_chunk": 256, "lds_stack": 32,
+                                               "workgroup_size": 96})
+    with pytest.raises(CavError, match="shade_workgroup_size"):
+        tune.set_launch(ctx, tune.WAVEFRONT, {"workgroups_per_cu": 2, "ray_chunk": 256, "lds_stack": 32,
+                                              "shade_workgroup_size": 512})
+    # 0 stands for 256
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    tune.set_launch(ctx, tune.STANDALONE, {"workgroups_per_cu": 2, "ray_chunk": 256, "lds_stack": 32})
+    got = tune.get_launch(ctx, tune.STANDALONE)
+    assert got["workgroup_size"] == 256 and got["shade_workgroup_size"] == 256
+    tune.set_launch(ctx, tune.STANDALONE, saved)
+
+
+@pytest.mark.parametrize("integrator", ["path", "rlpath"])
+def test_render_same_under_every_workgroup_size(ctx, integrator):
+    """The render loops' traversal and shading workgroup sizes change only
+    the speed: the film is bit-identical (the RL integrator in its
+    deterministic table-update mode)."""
+    print(f"[tune] render sizes {integrator}", flush=True)
+    sc = scenes.cornell_box(seed=3, width=40, height=32)
+    scene = R.Scene(ctx, sc)
+    saved = tune.get_launch(ctx, tune.WAVEFRONT)
+    kw = {"deterministic": True} if integrator == "rlpath" else {}
+    ref = None
+    try:
+        for size, shade, per_cu in ((256, 256, 0), (64, 64, 8), (128, 128, 3), (256, 64, 2), (64, 128, 0)):
+            tune.set_launch(ctx, tune.WAVEFRONT, dict(saved, workgroup_size=size, shade_workgroup_size=shade,
+                                                      workgroups_per_cu=per_cu))
+            img, _ = R.render(ctx, scene, 2, integrator=integrator, **kw)
+            if ref is None:
+                ref = img.rgb.copy()
+            np.testing.assert_array_equal(img.rgb, ref)
+    finally:
+        tune.set_launch(ctx, tune.WAVEFRONT, saved)
+        scene.close()
 
 
 def test_tuner_picks_caches_and_reuses(ctx, traced, tmp_path):
