@@ -156,10 +156,9 @@ __device__ __forceinline__ uint32_t morton_code<uint32_t>(v3 c) {
 
 // BVH/BVH.cu:79-92: code of the triangle-box centroid normalised by the scene
 // box; values[i] = i so the stable sort yields the original index. Also
-// counts the codes' 8-bit digits for every radix-sort pass (equal digits of a
-// wave are found by eight ballots and added once by their lowest lane; one
-// global add per digit and workgroup, over a bounded grid) and clears the
-// sort's look-back words.
+// counts the codes' 8-bit digits for every radix-sort pass (one LDS add per
+// digit and code, one global add per digit and workgroup, over a bounded
+// grid) and clears the sort's look-back words.
 constexpr int kMortonBlocks = 1024;
 constexpr int kMortonUnroll = 4;
 template <typename K>
@@ -178,10 +177,8 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tri
     const v3 shi = mk3(ord2f(ob[3]), ord2f(ob[4]), ord2f(ob[5]));
     const v3 sc = scale3(add3(slo, shi), 0.5f);
     const v3 se = sub3(shi, slo);
-    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
-    // every wave runs the same trip count (the ballots need all lanes); the
-    // triangles of kMortonUnroll trips are loaded before any is used (past
-    // the end: the last triangle again), so their loads overlap
+    // the triangles of kMortonUnroll trips are loaded before any is used
+    // (past the end: the last triangle again), so their loads overlap
     const int64_t trips = (n + stride - 1) / stride;
     const int64_t first = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     for (int64_t it = 0; it < trips; it += kMortonUnroll) {
@@ -207,17 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ tri
                 keys[i] = key;
                 vals[i] = (uint32_t)i;
             }
-            for (int p = 0; p < passes; ++p) {
-                const uint32_t d = (uint32_t)(key >> (8 * p)) & 255u;
-                uint64_t peers = __ballot(valid);
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const bool bit = (d >> b) & 1u;
-                    const uint64_t m = __ballot(bit);
-                    peers &= bit ? m : ~m;
-                }
-                if (valid && (peers & lt) == 0) atomicAdd(&h[p * 256 + d], (uint32_t)__popcll(peers));
-            }
+            if (valid)
+                for (int p = 0; p < passes; ++p) atomicAdd(&h[p * 256 + ((uint32_t)(key >> (8 * p)) & 255u)], 1u);
         }
     }
     __syncthreads();

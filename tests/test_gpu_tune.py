@@ -113,3 +113,21 @@ def test_tuner_picks_caches_and_reuses(ctx, traced, tmp_path):
     tune.set_launch(ctx, tune.STANDALONE, saved)
     tune.set_launch(ctx, tune.WAVEFRONT, wsaved)
     scene.close()
+
+
+def test_tune_rounds_recommends_and_caches(ctx, tmp_path):
+    """tune_rounds times build + traces per candidate round count, returns
+    the fastest without setting anything, and caches the recommendation;
+    more rounds never raise the tree's SAH cost."""
+    print("[tune] tune_rounds", flush=True)
+    tris = scenes.gmm_triangles(1 << 14, seed=8)
+    rays = scenes.random_rays(1 << 13, seed=9)
+    cache = str(tmp_path / "tune.json")
+    out = tune.tune_rounds(ctx, tris, rays, candidates=(0, 1, 2), cache_path=cache)
+    assert not out["cached"] and out["rounds"] in (0, 1, 2)
+    assert set(out["timings"]) == {"0", "1", "2"}
+    t = out["timings"]
+    assert out["ms"] == min(v["total_ms"] for v in t.values())
+    assert t["2"]["sah"] <= t["1"]["sah"] <= t["0"]["sah"]
+    again = tune.tune_rounds(ctx, tris, rays, candidates=(0, 1, 2), cache_path=cache)
+    assert again["cached"] and again["rounds"] == out["rounds"]
