@@ -178,11 +178,13 @@ class BVH:
     def trace_fetches(self, reset: bool = False) -> dict:
         """64-byte records (node visits + triangle tests) fetched by the
         closest-hit and any-hit launches since the last reset, and the rays
-        of the last any-hit launch finished by the cooperative pass."""
-        out = np.zeros(4, np.uint64)
+        of the last any-hit / closest-hit launch finished by the cooperative
+        passes (and of those, traced again on one lane)."""
+        out = np.zeros(6, np.uint64)
         check(self.ctx.handle, lib().cav_trace_fetches(self.ctx.handle, self.handle, _ptr(out), 1 if reset else 0))
         return {"closest": int(out[0]), "occluded": int(out[1]), "spilled_rays": int(out[2]),
-                "respilled_rays": int(out[3])}
+                "respilled_rays": int(out[3]), "long_closest_rays": int(out[4]),
+                "long_closest_retraced": int(out[5])}
 
     def max_depth(self) -> int:
         """Deepest leaf, in edges from the root."""

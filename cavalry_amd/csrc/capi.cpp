@@ -358,9 +358,12 @@ int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* l)
     if (!size_ok(l->workgroup_size)) return fail(ctx, CAV_ERR_INVALID, "workgroup_size must be 0, 64, 128 or 256");
     if (!size_ok(l->shade_workgroup_size))
         return fail(ctx, CAV_ERR_INVALID, "shade_workgroup_size must be 0, 64, 128 or 256");
+    if (l->handoff_steps < 0 || l->handoff_steps > (1 << 30))
+        return fail(ctx, CAV_ERR_INVALID, "handoff_steps must be in 0..2^30");
     ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack,
                                      l->workgroup_size ? l->workgroup_size : 256,
-                                     l->shade_workgroup_size ? l->shade_workgroup_size : 256};
+                                     l->shade_workgroup_size ? l->shade_workgroup_size : 256,
+                                     l->handoff_steps ? l->handoff_steps : 1024};
     return CAV_OK;
 }
 
@@ -369,43 +372,49 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
     if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
         return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
     const TraceLaunch& l = ctx->launch[which];
-    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block};
+    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block, l.handoff};
     return CAV_OK;
 }
 
-// Grows the BVH's any-hit spill pool to hold a batch of nrays (the stream
-// is drained before an old pool is freed).
-static int ensure_spill(cav_ctx* ctx, const cav_bvh* h, int64_t nrays) {
+// Grows the BVH's long-ray pass buffers (any-hit spill records and
+// closest-hit ray lists) to hold a batch of nrays (the stream is drained
+// before old buffers are freed); *coop receives them.
+static int ensure_spill(cav_ctx* ctx, const cav_bvh* h, int64_t nrays, TraceCoop* coop) {
     const uint32_t need = any_spill_capacity(nrays);
-    if (h->spill_cap >= need) return CAV_OK;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if (h->spill) (void)hipFree(h->spill);
-    h->spill = nullptr;
-    h->spill_cap = 0;
-    void* p = nullptr;
-    const int rc = dev_alloc(ctx, &p, (sizeof(AnySpill) + sizeof(uint32_t)) * (size_t)need, "any-hit spill pool");
-    if (rc != CAV_OK) return rc;
-    h->spill = static_cast<AnySpill*>(p);
-    h->spill_cap = need;
+    if (h->spill_cap < need) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (h->spill) (void)hipFree(h->spill);
+        h->spill = nullptr;
+        h->spill_cap = 0;
+        void* p = nullptr;
+        const int rc = dev_alloc(ctx, &p, trace_coop_bytes(need, need), "long-ray pass buffers");
+        if (rc != CAV_OK) return rc;
+        h->spill = p;
+        h->spill_cap = need;
+    }
+    *coop = trace_coop_in(h->spill, h->spill_cap, h->spill_cap);
     return CAV_OK;
 }
 
 int cav_trace_closest_async(cav_ctx* ctx, const cav_bvh* h, const float* rays, int64_t nrays,
                             int32_t* prim, float* t) {
     if (!ctx || !h || (nrays > 0 && (!rays || !prim || !t))) return CAV_ERR_INVALID;
+    TraceCoop coop;
+    const int rc = ensure_spill(ctx, h, nrays, &coop);
+    if (rc != CAV_OK) return rc;
     HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, nullptr, nullptr,
-                              h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
+                              h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, &coop));
     return CAV_OK;
 }
 
 int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* h, const float* rays,
                              const float* limit, int64_t nrays, uint8_t* visible) {
     if (!ctx || !h || (nrays > 0 && (!rays || !limit || !visible))) return CAV_ERR_INVALID;
-    const int rc = ensure_spill(ctx, h, nrays);
+    TraceCoop coop;
+    const int rc = ensure_spill(ctx, h, nrays, &coop);
     if (rc != CAV_OK) return rc;
     HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, nullptr, nullptr, nrays, visible,
-                               h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, h->spill,
-                               h->spill_cap));
+                               h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, &coop));
     return CAV_OK;
 }
 
@@ -427,23 +436,25 @@ int cav_trace_counters(cav_ctx* ctx, const cav_bvh* h, uint32_t* out4, int32_t r
     return CAV_OK;
 }
 
-int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out4, int32_t reset) {
-    if (!ctx || !h || !out4) return CAV_ERR_INVALID;
+int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* h, uint64_t* out6, int32_t reset) {
+    if (!ctx || !h || !out6) return CAV_ERR_INVALID;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    uint32_t w[3] = {0, 0, 0};  // spill count, spill cursor, failures
-    HIPCHK(ctx, hipMemcpyAsync(out4, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(w, h->b.counters + 20, 12, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t w[11] = {};  // words 20..30: the long-ray passes of the last launches
+    HIPCHK(ctx, hipMemcpyAsync(out6, h->b.counters + 16, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(w, h->b.counters + 20, sizeof w, hipMemcpyDeviceToHost, ctx->stream));
     if (reset) HIPCHK(ctx, hipMemsetAsync(h->b.counters + 16, 0, 16, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    out4[2] = w[0] < h->spill_cap ? w[0] : h->spill_cap;
-    out4[3] = w[2];
+    out6[2] = w[0] < h->spill_cap ? w[0] : h->spill_cap;  // any-hit rays spilled
+    out6[3] = w[2];                                         // of those, traced again on one lane
+    out6[4] = w[6] < h->spill_cap ? w[6] : h->spill_cap;  // closest-hit rays handed over
+    out6[5] = w[10];                                        // of those, traced again on one lane
     return CAV_OK;
 }
 
 int cav_bvh_max_depth(cav_ctx* ctx, const cav_bvh* h, int32_t* depth) {
     if (!ctx || !h || !depth) return CAV_ERR_INVALID;
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    // counters[8] is free scratch (the traversal uses words 0..7 and 16..19)
+    // counters[8] is free scratch (the traversal uses words 0..7 and 16..30)
     uint32_t* word = h->b.counters + 8;
     HIPCHK(ctx, levels_max_depth(h->b, h->n, word, ctx->stream));
     uint32_t d = 0;
@@ -460,7 +471,11 @@ int cav_trace_closest_device(cav_ctx* ctx, const cav_bvh* h, const float* rays, 
         return fail(ctx, CAV_ERR_INVALID, "null ray or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters, 0, 4, ctx->stream));
-    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters, ctx->launch[CAV_TRACE_STANDALONE], ctx->stream));
+    TraceCoop coop;
+    const int rc = ensure_spill(ctx, h, nrays, &coop);
+    if (rc != CAV_OK) return rc;
+    HIPCHK(ctx, trace_closest(h->rec, h->n, h->root, rays, nrays, prim, t, u, v, h->b.counters,
+                              ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, &coop));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -476,10 +491,11 @@ int cav_trace_occluded_device(cav_ctx* ctx, const cav_bvh* h, const float* rays,
         return fail(ctx, CAV_ERR_INVALID, "null ray, limit or output buffer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemsetAsync(h->b.counters + 1, 0, 4, ctx->stream));
-    const int rc = ensure_spill(ctx, h, nrays);
+    TraceCoop coop;
+    const int rc = ensure_spill(ctx, h, nrays, &coop);
     if (rc != CAV_OK) return rc;
     HIPCHK(ctx, trace_occluded(h->rec, h->n, rays, limit, src, dst, nrays, visible, h->b.counters,
-                               ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, h->spill, h->spill_cap));
+                               ctx->launch[CAV_TRACE_STANDALONE], ctx->stream, &coop));
     uint32_t ovf = 0;
     HIPCHK(ctx, hipMemcpyAsync(&ovf, h->b.counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

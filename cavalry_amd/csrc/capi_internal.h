@@ -28,8 +28,9 @@ struct cav_bvh {
     cav::TRec* rec = nullptr;   // 2n traversal records (sibling-pair layout)
     float* root = nullptr;      // 8 floats
     int32_t* mesh = nullptr;    // n, original order (may be null)
-    // any-hit spill pool (records + failure ids), grown to the largest batch traced
-    mutable cav::AnySpill* spill = nullptr;
+    // long-ray pass buffers (cav::trace_coop_bytes(spill_cap, spill_cap)),
+    // grown to the largest batch traced
+    mutable void* spill = nullptr;
     mutable uint32_t spill_cap = 0;
     cav_bvh_stats stats{};
     cav::Marker mk;

@@ -97,7 +97,9 @@ hipError_t treelet_round_impl(LbvhBuffers& b, int n, bool first, hipStream_t s, 
 // counters[16] (closest) and counters[18] (occluded) add up the 64-byte
 // records every launch fetched (never reset by a launch); counters[20..25]
 // are the cooperative any-hit pass's spill count, spill cursor, failure
-// count and ray dispenser (cleared by each launch).
+// count and ray dispenser, counters[26..30] the cooperative closest-hit
+// pass's long-ray count, cursor, phase-3 ray dispenser (64-bit, 28-29) and
+// phase-3 count (all cleared by each launch).
 // Launch shape of the persistent traversal kernels (and, for the render
 // loops, of the per-sample shading kernels). Each ray's node order is fixed
 // and each sample's shading is independent of where it runs, so results do
@@ -108,11 +110,9 @@ struct TraceLaunch {
     int lds_depth = 32;     // stack entries in LDS (32 or 48); deeper ones in scratch
     int block = 256;        // threads per traversal workgroup (64, 128 or 256)
     int shade_block = 256;  // threads per shading workgroup (64, 128 or 256)
+    int handoff = 1024;     // steps before a ray still in flight at the end goes to a cooperative pass
 };
 
-hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
-                         const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s);
 // Any-hit rays that outgrow a step budget hand their pending sibling pairs
 // to a cooperative pass (one wave per ray): one record each.
 constexpr int kSpillItems = 65;  // the current pair + a full stack
@@ -127,32 +127,62 @@ struct AnySpill {
 // every ray has been handed out, so about one per lane of the persistent grid
 // (at most 8 workgroups x 256 lanes on 256 CUs) plus the few rays a wave
 // still holds in its chunk; a full pool only means more rays finish on
-// their lane.
+// their lane. The closest-hit pass hands over ray ids only, as many.
 inline uint32_t any_spill_capacity(int64_t nrays) {
     const int64_t cap = int64_t(1) << 20;
     return (uint32_t)(nrays < cap ? nrays : cap);
 }
 
-// spill: nullptr (every ray traced by one lane) or spill_cap records
+// Device buffers of the cooperative passes that finish the long rays still
+// in flight once every ray has been handed out (one wave per ray instead of
+// one lane); a null buffer leaves every ray on its lane.
+struct TraceCoop {
+    AnySpill* spill = nullptr;  // any-hit: spill_cap records, then spill_cap ray ids
+    uint32_t spill_cap = 0;
+    uint32_t* longs = nullptr;  // closest-hit: long_cap ray ids, then long_cap ray ids
+    uint32_t long_cap = 0;
+};
+inline size_t trace_coop_bytes(uint32_t spill_cap, uint32_t long_cap) {
+    return (sizeof(AnySpill) + sizeof(uint32_t)) * (size_t)spill_cap + 2 * sizeof(uint32_t) * (size_t)long_cap;
+}
+// the buffers of one allocation of trace_coop_bytes(spill_cap, long_cap) bytes
+inline TraceCoop trace_coop_in(void* base, uint32_t spill_cap, uint32_t long_cap) {
+    TraceCoop c;
+    c.spill = spill_cap ? static_cast<AnySpill*>(base) : nullptr;
+    c.spill_cap = spill_cap;
+    c.longs = long_cap ? reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(base) +
+                                                     (sizeof(AnySpill) + sizeof(uint32_t)) * (size_t)spill_cap)
+                       : nullptr;
+    c.long_cap = long_cap;
+    return c;
+}
+
+// coop: null (every ray traced by one lane) or the cooperative passes' buffers
+hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
+                         const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
+                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s,
+                         const TraceCoop* coop = nullptr);
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
-                          hipStream_t s, AnySpill* spill = nullptr, uint32_t spill_cap = 0);
+                          hipStream_t s, const TraceCoop* coop = nullptr);
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
-                               const TraceLaunch& L, hipStream_t s);
+                               const TraceLaunch& L, hipStream_t s, const TraceCoop* coop = nullptr);
 hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
-                                uint32_t* counters, const TraceLaunch& L, hipStream_t s);
+                                uint32_t* counters, const TraceLaunch& L, hipStream_t s,
+                                const TraceCoop* coop = nullptr);
 // the same, launched through a sink (a stream or a graph under construction)
 void trace_closest_queue(const TRec* rec, int ntris, const float* root, const float* rays,
                          const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
-                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls);
+                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls, const TraceCoop* coop = nullptr);
 void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
                           int64_t capacity, const float* limit, const int32_t* src, const int32_t* dst,
-                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls);
+                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls,
+                          const TraceCoop* coop = nullptr);
 
 }  // namespace cav

@@ -352,7 +352,7 @@ template <typename K>
 hipError_t build_impl(LbvhBuffers& b, const float* tris, int n, int keybits, hipStream_t s,
                       Marker* mk) {
     const int passes = (keybits + 7) / 8;
-    const int64_t tiles = sort_tiles(n);
+    const int64_t tiles = onesweep_tiles(n);
     OnesweepBuffers<K> ob;
     ob.keys[0] = reinterpret_cast<K*>(b.keys[0]);
     ob.keys[1] = reinterpret_cast<K*>(b.keys[1]);

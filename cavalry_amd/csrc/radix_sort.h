@@ -24,7 +24,7 @@ struct OnesweepBuffers {
     K* keys[2] = {nullptr, nullptr};         // keys[0] holds the input
     uint32_t* vals[2] = {nullptr, nullptr};  // vals[0] holds the input
     const uint32_t* hist = nullptr;          // passes * 256 digit counts of the keys
-    uint32_t* status = nullptr;              // passes * 256 * tiles look-back words, zero before the sort
+    uint32_t* status = nullptr;              // passes * 256 * onesweep_tiles look-back words, zero before the sort
     uint32_t* tile_ctr = nullptr;            // passes tile counters, zero before the sort
     uint32_t* err = nullptr;                 // nonzero: a look-back wait exceeded its bound
     int result = 0;                          // index of the buffer holding the output
@@ -32,6 +32,7 @@ struct OnesweepBuffers {
 
 struct Marker;
 int64_t sort_tiles(int64_t n);
+int64_t onesweep_tiles(int64_t n);  // look-back tiles of the onesweep sort (at most sort_tiles)
 int sort_radix();
 hipError_t radix_sort_pairs(SortBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
                             Marker* mk);

@@ -237,61 +237,104 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const K* __restrict__ kin,
 // Per tile: rank the keys by digit (the ballot multisplit above), publish the
 // tile's per-digit counts, add up predecessors' counts back to the first
 // published inclusive prefix, publish this tile's inclusive prefix, and
-// scatter the tile through LDS so each digit run is written contiguously.
+// scatter the tile through LDS so each digit run is written contiguously
+// (the keys, then the values through the same LDS buffer).
 // Look-back words: flag << 30 | count (flag 1: this tile's count, 2: count
 // of this and all earlier tiles), read and written as agent-scope atomics.
 // The global start of every digit comes from the pass's digit histogram,
 // which the key producer accumulated beforehand.
+// The shape (512 threads x 16 keys, 4 predecessors per look-back step) was
+// the fastest of a sweep over 256..1024 threads, 16..32 keys per thread and
+// windows of 1..32 predecessors on 2^24 Morton codes: larger windows load
+// the look-back words faster than they are published.
+constexpr int kOsBlock = 512;
+constexpr int kOsWaves = kOsBlock / 64;
+constexpr int kOsItems = 16;
+constexpr int kOsTile = kOsBlock * kOsItems;
 constexpr uint32_t kAgg = 1u << 30, kIncl = 2u << 30, kCountMask = (1u << 30) - 1u;
 constexpr int kSpinLimit = 1 << 24;  // bounded wait; exceeding it flags *err (never expected)
-constexpr int kLookback = 8;          // predecessors read per look-back step
+constexpr int kLookback = 4;          // predecessors read per look-back step
+
+// Exclusive scan over the workgroup of one value per thread (threads past
+// the first 256 pass 0); returns the total too.
+__device__ __forceinline__ uint32_t os_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kOsWaves; ++w) {
+        const uint32_t s = wsum[w];
+        if (w < wave) off += s;
+        total += s;
+    }
+    __syncthreads();
+    return off + inc - v;
+}
 
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin,
-                                                     const uint32_t* __restrict__ vin,
-                                                     K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                     int64_t n, int shift,
-                                                     const uint32_t* __restrict__ phist,
-                                                     uint32_t* status, uint32_t* tile_ctr,
-                                                     uint32_t* err) {
-    __shared__ uint32_t wh[kWaves][kRadix];
+__global__ __launch_bounds__(kOsBlock) void k_onesweep(const K* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       int64_t n, int shift,
+                                                       const uint32_t* __restrict__ phist,
+                                                       uint32_t* status, uint32_t* tile_ctr,
+                                                       uint32_t* err) {
+    __shared__ uint32_t wh[kOsWaves][kRadix];
     __shared__ uint32_t tstart[kRadix];
     __shared__ uint32_t gbase[kRadix];
-    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t wsum[kOsWaves];
     __shared__ uint32_t stile;
-    __shared__ K skey[kTile];
-    __shared__ uint32_t sval[kTile];
+    __shared__ K stage[kOsTile];  // the tile's keys in digit order, then its values
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x == 0) stile = atomicAdd(tile_ctr, 1u);
-    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wh[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kOsWaves * kRadix; i += kOsBlock) (&wh[0][0])[i] = 0;
     __syncthreads();
     const int tile = (int)stile;
-    const int64_t tbase = (int64_t)tile * kTile;
-    const int tile_n = (int)((n - tbase) < kTile ? (n - tbase) : kTile);
+    const int64_t tbase = (int64_t)tile * kOsTile;
+    const int tile_n = (int)((n - tbase) < kOsTile ? (n - tbase) : kOsTile);
 
-    K key[kItems];
-    uint32_t val[kItems];
-    uint32_t rank[kItems];
-    const int wbase = wave * 64 * kItems;
+    // wave w owns elements [w*64*kOsItems, (w+1)*64*kOsItems) of the tile;
+    // item j of lane l is element w*64*kOsItems + j*64 + l, so (w, j, l) is
+    // element order. Loads go through one uniform base per wave.
+    K key[kOsItems];
+    uint32_t val[kOsItems];
+    uint32_t rank[kOsItems];
+    const int wbase = wave * 64 * kOsItems;
+    const K* kb = kin + tbase + wbase;
+    const uint32_t* vb = vin + tbase + wbase;
+    if (tile_n == kOsTile) {
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        int e = wbase + j * 64 + lane;
-        if (e < tile_n) { key[j] = kin[tbase + e]; val[j] = vin[tbase + e]; }
+        for (int j = 0; j < kOsItems; ++j) {
+            key[j] = kb[j * 64 + lane];
+            val[j] = vb[j * 64 + lane];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kOsItems; ++j) {
+            const bool in = wbase + j * 64 + lane < tile_n;
+            key[j] = in ? kb[j * 64 + lane] : (K)0;
+            val[j] = in ? vb[j * 64 + lane] : 0u;
+        }
     }
 
+    // lanes with equal digits: eight ballots; a lane keeps the lanes that
+    // agree with it on every bit (m XNOR its own bit, spread over 64 lanes)
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        const int e = wbase + j * 64 + lane;
-        const bool valid = e < tile_n;
+    for (int j = 0; j < kOsItems; ++j) {
+        const bool valid = wbase + j * 64 + lane < tile_n;
         const uint32_t d = valid ? digit_of(key[j], shift) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            const int32_t sb = -(int32_t)((d >> b) & 1u);
+            const uint64_t m = __ballot(sb != 0);
+            const uint64_t sm = (uint64_t)(uint32_t)sb | ((uint64_t)(uint32_t)sb << 32);
+            peers &= ~(m ^ sm);
         }
         uint32_t r = 0;
         if (valid) {
@@ -305,15 +348,16 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin,
     }
     __syncthreads();
 
-    {
+    if (threadIdx.x < kRadix) {
         const int d = threadIdx.x;
         uint32_t run = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            uint32_t c = wh[w][d];
+        for (int w = 0; w < kOsWaves; ++w) {
+            const uint32_t c = wh[w][d];
             wh[w][d] = run;
             run += c;
         }
+        tstart[d] = run;  // the tile's count of digit d, until the scan below
         // look-back for digit d: publish, then sum predecessors' counts
         uint32_t* my = status + (int64_t)tile * kRadix + d;
         uint32_t excl = 0;
@@ -351,42 +395,63 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin,
             }
             __hip_atomic_store(my, kIncl | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // start of digit d in the output: digits below d, then earlier tiles
-        uint32_t tot;
-        const uint32_t below = block_excl_scan256(phist[d], wsum, tot);
-        gbase[d] = below + excl;
-        tstart[d] = block_excl_scan256(run, wsum, tot);
+        gbase[d] = excl;
     }
     __syncthreads();
-
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        const int e = wbase + j * 64 + lane;
-        if (e < tile_n) {
-            const uint32_t d = digit_of(key[j], shift);
-            const uint32_t pos = tstart[d] + wh[wave][d] + rank[j];
-            skey[pos] = key[j];
-            sval[pos] = val[j];
+    {
+        // start of digit d in the output: digits below d, then earlier tiles;
+        // in the tile: digits below d
+        const bool dt = threadIdx.x < kRadix;
+        uint32_t tot;
+        const uint32_t below = os_excl_scan(dt ? phist[threadIdx.x] : 0u, wsum, tot);
+        const uint32_t ts = os_excl_scan(dt ? tstart[threadIdx.x] : 0u, wsum, tot);
+        if (dt) {
+            gbase[threadIdx.x] += below;
+            tstart[threadIdx.x] = ts;
         }
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < tile_n; i += kBlock) {
-        const K k = skey[i];
-        const uint32_t d = digit_of(k, shift);
-        const int64_t o = (int64_t)gbase[d] + (i - (int)tstart[d]);
-        kout[o] = k;
-        vout[o] = sval[i];
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        if (wbase + j * 64 + lane < tile_n) {
+            const uint32_t d = digit_of(key[j], shift);
+            rank[j] += tstart[d] + wh[wave][d];  // the item's position in the tile's digit order
+            stage[rank[j]] = key[j];
+        }
+    }
+    __syncthreads();
+    uint32_t o[kOsItems];
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        const int i = threadIdx.x + j * kOsBlock;
+        if (i < tile_n) {
+            const K k = stage[i];
+            const uint32_t d = digit_of(k, shift);
+            o[j] = gbase[d] + (uint32_t)(i - (int)tstart[d]);
+            kout[o[j]] = k;
+        }
+    }
+    __syncthreads();
+    uint32_t* sval = reinterpret_cast<uint32_t*>(stage);
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j)
+        if (wbase + j * 64 + lane < tile_n) sval[rank[j]] = val[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        const int i = threadIdx.x + j * kOsBlock;
+        if (i < tile_n) vout[o[j]] = sval[i];
     }
 }
 
 template <typename K>
 hipError_t onesweep_impl(OnesweepBuffers<K>& b, int64_t n, int key_bits, hipStream_t s, Marker* mk) {
     const int passes = (key_bits + 7) / 8;
-    const int64_t tiles = (n + kTile - 1) / kTile;
+    const int64_t tiles = onesweep_tiles(n);
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_onesweep<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, b.keys[cur], b.vals[cur],
+        hipLaunchKernelGGL(k_onesweep<K>, dim3((unsigned)tiles), dim3(kOsBlock), 0, s, b.keys[cur], b.vals[cur],
                            b.keys[cur ^ 1], b.vals[cur ^ 1], n, 8 * p, b.hist + p * kRadix,
                            b.status + (int64_t)p * kRadix * tiles, b.tile_ctr + p, b.err);
         mark(mk, "sort_onesweep");
@@ -433,6 +498,7 @@ hipError_t sort_impl(SortBuffers<K>& b, int64_t n, int key_bits, LaunchSink& ls,
 }  // namespace
 
 int64_t sort_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
+int64_t onesweep_tiles(int64_t n) { return (n + kOsTile - 1) / kOsTile; }
 
 hipError_t onesweep_sort_pairs(OnesweepBuffers<uint32_t>& b, int64_t n, int key_bits, hipStream_t s,
                                Marker* mk) {

@@ -100,6 +100,7 @@ struct RenderArgs {
     uint32_t* counts;          // queue counts (16 words)
     unsigned long long* rays;  // traced closest / shadow rays
     uint32_t* trace_counters;  // traversal counters (32 words, see lbvh.h)
+    TraceCoop coop;            // buffers of the long-ray passes of the traversals
     float* film_rgb;           // [w*h][3]
     float* film_w;             // [w*h]
     float4* film_partial;      // [9][w*h] per-source-pixel partial sums (room for double ones)

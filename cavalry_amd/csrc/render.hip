@@ -267,7 +267,7 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
     float* hu = a.scene.needs_bary() ? a.hit_u : nullptr;
     float* hv = a.scene.needs_bary() ? a.hit_v : nullptr;
     if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[0].ray, a.counts, nq,
-                                 a.hit_prim, a.hit_t, hu, hv, a.trace_counters, a.launch, s)) != hipSuccess)
+                                 a.hit_prim, a.hit_t, hu, hv, a.trace_counters, a.launch, s, &a.coop)) != hipSuccess)
         return e;
     if (a.sort_perm && (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts, nq, a.sort_hist, a.sort_perm, s)) !=
                            hipSuccess)
@@ -275,7 +275,7 @@ hipError_t render_direct(const RenderArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_direct, dim3(nb), dim3(kBlock), 0, s, a.scene, a.halton, a.q[0], a.sh, a.hit_prim, a.hit_t,
                        hu, hv, a.res, a.counts, a.sort_perm);
     if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nq, a.sh.limit,
-                                  a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s)) != hipSuccess)
+                                  a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s, &a.coop)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_nee, dim3(nb), dim3(kBlock), 0, s, a.sh, a.vis, a.res, a.counts);
     hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, 0);
@@ -295,7 +295,7 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
         const int qout = qin ^ 1;
         if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[qin].ray,
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
-                                     a.trace_counters, a.launch, s)) != hipSuccess)
+                                     a.trace_counters, a.launch, s, &a.coop)) != hipSuccess)
             return e;
         if (a.sort_perm &&
             (e = sort_hits_by_material(a.scene, a.hit_prim, a.counts + qin, nsamp, a.sort_hist, a.sort_perm, s)) !=
@@ -319,7 +319,7 @@ hipError_t render_path(const RenderArgs& a, hipStream_t s) {
         }
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2,
                                       nsamp, a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch,
-                                      s)) != hipSuccess)
+                                      s, &a.coop)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_nee, dim3(nb), dim3(kBlock), 0, s, a.sh, a.vis, a.res, a.counts);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s, a.counts, a.rays, qin);

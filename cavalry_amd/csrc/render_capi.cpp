@@ -272,6 +272,10 @@ int ensure_buffers(cav_ctx* ctx, RenderState* rs, int64_t nq, int64_t ntot, int6
     if (rc == CAV_OK) rc = grab(ctx, rs, a.counts, 64, "queue counts");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.rays, 64, "ray tallies");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.trace_counters, 128, "trace counters");
+    unsigned char* coop = nullptr;
+    const uint32_t cc = any_spill_capacity(nq);
+    if (rc == CAV_OK) rc = grab(ctx, rs, coop, trace_coop_bytes(cc, cc), "long-ray pass buffers");
+    if (rc == CAV_OK) a.coop = trace_coop_in(coop, cc, cc);
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_rgb, 12 * P, "film");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_w, 4 * P, "film weights");
     if (rc == CAV_OK) rc = grab(ctx, rs, a.film_partial, 9 * 32 * P, "film partial sums");

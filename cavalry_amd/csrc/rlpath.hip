@@ -1023,7 +1023,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
         const RenderSrc src{st, depth};
         if ((e = trace_closest_queue(a.scene.rec, a.scene.ntris, a.scene.root, a.q[qin].ray,
                                      a.counts + qin, nsamp, a.hit_prim, a.hit_t, hu, hv,
-                                     a.trace_counters, a.launch, s)) != hipSuccess)
+                                     a.trace_counters, a.launch, s, &a.coop)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_dim_max, dim3((unsigned)((nsamp + kMaxBlock - 1) / kMaxBlock)), dim3(kMaxBlock), 0, s,
                            a.counts, qin, rl.q[qin], a.hit_prim, rl.dims);
@@ -1045,7 +1045,7 @@ hipError_t render_rlpath(const RenderArgs& a, const RlArgs& rl, bool init_tables
         ls.launch("k_lighting", &k_lighting<RenderSrc>, dim3(nb), dim3(kBlock), src, a.scene, a.halton, a.q[qin],
                   rl.slots, a.sh, a.hit_prim, a.hit_t, hu, hv, a.counts, qin, perm);
         if ((e = trace_occluded_queue(a.scene.rec, a.scene.ntris, a.sh.ray, a.counts + 2, nsamp,
-                                      a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s)) != hipSuccess)
+                                      a.sh.limit, a.sh.src, a.sh.dst, a.vis, a.trace_counters, a.launch, s, &a.coop)) != hipSuccess)
             return e;
         const RlRecords rec{rl.rec_keys[0], rl.rec_val, rl.rec_count, rl.rec_idx[0]};
         if (rec.keys) ls.launch("k_rec_reset", &k_rec_reset, dim3(1), dim3(64), rl.rec_count);

@@ -65,6 +65,7 @@ struct PoolDev {
     uint32_t* counts;          // [0], [1] ray queues, [2] shadow queue
     unsigned long long* rays;  // traced closest / shadow rays
     uint32_t* trace_counters;
+    TraceCoop coop;      // buffers of the long-ray passes of the traversals
     const ReqDev* reqs;
     int64_t cap;
     PoolRl rl;

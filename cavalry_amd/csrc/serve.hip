@@ -172,14 +172,14 @@ void pool_bounce(const PoolDev& p, int qin, int kind, LaunchSink& ls) {
     const unsigned nb = (unsigned)((p.cap + kBlock - 1) / kBlock);
     trace_closest_queue(p.scene.rec, p.scene.ntris, p.scene.root, p.q[qin].ray, p.counts + qin, p.cap, p.hit_prim,
                         p.hit_t, p.scene.needs_bary() ? p.hit_u : nullptr, p.scene.needs_bary() ? p.hit_v : nullptr,
-                        p.trace_counters, p.launch, ls);
+                        p.trace_counters, p.launch, ls, &p.coop);
     const auto grid = [&p](int b) { return dim3((unsigned)((p.cap + b - 1) / b)); };
     if (p.launch.shade_block == 64) ls.launch("k_pool_shade", &k_pool_shade<64>, grid(64), dim3(64), p, qin);
     else if (p.launch.shade_block == 128) ls.launch("k_pool_shade", &k_pool_shade<128>, grid(128), dim3(128), p, qin);
     else ls.launch("k_pool_shade", &k_pool_shade<256>, grid(256), dim3(256), p, qin);
     if (kind != kBouncePlain) pool_rl_lighting(p, qin, ls);
     trace_occluded_queue(p.scene.rec, p.scene.ntris, p.sh.ray, p.counts + 2, p.cap, p.sh.limit, p.sh.src, p.sh.dst,
-                         p.vis, p.trace_counters, p.launch, ls);
+                         p.vis, p.trace_counters, p.launch, ls, &p.coop);
     ls.launch("k_pool_nee", &k_pool_nee, dim3(nb), dim3(kBlock), p);
     if (kind != kBouncePlain) pool_rl_learn(p, qin, kind == kBounceRlDet, ls);
     ls.launch("k_pool_step", &k_pool_step, dim3(1), dim3(64), p, qin);

@@ -264,6 +264,10 @@ int make_pool(cav_server* sv, cav_scene* sc, Pool*& out) {
     A(d.sh.contrib, 12 * C); A(d.sh_rid, 4 * C); A(d.vis, C); A(d.hit_prim, 4 * C); A(d.hit_t, 4 * C);
     A(d.hit_u, 4 * C); A(d.hit_v, 4 * C);
     A(d.counts, 64); A(d.rays, 64); A(d.trace_counters, 128);
+    unsigned char* coop = nullptr;
+    const uint32_t cc = any_spill_capacity((int64_t)C);
+    A(coop, trace_coop_bytes(cc, cc));
+    if (rc == CAV_OK) d.coop = trace_coop_in(coop, cc, cc);
     // rlpath tasks: per-slot learning state, queue and bounce state, records
     PoolRl& R = d.rl;
     A(R.req, sizeof(RlReqState) * kSlots); A(R.live, kSlots);

@@ -87,18 +87,32 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t
 // with many coincident Morton codes). Rays are handed out per wave from
 // chunks taken with one atomic. Each ray's node order is the reference's, so
 // results do not depend on which lane or wave traced it.
+//
+// Phase 1 of three: once every ray has been handed out, a ray that has taken
+// `lr.after` steps (0: never) is handed, while the list has room, to the
+// cooperative pass (k_closest_coop), which traces it again from the root
+// with a whole wave; the few steps taken here are dropped. ids (optional)
+// maps the dispensed index to the ray index.
+struct LongRays {
+    uint32_t* ids = nullptr;  // cap ray indices
+    uint32_t* n = nullptr;    // rays handed over (past cap: the rest stayed on their lanes)
+    uint32_t cap = 0;
+    int after = 0;            // steps before a ray may be handed over (0: never)
+};
+
 template <int kLdsDepth, int kB>
 __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
                                                   const float* __restrict__ root,
                                                   const float* __restrict__ rays, int64_t nrays_arg,
                                                   const uint32_t* __restrict__ nrays_dev,
+                                                  const uint32_t* __restrict__ ids,
                                                   int32_t* __restrict__ oprim,
                                                   float* __restrict__ ot, float* __restrict__ ou,
                                                   float* __restrict__ ov,
                                                   uint32_t* __restrict__ overflow,
                                                   unsigned long long* __restrict__ next_ray,
                                                   unsigned long long* __restrict__ fetched,
-                                                  int max_steps, int chunk) {
+                                                  LongRays lr, int max_steps, int chunk) {
     __shared__ int lstack[kLdsDepth][kB];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -109,6 +123,8 @@ __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
     float best = FLT_MAX, bu = 0.f, bv = 0.f;
     int bp = -1, cur = 0, steps = 0;
     bool hit = false;
+    bool may_hand = false;  // this lane's ray may still go to the cooperative pass
+    bool drained = false;   // every ray has been handed out
     uint32_t ovf = 0;
     int maxsp = 0;  // deepest stack this lane reached
     uint64_t nfetch = 0;  // 64-byte records this lane fetched (one per step)
@@ -141,11 +157,12 @@ __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
                 if (mine >= nrays) {
                     ri = -2;
                 } else {
-                    ri = mine;
+                    ri = ids ? (int64_t)ids[mine] : mine;
                     load_ray(rays, ri, o, d);
                     inv = div3(mk3(1.f, 1.f, 1.f), d);
                     best = FLT_MAX; hit = false; bp = -1; bu = 0.f; bv = 0.f;
                     sp = 0; cur = 0; steps = 0;
+                    may_hand = lr.after > 0;
                     // BVH/BVH.h:34-38: a ray missing the root box has no hit
                     if (slab(root[0], root[1], root[2], root[3], root[4], root[5], o, inv) < 0.f) {
                         oprim[ri] = -1; ot[ri] = 0.f;
@@ -161,6 +178,19 @@ __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
             continue;                        // lanes whose ray missed the root refill
         }
         if (ri < 0) continue;
+
+        if (may_hand && steps >= lr.after && !drained && ((steps - lr.after) & 255) == 0)
+            drained = (int64_t)__hip_atomic_load(next_ray, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nrays;
+        if (may_hand && drained && steps >= lr.after) {
+            // a long ray at the end: the cooperative pass traces it
+            const uint32_t slot = atomicAdd(lr.n, 1u);
+            if (slot < lr.cap) {
+                lr.ids[slot] = (uint32_t)ri;
+                ri = -1;
+                continue;
+            }
+            may_hand = false;  // the list is full: finish it here
+        }
 
         // ---- one traversal step for this lane's ray
         bool done = false;
@@ -381,7 +411,7 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
         if (!__ballot(ri >= 0)) break;  // every lane drained
         if (ri < 0) continue;
 
-        if (may_spill && steps >= spill_after && !drained && (steps & 255) == 0)
+        if (may_spill && steps >= spill_after && !drained && ((steps - spill_after) & 255) == 0)
             drained = (int64_t)__hip_atomic_load(next_ray, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nrays;
         if (may_spill && drained && steps >= spill_after) {
             // a long ray at the end: hand its pending pairs to the cooperative pass
@@ -570,6 +600,264 @@ __global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, 
     if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
 }
 
+// Phase 2 of the closest-hit traversal: one wave per long ray, from the
+// root. The wave keeps the pending nodes in LDS (with where their boxes
+// start) and takes up to 64 of them at a time, newest first, one per lane;
+// the hit children go back on top (the near one last, so it is taken first).
+// The nearest hit so far lives in LDS (float bits, atomic min). A node whose
+// box starts more than kTieMargin beyond it is not fetched, a child box that
+// starts so far is not pushed.
+//
+// The reference's answer (BVH/BVH.h:58-95) is the nearest hit its
+// depth-first order keeps: a later hit must be strictly nearer, and a child
+// whose box starts at or beyond the nearest hit so far is skipped. So the
+// nearest hit M is its answer unless a box on M's path starts at or beyond
+// an earlier hit X; such a box starts at or before M's own box (e_M), so X
+// lies at or before max(t_M, e_M). Every hit within kTieMargin of the
+// running nearest is recorded (a full list drops those no longer within
+// it), and M is the answer when no other recorded hit lies at or before
+// max(t_M, e_M) and e_M is within half the margin of t_M (rounding puts a
+// triangle's hit a hair before the start of its own box at most; a hit
+// that far before it -- an incidence within about 1e-4 rad of its plane --
+// would have to be a rival this pass prunes). Otherwise the ray goes to
+// phase 3, the reference's order on one lane, as it does when the pending
+// list outgrows LDS, when more than kCoopCands hits lie within the margin,
+// or when a node could sit kCoopDeep or more entries deep in the
+// reference's 64-entry stack (the depth counted here assumes every pair of
+// intersected children is pushed, a bound on the reference's).
+constexpr int kCoopNodes = 2048;
+constexpr int kCoopCands = 64;
+constexpr int kCoopDeep = 56;
+constexpr float kTieMargin = 1.0f + 1.0f / 1024.0f;
+constexpr float kHalfMargin = 1.0f + 1.0f / 2048.0f;
+__global__ __launch_bounds__(64) void k_closest_coop(const TRec* __restrict__ recs, const float* __restrict__ rays,
+                                                     int32_t* __restrict__ oprim, float* __restrict__ ot,
+                                                     float* __restrict__ ou, float* __restrict__ ov,
+                                                     unsigned long long* __restrict__ fetched,
+                                                     const uint32_t* __restrict__ long_ids, const uint32_t* long_n,
+                                                     uint32_t long_cap, uint32_t* next_long,
+                                                     uint32_t* __restrict__ fail_ids, uint32_t* fail_n) {
+    __shared__ int32_t wref[kCoopNodes];
+    __shared__ float wdist[kCoopNodes];  // where the node's box starts along the ray
+    __shared__ uint8_t wdep[kCoopNodes];
+    __shared__ float cand_t[kCoopCands];
+    __shared__ int32_t cand_p[kCoopCands];
+    __shared__ float cand_u[kCoopCands];
+    __shared__ float cand_v[kCoopCands];
+    __shared__ float cand_e[kCoopCands];  // where the hit triangle's own box starts
+    __shared__ uint32_t sbest;  // float bits of the nearest hit so far (hits are at t > 0)
+    __shared__ uint32_t ncand;
+    const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t total = min(*long_n, long_cap);
+    uint64_t nfetch = 0;
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(next_long, 1u);
+        idx = __shfl(idx, 0, 64);
+        if (idx >= total) break;
+        const int64_t ri = long_ids[idx];
+        v3 o, d;
+        load_ray(rays, ri, o, d);
+        const v3 inv = div3(mk3(1.f, 1.f, 1.f), d);
+        if (lane == 0) {
+            wref[0] = 0;  // the root (phase 1 found its box hit)
+            wdist[0] = 0.f;
+            wdep[0] = 0;
+            sbest = __float_as_uint(FLT_MAX);
+            ncand = 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        int count = 1;
+        bool failed = false;
+        while (count > 0) {
+            const int k = count < 64 ? count : 64;
+            const float reach = __uint_as_float(*(volatile uint32_t*)&sbest) * kTieMargin;
+            bool act = lane < k;
+            int ref = 0, dep = 0;
+            float edist = 0.f;
+            if (act) {
+                ref = wref[count - 1 - lane];
+                dep = wdep[count - 1 - lane];
+                edist = wdist[count - 1 - lane];
+                // a node whose box starts beyond the nearest hit found since it
+                // was pushed is not fetched
+                act = edist <= reach;
+            }
+            count -= k;
+            __builtin_amdgcn_wave_barrier();
+            int nout = 0, out0 = 0, out1 = 0, dep0 = 0, dep1 = 0;
+            float dist0 = 0.f, dist1 = 0.f;
+            bool bad = false;
+            bool pend = false;  // a recorded hit that did not fit in the full list
+            float pt = 0.f, pu = 0.f, pv = 0.f, pe = 0.f;
+            int pp = 0;
+            if (act) {
+                const float4* rec = reinterpret_cast<const float4*>(recs + (ref >= 0 ? ref : ~ref));
+                const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+                const int4 refs = reinterpret_cast<const int4*>(rec)[3];
+                ++nfetch;
+                if (ref >= 0) {  // BVH/BVH.h:72-95
+                    const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
+                    const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
+                    const bool hl = dl >= 0 && dl <= reach;
+                    const bool hr = dr >= 0 && dr <= reach;
+                    // the reference's stack depth, bounded: with both children
+                    // intersected, the near one is one entry deeper than the far one
+                    const bool both = dl >= 0 && dr >= 0;
+                    const bool lfar = dl > dr;
+                    const int dnear = both ? dep + 1 : dep;
+                    if (hl && hr) {  // far first, near last (taken first)
+                        out0 = lfar ? refs.x : refs.y;
+                        dist0 = lfar ? dl : dr;
+                        dep0 = dep;
+                        out1 = lfar ? refs.y : refs.x;
+                        dist1 = lfar ? dr : dl;
+                        dep1 = dnear;
+                        nout = 2;
+                    } else if (hl || hr) {
+                        out0 = hl ? refs.x : refs.y;
+                        dist0 = hl ? dl : dr;
+                        dep0 = (hl != lfar) ? dnear : dep;  // hl != lfar: the kept child is the near one
+                        nout = 1;
+                    }
+                    bad = dnear >= kCoopDeep;
+                } else {  // BVH/BVH.h:58-70
+                    const TTri tr{q0, q1, q2};
+                    float t, u, v;
+                    if (tri_hit(tr, o, d, t, u, v)) {
+                        const uint32_t old = atomicMin(&sbest, __float_as_uint(t));
+                        if (t <= __uint_as_float(old) * kTieMargin) {
+                            const uint32_t c = atomicAdd(&ncand, 1u);
+                            pt = t;
+                            pp = __float_as_int(tr.a.w);
+                            pu = u;
+                            pv = v;
+                            pe = edist;
+                            if (c < (uint32_t)kCoopCands) {
+                                cand_t[c] = pt;
+                                cand_p[c] = pp;
+                                cand_u[c] = pu;
+                                cand_v[c] = pv;
+                                cand_e[c] = pe;
+                            } else {
+                                pend = true;  // the list is full: compacted below
+                            }
+                        }
+                    }
+                }
+            }
+            if (__ballot(bad)) {
+                failed = true;
+                break;
+            }
+            if (__ballot(pend)) {
+                // keep the recorded hits still within kTieMargin of the nearest
+                // (the list holds kCoopCands = 64 of them, one per lane), then
+                // add the ones that did not fit
+                __builtin_amdgcn_wave_barrier();
+                const float lim = __uint_as_float(sbest) * kTieMargin;
+                const float ct = cand_t[lane], cu = cand_u[lane], cv = cand_v[lane], ce = cand_e[lane];
+                const int cp = cand_p[lane];
+                const bool keep = ct <= lim;
+                const uint64_t km = __ballot(keep);
+                const bool add = pend && pt <= lim;
+                const uint64_t am = __ballot(add);
+                const int nk = __popcll(km), na = __popcll(am);
+                __builtin_amdgcn_wave_barrier();
+                if (nk + na > kCoopCands) {
+                    failed = true;  // more than 64 hits within a hair of the nearest
+                    break;
+                }
+                if (keep) {
+                    const int i = __popcll(km & lt);
+                    cand_t[i] = ct;
+                    cand_p[i] = cp;
+                    cand_u[i] = cu;
+                    cand_v[i] = cv;
+                    cand_e[i] = ce;
+                }
+                if (add) {
+                    const int i = nk + __popcll(am & lt);
+                    cand_t[i] = pt;
+                    cand_p[i] = pp;
+                    cand_u[i] = pu;
+                    cand_v[i] = pv;
+                    cand_e[i] = pe;
+                }
+                if (lane == 0) ncand = (uint32_t)(nk + na);
+                __builtin_amdgcn_wave_barrier();
+            }
+            // append the children: a wave prefix over the lanes' counts
+            int pos = nout;
+#pragma unroll
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                const int v = __shfl_up(pos, dd, 64);
+                if (lane >= dd) pos += v;
+            }
+            const int added = __shfl(pos, 63, 64);
+            pos -= nout;  // exclusive
+            if (count + added > kCoopNodes) {
+                failed = true;
+                break;
+            }
+            if (nout > 0) {
+                wref[count + pos] = out0;
+                wdist[count + pos] = dist0;
+                wdep[count + pos] = (uint8_t)dep0;
+            }
+            if (nout > 1) {
+                wref[count + pos + 1] = out1;
+                wdist[count + pos + 1] = dist1;
+                wdep[count + pos + 1] = (uint8_t)dep1;
+            }
+            count += added;
+            __builtin_amdgcn_wave_barrier();
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (!failed) {
+            // The nearest hit M is the reference's answer unless the
+            // reference's order skips it: a box on M's path is skipped only
+            // at or beyond a hit found before it, and every such box starts
+            // at or before M's own box (e_M), so a competitor is a recorded
+            // hit other than M at or before max(t_M, e_M). Rounding puts e_M
+            // beyond t_M by a hair at most; past half the margin, phase 3.
+            const float bt = __uint_as_float(sbest);
+            const int nc = (int)min(ncand, (uint32_t)kCoopCands);
+            const bool at_min = lane < nc && cand_t[lane] == bt;
+            const uint64_t am = __ballot(at_min);
+            if (am == 0ull) {  // no hit
+                if (lane == 0) {
+                    oprim[ri] = -1;
+                    ot[ri] = 0.f;
+                    if (ou) ou[ri] = 0.f;
+                    if (ov) ov[ri] = 0.f;
+                }
+            } else if (__popcll(am) > 1) {
+                failed = true;  // two triangles at the nearest distance: the order decides
+            } else {
+                const int m = __ffsll((long long)am) - 1;
+                const float em = cand_e[m];
+                const float bound = fmaxf(bt, em);
+                const bool rival = lane < nc && lane != m && cand_t[lane] <= bound;
+                if (__ballot(rival) != 0ull || em > bt * kHalfMargin) {
+                    failed = true;
+                } else if (lane == m) {
+                    oprim[ri] = cand_p[m];
+                    ot[ri] = bt;
+                    if (ou) ou[ri] = cand_u[m];
+                    if (ov) ov[ri] = cand_v[m];
+                }
+            }
+        }
+        if (failed && lane == 0) fail_ids[atomicAdd(fail_n, 1u)] = (uint32_t)ri;
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) nfetch += __shfl_xor(nfetch, dd, 64);
+    if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
+}
+
 // 64-bit count of records fetched by closest-hit (words 16-17) or any-hit
 // (words 18-19) launches
 __host__ __device__ inline unsigned long long* fetch_word(uint32_t* counters, bool closest) {
@@ -645,10 +933,20 @@ void launch_any_pairs(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... a
     }
 }
 
-// Steps (sibling pairs) a ray must have taken, once the rays have all been
-// handed out, to be handed to the cooperative pass: the short rays still in
-// flight then finish where they are.
-constexpr int kAnySpillAfter = 1024;
+// L.handoff: steps (any-hit: sibling pairs) a ray must have taken, once the
+// rays have all been handed out, to go to a cooperative pass, so that the
+// short rays still in flight finish where they are. A closest-hit ray's
+// steps are dropped: its cooperative pass starts again from the root.
+
+// Workgroups of a cooperative pass: as many one-wave workgroups as fit.
+int coop_grid(const void* kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess || per <= 0) per = 4;
+    return cus * per;
+}
 
 // Any-hit in three launches: phase 1 (one lane per ray, long rays spilled),
 // phase 2 (a wave per spilled ray), phase 3 (rays phase 2 could not hold,
@@ -661,20 +959,11 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
     auto* disp = reinterpret_cast<unsigned long long*>(counters + 4);
     launch_any_pairs(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
                      counters + 1, disp, fetch_word(counters, false), coop ? spill : (AnySpill*)nullptr,
-                     counters + 20, coop ? spill_cap : 0u, coop ? kAnySpillAfter : 0, 2 * ntris - 1);
+                     counters + 20, coop ? spill_cap : 0u, coop ? L.handoff : 0, 2 * ntris - 1);
     if (!coop) return;
     uint32_t* fail_ids = reinterpret_cast<uint32_t*>(spill + spill_cap);
-    static const int coop_blocks = [] {
-        int dev = 0, cus = 0, per = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_any_coop), 64, 0) !=
-                hipSuccess || per <= 0)
-            per = 4;
-        return cus * per;
-    }();
-    ls.launch("k_any_coop", &k_any_coop, dim3((unsigned)coop_blocks), dim3(64), rec, rays, limit, src, dst, visible,
+    static const int blocks = coop_grid(reinterpret_cast<const void*>(&k_any_coop));
+    ls.launch("k_any_coop", &k_any_coop, dim3((unsigned)blocks), dim3(64), rec, rays, limit, src, dst, visible,
               counters + 1, fetch_word(counters, false), (const AnySpill*)spill, (const uint32_t*)(counters + 20),
               spill_cap, counters + 21, fail_ids, counters + 22);
     launch_any_pairs(L, (int64_t)spill_cap, ls, rec, rays, (int64_t)spill_cap, (const uint32_t*)(counters + 22),
@@ -683,66 +972,102 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
                      (AnySpill*)nullptr, counters + 20, 0u, 0, 2 * ntris - 1);
 }
 
+// Closest-hit in three launches, like any-hit: phase 1 (one lane per ray,
+// long rays handed over), phase 2 (a wave per long ray, k_closest_coop),
+// phase 3 (the rays phase 2 sent back, one lane each, the reference's
+// order). Words 26..30 of the counters: long rays, their cursor, phase-3
+// dispenser (64-bit, words 28-29), phase-3 rays.
+void trace_near(const TRec* rec, int ntris, const float* root, const float* rays, int64_t nrays,
+                const uint32_t* nrays_dev, int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
+                const TraceLaunch& L, const TraceCoop* coop, LaunchSink& ls) {
+    const bool on = coop && coop->longs && coop->long_cap > 0 && nrays < (int64_t(1) << 32);
+    LongRays lr;
+    if (on) {
+        lr.ids = coop->longs;
+        lr.n = counters + 26;
+        lr.cap = coop->long_cap;
+        lr.after = L.handoff;
+    }
+    launch_trace(L, nrays, ls, rec, root, rays, nrays, nrays_dev, (const uint32_t*)nullptr, prim, t, u, v, counters,
+                 reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true), lr, 2 * ntris - 1);
+    if (!on) return;
+    uint32_t* fail_ids = coop->longs + coop->long_cap;
+    static const int blocks = coop_grid(reinterpret_cast<const void*>(&k_closest_coop));
+    ls.launch("k_closest_coop", &k_closest_coop, dim3((unsigned)blocks), dim3(64), rec, rays, prim, t, u, v,
+              fetch_word(counters, true), (const uint32_t*)coop->longs, (const uint32_t*)(counters + 26),
+              coop->long_cap, counters + 27, fail_ids, counters + 30);
+    launch_trace(L, (int64_t)coop->long_cap, ls, rec, root, rays, (int64_t)coop->long_cap,
+                 (const uint32_t*)(counters + 30), (const uint32_t*)fail_ids, prim, t, u, v, counters,
+                 reinterpret_cast<unsigned long long*>(counters + 28), fetch_word(counters, true), LongRays{},
+                 2 * ntris - 1);
+}
+
 }  // namespace
 
 hipError_t trace_closest(const TRec* rec, int ntris, const float* root,
                          const float* rays, int64_t nrays, int32_t* prim, float* t, float* u,
-                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s) {
+                         float* v, uint32_t* counters, const TraceLaunch& L, hipStream_t s,
+                         const TraceCoop* coop) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 2, 0, 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(counters + 26, 0, 20, s);
     if (e != hipSuccess) return e;
     LaunchSink ls(s);
-    launch_trace(L, nrays, ls, rec, root, rays, nrays, (const uint32_t*)nullptr, prim, t, u, v, counters,
-                 reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true), 2 * ntris - 1);
+    trace_near(rec, ntris, root, rays, nrays, nullptr, prim, t, u, v, counters, L, coop, ls);
     return ls.err;
 }
 
 hipError_t trace_occluded(const TRec* rec, int ntris, const float* rays,
                           const float* limit, const int32_t* src, const int32_t* dst,
                           int64_t nrays, uint8_t* visible, uint32_t* counters, const TraceLaunch& L,
-                          hipStream_t s, AnySpill* spill, uint32_t spill_cap) {
+                          hipStream_t s, const TraceCoop* coop) {
     if (nrays <= 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(counters + 4, 0, 8, s);
     if (e == hipSuccess) e = hipMemsetAsync(counters + 20, 0, 24, s);
     if (e != hipSuccess) return e;
     LaunchSink ls(s);
-    trace_any(rec, ntris, rays, nrays, nullptr, limit, src, dst, visible, counters, L, spill, spill_cap, ls);
+    trace_any(rec, ntris, rays, nrays, nullptr, limit, src, dst, visible, counters, L,
+              coop ? coop->spill : nullptr, coop ? coop->spill_cap : 0u, ls);
     return ls.err;
 }
 
 // Queue variants: the ray count is read on the device (no host round trip),
-// the grid is the full persistent size. counters as above. The dispenser is
-// cleared by a kernel, so the sequence can be recorded into a graph.
+// the grid is the full persistent size. counters as above. The dispensers
+// and pass counters are cleared by a kernel, so the sequence can be recorded
+// into a graph.
 void trace_closest_queue(const TRec* rec, int ntris, const float* root, const float* rays,
                          const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
-                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
+                         uint32_t* counters, const TraceLaunch& L, LaunchSink& ls, const TraceCoop* coop) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 2, 2);
-    launch_trace(L, capacity, ls, rec, root, rays, capacity, nrays_dev, prim, t, u, v, counters,
-                 reinterpret_cast<unsigned long long*>(counters + 2), fetch_word(counters, true), 2 * ntris - 1);
+    if (coop && coop->longs) ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 26, 5);
+    trace_near(rec, ntris, root, rays, capacity, nrays_dev, prim, t, u, v, counters, L, coop, ls);
 }
 
 void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const uint32_t* nrays_dev,
                           int64_t capacity, const float* limit, const int32_t* src, const int32_t* dst,
-                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls) {
+                          uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls,
+                          const TraceCoop* coop) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 4, 2);
-    trace_any(rec, ntris, rays, capacity, nrays_dev, limit, src, dst, visible, counters, L, nullptr, 0, ls);
+    if (coop && coop->spill) ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 20, 6);
+    trace_any(rec, ntris, rays, capacity, nrays_dev, limit, src, dst, visible, counters, L,
+              coop ? coop->spill : nullptr, coop ? coop->spill_cap : 0u, ls);
 }
 
 hipError_t trace_closest_queue(const TRec* rec, int ntris, const float* root,
                                const float* rays, const uint32_t* nrays_dev, int64_t capacity,
                                int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
-                               const TraceLaunch& L, hipStream_t s) {
+                               const TraceLaunch& L, hipStream_t s, const TraceCoop* coop) {
     LaunchSink ls(s);
-    trace_closest_queue(rec, ntris, root, rays, nrays_dev, capacity, prim, t, u, v, counters, L, ls);
+    trace_closest_queue(rec, ntris, root, rays, nrays_dev, capacity, prim, t, u, v, counters, L, ls, coop);
     return ls.err;
 }
 
 hipError_t trace_occluded_queue(const TRec* rec, int ntris, const float* rays,
                                 const uint32_t* nrays_dev, int64_t capacity, const float* limit,
                                 const int32_t* src, const int32_t* dst, uint8_t* visible,
-                                uint32_t* counters, const TraceLaunch& L, hipStream_t s) {
+                                uint32_t* counters, const TraceLaunch& L, hipStream_t s, const TraceCoop* coop) {
     LaunchSink ls(s);
-    trace_occluded_queue(rec, ntris, rays, nrays_dev, capacity, limit, src, dst, visible, counters, L, ls);
+    trace_occluded_queue(rec, ntris, rays, nrays_dev, capacity, limit, src, dst, visible, counters, L, ls, coop);
     return ls.err;
 }
 

@@ -15,9 +15,11 @@ rays in flight (less L2 thrash), small or coherent ones prefer more
 
 `tune_trace` times candidate shapes on the caller's own rays (closest-hit and
 any-hit, best of `reps`), by coordinate descent from the current shape over
-workgroup size (at the same waves per CU), workgroups per CU, chunk and LDS
-depth, sets the fastest on the context and records it in a JSON cache keyed
-by device, tree size and batch size. A later call with the same key reuses
+workgroup size (at the same waves per CU), workgroups per CU, chunk, LDS
+depth and the steps after which a ray still in flight at the end of a launch
+goes to a cooperative pass (a whole wave per ray), sets the fastest on the
+context and records it in a JSON cache keyed by device, tree size and batch
+size. A later call with the same key reuses
 the cached shape without timing (`use_cache`). `tune_render` does the same
 for the render loops (WAVEFRONT) on one render, and also descends over the
 shading workgroup size.
@@ -44,6 +46,7 @@ SIZES = (64, 128, 256)          # threads per workgroup (traversal and shading)
 WAVES = (4, 8, 12, 16, 20)      # traversal waves per CU tried (1..5 workgroups of 256)
 PER_CU = tuple(sorted({w * 64 // b for w in WAVES for b in SIZES}))
 CHUNKS = (128, 256, 512, 1024)
+HANDOFFS = (256, 1024, 4096)    # steps before a ray still in flight at the end goes to a whole wave
 LDS = (32, 48)
 ROUNDS = (0, 1, 2, 3, 4, 6, 8)
 
@@ -56,14 +59,16 @@ def get_launch(ctx, which: int) -> dict:
     t = TraceLaunch()
     check(ctx.handle, lib().cav_trace_get_launch(ctx.handle, int(which), ctypes.byref(t)))
     return {"workgroups_per_cu": t.workgroups_per_cu, "ray_chunk": t.ray_chunk, "lds_stack": t.lds_stack,
-            "workgroup_size": t.workgroup_size, "shade_workgroup_size": t.shade_workgroup_size}
+            "workgroup_size": t.workgroup_size, "shade_workgroup_size": t.shade_workgroup_size,
+            "handoff_steps": t.handoff_steps}
 
 
 def set_launch(ctx, which: int, shape: dict) -> None:
-    """Sets a shape; missing workgroup sizes mean 256 (shapes cached before
-    the sizes were tunable)."""
+    """Sets a shape; missing workgroup sizes mean 256 and a missing hand-off
+    1024 steps (shapes cached before they were tunable)."""
     t = TraceLaunch(int(shape["workgroups_per_cu"]), int(shape["ray_chunk"]), int(shape["lds_stack"]),
-                    int(shape.get("workgroup_size", 0) or 0), int(shape.get("shade_workgroup_size", 0) or 0))
+                    int(shape.get("workgroup_size", 0) or 0), int(shape.get("shade_workgroup_size", 0) or 0),
+                    int(shape.get("handoff_steps", 0) or 0))
     check(ctx.handle, lib().cav_trace_set_launch(ctx.handle, int(which), ctypes.byref(t)))
 
 
@@ -102,13 +107,15 @@ def _store(path: str, key: str, entry: dict) -> None:
 
 def _key(shape: dict) -> str:
     return (f"{shape['workgroups_per_cu']}/{shape['ray_chunk']}/{shape['lds_stack']}/"
-            f"{shape.get('workgroup_size', 256) or 256}/{shape.get('shade_workgroup_size', 256) or 256}")
+            f"{shape.get('workgroup_size', 256) or 256}/{shape.get('shade_workgroup_size', 256) or 256}/"
+            f"{shape.get('handoff_steps', 1024) or 1024}")
 
 
 def _normal(shape: dict) -> dict:
     s = dict(shape)
     s["workgroup_size"] = s.get("workgroup_size", 0) or 256
     s["shade_workgroup_size"] = s.get("shade_workgroup_size", 0) or 256
+    s["handoff_steps"] = s.get("handoff_steps", 0) or 1024
     return s
 
 
@@ -120,9 +127,9 @@ def _per_cu_for(size: int) -> tuple:
 
 def _descend(ctx, which: int, timer, start: dict, reps: int, shade: bool = False) -> tuple[dict, dict]:
     """Coordinate descent from `start` over the workgroup size (keeping the
-    waves per CU), workgroups per CU, chunk, LDS depth and, with `shade`,
-    the shading workgroup size; returns the best shape and every timing
-    taken (ms)."""
+    waves per CU), workgroups per CU, chunk, LDS depth, hand-off steps and,
+    with `shade`, the shading workgroup size; returns the best shape and
+    every timing taken (ms)."""
     timings = {}
 
     def measure(shape):
@@ -145,7 +152,7 @@ def _descend(ctx, which: int, timer, start: dict, reps: int, shade: bool = False
         ms = measure(cand)
         if ms < best_ms:
             best, best_ms = cand, ms
-    fields = [("workgroups_per_cu", None), ("ray_chunk", CHUNKS), ("lds_stack", LDS)]
+    fields = [("workgroups_per_cu", None), ("ray_chunk", CHUNKS), ("lds_stack", LDS), ("handoff_steps", HANDOFFS)]
     if shade:
         fields.append(("shade_workgroup_size", SIZES))
     for field, values in fields:

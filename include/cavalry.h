@@ -137,13 +137,16 @@ int cav_trace_occluded_async(cav_ctx* ctx, const cav_bvh* bvh, const float* rays
  * launch. reset != 0 clears all four after reading. No counterpart in the
  * reference, which writes past its stack. */
 int cav_trace_counters(cav_ctx* ctx, const cav_bvh* bvh, uint32_t* out, int32_t reset);
-/* Work of a BVH's standalone launches: out[0] / out[1] = 64-byte traversal
- * records (node visits and triangle tests, one record each) fetched by every
- * closest-hit / any-hit launch since the last reset; out[2] = rays of the
- * last any-hit launch finished by a whole wave (long rays still in flight
- * once every ray was handed out), out[3] = those of them the wave could not
- * hold and one lane traced again from the root. reset != 0 clears out[0..1]
- * after reading. No counterpart
+/* Work of a BVH's standalone launches (out holds 6 words): out[0] / out[1] =
+ * 64-byte traversal records (node visits and triangle tests, one record
+ * each) fetched by every closest-hit / any-hit launch since the last reset;
+ * out[2] = rays of the last any-hit launch finished by a whole wave (long
+ * rays still in flight once every ray was handed out), out[3] = those of
+ * them the wave could not hold and one lane traced again from the root;
+ * out[4] = rays of the last closest-hit launch traced again by a whole wave
+ * from the root, out[5] = those of them one lane traced once more in the
+ * reference's order (two hits within a hair of each other, or a stack that
+ * could overflow). reset != 0 clears out[0..1] after reading. No counterpart
  * in the reference; it measures the algorithmic traffic of BVH::intersect /
  * testVisibility (BVH/BVH.h:23-189). */
 int cav_trace_fetches(cav_ctx* ctx, const cav_bvh* bvh, uint64_t* out, int32_t reset);
@@ -392,7 +395,9 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
  * the reference's 64 in scratch) and threads per traversal workgroup (64,
  * 128 or 256; 0: 256); for WAVEFRONT also the threads per workgroup of the
  * per-sample shading kernels (path shading, the RL integrator's Q-table
- * sampling; 64, 128 or 256; 0: 256). Each ray's node order is the
+ * sampling; 64, 128 or 256; 0: 256), and the steps a ray still in flight
+ * once every ray has been handed out must have taken to be finished by a
+ * whole wave instead of its lane (1..1<<30; 0: 1024). Each ray's node order is the
  * reference's and each sample's shading is its own either way, so results do
  * not depend on it. STANDALONE applies to cav_trace_*, WAVEFRONT to
  * cav_render and to servers opened on the context afterwards. No
@@ -405,6 +410,7 @@ typedef struct {
     int32_t lds_stack;
     int32_t workgroup_size;
     int32_t shade_workgroup_size;
+    int32_t handoff_steps;
 } cav_trace_launch;
 enum { CAV_TRACE_STANDALONE = 0, CAV_TRACE_WAVEFRONT = 1 };
 int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* launch);

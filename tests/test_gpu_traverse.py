@@ -51,3 +51,75 @@ def test_occluded_mesh_exclusion(ctx):
     bvh = BVH.build(ctx, tris, tri_mesh=mesh)
     vgot = bvh.test_visibility(rays, lim, src, dst)
     np.testing.assert_array_equal(vgot, vref)
+
+
+@pytest.mark.parametrize("n,nrays,edge,rounds", [(10000, 50000, 1e-3, 0), (200000, 100000, 0.01, 3),
+                                                 (4000, 30000, 0.3, 1)])
+def test_every_ray_through_the_cooperative_passes(ctx, n, nrays, edge, rounds):
+    """With a hand-off after one step, the rays still in flight once the
+    rays are all handed out -- here a large share: the batch is smaller than
+    the persistent grid -- are traced again by a whole wave (closest-hit) or
+    finished by one (any-hit). Hits and visibility stay the oracle's,
+    bit for bit; the rays the closest-hit pass sends back to one lane (two
+    hits within a hair, a deep stack) are traced in the reference's order."""
+    from cavalry_amd import tune
+    print(f"[trace coop] n={n} rays={nrays} edge={edge} rounds={rounds}", flush=True)
+    tris = scenes.gmm_triangles(n, seed=n + 11, edge=edge)
+    _, _, nd = oracle.lbvh_build(tris, 30)
+    if rounds:
+        oracle.treelet_optimize(nd, rounds)
+    rays = scenes.random_rays(nrays, seed=n + 12)
+    srays = scenes.random_rays(nrays, seed=n + 13, stream=2)
+    lim = scenes.shadow_limits(nrays, seed=n + 14)
+    lim[::5] = np.inf
+    ref = oracle.trace_closest(nd, tris, rays)
+    vref = oracle.trace_occluded(nd, tris, srays, lim)
+    bvh = BVH.build(ctx, tris, optimization_rounds=rounds)
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    try:
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=1))
+        got = bvh.intersect(rays)
+        f = bvh.trace_fetches()
+        vgot = bvh.test_visibility(srays, lim)
+        fv = bvh.trace_fetches()
+    finally:
+        tune.set_launch(ctx, tune.STANDALONE, saved)
+    print(f"[trace coop] handed over {f['long_closest_rays']} closest "
+          f"(retraced {f['long_closest_retraced']}), {fv['spilled_rays']} any-hit", flush=True)
+    np.testing.assert_array_equal(got["prim"], ref["prim"])
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(bits(got[k]), bits(ref[k]), err_msg=k)
+    np.testing.assert_array_equal(vgot, vref)
+    # (a lane asks whether the rays are all handed out at its ray's first
+    # step, then every 256 steps: the rays taken after the last wave drew
+    # its chunk go over)
+    assert f["long_closest_rays"] > nrays // 8 and fv["spilled_rays"] > nrays // 8
+    assert f["long_closest_retraced"] < f["long_closest_rays"]
+
+
+def test_cooperative_passes_match_one_lane_per_ray(ctx):
+    """At a size where long rays are the tail (2^18 triangles, 2^18 rays):
+    the default hand-off and no hand-off at all give the same hits and
+    visibility."""
+    from cavalry_amd import tune
+    print("[trace coop] default vs none at 2^18", flush=True)
+    tris = scenes.gmm_triangles(1 << 18, seed=21)
+    rays = scenes.random_rays(1 << 18, seed=22)
+    srays = scenes.random_rays(1 << 18, seed=23, stream=2)
+    lim = scenes.shadow_limits(1 << 18, seed=24)
+    bvh = BVH.build(ctx, tris, optimization_rounds=3)
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    try:
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=1 << 30))
+        a, va = bvh.intersect(rays), bvh.test_visibility(srays, lim)
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=64))
+        b, vb = bvh.intersect(rays), bvh.test_visibility(srays, lim)
+        f = bvh.trace_fetches()
+    finally:
+        tune.set_launch(ctx, tune.STANDALONE, saved)
+    print(f"[trace coop] {f}", flush=True)
+    assert f["long_closest_rays"] > 0
+    np.testing.assert_array_equal(a["prim"], b["prim"])
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=k)
+    np.testing.assert_array_equal(va, vb)

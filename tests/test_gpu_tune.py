@@ -65,9 +65,11 @@ def test_launch_arguments_checked(ctx):
 
 @pytest.mark.parametrize("integrator", ["path", "rlpath"])
 def test_render_same_under_every_workgroup_size(ctx, integrator):
-    """The render loops' traversal and shading workgroup sizes change only
-    the speed: the film is bit-identical (the RL integrator in its
-    deterministic table-update mode)."""
+    """The render loops' traversal and shading workgroup sizes and the
+    cooperative passes' hand-off (1 step: every ray of this small render
+    goes through them; 2^30: none) change only the speed: the film is
+    bit-identical (the RL integrator in its deterministic table-update
+    mode)."""
     print(f"[tune] render sizes {integrator}", flush=True)
     sc = scenes.cornell_box(seed=3, width=40, height=32)
     scene = R.Scene(ctx, sc)
@@ -75,9 +77,10 @@ def test_render_same_under_every_workgroup_size(ctx, integrator):
     kw = {"deterministic": True} if integrator == "rlpath" else {}
     ref = None
     try:
-        for size, shade, per_cu in ((256, 256, 0), (64, 64, 8), (128, 128, 3), (256, 64, 2), (64, 128, 0)):
+        for size, shade, per_cu, hand in ((256, 256, 0, 1 << 30), (64, 64, 8, 1), (128, 128, 3, 1024),
+                                          (256, 64, 2, 1), (64, 128, 0, 16)):
             tune.set_launch(ctx, tune.WAVEFRONT, dict(saved, workgroup_size=size, shade_workgroup_size=shade,
-                                                      workgroups_per_cu=per_cu))
+                                                      workgroups_per_cu=per_cu, handoff_steps=hand))
             img, _ = R.render(ctx, scene, 2, integrator=integrator, **kw)
             if ref is None:
                 ref = img.rgb.copy()
