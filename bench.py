@@ -498,6 +498,22 @@ def main() -> int:
         step(False)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
 
+    # the records the reference's visiting order fetches: one untimed trace
+    # of each kind with no hand-off to the cooperative passes (the closest-hit
+    # pass restarts its rays and visits in another order; the any-hit pass
+    # visits the same pairs)
+    from cavalry_amd import tune as _tune
+    shape = _tune.get_launch(ctx, _tune.STANDALONE)
+    _tune.set_launch(ctx, _tune.STANDALONE, dict(shape, handoff_steps=1 << 30))
+    bvh.trace_fetches(reset=True)
+    check(ctx.handle, lib().cav_trace_closest_async(
+        ctx.handle, bvh.handle, rays.data_ptr(), r, hit_prim.data_ptr(), hit_t.data_ptr()))
+    check(ctx.handle, lib().cav_trace_occluded_async(
+        ctx.handle, bvh.handle, srays.data_ptr(), lim.data_ptr(), r, vis.data_ptr()))
+    ctx.synchronize()
+    ref_fetch = bvh.trace_fetches(reset=True)
+    _tune.set_launch(ctx, _tune.STANDALONE, shape)
+
     torch.cuda.synchronize()
     ctx.synchronize()
     bvh.trace_counters(reset=True)  # count stack overflows of the timed traces only
@@ -560,8 +576,8 @@ def main() -> int:
     # the kernel over the timed steps) + the rays read and the hits written
     fetch = {k: v / args.steps for k, v in fetches.items()}
     trav_bytes = {
-        "closest": fetch["closest"] * TRACE_RECORD_BYTES + r * (24 + 8),
-        "occluded": fetch["occluded"] * TRACE_RECORD_BYTES + r * (24 + 4 + 1),
+        "closest": ref_fetch["closest"] * TRACE_RECORD_BYTES + r * (24 + 8),
+        "occluded": ref_fetch["occluded"] * TRACE_RECORD_BYTES + r * (24 + 4 + 1),
     }
     trav_ms = {"closest": stages["trace_closest"], "occluded": stages["trace_shadow"]}
     tdom = max(trav_ms, key=lambda k: trav_ms[k])
@@ -571,9 +587,11 @@ def main() -> int:
         "achieved": round(t_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(t_ach / HBM_PEAK_GBS, 4), "traffic": None,
         "avg_launch_ms": round(trav_ms[tdom], 4), "alg_bytes_per_launch": int(trav_bytes[tdom]),
-        "records_per_ray": round(fetch[tdom] / r, 1),
-        "note": "algorithmic bytes count every record the reference's visiting order fetches, "
-                "whether L2 or HBM serves it",
+        "records_per_ray": round(ref_fetch[tdom] / r, 1),
+        "records_fetched_per_ray": round(fetch[tdom] / r, 1),
+        "note": "algorithmic bytes count every record the reference's visiting order fetches "
+                "(records_per_ray, counted on an untimed trace without the cooperative passes), "
+                "whether L2 or HBM serves it; records_fetched_per_ray is what the timed traces fetched",
         "lbvh_build_frac": lbvh_roofline["build_frac"],
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
@@ -586,8 +604,14 @@ def main() -> int:
         if tr.get(lbvh_roofline["kernel"]):
             lbvh_roofline["traffic"] = tr[lbvh_roofline["kernel"]]
             lbvh_roofline["traffic_source"] = tr.get("_source", "profiles/traffic.json")
-    trav["records_per_ray_closest"] = round(fetch["closest"] / r, 1)
-    trav["records_per_ray_occluded"] = round(fetch["occluded"] / r, 1)
+    trav["records_per_ray_closest"] = round(ref_fetch["closest"] / r, 1)
+    trav["records_per_ray_occluded"] = round(ref_fetch["occluded"] / r, 1)
+    trav["records_fetched_per_ray_closest"] = round(fetch["closest"] / r, 1)
+    trav["records_fetched_per_ray_occluded"] = round(fetch["occluded"] / r, 1)
+    trav["long_closest_rays"] = int(fetches.get("long_closest_rays", 0))
+    trav["long_closest_retraced"] = int(fetches.get("long_closest_retraced", 0))
+    trav["spilled_any_rays"] = int(fetches.get("spilled_rays", 0))
+    trav["spilled_any_retraced"] = int(fetches.get("respilled_rays", 0))
 
     value = 2 * r / (ms_step * 1e-3) / 1e6
     out = {

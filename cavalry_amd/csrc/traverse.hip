@@ -936,7 +936,10 @@ void launch_any_pairs(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... a
 // L.handoff: steps (any-hit: sibling pairs) a ray must have taken, once the
 // rays have all been handed out, to go to a cooperative pass, so that the
 // short rays still in flight finish where they are. A closest-hit ray's
-// steps are dropped: its cooperative pass starts again from the root.
+// steps are dropped: its cooperative pass starts again from the root. From
+// kNoHandoff on, no ray goes and the passes are not launched (small scenes,
+// whose rays are all short, save the launches).
+constexpr int kNoHandoff = 1 << 30;
 
 // Workgroups of a cooperative pass: as many one-wave workgroups as fit.
 int coop_grid(const void* kernel) {
@@ -955,7 +958,7 @@ int coop_grid(const void* kernel) {
 void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, const uint32_t* nrays_dev,
                const float* limit, const int32_t* src, const int32_t* dst, uint8_t* visible, uint32_t* counters,
                const TraceLaunch& L, AnySpill* spill, uint32_t spill_cap, LaunchSink& ls) {
-    const bool coop = spill && spill_cap > 0 && nrays < (int64_t(1) << 32);
+    const bool coop = spill && spill_cap > 0 && nrays < (int64_t(1) << 32) && L.handoff < kNoHandoff;
     auto* disp = reinterpret_cast<unsigned long long*>(counters + 4);
     launch_any_pairs(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
                      counters + 1, disp, fetch_word(counters, false), coop ? spill : (AnySpill*)nullptr,
@@ -980,7 +983,7 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
 void trace_near(const TRec* rec, int ntris, const float* root, const float* rays, int64_t nrays,
                 const uint32_t* nrays_dev, int32_t* prim, float* t, float* u, float* v, uint32_t* counters,
                 const TraceLaunch& L, const TraceCoop* coop, LaunchSink& ls) {
-    const bool on = coop && coop->longs && coop->long_cap > 0 && nrays < (int64_t(1) << 32);
+    const bool on = coop && coop->longs && coop->long_cap > 0 && nrays < (int64_t(1) << 32) && L.handoff < kNoHandoff;
     LongRays lr;
     if (on) {
         lr.ids = coop->longs;
@@ -1039,7 +1042,8 @@ void trace_closest_queue(const TRec* rec, int ntris, const float* root, const fl
                          const uint32_t* nrays_dev, int64_t capacity, int32_t* prim, float* t, float* u, float* v,
                          uint32_t* counters, const TraceLaunch& L, LaunchSink& ls, const TraceCoop* coop) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 2, 2);
-    if (coop && coop->longs) ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 26, 5);
+    if (coop && coop->longs && L.handoff < kNoHandoff)
+        ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 26, 5);
     trace_near(rec, ntris, root, rays, capacity, nrays_dev, prim, t, u, v, counters, L, coop, ls);
 }
 
@@ -1048,7 +1052,8 @@ void trace_occluded_queue(const TRec* rec, int ntris, const float* rays, const u
                           uint8_t* visible, uint32_t* counters, const TraceLaunch& L, LaunchSink& ls,
                           const TraceCoop* coop) {
     ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 4, 2);
-    if (coop && coop->spill) ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 20, 6);
+    if (coop && coop->spill && L.handoff < kNoHandoff)
+        ls.launch("k_clear_words", &k_clear_words, dim3(1), dim3(64), counters + 20, 6);
     trace_any(rec, ntris, rays, capacity, nrays_dev, limit, src, dst, visible, counters, L,
               coop ? coop->spill : nullptr, coop ? coop->spill_cap : 0u, ls);
 }

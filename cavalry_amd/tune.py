@@ -46,7 +46,8 @@ SIZES = (64, 128, 256)          # threads per workgroup (traversal and shading)
 WAVES = (4, 8, 12, 16, 20)      # traversal waves per CU tried (1..5 workgroups of 256)
 PER_CU = tuple(sorted({w * 64 // b for w in WAVES for b in SIZES}))
 CHUNKS = (128, 256, 512, 1024)
-HANDOFFS = (256, 1024, 4096)    # steps before a ray still in flight at the end goes to a whole wave
+HANDOFFS = (256, 1024, 4096, 1 << 30)  # steps before a ray still in flight at the end goes to a
+                                       # whole wave (1 << 30: never; the passes are not launched)
 LDS = (32, 48)
 ROUNDS = (0, 1, 2, 3, 4, 6, 8)
 

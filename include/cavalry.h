@@ -397,7 +397,8 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
  * per-sample shading kernels (path shading, the RL integrator's Q-table
  * sampling; 64, 128 or 256; 0: 256), and the steps a ray still in flight
  * once every ray has been handed out must have taken to be finished by a
- * whole wave instead of its lane (1..1<<30; 0: 1024). Each ray's node order is the
+ * whole wave instead of its lane (1..1<<30; 0: 1024; 1<<30: never, and the
+ * cooperative launches are skipped). Each ray's node order is the
  * reference's and each sample's shading is its own either way, so results do
  * not depend on it. STANDALONE applies to cav_trace_*, WAVEFRONT to
  * cav_render and to servers opened on the context afterwards. No
