@@ -360,10 +360,12 @@ int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* l)
         return fail(ctx, CAV_ERR_INVALID, "shade_workgroup_size must be 0, 64, 128 or 256");
     if (l->handoff_steps < 0 || l->handoff_steps > (1 << 30))
         return fail(ctx, CAV_ERR_INVALID, "handoff_steps must be in 0..2^30");
+    if (l->any_hit_step < 0 || l->any_hit_step > CAV_ANY_STEP_PAIR)
+        return fail(ctx, CAV_ERR_INVALID, "any_hit_step must be 0, CAV_ANY_STEP_NODE or CAV_ANY_STEP_PAIR");
     ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack,
                                      l->workgroup_size ? l->workgroup_size : 256,
                                      l->shade_workgroup_size ? l->shade_workgroup_size : 256,
-                                     l->handoff_steps ? l->handoff_steps : 1024};
+                                     l->handoff_steps ? l->handoff_steps : 1024, l->any_hit_step};
     return CAV_OK;
 }
 
@@ -372,7 +374,7 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
     if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
         return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
     const TraceLaunch& l = ctx->launch[which];
-    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block, l.handoff};
+    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block, l.handoff, l.any_step};
     return CAV_OK;
 }
 

@@ -9,6 +9,7 @@
 // Layout read: TRec[2N] (64-byte records in the sibling-pair layout: an
 // internal node's two child boxes and child refs, or a leaf's v0/e1/e2 +
 // original index + mesh id), rays float[R][6].
+#include <algorithm>
 #include <cfloat>
 
 #include "dev_common.h"
@@ -493,6 +494,172 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
     if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
 }
 
+// Any-hit traversal one node per step, in the reference's order
+// (BVH/BVH.h:109-189: both children tested at their parent, the far one
+// stacked, the near one next). Same arguments and results as
+// k_trace_any_pairs; it fetches one 64-byte record per dependent step
+// instead of two, which is the cheaper trade on trees that stay in the
+// caches (half the box tests per fetch and no second member to mask). A
+// long ray at the end is spilled in the pair format: its current node and
+// every stacked node as a one-member pair carrying its stack depth.
+__device__ __forceinline__ uint32_t node_item(int ref) {
+    const uint32_t slot = (uint32_t)(ref >= 0 ? ref : ~ref);
+    return ((slot >> 1) << 2) | (1u << (slot & 1u));
+}
+
+template <int kLdsDepth, int kB>
+__global__ __launch_bounds__(kB) void k_trace_any(const TRec* __restrict__ recs,
+                                                      const float* __restrict__ rays, int64_t nrays_arg,
+                                                      const uint32_t* __restrict__ nrays_dev,
+                                                      const uint32_t* __restrict__ ids,
+                                                      const float* __restrict__ limit,
+                                                      const int32_t* __restrict__ src,
+                                                      const int32_t* __restrict__ dst,
+                                                      uint8_t* __restrict__ ovis,
+                                                      uint32_t* __restrict__ overflow,
+                                                      unsigned long long* __restrict__ next_ray,
+                                                      unsigned long long* __restrict__ fetched,
+                                                      AnySpill* __restrict__ spill, uint32_t* spill_n,
+                                                      uint32_t spill_cap, int spill_after,
+                                                      int max_steps, int chunk) {
+    __shared__ int lstack[kLdsDepth][kB];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t nrays = nrays_dev ? (int64_t)*nrays_dev : nrays_arg;
+
+    int64_t ri = -1;  // ray held by this lane; -1 = needs one, -2 = pool drained
+    v3 o = mk3(0, 0, 0), d = o, inv = o;
+    float lim = 0.f;
+    int sm = -1, dm = -1, cur = 0, steps = 0;
+    bool use_lim = false, may_spill = false;
+    bool drained = false;  // every ray has been handed out
+    uint32_t ovf = 0;
+    int maxsp = 0;  // deepest stack this lane reached
+    uint64_t nfetch = 0;
+    volatile int deep[kStack - kLdsDepth];  // in scratch memory, not registers
+    int sp = 0;
+    int64_t chunk_base = 0;
+    int chunk_avail = 0;
+
+    while (true) {
+        // ---- hand new rays to idle lanes (wave-uniform control flow)
+        const uint64_t need = __ballot(ri == -1);
+        if (need) {
+            const int cnt = __popcll(need);
+            const int rank = __popcll(need & lt);
+            int64_t mine;
+            if (chunk_avail < cnt) {
+                unsigned long long nb = 0;
+                const int first = __ffsll((long long)need) - 1;
+                if (lane == first) nb = atomicAdd(next_ray, (unsigned long long)chunk);
+                nb = __shfl(nb, first, 64);
+                mine = rank < chunk_avail ? chunk_base + rank : (int64_t)nb + (rank - chunk_avail);
+                chunk_base = (int64_t)nb + (cnt - chunk_avail);
+                chunk_avail = chunk - (cnt - chunk_avail);
+            } else {
+                mine = chunk_base + rank;
+                chunk_base += cnt;
+                chunk_avail -= cnt;
+            }
+            if (ri == -1) {
+                if (mine >= nrays) {
+                    ri = -2;
+                } else {
+                    ri = ids ? (int64_t)ids[mine] : mine;
+                    load_ray(rays, ri, o, d);
+                    inv = div3(mk3(1.f, 1.f, 1.f), d);
+                    sp = 0;
+                    cur = 0;
+                    steps = 0;
+                    may_spill = spill_after > 0;
+                    lim = limit[ri];
+                    use_lim = !(lim == INFINITY);
+                    sm = src ? src[ri] : -1;
+                    dm = dst ? dst[ri] : -1;
+                }
+            }
+        }
+        if (!__ballot(ri >= 0)) break;  // every lane drained
+        if (ri < 0) continue;
+
+        if (may_spill && steps >= spill_after && !drained && ((steps - spill_after) & 255) == 0)
+            drained = (int64_t)__hip_atomic_load(next_ray, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nrays;
+        if (may_spill && drained && steps >= spill_after) {
+            // a long ray at the end: hand its pending nodes to the cooperative pass
+            const uint32_t slot = atomicAdd(spill_n, 1u);
+            if (slot < spill_cap) {
+                AnySpill* rec = spill + slot;
+                rec->ray = ri;
+                rec->n = sp + 1;
+                rec->item[0] = node_item(cur);
+                rec->byte[0] = (uint8_t)sp;
+                for (int i = 0; i < sp; ++i) {
+                    const int ref = i < kLdsDepth ? *(const lds_int*)(&lstack[i][threadIdx.x]) : deep[i - kLdsDepth];
+                    rec->item[i + 1] = node_item(ref);
+                    rec->byte[i + 1] = (uint8_t)i;
+                }
+                ri = -1;
+                continue;
+            }
+            may_spill = false;  // the pool is full: finish it here
+        }
+
+        // ---- one traversal step for this lane's ray
+        bool done = false, hit = false;
+        if (++steps > max_steps) {
+            ++ovf;
+            done = true;
+        } else {
+            // one 64-byte fetch for either kind of step
+            const bool at_node = cur >= 0;
+            ++nfetch;
+            const float4* rec = reinterpret_cast<const float4*>(recs + (at_node ? cur : ~cur));
+            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+            int4 refs = reinterpret_cast<const int4*>(rec)[3];
+            asm volatile("" : "+v"(q1.w), "+v"(q2.w), "+v"(refs.x), "+v"(refs.y));
+            if (at_node) {  // BVH/BVH.h:158-184
+                const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
+                const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
+                const bool hl = dl >= 0, hr = dr >= 0;
+                bool next = false;
+                if (hl && hr) {
+                    const bool lfar = dl > dr;
+                    const int far = lfar ? refs.x : refs.y;
+                    const int near = lfar ? refs.y : refs.x;
+                    if (sp < kStack) stack_push<kLdsDepth, kB>(lstack, deep, sp, far); else ++ovf;
+                    maxsp = max(maxsp, sp);
+                    if (sp < kStack) { cur = near; next = true; } else ++ovf;
+                } else if (hl || hr) {
+                    if (sp < kStack) { cur = hl ? refs.x : refs.y; next = true; } else ++ovf;
+                }
+                if (!next) {
+                    if (sp == 0) done = true;
+                    else cur = stack_pop<kLdsDepth, kB>(lstack, deep, sp);
+                }
+            } else {  // BVH/BVH.h:136-156
+                const TTri tr{q0, q1, q2};
+                const int mesh = __float_as_int(tr.b.w);
+                float t, u, v;
+                hit = mesh != sm && mesh != dm && tri_hit(tr, o, d, t, u, v) && (!use_lim || t < lim);
+                if (hit || sp == 0) done = true;
+                else cur = stack_pop<kLdsDepth, kB>(lstack, deep, sp);
+            }
+        }
+        if (done) {
+            ovis[ri] = hit ? 0 : 1;
+            ri = -1;
+        }
+    }
+    if (ovf) atomicAdd(overflow, ovf);
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        maxsp = max(maxsp, __shfl_xor(maxsp, dd, 64));
+        nfetch += __shfl_xor(nfetch, dd, 64);
+    }
+    if (lane == 0 && maxsp > 0) atomicMax(overflow + 6, (uint32_t)maxsp);
+    if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
+}
+
 // Phase 2: one wave per spilled ray. The wave keeps the ray's pending pairs
 // in LDS and takes up to 64 of them at a time, newest first, one per lane;
 // their child pairs go back on top. It stops at the first occluder or when
@@ -626,6 +793,7 @@ __global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, 
 // reference's 64-entry stack (the depth counted here assumes every pair of
 // intersected children is pushed, a bound on the reference's).
 constexpr int kCoopNodes = 2048;
+constexpr int kCoopWidth = 64;  // pending nodes a wave takes per step (at most 64)
 constexpr int kCoopCands = 64;
 constexpr int kCoopDeep = 56;
 constexpr float kTieMargin = 1.0f + 1.0f / 1024.0f;
@@ -671,7 +839,7 @@ __global__ __launch_bounds__(64) void k_closest_coop(const TRec* __restrict__ re
         int count = 1;
         bool failed = false;
         while (count > 0) {
-            const int k = count < 64 ? count : 64;
+            const int k = count < kCoopWidth ? count : kCoopWidth;
             const float reach = __uint_as_float(*(volatile uint32_t*)&sbest) * kTieMargin;
             bool act = lane < k;
             int ref = 0, dep = 0;
@@ -933,13 +1101,39 @@ void launch_any_pairs(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... a
     }
 }
 
-// L.handoff: steps (any-hit: sibling pairs) a ray must have taken, once the
+// The one-node any-hit kernel, likewise.
+template <typename... A>
+void launch_any_nodes(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
+    const char* nm = "k_trace_any";
+    const bool deep = L.lds_depth >= 48;
+    if (L.block == 64) {
+        if (deep) launch_persistent<48, 64>(nm, &k_trace_any<48, 64>, L, work, ls, args...);
+        else launch_persistent<32, 64>(nm, &k_trace_any<32, 64>, L, work, ls, args...);
+    } else if (L.block == 128) {
+        if (deep) launch_persistent<48, 128>(nm, &k_trace_any<48, 128>, L, work, ls, args...);
+        else launch_persistent<32, 128>(nm, &k_trace_any<32, 128>, L, work, ls, args...);
+    } else {
+        if (deep) launch_persistent<48, 256>(nm, &k_trace_any<48, 256>, L, work, ls, args...);
+        else launch_persistent<32, 256>(nm, &k_trace_any<32, 256>, L, work, ls, args...);
+    }
+}
+
+// The any-hit kernel L.any_step names.
+template <typename... A>
+void launch_any(const TraceLaunch& L, int64_t work, LaunchSink& ls, A... args) {
+    if (any_step_pairs(L)) launch_any_pairs(L, work, ls, args...);
+    else launch_any_nodes(L, work, ls, args...);
+}
+
+// L.handoff: steps (any-hit: nodes or sibling pairs) a ray must have taken, once the
 // rays have all been handed out, to go to a cooperative pass, so that the
 // short rays still in flight finish where they are. A closest-hit ray's
 // steps are dropped: its cooperative pass starts again from the root. From
 // kNoHandoff on, no ray goes and the passes are not launched (small scenes,
 // whose rays are all short, save the launches).
 constexpr int kNoHandoff = 1 << 30;
+// lanes of a phase-3 launch (the rays the cooperative passes send back)
+constexpr int64_t kPhase3Lanes = 16384;
 
 // Workgroups of a cooperative pass: as many one-wave workgroups as fit.
 int coop_grid(const void* kernel) {
@@ -960,7 +1154,7 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
                const TraceLaunch& L, AnySpill* spill, uint32_t spill_cap, LaunchSink& ls) {
     const bool coop = spill && spill_cap > 0 && nrays < (int64_t(1) << 32) && L.handoff < kNoHandoff;
     auto* disp = reinterpret_cast<unsigned long long*>(counters + 4);
-    launch_any_pairs(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
+    launch_any(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
                      counters + 1, disp, fetch_word(counters, false), coop ? spill : (AnySpill*)nullptr,
                      counters + 20, coop ? spill_cap : 0u, coop ? L.handoff : 0, 2 * ntris - 1);
     if (!coop) return;
@@ -969,7 +1163,10 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
     ls.launch("k_any_coop", &k_any_coop, dim3((unsigned)blocks), dim3(64), rec, rays, limit, src, dst, visible,
               counters + 1, fetch_word(counters, false), (const AnySpill*)spill, (const uint32_t*)(counters + 20),
               spill_cap, counters + 21, fail_ids, counters + 22);
-    launch_any_pairs(L, (int64_t)spill_cap, ls, rec, rays, (int64_t)spill_cap, (const uint32_t*)(counters + 22),
+    // phase 3 on a small grid: its rays are few (normally none), and a
+    // full persistent grid costs tens of microseconds to launch and drain
+    launch_any(L, std::min<int64_t>(spill_cap, kPhase3Lanes), ls, rec, rays, (int64_t)spill_cap,
+                     (const uint32_t*)(counters + 22),
                      (const uint32_t*)fail_ids, limit, src, dst, visible, counters + 1,
                      reinterpret_cast<unsigned long long*>(counters + 24), fetch_word(counters, false),
                      (AnySpill*)nullptr, counters + 20, 0u, 0, 2 * ntris - 1);
@@ -999,7 +1196,7 @@ void trace_near(const TRec* rec, int ntris, const float* root, const float* rays
     ls.launch("k_closest_coop", &k_closest_coop, dim3((unsigned)blocks), dim3(64), rec, rays, prim, t, u, v,
               fetch_word(counters, true), (const uint32_t*)coop->longs, (const uint32_t*)(counters + 26),
               coop->long_cap, counters + 27, fail_ids, counters + 30);
-    launch_trace(L, (int64_t)coop->long_cap, ls, rec, root, rays, (int64_t)coop->long_cap,
+    launch_trace(L, std::min<int64_t>(coop->long_cap, kPhase3Lanes), ls, rec, root, rays, (int64_t)coop->long_cap,
                  (const uint32_t*)(counters + 30), (const uint32_t*)fail_ids, prim, t, u, v, counters,
                  reinterpret_cast<unsigned long long*>(counters + 28), fetch_word(counters, true), LongRays{},
                  2 * ntris - 1);

@@ -16,8 +16,9 @@ rays in flight (less L2 thrash), small or coherent ones prefer more
 `tune_trace` times candidate shapes on the caller's own rays (closest-hit and
 any-hit, best of `reps`), by coordinate descent from the current shape over
 workgroup size (at the same waves per CU), workgroups per CU, chunk, LDS
-depth and the steps after which a ray still in flight at the end of a launch
-goes to a cooperative pass (a whole wave per ray), sets the fastest on the
+depth, the steps after which a ray still in flight at the end of a launch
+goes to a cooperative pass (a whole wave per ray) and whether an any-hit
+step fetches one node or a pair of siblings, sets the fastest on the
 context and records it in a JSON cache keyed by device, tree size and batch
 size. A later call with the same key reuses
 the cached shape without timing (`use_cache`). `tune_render` does the same
@@ -49,6 +50,7 @@ CHUNKS = (128, 256, 512, 1024)
 HANDOFFS = (256, 1024, 4096, 1 << 30)  # steps before a ray still in flight at the end goes to a
                                        # whole wave (1 << 30: never; the passes are not launched)
 LDS = (32, 48)
+ANY_STEPS = (1, 2)              # any-hit step: one node, or a pair of sibling nodes (0: one node)
 ROUNDS = (0, 1, 2, 3, 4, 6, 8)
 
 
@@ -61,15 +63,16 @@ def get_launch(ctx, which: int) -> dict:
     check(ctx.handle, lib().cav_trace_get_launch(ctx.handle, int(which), ctypes.byref(t)))
     return {"workgroups_per_cu": t.workgroups_per_cu, "ray_chunk": t.ray_chunk, "lds_stack": t.lds_stack,
             "workgroup_size": t.workgroup_size, "shade_workgroup_size": t.shade_workgroup_size,
-            "handoff_steps": t.handoff_steps}
+            "handoff_steps": t.handoff_steps, "any_hit_step": t.any_hit_step}
 
 
 def set_launch(ctx, which: int, shape: dict) -> None:
-    """Sets a shape; missing workgroup sizes mean 256 and a missing hand-off
-    1024 steps (shapes cached before they were tunable)."""
+    """Sets a shape; missing workgroup sizes mean 256, a missing hand-off
+    1024 steps and a missing any-hit step one node (shapes cached before
+    they were tunable)."""
     t = TraceLaunch(int(shape["workgroups_per_cu"]), int(shape["ray_chunk"]), int(shape["lds_stack"]),
                     int(shape.get("workgroup_size", 0) or 0), int(shape.get("shade_workgroup_size", 0) or 0),
-                    int(shape.get("handoff_steps", 0) or 0))
+                    int(shape.get("handoff_steps", 0) or 0), int(shape.get("any_hit_step", 0) or 0))
     check(ctx.handle, lib().cav_trace_set_launch(ctx.handle, int(which), ctypes.byref(t)))
 
 
@@ -109,7 +112,7 @@ def _store(path: str, key: str, entry: dict) -> None:
 def _key(shape: dict) -> str:
     return (f"{shape['workgroups_per_cu']}/{shape['ray_chunk']}/{shape['lds_stack']}/"
             f"{shape.get('workgroup_size', 256) or 256}/{shape.get('shade_workgroup_size', 256) or 256}/"
-            f"{shape.get('handoff_steps', 1024) or 1024}")
+            f"{shape.get('handoff_steps', 1024) or 1024}/{shape.get('any_hit_step', 0) or 0}")
 
 
 def _normal(shape: dict) -> dict:
@@ -117,6 +120,7 @@ def _normal(shape: dict) -> dict:
     s["workgroup_size"] = s.get("workgroup_size", 0) or 256
     s["shade_workgroup_size"] = s.get("shade_workgroup_size", 0) or 256
     s["handoff_steps"] = s.get("handoff_steps", 0) or 1024
+    s["any_hit_step"] = s.get("any_hit_step", 0) or 0
     return s
 
 
@@ -128,8 +132,8 @@ def _per_cu_for(size: int) -> tuple:
 
 def _descend(ctx, which: int, timer, start: dict, reps: int, shade: bool = False) -> tuple[dict, dict]:
     """Coordinate descent from `start` over the workgroup size (keeping the
-    waves per CU), workgroups per CU, chunk, LDS depth, hand-off steps and,
-    with `shade`, the shading workgroup size; returns the best shape and
+    waves per CU), workgroups per CU, chunk, LDS depth, hand-off steps, the
+    any-hit step and, with `shade`, the shading workgroup size; returns the best shape and
     every timing taken (ms)."""
     timings = {}
 
@@ -153,7 +157,8 @@ def _descend(ctx, which: int, timer, start: dict, reps: int, shade: bool = False
         ms = measure(cand)
         if ms < best_ms:
             best, best_ms = cand, ms
-    fields = [("workgroups_per_cu", None), ("ray_chunk", CHUNKS), ("lds_stack", LDS), ("handoff_steps", HANDOFFS)]
+    fields = [("workgroups_per_cu", None), ("ray_chunk", CHUNKS), ("lds_stack", LDS), ("handoff_steps", HANDOFFS),
+              ("any_hit_step", ANY_STEPS)]
     if shade:
         fields.append(("shade_workgroup_size", SIZES))
     for field, values in fields:

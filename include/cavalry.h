@@ -398,9 +398,12 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
  * sampling; 64, 128 or 256; 0: 256), and the steps a ray still in flight
  * once every ray has been handed out must have taken to be finished by a
  * whole wave instead of its lane (1..1<<30; 0: 1024; 1<<30: never, and the
- * cooperative launches are skipped). Each ray's node order is the
- * reference's and each sample's shading is its own either way, so results do
- * not depend on it. STANDALONE applies to cav_trace_*, WAVEFRONT to
+ * cooperative launches are skipped), and what one any-hit step fetches
+ * (0 or CAV_ANY_STEP_NODE: one node; CAV_ANY_STEP_PAIR: a pair of sibling
+ * nodes sharing a 128-byte line, both tested).
+ * Each ray's closest-hit node order is the reference's, the any-hit answer
+ * does not depend on the order the reachable leaves are tested in, and each
+ * sample's shading is its own, so results do not depend on the shape. STANDALONE applies to cav_trace_*, WAVEFRONT to
  * cav_render and to servers opened on the context afterwards. No
  * counterpart in the reference (its launch shapes are compile-time, 256
  * threads: Utils/GpuCommons.h:63); cavalry_amd/tune.py times the candidates
@@ -412,8 +415,10 @@ typedef struct {
     int32_t workgroup_size;
     int32_t shade_workgroup_size;
     int32_t handoff_steps;
+    int32_t any_hit_step;
 } cav_trace_launch;
 enum { CAV_TRACE_STANDALONE = 0, CAV_TRACE_WAVEFRONT = 1 };
+enum { CAV_ANY_STEP_NODE = 1, CAV_ANY_STEP_PAIR = 2 };
 int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* launch);
 int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out);
 

@@ -37,7 +37,11 @@ def test_closest_and_occluded(ctx, n, nrays, edge, rounds):
     np.testing.assert_array_equal(vgot, vref)
 
 
-def test_occluded_mesh_exclusion(ctx):
+@pytest.mark.parametrize("step", [0, 1, 2])
+def test_occluded_mesh_exclusion(ctx, step):
+    """Both any-hit kernels (one node or a sibling pair per step; 0: one
+    node) skip the source and target meshes as the reference does."""
+    from cavalry_amd import tune
     n = 5000
     tris = scenes.gmm_triangles(n, seed=9)
     mesh = (np.arange(n) % 13).astype(np.int32)
@@ -49,21 +53,27 @@ def test_occluded_mesh_exclusion(ctx):
     dst = rng.integers(-1, 13, 20000).astype(np.int32)
     vref = oracle.trace_occluded(nd, tris, rays, lim, mesh, src, dst)
     bvh = BVH.build(ctx, tris, tri_mesh=mesh)
-    vgot = bvh.test_visibility(rays, lim, src, dst)
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    try:
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, any_hit_step=step))
+        vgot = bvh.test_visibility(rays, lim, src, dst)
+    finally:
+        tune.set_launch(ctx, tune.STANDALONE, saved)
     np.testing.assert_array_equal(vgot, vref)
 
 
+@pytest.mark.parametrize("step", [1, 2])
 @pytest.mark.parametrize("n,nrays,edge,rounds", [(10000, 50000, 1e-3, 0), (200000, 100000, 0.01, 3),
                                                  (4000, 30000, 0.3, 1)])
-def test_every_ray_through_the_cooperative_passes(ctx, n, nrays, edge, rounds):
+def test_every_ray_through_the_cooperative_passes(ctx, n, nrays, edge, rounds, step):
     """With a hand-off after one step, the rays still in flight once the
     rays are all handed out -- here a large share: the batch is smaller than
     the persistent grid -- are traced again by a whole wave (closest-hit) or
-    finished by one (any-hit). Hits and visibility stay the oracle's,
+    finished by one (any-hit, spilled by either any-hit kernel). Hits and visibility stay the oracle's,
     bit for bit; the rays the closest-hit pass sends back to one lane (two
     hits within a hair, a deep stack) are traced in the reference's order."""
     from cavalry_amd import tune
-    print(f"[trace coop] n={n} rays={nrays} edge={edge} rounds={rounds}", flush=True)
+    print(f"[trace coop] n={n} rays={nrays} edge={edge} rounds={rounds} any-hit step={step}", flush=True)
     tris = scenes.gmm_triangles(n, seed=n + 11, edge=edge)
     _, _, nd = oracle.lbvh_build(tris, 30)
     if rounds:
@@ -77,7 +87,7 @@ def test_every_ray_through_the_cooperative_passes(ctx, n, nrays, edge, rounds):
     bvh = BVH.build(ctx, tris, optimization_rounds=rounds)
     saved = tune.get_launch(ctx, tune.STANDALONE)
     try:
-        tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=1))
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=1, any_hit_step=step))
         got = bvh.intersect(rays)
         f = bvh.trace_fetches()
         vgot = bvh.test_visibility(srays, lim)
