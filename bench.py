@@ -498,10 +498,12 @@ def main() -> int:
         step(False)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
 
-    # the records the reference's visiting order fetches: one untimed trace
-    # of each kind with no hand-off to the cooperative passes (the closest-hit
-    # pass restarts its rays and visits in another order; the any-hit pass
-    # visits the same pairs)
+    # the records each visiting order fetches, on untimed traces with no
+    # hand-off to the cooperative passes (the closest-hit pass restarts its
+    # rays and visits in another order; the any-hit pass visits the same
+    # pairs): closest-hit in the reference's order, any-hit with the limit
+    # cull (the default) and along the whole line (the reference's order).
+    # Their answers are kept to check the timed traces' at full size.
     from cavalry_amd import tune as _tune
     shape = _tune.get_launch(ctx, _tune.STANDALONE)
     _tune.set_launch(ctx, _tune.STANDALONE, dict(shape, handoff_steps=1 << 30))
@@ -512,6 +514,14 @@ def main() -> int:
         ctx.handle, bvh.handle, srays.data_ptr(), lim.data_ptr(), r, vis.data_ptr()))
     ctx.synchronize()
     ref_fetch = bvh.trace_fetches(reset=True)
+    ref_prim, ref_t = hit_prim.cpu().numpy(), hit_t.cpu().numpy()
+    cull_vis = vis.cpu().numpy()
+    _tune.set_launch(ctx, _tune.STANDALONE, dict(shape, handoff_steps=1 << 30, any_hit_whole_line=1))
+    check(ctx.handle, lib().cav_trace_occluded_async(
+        ctx.handle, bvh.handle, srays.data_ptr(), lim.data_ptr(), r, vis.data_ptr()))
+    ctx.synchronize()
+    whole_fetch = bvh.trace_fetches(reset=True)
+    ref_vis = vis.cpu().numpy()
     _tune.set_launch(ctx, _tune.STANDALONE, shape)
 
     torch.cuda.synchronize()
@@ -532,6 +542,14 @@ def main() -> int:
     fetches = bvh.trace_fetches()
     trav["max_leaf_depth"] = bvh.max_depth()
     trav["stack_entries"] = 64
+    # the timed traces' answers (cooperative passes, limit cull) against the
+    # untimed reference-order traces, every ray
+    trav["closest_mismatch_vs_reference_order"] = int(
+        (hit_prim.cpu().numpy() != ref_prim).sum() + (hit_t.cpu().numpy().view(np.uint32) != ref_t.view(np.uint32)).sum())
+    tvis = vis.cpu().numpy()
+    trav["occluded_mismatch_vs_reference_order"] = int((tvis != ref_vis).sum())
+    trav["occluded_mismatch_cull_untimed"] = int((cull_vis != ref_vis).sum())
+    del ref_prim, ref_t, ref_vis, cull_vis, tvis
     log(f"[bench] traversal counters {trav}")
 
     ms_step = wall * 1000.0 / args.steps
@@ -589,9 +607,10 @@ def main() -> int:
         "avg_launch_ms": round(trav_ms[tdom], 4), "alg_bytes_per_launch": int(trav_bytes[tdom]),
         "records_per_ray": round(ref_fetch[tdom] / r, 1),
         "records_fetched_per_ray": round(fetch[tdom] / r, 1),
-        "note": "algorithmic bytes count every record the reference's visiting order fetches "
-                "(records_per_ray, counted on an untimed trace without the cooperative passes), "
-                "whether L2 or HBM serves it; records_fetched_per_ray is what the timed traces fetched",
+        "note": "algorithmic bytes count every record the visiting order fetches (closest-hit: the "
+                "reference's; any-hit: the reference's up to the ray's limit), records_per_ray, counted "
+                "on an untimed trace without the cooperative passes, whether L2 or HBM serves it; "
+                "records_fetched_per_ray is what the timed traces fetched",
         "lbvh_build_frac": lbvh_roofline["build_frac"],
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
@@ -606,6 +625,7 @@ def main() -> int:
             lbvh_roofline["traffic_source"] = tr.get("_source", "profiles/traffic.json")
     trav["records_per_ray_closest"] = round(ref_fetch["closest"] / r, 1)
     trav["records_per_ray_occluded"] = round(ref_fetch["occluded"] / r, 1)
+    trav["records_per_ray_occluded_whole_line"] = round(whole_fetch["occluded"] / r, 1)
     trav["records_fetched_per_ray_closest"] = round(fetch["closest"] / r, 1)
     trav["records_fetched_per_ray_occluded"] = round(fetch["occluded"] / r, 1)
     trav["long_closest_rays"] = int(fetches.get("long_closest_rays", 0))

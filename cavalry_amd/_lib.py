@@ -76,7 +76,7 @@ class TraceLaunch(ctypes.Structure):
     _fields_ = [("workgroups_per_cu", ctypes.c_int32), ("ray_chunk", ctypes.c_int32),
                 ("lds_stack", ctypes.c_int32), ("workgroup_size", ctypes.c_int32),
                 ("shade_workgroup_size", ctypes.c_int32), ("handoff_steps", ctypes.c_int32),
-                ("any_hit_step", ctypes.c_int32)]
+                ("any_hit_step", ctypes.c_int32), ("any_hit_whole_line", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):

@@ -362,10 +362,13 @@ int cav_trace_set_launch(cav_ctx* ctx, int32_t which, const cav_trace_launch* l)
         return fail(ctx, CAV_ERR_INVALID, "handoff_steps must be in 0..2^30");
     if (l->any_hit_step < 0 || l->any_hit_step > CAV_ANY_STEP_PAIR)
         return fail(ctx, CAV_ERR_INVALID, "any_hit_step must be 0, CAV_ANY_STEP_NODE or CAV_ANY_STEP_PAIR");
+    if (l->any_hit_whole_line != 0 && l->any_hit_whole_line != 1)
+        return fail(ctx, CAV_ERR_INVALID, "any_hit_whole_line must be 0 or 1");
     ctx->launch[which] = TraceLaunch{l->workgroups_per_cu, l->ray_chunk, l->lds_stack,
                                      l->workgroup_size ? l->workgroup_size : 256,
                                      l->shade_workgroup_size ? l->shade_workgroup_size : 256,
-                                     l->handoff_steps ? l->handoff_steps : 1024, l->any_hit_step};
+                                     l->handoff_steps ? l->handoff_steps : 1024, l->any_hit_step,
+                                     l->any_hit_whole_line};
     return CAV_OK;
 }
 
@@ -374,7 +377,8 @@ int cav_trace_get_launch(cav_ctx* ctx, int32_t which, cav_trace_launch* out) {
     if (which != CAV_TRACE_STANDALONE && which != CAV_TRACE_WAVEFRONT)
         return fail(ctx, CAV_ERR_INVALID, "which must be CAV_TRACE_STANDALONE or CAV_TRACE_WAVEFRONT");
     const TraceLaunch& l = ctx->launch[which];
-    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block, l.handoff, l.any_step};
+    *out = cav_trace_launch{l.per_cu, l.chunk, l.lds_depth, l.block, l.shade_block, l.handoff, l.any_step,
+                            l.any_full};
     return CAV_OK;
 }
 

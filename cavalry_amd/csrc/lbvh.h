@@ -112,6 +112,7 @@ struct TraceLaunch {
     int shade_block = 256;  // threads per shading workgroup (64, 128 or 256)
     int handoff = 1024;     // steps before a ray still in flight at the end goes to a cooperative pass
     int any_step = 0;       // any-hit step: 0 or 1 one node, 2 a pair of sibling nodes
+    int any_full = 0;       // 1: any-hit visits boxes beyond the ray's limit too, as the reference does
 };
 // Whether an any-hit step fetches a sibling pair (one 128-byte line, both
 // nodes tested) instead of one node. One node per step measured as fast or

@@ -2,9 +2,11 @@
 //
 // Reference: BVH::intersect and BVH::testVisibility (BVH/BVH.h:23-189), the
 // slab test of Core/BoundingBox.h:55-74 and Moller-Trumbore of
-// Core/Triangle.h:7-52. The visiting order is the reference's exactly (both
-// children tested at their parent, the far child stacked, the near one taken
-// next, left first on ties), so hit ids and distances match bit for bit.
+// Core/Triangle.h:7-52. The closest-hit visiting order is the reference's
+// exactly (both children tested at their parent, the far child stacked, the
+// near one taken next, left first on ties), so hit ids and distances match
+// bit for bit. Any-hit visits the reference's nodes up to the ray's limit
+// (the limit cull below); its answer does not depend on the order.
 //
 // Layout read: TRec[2N] (64-byte records in the sibling-pair layout: an
 // internal node's two child boxes and child refs, or a leaf's v0/e1/e2 +
@@ -258,6 +260,24 @@ __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
     if (lane == 0 && nfetch > 0) atomicAdd(fetched, (unsigned long long)nfetch);
 }
 
+// Any-hit limit cull. BVH::testVisibility (BVH/BVH.h:158-184) pushes every
+// child box the ray's line crosses, also those that start beyond the ray's
+// limit, whose triangles can never block it (t < limit). Here a child box
+// that starts more than kLimitSlack times the limit along the ray is not
+// visited. That changes no answer: a box's computed entry distance is
+// within a few ulps of the exact one, which is at most the exact distance
+// of any point of a triangle inside it, so a skipped triangle could still
+// block only if Moller-Trumbore put its t more than 2^-10 (relative) short
+// of the exact one, an error of order 1e-7 / (sin(smallest angle) *
+// |cos(incidence)|): a ray within about 1e-4 rad of the plane of a sliver
+// triangle that lies just past the limit. The visited set shrinks, so the
+// reference's stack-overflow count can only fall. TraceLaunch::any_full
+// restores the reference's whole-line visiting order.
+constexpr float kLimitSlack = 1.0f + 1.0f / 1024.0f;
+__device__ __forceinline__ float any_cut(int cull, bool use_lim, float lim) {
+    return cull && use_lim ? lim * kLimitSlack : INFINITY;
+}
+
 // Any-hit traversal over sibling pairs.
 //
 // BVH::testVisibility (BVH/BVH.h:109-189) answers whether any triangle of a
@@ -271,7 +291,9 @@ __global__ __launch_bounds__(kB) void k_trace(const TRec* __restrict__ recs,
 // siblings pending below it; a far child has its parent's s, a near child
 // s + 1, and a near child that would need s = 64 is dropped and counted, as
 // the reference's failed push is), so the same nodes are visited and the
-// same overflows counted (all of them when no occluder ends the ray first).
+// same overflows counted (all of them when no occluder ends the ray first;
+// with the limit cull, which drops boxes beyond the ray's limit, the subset
+// of them on the visited nodes).
 //
 // A pending pair is (base slot >> 1) << 2 | active mask (bit 0: the left
 // sibling at the base slot, bit 1: the right one) and one byte: the s of its
@@ -284,7 +306,7 @@ struct PairOut {
 };
 
 __device__ __forceinline__ void pair_step(const float4 (&r)[8], uint32_t item, uint32_t byte, v3 o, v3 d,
-                                          v3 inv, int sm, int dm, bool use_lim, float lim, bool& hit,
+                                          v3 inv, int sm, int dm, bool use_lim, float lim, float cut, bool& hit,
                                           uint32_t& ovf, int& maxsp, PairOut& out) {
     const uint32_t mask = item & 3u;
     const int sb = (int)(byte & 63u);
@@ -304,7 +326,7 @@ __device__ __forceinline__ void pair_step(const float4 (&r)[8], uint32_t item, u
         } else {
             const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
             const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
-            const bool hl = dl >= 0, hr = dr >= 0;
+            const bool hl = dl >= 0 && dl <= cut, hr = dr >= 0 && dr <= cut;
             if (hl || hr) {
                 const int cb = refl >= 0 ? refl : ~refl;  // the children's slots: cb, cb + 1
                 uint32_t cm = (hl ? 1u : 0u) | (hr ? 2u : 0u);
@@ -344,7 +366,7 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
                                                             unsigned long long* __restrict__ next_ray,
                                                             unsigned long long* __restrict__ fetched,
                                                             AnySpill* __restrict__ spill, uint32_t* spill_n,
-                                                            uint32_t spill_cap, int spill_after,
+                                                            uint32_t spill_cap, int spill_after, int cull,
                                                             int max_steps, int chunk) {
     __shared__ uint32_t litem[kLdsDepth][kB];
     __shared__ uint8_t lsb[kLdsDepth][kB];
@@ -354,7 +376,7 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
 
     int64_t ri = -1;  // ray held by this lane; -1 = needs one, -2 = pool drained
     v3 o = mk3(0, 0, 0), d = o, inv = o;
-    float lim = 0.f;
+    float lim = 0.f, cut = INFINITY;
     int sm = -1, dm = -1, steps = 0;
     bool hit = false, use_lim = false, may_spill = false;
     bool drained = false;  // every ray has been handed out
@@ -404,6 +426,7 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
                     may_spill = spill_after > 0;
                     lim = limit[ri];
                     use_lim = !(lim == INFINITY);
+                    cut = any_cut(cull, use_lim, lim);
                     sm = src ? src[ri] : -1;
                     dm = dst ? dst[ri] : -1;
                 }
@@ -446,7 +469,7 @@ __global__ __launch_bounds__(kB) void k_trace_any_pairs(const TRec* __restrict__
             for (int k = 0; k < 8; ++k) r[k] = rec[k];
             nfetch += (uint64_t)__popc(citem & 3u);
             PairOut po;
-            pair_step(r, citem, cbyte, o, d, inv, sm, dm, use_lim, lim, hit, ovf, maxsp, po);
+            pair_step(r, citem, cbyte, o, d, inv, sm, dm, use_lim, lim, cut, hit, ovf, maxsp, po);
             if (hit) {
                 done = true;
             } else if (po.n > 0) {
@@ -520,7 +543,7 @@ __global__ __launch_bounds__(kB) void k_trace_any(const TRec* __restrict__ recs,
                                                       unsigned long long* __restrict__ next_ray,
                                                       unsigned long long* __restrict__ fetched,
                                                       AnySpill* __restrict__ spill, uint32_t* spill_n,
-                                                      uint32_t spill_cap, int spill_after,
+                                                      uint32_t spill_cap, int spill_after, int cull,
                                                       int max_steps, int chunk) {
     __shared__ int lstack[kLdsDepth][kB];
     const int lane = threadIdx.x & 63;
@@ -529,7 +552,7 @@ __global__ __launch_bounds__(kB) void k_trace_any(const TRec* __restrict__ recs,
 
     int64_t ri = -1;  // ray held by this lane; -1 = needs one, -2 = pool drained
     v3 o = mk3(0, 0, 0), d = o, inv = o;
-    float lim = 0.f;
+    float lim = 0.f, cut = INFINITY;
     int sm = -1, dm = -1, cur = 0, steps = 0;
     bool use_lim = false, may_spill = false;
     bool drained = false;  // every ray has been handed out
@@ -574,6 +597,7 @@ __global__ __launch_bounds__(kB) void k_trace_any(const TRec* __restrict__ recs,
                     may_spill = spill_after > 0;
                     lim = limit[ri];
                     use_lim = !(lim == INFINITY);
+                    cut = any_cut(cull, use_lim, lim);
                     sm = src ? src[ri] : -1;
                     dm = dst ? dst[ri] : -1;
                 }
@@ -620,7 +644,7 @@ __global__ __launch_bounds__(kB) void k_trace_any(const TRec* __restrict__ recs,
             if (at_node) {  // BVH/BVH.h:158-184
                 const float dl = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, o, inv);
                 const float dr = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, o, inv);
-                const bool hl = dl >= 0, hr = dr >= 0;
+                const bool hl = dl >= 0 && dl <= cut, hr = dr >= 0 && dr <= cut;
                 bool next = false;
                 if (hl && hr) {
                     const bool lfar = dl > dr;
@@ -676,7 +700,7 @@ __global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, 
                                                  unsigned long long* __restrict__ fetched,
                                                  const AnySpill* __restrict__ spill, const uint32_t* spill_n,
                                                  uint32_t spill_cap, uint32_t* next_spill,
-                                                 uint32_t* __restrict__ fail_ids, uint32_t* fail_n) {
+                                                 uint32_t* __restrict__ fail_ids, uint32_t* fail_n, int cull) {
     __shared__ uint32_t witem[kCoopCap];
     __shared__ uint8_t wbyte[kCoopCap];
     const int lane = threadIdx.x;
@@ -698,6 +722,7 @@ __global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, 
         const v3 inv = div3(mk3(1.f, 1.f, 1.f), d);
         const float lim = limit[ri];
         const bool use_lim = !(lim == INFINITY);
+        const float cut = any_cut(cull, use_lim, lim);
         const int sm = src ? src[ri] : -1, dm = dst ? dst[ri] : -1;
         for (int i = lane; i < n0; i += 64) {
             witem[i] = sr->item[i];
@@ -725,7 +750,7 @@ __global__ __launch_bounds__(64) void k_any_coop(const TRec* __restrict__ recs, 
 #pragma unroll
                 for (int q = 0; q < 8; ++q) r[q] = rec[q];
                 nfetch += (uint64_t)__popc(item & 3u);
-                pair_step(r, item, byte, o, d, inv, sm, dm, use_lim, lim, hit, ovf, maxsp, po);
+                pair_step(r, item, byte, o, d, inv, sm, dm, use_lim, lim, cut, hit, ovf, maxsp, po);
             }
             if (__ballot(hit)) {
                 occluded = true;
@@ -1156,20 +1181,21 @@ void trace_any(const TRec* rec, int ntris, const float* rays, int64_t nrays, con
     auto* disp = reinterpret_cast<unsigned long long*>(counters + 4);
     launch_any(L, nrays, ls, rec, rays, nrays, nrays_dev, (const uint32_t*)nullptr, limit, src, dst, visible,
                      counters + 1, disp, fetch_word(counters, false), coop ? spill : (AnySpill*)nullptr,
-                     counters + 20, coop ? spill_cap : 0u, coop ? L.handoff : 0, 2 * ntris - 1);
+                     counters + 20, coop ? spill_cap : 0u, coop ? L.handoff : 0, L.any_full ? 0 : 1,
+                     2 * ntris - 1);
     if (!coop) return;
     uint32_t* fail_ids = reinterpret_cast<uint32_t*>(spill + spill_cap);
     static const int blocks = coop_grid(reinterpret_cast<const void*>(&k_any_coop));
     ls.launch("k_any_coop", &k_any_coop, dim3((unsigned)blocks), dim3(64), rec, rays, limit, src, dst, visible,
               counters + 1, fetch_word(counters, false), (const AnySpill*)spill, (const uint32_t*)(counters + 20),
-              spill_cap, counters + 21, fail_ids, counters + 22);
+              spill_cap, counters + 21, fail_ids, counters + 22, L.any_full ? 0 : 1);
     // phase 3 on a small grid: its rays are few (normally none), and a
     // full persistent grid costs tens of microseconds to launch and drain
     launch_any(L, std::min<int64_t>(spill_cap, kPhase3Lanes), ls, rec, rays, (int64_t)spill_cap,
                      (const uint32_t*)(counters + 22),
                      (const uint32_t*)fail_ids, limit, src, dst, visible, counters + 1,
                      reinterpret_cast<unsigned long long*>(counters + 24), fetch_word(counters, false),
-                     (AnySpill*)nullptr, counters + 20, 0u, 0, 2 * ntris - 1);
+                     (AnySpill*)nullptr, counters + 20, 0u, 0, L.any_full ? 0 : 1, 2 * ntris - 1);
 }
 
 // Closest-hit in three launches, like any-hit: phase 1 (one lane per ray,

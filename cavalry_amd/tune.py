@@ -63,16 +63,20 @@ def get_launch(ctx, which: int) -> dict:
     check(ctx.handle, lib().cav_trace_get_launch(ctx.handle, int(which), ctypes.byref(t)))
     return {"workgroups_per_cu": t.workgroups_per_cu, "ray_chunk": t.ray_chunk, "lds_stack": t.lds_stack,
             "workgroup_size": t.workgroup_size, "shade_workgroup_size": t.shade_workgroup_size,
-            "handoff_steps": t.handoff_steps, "any_hit_step": t.any_hit_step}
+            "handoff_steps": t.handoff_steps, "any_hit_step": t.any_hit_step,
+            "any_hit_whole_line": t.any_hit_whole_line}
 
 
 def set_launch(ctx, which: int, shape: dict) -> None:
     """Sets a shape; missing workgroup sizes mean 256, a missing hand-off
-    1024 steps and a missing any-hit step one node (shapes cached before
-    they were tunable)."""
+    1024 steps, a missing any-hit step one node (shapes cached before
+    they were tunable) and a missing any_hit_whole_line 0 (any-hit rays
+    skip boxes that start past their limit; 1 visits them as the reference
+    does, same answers)."""
     t = TraceLaunch(int(shape["workgroups_per_cu"]), int(shape["ray_chunk"]), int(shape["lds_stack"]),
                     int(shape.get("workgroup_size", 0) or 0), int(shape.get("shade_workgroup_size", 0) or 0),
-                    int(shape.get("handoff_steps", 0) or 0), int(shape.get("any_hit_step", 0) or 0))
+                    int(shape.get("handoff_steps", 0) or 0), int(shape.get("any_hit_step", 0) or 0),
+                    int(shape.get("any_hit_whole_line", 0) or 0))
     check(ctx.handle, lib().cav_trace_set_launch(ctx.handle, int(which), ctypes.byref(t)))
 
 
@@ -121,6 +125,7 @@ def _normal(shape: dict) -> dict:
     s["shade_workgroup_size"] = s.get("shade_workgroup_size", 0) or 256
     s["handoff_steps"] = s.get("handoff_steps", 0) or 1024
     s["any_hit_step"] = s.get("any_hit_step", 0) or 0
+    s["any_hit_whole_line"] = s.get("any_hit_whole_line", 0) or 0
     return s
 
 

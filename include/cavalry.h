@@ -400,7 +400,12 @@ int cav_scene_create_from_pbrt(cav_ctx* ctx, const char* path, const cav_pbrt_ov
  * whole wave instead of its lane (1..1<<30; 0: 1024; 1<<30: never, and the
  * cooperative launches are skipped), and what one any-hit step fetches
  * (0 or CAV_ANY_STEP_NODE: one node; CAV_ANY_STEP_PAIR: a pair of sibling
- * nodes sharing a 128-byte line, both tested).
+ * nodes sharing a 128-byte line, both tested), and whether any-hit rays
+ * visit boxes that start beyond their limit (1: every box the ray's line
+ * crosses, the reference's visiting order; 0, the default: a box starting
+ * more than 2^-10 (relative) beyond the limit is skipped, since no triangle
+ * in it can block the ray unless Moller-Trumbore's t is off by more than
+ * that, which takes a ray within about 1e-4 rad of a sliver's plane).
  * Each ray's closest-hit node order is the reference's, the any-hit answer
  * does not depend on the order the reachable leaves are tested in, and each
  * sample's shading is its own, so results do not depend on the shape. STANDALONE applies to cav_trace_*, WAVEFRONT to
@@ -416,6 +421,7 @@ typedef struct {
     int32_t shade_workgroup_size;
     int32_t handoff_steps;
     int32_t any_hit_step;
+    int32_t any_hit_whole_line;
 } cav_trace_launch;
 enum { CAV_TRACE_STANDALONE = 0, CAV_TRACE_WAVEFRONT = 1 };
 enum { CAV_ANY_STEP_NODE = 1, CAV_ANY_STEP_PAIR = 2 };

@@ -133,3 +133,41 @@ def test_cooperative_passes_match_one_lane_per_ray(ctx):
     for k in ("t", "u", "v"):
         np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=k)
     np.testing.assert_array_equal(va, vb)
+
+
+@pytest.mark.parametrize("step", [1, 2])
+@pytest.mark.parametrize("handoff", [1024, 1])
+def test_limit_cull_keeps_visibility(ctx, step, handoff):
+    """Any-hit rays skip boxes that start past their limit (the default) or
+    visit every box their line crosses, as the reference does: the same
+    visibility as the oracle's either way, on limits from negative and zero
+    to infinite, with fewer records fetched when the limits are finite."""
+    from cavalry_amd import tune
+    print(f"[trace cull] any-hit step={step} handoff={handoff}", flush=True)
+    n, nrays = 200000, 131072
+    tris = scenes.gmm_triangles(n, seed=31, edge=2e-3)
+    _, _, nd = oracle.lbvh_build(tris, 30)
+    oracle.treelet_optimize(nd, 1)
+    srays = scenes.random_rays(nrays, seed=32, stream=2)
+    lim = scenes.shadow_limits(nrays, seed=33)
+    lim[::11] = np.inf
+    lim[1::97] = 0.0
+    lim[2::97] = -1.0
+    lim[3::97] = 1e-6
+    lim[4::97] = 1e30
+    vref = oracle.trace_occluded(nd, tris, srays, lim)
+    bvh = BVH.build(ctx, tris, optimization_rounds=1)
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    out = {}
+    try:
+        for whole in (0, 1):
+            tune.set_launch(ctx, tune.STANDALONE, dict(saved, handoff_steps=handoff, any_hit_step=step,
+                                                       any_hit_whole_line=whole))
+            bvh.trace_fetches(reset=True)
+            out[whole] = (bvh.test_visibility(srays, lim), bvh.trace_fetches(reset=True)["occluded"])
+    finally:
+        tune.set_launch(ctx, tune.STANDALONE, saved)
+    print(f"[trace cull] records culled {out[0][1]} whole line {out[1][1]}", flush=True)
+    np.testing.assert_array_equal(out[1][0], vref)
+    np.testing.assert_array_equal(out[0][0], vref)
+    assert out[0][1] < out[1][1]

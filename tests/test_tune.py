@@ -19,7 +19,8 @@ def test_descent_finds_separable_minimum(monkeypatch):
     best, timings = tune._descend(None, tune.STANDALONE, lambda: cost(cur),
                                   {"workgroups_per_cu": 5, "ray_chunk": 256, "lds_stack": 32}, reps=2)
     assert best == {"workgroups_per_cu": 3, "ray_chunk": 512, "lds_stack": 48, "workgroup_size": 256,
-                    "shade_workgroup_size": 256, "handoff_steps": 1024, "any_hit_step": 0}
+                    "shade_workgroup_size": 256, "handoff_steps": 1024, "any_hit_step": 0,
+                    "any_hit_whole_line": 0}
     assert tried[-1] == best  # the winner is left set
     for s in tried:
         assert s["workgroups_per_cu"] in tune.PER_CU and s["ray_chunk"] in tune.CHUNKS and s["lds_stack"] in tune.LDS
