@@ -166,7 +166,8 @@ def render_bench(renders: int, cpu: bool, tune_launch: bool = True) -> dict:
     return out
 
 
-def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = True) -> dict:
+def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = True,
+                  deterministic_rl: bool = True) -> dict:
     """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
     triangles, 1024^2 Kd textures on about a third of the meshes): for each
     arrival rate, one seeded Poisson trace of render requests (resolution
@@ -176,7 +177,9 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
     arrival order) and through the server (continuous batching; rlpath
     requests pooled one round at a time; bounce steps replayed from graphs).
     The plain renderer renders each rlpath request with the server's round
-    size, so both make the same film. Latency = arrival -> film ready.
+    size and both sum rlpath's table updates in sample order (deterministic
+    mode), so both make the same film, bit for bit. Latency = arrival ->
+    film ready.
     Throughput = traced rays / (first arrival -> last film)."""
     import numpy as np
     from cavalry_amd import Context, render as R, scenes
@@ -200,7 +203,7 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
         rng = np.random.default_rng(seed)
         trace = poisson_trace(rng, n_requests, rate, res_scenes,
                               [(256, 256), (512, 512), (1280, 720), (1920, 1080)], [1, 4, 16, 64],
-                              ["path", "rlpath"], 8, camera)
+                              ["path", "rlpath"], 8, camera, deterministic=deterministic_rl)
         srv = Server(ctx, **cfg)
         rl_round = (lambda r, srv=srv: srv.rl_round_spp(r.camera.width * r.camera.height, r.spp))
         log(f"[bench] serving at {rate}/s: plain per-request renderer, {n_requests} requests")
@@ -210,22 +213,26 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
         st = srv.close()
         served.rays += st["pool_closest_rays"] + st["pool_shadow_rays"]
         same = True
+        same_rl = True
         worst_rl = 0.0
         for r, a, b in zip(trace, plain.films, served.films):
+            ident = np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
             if r.integrator == "rlpath":
-                # Q proposals land through float atomics, so an rlpath film is
-                # not bit-reproducible; compared in the display range so a
-                # single firefly does not dominate
+                # with float-atomic table updates an rlpath film is not
+                # bit-reproducible (the deterministic mode's is); compared in
+                # the display range so a single firefly does not dominate
+                same_rl = same_rl and ident
                 ia = np.clip(a.image.astype(np.float64), 0, 1)
                 ib = np.clip(b.image.astype(np.float64), 0, 1)
                 worst_rl = max(worst_rl, float(np.mean((ib - ia) ** 2 / (ia * ia + 1e-2))))
             else:
-                same = same and np.array_equal(a.rgb, b.rgb) and np.array_equal(a.weight, b.weight)
+                same = same and ident
         p, s_ = plain.summary(), served.summary()
         per_rate.append({
             "rate_per_s": rate, "plain": p, "served": s_,
             "speedup_mrays": round(s_["mrays_s"] / p["mrays_s"], 3),
             "path_films_identical": bool(same),
+            "rlpath_films_identical": bool(same_rl),
             "rlpath_max_rel_mse_vs_plain_ldr": worst_rl,
             "server_steps": st["steps"], "graph_pools": st["graph_pools"],
         })
@@ -237,11 +244,13 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
     return {
         "workload": f"{n_requests} requests per rate, Poisson arrivals, 4 resident interior scenes "
                     f"({', '.join(str(len(a.tris)) for a in arrays)} tris, 4 1024^2 textures each), "
-                    "res 256^2..1920x1080, spp 1/4/16/64, path/rlpath 50/50, max depth 8",
+                    "res 256^2..1920x1080, spp 1/4/16/64, path/rlpath 50/50, max depth 8"
+                    + (", rlpath deterministic" if deterministic_rl else ""),
         "rates": per_rate,
         "speedup_mrays_min": min(r["speedup_mrays"] for r in per_rate),
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
         "use_graphs": use_graphs,
+        "rlpath_deterministic": deterministic_rl,
     }
 
 
@@ -402,6 +411,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24, help="requests per arrival rate")
     ap.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--serve-no-graphs", action="store_true")
+    ap.add_argument("--serve-rl-atomic", action="store_true",
+                    help="rlpath requests with float-atomic table updates (films not bit-reproducible)")
     ap.add_argument("--no-equal-time", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--c2-tris", default="2000000,8000000", help="triangle counts of the at-scale interiors")
@@ -410,6 +421,8 @@ def main() -> int:
     ap.add_argument("--et-tris", type=int, default=2_000_000)
     ap.add_argument("--et-res", default="320x180", help="WxH of the equal-time comparison")
     ap.add_argument("--et-ref-spp", type=int, default=16384)
+    ap.add_argument("--launch", default="",
+                    help="JSON launch shape of the standalone traces instead of autotuning one")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default traversal launch shapes instead of autotuning them")
     ap.add_argument("--tune-rays", type=int, default=1 << 22,
@@ -453,7 +466,14 @@ def main() -> int:
     log("[bench] first build (allocates)")
     bvh = BVH.build(ctx, tris, optimization_rounds=args.rounds, morton_bits=30, device=True)
     launch = None
-    if not args.probe and not args.no_tune:
+    if args.launch:
+        # a fixed launch shape (as the autotuner reports it), e.g. to profile
+        # the shape a timed run chose
+        from cavalry_amd import tune
+        tune.set_launch(ctx, tune.STANDALONE, json.loads(args.launch))
+        launch = {"shape": tune.get_launch(ctx, tune.STANDALONE), "fixed": True}
+        log(f"[bench] launch shape fixed to {launch['shape']}")
+    elif not args.probe and not args.no_tune:
         # the traversal launch shape, autotuned on a prefix of the same rays
         # (before the warmup; not part of the timed steps)
         from cavalry_amd import tune
@@ -691,7 +711,8 @@ def main() -> int:
     if not args.no_serve:
         log("[bench] serving")
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        out["serving"] = serving_bench(args.serve_requests, rates, args.seed, not args.serve_no_graphs)
+        out["serving"] = serving_bench(args.serve_requests, rates, args.seed, not args.serve_no_graphs,
+                                       not args.serve_rl_atomic)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)

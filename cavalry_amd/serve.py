@@ -112,6 +112,7 @@ class Req:
     integrator: str
     max_depth: int
     arrival: float    # seconds from the start of the trace
+    deterministic: bool = False  # rlpath: table updates summed in sample order (reproducible films)
 
 
 @dataclass
@@ -130,10 +131,11 @@ class TraceResult:
 
 
 def poisson_trace(rng: np.random.Generator, n: int, rate: float, scenes: list, resolutions: list, spps: list,
-                  integrators: list, max_depth: int, camera_fn) -> list:
+                  integrators: list, max_depth: int, camera_fn, deterministic: bool = False) -> list:
     """n requests with exponential inter-arrival times of mean 1/rate; each
     picks a scene, resolution, spp and integrator uniformly, and a camera
-    from camera_fn(rng, scene_index, width, height)."""
+    from camera_fn(rng, scene_index, width, height). deterministic applies
+    to the rlpath requests."""
     t = 0.0
     out = []
     for _ in range(n):
@@ -141,7 +143,7 @@ def poisson_trace(rng: np.random.Generator, n: int, rate: float, scenes: list, r
         s = int(rng.integers(len(scenes)))
         w, h = resolutions[int(rng.integers(len(resolutions)))]
         out.append(Req(s, camera_fn(rng, s, w, h), int(spps[int(rng.integers(len(spps)))]),
-                       integrators[int(rng.integers(len(integrators)))], max_depth, t))
+                       integrators[int(rng.integers(len(integrators)))], max_depth, t, deterministic))
     return out
 
 
@@ -159,7 +161,7 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
             if delay > 0:
                 time.sleep(delay)
             tickets[k] = server.submit(scenes[r.scene], r.spp, r.integrator, camera=r.camera,
-                                       max_depth=r.max_depth)
+                                       max_depth=r.max_depth, deterministic=r.deterministic)
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -197,7 +199,7 @@ def run_plain(ctx, scenes: list, trace: list, rl_round=None) -> TraceResult:
         arrived = start + r.arrival
         spr = rl_round(r) if (rl_round is not None and r.integrator == "rlpath") else 0
         film, st = render(ctx, scenes[r.scene], r.spp, integrator=r.integrator, camera=r.camera,
-                          max_depth=r.max_depth, spp_per_round=spr)
+                          max_depth=r.max_depth, spp_per_round=spr, deterministic=r.deterministic)
         done = time.perf_counter()
         first = arrived if first is None else first
         res.latencies_ms.append((done - arrived) * 1e3)
