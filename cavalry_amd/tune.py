@@ -217,10 +217,10 @@ def tune_trace(ctx, bvh, rays, nrays: int, shadow_rays=None, limit=None, reps: i
 
 
 def tune_render(ctx, scene, spp: int, integrator: str = "path", camera=None, max_depth=None, reps: int = 2,
-                cache_path: str | None = None, use_cache: bool = True) -> dict:
+                cache_path: str | None = None, use_cache: bool = True, **render_kw) -> dict:
     """Tunes the WAVEFRONT launch shape, shading workgroup size included, on
-    one render (device time of cav_render). Returns {"shape", "key", "ms",
-    "timings", "cached"}."""
+    one render (device time of cav_render; render_kw go to render.render,
+    e.g. q_grid). Returns {"shape", "key", "ms", "timings", "cached"}."""
     from .render import render
 
     path = cache_path or default_cache_path()
@@ -239,7 +239,7 @@ def tune_render(ctx, scene, spp: int, integrator: str = "path", camera=None, max
                     "cached": True}
 
     def timer():
-        _, st = render(ctx, scene, spp, integrator=integrator, camera=cam, max_depth=max_depth)
+        _, st = render(ctx, scene, spp, integrator=integrator, camera=cam, max_depth=max_depth, **render_kw)
         return float(st["ms_total"])
 
     best, timings = _descend(ctx, WAVEFRONT, timer, get_launch(ctx, WAVEFRONT), reps, shade=True)
