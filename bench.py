@@ -167,7 +167,7 @@ def render_bench(renders: int, cpu: bool, tune_launch: bool = True) -> dict:
 
 
 def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = True,
-                  deterministic_rl: bool = True, pool_tasks: int = 8 << 20) -> dict:
+                  deterministic_rl: bool = True, pool_tasks: int = 32 << 20) -> dict:
     """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
     triangles, 1024^2 Kd textures on about a third of the meshes): for each
     arrival rate, one seeded Poisson trace of render requests (resolution
@@ -421,7 +421,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24, help="requests per arrival rate")
     ap.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--serve-no-graphs", action="store_true")
-    ap.add_argument("--serve-pool-tasks", type=int, default=8 << 20, help="task slots of each scene's ray pool")
+    ap.add_argument("--serve-pool-tasks", type=int, default=32 << 20, help="task slots of each scene's ray pool")
     ap.add_argument("--serve-rl-atomic", action="store_true",
                     help="rlpath requests with float-atomic table updates (films not bit-reproducible)")
     ap.add_argument("--no-equal-time", action="store_true")
