@@ -328,17 +328,17 @@ def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: i
     }
 
 
-def config2_bench(seed: int, sizes: list, width: int, height: int, spp: int, cpu: bool,
-                  tune_launch: bool = True) -> dict:
+def config2_bench(seed: int, sizes: list, width: int, height: int, spp: int, cpu: bool) -> dict:
     """Wavefront path tracing at scale: the procedural interior (displaced
     icospheres and boxes, 64 quad lights, matte / plastic / mirror / glass,
     1024^2 textures) at each triangle count, 3 treelet rounds, max depth 16
     (Russian roulette from bounce 3), width x height at spp samples per
     pixel, with `path` and with `rlpath` (64^3 Q entries x 128 bins,
     material-sorted hits). Mrays/s = (closest + shadow rays) / device time
-    of the render (scene upload and BVH build excluded). With tune_launch,
-    the render loops' launch shape is autotuned per scene and integrator on
-    a 2-spp render of the same view first (not timed). The CPU oracle
+    of the render (scene upload and BVH build excluded), with the render
+    loops' default launch shape (shapes autotuned on short renders of the
+    same view measured slower at 64 spp: 2M triangles, path 1517 against
+    2200 Mrays/s). The CPU oracle
     renders a 256x256 crop at 4 spp of the first scene with `path`, over the
     GPU's BVH (bit-identical to the oracle's own; its 3 treelet rounds would
     take the oracle ~30 s single-threaded), timed on this host's cores."""
@@ -356,13 +356,7 @@ def config2_bench(seed: int, sizes: list, width: int, height: int, spp: int, cpu
         cam = look_at(arr.eye, arr.center, arr.up, arr.fov, width, height)
         rec = {"n_tris": int(len(arr.tris))}
         R.render(ctx, sc, 1, camera=cam, max_depth=16)  # warm-up (allocates)
-        rec["launch"] = {}
         for integ, kw in (("path", {}), ("rlpath", {"q_grid": 64, "sort_by_material": True})):
-            if tune_launch:
-                from cavalry_amd import tune
-                res = tune.tune_render(ctx, sc, 2, integrator=integ, camera=cam, max_depth=16, reps=1,
-                                       use_cache=False, **kw)
-                rec["launch"][integ] = res["shape"]
             film, st = R.render(ctx, sc, spp, integrator=integ, camera=cam, max_depth=16, **kw)
             rays = int(st["closest_rays"] + st["shadow_rays"])
             rec[integ] = {"mrays_s": round(rays / (st["ms_total"] * 1e-3) / 1e6, 1),
@@ -714,8 +708,7 @@ def main() -> int:
         w, h = (int(x) for x in args.c2_res.lower().split("x"))
         sizes = [int(x) for x in args.c2_tris.split(",") if x]
         log(f"[bench] path tracing at scale ({w}x{h}, {args.c2_spp} spp, {sizes} tris)")
-        out["path_tracing_at_scale"] = config2_bench(args.seed, sizes, w, h, args.c2_spp, not args.no_cpu,
-                                                     not args.no_tune)
+        out["path_tracing_at_scale"] = config2_bench(args.seed, sizes, w, h, args.c2_spp, not args.no_cpu)
     if not args.no_equal_time:
         w, h = (int(x) for x in args.et_res.lower().split("x"))
         log(f"[bench] rlpath vs path at equal time ({w}x{h}, {args.et_tris} tris)")

@@ -120,6 +120,10 @@ struct RenderSrc {
     int depth_;
     __device__ bool take(int64_t) const { return true; }
     __device__ const RlReqState& get(int64_t) const { return s; }
+    // the state a task reads; the same for every lane, so it stays in the
+    // kernel argument (selecting between it and another address would make
+    // the compiler copy the argument to scratch memory in every thread)
+    __device__ const RlReqState& state(int64_t, bool) const { return s; }
     __device__ int depth(int64_t) const { return depth_; }
     __device__ int slot(int64_t) const { return 0; }
     __device__ void tag_shadow(uint32_t, int64_t) const {}
@@ -132,6 +136,8 @@ struct PoolSrc {
     uint8_t* live;         // per request slot: it had a hit this bounce
     __device__ bool take(int64_t i) const { return rl[rid[i]].t.Q != nullptr; }
     __device__ const RlReqState& get(int64_t i) const { return rl[rid[i]]; }
+    // the task's request slot (slot 0 for a lane without a task: never read)
+    __device__ const RlReqState& state(int64_t i, bool live) const { return rl[live ? rid[i] : 0]; }
     __device__ int depth(int64_t i) const { return dep[i]; }
     __device__ int slot(int64_t i) const { return rid[i]; }
     __device__ void tag_shadow(uint32_t ss, int64_t i) const { sh_rid[ss] = rid[i]; }
@@ -216,9 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_lighting(Src src, SceneDev sc, Halto
     const int64_t i = live ? task_slot(perm, j) : 0;
     live = live && src.take(i);
     const int prim = live ? hit_prim[i] : -1;
-    const RlReqState* R = live ? &src.get(i) : nullptr;
+    const RlReqState& R = src.state(i, live);
     if (live && prim < 0 && sc.env_miss >= 0) {  // intersectScene's miss (:292-296)
-        float* acc = R->res + (int64_t)qi.sid[i] * 3;
+        float* acc = R.res + (int64_t)qi.sid[i] * 3;
         const v3 e = mul3(miss_radiance(sc, mk3(qi.ray[i * 6 + 3], qi.ray[i * 6 + 4], qi.ray[i * 6 + 5])),
                           mk3(qi.mult[i * 3], qi.mult[i * 3 + 1], qi.mult[i * 3 + 2]));
         acc[0] = acc[0] + e.x; acc[1] = acc[1] + e.y; acc[2] = acc[2] + e.z;
@@ -228,9 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_lighting(Src src, SceneDev sc, Halto
     float lim = 0.f;
     int mesh = 0, dst = 0;
     if (live) {
-        const int D = R->dims[0];
+        const int D = R.dims[0];
         const int depth = src.depth(i);
-        float* res = R->res;
+        float* res = R.res;
         const uint32_t sid = qi.sid[i];
         const float* r = qi.ray + i * 6;
         const v3 ro = mk3(r[0], r[1], r[2]), rd = mk3(r[3], r[4], r[5]);
@@ -255,8 +261,8 @@ __global__ __launch_bounds__(kBlock) void k_lighting(Src src, SceneDev sc, Halto
             }
             res[sid * 3] = acc.x; res[sid * 3 + 1] = acc.y; res[sid * 3 + 2] = acc.z;
         }
-        const int entry = q_entry(R->gr, pos);
-        const float* cdf = R->t.ldist + (int64_t)entry * sc.nlights;
+        const int entry = q_entry(R.gr, pos);
+        const float* cdf = R.t.ldist + (int64_t)entry * sc.nlights;
         const int li = dist_index(cdf, sc.nlights, halton(h, sid, D));
         const float sel = li == 0 ? cdf[0] : cdf[li] - cdf[li - 1];
         const LightPick L = sample_light(sc, h, sid, D + 5, pos, sc.lights[li]);
@@ -428,7 +434,7 @@ __global__ __launch_bounds__(B) void k_qdist(Src src, SceneDev sc, HaltonDev h, 
     live = live && src.take(i);
     const int prim = live ? hit_prim[i] : -1;
     live = live && prim >= 0;
-    const RlReqState* R = live ? &src.get(i) : nullptr;
+    const RlReqState& R = src.state(i, live);
     float *up_fl = nullptr, *up_fp = nullptr, *up_fd = nullptr;  // the words the updates go to (atomics)
     int32_t *up_cl = nullptr, *up_cp = nullptr, *up_cd = nullptr;
     uint32_t rlx = 0;
@@ -441,11 +447,11 @@ __global__ __launch_bounds__(B) void k_qdist(Src src, SceneDev sc, HaltonDev h, 
     float v_l = 0.f, v_p = 0.f, v_d = 0.f;
     uint32_t sid_up = 0;
     if (live) {
-        const RlTables& t = R->t;
-        float* res = R->res;
-        dims = R->dims;
-        rlx = R->rlx;
-        use_rec = R->det != 0;
+        const RlTables& t = R.t;
+        float* res = R.res;
+        dims = R.dims;
+        rlx = R.rlx;
+        use_rec = R.det != 0;
         const int D = dims[0] + 5 + sc.light_dims;
         const uint32_t sid = qi.sid[i];
         sid_up = sid;
