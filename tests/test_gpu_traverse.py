@@ -171,3 +171,78 @@ def test_limit_cull_keeps_visibility(ctx, step, handoff):
     np.testing.assert_array_equal(out[1][0], vref)
     np.testing.assert_array_equal(out[0][0], vref)
     assert out[0][1] < out[1][1]
+
+
+def _deep_chain():
+    """125 triangles whose 63-bit Morton codes are 0 (65 of them) and the 60
+    powers of two 2^0 .. 2^59: the Karras tree is a 60-level caterpillar
+    over an index-split subtree, and every box contains the point
+    (0.6, 0.5, 0.5). Box centroids sit at 0.5 + 2^(j-21) on one axis in a
+    scene box of [0, 1]^3, so the reference's quantisation gives 2^j exactly."""
+    def tri(c, a):
+        c = np.asarray(c, np.float64)
+        return np.concatenate([c + (-a, -a, -a), c + (a, a, -a), c + (0, 0, a)])
+    t = []
+    for axis in range(3):
+        for j in range(20):
+            c = [0.5, 0.5, 0.5]
+            c[axis] += 2.0 ** (j - 21)
+            t.append(tri(c, 0.25))
+    t.append(tri([0.25, 0.25, 0.25], 0.25))  # makes the scene box [0, 1]^3
+    for i in range(64):
+        t.append(tri([0.5 - 0.002 * (i + 1), 0.49, 0.49], 0.25))
+    return np.stack(t).astype(np.float32)
+
+
+def test_stack_overflow_reports_the_reference_answers(ctx):
+    """Rays from inside every box of a tree deeper than the reference's
+    64-entry stack: pushes past the 64th entry are dropped, as the reference
+    drops them (BVH/BVH.h:41,122), so the answers are the reference's partial
+    ones, bit for bit (any-hit in the reference's whole-line order).
+    cav_trace_closest / cav_trace_occluded return CAV_ERR_OVERFLOW and the
+    counters hold the oracle's count of dropped pushes."""
+    import ctypes
+
+    from cavalry_amd._lib import lib
+    print("[trace overflow] 125 triangles, 63-bit codes, 4096 rays", flush=True)
+    tris = _deep_chain()
+    codes, _, nd = oracle.lbvh_build(tris, 63)
+    assert len(np.unique(codes)) == 61
+    rng = np.random.default_rng(1)
+    n = 4096
+    o = np.float32([0.6, 0.5, 0.5]) + rng.uniform(-0.01, 0.01, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.ascontiguousarray(np.concatenate([o, d], 1), np.float32)
+    lim = np.full(n, np.inf, np.float32)
+    lim[::2] = rng.uniform(0.05, 1.0, n // 2).astype(np.float32)
+    ref = oracle.trace_closest(nd, tris, rays)
+    assert ref["overflow"] > 0
+    vref = np.empty(n, np.uint8)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    vovf = oracle.lib().ref_trace_occluded(p(nd.box), p(nd.left), p(nd.right), p(nd.prim), p(tris), None,
+                                           p(rays), p(lim), None, None, ctypes.c_int64(n), p(vref))
+    bvh = BVH.build(ctx, tris, optimization_rounds=0, morton_bits=63)
+    prim = np.empty(n, np.int32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    bvh.trace_counters(reset=True)
+    rc = lib().cav_trace_closest(ctx.handle, bvh.handle, p(rays), n, p(prim), p(t), p(u), p(v))
+    c = bvh.trace_counters(reset=True)
+    # any-hit along the whole line, the reference's visited set (the limit
+    # cull visits fewer boxes, so it drops other pushes, or none)
+    from cavalry_amd import tune
+    saved = tune.get_launch(ctx, tune.STANDALONE)
+    vis = np.empty(n, np.uint8)
+    try:
+        tune.set_launch(ctx, tune.STANDALONE, dict(saved, any_hit_whole_line=1))
+        rcv = lib().cav_trace_occluded(ctx.handle, bvh.handle, p(rays), p(lim), None, None, n, p(vis))
+    finally:
+        tune.set_launch(ctx, tune.STANDALONE, saved)
+    cv = bvh.trace_counters(reset=True)
+    print(f"[trace overflow] closest rc {rc} dropped {c['overflow_closest']} (oracle {ref['overflow']}); "
+          f"any-hit rc {rcv} dropped {cv['overflow_occluded']} (oracle {vovf})", flush=True)
+    assert rc == 4 and c["overflow_closest"] == ref["overflow"]
+    np.testing.assert_array_equal(prim, ref["prim"])
+    np.testing.assert_array_equal(bits(t), bits(ref["t"]))
+    assert vovf > 0 and rcv == 4 and cv["overflow_occluded"] == vovf
+    np.testing.assert_array_equal(vis, vref)
