@@ -409,6 +409,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=1 << 20)
     ap.add_argument("--cpu-rays", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="time only the CPU oracle on --cpu-n triangles and --cpu-rays rays, print its JSON and exit")
     ap.add_argument("--renders", type=int, default=3)
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -441,6 +443,9 @@ def main() -> int:
               "this build has no multi-GPU path", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+    if args.cpu_only:
+        print(json.dumps(cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)), flush=True)
         return 0
 
     import numpy as np
