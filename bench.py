@@ -77,10 +77,14 @@ def cpu_baseline(n: int, nrays: int, rounds: int, seed: int) -> dict:
     t0 = time.perf_counter()
     _, _, nd = oracle.lbvh_build(tris, 30)
     t1 = time.perf_counter()
-    oracle.treelet_optimize(nd, rounds)
+    log(f"[bench] cpu: LBVH build {t1 - t0:.1f} s")
+    for k in range(rounds):  # one round at a time, each a line of progress
+        oracle.treelet_optimize(nd, 1)
+        log(f"[bench] cpu: treelet round {k + 1}/{rounds}")
     t2 = time.perf_counter()
     oracle.trace_closest(nd, tris, rays)
     t3 = time.perf_counter()
+    log(f"[bench] cpu: closest-hit {t3 - t2:.1f} s")
     oracle.trace_occluded(nd, tris, srays, lim)
     t4 = time.perf_counter()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
