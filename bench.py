@@ -211,9 +211,11 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
         srv = Server(ctx, **cfg)
         rl_round = (lambda r, srv=srv: srv.rl_round_spp(r.camera.width * r.camera.height, r.spp))
         log(f"[bench] serving at {rate}/s: plain per-request renderer, {n_requests} requests")
-        plain = run_plain(ctx, res_scenes, trace, rl_round=rl_round)
+        prog = (lambda k, n, rate=rate: log(f"[bench] serving at {rate}/s: {abs(k)}/{n} requests "
+                                             f"{'submitted' if k < 0 else 'done'}"))
+        plain = run_plain(ctx, res_scenes, trace, rl_round=rl_round, progress=prog)
         log(f"[bench] serving at {rate}/s: server (continuous batching)")
-        served = run_served(srv, res_scenes, trace)
+        served = run_served(srv, res_scenes, trace, progress=prog)
         st = srv.close()
         served.rays += st["pool_closest_rays"] + st["pool_shadow_rays"]
         same = True

@@ -147,11 +147,13 @@ def poisson_trace(rng: np.random.Generator, n: int, rate: float, scenes: list, r
     return out
 
 
-def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
+def run_served(server: Server, scenes: list, trace: list, progress=None) -> TraceResult:
     """Submits each request at its arrival time (from a thread), waits for
     all, and measures arrival -> film-ready latency on the server's clock.
     `rays` stays 0: the requests share the pools' ray tallies, which
-    cav_serve_close reports."""
+    cav_serve_close reports. progress(done, total), if given, is called as
+    requests finish (every 16th and the last), and progress(-submitted,
+    total) as they are submitted."""
     tickets = [None] * len(trace)
     start = time.perf_counter()
 
@@ -162,6 +164,8 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
                 time.sleep(delay)
             tickets[k] = server.submit(scenes[r.scene], r.spp, r.integrator, camera=r.camera,
                                        max_depth=r.max_depth, deterministic=r.deterministic)
+            if progress is not None and ((k + 1) % 16 == 0 or k + 1 == len(trace)):
+                progress(-(k + 1), len(trace))
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -169,8 +173,10 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
     res = TraceResult()
     first_submit, last_done = None, 0.0
     rl_rays = 0
-    for t in tickets:
+    for k, t in enumerate(tickets):
         out = server.poll(t, wait=True)
+        if progress is not None and ((k + 1) % 16 == 0 or k + 1 == len(tickets)):
+            progress(k + 1, len(tickets))
         if out["status"] != DONE:
             raise RuntimeError(f"request {t.id} ended with status {out['status']} (error {out['error']}): "
                                f"{out.get('message', '')}")
@@ -184,11 +190,11 @@ def run_served(server: Server, scenes: list, trace: list) -> TraceResult:
     return res
 
 
-def run_plain(ctx, scenes: list, trace: list, rl_round=None) -> TraceResult:
+def run_plain(ctx, scenes: list, trace: list, rl_round=None, progress=None) -> TraceResult:
     """The plain per-request renderer: one cav_render at a time, in arrival
     order, each starting when it has arrived and the previous one is done.
     rl_round(req) gives an rlpath request's spp_per_round (None: one round,
-    as device memory allows)."""
+    as device memory allows); progress as for run_served."""
     res = TraceResult()
     start = time.perf_counter()
     first = None
@@ -205,5 +211,7 @@ def run_plain(ctx, scenes: list, trace: list, rl_round=None) -> TraceResult:
         res.latencies_ms.append((done - arrived) * 1e3)
         res.rays += st["closest_rays"] + st["shadow_rays"]
         res.films.append(film)
+        if progress is not None and (len(res.films) % 16 == 0 or len(res.films) == len(trace)):
+            progress(len(res.films), len(trace))
     res.makespan_s = time.perf_counter() - first
     return res
