@@ -420,8 +420,8 @@ def main() -> int:
     ap.add_argument("--renders", type=int, default=3)
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24, help="requests per arrival rate")
-    ap.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates (req/s), comma-separated")
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests per arrival rate")
+    ap.add_argument("--serve-rates", default="5,50", help="Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--serve-no-graphs", action="store_true")
     ap.add_argument("--serve-pool-tasks", type=int, default=32 << 20, help="task slots of each scene's ray pool")
     ap.add_argument("--serve-rl-atomic", action="store_true",
