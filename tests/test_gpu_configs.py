@@ -155,3 +155,23 @@ def test_interior_rlpath_deterministic_matches_oracle(ctx, interior2m, q_grid):
     assert st["closest_rays"] + st["shadow_rays"] == rays
     np.testing.assert_array_equal(film.weight, w)
     np.testing.assert_array_equal(bits(film.rgb), bits(rgb))
+
+
+def test_interior_limit_cull_keeps_the_film(ctx, interior2m):
+    """The whole 1920x1080 config-2 film (2 spp, depth 16): any-hit with the
+    limit cull (the default) and along the whole line, the reference's
+    visiting order, give the same film bit for bit (every shadow ray's
+    visibility is the same)."""
+    from cavalry_amd import tune
+    arr, sc, _ = interior2m
+    cam = look_at(arr.eye, arr.center, arr.up, arr.fov, 1920, 1080)
+    print("[config2] path 1920x1080 2 spp depth 16, limit cull vs whole line", flush=True)
+    saved = tune.get_launch(ctx, tune.WAVEFRONT)
+    try:
+        a, _ = R.render(ctx, sc, 2, camera=cam, max_depth=16)
+        tune.set_launch(ctx, tune.WAVEFRONT, dict(saved, any_hit_whole_line=1))
+        b, _ = R.render(ctx, sc, 2, camera=cam, max_depth=16)
+    finally:
+        tune.set_launch(ctx, tune.WAVEFRONT, saved)
+    np.testing.assert_array_equal(a.weight, b.weight)
+    np.testing.assert_array_equal(bits(a.rgb), bits(b.rgb))
