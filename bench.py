@@ -287,12 +287,12 @@ def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: i
                                spp_per_round=min(ref_spp, 1024))
     ref = ref_film.rgb.astype(np.float64) / np.maximum(ref_film.weight.astype(np.float64), 1e-30)[..., None]
 
-    def run(integrator, spp, q_grid=64):
+    def run(integrator, spp, q_grid=64, alpha_scale=1.0):
         # each integrator with its faster hit order: rlpath's 128-bin BSDF
         # evaluation diverges less over material-sorted hits (-12%), path's
         # shading is too cheap to pay for the sort (+5%)
         film, st = R.render(ctx, scene, spp, integrator=integrator, camera=cam, q_grid=q_grid,
-                            sort_by_material=integrator == "rlpath")
+                            alpha_scale=alpha_scale, sort_by_material=integrator == "rlpath")
         return {"spp": int(spp), "ms": round(float(st["ms_total"]), 3),
                 "rel_mse": R.rel_mse(film.image, ref),
                 "rel_mse_ldr": R.rel_mse(np.clip(film.image, 0, 1), np.clip(ref, 0, 1)),
@@ -317,13 +317,22 @@ def equal_time_bench(seed: int, n_tris: int, width: int, height: int, ref_spp: i
             b[f"rlpath_q{q}_over_path_error"] = round(rl["rel_mse"] / p["rel_mse"], 4)
             log(f"[bench] equal time {p['ms']:.1f} ms: path {spp} spp rel-MSE {p['rel_mse']:.3e}, "
                 f"rlpath {q}^3 {rl_spp} spp ({rl['ms']:.1f} ms) rel-MSE {rl['rel_mse']:.3e}")
+            if q == 64:
+                # the workload's learning rate: 0.3 times the reference's
+                # n / (T + n) (the same per-spp cost)
+                rl3 = run("rlpath", rl_spp, q, alpha_scale=0.3)
+                b["rlpath_q64_alpha0.3"] = rl3
+                b["rlpath_q64_alpha0.3_over_path_error"] = round(rl3["rel_mse"] / p["rel_mse"], 4)
+                log(f"[bench] equal time {p['ms']:.1f} ms: rlpath 64^3 alpha x0.3 {rl_spp} spp "
+                    f"({rl3['ms']:.1f} ms) rel-MSE {rl3['rel_mse']:.3e}")
         budgets.append(b)
     scene.close()
     ctx.close()
     return {
         "workload": (f"interior scene ({len(sc.tris)} tris, 4 1024^2 textures, 64 quad lights), "
                      f"{width}x{height}, max depth 16, 3 treelet rounds; reference: path at {ref_spp} "
-                     "spp with binary64 film sums; rlpath with 64^3 and 32^3 Q entries, material-sorted hits"),
+                     "spp with binary64 film sums; rlpath with 64^3 and 32^3 Q entries (the reference's "
+                     "learning rate n/(T+n)) and with 64^3 entries at 0.3 n/(T+n), material-sorted hits"),
         "reference": {"spp": ref_spp, "ms": round(float(ref_st["ms_total"]), 1),
                       "mrays_s": round(float(ref_st["mrays_per_s"]), 1),
                       # rel-MSE falls as 1 / spp: the reference's own error, from path at 1024 spp
