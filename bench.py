@@ -171,7 +171,7 @@ def render_bench(renders: int, cpu: bool, tune_launch: bool = True) -> dict:
 
 
 def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = True,
-                  deterministic_rl: bool = True, pool_tasks: int = 32 << 20) -> dict:
+                  deterministic_rl: bool = True, pool_tasks: int = 32 << 20, steps_ahead: int = 0) -> dict:
     """Serving on four resident interior scenes (about 0.1M, 0.5M, 2M and 8M
     triangles, 1024^2 Kd textures on about a third of the meshes): for each
     arrival rate, one seeded Poisson trace of render requests (resolution
@@ -201,7 +201,8 @@ def serving_bench(n_requests: int, rates: list, seed: int, use_graphs: bool = Tr
         at = np.float32([rng.uniform(2, 8), rng.uniform(0.3, 1.5), rng.uniform(4, 9)])
         return look_at(eye, at, (0, 1, 0), 60.0, w, h)
 
-    cfg = dict(arena_bytes=64 << 30, host_arena_bytes=4 << 30, pool_tasks=pool_tasks, use_graphs=use_graphs)
+    cfg = dict(arena_bytes=64 << 30, host_arena_bytes=4 << 30, pool_tasks=pool_tasks, use_graphs=use_graphs,
+               max_steps_ahead=steps_ahead)
     per_rate = []
     for rate in rates:
         rng = np.random.default_rng(seed)
@@ -433,6 +434,8 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="5,50", help="Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--serve-no-graphs", action="store_true")
     ap.add_argument("--serve-pool-tasks", type=int, default=32 << 20, help="task slots of each scene's ray pool")
+    ap.add_argument("--serve-steps-ahead", type=int, default=0,
+                    help="bounce steps the server's host thread may run ahead of the device (0: the library's 4)")
     ap.add_argument("--serve-rl-atomic", action="store_true",
                     help="rlpath requests with float-atomic table updates (films not bit-reproducible)")
     ap.add_argument("--no-equal-time", action="store_true")
@@ -737,7 +740,7 @@ def main() -> int:
         log("[bench] serving")
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = serving_bench(args.serve_requests, rates, args.seed, not args.serve_no_graphs,
-                                       not args.serve_rl_atomic, args.serve_pool_tasks)
+                                       not args.serve_rl_atomic, args.serve_pool_tasks, args.serve_steps_ahead)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at N={args.cpu_n}, rays={args.cpu_rays}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_rays, args.rounds, args.seed)
